@@ -1,0 +1,399 @@
+/*
+ * rosy_amd.h -- C ABI of the MI355X raster-and-shade frame path.
+ *
+ * This header is the drop-in boundary that replaces the renderer-facing surface of Rosy's Vulkan
+ * renderer (src/Engine/Graphics.h:11-21 and graphics_device in src/Engine/Graphics.cpp).  Every
+ * record type keeps the reference's byte layout (packed, little-endian, static_assert'ed sizes),
+ * every entry point names the reference function it stands in for, and every call returns the
+ * reference's numeric rosy::result code (src/Engine/Types.h:13-31).  No C++ types or exceptions
+ * cross this boundary; all pointers are borrowed for the duration of the call only.
+ */
+#ifndef ROSY_AMD_H
+#define ROSY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RSY_ABI_VERSION 1u
+#define RSY_ABSENT 0xFFFFFFFFu /* UINT32_MAX == not present (src/Asset/Asset.h:38-45) */
+
+/* rosy::result, same numeric values (src/Engine/Types.h:13-31). */
+typedef enum rsy_result {
+    RSY_OK = 0,
+    RSY_STATE_CHANGED = 1,
+    RSY_ERROR = 2,
+    RSY_INVALID_ARGUMENT = 3,
+    RSY_ALLOCATION_FAILURE = 4,
+    RSY_GRAPHICS_INIT_FAILURE = 5,
+    RSY_GRAPHICS_FRAME_FAILURE = 6,
+    RSY_GRAPHICS_SWAPCHAIN_FAILURE = 7,
+    RSY_OPEN_FAILED = 8,
+    RSY_WRITE_FAILED = 9,
+    RSY_COPY_FAILED = 10,
+    RSY_READ_FAILED = 11,
+    RSY_CREATE_FAILED = 12,
+    RSY_UPDATE_FAILED = 13,
+    RSY_OVERFLOW = 14,
+    RSY_INVALID_STATE = 15
+} rsy_result;
+
+#pragma pack(push, 1)
+
+/* Vertex: rosy_asset::position (src/Asset/Asset.h:92-99) == BasicInputVertex
+ * (src/shaders/data.slang:24-30).  64 bytes, AoS. */
+typedef struct rsy_position {
+    float vertex[3];
+    float normal[3];
+    float tangents[4]; /* w = handedness */
+    float color[4];
+    float texture_coordinates[2];
+} rsy_position;
+
+/* gpu_material (src/Engine/Graphics.cpp:345-360) == BasicMaterialData (src/shaders/data.slang:77-91).
+ * Image indices refer to the texture table given to rsy_set_materials, sampler indices to the
+ * sampler table.  RSY_ABSENT marks a missing map.  64 bytes. */
+typedef struct rsy_gpu_material {
+    float color[4];
+    float metallic_factor;
+    float roughness_factor;
+    float alpha_cutoff;
+    uint32_t alpha_mode;
+    uint32_t color_sampled_image_index;
+    uint32_t color_sampler_index;
+    uint32_t normal_sampled_image_index;
+    uint32_t normal_sampler_index;
+    uint32_t metallic_sampled_image_index;
+    uint32_t metallic_sampler_index;
+    uint32_t mixmap_sampled_image_index;
+    uint32_t mixmap_sampler_index;
+} rsy_gpu_material;
+
+/* graphic_object_data (src/Engine/Graphics.cpp:422-427) == BasicGraphicsData
+ * (src/shaders/data.slang:32-36).  Column-major matrices.  164 bytes. */
+typedef struct rsy_object_record {
+    float transform[16];
+    float to_object_space_transform[16];
+    float normal_transform[9];
+} rsy_object_record;
+
+/* gpu_scene_data (src/Engine/Graphics.cpp:281-306) == SceneData (src/shaders/data.slang:45-70).
+ * 408 bytes.  csm_index_sampler / csm_index_near are kept for layout; the shadow map used is the
+ * one the frame renders. */
+typedef struct rsy_scene_data {
+    float view[16];
+    float proj[16];
+    float view_projection[16];
+    float shadow_projection_near[16];
+    float sun_position[4];
+    float camera_position[4];
+    float ambient_color[4];
+    float sunlight_color[4];
+    uint32_t flip_lights[4];
+    uint32_t flip_tangents[4];
+    float draw_extent[2];
+    float light_intensity;
+    uint32_t csm_index_sampler;
+    uint32_t csm_index_near;
+    uint32_t fragment_output;
+    uint32_t light_enabled;
+    uint32_t tangent_space_enabled;
+    uint32_t shadows_enabled;
+    uint32_t normal_maps_enabled;
+    uint32_t inverse_bnt;
+    uint32_t ignore_asset_tangent_sign;
+    uint32_t ensure_orthogonal_bitangent;
+    uint32_t brdf_lighting_enabled;
+} rsy_scene_data;
+
+/* surface_graphics_data (src/Engine/Types.h:39-48) with fixed-width fields.  48 bytes. */
+typedef struct rsy_draw {
+    uint64_t mesh_index;
+    uint64_t graphic_objects_offset;
+    uint64_t graphics_object_index;
+    uint64_t material_index; /* RSY_ABSENT (as u64 0xFFFFFFFF) == no material */
+    uint32_t index_count;
+    uint32_t start_index;
+    uint32_t blended;
+    uint32_t reserved;
+} rsy_draw;
+
+/* gpu_mesh_buffers (src/Engine/Graphics.cpp:376-381) plus the vertex count.  The vertex offset
+ * is in bytes into the global vertex buffer, as in Graphics.cpp:4322; indices are mesh-local.
+ * 24 bytes. */
+typedef struct rsy_mesh_range {
+    uint64_t vertex_buffer_offset;
+    uint32_t index_offset;
+    uint32_t num_indices;
+    uint32_t num_vertices;
+    uint32_t reserved;
+} rsy_mesh_range;
+
+/* rosy_asset::sampler (src/Asset/Asset.h:48-54): filter codes 0-2 nearest, 3-5 / absent linear
+ * (filter_to_val Graphics.cpp:250-264); wrap 0 clamp-to-edge, 1 mirrored repeat, 2 repeat
+ * (wrap_to_val Graphics.cpp:266-278).  8 bytes. */
+typedef struct rsy_sampler {
+    uint16_t min_filter;
+    uint16_t mag_filter;
+    uint16_t wrap_s;
+    uint16_t wrap_t;
+} rsy_sampler;
+
+#pragma pack(pop)
+
+/* Texture formats.  The reference's BC7 images are decoded formats here (uncompressed, mip 0):
+ * color and metallic/mixmap maps are sRGB-encoded (Graphics.cpp:3632-3647), normal maps UNORM. */
+typedef enum rsy_texture_format {
+    RSY_TEX_RGBA8_SRGB = 0,
+    RSY_TEX_RGBA8_UNORM = 1,
+    RSY_TEX_RG8_UNORM = 2,
+    RSY_TEX_RG8_SRGB = 3
+} rsy_texture_format;
+
+typedef struct rsy_texture_desc {
+    uint32_t width;
+    uint32_t height;
+    uint32_t format; /* rsy_texture_format */
+    uint32_t reserved;
+    const void* pixels; /* width*height*bytes_per_texel, row-major, row 0 = v 0 */
+} rsy_texture_desc;
+
+/* Fixed-function state the reference sets with dynamic state (Graphics.cpp:4988-5045, 5226-5231)
+ * from draw_config_state / light_read_write_state (Types.h:131-172). */
+typedef enum rsy_pcf_mode {
+    RSY_PCF_REFERENCE_5TAP = 0, /* pcf.slang:1-12: 5 bilinear-compare taps, offset 0.00025 UV */
+    RSY_PCF_GRID_3X3 = 1        /* 9 bilinear-compare taps at one-texel spacing */
+} rsy_pcf_mode;
+
+typedef struct rsy_raster_state {
+    uint32_t cull_enabled;                  /* cull FRONT when set (Graphics.cpp:5010-5015) */
+    uint32_t reverse_winding_order_enabled; /* front = CCW when set, else CW */
+    uint32_t depth_bias_enabled;            /* shadow pass only (Graphics.cpp:5224-5231) */
+    float depth_bias_constant;
+    float depth_bias_clamp;
+    float depth_bias_slope_factor;
+    uint32_t pcf_mode;  /* rsy_pcf_mode */
+    uint32_t tile_size; /* raster bin size in pixels: 8, 16 or 32; 0 = autotuned */
+    float clear_color[4];
+} rsy_raster_state;
+
+/* A camera view rendered into a visibility buffer and shaded. */
+typedef struct rsy_camera_view {
+    uint32_t width;
+    uint32_t height;
+    uint32_t reserved[2];
+    float view[16];
+    float proj[16];
+    float view_projection[16];
+    float camera_position[4]; /* w kept as given (the reference leaves it 0) */
+} rsy_camera_view;
+
+/* A shadow-map view (depth only).  split_far is the view-space distance up to which the cascade
+ * is selected by the shading kernel; the single reference cascade uses +inf. */
+typedef struct rsy_shadow_cascade {
+    uint32_t size; /* square map, size x size texels */
+    float split_far;
+    uint32_t reserved[2];
+    float light_view_projection[16];
+} rsy_shadow_cascade;
+
+typedef enum rsy_shade_mode {
+    RSY_SHADE_NONE = 0,   /* visibility buffers and shadow maps only */
+    RSY_SHADE_PLAIN = 1,  /* Cook-Torrance + PCF over the plain textures (basic.slang:137-344) */
+    RSY_SHADE_NEURAL = 2  /* same BRDF; material maps decoded by the neural material decoder */
+} rsy_shade_mode;
+
+typedef struct rsy_frame_desc {
+    uint32_t n_camera_views; /* 0 = one view from the scene data (view_projection, draw_extent) */
+    uint32_t n_cascades;     /* 0 = one cascade from scene data shadow_projection_near */
+    const rsy_camera_view* camera_views;
+    const rsy_shadow_cascade* cascades;
+    uint32_t shade_mode;      /* rsy_shade_mode */
+    uint32_t default_shadow_size; /* used when n_cascades == 0 */
+} rsy_frame_desc;
+
+typedef struct rsy_config {
+    uint32_t abi_version;      /* RSY_ABI_VERSION */
+    int32_t device;            /* HIP device ordinal */
+    uint32_t max_window_width; /* rosy::config (Types.h:33-37) */
+    uint32_t max_window_height;
+} rsy_config;
+
+typedef struct rsy_frame_stats {
+    /* graphics_stats (src/Engine/DebugUI.h:7-13), counted as in Graphics.cpp:5285-5286 */
+    uint64_t triangle_count;
+    uint64_t draw_call_count;
+    /* per-frame raster counters */
+    uint64_t triangles_setup;   /* survived clip, cull and bbox rejection (all views) */
+    uint64_t triangles_clipped; /* needed near/far/guard-band clipping */
+    uint64_t tile_pairs;        /* (triangle, tile) bin entries */
+    uint32_t tile_size;
+    uint32_t overflowed;        /* a bin or record buffer was too small; the frame was re-run */
+    float gpu_ms;               /* device time of the last frame (HIP events) */
+    float stage_ms[8];          /* geometry, bin, raster, shade, ... (HIP events, if timing enabled) */
+} rsy_frame_stats;
+
+typedef struct rsy_ctx rsy_ctx;
+
+/* ---------------------------------------------------------------------------------------------
+ * Context lifetime.  Replaces graphics::init / graphics_device::init (Graphics.cpp:544-738,
+ * 6119-6170) and graphics::deinit.  One context per thread; calls on a context are serialised.
+ * ------------------------------------------------------------------------------------------- */
+rsy_result rsy_create(const rsy_config* cfg, rsy_ctx** out_ctx);
+void rsy_destroy(rsy_ctx* ctx);
+const char* rsy_last_error(const rsy_ctx* ctx);
+uint32_t rsy_abi_version(void);
+
+/* ---------------------------------------------------------------------------------------------
+ * Data entry points (graphics_device::set_asset, Graphics.cpp:3525-4689).
+ * ------------------------------------------------------------------------------------------- */
+
+/* Global vertex buffer (AoS 64-byte vertices), global index buffer (mesh-local u32 indices) and
+ * the mesh table.  Replaces the vertex/index upload in set_asset (Graphics.cpp:4310-4560). */
+rsy_result rsy_set_geometry(rsy_ctx* ctx, const rsy_position* vertices, uint64_t n_vertices,
+                            const uint32_t* indices, uint64_t n_indices,
+                            const rsy_mesh_range* meshes, uint32_t n_meshes);
+
+/* Materials, samplers and decoded texture images.  Replaces the image, sampler and material
+ * upload in set_asset (Graphics.cpp:3590-4160). */
+rsy_result rsy_set_materials(rsy_ctx* ctx, const rsy_gpu_material* materials, uint32_t n_materials,
+                             const rsy_texture_desc* textures, uint32_t n_textures,
+                             const rsy_sampler* samplers, uint32_t n_samplers);
+
+/* Object records and surfaces; builds the shadow (all), opaque (!blended) and blended lists in
+ * the order given.  Replaces graphics_device::set_graphic_objects (Graphics.cpp:4691-4921). */
+rsy_result rsy_set_graphic_objects(rsy_ctx* ctx, const rsy_object_record* objects, uint32_t n_objects,
+                                   const rsy_draw* surfaces, uint32_t n_surfaces);
+
+/* Partial object-record update.  Replaces update_graphic_objects (Graphics.cpp:4923-4927). */
+rsy_result rsy_update_graphic_objects(rsy_ctx* ctx, uint64_t offset, const rsy_object_record* objects,
+                                      uint32_t n_objects);
+
+/* Per-frame camera + light uniforms and flags.  Replaces graphics_device::update
+ * (Graphics.cpp:6016-6061); the record is copied, not retained. */
+rsy_result rsy_update(rsy_ctx* ctx, const rsy_scene_data* scene_data, const rsy_raster_state* state);
+
+/* Render one frame: every shadow cascade (shadow list), then every camera view (opaque list) into
+ * visibility buffers, then deferred shading.  Replaces graphics_device::render
+ * (Graphics.cpp:4934-6014) up to the MSAA resolve.  Work is enqueued on the context's stream;
+ * outputs stay valid until the next render call. */
+rsy_result rsy_render(rsy_ctx* ctx, const rsy_frame_desc* frame);
+
+/* Capture the frame described by `frame` as a HIP graph and replay it `n_replays` times. */
+rsy_result rsy_render_graph(rsy_ctx* ctx, const rsy_frame_desc* frame, uint32_t n_replays);
+
+/* Wait for the context's stream. */
+rsy_result rsy_synchronize(rsy_ctx* ctx);
+
+/* ---------------------------------------------------------------------------------------------
+ * Outputs (host readback; synchronises the stream).
+ * ------------------------------------------------------------------------------------------- */
+
+/* Visibility word per pixel: (float depth bits << 32) | (triangle id + 1); 0 = background.
+ * Triangle ids count triangles over the opaque list in order. */
+rsy_result rsy_read_visibility(rsy_ctx* ctx, uint32_t view, uint64_t* out, uint64_t n_pixels);
+/* Linear RGBA32F colour per pixel (the reference's draw image before resolve/blit). */
+rsy_result rsy_read_color(rsy_ctx* ctx, uint32_t view, float* out_rgba, uint64_t n_pixels);
+/* Shadow-map depth per texel (float, 0 = clear). */
+rsy_result rsy_read_shadow_map(rsy_ctx* ctx, uint32_t cascade, float* out, uint64_t n_texels);
+/* Per-pixel shadow factor written when RSY_DEBUG_SHADOW_FACTOR is enabled (else zeros). */
+rsy_result rsy_read_shadow_factor(rsy_ctx* ctx, uint32_t view, float* out, uint64_t n_pixels);
+rsy_result rsy_get_frame_stats(rsy_ctx* ctx, rsy_frame_stats* out);
+
+/* Device pointers of the last frame's outputs, for consumers on the same device. */
+rsy_result rsy_get_device_outputs(rsy_ctx* ctx, uint32_t view, const void** vis, const void** color);
+
+/* Debug and tuning switches. */
+#define RSY_DEBUG_SHADOW_FACTOR 1u /* also write the per-pixel PCF shadow factor */
+#define RSY_DEBUG_STAGE_TIMING 2u  /* record HIP events between stages */
+rsy_result rsy_set_debug_flags(rsy_ctx* ctx, uint32_t flags);
+
+/* Autotuner: renders the given frame once per candidate bin size (8, 16, 32) and keeps the
+ * fastest for later frames with tile_size == 0.  Writes the chosen size. */
+rsy_result rsy_autotune(rsy_ctx* ctx, const rsy_frame_desc* frame, uint32_t n_iters, uint32_t* chosen_tile);
+
+/* ---------------------------------------------------------------------------------------------
+ * Input producers: closed-form restatements of the reference's camera, light and node math.
+ * All matrices are column-major float[16] (glm value_ptr order).
+ * ------------------------------------------------------------------------------------------- */
+
+/* synthetic_camera::get_projection (src/Engine/Camera.cpp:62-83): reverse-Z, Vulkan y flip.
+ * fov in radians. */
+void rsy_camera_projection(double aspect, double near_plane, double far_plane, double fov, float out[16]);
+/* synthetic_camera::get_view_matrix / get_rotation_matrix (Camera.cpp:85-98). */
+void rsy_camera_view_matrix(const float position[3], float pitch, float yaw, float out[16]);
+/* Level.cpp:464-528: sun position and shadowProjNear = lp * inverse(light_line_rot * T(0,0,d)). */
+void rsy_light_matrices(float sun_distance, float sun_pitch, float sun_yaw, float cascade_level,
+                        float orthographic_depth, float out_sun_position[4], float out_shadow_projection[16]);
+/* node_state::get_world_space_transform (Node.cpp:96-103) with identity coordinate system and
+ * parent, plus to-object = inverse(W) and normal = transpose(inverse(mat3(W))) (Node.cpp:225-247). */
+void rsy_object_record_from_trs(const float translate[3], float scale, float yaw, rsy_object_record* out);
+/* Matrix product c = a * b (column-major, FP32 fmaf chain in k order). */
+void rsy_mat4_mul(const float a[16], const float b[16], float c[16]);
+
+/* ---------------------------------------------------------------------------------------------
+ * Synthetic scene generator for the five workloads (seeded; host-only).
+ * ------------------------------------------------------------------------------------------- */
+typedef struct rsy_scene_desc {
+    uint32_t workload;        /* 0 sphere, 1 scene-scale, 2 batched views, 3 learned decoder, 4 combined */
+    uint32_t reserved;
+    uint64_t seed;
+    /* overrides, 0 = the workload's default */
+    uint32_t width;
+    uint32_t height;
+    uint32_t shadow_size;
+    uint32_t total_triangles;
+    uint32_t n_meshes;
+    uint32_t n_materials;
+    uint32_t texture_size;
+    uint32_t reserved2;
+} rsy_scene_desc;
+
+typedef struct rsy_scene {
+    const rsy_position* vertices;
+    uint64_t n_vertices;
+    const uint32_t* indices;
+    uint64_t n_indices;
+    const rsy_mesh_range* meshes;
+    uint32_t n_meshes;
+    uint32_t n_objects;
+    const rsy_object_record* objects;
+    const rsy_draw* surfaces;
+    uint32_t n_surfaces;
+    uint32_t n_materials;
+    const rsy_gpu_material* materials;
+    const rsy_texture_desc* textures;
+    uint32_t n_textures;
+    uint32_t n_samplers;
+    const rsy_sampler* samplers;
+    rsy_scene_data scene_data;
+    rsy_raster_state raster_state;
+    uint32_t n_camera_views;
+    uint32_t n_cascades;
+    const rsy_camera_view* camera_views;
+    const rsy_shadow_cascade* cascades;
+    uint32_t default_shadow_size;
+    uint32_t workload;
+} rsy_scene;
+
+rsy_result rsy_scene_generate(const rsy_scene_desc* desc, rsy_scene** out_scene);
+void rsy_scene_free(rsy_scene* scene);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#ifdef __cplusplus
+static_assert(sizeof(rsy_position) == 64, "rsy_position must be 64 bytes (Asset.h:92-99)");
+static_assert(sizeof(rsy_gpu_material) == 64, "rsy_gpu_material must be 64 bytes (Graphics.cpp:345-360)");
+static_assert(sizeof(rsy_object_record) == 164, "rsy_object_record must be 164 bytes (Graphics.cpp:422-427)");
+static_assert(sizeof(rsy_scene_data) == 408, "rsy_scene_data must be 408 bytes (Graphics.cpp:281-306)");
+static_assert(sizeof(rsy_draw) == 48, "rsy_draw must be 48 bytes");
+static_assert(sizeof(rsy_mesh_range) == 24, "rsy_mesh_range must be 24 bytes");
+static_assert(sizeof(rsy_sampler) == 8, "rsy_sampler must be 8 bytes (Asset.h:48-54)");
+#endif
+
+#endif /* ROSY_AMD_H */

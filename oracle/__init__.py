@@ -1,0 +1,139 @@
+"""ctypes binding of the CPU oracle (liboracle.so).  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module, and
+only as the checker or the timed CPU baseline; the product (rosy_amd) never uses it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from rosy_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class Geometry(C.Structure):
+    _fields_ = [("vertices", C.POINTER(abi.Position)), ("n_vertices", C.c_uint64),
+                ("indices", C.POINTER(C.c_uint32)), ("n_indices", C.c_uint64),
+                ("meshes", C.POINTER(abi.MeshRange)), ("n_meshes", C.c_uint32), ("n_objects", C.c_uint32),
+                ("objects", C.POINTER(abi.ObjectRecord)), ("draws", C.POINTER(abi.Draw)), ("n_draws", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class RasterStats(C.Structure):
+    _fields_ = [("triangles_in", C.c_uint64), ("triangles_setup", C.c_uint64), ("triangles_clipped", C.c_uint64),
+                ("fragments", C.c_uint64)]
+
+
+class ShadeIn(C.Structure):
+    _fields_ = [("materials", C.POINTER(abi.GpuMaterial)), ("n_materials", C.c_uint32), ("n_textures", C.c_uint32),
+                ("textures", C.POINTER(abi.TextureDesc)), ("samplers", C.POINTER(abi.Sampler)),
+                ("n_samplers", C.c_uint32), ("n_cascades", C.c_uint32),
+                ("cascades", C.POINTER(abi.ShadowCascade)), ("shadow_maps", C.POINTER(C.POINTER(C.c_float))),
+                ("scene", C.POINTER(abi.SceneData)), ("raster", C.POINTER(abi.RasterState)),
+                ("view", C.POINTER(abi.CameraView))]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        L.orc_raster_view.restype = C.c_int
+        L.orc_raster_view.argtypes = [C.POINTER(Geometry), C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int,
+                                      C.POINTER(abi.RasterState), C.c_void_p, C.c_void_p, C.POINTER(RasterStats),
+                                      C.c_int]
+        L.orc_count_tile_pairs.restype = C.c_uint64
+        L.orc_count_tile_pairs.argtypes = [C.POINTER(Geometry), C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int,
+                                           C.POINTER(abi.RasterState), C.c_uint32, C.c_int]
+        L.orc_shade_view.restype = C.c_int
+        L.orc_shade_view.argtypes = [C.POINTER(Geometry), C.POINTER(ShadeIn), C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_srgb_to_linear.restype = C.c_double
+        L.orc_srgb_to_linear.argtypes = [C.c_uint32]
+        _lib = L
+    return _lib
+
+
+def threads():
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
+def draw_list(scene, shadow):
+    """Shadow list = every surface; opaque list = surfaces with blended == 0 (Graphics.cpp:4716-4735)."""
+    surf = scene.surfaces()
+    keep = [d for d in surf if shadow or not d.blended]
+    arr = (abi.Draw * max(1, len(keep)))()
+    for i, d in enumerate(keep):
+        arr[i] = d
+    return arr, len(keep)
+
+
+def geometry(scene, shadow):
+    s = scene.s
+    draws, n = draw_list(scene, shadow)
+    g = Geometry(s.vertices, s.n_vertices, s.indices, s.n_indices, s.meshes, s.n_meshes, s.n_objects, s.objects,
+                 C.cast(draws, C.POINTER(abi.Draw)), n, 0)
+    g._keep = draws
+    return g
+
+
+def raster_view(scene, view_projection, width, height, shadow, raster_state, n_threads=None):
+    g = geometry(scene, shadow)
+    vp = (C.c_float * 16)(*view_projection)
+    st = RasterStats()
+    if shadow:
+        out = np.zeros((height, width), dtype=np.float32)
+        rc = lib().orc_raster_view(C.byref(g), vp, width, height, 1, C.byref(raster_state), None,
+                                   out.ctypes.data, C.byref(st), n_threads or threads())
+    else:
+        out = np.zeros((height, width), dtype=np.uint64)
+        rc = lib().orc_raster_view(C.byref(g), vp, width, height, 0, C.byref(raster_state), out.ctypes.data,
+                                   None, C.byref(st), n_threads or threads())
+    if rc != 0:
+        raise RuntimeError(f"orc_raster_view failed: {rc}")
+    return out, st
+
+
+def tile_pairs(scene, view_projection, width, height, shadow, raster_state, tile, n_threads=None):
+    g = geometry(scene, shadow)
+    vp = (C.c_float * 16)(*view_projection)
+    return int(lib().orc_count_tile_pairs(C.byref(g), vp, width, height, 1 if shadow else 0, C.byref(raster_state),
+                                          tile, n_threads or threads()))
+
+
+def shade_view(scene, view, cascades, shadow_maps, vis, scene_data=None, raster_state=None, override_shadow=None,
+               n_threads=None):
+    """FP64 shading of one camera view.  Returns (rgba float64 [h,w,4], shadow factor [h,w], margin [h,w])."""
+    s = scene.s
+    g = geometry(scene, False)
+    maps = [np.ascontiguousarray(m, dtype=np.float32) for m in shadow_maps]
+    ptrs = (C.POINTER(C.c_float) * max(1, len(maps)))(*[m.ctypes.data_as(C.POINTER(C.c_float)) for m in maps])
+    casc = (abi.ShadowCascade * max(1, len(cascades)))(*cascades)
+    sd = scene_data if scene_data is not None else s.scene_data
+    rs = raster_state if raster_state is not None else s.raster_state
+    cv = abi.CameraView()
+    C.memmove(C.byref(cv), C.byref(view), C.sizeof(abi.CameraView))
+    si = ShadeIn(s.materials, s.n_materials, s.n_textures, s.textures, s.samplers, s.n_samplers, len(cascades),
+                 C.cast(casc, C.POINTER(abi.ShadowCascade)), C.cast(ptrs, C.POINTER(C.POINTER(C.c_float))),
+                 C.pointer(sd), C.pointer(rs), C.pointer(cv))
+    h, w = vis.shape
+    vis = np.ascontiguousarray(vis, dtype=np.uint64)
+    rgba = np.zeros((h, w, 4), dtype=np.float64)
+    shadow = np.zeros((h, w), dtype=np.float64)
+    margin = np.zeros((h, w), dtype=np.float64)
+    ov = None
+    if override_shadow is not None:
+        ov = np.ascontiguousarray(override_shadow, dtype=np.float32)
+    rc = lib().orc_shade_view(C.byref(g), C.byref(si), vis.ctypes.data, rgba.ctypes.data, shadow.ctypes.data,
+                              margin.ctypes.data, ov.ctypes.data if ov is not None else None,
+                              n_threads or threads())
+    if rc != 0:
+        raise RuntimeError(f"orc_shade_view failed: {rc}")
+    return rgba, shadow, margin

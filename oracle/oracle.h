@@ -1,0 +1,100 @@
+/*
+ * oracle.h -- CPU oracle of the raster-and-shade frame path.  TEST INFRASTRUCTURE ONLY.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this library, and
+ * only as the checker or the timed CPU baseline.  The product (rosy_amd) never links or calls it.
+ *
+ * What it restates (citations relative to the reference tree):
+ *   - shadow.slang:6-20 and basic.slang:15-119: clip = (VP * W) * p for every vertex;
+ *   - the Vulkan fixed-function state of Graphics.cpp:4988-5045, 5195-5292, 5553-5653: cull FRONT
+ *     with front = CW, depth test GREATER_OR_EQUAL with later draws winning ties, depth clear 0,
+ *     depth bias in the shadow pass only; rasterised here by the fixed-point rules below;
+ *   - basic.slang:137-344, lighting.slang:5-43, pcf.slang:1-12 and the compare sampler of
+ *     Graphics.cpp:2697-2725: deferred Cook-Torrance + PCF shading, in FP64.
+ *
+ * Fixed-point rules (identical by definition in the HIP kernels; Vulkan leaves them to the driver):
+ *   clip row r    = fma(M[2][r], z, fma(M[1][r], y, fma(M[0][r], x, M[3][r]))), M = VP * W by an
+ *                   fmaf chain in k order;
+ *   clip planes   = w-z >= 0 (near, reverse-Z), z >= 0 (far), |x|,|y| <= 4w guard band;
+ *                   Sutherland-Hodgman in that order, intersections from the inside vertex, fan;
+ *   snap          = X = rint(256 * fma(x/w, W/2, W/2)), Y likewise; 8 sub-pixel bits;
+ *   cull          = area2 = (X1-X0)(Y2-Y0)-(X2-X0)(Y1-Y0); front iff area2 > 0 (front = CW);
+ *   coverage      = pixel centres, int64 edge functions, top-left rule (y down);
+ *   depth         = plane from the snapped vertices in FP32 (no contraction), clamped to [0,1];
+ *   depth bias    = o = m*slope + r*constant, m = max|dz/dx|,|dz/dy| per pixel, r = 2^(e-23);
+ *   visibility    = max over fragments of (depth bits << 32 | tri + 1).
+ */
+#ifndef ROSY_ORACLE_H
+#define ROSY_ORACLE_H
+
+#include <stdint.h>
+
+#include "rosy_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_geometry {
+    const rsy_position* vertices;
+    uint64_t n_vertices;
+    const uint32_t* indices;
+    uint64_t n_indices;
+    const rsy_mesh_range* meshes;
+    uint32_t n_meshes;
+    uint32_t n_objects;
+    const rsy_object_record* objects;
+    const rsy_draw* draws; /* the draw list of the view, in order */
+    uint32_t n_draws;
+    uint32_t reserved;
+} orc_geometry;
+
+typedef struct orc_raster_stats {
+    uint64_t triangles_in;
+    uint64_t triangles_setup; /* records after clip/cull/bbox */
+    uint64_t triangles_clipped;
+    uint64_t fragments;       /* covered pixel samples */
+} orc_raster_stats;
+
+/* Rasterise one view.  is_shadow = 0: camera view, writes out_vis (w*h u64, zero = background).
+ * is_shadow = 1: depth-only shadow view with depth bias, writes out_depth (w*h float). */
+int orc_raster_view(const orc_geometry* g, const float view_projection[16], uint32_t width, uint32_t height,
+                    int is_shadow, const rsy_raster_state* rs, uint64_t* out_vis, float* out_depth,
+                    orc_raster_stats* stats, int n_threads);
+
+/* Number of (record, tile) pairs when records are binned by their pixel bounding boxes into
+ * tile x tile bins (the bin traffic term of the roofline). */
+uint64_t orc_count_tile_pairs(const orc_geometry* g, const float view_projection[16], uint32_t width,
+                              uint32_t height, int is_shadow, const rsy_raster_state* rs, uint32_t tile,
+                              int n_threads);
+
+typedef struct orc_shade_in {
+    const rsy_gpu_material* materials;
+    uint32_t n_materials;
+    uint32_t n_textures;
+    const rsy_texture_desc* textures;
+    const rsy_sampler* samplers;
+    uint32_t n_samplers;
+    uint32_t n_cascades;
+    const rsy_shadow_cascade* cascades;
+    const float* const* shadow_maps; /* n_cascades maps, size*size floats each */
+    const rsy_scene_data* scene;
+    const rsy_raster_state* raster;
+    const rsy_camera_view* view;
+} orc_shade_in;
+
+/* FP64 deferred shading of one camera view from its visibility buffer.
+ * out_rgba: w*h*4 doubles.  out_shadow (nullable): per-pixel PCF factor (1 where not evaluated).
+ * out_margin (nullable): per-pixel min |ref - texel| over compares with non-zero weight.
+ * override_shadow (nullable): use these per-pixel shadow factors instead of the PCF result. */
+int orc_shade_view(const orc_geometry* g, const orc_shade_in* in, const uint64_t* vis, double* out_rgba,
+                   double* out_shadow, double* out_margin, const float* override_shadow, int n_threads);
+
+/* Exact sRGB EOTF (basic.slang:121-128 form) of an 8-bit code. */
+double orc_srgb_to_linear(uint32_t code);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

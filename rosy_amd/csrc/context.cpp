@@ -1,0 +1,1071 @@
+// Host orchestration behind the C ABI: device buffers, draw lists, frame layout, kernel sequence,
+// HIP-graph capture, overflow recovery and readback.
+//
+// Stands in for graphics_device (src/Engine/Graphics.cpp:443-6111): set_asset (3525-4689),
+// set_graphic_objects (4691-4921), update_graphic_objects (4923-4927), update (6016-6061) and
+// render (4934-6014).  One context owns one HIP stream; every call on a context is serialised by the
+// caller, as the reference's single-threaded main loop does (Engine.cpp:175-224).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "frame_types.h"
+#include "host_math.h"
+#include "kernels.h"
+#include "rosy_amd.h"
+
+using namespace rsy;
+
+namespace {
+
+constexpr uint32_t kMaxViewExtent = 8192;
+constexpr uint32_t kMaxViews = 16;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    // Grows (never shrinks); contents are not preserved.
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        release();
+        const size_t want = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        cap = want;
+        return hipSuccess;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipHostMalloc(&p, std::max<size_t>(bytes, 256), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        cap = std::max<size_t>(bytes, 256);
+        return hipSuccess;
+    }
+};
+
+struct FrameViewDesc {
+    bool shadow;
+    uint32_t width, height;
+    float vp[16];
+    // camera only
+    float camera_position[4];
+    // shadow only
+    float split_far;
+};
+
+struct Layout {
+    // key
+    std::vector<FrameViewDesc> views;
+    uint32_t tile = 0;
+    uint64_t lists_version = 0;
+    bool shade = false;
+    // derived
+    std::vector<ViewGPU> vg;
+    std::vector<BlockEntry> vblocks, tblocks;
+    uint32_t total_bins = 0, total_mvps = 0, max_draws = 0;
+    uint64_t total_xv = 0;
+    uint32_t rec_total = 0, clip_total = 0;
+    uint32_t n_shadow = 0, n_camera = 0;
+    bool valid = false;
+};
+
+bool same_views(const std::vector<FrameViewDesc>& a, const std::vector<FrameViewDesc>& b) {
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); ++i)
+        if (a[i].shadow != b[i].shadow || a[i].width != b[i].width || a[i].height != b[i].height) return false;
+    return true;
+}
+
+uint32_t ilog2(uint32_t v) {
+    uint32_t r = 0;
+    while ((1u << (r + 1)) <= v) ++r;
+    return r;
+}
+
+}  // namespace
+
+struct rsy_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    hipEvent_t ev_stage[5] = {};
+    std::string err;
+    uint32_t debug_flags = 0;
+
+    // geometry
+    DevBuf vertices, px, py, pz, indices;
+    uint64_t n_vertices = 0, n_indices = 0;
+    std::vector<rsy_mesh_range> meshes;
+    bool has_geometry = false;
+    // objects + surfaces
+    DevBuf objects;
+    std::vector<rsy_object_record> h_objects;
+    std::vector<rsy_draw> surfaces;
+    bool has_objects = false;
+    // materials
+    DevBuf materials, tex_table, texels, samplers, srgb_lut;
+    uint32_t n_materials = 0, n_textures = 0, n_samplers = 0;
+    std::vector<uint32_t> tex_channels;
+    // draw lists: 0 = shadow (all), 1 = opaque (!blended), 2 = blended (kept, not rasterised)
+    std::vector<DrawGPU> lists[3];
+    std::vector<uint32_t> list_index[3];  // surface indices
+    uint32_t list_tris[3] = {0, 0, 0};
+    uint64_t list_verts[3] = {0, 0, 0};
+    DevBuf lists_d[2];
+    uint64_t lists_version = 0;
+    bool lists_dirty = true;
+    // per-frame uniforms
+    rsy_scene_data sd{};
+    rsy_raster_state rs{};
+    bool has_scene = false;
+    DevBuf scene_d;
+
+    // frame buffers
+    Layout layout;
+    DevBuf views_d, mvps, xv, recs, clipq, counters, bin_counts, bin_offsets, cursor, block_sums, total_pairs,
+        lists_mem, vblocks_d, tblocks_d, shade_views_d, shade_cascades_d, xv_bases_d;
+    uint32_t list_cap = 0;
+    uint32_t rec_scale = 1;  // grows on overflow
+    std::vector<DevBuf> vis_out, color_out, shadow_factor_out, shadow_maps;
+    PinnedBuf staging, readback_counters;
+    std::vector<ShadeView> h_shade_views;
+    std::vector<ShadeCascade> h_shade_cascades;
+
+    // last frame (for re-running after an overflow)
+    std::vector<FrameViewDesc> last_views;
+    uint32_t last_shade = 0;
+    bool frame_pending = false;
+    bool frame_validated = false;
+    rsy_frame_stats stats{};
+    uint32_t autotuned_tile = 0;
+
+    // graph cache
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_valid = false;
+
+    hipError_t last = hipSuccess;
+};
+
+namespace {
+
+rsy_result fail(rsy_ctx* c, rsy_result r, const std::string& msg) {
+    if (c) c->err = msg;
+    return r;
+}
+
+rsy_result hip_fail(rsy_ctx* c, hipError_t e, const char* what) {
+    if (c) c->err = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation ? RSY_ALLOCATION_FAILURE : RSY_GRAPHICS_FRAME_FAILURE;
+}
+
+#define HIP_TRY(ctx, expr, what)                    \
+    do {                                            \
+        hipError_t e_ = (expr);                     \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, what); \
+    } while (0)
+
+void invalidate_graph(rsy_ctx* c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    c->graph_exec = nullptr;
+    c->graph_valid = false;
+}
+
+// Rebuilds the per-list draw tables (Graphics.cpp:4716-4735: shadow = every surface,
+// opaque = !blended, blended = blended) from the surfaces and the mesh table.
+rsy_result rebuild_lists(rsy_ctx* c) {
+    for (int l = 0; l < 3; ++l) {
+        c->lists[l].clear();
+        c->list_index[l].clear();
+        c->list_tris[l] = 0;
+        c->list_verts[l] = 0;
+    }
+    for (uint32_t i = 0; i < c->surfaces.size(); ++i) {
+        const rsy_draw& s = c->surfaces[i];
+        if (s.mesh_index >= c->meshes.size()) return fail(c, RSY_INVALID_ARGUMENT, "surface mesh_index out of range");
+        const rsy_mesh_range& m = c->meshes[s.mesh_index];
+        const uint64_t obj = s.graphic_objects_offset + s.graphics_object_index;
+        if (obj >= c->h_objects.size()) return fail(c, RSY_INVALID_ARGUMENT, "surface object index out of range");
+        if (uint64_t(s.start_index) + s.index_count > m.num_indices)
+            return fail(c, RSY_INVALID_ARGUMENT, "surface index range exceeds its mesh");
+        DrawGPU d{};
+        d.first_index = m.index_offset + s.start_index;
+        d.tri_count = s.index_count / 3;
+        d.vtx_base = uint32_t(m.vertex_buffer_offset / sizeof(rsy_position));
+        d.vtx_count = m.num_vertices;
+        d.object = uint32_t(obj);
+        d.material = s.material_index < 0xFFFFFFFFull ? uint32_t(s.material_index) : 0xFFFFFFFFu;
+        const int lists_of[2] = {0, s.blended ? 2 : 1};
+        for (int l : lists_of) {
+            d.tri_base = c->list_tris[l];
+            d.xv_off = uint32_t(c->list_verts[l]);
+            c->lists[l].push_back(d);
+            c->list_index[l].push_back(i);
+            c->list_tris[l] += d.tri_count;
+            c->list_verts[l] += d.vtx_count;
+        }
+    }
+    for (int l = 0; l < 3; ++l)
+        if (c->list_tris[l] >= 0x7fffffffu || c->list_verts[l] >= 0x7fffffffull)
+            return fail(c, RSY_OVERFLOW, "draw list too large");
+    for (int l = 0; l < 2; ++l) {
+        const size_t bytes = std::max<size_t>(1, c->lists[l].size()) * sizeof(DrawGPU);
+        HIP_TRY(c, c->lists_d[l].ensure(bytes), "allocate draw list");
+        if (!c->lists[l].empty())
+            HIP_TRY(c, hipMemcpy(c->lists_d[l].p, c->lists[l].data(), c->lists[l].size() * sizeof(DrawGPU), hipMemcpyHostToDevice),
+                    "upload draw list");
+    }
+    c->lists_version++;
+    c->lists_dirty = false;
+    invalidate_graph(c);
+    return RSY_OK;
+}
+
+rsy_result resolve_views(rsy_ctx* c, const rsy_frame_desc* f, std::vector<FrameViewDesc>& views, uint32_t& shade) {
+    views.clear();
+    shade = f ? f->shade_mode : RSY_SHADE_PLAIN;
+    if (shade > RSY_SHADE_NEURAL) return fail(c, RSY_INVALID_ARGUMENT, "unknown shade mode");
+    if (shade == RSY_SHADE_NEURAL) return fail(c, RSY_INVALID_STATE, "neural material decoder not loaded");
+    // shadow cascades first (the reference records the shadow pass before the main pass)
+    if (f && f->n_cascades) {
+        if (!f->cascades) return fail(c, RSY_INVALID_ARGUMENT, "cascades == NULL");
+        for (uint32_t i = 0; i < f->n_cascades; ++i) {
+            FrameViewDesc v{};
+            v.shadow = true;
+            v.width = v.height = f->cascades[i].size;
+            std::memcpy(v.vp, f->cascades[i].light_view_projection, sizeof(v.vp));
+            v.split_far = f->cascades[i].split_far;
+            views.push_back(v);
+        }
+    } else {
+        FrameViewDesc v{};
+        v.shadow = true;
+        v.width = v.height = (f && f->default_shadow_size) ? f->default_shadow_size : 2048u;
+        std::memcpy(v.vp, c->sd.shadow_projection_near, sizeof(v.vp));
+        v.split_far = INFINITY;
+        views.push_back(v);
+    }
+    if (f && f->n_camera_views) {
+        if (!f->camera_views) return fail(c, RSY_INVALID_ARGUMENT, "camera_views == NULL");
+        for (uint32_t i = 0; i < f->n_camera_views; ++i) {
+            FrameViewDesc v{};
+            v.shadow = false;
+            v.width = f->camera_views[i].width;
+            v.height = f->camera_views[i].height;
+            std::memcpy(v.vp, f->camera_views[i].view_projection, sizeof(v.vp));
+            std::memcpy(v.camera_position, f->camera_views[i].camera_position, sizeof(v.camera_position));
+            views.push_back(v);
+        }
+    } else {
+        FrameViewDesc v{};
+        v.shadow = false;
+        v.width = uint32_t(c->sd.draw_extent[0]);
+        v.height = uint32_t(c->sd.draw_extent[1]);
+        std::memcpy(v.vp, c->sd.view_projection, sizeof(v.vp));
+        std::memcpy(v.camera_position, c->sd.camera_position, sizeof(v.camera_position));
+        views.push_back(v);
+    }
+    if (views.size() > kMaxViews) return fail(c, RSY_INVALID_ARGUMENT, "too many views");
+    for (const auto& v : views)
+        if (v.width == 0 || v.height == 0 || v.width > kMaxViewExtent || v.height > kMaxViewExtent)
+            return fail(c, RSY_INVALID_ARGUMENT, "view extent must be in [1, 8192]");
+    return RSY_OK;
+}
+
+uint32_t choose_tile(const rsy_ctx* c) {
+    if (c->rs.tile_size == 8 || c->rs.tile_size == 16 || c->rs.tile_size == 32) return c->rs.tile_size;
+    return c->autotuned_tile ? c->autotuned_tile : 16u;
+}
+
+// Computes bin, vertex, record and clip-queue layout of a frame and (re)allocates its buffers.
+rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint32_t tile, bool shade) {
+    Layout& L = c->layout;
+    const bool reuse = L.valid && same_views(L.views, views) && L.tile == tile && L.lists_version == c->lists_version &&
+                       L.shade == shade;
+    L.views = views;  // matrices may change without a new layout
+    if (reuse) {
+        for (size_t i = 0; i < views.size(); ++i) std::memcpy(L.vg[i].view_projection, views[i].vp, sizeof(views[i].vp));
+        return RSY_OK;
+    }
+    invalidate_graph(c);
+    L.valid = false;
+    L.tile = tile;
+    L.lists_version = c->lists_version;
+    L.shade = shade;
+    L.vg.clear();
+    L.vblocks.clear();
+    L.tblocks.clear();
+    L.total_bins = L.total_mvps = L.max_draws = 0;
+    L.total_xv = 0;
+    L.rec_total = L.clip_total = 0;
+    L.n_shadow = L.n_camera = 0;
+    const uint32_t shift = ilog2(tile);
+    for (uint32_t vi = 0; vi < views.size(); ++vi) {
+        const FrameViewDesc& f = views[vi];
+        ViewGPU g{};
+        g.width = f.width;
+        g.height = f.height;
+        g.tile = tile;
+        g.tiles_x = (f.width + tile - 1) >> shift;
+        g.tiles_y = (f.height + tile - 1) >> shift;
+        g.tile_base = L.total_bins;
+        g.is_shadow = f.shadow ? 1 : 0;
+        g.list = f.shadow ? 0 : 1;
+        const auto& list = c->lists[g.list];
+        g.n_draws = uint32_t(list.size());
+        g.mvp_base = L.total_mvps;
+        g.xv_base = uint32_t(L.total_xv);
+        const uint32_t tris = c->list_tris[g.list];
+        const uint32_t clip_cap = std::max<uint32_t>(4096, tris / 8) * c->rec_scale;
+        g.rec_base = L.rec_total;
+        g.rec_cap = tris + 2 * clip_cap;
+        g.clip_base = L.clip_total;
+        g.clip_cap = clip_cap;
+        g.cull = c->rs.cull_enabled ? 1 : 0;
+        g.reverse = c->rs.reverse_winding_order_enabled ? 1 : 0;
+        g.bias = c->rs.depth_bias_enabled ? 1 : 0;
+        g.bias_const = c->rs.depth_bias_constant;
+        g.bias_clamp = c->rs.depth_bias_clamp;
+        g.bias_slope = c->rs.depth_bias_slope_factor;
+        std::memcpy(g.view_projection, f.vp, sizeof(f.vp));
+        L.total_bins += g.tiles_x * g.tiles_y;
+        L.total_mvps += g.n_draws;
+        L.max_draws = std::max(L.max_draws, g.n_draws);
+        L.total_xv += c->list_verts[g.list];
+        if (uint64_t(L.rec_total) + g.rec_cap >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "record buffer too large");
+        L.rec_total += g.rec_cap;
+        L.clip_total += g.clip_cap;
+        for (uint32_t d = 0; d < g.n_draws; ++d) {
+            for (uint32_t s = 0; s < list[d].vtx_count; s += 256) L.vblocks.push_back({vi, d, s, 0});
+            for (uint32_t s = 0; s < list[d].tri_count; s += 256) L.tblocks.push_back({vi, d, s, 0});
+        }
+        if (f.shadow) ++L.n_shadow; else ++L.n_camera;
+        L.vg.push_back(g);
+    }
+    if (L.total_xv >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "transformed vertex buffer too large");
+    // buffers
+    const size_t nv = views.size();
+    HIP_TRY(c, c->views_d.ensure(nv * sizeof(ViewGPU)), "allocate views");
+    HIP_TRY(c, c->mvps.ensure(size_t(std::max<uint32_t>(1, L.total_mvps)) * 64), "allocate mvps");
+    HIP_TRY(c, c->xv.ensure(size_t(std::max<uint64_t>(1, L.total_xv)) * sizeof(float4)), "allocate vertices");
+    HIP_TRY(c, c->recs.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * sizeof(TriRec)), "allocate records");
+    HIP_TRY(c, c->clipq.ensure(size_t(std::max<uint32_t>(1, L.clip_total)) * sizeof(uint2)), "allocate clip queue");
+    HIP_TRY(c, c->counters.ensure(nv * kCtrPerView * sizeof(uint32_t)), "allocate counters");
+    HIP_TRY(c, c->bin_counts.ensure(size_t(L.total_bins) * 4), "allocate bins");
+    HIP_TRY(c, c->bin_offsets.ensure(size_t(L.total_bins) * 4), "allocate bins");
+    HIP_TRY(c, c->cursor.ensure(size_t(L.total_bins) * 4), "allocate bins");
+    HIP_TRY(c, c->block_sums.ensure(size_t(L.total_bins / 2048 + 2) * 4), "allocate scan");
+    HIP_TRY(c, c->total_pairs.ensure(16), "allocate scan");
+    HIP_TRY(c, c->vblocks_d.ensure(std::max<size_t>(1, L.vblocks.size()) * sizeof(BlockEntry)), "allocate blocks");
+    HIP_TRY(c, c->tblocks_d.ensure(std::max<size_t>(1, L.tblocks.size()) * sizeof(BlockEntry)), "allocate blocks");
+    if (!L.vblocks.empty())
+        HIP_TRY(c, hipMemcpy(c->vblocks_d.p, L.vblocks.data(), L.vblocks.size() * sizeof(BlockEntry), hipMemcpyHostToDevice), "upload blocks");
+    if (!L.tblocks.empty())
+        HIP_TRY(c, hipMemcpy(c->tblocks_d.p, L.tblocks.data(), L.tblocks.size() * sizeof(BlockEntry), hipMemcpyHostToDevice), "upload blocks");
+    // bin lists: start from an estimate, grow on overflow
+    uint64_t want_pairs = 0;
+    for (const auto& g : L.vg) want_pairs += uint64_t(c->list_tris[g.list]) * 3 + g.tiles_x * g.tiles_y;
+    want_pairs *= c->rec_scale;
+    if (want_pairs > c->list_cap) {
+        want_pairs = std::min<uint64_t>(want_pairs, 0xfffffff0ull);
+        HIP_TRY(c, c->lists_mem.ensure(size_t(want_pairs) * 4), "allocate bin lists");
+        c->list_cap = uint32_t(want_pairs);
+    }
+    // outputs
+    c->vis_out.resize(L.n_camera);
+    c->color_out.resize(L.n_camera);
+    c->shadow_factor_out.resize(L.n_camera);
+    c->shadow_maps.resize(L.n_shadow);
+    uint32_t ci = 0, si = 0;
+    for (auto& g : L.vg) {
+        const size_t px = size_t(g.width) * g.height;
+        if (g.is_shadow) {
+            HIP_TRY(c, c->shadow_maps[si].ensure(px * 4), "allocate shadow map");
+            g.out = reinterpret_cast<uint64_t>(c->shadow_maps[si].p);
+            ++si;
+        } else {
+            HIP_TRY(c, c->vis_out[ci].ensure(px * 8), "allocate visibility buffer");
+            HIP_TRY(c, c->color_out[ci].ensure(px * 16), "allocate colour buffer");
+            HIP_TRY(c, c->shadow_factor_out[ci].ensure(px * 4), "allocate shadow factor");
+            g.out = reinterpret_cast<uint64_t>(c->vis_out[ci].p);
+            ++ci;
+        }
+    }
+    HIP_TRY(c, c->staging.ensure(nv * sizeof(ViewGPU) + 64 * 1024), "allocate staging");
+    HIP_TRY(c, c->readback_counters.ensure(nv * kCtrPerView * 4 + 16), "allocate readback");
+    L.valid = true;
+    return RSY_OK;
+}
+
+// Enqueues one frame on the context stream (graph-capturable: kernels and pinned-memory copies only).
+rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
+    Layout& L = c->layout;
+    hipStream_t s = c->stream;
+    const uint32_t nv = uint32_t(L.vg.size());
+    const uint32_t shift = ilog2(L.tile);
+    const bool timing = (c->debug_flags & RSY_DEBUG_STAGE_TIMING) && !capturing;
+    if (!capturing) HIP_TRY(c, hipEventRecord(c->ev_begin, s), "event");
+    // views -> device through pinned staging
+    std::memcpy(c->staging.p, L.vg.data(), nv * sizeof(ViewGPU));
+    HIP_TRY(c, hipMemcpyAsync(c->views_d.p, c->staging.p, nv * sizeof(ViewGPU), hipMemcpyHostToDevice, s), "upload views");
+    HIP_TRY(c, launch_fill_u32(c->counters.as<uint32_t>(), nv * kCtrPerView, 0u, s), "clear counters");
+    HIP_TRY(c, launch_fill_u32(c->bin_counts.as<uint32_t>(), L.total_bins, 0u, s), "clear bins");
+
+    GeometryLaunch g{};
+    g.views = c->views_d.as<ViewGPU>();
+    g.n_views = nv;
+    g.max_draws = std::max<uint32_t>(1, L.max_draws);
+    g.draws0 = c->lists_d[0].as<DrawGPU>();
+    g.draws1 = c->lists_d[1].as<DrawGPU>();
+    g.objects = c->objects.as<float>();
+    g.mvps = c->mvps.as<float>();
+    g.vblocks = c->vblocks_d.as<BlockEntry>();
+    g.n_vblocks = uint32_t(L.vblocks.size());
+    g.tblocks = c->tblocks_d.as<BlockEntry>();
+    g.n_tblocks = uint32_t(L.tblocks.size());
+    g.px = c->px.as<float>();
+    g.py = c->py.as<float>();
+    g.pz = c->pz.as<float>();
+    g.indices = c->indices.as<uint32_t>();
+    g.xv = c->xv.as<float4>();
+    g.recs = c->recs.as<TriRec>();
+    g.clipq = c->clipq.as<uint2>();
+    g.counters = c->counters.as<uint32_t>();
+    g.bin_counts = c->bin_counts.as<uint32_t>();
+    g.shift = shift;
+    g.clip_blocks = 64;
+    HIP_TRY(c, launch_geometry(g, s), "geometry kernels");
+    if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[0], s), "event");
+
+    BinLaunch b{};
+    b.views = g.views;
+    b.n_views = nv;
+    b.total_bins = L.total_bins;
+    b.bin_counts = g.bin_counts;
+    b.bin_offsets = c->bin_offsets.as<uint32_t>();
+    b.cursor = c->cursor.as<uint32_t>();
+    b.block_sums = c->block_sums.as<uint32_t>();
+    b.total_pairs = c->total_pairs.as<uint32_t>();
+    b.recs = g.recs;
+    b.counters = g.counters;
+    b.lists = c->lists_mem.as<uint32_t>();
+    b.list_cap = c->list_cap;
+    b.shift = shift;
+    b.scatter_blocks = 512;
+    HIP_TRY(c, launch_binning(b, s), "binning kernels");
+    if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[1], s), "event");
+
+    RasterLaunch r{};
+    r.views = g.views;
+    r.n_views = nv;
+    r.total_bins = L.total_bins;
+    r.tile = L.tile;
+    r.bin_counts = g.bin_counts;
+    r.bin_offsets = b.bin_offsets;
+    r.lists = b.lists;
+    r.list_cap = c->list_cap;
+    r.recs = g.recs;
+    r.rec_cap_total = L.rec_total;
+    HIP_TRY(c, launch_raster(r, s), "raster kernel");
+    if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[2], s), "event");
+
+    if (L.shade && L.n_camera) {
+        ShadeLaunch sh{};
+        // shading views and cascades (tiny; staged right after the view records)
+        c->h_shade_views.clear();
+        c->h_shade_cascades.clear();
+        std::vector<uint32_t> xv_bases;
+        uint32_t blocks = 0, ci = 0;
+        for (uint32_t i = 0; i < nv; ++i) {
+            const ViewGPU& vg = L.vg[i];
+            if (vg.is_shadow) {
+                ShadeCascade sc{};
+                sc.size = vg.width;
+                sc.split_far = L.views[i].split_far;
+                sc.map = reinterpret_cast<const float*>(vg.out);
+                std::memcpy(sc.m, vg.view_projection, sizeof(sc.m));
+                c->h_shade_cascades.push_back(sc);
+            } else {
+                ShadeView sv{};
+                sv.width = vg.width;
+                sv.height = vg.height;
+                sv.blocks_x = (vg.width + 15) / 16;
+                sv.block_base = blocks;
+                blocks += sv.blocks_x * ((vg.height + 15) / 16);
+                sv.vis = reinterpret_cast<const unsigned long long*>(vg.out);
+                sv.color = c->color_out[ci].as<float4>();
+                sv.shadow_factor = (c->debug_flags & RSY_DEBUG_SHADOW_FACTOR) ? c->shadow_factor_out[ci].as<float>() : nullptr;
+                std::memcpy(sv.view_projection, vg.view_projection, sizeof(sv.view_projection));
+                std::memcpy(sv.camera_position, L.views[i].camera_position, sizeof(sv.camera_position));
+                c->h_shade_views.push_back(sv);
+                xv_bases.push_back(vg.xv_base);
+                ++ci;
+            }
+        }
+        const size_t off_views = 4096 * 4;  // inside staging, after the ViewGPU records
+        const size_t nsv = c->h_shade_views.size(), nsc = c->h_shade_cascades.size();
+        const size_t bytes = nsv * sizeof(ShadeView) + nsc * sizeof(ShadeCascade) + nsv * 4 + 64;
+        HIP_TRY(c, c->shade_views_d.ensure(nsv * sizeof(ShadeView) + 64), "allocate shade views");
+        HIP_TRY(c, c->shade_cascades_d.ensure(std::max<size_t>(1, nsc) * sizeof(ShadeCascade) + 64), "allocate shade cascades");
+        HIP_TRY(c, c->xv_bases_d.ensure(nsv * 4 + 64), "allocate xv bases");
+        if (c->staging.cap < nv * sizeof(ViewGPU) + off_views + bytes)
+            return fail(c, RSY_ALLOCATION_FAILURE, "staging too small");
+        uint8_t* st = static_cast<uint8_t*>(c->staging.p) + nv * sizeof(ViewGPU);
+        st = reinterpret_cast<uint8_t*>((reinterpret_cast<uintptr_t>(st) + 255) & ~uintptr_t(255));
+        std::memcpy(st, c->h_shade_views.data(), nsv * sizeof(ShadeView));
+        uint8_t* st2 = st + ((nsv * sizeof(ShadeView) + 255) & ~size_t(255));
+        if (nsc) std::memcpy(st2, c->h_shade_cascades.data(), nsc * sizeof(ShadeCascade));
+        uint8_t* st3 = st2 + ((nsc * sizeof(ShadeCascade) + 255) & ~size_t(255));
+        std::memcpy(st3, xv_bases.data(), nsv * 4);
+        HIP_TRY(c, hipMemcpyAsync(c->shade_views_d.p, st, nsv * sizeof(ShadeView), hipMemcpyHostToDevice, s), "upload");
+        if (nsc)
+            HIP_TRY(c, hipMemcpyAsync(c->shade_cascades_d.p, st2, nsc * sizeof(ShadeCascade), hipMemcpyHostToDevice, s), "upload");
+        HIP_TRY(c, hipMemcpyAsync(c->xv_bases_d.p, st3, nsv * 4, hipMemcpyHostToDevice, s), "upload");
+        sh.views = c->shade_views_d.as<ShadeView>();
+        sh.n_views = uint32_t(nsv);
+        sh.total_blocks = blocks;
+        sh.cascades = c->shade_cascades_d.as<ShadeCascade>();
+        sh.n_cascades = uint32_t(nsc);
+        sh.pcf_mode = c->rs.pcf_mode;
+        sh.scene = c->scene_d.p;
+        sh.draws = c->lists_d[1].as<DrawGPU>();
+        sh.n_draws = uint32_t(c->lists[1].size());
+        sh.indices = g.indices;
+        sh.vertices = c->vertices.p;
+        sh.objects = g.objects;
+        sh.materials = c->materials.p;
+        sh.n_materials = c->n_materials;
+        sh.textures = c->tex_table.as<TexGPU>();
+        sh.n_textures = c->n_textures;
+        sh.texels = c->texels.as<uint8_t>();
+        sh.samplers = c->samplers.p;
+        sh.n_samplers = c->n_samplers;
+        std::memcpy(sh.clear_color, c->rs.clear_color, sizeof(sh.clear_color));
+        sh.srgb_lut = c->srgb_lut.as<float>();
+        sh.xv = g.xv;
+        sh.xv_bases = c->xv_bases_d.as<uint32_t>();
+        HIP_TRY(c, launch_shade(sh, s), "shade kernel");
+    }
+    if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[3], s), "event");
+    if (!capturing) {
+        HIP_TRY(c, hipMemcpyAsync(c->readback_counters.p, c->counters.p, nv * kCtrPerView * 4, hipMemcpyDeviceToHost, s), "readback");
+        HIP_TRY(c, hipMemcpyAsync(static_cast<uint8_t*>(c->readback_counters.p) + nv * kCtrPerView * 4, c->total_pairs.p, 4,
+                                  hipMemcpyDeviceToHost, s), "readback");
+        HIP_TRY(c, hipEventRecord(c->ev_end, s), "event");
+    }
+    return RSY_OK;
+}
+
+rsy_result ready_to_render(rsy_ctx* c) {
+    if (!c->has_geometry) return fail(c, RSY_INVALID_STATE, "rsy_set_geometry has not been called");
+    if (!c->has_objects) return fail(c, RSY_INVALID_STATE, "rsy_set_graphic_objects has not been called");
+    if (!c->has_scene) return fail(c, RSY_INVALID_STATE, "rsy_update has not been called");
+    if (c->lists_dirty) {
+        rsy_result r = rebuild_lists(c);
+        if (r != RSY_OK) return r;
+    }
+    return RSY_OK;
+}
+
+// Reads the last frame's counters (after a synchronize) and reports whether it overflowed.
+bool collect_stats(rsy_ctx* c) {
+    const Layout& L = c->layout;
+    const uint32_t nv = uint32_t(L.vg.size());
+    const uint32_t* ctr = static_cast<const uint32_t*>(c->readback_counters.p);
+    const uint32_t total_pairs = ctr[nv * kCtrPerView];
+    bool overflow = total_pairs > c->list_cap;
+    rsy_frame_stats st{};
+    for (uint32_t v = 0; v < nv; ++v) {
+        const uint32_t* vc = ctr + v * kCtrPerView;
+        overflow |= vc[kCtrOverflow] != 0;
+        st.triangles_setup += vc[kCtrRecords];
+        st.triangles_clipped += vc[kCtrClipped];
+        st.tile_pairs += vc[kCtrPairs];
+        const uint32_t list = L.vg[v].list;
+        st.triangle_count += c->list_tris[list];
+        st.draw_call_count += c->lists[list].size();
+    }
+    st.tile_size = L.tile;
+    st.overflowed = overflow ? 1 : 0;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev_begin, c->ev_end) == hipSuccess) st.gpu_ms = ms;
+    if (c->debug_flags & RSY_DEBUG_STAGE_TIMING) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, c->ev_begin, c->ev_stage[0]) == hipSuccess) st.stage_ms[0] = t;
+        if (hipEventElapsedTime(&t, c->ev_stage[0], c->ev_stage[1]) == hipSuccess) st.stage_ms[1] = t;
+        if (hipEventElapsedTime(&t, c->ev_stage[1], c->ev_stage[2]) == hipSuccess) st.stage_ms[2] = t;
+        if (hipEventElapsedTime(&t, c->ev_stage[2], c->ev_stage[3]) == hipSuccess) st.stage_ms[3] = t;
+    }
+    c->stats = st;
+    return overflow;
+}
+
+rsy_result render_now(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint32_t shade) {
+    rsy_result r = build_layout(c, views, choose_tile(c), shade != RSY_SHADE_NONE);
+    if (r != RSY_OK) return r;
+    r = enqueue_frame(c, false);
+    if (r != RSY_OK) return r;
+    c->last_views = views;
+    c->last_shade = shade;
+    c->frame_pending = true;
+    return RSY_OK;
+}
+
+// Waits for the stream; if the last frame overflowed a bin or record buffer, grows the buffers and
+// renders it again (bounded number of attempts).
+rsy_result finish_frame(rsy_ctx* c) {
+    HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
+    if (!c->frame_pending) return RSY_OK;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        if (!collect_stats(c)) {
+            c->frame_pending = false;
+            return RSY_OK;
+        }
+        // grow: bin list capacity to the measured pair count, record/clip capacity x2
+        const uint32_t nv = uint32_t(c->layout.vg.size());
+        const uint32_t total_pairs = static_cast<const uint32_t*>(c->readback_counters.p)[nv * kCtrPerView];
+        bool rec_over = false;
+        for (uint32_t v = 0; v < nv; ++v)
+            rec_over |= static_cast<const uint32_t*>(c->readback_counters.p)[v * kCtrPerView + kCtrOverflow] != 0;
+        if (rec_over) {
+            c->rec_scale *= 2;
+            c->layout.valid = false;
+        }
+        if (total_pairs > c->list_cap) {
+            const uint64_t want = std::min<uint64_t>(uint64_t(total_pairs) + total_pairs / 4 + 1024, 0xfffffff0ull);
+            if (want <= c->list_cap) return fail(c, RSY_OVERFLOW, "bin lists exceed 2^32 entries");
+            HIP_TRY(c, c->lists_mem.ensure(size_t(want) * 4), "grow bin lists");
+            c->list_cap = uint32_t(want);
+            invalidate_graph(c);
+        }
+        rsy_result r = render_now(c, c->last_views, c->last_shade);
+        if (r != RSY_OK) return r;
+        HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
+    }
+    return fail(c, RSY_OVERFLOW, "frame still overflows after growing its buffers");
+}
+
+std::vector<float> make_srgb_lut() {
+    std::vector<float> lut(256);
+    for (int i = 0; i < 256; ++i) {
+        const double v = i / 255.0;
+        lut[i] = float(v <= 0.04045 ? v / 12.92 : std::pow((v + 0.055) / 1.055, 2.4));
+    }
+    return lut;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t rsy_abi_version(void) { return RSY_ABI_VERSION; }
+
+const char* rsy_last_error(const rsy_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+rsy_result rsy_create(const rsy_config* cfg, rsy_ctx** out_ctx) {
+    if (!out_ctx) return RSY_INVALID_ARGUMENT;
+    *out_ctx = nullptr;
+    if (cfg && cfg->abi_version != RSY_ABI_VERSION) return RSY_INVALID_ARGUMENT;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return RSY_GRAPHICS_INIT_FAILURE;
+    const int dev = cfg ? cfg->device : 0;
+    if (dev < 0 || dev >= n) return RSY_INVALID_ARGUMENT;
+    rsy_ctx* c = new (std::nothrow) rsy_ctx();
+    if (!c) return RSY_ALLOCATION_FAILURE;
+    c->device = dev;
+    if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess) {
+        delete c;
+        return RSY_GRAPHICS_INIT_FAILURE;
+    }
+    for (auto& e : c->ev_stage)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete c;
+            return RSY_GRAPHICS_INIT_FAILURE;
+        }
+    const std::vector<float> lut = make_srgb_lut();
+    if (c->srgb_lut.ensure(256 * 4) != hipSuccess ||
+        hipMemcpy(c->srgb_lut.p, lut.data(), 256 * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        c->scene_d.ensure(sizeof(rsy_scene_data)) != hipSuccess) {
+        delete c;
+        return RSY_ALLOCATION_FAILURE;
+    }
+    c->rs.tile_size = 0;
+    c->rs.clear_color[1] = 0.05f;
+    c->rs.clear_color[2] = 0.1f;
+    c->rs.clear_color[3] = 1.f;
+    *out_ctx = c;
+    return RSY_OK;
+}
+
+void rsy_destroy(rsy_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    invalidate_graph(ctx);
+    if (ctx->ev_begin) (void)hipEventDestroy(ctx->ev_begin);
+    if (ctx->ev_end) (void)hipEventDestroy(ctx->ev_end);
+    for (auto& e : ctx->ev_stage)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+rsy_result rsy_set_geometry(rsy_ctx* c, const rsy_position* vertices, uint64_t n_vertices, const uint32_t* indices,
+                            uint64_t n_indices, const rsy_mesh_range* meshes, uint32_t n_meshes) {
+    if (!c) return RSY_INVALID_ARGUMENT;
+    if ((n_vertices && !vertices) || (n_indices && !indices) || (n_meshes && !meshes))
+        return fail(c, RSY_INVALID_ARGUMENT, "null geometry pointer");
+    if (n_vertices >= 0xffffffffull || n_indices >= 0xffffffffull)
+        return fail(c, RSY_OVERFLOW, "geometry exceeds 2^32 vertices or indices");
+    for (uint32_t m = 0; m < n_meshes; ++m) {
+        const rsy_mesh_range& r = meshes[m];
+        if (r.vertex_buffer_offset % sizeof(rsy_position) != 0)
+            return fail(c, RSY_INVALID_ARGUMENT, "vertex_buffer_offset must be a multiple of 64");
+        const uint64_t vb = r.vertex_buffer_offset / sizeof(rsy_position);
+        if (vb + r.num_vertices > n_vertices || uint64_t(r.index_offset) + r.num_indices > n_indices)
+            return fail(c, RSY_INVALID_ARGUMENT, "mesh range outside the buffers");
+        for (uint64_t i = r.index_offset; i < uint64_t(r.index_offset) + r.num_indices; ++i)
+            if (indices[i] >= r.num_vertices) return fail(c, RSY_INVALID_ARGUMENT, "index outside its mesh");
+    }
+    HIP_TRY(c, hipSetDevice(c->device), "set device");
+    HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
+    const size_t nv = std::max<uint64_t>(1, n_vertices);
+    HIP_TRY(c, c->vertices.ensure(nv * sizeof(rsy_position)), "allocate vertices");
+    HIP_TRY(c, c->px.ensure(nv * 4), "allocate positions");
+    HIP_TRY(c, c->py.ensure(nv * 4), "allocate positions");
+    HIP_TRY(c, c->pz.ensure(nv * 4), "allocate positions");
+    HIP_TRY(c, c->indices.ensure(std::max<uint64_t>(1, n_indices) * 4), "allocate indices");
+    if (n_vertices) {
+        HIP_TRY(c, hipMemcpy(c->vertices.p, vertices, n_vertices * sizeof(rsy_position), hipMemcpyHostToDevice), "upload vertices");
+        std::vector<float> sx(n_vertices), sy(n_vertices), sz(n_vertices);
+        for (uint64_t i = 0; i < n_vertices; ++i) {
+            sx[i] = vertices[i].vertex[0];
+            sy[i] = vertices[i].vertex[1];
+            sz[i] = vertices[i].vertex[2];
+        }
+        HIP_TRY(c, hipMemcpy(c->px.p, sx.data(), n_vertices * 4, hipMemcpyHostToDevice), "upload positions");
+        HIP_TRY(c, hipMemcpy(c->py.p, sy.data(), n_vertices * 4, hipMemcpyHostToDevice), "upload positions");
+        HIP_TRY(c, hipMemcpy(c->pz.p, sz.data(), n_vertices * 4, hipMemcpyHostToDevice), "upload positions");
+    }
+    if (n_indices) HIP_TRY(c, hipMemcpy(c->indices.p, indices, n_indices * 4, hipMemcpyHostToDevice), "upload indices");
+    c->n_vertices = n_vertices;
+    c->n_indices = n_indices;
+    c->meshes.assign(meshes, meshes + n_meshes);
+    c->has_geometry = true;
+    c->lists_dirty = true;
+    c->layout.valid = false;
+    invalidate_graph(c);
+    return RSY_OK;
+}
+
+rsy_result rsy_set_materials(rsy_ctx* c, const rsy_gpu_material* materials, uint32_t n_materials,
+                             const rsy_texture_desc* textures, uint32_t n_textures, const rsy_sampler* samplers,
+                             uint32_t n_samplers) {
+    if (!c) return RSY_INVALID_ARGUMENT;
+    if ((n_materials && !materials) || (n_textures && !textures) || (n_samplers && !samplers))
+        return fail(c, RSY_INVALID_ARGUMENT, "null material pointer");
+    std::vector<TexGPU> table(n_textures);
+    uint64_t pool = 0;
+    for (uint32_t t = 0; t < n_textures; ++t) {
+        const rsy_texture_desc& d = textures[t];
+        if (d.width == 0 || d.height == 0 || d.width > 16384 || d.height > 16384 || !d.pixels || d.format > RSY_TEX_RG8_SRGB)
+            return fail(c, RSY_INVALID_ARGUMENT, "invalid texture description");
+        const uint32_t ch = (d.format == RSY_TEX_RG8_UNORM || d.format == RSY_TEX_RG8_SRGB) ? 2 : 4;
+        table[t] = {pool, d.width, d.height, ch, (d.format == RSY_TEX_RGBA8_SRGB || d.format == RSY_TEX_RG8_SRGB) ? 1u : 0u};
+        pool += (uint64_t(d.width) * d.height * ch + 255) & ~uint64_t(255);
+    }
+    auto check_img = [&](uint32_t i) { return i == RSY_ABSENT || i < n_textures; };
+    for (uint32_t m = 0; m < n_materials; ++m) {
+        const rsy_gpu_material& g = materials[m];
+        if (!check_img(g.color_sampled_image_index) || !check_img(g.normal_sampled_image_index) ||
+            !check_img(g.metallic_sampled_image_index) || !check_img(g.mixmap_sampled_image_index))
+            return fail(c, RSY_INVALID_ARGUMENT, "material image index out of range");
+    }
+    HIP_TRY(c, hipSetDevice(c->device), "set device");
+    HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
+    HIP_TRY(c, c->materials.ensure(std::max<size_t>(1, n_materials) * sizeof(rsy_gpu_material)), "allocate materials");
+    HIP_TRY(c, c->tex_table.ensure(std::max<size_t>(1, n_textures) * sizeof(TexGPU)), "allocate textures");
+    HIP_TRY(c, c->texels.ensure(std::max<uint64_t>(256, pool)), "allocate texels");
+    HIP_TRY(c, c->samplers.ensure(std::max<size_t>(1, n_samplers) * sizeof(rsy_sampler)), "allocate samplers");
+    if (n_materials) HIP_TRY(c, hipMemcpy(c->materials.p, materials, n_materials * sizeof(rsy_gpu_material), hipMemcpyHostToDevice), "upload materials");
+    if (n_textures) HIP_TRY(c, hipMemcpy(c->tex_table.p, table.data(), n_textures * sizeof(TexGPU), hipMemcpyHostToDevice), "upload textures");
+    for (uint32_t t = 0; t < n_textures; ++t)
+        HIP_TRY(c, hipMemcpy(c->texels.as<uint8_t>() + table[t].offset, textures[t].pixels,
+                             uint64_t(table[t].width) * table[t].height * table[t].channels, hipMemcpyHostToDevice), "upload texels");
+    if (n_samplers) HIP_TRY(c, hipMemcpy(c->samplers.p, samplers, n_samplers * sizeof(rsy_sampler), hipMemcpyHostToDevice), "upload samplers");
+    c->n_materials = n_materials;
+    c->n_textures = n_textures;
+    c->n_samplers = n_samplers;
+    c->tex_channels.clear();
+    for (auto& t : table) c->tex_channels.push_back(t.channels);
+    invalidate_graph(c);
+    return RSY_OK;
+}
+
+rsy_result rsy_set_graphic_objects(rsy_ctx* c, const rsy_object_record* objects, uint32_t n_objects,
+                                   const rsy_draw* surfaces, uint32_t n_surfaces) {
+    if (!c) return RSY_INVALID_ARGUMENT;
+    // Graphics.cpp:4693-4697: an empty object set is an invalid argument
+    if (n_objects == 0 || !objects || (n_surfaces && !surfaces))
+        return fail(c, RSY_INVALID_ARGUMENT, "no graphic objects");
+    HIP_TRY(c, hipSetDevice(c->device), "set device");
+    HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
+    HIP_TRY(c, c->objects.ensure(size_t(n_objects) * sizeof(rsy_object_record)), "allocate objects");
+    HIP_TRY(c, hipMemcpy(c->objects.p, objects, size_t(n_objects) * sizeof(rsy_object_record), hipMemcpyHostToDevice), "upload objects");
+    c->h_objects.assign(objects, objects + n_objects);
+    c->surfaces.assign(surfaces, surfaces + n_surfaces);
+    c->has_objects = true;
+    c->lists_dirty = true;
+    c->layout.valid = false;
+    invalidate_graph(c);
+    if (c->has_geometry) return rebuild_lists(c);
+    return RSY_OK;
+}
+
+rsy_result rsy_update_graphic_objects(rsy_ctx* c, uint64_t offset, const rsy_object_record* objects, uint32_t n_objects) {
+    if (!c) return RSY_INVALID_ARGUMENT;
+    if (!c->has_objects) return fail(c, RSY_INVALID_STATE, "rsy_set_graphic_objects has not been called");
+    if (n_objects && !objects) return fail(c, RSY_INVALID_ARGUMENT, "null objects");
+    if (offset + n_objects > c->h_objects.size()) return fail(c, RSY_INVALID_ARGUMENT, "object update out of range");
+    HIP_TRY(c, hipSetDevice(c->device), "set device");
+    HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
+    std::copy(objects, objects + n_objects, c->h_objects.begin() + offset);
+    if (n_objects)
+        HIP_TRY(c, hipMemcpy(c->objects.as<uint8_t>() + offset * sizeof(rsy_object_record), objects,
+                             size_t(n_objects) * sizeof(rsy_object_record), hipMemcpyHostToDevice), "upload objects");
+    return RSY_OK;
+}
+
+rsy_result rsy_update(rsy_ctx* c, const rsy_scene_data* sd, const rsy_raster_state* state) {
+    if (!c || !sd) return RSY_INVALID_ARGUMENT;
+    if (state) {
+        if (state->tile_size != 0 && state->tile_size != 8 && state->tile_size != 16 && state->tile_size != 32)
+            return fail(c, RSY_INVALID_ARGUMENT, "tile_size must be 0, 8, 16 or 32");
+        if (state->pcf_mode > RSY_PCF_GRID_3X3) return fail(c, RSY_INVALID_ARGUMENT, "unknown pcf mode");
+        const bool raster_changed = state->cull_enabled != c->rs.cull_enabled ||
+                                    state->reverse_winding_order_enabled != c->rs.reverse_winding_order_enabled ||
+                                    state->depth_bias_enabled != c->rs.depth_bias_enabled ||
+                                    state->depth_bias_constant != c->rs.depth_bias_constant ||
+                                    state->depth_bias_clamp != c->rs.depth_bias_clamp ||
+                                    state->depth_bias_slope_factor != c->rs.depth_bias_slope_factor ||
+                                    state->tile_size != c->rs.tile_size || state->pcf_mode != c->rs.pcf_mode ||
+                                    std::memcmp(state->clear_color, c->rs.clear_color, sizeof(c->rs.clear_color)) != 0;
+        c->rs = *state;
+        if (raster_changed) c->layout.valid = false;
+    }
+    HIP_TRY(c, hipSetDevice(c->device), "set device");
+    HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
+    c->sd = *sd;
+    HIP_TRY(c, hipMemcpy(c->scene_d.p, sd, sizeof(rsy_scene_data), hipMemcpyHostToDevice), "upload scene data");
+    c->has_scene = true;
+    return RSY_OK;
+}
+
+rsy_result rsy_render(rsy_ctx* c, const rsy_frame_desc* frame) {
+    if (!c) return RSY_INVALID_ARGUMENT;
+    HIP_TRY(c, hipSetDevice(c->device), "set device");
+    rsy_result r = ready_to_render(c);
+    if (r != RSY_OK) return r;
+    if (c->frame_pending) {
+        r = finish_frame(c);
+        if (r != RSY_OK) return r;
+    }
+    std::vector<FrameViewDesc> views;
+    uint32_t shade = 0;
+    r = resolve_views(c, frame, views, shade);
+    if (r != RSY_OK) return r;
+    return render_now(c, views, shade);
+}
+
+rsy_result rsy_synchronize(rsy_ctx* c) {
+    if (!c) return RSY_INVALID_ARGUMENT;
+    HIP_TRY(c, hipSetDevice(c->device), "set device");
+    return finish_frame(c);
+}
+
+rsy_result rsy_render_graph(rsy_ctx* c, const rsy_frame_desc* frame, uint32_t n_replays) {
+    if (!c) return RSY_INVALID_ARGUMENT;
+    HIP_TRY(c, hipSetDevice(c->device), "set device");
+    // one eager frame sizes every buffer (and re-runs on overflow)
+    rsy_result r = rsy_render(c, frame);
+    if (r != RSY_OK) return r;
+    r = finish_frame(c);
+    if (r != RSY_OK) return r;
+    if (!c->graph_valid) {
+        invalidate_graph(c);
+        hipGraph_t graph = nullptr;
+        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        r = enqueue_frame(c, true);
+        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (r != RSY_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return r;
+        }
+        if (e != hipSuccess) return hip_fail(c, e, "end capture");
+        e = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return hip_fail(c, e, "instantiate graph");
+        c->graph_valid = true;
+    }
+    HIP_TRY(c, hipEventRecord(c->ev_begin, c->stream), "event");
+    for (uint32_t i = 0; i < n_replays; ++i) HIP_TRY(c, hipGraphLaunch(c->graph_exec, c->stream), "graph launch");
+    const uint32_t nv = uint32_t(c->layout.vg.size());
+    HIP_TRY(c, hipMemcpyAsync(c->readback_counters.p, c->counters.p, nv * kCtrPerView * 4, hipMemcpyDeviceToHost, c->stream), "readback");
+    HIP_TRY(c, hipMemcpyAsync(static_cast<uint8_t*>(c->readback_counters.p) + nv * kCtrPerView * 4, c->total_pairs.p, 4,
+                              hipMemcpyDeviceToHost, c->stream), "readback");
+    HIP_TRY(c, hipEventRecord(c->ev_end, c->stream), "event");
+    HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
+    const bool over = collect_stats(c);
+    if (n_replays) c->stats.gpu_ms /= float(n_replays);
+    if (over) return fail(c, RSY_OVERFLOW, "graph replay overflowed its buffers");
+    return RSY_OK;
+}
+
+static rsy_result camera_index(rsy_ctx* c, uint32_t view, uint32_t& idx) {
+    if (!c->layout.valid) return fail(c, RSY_INVALID_STATE, "no frame rendered");
+    if (view >= c->layout.n_camera) return fail(c, RSY_INVALID_ARGUMENT, "camera view index out of range");
+    idx = view;
+    return RSY_OK;
+}
+
+static const ViewGPU* camera_vg(rsy_ctx* c, uint32_t view) {
+    uint32_t k = 0;
+    for (const auto& g : c->layout.vg)
+        if (!g.is_shadow && k++ == view) return &g;
+    return nullptr;
+}
+
+rsy_result rsy_read_visibility(rsy_ctx* c, uint32_t view, uint64_t* out, uint64_t n_pixels) {
+    if (!c || !out) return RSY_INVALID_ARGUMENT;
+    rsy_result r = rsy_synchronize(c);
+    if (r != RSY_OK) return r;
+    uint32_t idx;
+    if ((r = camera_index(c, view, idx)) != RSY_OK) return r;
+    const ViewGPU* g = camera_vg(c, view);
+    if (n_pixels != uint64_t(g->width) * g->height) return fail(c, RSY_INVALID_ARGUMENT, "pixel count mismatch");
+    HIP_TRY(c, hipMemcpy(out, c->vis_out[idx].p, n_pixels * 8, hipMemcpyDeviceToHost), "read visibility");
+    return RSY_OK;
+}
+
+rsy_result rsy_read_color(rsy_ctx* c, uint32_t view, float* out, uint64_t n_pixels) {
+    if (!c || !out) return RSY_INVALID_ARGUMENT;
+    rsy_result r = rsy_synchronize(c);
+    if (r != RSY_OK) return r;
+    uint32_t idx;
+    if ((r = camera_index(c, view, idx)) != RSY_OK) return r;
+    if (!c->layout.shade) return fail(c, RSY_INVALID_STATE, "last frame was not shaded");
+    const ViewGPU* g = camera_vg(c, view);
+    if (n_pixels != uint64_t(g->width) * g->height) return fail(c, RSY_INVALID_ARGUMENT, "pixel count mismatch");
+    HIP_TRY(c, hipMemcpy(out, c->color_out[idx].p, n_pixels * 16, hipMemcpyDeviceToHost), "read colour");
+    return RSY_OK;
+}
+
+rsy_result rsy_read_shadow_factor(rsy_ctx* c, uint32_t view, float* out, uint64_t n_pixels) {
+    if (!c || !out) return RSY_INVALID_ARGUMENT;
+    rsy_result r = rsy_synchronize(c);
+    if (r != RSY_OK) return r;
+    uint32_t idx;
+    if ((r = camera_index(c, view, idx)) != RSY_OK) return r;
+    if (!(c->debug_flags & RSY_DEBUG_SHADOW_FACTOR)) return fail(c, RSY_INVALID_STATE, "shadow factor output disabled");
+    const ViewGPU* g = camera_vg(c, view);
+    if (n_pixels != uint64_t(g->width) * g->height) return fail(c, RSY_INVALID_ARGUMENT, "pixel count mismatch");
+    HIP_TRY(c, hipMemcpy(out, c->shadow_factor_out[idx].p, n_pixels * 4, hipMemcpyDeviceToHost), "read shadow factor");
+    return RSY_OK;
+}
+
+rsy_result rsy_read_shadow_map(rsy_ctx* c, uint32_t cascade, float* out, uint64_t n_texels) {
+    if (!c || !out) return RSY_INVALID_ARGUMENT;
+    rsy_result r = rsy_synchronize(c);
+    if (r != RSY_OK) return r;
+    if (!c->layout.valid || cascade >= c->layout.n_shadow) return fail(c, RSY_INVALID_ARGUMENT, "cascade index out of range");
+    const ViewGPU& g = c->layout.vg[cascade];  // shadow views come first
+    if (n_texels != uint64_t(g.width) * g.height) return fail(c, RSY_INVALID_ARGUMENT, "texel count mismatch");
+    HIP_TRY(c, hipMemcpy(out, c->shadow_maps[cascade].p, n_texels * 4, hipMemcpyDeviceToHost), "read shadow map");
+    return RSY_OK;
+}
+
+rsy_result rsy_get_frame_stats(rsy_ctx* c, rsy_frame_stats* out) {
+    if (!c || !out) return RSY_INVALID_ARGUMENT;
+    rsy_result r = rsy_synchronize(c);
+    if (r != RSY_OK) return r;
+    *out = c->stats;
+    return RSY_OK;
+}
+
+rsy_result rsy_get_device_outputs(rsy_ctx* c, uint32_t view, const void** vis, const void** color) {
+    if (!c) return RSY_INVALID_ARGUMENT;
+    uint32_t idx;
+    rsy_result r = camera_index(c, view, idx);
+    if (r != RSY_OK) return r;
+    if (vis) *vis = c->vis_out[idx].p;
+    if (color) *color = c->color_out[idx].p;
+    return RSY_OK;
+}
+
+rsy_result rsy_set_debug_flags(rsy_ctx* c, uint32_t flags) {
+    if (!c) return RSY_INVALID_ARGUMENT;
+    if (flags != c->debug_flags) invalidate_graph(c);
+    c->debug_flags = flags;
+    return RSY_OK;
+}
+
+rsy_result rsy_autotune(rsy_ctx* c, const rsy_frame_desc* frame, uint32_t n_iters, uint32_t* chosen_tile) {
+    if (!c) return RSY_INVALID_ARGUMENT;
+    const uint32_t saved = c->rs.tile_size;
+    float best_ms = 1e30f;
+    uint32_t best = 16;
+    for (uint32_t t : {8u, 16u, 32u}) {
+        c->rs.tile_size = t;
+        c->layout.valid = false;
+        rsy_result r = rsy_render_graph(c, frame, std::max<uint32_t>(1, n_iters));
+        if (r != RSY_OK) {
+            c->rs.tile_size = saved;
+            return r;
+        }
+        if (c->stats.gpu_ms < best_ms) {
+            best_ms = c->stats.gpu_ms;
+            best = t;
+        }
+    }
+    c->rs.tile_size = saved;
+    c->autotuned_tile = best;
+    c->layout.valid = false;
+    invalidate_graph(c);
+    if (chosen_tile) *chosen_tile = best;
+    return RSY_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+// Device-side records shared by the host orchestration (context.cpp) and the kernels.
+#pragma once
+#include <cstdint>
+
+namespace rsy {
+
+// One triangle after clip, snap, cull and setup: 48 bytes, three 16-byte loads.
+struct alignas(16) TriRec {
+    int32_t X0, Y0, X1, Y1;  // snapped vertices, 8 sub-pixel bits
+    int32_t X2, Y2;
+    float z0;                // depth plane at (X0, Y0) (depth bias folded in for shadow views)
+    float dzdx;              // per sub-pixel unit
+    float dzdy;
+    uint32_t id;             // triangle id in the view's draw list
+    uint32_t bbx;            // px0 | px1 << 16 (inclusive pixel bounds)
+    uint32_t bby;            // py0 | py1 << 16
+};
+static_assert(sizeof(TriRec) == 48, "TriRec is 48 bytes");
+
+// Per-draw data of one draw list (shadow or opaque), in list order.
+struct DrawGPU {
+    uint32_t first_index;  // mesh.index_offset + start_index
+    uint32_t tri_count;
+    uint32_t tri_base;     // prefix over the list
+    uint32_t vtx_base;     // first vertex of the mesh in the global vertex buffer
+    uint32_t vtx_count;
+    uint32_t xv_off;       // offset of this draw's transformed vertices inside a view's block
+    uint32_t object;       // graphic_objects_offset + graphics_object_index
+    uint32_t material;     // material index or 0xFFFFFFFF
+};
+
+enum : uint32_t { kCtrRecords = 0, kCtrClip = 1, kCtrPairs = 2, kCtrOverflow = 3, kCtrClipped = 4, kCtrPerView = 8 };
+
+struct ViewGPU {
+    uint32_t width, height;
+    uint32_t tile, tiles_x, tiles_y;
+    uint32_t tile_base;     // first bin of this view in the global bin arrays
+    uint32_t is_shadow;
+    uint32_t list;          // 0 = shadow list, 1 = opaque list
+    uint32_t n_draws;
+    uint32_t mvp_base;      // first MVP matrix of this view (one per draw)
+    uint32_t xv_base;       // first transformed vertex of this view
+    uint32_t rec_base, rec_cap;
+    uint32_t clip_base, clip_cap;
+    uint32_t cull, reverse, bias;
+    float bias_const, bias_clamp, bias_slope;
+    uint32_t pad;
+    uint64_t out;           // device address: u64 visibility (camera) or u32 depth bits (shadow)
+    float view_projection[16];
+};
+
+struct BlockEntry {  // work of one 256-thread block of the transform / setup kernels
+    uint32_t view, draw, start, pad;
+};
+
+struct TexGPU {
+    uint64_t offset;  // bytes into the texel pool
+    uint32_t width, height;
+    uint32_t channels;  // 2 or 4
+    uint32_t srgb;
+};
+
+}  // namespace rsy

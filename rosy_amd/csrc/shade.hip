@@ -1,0 +1,461 @@
+// Deferred shading kernel (FP32) over the visibility buffers.
+//
+// For each pixel the visible triangle's three vertices go through vertexMain (basic.slang:15-119),
+// the varyings are interpolated perspective-correctly at the pixel centre (clip-space barycentrics
+// from the transformed vertices of the raster pass), and fragmentMain (basic.slang:137-344) runs with
+// the GGX/Smith/Schlick terms of lighting.slang:5-43 and the bilinear-compare PCF of pcf.slang:1-12
+// (compare sampler Graphics.cpp:2697-2725).  Textures are sampled at mip 0 with sRGB decode before
+// filtering through a 256-entry table in LDS.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fixed_point.h"
+#include "frame_types.h"
+#include "kernels.h"
+#include "rosy_amd.h"
+
+namespace rsy {
+namespace {
+
+constexpr uint32_t kLdsDraws = 2048;
+
+struct f3 {
+    float x, y, z;
+};
+__device__ inline f3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ inline f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ inline f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ inline f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ inline float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline f3 cross(f3 a, f3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ inline f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+__device__ inline float clamp01(float x) { return fminf(1.0f, fmaxf(0.0f, x)); }
+
+__device__ inline float4 mul4(const float* m, float4 v) {
+    float4 r;
+    r.x = m[0] * v.x + m[4] * v.y + m[8] * v.z + m[12] * v.w;
+    r.y = m[1] * v.x + m[5] * v.y + m[9] * v.z + m[13] * v.w;
+    r.z = m[2] * v.x + m[6] * v.y + m[10] * v.z + m[14] * v.w;
+    r.w = m[3] * v.x + m[7] * v.y + m[11] * v.z + m[15] * v.w;
+    return r;
+}
+__device__ inline f3 mul3(const float* m, f3 v) {
+    return {m[0] * v.x + m[3] * v.y + m[6] * v.z, m[1] * v.x + m[4] * v.y + m[7] * v.z,
+            m[2] * v.x + m[5] * v.y + m[8] * v.z};
+}
+__device__ inline f3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+
+__device__ inline bool valid_index(uint32_t i) { return i < 4294967295u; }
+
+__device__ inline int wrap_coord(int i, int n, uint32_t mode) {
+    if (mode == 0) return min(max(i, 0), n - 1);
+    if (mode == 1) {
+        const int period = 2 * n;
+        int m = i % period;
+        m = m < 0 ? m + period : m;
+        return m < n ? m : period - 1 - m;
+    }
+    int m = i % n;
+    return m < 0 ? m + n : m;
+}
+
+struct TexRef {
+    const uint8_t* base;
+    int w, h, ch, srgb;
+};
+
+__device__ inline float4 texel(const TexRef& t, int x, int y, const float* lut) {
+    const size_t o = size_t(y) * t.w + x;
+    float4 c = make_float4(0.f, 0.f, 0.f, 1.f);
+    if (t.ch == 4) {
+        const uint32_t p = reinterpret_cast<const uint32_t*>(t.base)[o];
+        const uint32_t r = p & 0xffu, g = (p >> 8) & 0xffu, b = (p >> 16) & 0xffu, a = p >> 24;
+        c.x = t.srgb ? lut[r] : float(r) * (1.0f / 255.0f);
+        c.y = t.srgb ? lut[g] : float(g) * (1.0f / 255.0f);
+        c.z = t.srgb ? lut[b] : float(b) * (1.0f / 255.0f);
+        c.w = float(a) * (1.0f / 255.0f);
+    } else {
+        const uint32_t p = reinterpret_cast<const uint16_t*>(t.base)[o];
+        const uint32_t r = p & 0xffu, g = p >> 8;
+        c.x = t.srgb ? lut[r] : float(r) * (1.0f / 255.0f);
+        c.y = t.srgb ? lut[g] : float(g) * (1.0f / 255.0f);
+    }
+    return c;
+}
+
+__device__ float4 sample_tex(const TexRef& t, const rsy_sampler* s, float u, float v, const float* lut) {
+    const uint32_t ws = s ? s->wrap_s : 2u, wt = s ? s->wrap_t : 2u;
+    const bool nearest = s && s->mag_filter <= 2;
+    if (nearest) {
+        const int x = wrap_coord(int(floorf(u * t.w)), t.w, ws), y = wrap_coord(int(floorf(v * t.h)), t.h, wt);
+        return texel(t, x, y, lut);
+    }
+    const float x = u * float(t.w) - 0.5f, y = v * float(t.h) - 0.5f;
+    const float fx0 = floorf(x), fy0 = floorf(y);
+    const float ax = x - fx0, ay = y - fy0;
+    const int x0 = int(fx0), y0 = int(fy0);
+    const int xa = wrap_coord(x0, t.w, ws), xb = wrap_coord(x0 + 1, t.w, ws);
+    const int ya = wrap_coord(y0, t.h, wt), yb = wrap_coord(y0 + 1, t.h, wt);
+    const float4 c00 = texel(t, xa, ya, lut), c10 = texel(t, xb, ya, lut);
+    const float4 c01 = texel(t, xa, yb, lut), c11 = texel(t, xb, yb, lut);
+    const float w00 = (1 - ax) * (1 - ay), w10 = ax * (1 - ay), w01 = (1 - ax) * ay, w11 = ax * ay;
+    float4 r;
+    r.x = w00 * c00.x + w10 * c10.x + w01 * c01.x + w11 * c11.x;
+    r.y = w00 * c00.y + w10 * c10.y + w01 * c01.y + w11 * c11.y;
+    r.z = w00 * c00.z + w10 * c10.z + w01 * c01.z + w11 * c11.z;
+    r.w = w00 * c00.w + w10 * c10.w + w01 * c01.w + w11 * c11.w;
+    return r;
+}
+
+__device__ inline float cmp_tap(const float* map, int S, float u, float v, float ref) {
+    const float x = u * float(S) - 0.5f, y = v * float(S) - 0.5f;
+    const float fx0 = floorf(fminf(fmaxf(x, -2.0f), float(S) + 1.0f));
+    const float fy0 = floorf(fminf(fmaxf(y, -2.0f), float(S) + 1.0f));
+    const float ax = x - fx0, ay = y - fy0;
+    const int x0 = int(fx0), y0 = int(fy0);
+    const int xa = min(max(x0, 0), S - 1), xb = min(max(x0 + 1, 0), S - 1);
+    const int ya = min(max(y0, 0), S - 1), yb = min(max(y0 + 1, 0), S - 1);
+    const float t00 = map[size_t(ya) * S + xa], t10 = map[size_t(ya) * S + xb];
+    const float t01 = map[size_t(yb) * S + xa], t11 = map[size_t(yb) * S + xb];
+    return (1 - ax) * (1 - ay) * (ref >= t00 ? 1.f : 0.f) + ax * (1 - ay) * (ref >= t10 ? 1.f : 0.f) +
+           (1 - ax) * ay * (ref >= t01 ? 1.f : 0.f) + ax * ay * (ref >= t11 ? 1.f : 0.f);
+}
+
+struct VOut {
+    f3 normal, world_normal, view_dir, light_dir, tangent, bitangent;
+    float4 color;
+    float u, v, sigma;
+    float4 csm;
+};
+
+__device__ VOut vertex_main(const rsy_scene_data& sd, const rsy_position& vin, const float* gd,
+                            bool has_material, uint32_t normal_index, const float* cascade_m) {
+    VOut o;
+    float4 nt = make_float4(vin.tangents[0], vin.tangents[1], vin.tangents[2], vin.tangents[3]);
+    if (sd.ignore_asset_tangent_sign == 1) nt.w = 1.f;
+    if (sd.flip_tangents[0] == 1) nt.x = -nt.x;
+    if (sd.flip_tangents[1] == 1) nt.y = -nt.y;
+    if (sd.flip_tangents[2] == 1) nt.z = -nt.z;
+    if (sd.flip_tangents[3] == 1) nt.w = -nt.w;
+    bool tangent_space = sd.tangent_space_enabled == 1;
+    if (has_material && !valid_index(normal_index)) tangent_space = false;
+    const f3 vpos = mk(vin.vertex[0], vin.vertex[1], vin.vertex[2]);
+    const f3 vnorm = mk(vin.normal[0], vin.normal[1], vin.normal[2]);
+    f3 normal = vnorm, tangent = mk(nt.x, nt.y, nt.z);
+    f3 bitangent = cross(normal, tangent) * nt.w;
+    if (sd.ensure_orthogonal_bitangent == 1) {
+        const f3 np = bitangent * dot(normal, bitangent);
+        const f3 on = normalize(normal - np);
+        const f3 tp = np - bitangent * dot(tangent, bitangent);
+        const f3 ot = normalize(tangent - tp);
+        bitangent = cross(on, ot) * nt.w;
+    }
+    const float* W = gd;
+    const float* toObj = gd + 16;
+    const float* nmat = gd + 32;
+    const float4 world = mul4(W, make_float4(vpos.x, vpos.y, vpos.z, 1.f));
+    const float4 cam = make_float4(sd.camera_position[0], sd.camera_position[1], sd.camera_position[2], sd.camera_position[3]);
+    const float4 sun = make_float4(sd.sun_position[0], sd.sun_position[1], sd.sun_position[2], sd.sun_position[3]);
+    f3 view_dir = xyz(cam) - xyz(world);
+    f3 light_dir = xyz(sun) - xyz(world);
+    if (tangent_space) {
+        view_dir = xyz(mul4(toObj, cam)) - vpos;
+        light_dir = xyz(mul4(toObj, sun)) - vpos;
+    } else {
+        normal = normalize(mul3(nmat, vnorm));
+        tangent = normalize(mul3(nmat, mk(nt.x, nt.y, nt.z)));
+        tangent = normalize(tangent - normal * dot(normal, tangent));
+    }
+    if (sd.flip_lights[0] == 1) light_dir.x = -light_dir.x;
+    if (sd.flip_lights[1] == 1) light_dir.y = -light_dir.y;
+    if (sd.flip_lights[2] == 1) light_dir.z = -light_dir.z;
+    o.normal = normal;
+    o.world_normal = mul3(nmat, vnorm);
+    o.color = make_float4(vin.color[0], vin.color[1], vin.color[2], vin.color[3]);
+    o.tangent = mk(nt.x, nt.y, nt.z);
+    o.bitangent = bitangent;
+    o.view_dir = view_dir;
+    o.light_dir = light_dir;
+    o.sigma = nt.w;
+    o.u = vin.texture_coordinates[0];
+    o.v = vin.texture_coordinates[1];
+    o.csm = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (cascade_m) {
+        const float4 s = mul4(cascade_m, world);
+        o.csm = make_float4(0.5f * s.x + 0.5f * s.w, 0.5f * s.y + 0.5f * s.w, s.z, s.w);
+    }
+    return o;
+}
+
+// True when any sampler of the material clamps (the uv period shift is then not an identity).
+__device__ inline bool uses_clamp(const rsy_gpu_material* md, const rsy_sampler* samplers, uint32_t n_samplers) {
+    const uint32_t idx[4] = {md->color_sampler_index, md->normal_sampler_index, md->metallic_sampler_index,
+                             md->mixmap_sampler_index};
+    bool clamp = false;
+    for (int k = 0; k < 4; ++k)
+        if (idx[k] < n_samplers) clamp |= samplers[idx[k]].wrap_s == 0 || samplers[idx[k]].wrap_t == 0;
+    return clamp;
+}
+
+__device__ inline f3 lerp3(f3 a, f3 b, f3 c, float w0, float w1, float w2) { return a * w0 + b * w1 + c * w2; }
+__device__ inline float4 lerp4(float4 a, float4 b, float4 c, float w0, float w1, float w2) {
+    return make_float4(a.x * w0 + b.x * w1 + c.x * w2, a.y * w0 + b.y * w1 + c.y * w2, a.z * w0 + b.z * w1 + c.z * w2,
+                       a.w * w0 + b.w * w1 + c.w * w2);
+}
+
+__device__ inline void mr_from(int channels, float4 s, float& rough, float& metal) {
+    if (channels == 2) {
+        metal = s.x;
+        rough = s.y;
+    } else {
+        rough = s.y;
+        metal = s.z;
+    }
+}
+
+__device__ inline float distribution_ggx(f3 N, f3 H, float r) {
+    const float a = r * r, a2 = a * a;
+    const float ndh = fmaxf(dot(N, H), 0.f), ndh2 = ndh * ndh;
+    float den = ndh2 * (a2 - 1.f) + 1.f;
+    den = 3.14159265359f * den * den;
+    return a2 / den;
+}
+__device__ inline float schlick_ggx(float ndv, float r) {
+    const float rr = r + 1.f, k = rr * rr / 8.f;
+    return ndv / (ndv * (1.f - k) + k);
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
+    __shared__ float lut[256];
+    __shared__ uint32_t tri_base[kLdsDraws];
+    lut[threadIdx.x] = L.srgb_lut[threadIdx.x];
+    const uint32_t nd_lds = L.n_draws < kLdsDraws ? L.n_draws : kLdsDraws;
+    for (uint32_t i = threadIdx.x; i < nd_lds; i += 256) tri_base[i] = L.draws[i].tri_base;
+    __syncthreads();
+
+    uint32_t vi = 0;
+    while (vi + 1 < L.n_views && L.views[vi + 1].block_base <= blockIdx.x) ++vi;
+    const ShadeView& V = L.views[vi];
+    const uint32_t lb = blockIdx.x - V.block_base;
+    const uint32_t x = (lb % V.blocks_x) * 16 + (threadIdx.x & 15);
+    const uint32_t y = (lb / V.blocks_x) * 16 + (threadIdx.x >> 4);
+    if (x >= V.width || y >= V.height) return;
+    const size_t o = size_t(y) * V.width + x;
+    const unsigned long long vis = V.vis[o];
+    if (vis == 0ull) {
+        V.color[o] = make_float4(L.clear_color[0], L.clear_color[1], L.clear_color[2], L.clear_color[3]);
+        if (V.shadow_factor) V.shadow_factor[o] = 1.f;
+        return;
+    }
+    const rsy_scene_data& sd = *static_cast<const rsy_scene_data*>(L.scene);
+    const uint32_t tri = uint32_t(vis & 0xffffffffull) - 1u;
+    // draw lookup: last draw whose tri_base <= tri
+    uint32_t lo = 0, hi = L.n_draws;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t b = mid < kLdsDraws ? tri_base[mid] : L.draws[mid].tri_base;
+        if (b <= tri) lo = mid; else hi = mid;
+    }
+    const DrawGPU dr = L.draws[lo];
+    const uint32_t local = tri - dr.tri_base;
+    const float* gd = L.objects + size_t(dr.object) * 41u;
+    const bool has_material = dr.material < L.n_materials;
+    const rsy_gpu_material* mats = static_cast<const rsy_gpu_material*>(L.materials);
+    const rsy_gpu_material* md = has_material ? &mats[dr.material] : nullptr;
+    const rsy_position* verts = static_cast<const rsy_position*>(L.vertices);
+    const rsy_sampler* samplers = static_cast<const rsy_sampler*>(L.samplers);
+    const uint32_t* ip = L.indices + dr.first_index + 3u * local;
+    const uint32_t i0 = ip[0], i1 = ip[1], i2 = ip[2];
+    const float4* xv = L.xv + L.xv_bases[vi] + dr.xv_off;
+    const float4 c0 = xv[i0], c1 = xv[i1], c2 = xv[i2];
+
+    // Perspective-correct barycentrics at the pixel centre: from the exact integer edge functions
+    // of the snapped triangle (the one the raster pass covered) divided by each vertex's w, or, for
+    // a triangle that needed clipping, from b ~ cross(x - u w, y - v w) over the clip positions.
+    f3 lam;
+    if (all_inside(c0, c1, c2)) {
+        const float hw = 0.5f * float(V.width), hh = 0.5f * float(V.height);
+        int32_t X0, Y0, X1, Y1, X2, Y2;
+        float zs;
+        snap(c0, hw, hh, X0, Y0, zs);
+        snap(c1, hw, hh, X1, Y1, zs);
+        snap(c2, hw, hh, X2, Y2, zs);
+        const int32_t Px = int32_t(x) * 256 + 128, Py = int32_t(y) * 256 + 128;
+        const int64_t e0 = int64_t(X1 - Px) * (Y2 - Py) - int64_t(X2 - Px) * (Y1 - Py);
+        const int64_t e1 = int64_t(X2 - Px) * (Y0 - Py) - int64_t(X0 - Px) * (Y2 - Py);
+        const int64_t e2 = int64_t(X0 - Px) * (Y1 - Py) - int64_t(X1 - Px) * (Y0 - Py);
+        lam = mk(float(e0) / c0.w, float(e1) / c1.w, float(e2) / c2.w);
+    } else {
+        const float u = (float(x) + 0.5f) * (2.0f / float(V.width)) - 1.0f;
+        const float v = (float(y) + 0.5f) * (2.0f / float(V.height)) - 1.0f;
+        const f3 a = mk(c0.x - u * c0.w, c1.x - u * c1.w, c2.x - u * c2.w);
+        const f3 b = mk(c0.y - v * c0.w, c1.y - v * c1.w, c2.y - v * c2.w);
+        lam = cross(a, b);
+    }
+    lam = lam * (1.0f / (lam.x + lam.y + lam.z));
+    const float view_depth = c0.w * lam.x + c1.w * lam.y + c2.w * lam.z;
+    int cascade = -1;
+    for (uint32_t k = 0; k < L.n_cascades; ++k)
+        if (view_depth <= L.cascades[k].split_far) {
+            cascade = int(k);
+            break;
+        }
+    const float* cm = cascade >= 0 ? L.cascades[cascade].m : nullptr;
+    const uint32_t nti = has_material ? md->normal_sampled_image_index : RSY_ABSENT;
+    const VOut v0 = vertex_main(sd, verts[dr.vtx_base + i0], gd, has_material, nti, cm);
+    const VOut v1 = vertex_main(sd, verts[dr.vtx_base + i1], gd, has_material, nti, cm);
+    const VOut v2 = vertex_main(sd, verts[dr.vtx_base + i2], gd, has_material, nti, cm);
+    const float w0 = lam.x, w1 = lam.y, w2 = lam.z;
+    const f3 bn = lerp3(v0.normal, v1.normal, v2.normal, w0, w1, w2);
+    const f3 bwn = lerp3(v0.world_normal, v1.world_normal, v2.world_normal, w0, w1, w2);
+    const f3 bview = lerp3(v0.view_dir, v1.view_dir, v2.view_dir, w0, w1, w2);
+    const f3 blight = lerp3(v0.light_dir, v1.light_dir, v2.light_dir, w0, w1, w2);
+    const f3 btan = lerp3(v0.tangent, v1.tangent, v2.tangent, w0, w1, w2);
+    const f3 bbit = lerp3(v0.bitangent, v1.bitangent, v2.bitangent, w0, w1, w2);
+    const float4 bcol = lerp4(v0.color, v1.color, v2.color, w0, w1, w2);
+    // Texture coordinates relative to vertex 0 with a whole even number of periods removed: exact
+    // for REPEAT and MIRRORED_REPEAT and keeps |uv| small, so the FP32 texel address keeps its
+    // sub-texel precision at large uv.  Clamp-to-edge samplers keep the raw value.
+    float su = 0.f, sv = 0.f;
+    if (has_material && !uses_clamp(md, samplers, L.n_samplers)) {
+        su = 2.0f * floorf(v0.u * 0.5f);
+        sv = 2.0f * floorf(v0.v * 0.5f);
+    }
+    const float tu = (v0.u - su) + w1 * (v1.u - v0.u) + w2 * (v2.u - v0.u);
+    const float tv = (v0.v - sv) + w1 * (v1.v - v0.v) + w2 * (v2.v - v0.v);
+    const float sigma = v0.sigma * w0 + v1.sigma * w1 + v2.sigma * w2;
+    const float4 csm = lerp4(v0.csm, v1.csm, v2.csm, w0, w1, w2);
+
+    const f3 ambient = mk(sd.ambient_color[0], sd.ambient_color[1], sd.ambient_color[2]);
+    float4 out = bcol;
+    float4 image = out;
+    float alpha = bcol.w;
+    float srough = 0.f, smetal = 0.f;
+    f3 light_normal = normalize(bn);
+    const f3 light_dir = normalize(blight);
+    if (has_material) {
+        out = make_float4(md->color[0], md->color[1], md->color[2], md->color[3]);
+        image = out;
+        alpha = fminf(md->color[3], alpha);
+        if (sd.normal_maps_enabled == 1 && valid_index(nti) && nti < L.n_textures) {
+            const TexGPU tg = L.textures[nti];
+            const TexRef t{L.texels + tg.offset, int(tg.width), int(tg.height), int(tg.channels), int(tg.srgb)};
+            const rsy_sampler* smp = md->normal_sampler_index < L.n_samplers ? &samplers[md->normal_sampler_index] : nullptr;
+            const float4 s = sample_tex(t, smp, tu, tv, lut);
+            f3 m = mk(s.x * 2.f - 1.f, s.y * 2.f - 1.f, s.z * 2.f - 1.f);
+            if (t.ch == 2) m.z = sqrtf(fmaxf(0.f, 1.f - m.x * m.x - m.y * m.y));
+            const f3 n = normalize(bn);
+            const f3 tt = normalize(btan - n * dot(btan, n));
+            const f3 bb = normalize(cross(bn, btan) * sigma);
+            light_normal = normalize(tt * m.x + bb * m.y + n * m.z);
+        }
+        const uint32_t cti = md->color_sampled_image_index;
+        if (valid_index(cti) && cti < L.n_textures) {
+            const TexGPU tg = L.textures[cti];
+            const TexRef t{L.texels + tg.offset, int(tg.width), int(tg.height), int(tg.channels), int(tg.srgb)};
+            const rsy_sampler* smp = md->color_sampler_index < L.n_samplers ? &samplers[md->color_sampler_index] : nullptr;
+            image = sample_tex(t, smp, tu, tv, lut);
+            alpha = fminf(image.w, alpha);
+        }
+        const uint32_t mixti = md->mixmap_sampled_image_index, metti = md->metallic_sampled_image_index;
+        const bool use_mix = valid_index(mixti) && mixti < L.n_textures;
+        const uint32_t mti = use_mix ? mixti : metti;
+        const uint32_t msi = use_mix ? md->mixmap_sampler_index : md->metallic_sampler_index;
+        if (valid_index(mti) && mti < L.n_textures) {
+            const TexGPU tg = L.textures[mti];
+            const TexRef t{L.texels + tg.offset, int(tg.width), int(tg.height), int(tg.channels), int(tg.srgb)};
+            const rsy_sampler* smp = msi < L.n_samplers ? &samplers[msi] : nullptr;
+            mr_from(t.ch, sample_tex(t, smp, tu, tv, lut), srough, smetal);
+        }
+    }
+    const f3 Vd = normalize(bview);
+    const f3 Ld = light_dir;
+    const f3 N = light_normal;
+    const f3 H = normalize(Ld + Vd);
+    const float cos_theta = clamp01(dot(Ld, N)), cos_phi = clamp01(dot(H, N));
+    const float l_dot_n = cos_theta, l_dot_h = clamp01(dot(Ld, H)), n_dot_v = clamp01(dot(N, Vd));
+    const f3 half = mk(0.5f, 0.5f, 0.5f);
+    f3 c3;
+    bool set = true;
+    switch (sd.fragment_output) {
+        case 0: c3 = mk(image.x, image.y, image.z); break;
+        case 1: c3 = normalize(bwn) * 0.5f + half; break;
+        case 2: c3 = normalize(btan) * 0.5f + half; break;
+        case 3: c3 = normalize(blight) * 0.5f + half; break;
+        case 4: c3 = normalize(bview) * 0.5f + half; break;
+        case 5: c3 = mk(bcol.x, bcol.y, bcol.z); break;
+        case 6: c3 = light_normal * 0.5f + half; break;
+        case 7: c3 = bbit * 0.5f + half; break;
+        case 8: c3 = mk(cos_theta, cos_theta, cos_theta) * 0.5f + half; break;
+        case 9: c3 = mk(cos_phi, cos_phi, cos_phi) * 0.5f + half; break;
+        case 10: c3 = mk(l_dot_h, l_dot_h, l_dot_h) * 0.5f + half; break;
+        case 11: c3 = mk(n_dot_v, n_dot_v, n_dot_v) * 0.5f + half; break;
+        case 12: c3 = mk(tu + su, tv + sv, 0.f); break;
+        case 13: c3 = mk(srough, 0.05f, 0.05f); break;
+        case 14: c3 = mk(smetal, 0.05f, 0.05f); break;
+        case 15: c3 = mk(1.f - alpha, 0.05f, 0.05f); break;
+        default: set = false; c3 = mk(0.f, 0.f, 0.f); break;
+    }
+    if (set) out = make_float4(c3.x, c3.y, c3.z, alpha);
+    float shadow = 1.f;
+    if (sd.light_enabled > 0) {
+        if (sd.shadows_enabled == 1 && cascade >= 0) {
+            const ShadeCascade& C = L.cascades[cascade];
+            const int S = int(C.size);
+            if (L.pcf_mode == RSY_PCF_GRID_3X3) {
+                float r = 0.f;
+                const float inv = 1.0f / float(S);
+                for (int j = -1; j <= 1; ++j)
+                    for (int i = -1; i <= 1; ++i) r += cmp_tap(C.map, S, csm.x + float(i) * inv, csm.y + float(j) * inv, csm.z);
+                shadow = r * (1.0f / 9.0f);
+            } else {
+                const float so = 0.00025f;
+                float r = 0.f;
+                r += cmp_tap(C.map, S, csm.x - so, csm.y - so, csm.z);
+                r += cmp_tap(C.map, S, csm.x + so * 2.f, csm.y, csm.z);
+                r += cmp_tap(C.map, S, csm.x, csm.y + so * 2.f, csm.z);
+                r += cmp_tap(C.map, S, csm.x + so * 2.f, csm.y + so * 2.f, csm.z);
+                r += cmp_tap(C.map, S, csm.x - so * 2.f, csm.y - so * 2.f, csm.z);
+                shadow = r * (1.0f / 5.0f);
+            }
+        }
+        const f3 sun_col = mk(sd.sunlight_color[0], sd.sunlight_color[1], sd.sunlight_color[2]);
+        if (sd.brdf_lighting_enabled == 0) {
+            const f3 lit = sun_col * fmaxf(cos_theta, 0.f) * shadow;
+            out = make_float4(out.x * fmaxf(ambient.x, lit.x), out.y * fmaxf(ambient.y, lit.y),
+                              out.z * fmaxf(ambient.z, lit.z), out.w * alpha);
+        } else {
+            const float roughness = srough, metallic = smetal;
+            const f3 albedo = mk(out.x, out.y, out.z);
+            const f3 f0 = mk(0.04f, 0.04f, 0.04f) + (albedo - mk(0.04f, 0.04f, 0.04f)) * metallic;
+            const float ndf = distribution_ggx(N, H, roughness);
+            const float g = schlick_ggx(fmaxf(dot(N, Ld), 0.f), roughness) * schlick_ggx(fmaxf(dot(N, Vd), 0.f), roughness);
+            const float q = clamp01(1.f - fmaxf(dot(H, Vd), 0.f));
+            const float q5 = q * q * q * q * q;
+            const f3 F = f0 + (mk(1.f, 1.f, 1.f) - f0) * q5;
+            const f3 kD = (mk(1.f, 1.f, 1.f) - F) * (1.f - metallic);
+            const f3 spec = F * (ndf * g / (4.f * n_dot_v * l_dot_n + 0.0001f));
+            const f3 diffuse = albedo * (1.0f / 3.14159265f);
+            const f3 sun_light = sun_col * sd.light_intensity;
+            const f3 brdf = (kD * diffuse + spec) * sun_light * l_dot_n;
+            const f3 amb = ambient * diffuse;
+            const f3 sh = brdf * shadow;
+            out = make_float4(fmaxf(amb.x, sh.x), fmaxf(amb.y, sh.y), fmaxf(amb.z, sh.z), alpha);
+        }
+    }
+    if (has_material) out.w = alpha;
+    V.color[o] = out;
+    if (V.shadow_factor) V.shadow_factor[o] = shadow;
+}
+
+hipError_t launch_shade(const ShadeLaunch& l, hipStream_t s) {
+    if (l.total_blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_shade, dim3(l.total_blocks), dim3(256), 0, s, l);
+    return hipGetLastError();
+}
+
+}  // namespace rsy
