@@ -233,7 +233,9 @@ typedef struct rsy_frame_stats {
     uint32_t tile_size;
     uint32_t overflowed;        /* a bin or record buffer was too small; the frame was re-run */
     float gpu_ms;               /* device time of the last frame (HIP events) */
-    float stage_ms[8];          /* geometry, bin, raster, shade, ... (HIP events, if timing enabled) */
+    float stage_ms[8];          /* with RSY_DEBUG_STAGE_TIMING (HIP events on the context stream):
+                                   [0] geometry (mvp, transform, setup, clip), [1] binning (scan,
+                                   scatter), [2] tile raster kernel, [3] shading kernel */
 } rsy_frame_stats;
 
 typedef struct rsy_ctx rsy_ctx;

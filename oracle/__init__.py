@@ -57,6 +57,12 @@ def lib():
                                      C.c_void_p, C.c_void_p, C.c_int]
         L.orc_srgb_to_linear.restype = C.c_double
         L.orc_srgb_to_linear.argtypes = [C.c_uint32]
+        for name in ("orc_ggx_d", "orc_schlick_ggx", "orc_schlick_fresnel"):
+            getattr(L, name).restype = C.c_double
+            getattr(L, name).argtypes = [C.c_double, C.c_double]
+        L.orc_pcf.restype = C.c_double
+        L.orc_pcf.argtypes = [C.POINTER(C.c_float), C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint32,
+                              C.POINTER(C.c_double)]
         _lib = L
     return _lib
 

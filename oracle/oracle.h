@@ -93,6 +93,14 @@ int orc_shade_view(const orc_geometry* g, const orc_shade_in* in, const uint64_t
 /* Exact sRGB EOTF (basic.slang:121-128 form) of an 8-bit code. */
 double orc_srgb_to_linear(uint32_t code);
 
+/* Terms of the oracle's BRDF and shadow filter, exported so tests can pin them on their own. */
+double orc_ggx_d(double n_dot_h, double roughness);          /* lighting.slang:10-22 */
+double orc_schlick_ggx(double n_dot_v, double roughness);    /* lighting.slang:24-33 */
+double orc_schlick_fresnel(double v_dot_h, double f0);       /* lighting.slang:5-8 (one channel) */
+/* PCF shadow factor at (u, v) with reference depth ref over a size x size map (pcf.slang:1-12 for
+ * RSY_PCF_REFERENCE_5TAP, 3x3 one-texel grid for RSY_PCF_GRID_3X3). */
+double orc_pcf(const float* map, uint32_t size, double u, double v, double ref, uint32_t mode, double* margin);
+
 #ifdef __cplusplus
 }
 #endif

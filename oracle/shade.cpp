@@ -125,6 +125,24 @@ double cmp_tap(const float* map, int S, double u, double v, double ref, double& 
     return r;
 }
 
+// pcf.slang:1-12 (5 taps, offset 0.00025 UV) or a 3x3 grid of one-texel offsets.
+double pcf(const float* map, int S, double x, double y, double ref, uint32_t mode, double& margin) {
+    if (mode == RSY_PCF_GRID_3X3) {
+        double r = 0.0;
+        for (int j = -1; j <= 1; ++j)
+            for (int i = -1; i <= 1; ++i) r += cmp_tap(map, S, x + double(i) / S, y + double(j) / S, ref, margin);
+        return r * (1.0 / 9.0);
+    }
+    const double o = 0.00025;
+    double r = 0.0;
+    r += cmp_tap(map, S, x - o, y - o, ref, margin);
+    r += cmp_tap(map, S, x + o * 2.0, y, ref, margin);
+    r += cmp_tap(map, S, x, y + o * 2.0, ref, margin);
+    r += cmp_tap(map, S, x + o * 2.0, y + o * 2.0, ref, margin);
+    r += cmp_tap(map, S, x - o * 2.0, y - o * 2.0, ref, margin);
+    return r * (1.0 / 5.0);
+}
+
 struct VOut {  // BasicVertexStageOutput (data.slang:3-22) + CSM
     D3 vertex, normal, world_normal, view_dir, light_dir, tangent, bitangent;
     D4 color;
@@ -399,25 +417,9 @@ void shade_pixel(const Ctx& c, uint32_t px, uint32_t py, uint64_t vis, double ou
     if (sd.light_enabled > 0) {
         double shadow = 1.0;
         if (sd.shadows_enabled == 1 && cascade >= 0) {
-            const int S = int(in.cascades[cascade].size);
-            const float* map = in.shadow_maps[cascade];
             double margin = std::numeric_limits<double>::infinity();
-            const double x = bv.csm.x, y = bv.csm.y, ref = bv.csm.z;
-            if (in.raster->pcf_mode == RSY_PCF_GRID_3X3) {
-                double r = 0.0;
-                for (int j = -1; j <= 1; ++j)
-                    for (int i = -1; i <= 1; ++i) r += cmp_tap(map, S, x + double(i) / S, y + double(j) / S, ref, margin);
-                shadow = r * (1.0 / 9.0);
-            } else {
-                const double o = 0.00025;
-                double r = 0.0;
-                r += cmp_tap(map, S, x - o, y - o, ref, margin);
-                r += cmp_tap(map, S, x + o * 2.0, y, ref, margin);
-                r += cmp_tap(map, S, x, y + o * 2.0, ref, margin);
-                r += cmp_tap(map, S, x + o * 2.0, y + o * 2.0, ref, margin);
-                r += cmp_tap(map, S, x - o * 2.0, y - o * 2.0, ref, margin);
-                shadow = r * (1.0 / 5.0);
-            }
+            shadow = pcf(in.shadow_maps[cascade], int(in.cascades[cascade].size), bv.csm.x, bv.csm.y, bv.csm.z,
+                         in.raster->pcf_mode, margin);
             if (margin_out) *margin_out = margin;
         }
         if (shadow_out) *shadow_out = shadow;
@@ -458,6 +460,18 @@ void shade_pixel(const Ctx& c, uint32_t px, uint32_t py, uint64_t vis, double ou
 extern "C" {
 
 double orc_srgb_to_linear(uint32_t code) { return srgb_eotf(double(code & 0xffu) / 255.0); }
+
+double orc_ggx_d(double n_dot_h, double roughness) {
+    return distribution_ggx(D3{0, 0, 1}, D3{std::sqrt(std::max(0.0, 1 - n_dot_h * n_dot_h)), 0, n_dot_h}, roughness);
+}
+double orc_schlick_ggx(double n_dot_v, double roughness) { return schlick_ggx(n_dot_v, roughness); }
+double orc_schlick_fresnel(double v_dot_h, double f0) { return schlick_fresnel(v_dot_h, D3{f0, f0, f0}).x; }
+double orc_pcf(const float* map, uint32_t size, double u, double v, double ref, uint32_t mode, double* margin) {
+    double m = std::numeric_limits<double>::infinity();
+    const double r = pcf(map, int(size), u, v, ref, mode, m);
+    if (margin) *margin = m;
+    return r;
+}
 
 int orc_shade_view(const orc_geometry* g, const orc_shade_in* in, const uint64_t* vis, double* out_rgba,
                    double* out_shadow, double* out_margin, const float* override_shadow, int n_threads) {

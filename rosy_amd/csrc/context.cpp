@@ -30,6 +30,23 @@ constexpr uint32_t kMaxViews = 16;
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), cap(o.cap) {
+        o.p = nullptr;
+        o.cap = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            cap = o.cap;
+            o.p = nullptr;
+            o.cap = 0;
+        }
+        return *this;
+    }
     ~DevBuf() { release(); }
     void release() {
         if (p) (void)hipFree(p);
@@ -58,6 +75,9 @@ struct DevBuf {
 struct PinnedBuf {
     void* p = nullptr;
     size_t cap = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
     ~PinnedBuf() {
         if (p) (void)hipHostFree(p);
     }
@@ -576,9 +596,12 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
         sh.srgb_lut = c->srgb_lut.as<float>();
         sh.xv = g.xv;
         sh.xv_bases = c->xv_bases_d.as<uint32_t>();
+        if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[3], s), "event");
         HIP_TRY(c, launch_shade(sh, s), "shade kernel");
+    } else if (timing) {
+        HIP_TRY(c, hipEventRecord(c->ev_stage[3], s), "event");
     }
-    if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[3], s), "event");
+    if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[4], s), "event");
     if (!capturing) {
         HIP_TRY(c, hipMemcpyAsync(c->readback_counters.p, c->counters.p, nv * kCtrPerView * 4, hipMemcpyDeviceToHost, s), "readback");
         HIP_TRY(c, hipMemcpyAsync(static_cast<uint8_t*>(c->readback_counters.p) + nv * kCtrPerView * 4, c->total_pairs.p, 4,
@@ -626,7 +649,7 @@ bool collect_stats(rsy_ctx* c) {
         if (hipEventElapsedTime(&t, c->ev_begin, c->ev_stage[0]) == hipSuccess) st.stage_ms[0] = t;
         if (hipEventElapsedTime(&t, c->ev_stage[0], c->ev_stage[1]) == hipSuccess) st.stage_ms[1] = t;
         if (hipEventElapsedTime(&t, c->ev_stage[1], c->ev_stage[2]) == hipSuccess) st.stage_ms[2] = t;
-        if (hipEventElapsedTime(&t, c->ev_stage[2], c->ev_stage[3]) == hipSuccess) st.stage_ms[3] = t;
+        if (hipEventElapsedTime(&t, c->ev_stage[3], c->ev_stage[4]) == hipSuccess) st.stage_ms[3] = t;
     }
     c->stats = st;
     return overflow;

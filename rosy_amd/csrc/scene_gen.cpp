@@ -587,6 +587,11 @@ rsy_result rsy_scene_generate(const rsy_scene_desc* desc, rsy_scene** out_scene)
         } else {
             std::vector<uint32_t> budget = triangle_budget(wl.n_meshes, wl.min_tris, wl.max_tris, wl.total_tris, geo);
             const uint32_t ground = uint32_t(std::max_element(budget.begin(), budget.end()) - budget.begin());
+            {  // rounding residue of the power-law budget goes to the ground plane
+                int64_t sum = 0;
+                for (uint32_t b : budget) sum += b;
+                budget[ground] = uint32_t(int64_t(budget[ground]) + int64_t(wl.total_tris) - sum);
+            }
             std::vector<int> kind(wl.n_meshes);  // 0 box, 1 column, 2 plane
             for (uint32_t i = 0; i < wl.n_meshes; ++i) {
                 const double r = geo.uniform();
@@ -599,12 +604,13 @@ rsy_result rsy_scene_generate(const rsy_scene_desc* desc, rsy_scene** out_scene)
             for (uint32_t i = 0; i < wl.n_meshes; ++i) {
                 const uint32_t t = budget[i];
                 if (kind[i] == 0) {
-                    const int n = std::max(1, int(std::floor(std::sqrt(t / 12.0))));
+                    int n = std::max(1, int(std::lround(std::sqrt(t / 12.0))));
+                    if (12 * n * n < int(wl.min_tris)) ++n;
                     build_box(built[i], geo.range(0.4, 2.5), geo.range(0.4, 3.0), geo.range(0.4, 2.5), n, 4.0);
                     deficit += int64_t(t) - built[i].tris();
                 } else if (kind[i] == 1) {
                     const int s = std::max(6, int(std::sqrt(t / 2.0)));
-                    const int k = std::max(1, int(t / 2) / s - 1);
+                    const int k = std::max(1, (int(t / 2) + s - 1) / s - 1);
                     build_column(built[i], geo.range(0.25, 1.0), geo.range(1.5, 9.0), s, k, 4.0);
                     deficit += int64_t(t) - built[i].tris();
                 }
@@ -615,7 +621,7 @@ rsy_result rsy_scene_generate(const rsy_scene_desc* desc, rsy_scene** out_scene)
             for (size_t pi = 0; pi < planes.size(); ++pi) {
                 const uint32_t i = planes[pi];
                 int64_t t = budget[i];
-                if (pi == 0) t += deficit;  // the ground plane (largest budget) absorbs the residue
+                if (i == ground) t += deficit;  // the ground plane (largest budget) absorbs the residue
                 t = std::max<int64_t>(2, t);
                 const int64_t quads = t / 2;
                 double su, sv;
