@@ -176,7 +176,7 @@ typedef struct rsy_raster_state {
     float depth_bias_clamp;
     float depth_bias_slope_factor;
     uint32_t pcf_mode;  /* rsy_pcf_mode */
-    uint32_t tile_size; /* raster bin size in pixels: 8, 16 or 32; 0 = autotuned */
+    uint32_t tile_size; /* raster bin size in pixels: 32 or 64; 0 = autotuned */
     float clear_color[4];
 } rsy_raster_state;
 
@@ -313,7 +313,7 @@ rsy_result rsy_get_device_outputs(rsy_ctx* ctx, uint32_t view, const void** vis,
 #define RSY_DEBUG_STAGE_TIMING 2u  /* record HIP events between stages */
 rsy_result rsy_set_debug_flags(rsy_ctx* ctx, uint32_t flags);
 
-/* Autotuner: renders the given frame once per candidate bin size (8, 16, 32) and keeps the
+/* Autotuner: replays the given frame n_iters times per candidate bin size (32, 64) and keeps the
  * fastest for later frames with tile_size == 0.  Writes the chosen size. */
 rsy_result rsy_autotune(rsy_ctx* ctx, const rsy_frame_desc* frame, uint32_t n_iters, uint32_t* chosen_tile);
 

@@ -115,7 +115,7 @@ struct Layout {
     // derived
     std::vector<ViewGPU> vg;
     std::vector<BlockEntry> vblocks, tblocks;
-    uint32_t total_bins = 0, total_mvps = 0, max_draws = 0;
+    uint32_t total_bins = 0;
     uint64_t total_xv = 0;
     uint32_t rec_total = 0, clip_total = 0;
     uint32_t n_shadow = 0, n_camera = 0;
@@ -175,7 +175,7 @@ struct rsy_ctx {
 
     // frame buffers
     Layout layout;
-    DevBuf views_d, mvps, xv, recs, clipq, counters, bin_counts, bin_offsets, cursor, block_sums, total_pairs,
+    DevBuf views_d, xv, recs, clipq, counters, bin_counts, bin_offsets, cursor, block_valid, totals,
         lists_mem, vblocks_d, tblocks_d, shade_views_d, shade_cascades_d, xv_bases_d;
     uint32_t list_cap = 0;
     uint32_t rec_scale = 1;  // grows on overflow
@@ -183,6 +183,8 @@ struct rsy_ctx {
     PinnedBuf staging, readback_counters;
     std::vector<ShadeView> h_shade_views;
     std::vector<ShadeCascade> h_shade_cascades;
+    std::vector<uint32_t> h_xv_bases;
+    uint32_t shade_blocks = 0;
 
     // last frame (for re-running after an overflow)
     std::vector<FrameViewDesc> last_views;
@@ -325,8 +327,8 @@ rsy_result resolve_views(rsy_ctx* c, const rsy_frame_desc* f, std::vector<FrameV
 }
 
 uint32_t choose_tile(const rsy_ctx* c) {
-    if (c->rs.tile_size == 8 || c->rs.tile_size == 16 || c->rs.tile_size == 32) return c->rs.tile_size;
-    return c->autotuned_tile ? c->autotuned_tile : 16u;
+    if (c->rs.tile_size == 32 || c->rs.tile_size == 64) return c->rs.tile_size;
+    return c->autotuned_tile ? c->autotuned_tile : 32u;
 }
 
 // Computes bin, vertex, record and clip-queue layout of a frame and (re)allocates its buffers.
@@ -347,7 +349,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     L.vg.clear();
     L.vblocks.clear();
     L.tblocks.clear();
-    L.total_bins = L.total_mvps = L.max_draws = 0;
+    L.total_bins = 0;
     L.total_xv = 0;
     L.rec_total = L.clip_total = 0;
     L.n_shadow = L.n_camera = 0;
@@ -365,11 +367,11 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         g.list = f.shadow ? 0 : 1;
         const auto& list = c->lists[g.list];
         g.n_draws = uint32_t(list.size());
-        g.mvp_base = L.total_mvps;
         g.xv_base = uint32_t(L.total_xv);
         const uint32_t tris = c->list_tris[g.list];
         const uint32_t clip_cap = std::max<uint32_t>(4096, tris / 8) * c->rec_scale;
         g.rec_base = L.rec_total;
+        g.tri_count = tris;
         g.rec_cap = tris + 2 * clip_cap;
         g.clip_base = L.clip_total;
         g.clip_cap = clip_cap;
@@ -381,8 +383,6 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         g.bias_slope = c->rs.depth_bias_slope_factor;
         std::memcpy(g.view_projection, f.vp, sizeof(f.vp));
         L.total_bins += g.tiles_x * g.tiles_y;
-        L.total_mvps += g.n_draws;
-        L.max_draws = std::max(L.max_draws, g.n_draws);
         L.total_xv += c->list_verts[g.list];
         if (uint64_t(L.rec_total) + g.rec_cap >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "record buffer too large");
         L.rec_total += g.rec_cap;
@@ -398,7 +398,6 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     // buffers
     const size_t nv = views.size();
     HIP_TRY(c, c->views_d.ensure(nv * sizeof(ViewGPU)), "allocate views");
-    HIP_TRY(c, c->mvps.ensure(size_t(std::max<uint32_t>(1, L.total_mvps)) * 64), "allocate mvps");
     HIP_TRY(c, c->xv.ensure(size_t(std::max<uint64_t>(1, L.total_xv)) * sizeof(float4)), "allocate vertices");
     HIP_TRY(c, c->recs.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * sizeof(TriRec)), "allocate records");
     HIP_TRY(c, c->clipq.ensure(size_t(std::max<uint32_t>(1, L.clip_total)) * sizeof(uint2)), "allocate clip queue");
@@ -406,8 +405,8 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     HIP_TRY(c, c->bin_counts.ensure(size_t(L.total_bins) * 4), "allocate bins");
     HIP_TRY(c, c->bin_offsets.ensure(size_t(L.total_bins) * 4), "allocate bins");
     HIP_TRY(c, c->cursor.ensure(size_t(L.total_bins) * 4), "allocate bins");
-    HIP_TRY(c, c->block_sums.ensure(size_t(L.total_bins / 2048 + 2) * 4), "allocate scan");
-    HIP_TRY(c, c->total_pairs.ensure(16), "allocate scan");
+    HIP_TRY(c, c->block_valid.ensure(std::max<size_t>(1, L.tblocks.size()) * 4), "allocate statistics");
+    HIP_TRY(c, c->totals.ensure(16), "allocate statistics");
     HIP_TRY(c, c->vblocks_d.ensure(std::max<size_t>(1, L.vblocks.size()) * sizeof(BlockEntry)), "allocate blocks");
     HIP_TRY(c, c->tblocks_d.ensure(std::max<size_t>(1, L.tblocks.size()) * sizeof(BlockEntry)), "allocate blocks");
     if (!L.vblocks.empty())
@@ -443,13 +442,73 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
             ++ci;
         }
     }
-    HIP_TRY(c, c->staging.ensure(nv * sizeof(ViewGPU) + 64 * 1024), "allocate staging");
-    HIP_TRY(c, c->readback_counters.ensure(nv * kCtrPerView * 4 + 16), "allocate readback");
+    HIP_TRY(c, c->readback_counters.ensure(nv * kCtrPerView * 4 + 64), "allocate readback");
     L.valid = true;
     return RSY_OK;
 }
 
-// Enqueues one frame on the context stream (graph-capturable: kernels and pinned-memory copies only).
+// Per-frame constants (view records, shading views and cascades) go to the device through pinned
+// staging with async copies on the context stream.  They are not part of the captured frame graph:
+// a replayed graph reads whatever the last upload left in device memory.
+rsy_result upload_frame(rsy_ctx* c) {
+    Layout& L = c->layout;
+    hipStream_t s = c->stream;
+    const uint32_t nv = uint32_t(L.vg.size());
+    c->h_shade_views.clear();
+    c->h_shade_cascades.clear();
+    c->h_xv_bases.clear();
+    uint32_t blocks = 0, ci = 0;
+    for (uint32_t i = 0; i < nv; ++i) {
+        const ViewGPU& vg = L.vg[i];
+        if (vg.is_shadow) {
+            ShadeCascade sc{};
+            sc.size = vg.width;
+            sc.split_far = L.views[i].split_far;
+            sc.map = reinterpret_cast<const float*>(vg.out);
+            std::memcpy(sc.m, vg.view_projection, sizeof(sc.m));
+            c->h_shade_cascades.push_back(sc);
+        } else {
+            ShadeView sv{};
+            sv.width = vg.width;
+            sv.height = vg.height;
+            sv.blocks_x = (vg.width + 15) / 16;
+            sv.block_base = blocks;
+            blocks += sv.blocks_x * ((vg.height + 15) / 16);
+            sv.vis = reinterpret_cast<const unsigned long long*>(vg.out);
+            sv.color = c->color_out[ci].as<float4>();
+            sv.shadow_factor = (c->debug_flags & RSY_DEBUG_SHADOW_FACTOR) ? c->shadow_factor_out[ci].as<float>() : nullptr;
+            std::memcpy(sv.view_projection, vg.view_projection, sizeof(sv.view_projection));
+            std::memcpy(sv.camera_position, L.views[i].camera_position, sizeof(sv.camera_position));
+            c->h_shade_views.push_back(sv);
+            c->h_xv_bases.push_back(vg.xv_base);
+            ++ci;
+        }
+    }
+    c->shade_blocks = blocks;
+    const size_t nsv = c->h_shade_views.size(), nsc = c->h_shade_cascades.size();
+    const size_t a0 = 0, a1 = (nv * sizeof(ViewGPU) + 255) & ~size_t(255);
+    const size_t a2 = a1 + ((nsv * sizeof(ShadeView) + 255) & ~size_t(255));
+    const size_t a3 = a2 + ((nsc * sizeof(ShadeCascade) + 255) & ~size_t(255));
+    const size_t total = a3 + nsv * 4 + 256;
+    HIP_TRY(c, c->shade_views_d.ensure(std::max<size_t>(1, nsv) * sizeof(ShadeView) + 64), "allocate shade views");
+    HIP_TRY(c, c->shade_cascades_d.ensure(std::max<size_t>(1, nsc) * sizeof(ShadeCascade) + 64), "allocate cascades");
+    HIP_TRY(c, c->xv_bases_d.ensure(std::max<size_t>(1, nsv) * 4 + 64), "allocate xv bases");
+    HIP_TRY(c, c->staging.ensure(total), "allocate staging");
+    uint8_t* st = static_cast<uint8_t*>(c->staging.p);
+    std::memcpy(st + a0, L.vg.data(), nv * sizeof(ViewGPU));
+    if (nsv) std::memcpy(st + a1, c->h_shade_views.data(), nsv * sizeof(ShadeView));
+    if (nsc) std::memcpy(st + a2, c->h_shade_cascades.data(), nsc * sizeof(ShadeCascade));
+    if (nsv) std::memcpy(st + a3, c->h_xv_bases.data(), nsv * 4);
+    HIP_TRY(c, hipMemcpyAsync(c->views_d.p, st + a0, nv * sizeof(ViewGPU), hipMemcpyHostToDevice, s), "upload views");
+    if (nsv)
+        HIP_TRY(c, hipMemcpyAsync(c->shade_views_d.p, st + a1, nsv * sizeof(ShadeView), hipMemcpyHostToDevice, s), "upload");
+    if (nsc)
+        HIP_TRY(c, hipMemcpyAsync(c->shade_cascades_d.p, st + a2, nsc * sizeof(ShadeCascade), hipMemcpyHostToDevice, s), "upload");
+    if (nsv) HIP_TRY(c, hipMemcpyAsync(c->xv_bases_d.p, st + a3, nsv * 4, hipMemcpyHostToDevice, s), "upload");
+    return RSY_OK;
+}
+
+// Enqueues the kernels of one frame on the context stream (graph-capturable: kernels only).
 rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     Layout& L = c->layout;
     hipStream_t s = c->stream;
@@ -457,20 +516,13 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     const uint32_t shift = ilog2(L.tile);
     const bool timing = (c->debug_flags & RSY_DEBUG_STAGE_TIMING) && !capturing;
     if (!capturing) HIP_TRY(c, hipEventRecord(c->ev_begin, s), "event");
-    // views -> device through pinned staging
-    std::memcpy(c->staging.p, L.vg.data(), nv * sizeof(ViewGPU));
-    HIP_TRY(c, hipMemcpyAsync(c->views_d.p, c->staging.p, nv * sizeof(ViewGPU), hipMemcpyHostToDevice, s), "upload views");
-    HIP_TRY(c, launch_fill_u32(c->counters.as<uint32_t>(), nv * kCtrPerView, 0u, s), "clear counters");
-    HIP_TRY(c, launch_fill_u32(c->bin_counts.as<uint32_t>(), L.total_bins, 0u, s), "clear bins");
 
     GeometryLaunch g{};
     g.views = c->views_d.as<ViewGPU>();
     g.n_views = nv;
-    g.max_draws = std::max<uint32_t>(1, L.max_draws);
     g.draws0 = c->lists_d[0].as<DrawGPU>();
     g.draws1 = c->lists_d[1].as<DrawGPU>();
     g.objects = c->objects.as<float>();
-    g.mvps = c->mvps.as<float>();
     g.vblocks = c->vblocks_d.as<BlockEntry>();
     g.n_vblocks = uint32_t(L.vblocks.size());
     g.tblocks = c->tblocks_d.as<BlockEntry>();
@@ -484,8 +536,10 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     g.clipq = c->clipq.as<uint2>();
     g.counters = c->counters.as<uint32_t>();
     g.bin_counts = c->bin_counts.as<uint32_t>();
+    g.block_valid = c->block_valid.as<uint32_t>();
     g.shift = shift;
     g.clip_blocks = 64;
+    g.total_bins = L.total_bins;
     HIP_TRY(c, launch_geometry(g, s), "geometry kernels");
     if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[0], s), "event");
 
@@ -496,8 +550,9 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     b.bin_counts = g.bin_counts;
     b.bin_offsets = c->bin_offsets.as<uint32_t>();
     b.cursor = c->cursor.as<uint32_t>();
-    b.block_sums = c->block_sums.as<uint32_t>();
-    b.total_pairs = c->total_pairs.as<uint32_t>();
+    b.block_valid = c->block_valid.as<uint32_t>();
+    b.n_tblocks = uint32_t(L.tblocks.size());
+    b.totals = c->totals.as<uint32_t>();
     b.recs = g.recs;
     b.counters = g.counters;
     b.lists = c->lists_mem.as<uint32_t>();
@@ -523,61 +578,11 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
 
     if (L.shade && L.n_camera) {
         ShadeLaunch sh{};
-        // shading views and cascades (tiny; staged right after the view records)
-        c->h_shade_views.clear();
-        c->h_shade_cascades.clear();
-        std::vector<uint32_t> xv_bases;
-        uint32_t blocks = 0, ci = 0;
-        for (uint32_t i = 0; i < nv; ++i) {
-            const ViewGPU& vg = L.vg[i];
-            if (vg.is_shadow) {
-                ShadeCascade sc{};
-                sc.size = vg.width;
-                sc.split_far = L.views[i].split_far;
-                sc.map = reinterpret_cast<const float*>(vg.out);
-                std::memcpy(sc.m, vg.view_projection, sizeof(sc.m));
-                c->h_shade_cascades.push_back(sc);
-            } else {
-                ShadeView sv{};
-                sv.width = vg.width;
-                sv.height = vg.height;
-                sv.blocks_x = (vg.width + 15) / 16;
-                sv.block_base = blocks;
-                blocks += sv.blocks_x * ((vg.height + 15) / 16);
-                sv.vis = reinterpret_cast<const unsigned long long*>(vg.out);
-                sv.color = c->color_out[ci].as<float4>();
-                sv.shadow_factor = (c->debug_flags & RSY_DEBUG_SHADOW_FACTOR) ? c->shadow_factor_out[ci].as<float>() : nullptr;
-                std::memcpy(sv.view_projection, vg.view_projection, sizeof(sv.view_projection));
-                std::memcpy(sv.camera_position, L.views[i].camera_position, sizeof(sv.camera_position));
-                c->h_shade_views.push_back(sv);
-                xv_bases.push_back(vg.xv_base);
-                ++ci;
-            }
-        }
-        const size_t off_views = 4096 * 4;  // inside staging, after the ViewGPU records
-        const size_t nsv = c->h_shade_views.size(), nsc = c->h_shade_cascades.size();
-        const size_t bytes = nsv * sizeof(ShadeView) + nsc * sizeof(ShadeCascade) + nsv * 4 + 64;
-        HIP_TRY(c, c->shade_views_d.ensure(nsv * sizeof(ShadeView) + 64), "allocate shade views");
-        HIP_TRY(c, c->shade_cascades_d.ensure(std::max<size_t>(1, nsc) * sizeof(ShadeCascade) + 64), "allocate shade cascades");
-        HIP_TRY(c, c->xv_bases_d.ensure(nsv * 4 + 64), "allocate xv bases");
-        if (c->staging.cap < nv * sizeof(ViewGPU) + off_views + bytes)
-            return fail(c, RSY_ALLOCATION_FAILURE, "staging too small");
-        uint8_t* st = static_cast<uint8_t*>(c->staging.p) + nv * sizeof(ViewGPU);
-        st = reinterpret_cast<uint8_t*>((reinterpret_cast<uintptr_t>(st) + 255) & ~uintptr_t(255));
-        std::memcpy(st, c->h_shade_views.data(), nsv * sizeof(ShadeView));
-        uint8_t* st2 = st + ((nsv * sizeof(ShadeView) + 255) & ~size_t(255));
-        if (nsc) std::memcpy(st2, c->h_shade_cascades.data(), nsc * sizeof(ShadeCascade));
-        uint8_t* st3 = st2 + ((nsc * sizeof(ShadeCascade) + 255) & ~size_t(255));
-        std::memcpy(st3, xv_bases.data(), nsv * 4);
-        HIP_TRY(c, hipMemcpyAsync(c->shade_views_d.p, st, nsv * sizeof(ShadeView), hipMemcpyHostToDevice, s), "upload");
-        if (nsc)
-            HIP_TRY(c, hipMemcpyAsync(c->shade_cascades_d.p, st2, nsc * sizeof(ShadeCascade), hipMemcpyHostToDevice, s), "upload");
-        HIP_TRY(c, hipMemcpyAsync(c->xv_bases_d.p, st3, nsv * 4, hipMemcpyHostToDevice, s), "upload");
         sh.views = c->shade_views_d.as<ShadeView>();
-        sh.n_views = uint32_t(nsv);
-        sh.total_blocks = blocks;
+        sh.n_views = uint32_t(c->h_shade_views.size());
+        sh.total_blocks = c->shade_blocks;
         sh.cascades = c->shade_cascades_d.as<ShadeCascade>();
-        sh.n_cascades = uint32_t(nsc);
+        sh.n_cascades = uint32_t(c->h_shade_cascades.size());
         sh.pcf_mode = c->rs.pcf_mode;
         sh.scene = c->scene_d.p;
         sh.draws = c->lists_d[1].as<DrawGPU>();
@@ -604,7 +609,7 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[4], s), "event");
     if (!capturing) {
         HIP_TRY(c, hipMemcpyAsync(c->readback_counters.p, c->counters.p, nv * kCtrPerView * 4, hipMemcpyDeviceToHost, s), "readback");
-        HIP_TRY(c, hipMemcpyAsync(static_cast<uint8_t*>(c->readback_counters.p) + nv * kCtrPerView * 4, c->total_pairs.p, 4,
+        HIP_TRY(c, hipMemcpyAsync(static_cast<uint8_t*>(c->readback_counters.p) + nv * kCtrPerView * 4, c->totals.p, 12,
                                   hipMemcpyDeviceToHost, s), "readback");
         HIP_TRY(c, hipEventRecord(c->ev_end, s), "event");
     }
@@ -627,15 +632,16 @@ bool collect_stats(rsy_ctx* c) {
     const Layout& L = c->layout;
     const uint32_t nv = uint32_t(L.vg.size());
     const uint32_t* ctr = static_cast<const uint32_t*>(c->readback_counters.p);
-    const uint32_t total_pairs = ctr[nv * kCtrPerView];
-    bool overflow = total_pairs > c->list_cap;
+    const uint32_t* totals = ctr + nv * kCtrPerView;
+    bool overflow = totals[2] != 0;
     rsy_frame_stats st{};
+    st.tile_pairs = totals[0];
+    st.triangles_setup = totals[1];
     for (uint32_t v = 0; v < nv; ++v) {
         const uint32_t* vc = ctr + v * kCtrPerView;
         overflow |= vc[kCtrOverflow] != 0;
         st.triangles_setup += vc[kCtrRecords];
         st.triangles_clipped += vc[kCtrClipped];
-        st.tile_pairs += vc[kCtrPairs];
         const uint32_t list = L.vg[v].list;
         st.triangle_count += c->list_tris[list];
         st.draw_call_count += c->lists[list].size();
@@ -658,6 +664,8 @@ bool collect_stats(rsy_ctx* c) {
 rsy_result render_now(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint32_t shade) {
     rsy_result r = build_layout(c, views, choose_tile(c), shade != RSY_SHADE_NONE);
     if (r != RSY_OK) return r;
+    r = upload_frame(c);
+    if (r != RSY_OK) return r;
     r = enqueue_frame(c, false);
     if (r != RSY_OK) return r;
     c->last_views = views;
@@ -679,6 +687,7 @@ rsy_result finish_frame(rsy_ctx* c) {
         // grow: bin list capacity to the measured pair count, record/clip capacity x2
         const uint32_t nv = uint32_t(c->layout.vg.size());
         const uint32_t total_pairs = static_cast<const uint32_t*>(c->readback_counters.p)[nv * kCtrPerView];
+        const bool list_over = static_cast<const uint32_t*>(c->readback_counters.p)[nv * kCtrPerView + 2] != 0;
         bool rec_over = false;
         for (uint32_t v = 0; v < nv; ++v)
             rec_over |= static_cast<const uint32_t*>(c->readback_counters.p)[v * kCtrPerView + kCtrOverflow] != 0;
@@ -686,7 +695,7 @@ rsy_result finish_frame(rsy_ctx* c) {
             c->rec_scale *= 2;
             c->layout.valid = false;
         }
-        if (total_pairs > c->list_cap) {
+        if (list_over) {
             const uint64_t want = std::min<uint64_t>(uint64_t(total_pairs) + total_pairs / 4 + 1024, 0xfffffff0ull);
             if (want <= c->list_cap) return fail(c, RSY_OVERFLOW, "bin lists exceed 2^32 entries");
             HIP_TRY(c, c->lists_mem.ensure(size_t(want) * 4), "grow bin lists");
@@ -895,8 +904,8 @@ rsy_result rsy_update_graphic_objects(rsy_ctx* c, uint64_t offset, const rsy_obj
 rsy_result rsy_update(rsy_ctx* c, const rsy_scene_data* sd, const rsy_raster_state* state) {
     if (!c || !sd) return RSY_INVALID_ARGUMENT;
     if (state) {
-        if (state->tile_size != 0 && state->tile_size != 8 && state->tile_size != 16 && state->tile_size != 32)
-            return fail(c, RSY_INVALID_ARGUMENT, "tile_size must be 0, 8, 16 or 32");
+        if (state->tile_size != 0 && state->tile_size != 32 && state->tile_size != 64)
+            return fail(c, RSY_INVALID_ARGUMENT, "tile_size must be 0, 32 or 64");
         if (state->pcf_mode > RSY_PCF_GRID_3X3) return fail(c, RSY_INVALID_ARGUMENT, "unknown pcf mode");
         const bool raster_changed = state->cull_enabled != c->rs.cull_enabled ||
                                     state->reverse_winding_order_enabled != c->rs.reverse_winding_order_enabled ||
@@ -967,7 +976,7 @@ rsy_result rsy_render_graph(rsy_ctx* c, const rsy_frame_desc* frame, uint32_t n_
     for (uint32_t i = 0; i < n_replays; ++i) HIP_TRY(c, hipGraphLaunch(c->graph_exec, c->stream), "graph launch");
     const uint32_t nv = uint32_t(c->layout.vg.size());
     HIP_TRY(c, hipMemcpyAsync(c->readback_counters.p, c->counters.p, nv * kCtrPerView * 4, hipMemcpyDeviceToHost, c->stream), "readback");
-    HIP_TRY(c, hipMemcpyAsync(static_cast<uint8_t*>(c->readback_counters.p) + nv * kCtrPerView * 4, c->total_pairs.p, 4,
+    HIP_TRY(c, hipMemcpyAsync(static_cast<uint8_t*>(c->readback_counters.p) + nv * kCtrPerView * 4, c->totals.p, 12,
                               hipMemcpyDeviceToHost, c->stream), "readback");
     HIP_TRY(c, hipEventRecord(c->ev_end, c->stream), "event");
     HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
@@ -1069,8 +1078,8 @@ rsy_result rsy_autotune(rsy_ctx* c, const rsy_frame_desc* frame, uint32_t n_iter
     if (!c) return RSY_INVALID_ARGUMENT;
     const uint32_t saved = c->rs.tile_size;
     float best_ms = 1e30f;
-    uint32_t best = 16;
-    for (uint32_t t : {8u, 16u, 32u}) {
+    uint32_t best = 32;
+    for (uint32_t t : {32u, 64u}) {
         c->rs.tile_size = t;
         c->layout.valid = false;
         rsy_result r = rsy_render_graph(c, frame, std::max<uint32_t>(1, n_iters));

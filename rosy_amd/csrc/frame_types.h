@@ -29,7 +29,9 @@ struct DrawGPU {
     uint32_t material;     // material index or 0xFFFFFFFF
 };
 
-enum : uint32_t { kCtrRecords = 0, kCtrClip = 1, kCtrPairs = 2, kCtrOverflow = 3, kCtrClipped = 4, kCtrPerView = 8 };
+// Per-view counters: records appended by the clipper, queued clipped triangles, overflow flags,
+// clipped-triangle statistic.
+enum : uint32_t { kCtrRecords = 0, kCtrClip = 1, kCtrOverflow = 3, kCtrClipped = 4, kCtrPerView = 8 };
 
 struct ViewGPU {
     uint32_t width, height;
@@ -38,9 +40,9 @@ struct ViewGPU {
     uint32_t is_shadow;
     uint32_t list;          // 0 = shadow list, 1 = opaque list
     uint32_t n_draws;
-    uint32_t mvp_base;      // first MVP matrix of this view (one per draw)
     uint32_t xv_base;       // first transformed vertex of this view
-    uint32_t rec_base, rec_cap;
+    uint32_t rec_base, rec_cap;  // triangle slots [rec_base, rec_base + tri_count), then clipper output
+    uint32_t tri_count;
     uint32_t clip_base, clip_cap;
     uint32_t cull, reverse, bias;
     float bias_const, bias_clamp, bias_slope;

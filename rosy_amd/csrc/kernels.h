@@ -11,11 +11,9 @@ namespace rsy {
 struct GeometryLaunch {
     const ViewGPU* views;
     uint32_t n_views;
-    uint32_t max_draws;
     const DrawGPU* draws0;  // shadow list
     const DrawGPU* draws1;  // opaque list
     const float* objects;   // packed 164-byte object records
-    float* mvps;
     const BlockEntry* vblocks;
     uint32_t n_vblocks;
     const BlockEntry* tblocks;
@@ -27,8 +25,10 @@ struct GeometryLaunch {
     uint2* clipq;
     uint32_t* counters;
     uint32_t* bin_counts;
+    uint32_t* block_valid;  // records set up per setup block
     uint32_t shift;  // log2(tile)
     uint32_t clip_blocks;
+    uint32_t total_bins;
 };
 
 struct BinLaunch {
@@ -38,8 +38,9 @@ struct BinLaunch {
     uint32_t* bin_counts;
     uint32_t* bin_offsets;
     uint32_t* cursor;
-    uint32_t* block_sums;
-    uint32_t* total_pairs;
+    const uint32_t* block_valid;
+    uint32_t n_tblocks;
+    uint32_t* totals;  // [0] (record, bin) pairs, [1] records set up, [2] bin lists overflowed
     const TriRec* recs;
     const uint32_t* counters;
     uint32_t* lists;
@@ -58,7 +59,7 @@ struct RasterLaunch {
     const uint32_t* lists;
     uint32_t list_cap;
     const TriRec* recs;
-    uint32_t rec_cap_total;
+    uint64_t rec_cap_total;
 };
 
 // Per-pixel shading inputs of one camera view.

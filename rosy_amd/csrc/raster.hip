@@ -1,19 +1,22 @@
-// Geometry, binning and tile-raster kernels of the frame path (gfx950, wave64).
+// Geometry, binning and bin-raster kernels of the frame path (gfx950, wave64).
 //
 // Replaces the reference's shadow and main raster passes (shadow.slang:6-20, basic.slang:100-116
 // and the fixed-function state of Graphics.cpp:4988-5045, 5195-5292, 5553-5653) with a software
-// pipeline:
-//   k_mvp        per (view, draw)   M = VP * W                       (fmaf chain, k order)
-//   k_transform  per (view, vertex) clip = M * p from SoA x/y/z      (12 B in, 16 B out, coalesced)
-//   k_setup      per (view, tri)    clip test, snap, cull, bbox, depth plane, bias; records are
-//                                   compacted with a wave ballot and each record's bins are counted
-//                                   (wave-cooperative for triangles covering many bins)
+// pipeline.  Device-scope atomics are the scarce resource on MI355X, so every stage is laid out to
+// issue few of them:
+//   k_transform  per (view, vertex) clip = (VP * W) * p from SoA x/y/z (12 B in, 16 B out, coalesced);
+//                                   the block's VP * W is formed once in LDS (fmaf chain, k order)
+//   k_setup      per (view, tri)    clip test, snap, cull, bbox, depth plane, bias; the record goes
+//                                   to the triangle's own slot (no append); bin counts use one atomic
+//                                   per distinct bin per wave (ballot on the bin key), records that
+//                                   span many bins are spread over the wave
 //   k_clip       per clipped tri    near/far/guard-band Sutherland-Hodgman + fan, one wave each
-//   k_scan_*     all bins           exclusive prefix sum of the bin counts
-//   k_scatter    per record         bin lists (triangle record indices)
-//   k_raster     per (view, bin)    LDS-staged records; small triangles walk their pixels with LDS
-//                                   atomic max, large ones are tested pixel-parallel in registers;
-//                                   the bin is written once (the clear never touches HBM)
+//   k_scan       all bins           single-workgroup exclusive prefix sum (+ per-frame statistics)
+//   k_scatter    per record         bin lists, slots handed out by the same ballot aggregation
+//   k_raster     per (view, bin)    one workgroup per 32x32 or 64x64 bin, the bin's depth/ID tile in
+//                                   LDS: small triangles walk their own pixels, larger ones are
+//                                   covered by the whole workgroup; LDS atomic max resolves depth and
+//                                   the bin is written to HBM once (the clear never touches HBM)
 // Every floating-point step is compiled without contraction; the CPU oracle restates the same
 // rules, so visibility and depth are bit-exact by construction.
 #include <hip/hip_runtime.h>
@@ -26,8 +29,6 @@
 
 namespace rsy {
 namespace {
-
-constexpr int kSmallPixels = 24;  // records with at most this many bbox pixels in a bin walk them serially
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
@@ -122,31 +123,117 @@ __device__ inline void rec_tiles(const TriRec& r, uint32_t shift, int& tx0, int&
     ty1 = int(r.bby >> 16) >> shift;
 }
 
-// Bin counting for one record per active lane: lanes with few bins loop themselves, records with
-// many bins are spread over the whole wave.  Must be called by a converged wave.
-__device__ void count_bins(bool active, const TriRec& r, const ViewGPU& V, uint32_t shift, uint32_t* bin_counts,
-                           uint32_t* pair_counter) {
+constexpr uint32_t kNoBin = 0xffffffffu;
+constexpr uint32_t kInvalidBox = 0xffffffffu;  // bbx of a triangle slot that produced no record
+
+// Adds 1 to counts[key] for every lane whose key is not kNoBin with one atomic per distinct key in
+// the wave (lanes of one wave hold neighbouring triangles of one mesh, so keys repeat).  With kSlot,
+// returns each lane's own slot: the counter value before the wave's add plus the lane's rank among
+// the lanes sharing its key.  Must be called by a converged wave.
+template <bool kSlot>
+__device__ inline uint32_t aggregated_add(uint32_t key, uint32_t* counts) {
+    bool pending = key != kNoBin;
+    uint32_t slot = 0;
+    uint64_t todo = __ballot(pending);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t lkey = uint32_t(__builtin_amdgcn_readlane(int(key), leader));
+        const bool mine = pending && key == lkey;
+        const uint64_t mask = __ballot(mine);
+        uint32_t base = 0;
+        if (int(lane_id()) == leader) base = atomicAdd(&counts[lkey], uint32_t(__popcll(mask)));
+        if (kSlot) base = uint32_t(__builtin_amdgcn_readlane(int(base), leader));
+        if (mine) {
+            slot = base + uint32_t(__popcll(mask & lanemask_lt()));
+            pending = false;
+        }
+        todo &= ~mask;
+    }
+    return slot;
+}
+
+constexpr int kSmallBins = 4;  // records touching at most this many bins are binned per lane
+
+// Bin counting for one record per active lane.  Must be called by a converged wave.
+__device__ void count_bins(bool active, const TriRec& r, const ViewGPU& V, uint32_t shift, uint32_t* bin_counts) {
     int tx0 = 0, ty0 = 0, tx1 = -1, ty1 = -1;
     if (active) rec_tiles(r, shift, tx0, ty0, tx1, ty1);
-    const int n = active ? (tx1 - tx0 + 1) * (ty1 - ty0 + 1) : 0;
-    const bool small = active && n <= 4;
-    uint32_t* counts = bin_counts + V.tile_base;
-    if (small)
-        for (int ty = ty0; ty <= ty1; ++ty)
-            for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&counts[ty * int(V.tiles_x) + tx], 1u);
+    const int bw = tx1 - tx0 + 1;
+    const int n = active ? bw * (ty1 - ty0 + 1) : 0;
+    const bool small = active && n <= kSmallBins;
+#pragma unroll
+    for (int k = 0; k < kSmallBins; ++k) {
+        const uint32_t key =
+            (small && k < n) ? V.tile_base + uint32_t((ty0 + k / bw) * int(V.tiles_x) + tx0 + k % bw) : kNoBin;
+        aggregated_add<false>(key, bin_counts);
+    }
     uint64_t big = __ballot(active && !small);
     while (big) {
         const int l = __ffsll((unsigned long long)big) - 1;
         big &= big - 1;
-        const int btx0 = __shfl(tx0, l), bty0 = __shfl(ty0, l), btx1 = __shfl(tx1, l), bty1 = __shfl(ty1, l);
-        const int bw = btx1 - btx0 + 1, bn = bw * (bty1 - bty0 + 1);
+        const int btx0 = __builtin_amdgcn_readlane(tx0, l), bty0 = __builtin_amdgcn_readlane(ty0, l);
+        const int btx1 = __builtin_amdgcn_readlane(tx1, l), bty1 = __builtin_amdgcn_readlane(ty1, l);
+        const int bbw = btx1 - btx0 + 1, bn = bbw * (bty1 - bty0 + 1);
         for (int k = int(lane_id()); k < bn; k += 64)
-            atomicAdd(&counts[(bty0 + k / bw) * int(V.tiles_x) + btx0 + k % bw], 1u);
+            atomicAdd(&bin_counts[V.tile_base + (bty0 + k / bbw) * int(V.tiles_x) + btx0 + k % bbw], 1u);
     }
-    // wave-aggregated pair statistic
-    uint32_t s = uint32_t(n);
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if (lane_id() == 0 && s) atomicAdd(pair_counter, s);
+}
+
+// Edge functions of a record (inside >= 0 with ties broken by the top-left rule, y down).
+struct Edges {
+    int32_t X[3], Y[3], A[3], B[3];
+    bool tl[3];
+};
+
+__device__ inline void make_edges(const TriRec& r, Edges& e) {
+    e.X[0] = r.X0; e.X[1] = r.X1; e.X[2] = r.X2;
+    e.Y[0] = r.Y0; e.Y[1] = r.Y1; e.Y[2] = r.Y2;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int f = k == 2 ? 0 : k + 1;
+        e.A[k] = e.Y[f] - e.Y[k];
+        e.B[k] = e.X[k] - e.X[f];
+        e.tl[k] = e.A[k] > 0 || (e.A[k] == 0 && e.B[k] > 0);
+    }
+}
+
+// Visibility word of pixel (x, y) for the record (depth bits << 32 | id + 1, or depth bits alone in a
+// shadow view), or 0 when the pixel centre is not covered.
+__device__ inline unsigned long long fragment(const TriRec& r, const Edges& e, int x, int y, bool shadow) {
+    const int32_t Px = x * 256 + 128, Py = y * 256 + 128;
+    bool inside = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int64_t ev = int64_t(e.A[k]) * (Px - e.X[k]) + int64_t(e.B[k]) * (Py - e.Y[k]);
+        inside &= ev > 0 || (ev == 0 && e.tl[k]);
+    }
+    if (!inside) return 0ull;
+    float z = fmaf(r.dzdy, float(Py - r.Y0), fmaf(r.dzdx, float(Px - r.X0), r.z0));
+    z = fminf(fmaxf(z, 0.0f), 1.0f);
+    const uint32_t zb = fbits(z) & 0x7fffffffu;
+    return shadow ? (unsigned long long)zb : ((unsigned long long)zb << 32) | (unsigned long long)(r.id + 1u);
+}
+
+// Exclusive prefix sum over the workgroup (wave scans with shuffles, wave totals through LDS).
+// Every thread of the block must call it; it synchronises the block.
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds_waves, uint32_t& total) {
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (int(lane_id()) >= off) x += y;
+    }
+    const uint32_t wave = threadIdx.x / 64, nwaves = blockDim.x / 64;
+    if (lane_id() == 63) lds_waves[wave] = x;
+    __syncthreads();
+    uint32_t wave_off = 0, sum = 0;
+    for (uint32_t w = 0; w < nwaves; ++w) {
+        const uint32_t s = lds_waves[w];
+        if (w < wave) wave_off += s;
+        sum += s;
+    }
+    __syncthreads();
+    total = sum;
+    return wave_off + x - v;
 }
 
 }  // namespace
@@ -157,40 +244,39 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
-// MVP per (view, draw): views[v].view_projection * objects[draw.object].transform.
-__global__ void k_mvp(const ViewGPU* views, uint32_t n_views, const DrawGPU* draws0, const DrawGPU* draws1,
-                      const float* objects, float* mvps) {
-    const uint32_t v = blockIdx.y;
-    if (v >= n_views) return;
-    const ViewGPU& V = views[v];
-    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= V.n_draws) return;
-    const DrawGPU& dr = (V.list ? draws1 : draws0)[d];
-    const float* a = V.view_projection;
-    const float* b = objects + size_t(dr.object) * 41u;  // 164-byte packed records
-    float* c = mvps + size_t(V.mvp_base + d) * 16u;
-#pragma unroll
-    for (int col = 0; col < 4; ++col)
-#pragma unroll
-        for (int row = 0; row < 4; ++row) {
-            float s = a[row] * b[col * 4 + 0];
-            s = fmaf(a[4 + row], b[col * 4 + 1], s);
-            s = fmaf(a[8 + row], b[col * 4 + 2], s);
-            s = fmaf(a[12 + row], b[col * 4 + 3], s);
-            c[col * 4 + row] = s;
-        }
+// Zeroes two ranges (the per-view counters and the bin counts) in one launch; graph-safe.
+__global__ void k_clear2(uint32_t* a, uint32_t na, uint32_t* b, uint32_t nb) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na + nb; i += gridDim.x * blockDim.x) {
+        if (i < na) a[i] = 0u;
+        else b[i - na] = 0u;
+    }
 }
 
+// clip = (VP * W) * p per (view, vertex).  Every block covers vertices of one (view, draw); its
+// MVP = views[v].view_projection * objects[draw.object].transform is formed once in LDS by the fmaf
+// chain in k order, then each thread reads its vertex from the SoA x/y/z streams (coalesced 12 B)
+// and writes the clip position (coalesced 16 B).
 __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, const ViewGPU* views,
                                                    const DrawGPU* draws0, const DrawGPU* draws1,
-                                                   const float* mvps, const float* px, const float* py,
+                                                   const float* objects, const float* px, const float* py,
                                                    const float* pz, float4* xv) {
+    __shared__ float m[16];
     const BlockEntry e = blocks[blockIdx.x];
     const ViewGPU& V = views[e.view];
     const DrawGPU& dr = (V.list ? draws1 : draws0)[e.draw];
+    if (threadIdx.x < 16) {
+        const int col = threadIdx.x >> 2, row = threadIdx.x & 3;
+        const float* a = V.view_projection;
+        const float* b = objects + size_t(dr.object) * 41u;  // 164-byte packed records
+        float acc = a[row] * b[col * 4 + 0];
+        acc = fmaf(a[4 + row], b[col * 4 + 1], acc);
+        acc = fmaf(a[8 + row], b[col * 4 + 2], acc);
+        acc = fmaf(a[12 + row], b[col * 4 + 3], acc);
+        m[threadIdx.x] = acc;
+    }
+    __syncthreads();
     const uint32_t v = e.start + threadIdx.x;
     if (v >= dr.vtx_count) return;
-    const float* m = mvps + size_t(V.mvp_base + e.draw) * 16u;
     const uint32_t g = dr.vtx_base + v;
     xv[size_t(V.xv_base) + dr.xv_off + v] = transform(m, px[g], py[g], pz[g]);
 }
@@ -198,7 +284,8 @@ __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, con
 __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const ViewGPU* views, const DrawGPU* draws0,
                                                const DrawGPU* draws1, const uint32_t* indices, const float4* xv,
                                                TriRec* recs, uint2* clipq, uint32_t* counters,
-                                               uint32_t* bin_counts, uint32_t shift) {
+                                               uint32_t* bin_counts, uint32_t* block_valid, uint32_t shift) {
+    __shared__ uint32_t wave_valid[4];
     const BlockEntry e = blocks[blockIdx.x];
     const ViewGPU& V = views[e.view];
     const DrawGPU& dr = (V.list ? draws1 : draws0)[e.draw];
@@ -223,24 +310,23 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
         status = reject ? 0 : (all_inside(c[0], c[1], c[2]) ? 1 : 2);
     }
     TriRec r;
-    bool has = false;
-    if (status == 1) has = setup_tri(c, dr.tri_base + t, V, r);
-
+    const bool has = status == 1 && setup_tri(c, dr.tri_base + t, V, r);
+    if (valid) {
+        // every triangle owns one record slot; a slot without a record is marked by its box
+        const size_t slot = size_t(V.rec_base) + dr.tri_base + t;
+        if (has) recs[slot] = r;
+        else *reinterpret_cast<uint2*>(&recs[slot].bbx) = make_uint2(kInvalidBox, kInvalidBox);
+    }
     const uint32_t cslot = wave_append(status == 2, &ctr[kCtrClip]);
     if (status == 2) {
         if (cslot < V.clip_cap) clipq[V.clip_base + cslot] = make_uint2(e.draw, t);
         else atomicOr(&ctr[kCtrOverflow], 1u);
     }
-    const uint32_t rslot = wave_append(has, &ctr[kCtrRecords]);
-    if (has) {
-        if (rslot < V.rec_cap) {
-            recs[V.rec_base + rslot] = r;
-        } else {
-            atomicOr(&ctr[kCtrOverflow], 2u);
-            has = false;
-        }
-    }
-    count_bins(has, r, V, shift, bin_counts, &ctr[kCtrPairs]);
+    count_bins(has, r, V, shift, bin_counts);
+    const uint32_t nvalid = uint32_t(__popcll(__ballot(has)));
+    if (lane_id() == 0) wave_valid[threadIdx.x >> 6] = nvalid;
+    __syncthreads();
+    if (threadIdx.x == 0) block_valid[blockIdx.x] = wave_valid[0] + wave_valid[1] + wave_valid[2] + wave_valid[3];
 }
 
 // One wave per queued triangle: lane 0 clips and sets up the fan; the wave counts the bins.
@@ -300,8 +386,8 @@ __global__ __launch_bounds__(256) void k_clip(const ViewGPU* views, const DrawGP
                     TriRec r;
                     if (setup_tri(tv, dr.tri_base + ent.y, V, r)) {
                         const uint32_t slot = atomicAdd(&ctr[kCtrRecords], 1u);
-                        if (slot < V.rec_cap) {
-                            recs[V.rec_base + slot] = r;
+                        if (V.tri_count + slot < V.rec_cap) {
+                            recs[size_t(V.rec_base) + V.tri_count + slot] = r;
                             out[nout++] = r;
                         } else {
                             atomicOr(&ctr[kCtrOverflow], 2u);
@@ -316,260 +402,206 @@ __global__ __launch_bounds__(256) void k_clip(const ViewGPU* views, const DrawGP
             TriRec r;
             r.bbx = __shfl(out[k].bbx, 0);
             r.bby = __shfl(out[k].bby, 0);
-            count_bins(lane_id() == 0, r, V, shift, bin_counts, &ctr[kCtrPairs]);
+            count_bins(lane_id() == 0, r, V, shift, bin_counts);
         }
     }
 }
 
-// Exclusive scan of n bin counts: per-block sums, a one-block scan of the sums, then the local scan.
-constexpr uint32_t kScanPerThread = 8;
-constexpr uint32_t kScanBlock = 256 * kScanPerThread;
+// Exclusive prefix sum of the bin counts in one workgroup (the bin count is a few tens of
+// thousands), plus the frame statistics: total (record, bin) pairs, records set up, and the bin-list
+// overflow flag.
+constexpr uint32_t kScanThreads = 1024;
+constexpr uint32_t kScanPerThread = 4;
 
-__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds_waves, uint32_t& total) {
-    // inclusive scan within the wave
-    uint32_t x = v;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off);
-        if (int(lane_id()) >= off) x += y;
-    }
-    const uint32_t wave = threadIdx.x / 64, nwaves = blockDim.x / 64;
-    if (lane_id() == 63) lds_waves[wave] = x;
-    __syncthreads();
-    uint32_t wave_off = 0, sum = 0;
-    for (uint32_t w = 0; w < nwaves; ++w) {
-        const uint32_t s = lds_waves[w];
-        if (w < wave) wave_off += s;
-        sum += s;
-    }
-    __syncthreads();
-    total = sum;
-    return wave_off + x - v;
-}
-
-__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* counts, uint32_t n, uint32_t* block_sums) {
-    __shared__ uint32_t lds[4];
-    const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * kScanPerThread;
-    uint32_t s = 0;
-    for (uint32_t k = 0; k < kScanPerThread; ++k)
-        if (base + k < n) s += counts[base + k];
-    uint32_t total;
-    block_exclusive_scan(s, lds, total);
-    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
-}
-
-__global__ __launch_bounds__(256) void k_scan_top(uint32_t* block_sums, uint32_t nb, uint32_t* total_out) {
-    __shared__ uint32_t lds[4];
+__global__ __launch_bounds__(1024) void k_scan(const uint32_t* counts, uint32_t n, uint32_t* offsets,
+                                               uint32_t* cursor, const uint32_t* block_valid, uint32_t n_blocks,
+                                               uint32_t list_cap, uint32_t* totals) {
+    __shared__ uint32_t lds[16];
     uint32_t carry = 0;
-    for (uint32_t base = 0; base < nb; base += 256) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t v = i < nb ? block_sums[i] : 0;
+    for (uint32_t base = 0; base < n; base += kScanThreads * kScanPerThread) {
+        const uint32_t i0 = base + threadIdx.x * kScanPerThread;
+        uint32_t v[kScanPerThread];
+        uint32_t s = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPerThread; ++k) {
+            v[k] = i0 + k < n ? counts[i0 + k] : 0u;
+            s += v[k];
+        }
         uint32_t total;
-        const uint32_t ex = block_exclusive_scan(v, lds, total);
-        if (i < nb) block_sums[i] = carry + ex;
+        uint32_t run = carry + block_exclusive_scan(s, lds, total);
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPerThread; ++k)
+            if (i0 + k < n) {
+                offsets[i0 + k] = run;
+                cursor[i0 + k] = run;
+                run += v[k];
+            }
         carry += total;
     }
-    if (threadIdx.x == 0) *total_out = carry;
-}
-
-__global__ __launch_bounds__(256) void k_scan_down(const uint32_t* counts, uint32_t n, const uint32_t* block_sums,
-                                                   uint32_t* offsets, uint32_t* cursor) {
-    __shared__ uint32_t lds[4];
-    const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * kScanPerThread;
-    uint32_t v[kScanPerThread];
-    uint32_t s = 0;
-    for (uint32_t k = 0; k < kScanPerThread; ++k) {
-        v[k] = base + k < n ? counts[base + k] : 0u;
-        s += v[k];
+    uint32_t vs = 0;
+    for (uint32_t i = threadIdx.x; i < n_blocks; i += kScanThreads) vs += block_valid[i];
+    uint32_t total_valid;
+    block_exclusive_scan(vs, lds, total_valid);
+    if (threadIdx.x == 0) {
+        totals[0] = carry;
+        totals[1] = total_valid;
+        totals[2] = carry > list_cap ? 1u : 0u;
     }
-    uint32_t total;
-    uint32_t run = block_exclusive_scan(s, lds, total) + block_sums[blockIdx.x];
-    for (uint32_t k = 0; k < kScanPerThread; ++k)
-        if (base + k < n) {
-            offsets[base + k] = run;
-            cursor[base + k] = run;
-            run += v[k];
-        }
 }
 
-// Bin lists: every record index is appended to each bin its pixel bounds touch.
+// Bin lists: every record index is appended to each bin its pixel bounds touch.  Slots come from
+// the bin cursors with one atomic per distinct bin per wave.
 __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, const TriRec* recs, const uint32_t* counters,
                                                  uint32_t* cursor, uint32_t* lists, uint32_t list_cap,
                                                  uint32_t shift) {
     const uint32_t view = blockIdx.y;
     const ViewGPU& V = views[view];
-    const uint32_t n = min(counters[view * kCtrPerView + kCtrRecords], V.rec_cap);
-    uint32_t* cur = cursor + V.tile_base;
+    const uint32_t extra = min(counters[view * kCtrPerView + kCtrRecords], V.rec_cap - V.tri_count);
+    const uint32_t n = V.tri_count + extra;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const uint32_t i = base + threadIdx.x;
-        const bool active = i < n;
-        int tx0 = 0, ty0 = 0, tx1 = -1, ty1 = -1;
         const uint32_t ri = V.rec_base + i;
-        if (active) {
+        bool active = false;
+        int tx0 = 0, ty0 = 0, tx1 = -1, ty1 = -1;
+        if (i < n) {
             const uint2 bb = *reinterpret_cast<const uint2*>(&recs[ri].bbx);
-            TriRec r;
-            r.bbx = bb.x;
-            r.bby = bb.y;
-            rec_tiles(r, shift, tx0, ty0, tx1, ty1);
+            if (bb.x != kInvalidBox) {
+                active = true;
+                TriRec r;
+                r.bbx = bb.x;
+                r.bby = bb.y;
+                rec_tiles(r, shift, tx0, ty0, tx1, ty1);
+            }
         }
-        const int nb = active ? (tx1 - tx0 + 1) * (ty1 - ty0 + 1) : 0;
-        const bool small = active && nb <= 4;
-        if (small)
-            for (int ty = ty0; ty <= ty1; ++ty)
-                for (int tx = tx0; tx <= tx1; ++tx) {
-                    const uint32_t slot = atomicAdd(&cur[ty * int(V.tiles_x) + tx], 1u);
-                    if (slot < list_cap) lists[slot] = ri;
-                }
+        const int bw = tx1 - tx0 + 1;
+        const int nb = active ? bw * (ty1 - ty0 + 1) : 0;
+        const bool small = active && nb <= kSmallBins;
+#pragma unroll
+        for (int k = 0; k < kSmallBins; ++k) {
+            const uint32_t key =
+                (small && k < nb) ? V.tile_base + uint32_t((ty0 + k / bw) * int(V.tiles_x) + tx0 + k % bw) : kNoBin;
+            const uint32_t slot = aggregated_add<true>(key, cursor);
+            if (key != kNoBin && slot < list_cap) lists[slot] = ri;
+        }
         uint64_t big = __ballot(active && !small);
         while (big) {
             const int l = __ffsll((unsigned long long)big) - 1;
             big &= big - 1;
-            const int btx0 = __shfl(tx0, l), bty0 = __shfl(ty0, l), btx1 = __shfl(tx1, l), bty1 = __shfl(ty1, l);
-            const uint32_t bri = __shfl(ri, l);
-            const int bw = btx1 - btx0 + 1, bn = bw * (bty1 - bty0 + 1);
+            const int btx0 = __builtin_amdgcn_readlane(tx0, l), bty0 = __builtin_amdgcn_readlane(ty0, l);
+            const int btx1 = __builtin_amdgcn_readlane(tx1, l), bty1 = __builtin_amdgcn_readlane(ty1, l);
+            const uint32_t bri = uint32_t(__builtin_amdgcn_readlane(int(ri), l));
+            const int bbw = btx1 - btx0 + 1, bn = bbw * (bty1 - bty0 + 1);
             for (int k = int(lane_id()); k < bn; k += 64) {
-                const uint32_t slot = atomicAdd(&cur[(bty0 + k / bw) * int(V.tiles_x) + btx0 + k % bw], 1u);
+                const uint32_t slot = atomicAdd(&cursor[V.tile_base + (bty0 + k / bbw) * int(V.tiles_x) + btx0 + k % bbw], 1u);
                 if (slot < list_cap) lists[slot] = bri;
             }
         }
     }
 }
 
-// Tile raster.  One workgroup per bin; blocks are mapped so that neighbouring bins run on the same
-// XCD (the hardware deals workgroups round-robin over the 8 XCDs, each with its own L2).
+// Bin raster: one workgroup per bin of T x T pixels, the bin's visibility (or depth) tile in LDS.
+// The bin list is taken 256 records at a time; each record's box is clipped to the bin and its rows
+// become tasks: a prefix sum over the rows numbers them, and every thread takes tasks
+// (record, row) in turn, so small and large triangles spread evenly over the workgroup.  Along a
+// row the three int64 edge functions advance by 256 * A per pixel (exact), the depth plane is
+// evaluated per pixel by the same FP32 rule as setup, and fragments merge with LDS atomic max
+// (depth bits above triangle id + 1: nearer depth wins under reverse-Z, the later triangle wins
+// ties).  Workgroups are mapped so that neighbouring bins run on the same XCD (workgroups are dealt
+// round-robin over the 8 XCDs, each with its own L2).
 template <int T>
 __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n_views, uint32_t total_bins,
                                                 const uint32_t* bin_counts, const uint32_t* bin_offsets,
                                                 const uint32_t* lists, uint32_t list_cap, const TriRec* recs,
-                                                uint32_t rec_cap_total) {
-    constexpr int NT = (T * T < 256) ? T * T : 256;  // threads per workgroup
-    constexpr int PPT = T * T / NT;                  // pixels per thread
-    __shared__ TriRec big_recs[NT];
-    __shared__ unsigned long long best_lds[T * T];
-    __shared__ uint32_t big_n;
+                                                uint64_t rec_cap_total) {
+    __shared__ unsigned long long tile[T * T];
+    __shared__ TriRec chunk_recs[256];
+    __shared__ uint32_t chunk_box[256];   // clipped box: x0 | x1 << 8 | y0 << 16 | y1 << 24 (bin-local)
+    __shared__ uint32_t row_start[257];   // exclusive prefix of row counts
+    __shared__ uint32_t scan_lds[4];
 
-    const uint32_t chunk = (total_bins + 7) / 8;
-    const uint32_t bin = (blockIdx.x % 8) * chunk + blockIdx.x / 8;
+    const uint32_t groups = (total_bins + 7) / 8;
+    const uint32_t bin = (blockIdx.x % 8) * groups + blockIdx.x / 8;
     if (bin >= total_bins) return;
     uint32_t view = 0;
     while (view + 1 < n_views && views[view + 1].tile_base <= bin) ++view;
     const ViewGPU& V = views[view];
     const uint32_t local_bin = bin - V.tile_base;
-    const int tx = int(local_bin % V.tiles_x), ty = int(local_bin / V.tiles_x);
-    const int ox = tx * T, oy = ty * T;
+    const int ox = int(local_bin % V.tiles_x) * T, oy = int(local_bin / V.tiles_x) * T;
     const bool shadow = V.is_shadow != 0;
 
-    for (int p = threadIdx.x; p < T * T; p += NT) best_lds[p] = 0ull;
-    if (threadIdx.x == 0) big_n = 0;
-    unsigned long long best[PPT];
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) best[k] = 0ull;
-    __syncthreads();
+    for (int p = threadIdx.x; p < T * T; p += 256) tile[p] = 0ull;
 
-    const uint32_t count = bin_counts[bin];
     const uint32_t off = bin_offsets[bin];
-    const uint32_t end = min(off + count, list_cap);
-    for (uint32_t base = off; base < end; base += NT) {
+    const uint32_t end = min(off + bin_counts[bin], list_cap);
+    for (uint32_t base = off; base < end; base += 256) {
         const uint32_t i = base + threadIdx.x;
+        uint32_t rows = 0;
         if (i < end) {
             const uint32_t ri = lists[i];
             if (ri < rec_cap_total) {
                 const TriRec r = recs[ri];
-                const int bx0 = max(int(r.bbx & 0xffffu), ox), bx1 = min(int(r.bbx >> 16), ox + T - 1);
-                const int by0 = max(int(r.bby & 0xffffu), oy), by1 = min(int(r.bby >> 16), oy + T - 1);
-                const int area = (bx1 - bx0 + 1) * (by1 - by0 + 1);
+                const int bx0 = max(int(r.bbx & 0xffffu), ox) - ox, bx1 = min(int(r.bbx >> 16), ox + T - 1) - ox;
+                const int by0 = max(int(r.bby & 0xffffu), oy) - oy, by1 = min(int(r.bby >> 16), oy + T - 1) - oy;
                 if (bx0 <= bx1 && by0 <= by1) {
-                    if (area <= kSmallPixels) {
-                        int32_t A[3], B[3], X[3], Y[3];
-                        X[0] = r.X0; X[1] = r.X1; X[2] = r.X2;
-                        Y[0] = r.Y0; Y[1] = r.Y1; Y[2] = r.Y2;
-                        bool tl[3];
-#pragma unroll
-                        for (int e = 0; e < 3; ++e) {
-                            const int f = e == 2 ? 0 : e + 1;
-                            A[e] = Y[f] - Y[e];
-                            B[e] = X[e] - X[f];
-                            tl[e] = A[e] > 0 || (A[e] == 0 && B[e] > 0);
-                        }
-                        for (int y = by0; y <= by1; ++y) {
-                            const int32_t Py = y * 256 + 128;
-                            for (int x = bx0; x <= bx1; ++x) {
-                                const int32_t Px = x * 256 + 128;
-                                bool inside = true;
-#pragma unroll
-                                for (int e = 0; e < 3; ++e) {
-                                    const int64_t ev = int64_t(A[e]) * (Px - X[e]) + int64_t(B[e]) * (Py - Y[e]);
-                                    inside &= ev > 0 || (ev == 0 && tl[e]);
-                                }
-                                if (inside) {
-                                    float z = fmaf(r.dzdy, float(Py - r.Y0), fmaf(r.dzdx, float(Px - r.X0), r.z0));
-                                    z = fminf(fmaxf(z, 0.0f), 1.0f);
-                                    const uint32_t zb = fbits(z) & 0x7fffffffu;
-                                    const unsigned long long w =
-                                        shadow ? (unsigned long long)zb
-                                               : ((unsigned long long)zb << 32) | (unsigned long long)(r.id + 1u);
-                                    atomicMax(&best_lds[(y - oy) * T + (x - ox)], w);
-                                }
-                            }
-                        }
-                    } else {
-                        const uint32_t slot = atomicAdd(&big_n, 1u);
-                        big_recs[slot] = r;
-                    }
+                    rows = uint32_t(by1 - by0 + 1);
+                    chunk_recs[threadIdx.x] = r;
+                    chunk_box[threadIdx.x] = uint32_t(bx0) | (uint32_t(bx1) << 8) | (uint32_t(by0) << 16) |
+                                             (uint32_t(by1) << 24);
                 }
             }
         }
+        uint32_t total_rows;
+        const uint32_t first = block_exclusive_scan(rows, scan_lds, total_rows);  // synchronises
+        row_start[threadIdx.x] = first;
+        if (threadIdx.x == 255) row_start[256] = first + rows;
         __syncthreads();
-        const uint32_t nbig = big_n;
-        for (uint32_t j = 0; j < nbig; ++j) {
-            const TriRec r = big_recs[j];
-            int32_t A[3], B[3], X[3], Y[3];
-            X[0] = r.X0; X[1] = r.X1; X[2] = r.X2;
-            Y[0] = r.Y0; Y[1] = r.Y1; Y[2] = r.Y2;
-            bool tl[3];
-#pragma unroll
-            for (int e = 0; e < 3; ++e) {
-                const int f = e == 2 ? 0 : e + 1;
-                A[e] = Y[f] - Y[e];
-                B[e] = X[e] - X[f];
-                tl[e] = A[e] > 0 || (A[e] == 0 && B[e] > 0);
+        for (uint32_t task = threadIdx.x; task < total_rows; task += 256) {
+            // record j: the last record whose first row is <= task (records without rows share
+            // their successor's start and are skipped by the upper-bound search)
+            uint32_t lo = 0, hi = 256;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (row_start[mid] <= task) lo = mid; else hi = mid;
             }
-            const int bx0 = int(r.bbx & 0xffffu), bx1 = int(r.bbx >> 16);
-            const int by0 = int(r.bby & 0xffffu), by1 = int(r.bby >> 16);
+            const uint32_t j = lo;
+            const TriRec r = chunk_recs[j];
+            const uint32_t box = chunk_box[j];
+            const int bx0 = int(box & 0xffu), bx1 = int((box >> 8) & 0xffu);
+            const int y = int((box >> 16) & 0xffu) + int(task - row_start[j]);
+            Edges ed;
+            make_edges(r, ed);
+            const int32_t Py = (oy + y) * 256 + 128;
+            int32_t Px = (ox + bx0) * 256 + 128;
+            int64_t e[3];
 #pragma unroll
-            for (int k = 0; k < PPT; ++k) {
-                const int p = threadIdx.x + k * NT;
-                const int x = ox + p % T, y = oy + p / T;
-                if (x < bx0 || x > bx1 || y < by0 || y > by1) continue;
-                const int32_t Px = x * 256 + 128, Py = y * 256 + 128;
+            for (int k = 0; k < 3; ++k) e[k] = int64_t(ed.A[k]) * (Px - ed.X[k]) + int64_t(ed.B[k]) * (Py - ed.Y[k]);
+            const float fy = float(Py - r.Y0);
+            unsigned long long* row = tile + y * T;
+            for (int x = bx0; x <= bx1; ++x, Px += 256) {
                 bool inside = true;
 #pragma unroll
-                for (int e = 0; e < 3; ++e) {
-                    const int64_t ev = int64_t(A[e]) * (Px - X[e]) + int64_t(B[e]) * (Py - Y[e]);
-                    inside &= ev > 0 || (ev == 0 && tl[e]);
+                for (int k = 0; k < 3; ++k) inside &= e[k] > 0 || (e[k] == 0 && ed.tl[k]);
+                if (inside) {
+                    float z = fmaf(r.dzdy, fy, fmaf(r.dzdx, float(Px - r.X0), r.z0));
+                    z = fminf(fmaxf(z, 0.0f), 1.0f);
+                    const uint32_t zb = fbits(z) & 0x7fffffffu;
+                    const unsigned long long w =
+                        shadow ? (unsigned long long)zb : ((unsigned long long)zb << 32) | (unsigned long long)(r.id + 1u);
+                    atomicMax(&row[x], w);
                 }
-                if (!inside) continue;
-                float z = fmaf(r.dzdy, float(Py - r.Y0), fmaf(r.dzdx, float(Px - r.X0), r.z0));
-                z = fminf(fmaxf(z, 0.0f), 1.0f);
-                const uint32_t zb = fbits(z) & 0x7fffffffu;
-                const unsigned long long w =
-                    shadow ? (unsigned long long)zb : ((unsigned long long)zb << 32) | (unsigned long long)(r.id + 1u);
-                best[k] = w > best[k] ? w : best[k];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) e[k] += int64_t(ed.A[k]) * 256;
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0) big_n = 0;
-        __syncthreads();
     }
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-        const int p = threadIdx.x + k * NT;
+    __syncthreads();
+    for (int p = threadIdx.x; p < T * T; p += 256) {
         const int x = ox + p % T, y = oy + p / T;
-        const unsigned long long a = best_lds[p];
-        const unsigned long long w = a > best[k] ? a : best[k];
         if (x < int(V.width) && y < int(V.height)) {
             const size_t o = size_t(y) * V.width + x;
-            if (shadow) reinterpret_cast<uint32_t*>(V.out)[o] = uint32_t(w);
-            else reinterpret_cast<unsigned long long*>(V.out)[o] = w;
+            if (shadow) reinterpret_cast<uint32_t*>(V.out)[o] = uint32_t(tile[p]);
+            else reinterpret_cast<unsigned long long*>(V.out)[o] = tile[p];
         }
     }
 }
@@ -586,25 +618,23 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s) {
 
 hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
     if (g.n_views == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mvp, dim3((g.max_draws + 63) / 64, g.n_views), dim3(64), 0, s, g.views, g.n_views, g.draws0,
-                       g.draws1, g.objects, g.mvps);
+    const uint32_t nclear = g.n_views * kCtrPerView + g.total_bins;
+    hipLaunchKernelGGL(k_clear2, dim3(min((nclear + 255) / 256, 2048u)), dim3(256), 0, s, g.counters,
+                       g.n_views * kCtrPerView, g.bin_counts, g.total_bins);
     if (g.n_vblocks)
         hipLaunchKernelGGL(k_transform, dim3(g.n_vblocks), dim3(256), 0, s, g.vblocks, g.views, g.draws0, g.draws1,
-                           g.mvps, g.px, g.py, g.pz, g.xv);
+                           g.objects, g.px, g.py, g.pz, g.xv);
     if (g.n_tblocks)
         hipLaunchKernelGGL(k_setup, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.draws0, g.draws1,
-                           g.indices, g.xv, g.recs, g.clipq, g.counters, g.bin_counts, g.shift);
+                           g.indices, g.xv, g.recs, g.clipq, g.counters, g.bin_counts, g.block_valid, g.shift);
     hipLaunchKernelGGL(k_clip, dim3(g.clip_blocks, g.n_views), dim3(256), 0, s, g.views, g.draws0, g.draws1,
                        g.indices, g.xv, g.clipq, g.recs, g.counters, g.bin_counts, g.shift);
     return hipGetLastError();
 }
 
 hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
-    const uint32_t nb = (b.total_bins + kScanBlock - 1) / kScanBlock;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, b.bin_counts, b.total_bins, b.block_sums);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, b.block_sums, nb, b.total_pairs);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(256), 0, s, b.bin_counts, b.total_bins, b.block_sums,
-                       b.bin_offsets, b.cursor);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, b.bin_offsets, b.cursor,
+                       b.block_valid, b.n_tblocks, b.list_cap, b.totals);
     hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.recs, b.counters,
                        b.cursor, b.lists, b.list_cap, b.shift);
     return hipGetLastError();
@@ -613,16 +643,12 @@ hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
 hipError_t launch_raster(const RasterLaunch& r, hipStream_t s) {
     const uint32_t grid = ((r.total_bins + 7) / 8) * 8;
     switch (r.tile) {
-        case 8:
-            hipLaunchKernelGGL(k_raster<8>, dim3(grid), dim3(64), 0, s, r.views, r.n_views, r.total_bins,
-                               r.bin_counts, r.bin_offsets, r.lists, r.list_cap, r.recs, r.rec_cap_total);
-            break;
-        case 16:
-            hipLaunchKernelGGL(k_raster<16>, dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.total_bins,
-                               r.bin_counts, r.bin_offsets, r.lists, r.list_cap, r.recs, r.rec_cap_total);
-            break;
         case 32:
             hipLaunchKernelGGL(k_raster<32>, dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.total_bins,
+                               r.bin_counts, r.bin_offsets, r.lists, r.list_cap, r.recs, r.rec_cap_total);
+            break;
+        case 64:
+            hipLaunchKernelGGL(k_raster<64>, dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.total_bins,
                                r.bin_counts, r.bin_offsets, r.lists, r.list_cap, r.recs, r.rec_cap_total);
             break;
         default: return hipErrorInvalidValue;
