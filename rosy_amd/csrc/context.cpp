@@ -117,7 +117,7 @@ struct Layout {
     std::vector<BlockEntry> vblocks, tblocks;
     uint32_t total_bins = 0;
     uint64_t total_xv = 0;
-    uint32_t rec_total = 0, clip_total = 0;
+    uint32_t rec_total = 0, clip_total = 0, big_total = 0;
     uint32_t n_shadow = 0, n_camera = 0;
     bool valid = false;
 };
@@ -175,7 +175,7 @@ struct rsy_ctx {
 
     // frame buffers
     Layout layout;
-    DevBuf views_d, xv, recs, clipq, counters, bin_counts, bin_offsets, cursor, block_valid, totals,
+    DevBuf views_d, xv, recs, clipq, counters, bin_counts, bin_offsets, pair_slots, big_pairs, block_valid, totals,
         lists_mem, vblocks_d, tblocks_d, shade_views_d, shade_cascades_d, xv_bases_d;
     uint32_t list_cap = 0;
     uint32_t rec_scale = 1;  // grows on overflow
@@ -351,7 +351,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     L.tblocks.clear();
     L.total_bins = 0;
     L.total_xv = 0;
-    L.rec_total = L.clip_total = 0;
+    L.rec_total = L.clip_total = L.big_total = 0;
     L.n_shadow = L.n_camera = 0;
     const uint32_t shift = ilog2(tile);
     for (uint32_t vi = 0; vi < views.size(); ++vi) {
@@ -375,6 +375,8 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         g.rec_cap = tris + 2 * clip_cap;
         g.clip_base = L.clip_total;
         g.clip_cap = clip_cap;
+        g.big_base = L.big_total;
+        g.big_cap = std::max<uint32_t>(65536, tris / 4) * c->rec_scale;
         g.cull = c->rs.cull_enabled ? 1 : 0;
         g.reverse = c->rs.reverse_winding_order_enabled ? 1 : 0;
         g.bias = c->rs.depth_bias_enabled ? 1 : 0;
@@ -387,6 +389,8 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         if (uint64_t(L.rec_total) + g.rec_cap >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "record buffer too large");
         L.rec_total += g.rec_cap;
         L.clip_total += g.clip_cap;
+        if (uint64_t(L.big_total) + g.big_cap >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "bin pair buffer too large");
+        L.big_total += g.big_cap;
         for (uint32_t d = 0; d < g.n_draws; ++d) {
             for (uint32_t s = 0; s < list[d].vtx_count; s += 256) L.vblocks.push_back({vi, d, s, 0});
             for (uint32_t s = 0; s < list[d].tri_count; s += 256) L.tblocks.push_back({vi, d, s, 0});
@@ -404,7 +408,8 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     HIP_TRY(c, c->counters.ensure(nv * kCtrPerView * sizeof(uint32_t)), "allocate counters");
     HIP_TRY(c, c->bin_counts.ensure(size_t(L.total_bins) * 4), "allocate bins");
     HIP_TRY(c, c->bin_offsets.ensure(size_t(L.total_bins) * 4), "allocate bins");
-    HIP_TRY(c, c->cursor.ensure(size_t(L.total_bins) * 4), "allocate bins");
+    HIP_TRY(c, c->pair_slots.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * 16), "allocate bin slots");
+    HIP_TRY(c, c->big_pairs.ensure(size_t(std::max<uint32_t>(1, L.big_total)) * 16), "allocate bin pairs");
     HIP_TRY(c, c->block_valid.ensure(std::max<size_t>(1, L.tblocks.size()) * 4), "allocate statistics");
     HIP_TRY(c, c->totals.ensure(16), "allocate statistics");
     HIP_TRY(c, c->vblocks_d.ensure(std::max<size_t>(1, L.vblocks.size()) * sizeof(BlockEntry)), "allocate blocks");
@@ -536,6 +541,8 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     g.clipq = c->clipq.as<uint2>();
     g.counters = c->counters.as<uint32_t>();
     g.bin_counts = c->bin_counts.as<uint32_t>();
+    g.pair_slots = c->pair_slots.as<uint32_t>();
+    g.big_pairs = c->big_pairs.as<uint4>();
     g.block_valid = c->block_valid.as<uint32_t>();
     g.shift = shift;
     g.clip_blocks = 64;
@@ -549,7 +556,8 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     b.total_bins = L.total_bins;
     b.bin_counts = g.bin_counts;
     b.bin_offsets = c->bin_offsets.as<uint32_t>();
-    b.cursor = c->cursor.as<uint32_t>();
+    b.pair_slots = c->pair_slots.as<uint32_t>();
+    b.big_pairs = c->big_pairs.as<uint4>();
     b.block_valid = c->block_valid.as<uint32_t>();
     b.n_tblocks = uint32_t(L.tblocks.size());
     b.totals = c->totals.as<uint32_t>();

@@ -29,9 +29,9 @@ struct DrawGPU {
     uint32_t material;     // material index or 0xFFFFFFFF
 };
 
-// Per-view counters: records appended by the clipper, queued clipped triangles, overflow flags,
-// clipped-triangle statistic.
-enum : uint32_t { kCtrRecords = 0, kCtrClip = 1, kCtrOverflow = 3, kCtrClipped = 4, kCtrPerView = 8 };
+// Per-view counters: records appended by the clipper, queued clipped triangles, entries of the
+// big-pair list, overflow flags, clipped-triangle statistic.
+enum : uint32_t { kCtrRecords = 0, kCtrClip = 1, kCtrBig = 2, kCtrOverflow = 3, kCtrClipped = 4, kCtrPerView = 8 };
 
 struct ViewGPU {
     uint32_t width, height;
@@ -44,9 +44,10 @@ struct ViewGPU {
     uint32_t rec_base, rec_cap;  // triangle slots [rec_base, rec_base + tri_count), then clipper output
     uint32_t tri_count;
     uint32_t clip_base, clip_cap;
+    uint32_t big_base, big_cap;  // (bin, slot, record) entries of records with many bins
     uint32_t cull, reverse, bias;
     float bias_const, bias_clamp, bias_slope;
-    uint32_t pad;
+    uint32_t pad[3];
     uint64_t out;           // device address: u64 visibility (camera) or u32 depth bits (shadow)
     float view_projection[16];
 };

@@ -25,6 +25,8 @@ struct GeometryLaunch {
     uint2* clipq;
     uint32_t* counters;
     uint32_t* bin_counts;
+    uint32_t* pair_slots;   // per record slot: its slot inside each of its (at most 4) bins
+    uint4* big_pairs;       // (bin, slot, record) of records with more bins
     uint32_t* block_valid;  // records set up per setup block
     uint32_t shift;  // log2(tile)
     uint32_t clip_blocks;
@@ -37,7 +39,8 @@ struct BinLaunch {
     uint32_t total_bins;
     uint32_t* bin_counts;
     uint32_t* bin_offsets;
-    uint32_t* cursor;
+    const uint32_t* pair_slots;
+    const uint4* big_pairs;
     const uint32_t* block_valid;
     uint32_t n_tblocks;
     uint32_t* totals;  // [0] (record, bin) pairs, [1] records set up, [2] bin lists overflowed

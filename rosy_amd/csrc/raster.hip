@@ -123,61 +123,9 @@ __device__ inline void rec_tiles(const TriRec& r, uint32_t shift, int& tx0, int&
     ty1 = int(r.bby >> 16) >> shift;
 }
 
-constexpr uint32_t kNoBin = 0xffffffffu;
 constexpr uint32_t kInvalidBox = 0xffffffffu;  // bbx of a triangle slot that produced no record
 
-// Adds 1 to counts[key] for every lane whose key is not kNoBin with one atomic per distinct key in
-// the wave (lanes of one wave hold neighbouring triangles of one mesh, so keys repeat).  With kSlot,
-// returns each lane's own slot: the counter value before the wave's add plus the lane's rank among
-// the lanes sharing its key.  Must be called by a converged wave.
-template <bool kSlot>
-__device__ inline uint32_t aggregated_add(uint32_t key, uint32_t* counts) {
-    bool pending = key != kNoBin;
-    uint32_t slot = 0;
-    uint64_t todo = __ballot(pending);
-    while (todo) {
-        const int leader = __ffsll((unsigned long long)todo) - 1;
-        const uint32_t lkey = uint32_t(__builtin_amdgcn_readlane(int(key), leader));
-        const bool mine = pending && key == lkey;
-        const uint64_t mask = __ballot(mine);
-        uint32_t base = 0;
-        if (int(lane_id()) == leader) base = atomicAdd(&counts[lkey], uint32_t(__popcll(mask)));
-        if (kSlot) base = uint32_t(__builtin_amdgcn_readlane(int(base), leader));
-        if (mine) {
-            slot = base + uint32_t(__popcll(mask & lanemask_lt()));
-            pending = false;
-        }
-        todo &= ~mask;
-    }
-    return slot;
-}
-
-constexpr int kSmallBins = 4;  // records touching at most this many bins are binned per lane
-
-// Bin counting for one record per active lane.  Must be called by a converged wave.
-__device__ void count_bins(bool active, const TriRec& r, const ViewGPU& V, uint32_t shift, uint32_t* bin_counts) {
-    int tx0 = 0, ty0 = 0, tx1 = -1, ty1 = -1;
-    if (active) rec_tiles(r, shift, tx0, ty0, tx1, ty1);
-    const int bw = tx1 - tx0 + 1;
-    const int n = active ? bw * (ty1 - ty0 + 1) : 0;
-    const bool small = active && n <= kSmallBins;
-#pragma unroll
-    for (int k = 0; k < kSmallBins; ++k) {
-        const uint32_t key =
-            (small && k < n) ? V.tile_base + uint32_t((ty0 + k / bw) * int(V.tiles_x) + tx0 + k % bw) : kNoBin;
-        aggregated_add<false>(key, bin_counts);
-    }
-    uint64_t big = __ballot(active && !small);
-    while (big) {
-        const int l = __ffsll((unsigned long long)big) - 1;
-        big &= big - 1;
-        const int btx0 = __builtin_amdgcn_readlane(tx0, l), bty0 = __builtin_amdgcn_readlane(ty0, l);
-        const int btx1 = __builtin_amdgcn_readlane(tx1, l), bty1 = __builtin_amdgcn_readlane(ty1, l);
-        const int bbw = btx1 - btx0 + 1, bn = bbw * (bty1 - bty0 + 1);
-        for (int k = int(lane_id()); k < bn; k += 64)
-            atomicAdd(&bin_counts[V.tile_base + (bty0 + k / bbw) * int(V.tiles_x) + btx0 + k % bbw], 1u);
-    }
-}
+constexpr int kSmallBins = 4;  // records touching at most this many bins are binned in the block's LDS table
 
 // Edge functions of a record (inside >= 0 with ties broken by the top-left rule, y down).
 struct Edges {
@@ -281,11 +229,49 @@ __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, con
     xv[size_t(V.xv_base) + dr.xv_off + v] = transform(m, px[g], py[g], pz[g]);
 }
 
+// Bins of a record with at most kSmallBins bins go through the block's LDS table in k_setup; the
+// others (and every clipper record) are binned by the whole wave with one global atomic per bin,
+// their (bin, slot, record) triples appended to the view's big-pair list.
+__device__ void bin_wide(bool active, const TriRec& r, uint32_t ri, const ViewGPU& V, uint32_t shift,
+                         uint32_t* bin_counts, uint4* big_pairs, uint32_t* ctr) {
+    int tx0 = 0, ty0 = 0, tx1 = -1, ty1 = -1;
+    if (active) rec_tiles(r, shift, tx0, ty0, tx1, ty1);
+    uint64_t todo = __ballot(active);
+    while (todo) {
+        const int l = __ffsll((unsigned long long)todo) - 1;
+        todo &= todo - 1;
+        const int btx0 = __builtin_amdgcn_readlane(tx0, l), bty0 = __builtin_amdgcn_readlane(ty0, l);
+        const int btx1 = __builtin_amdgcn_readlane(tx1, l), bty1 = __builtin_amdgcn_readlane(ty1, l);
+        const uint32_t bri = uint32_t(__builtin_amdgcn_readlane(int(ri), l));
+        const int bw = btx1 - btx0 + 1, bn = bw * (bty1 - bty0 + 1);
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(&ctr[kCtrBig], uint32_t(bn));
+        base = uint32_t(__builtin_amdgcn_readlane(int(base), 0));
+        for (int k = int(lane_id()); k < bn; k += 64) {
+            const uint32_t key = V.tile_base + uint32_t((bty0 + k / bw) * int(V.tiles_x) + btx0 + k % bw);
+            const uint32_t local = atomicAdd(&bin_counts[key], 1u);
+            if (base + uint32_t(k) < V.big_cap) big_pairs[V.big_base + base + k] = make_uint4(key, local, bri, 0u);
+            else atomicOr(&ctr[kCtrOverflow], 4u);
+        }
+    }
+}
+
+constexpr uint32_t kHash = 2048;  // LDS bin table of one setup block (256 threads x 4 bins at most)
+constexpr uint32_t kEmpty = 0xffffffffu;
+
 __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const ViewGPU* views, const DrawGPU* draws0,
                                                const DrawGPU* draws1, const uint32_t* indices, const float4* xv,
                                                TriRec* recs, uint2* clipq, uint32_t* counters,
-                                               uint32_t* bin_counts, uint32_t* block_valid, uint32_t shift) {
+                                               uint32_t* bin_counts, uint32_t* pair_slots, uint4* big_pairs,
+                                               uint32_t* block_valid, uint32_t shift) {
+    __shared__ uint32_t hkey[kHash];
+    __shared__ uint32_t hcount[kHash];
+    __shared__ uint32_t hbase[kHash];
     __shared__ uint32_t wave_valid[4];
+    for (uint32_t i = threadIdx.x; i < kHash; i += 256) {
+        hkey[i] = kEmpty;
+        hcount[i] = 0u;
+    }
     const BlockEntry e = blocks[blockIdx.x];
     const ViewGPU& V = views[e.view];
     const DrawGPU& dr = (V.list ? draws1 : draws0)[e.draw];
@@ -311,9 +297,8 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     }
     TriRec r;
     const bool has = status == 1 && setup_tri(c, dr.tri_base + t, V, r);
+    const uint32_t slot = V.rec_base + dr.tri_base + t;  // every triangle owns one record slot
     if (valid) {
-        // every triangle owns one record slot; a slot without a record is marked by its box
-        const size_t slot = size_t(V.rec_base) + dr.tri_base + t;
         if (has) recs[slot] = r;
         else *reinterpret_cast<uint2*>(&recs[slot].bbx) = make_uint2(kInvalidBox, kInvalidBox);
     }
@@ -322,7 +307,45 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
         if (cslot < V.clip_cap) clipq[V.clip_base + cslot] = make_uint2(e.draw, t);
         else atomicOr(&ctr[kCtrOverflow], 1u);
     }
-    count_bins(has, r, V, shift, bin_counts);
+    int tx0 = 0, ty0 = 0, tx1 = -1, ty1 = -1;
+    if (has) rec_tiles(r, shift, tx0, ty0, tx1, ty1);
+    const int bw = tx1 - tx0 + 1;
+    const int n = has ? bw * (ty1 - ty0 + 1) : 0;
+    const bool small = has && n <= kSmallBins;
+    __syncthreads();  // table cleared
+    // small records: rank within the block for each bin, through the LDS table
+    uint32_t hidx[kSmallBins], rank[kSmallBins];
+#pragma unroll
+    for (int k = 0; k < kSmallBins; ++k) {
+        hidx[k] = 0u;
+        rank[k] = 0u;
+        if (small && k < n) {
+            const uint32_t key = V.tile_base + uint32_t((ty0 + k / bw) * int(V.tiles_x) + tx0 + k % bw);
+            uint32_t h = (key * 2654435761u) >> 21;  // 11-bit multiplicative hash
+            while (true) {
+                const uint32_t prev = atomicCAS(&hkey[h], kEmpty, key);
+                if (prev == kEmpty || prev == key) break;
+                h = (h + 1) & (kHash - 1);
+            }
+            hidx[k] = h;
+            rank[k] = atomicAdd(&hcount[h], 1u);
+        }
+    }
+    __syncthreads();
+    // one global atomic per distinct bin of the block: the block's first slot in that bin
+    for (uint32_t i = threadIdx.x; i < kHash; i += 256)
+        if (hkey[i] != kEmpty) hbase[i] = atomicAdd(&bin_counts[hkey[i]], hcount[i]);
+    __syncthreads();
+    if (small) {
+        uint4 out = make_uint4(0u, 0u, 0u, 0u);
+        out.x = hbase[hidx[0]] + rank[0];
+        if (n > 1) out.y = hbase[hidx[1]] + rank[1];
+        if (n > 2) out.z = hbase[hidx[2]] + rank[2];
+        if (n > 3) out.w = hbase[hidx[3]] + rank[3];
+        reinterpret_cast<uint4*>(pair_slots)[slot] = out;
+    }
+    // records with many bins: the whole wave spreads their bins
+    bin_wide(has && !small, r, slot, V, shift, bin_counts, big_pairs, ctr);
     const uint32_t nvalid = uint32_t(__popcll(__ballot(has)));
     if (lane_id() == 0) wave_valid[threadIdx.x >> 6] = nvalid;
     __syncthreads();
@@ -333,7 +356,7 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
 __global__ __launch_bounds__(256) void k_clip(const ViewGPU* views, const DrawGPU* draws0, const DrawGPU* draws1,
                                               const uint32_t* indices, const float4* xv, const uint2* clipq,
                                               TriRec* recs, uint32_t* counters, uint32_t* bin_counts,
-                                              uint32_t shift) {
+                                              uint4* big_pairs, uint32_t shift) {
     const uint32_t view = blockIdx.y;
     const ViewGPU& V = views[view];
     uint32_t* ctr = counters + view * kCtrPerView;
@@ -343,6 +366,7 @@ __global__ __launch_bounds__(256) void k_clip(const ViewGPU* views, const DrawGP
     const uint32_t stride = gridDim.x * waves_per_block;
     for (uint32_t q = wid; q < n; q += stride) {
         TriRec out[7];
+        uint32_t out_slot[7];
         int nout = 0;
         if (lane_id() == 0) {
             const uint2 ent = clipq[V.clip_base + q];
@@ -388,6 +412,7 @@ __global__ __launch_bounds__(256) void k_clip(const ViewGPU* views, const DrawGP
                         const uint32_t slot = atomicAdd(&ctr[kCtrRecords], 1u);
                         if (V.tri_count + slot < V.rec_cap) {
                             recs[size_t(V.rec_base) + V.tri_count + slot] = r;
+                            out_slot[nout] = V.rec_base + V.tri_count + slot;
                             out[nout++] = r;
                         } else {
                             atomicOr(&ctr[kCtrOverflow], 2u);
@@ -402,7 +427,8 @@ __global__ __launch_bounds__(256) void k_clip(const ViewGPU* views, const DrawGP
             TriRec r;
             r.bbx = __shfl(out[k].bbx, 0);
             r.bby = __shfl(out[k].bby, 0);
-            count_bins(lane_id() == 0, r, V, shift, bin_counts);
+            const uint32_t ri = uint32_t(__shfl(int(out_slot[k]), 0));
+            bin_wide(lane_id() == 0, r, ri, V, shift, bin_counts, big_pairs, ctr);
         }
     }
 }
@@ -414,7 +440,7 @@ constexpr uint32_t kScanThreads = 1024;
 constexpr uint32_t kScanPerThread = 4;
 
 __global__ __launch_bounds__(1024) void k_scan(const uint32_t* counts, uint32_t n, uint32_t* offsets,
-                                               uint32_t* cursor, const uint32_t* block_valid, uint32_t n_blocks,
+                                               const uint32_t* block_valid, uint32_t n_blocks,
                                                uint32_t list_cap, uint32_t* totals) {
     __shared__ uint32_t lds[16];
     uint32_t carry = 0;
@@ -433,7 +459,6 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t* counts, uint32_t 
         for (uint32_t k = 0; k < kScanPerThread; ++k)
             if (i0 + k < n) {
                 offsets[i0 + k] = run;
-                cursor[i0 + k] = run;
                 run += v[k];
             }
         carry += total;
@@ -449,71 +474,92 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t* counts, uint32_t 
     }
 }
 
-// Bin lists: every record index is appended to each bin its pixel bounds touch.  Slots come from
-// the bin cursors with one atomic per distinct bin per wave.
+// Bin lists without atomics: every (record, bin) pair already holds its slot inside the bin (from
+// k_setup's block table, or the big-pair list), so the entry goes to bin_offsets[bin] + slot.
 __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, const TriRec* recs, const uint32_t* counters,
-                                                 uint32_t* cursor, uint32_t* lists, uint32_t list_cap,
+                                                 const uint32_t* bin_offsets, const uint32_t* pair_slots,
+                                                 const uint4* big_pairs, uint32_t* lists, uint32_t list_cap,
                                                  uint32_t shift) {
     const uint32_t view = blockIdx.y;
     const ViewGPU& V = views[view];
-    const uint32_t extra = min(counters[view * kCtrPerView + kCtrRecords], V.rec_cap - V.tri_count);
-    const uint32_t n = V.tri_count + extra;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t ri = V.rec_base + i;
-        bool active = false;
-        int tx0 = 0, ty0 = 0, tx1 = -1, ty1 = -1;
-        if (i < n) {
+    const uint32_t nbig = min(counters[view * kCtrPerView + kCtrBig], V.big_cap);
+    const uint32_t n = V.tri_count + nbig;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (i < V.tri_count) {
+            const uint32_t ri = V.rec_base + i;
             const uint2 bb = *reinterpret_cast<const uint2*>(&recs[ri].bbx);
-            if (bb.x != kInvalidBox) {
-                active = true;
-                TriRec r;
-                r.bbx = bb.x;
-                r.bby = bb.y;
-                rec_tiles(r, shift, tx0, ty0, tx1, ty1);
+            if (bb.x == kInvalidBox) continue;
+            TriRec r;
+            r.bbx = bb.x;
+            r.bby = bb.y;
+            int tx0, ty0, tx1, ty1;
+            rec_tiles(r, shift, tx0, ty0, tx1, ty1);
+            const int bw = tx1 - tx0 + 1, nb = bw * (ty1 - ty0 + 1);
+            if (nb > kSmallBins) continue;  // binned through the big-pair list
+            const uint4 sl = reinterpret_cast<const uint4*>(pair_slots)[ri];
+            const uint32_t slots[4] = {sl.x, sl.y, sl.z, sl.w};
+            for (int k = 0; k < nb; ++k) {
+                const uint32_t key = V.tile_base + uint32_t((ty0 + k / bw) * int(V.tiles_x) + tx0 + k % bw);
+                const uint32_t pos = bin_offsets[key] + slots[k];
+                if (pos < list_cap) lists[pos] = ri;
             }
-        }
-        const int bw = tx1 - tx0 + 1;
-        const int nb = active ? bw * (ty1 - ty0 + 1) : 0;
-        const bool small = active && nb <= kSmallBins;
-#pragma unroll
-        for (int k = 0; k < kSmallBins; ++k) {
-            const uint32_t key =
-                (small && k < nb) ? V.tile_base + uint32_t((ty0 + k / bw) * int(V.tiles_x) + tx0 + k % bw) : kNoBin;
-            const uint32_t slot = aggregated_add<true>(key, cursor);
-            if (key != kNoBin && slot < list_cap) lists[slot] = ri;
-        }
-        uint64_t big = __ballot(active && !small);
-        while (big) {
-            const int l = __ffsll((unsigned long long)big) - 1;
-            big &= big - 1;
-            const int btx0 = __builtin_amdgcn_readlane(tx0, l), bty0 = __builtin_amdgcn_readlane(ty0, l);
-            const int btx1 = __builtin_amdgcn_readlane(tx1, l), bty1 = __builtin_amdgcn_readlane(ty1, l);
-            const uint32_t bri = uint32_t(__builtin_amdgcn_readlane(int(ri), l));
-            const int bbw = btx1 - btx0 + 1, bn = bbw * (bty1 - bty0 + 1);
-            for (int k = int(lane_id()); k < bn; k += 64) {
-                const uint32_t slot = atomicAdd(&cursor[V.tile_base + (bty0 + k / bbw) * int(V.tiles_x) + btx0 + k % bbw], 1u);
-                if (slot < list_cap) lists[slot] = bri;
-            }
+        } else {
+            const uint4 bp = big_pairs[V.big_base + (i - V.tri_count)];
+            const uint32_t pos = bin_offsets[bp.x] + bp.y;
+            if (pos < list_cap) lists[pos] = bp.z;
         }
     }
 }
 
-// Bin raster: one workgroup per bin of T x T pixels, the bin's visibility (or depth) tile in LDS.
-// The bin list is taken 256 records at a time; each record's box is clipped to the bin and its rows
-// become tasks: a prefix sum over the rows numbers them, and every thread takes tasks
-// (record, row) in turn, so small and large triangles spread evenly over the workgroup.  Along a
-// row the three int64 edge functions advance by 256 * A per pixel (exact), the depth plane is
-// evaluated per pixel by the same FP32 rule as setup, and fragments merge with LDS atomic max
-// (depth bits above triangle id + 1: nearer depth wins under reverse-Z, the later triangle wins
-// ties).  Workgroups are mapped so that neighbouring bins run on the same XCD (workgroups are dealt
-// round-robin over the 8 XCDs, each with its own L2).
+// One row of a record inside the bin: E_k(x) for the three edges advances by 256 * A_k per pixel
+// (exact integer steps; EdgeT = int32_t when the triangle's box is under 64 pixels on each side, so
+// every |E| < 2^30, else int64_t), depth is the FP32 plane rule of setup, and covered pixels merge
+// into the LDS row with atomic max.
+template <typename EdgeT, int STRIDE>
+__device__ inline void raster_row(const TriRec& r, const Edges& ed, int ox, int oy, int y, int bx0, int bx1,
+                                  bool shadow, unsigned long long* tile) {
+    const int32_t Py = (oy + y) * 256 + 128;
+    int32_t Px = (ox + bx0) * 256 + 128;
+    EdgeT e[3], step[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        e[k] = EdgeT(ed.A[k]) * EdgeT(Px - ed.X[k]) + EdgeT(ed.B[k]) * EdgeT(Py - ed.Y[k]);
+        step[k] = EdgeT(ed.A[k]) * EdgeT(256);
+    }
+    const float fy = float(Py - r.Y0);
+    const unsigned long long idw = shadow ? 0ull : (unsigned long long)(r.id + 1u);
+    const int sh = shadow ? 0 : 32;
+    unsigned long long* row = tile + y * STRIDE;
+    for (int x = bx0; x <= bx1; ++x, Px += 256) {
+        bool inside = true;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) inside &= e[k] > 0 || (e[k] == 0 && ed.tl[k]);
+        if (inside) {
+            float z = fmaf(r.dzdy, fy, fmaf(r.dzdx, float(Px - r.X0), r.z0));
+            z = fminf(fmaxf(z, 0.0f), 1.0f);
+            const uint32_t zb = fbits(z) & 0x7fffffffu;
+            atomicMax(&row[x], ((unsigned long long)zb << sh) | idw);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) e[k] += step[k];
+    }
+}
+
+// Bin raster: one workgroup per bin of T x T pixels, the bin's visibility (or depth) tile in LDS
+// (rows padded to T + 1 words so that lanes on the same column of different rows fall in different
+// banks).  The bin list is taken 256 records at a time; each record's box is clipped to the bin and
+// its rows become tasks: a prefix sum over the rows numbers them, and every thread takes tasks
+// (record, row) in turn, so small and large triangles spread evenly over the workgroup.  Fragments
+// merge with LDS atomic max (depth bits above triangle id + 1: nearer depth wins under reverse-Z,
+// the later triangle wins ties).  Workgroups are mapped so that neighbouring bins run on the same
+// XCD (workgroups are dealt round-robin over the 8 XCDs, each with its own L2).
 template <int T>
 __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n_views, uint32_t total_bins,
                                                 const uint32_t* bin_counts, const uint32_t* bin_offsets,
                                                 const uint32_t* lists, uint32_t list_cap, const TriRec* recs,
                                                 uint64_t rec_cap_total) {
-    __shared__ unsigned long long tile[T * T];
+    constexpr int S = T + 1;  // padded row stride (64-bit words)
+    __shared__ unsigned long long tile[T * S];
     __shared__ TriRec chunk_recs[256];
     __shared__ uint32_t chunk_box[256];   // clipped box: x0 | x1 << 8 | y0 << 16 | y1 << 24 (bin-local)
     __shared__ uint32_t row_start[257];   // exclusive prefix of row counts
@@ -529,7 +575,7 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
     const int ox = int(local_bin % V.tiles_x) * T, oy = int(local_bin / V.tiles_x) * T;
     const bool shadow = V.is_shadow != 0;
 
-    for (int p = threadIdx.x; p < T * T; p += 256) tile[p] = 0ull;
+    for (int p = threadIdx.x; p < T * S; p += 256) tile[p] = 0ull;
 
     const uint32_t off = bin_offsets[bin];
     const uint32_t end = min(off + bin_counts[bin], list_cap);
@@ -556,42 +602,26 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
         if (threadIdx.x == 255) row_start[256] = first + rows;
         __syncthreads();
         for (uint32_t task = threadIdx.x; task < total_rows; task += 256) {
-            // record j: the last record whose first row is <= task (records without rows share
-            // their successor's start and are skipped by the upper-bound search)
+            // the record whose rows contain `task`: the last record whose first row is <= task
             uint32_t lo = 0, hi = 256;
-            while (hi - lo > 1) {
+#pragma unroll
+            for (int it = 0; it < 8; ++it) {
                 const uint32_t mid = (lo + hi) >> 1;
                 if (row_start[mid] <= task) lo = mid; else hi = mid;
             }
-            const uint32_t j = lo;
-            const TriRec r = chunk_recs[j];
-            const uint32_t box = chunk_box[j];
+            const TriRec r = chunk_recs[lo];
+            const uint32_t box = chunk_box[lo];
             const int bx0 = int(box & 0xffu), bx1 = int((box >> 8) & 0xffu);
-            const int y = int((box >> 16) & 0xffu) + int(task - row_start[j]);
+            const int y = int((box >> 16) & 0xffu) + int(task - row_start[lo]);
             Edges ed;
             make_edges(r, ed);
-            const int32_t Py = (oy + y) * 256 + 128;
-            int32_t Px = (ox + bx0) * 256 + 128;
-            int64_t e[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) e[k] = int64_t(ed.A[k]) * (Px - ed.X[k]) + int64_t(ed.B[k]) * (Py - ed.Y[k]);
-            const float fy = float(Py - r.Y0);
-            unsigned long long* row = tile + y * T;
-            for (int x = bx0; x <= bx1; ++x, Px += 256) {
-                bool inside = true;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) inside &= e[k] > 0 || (e[k] == 0 && ed.tl[k]);
-                if (inside) {
-                    float z = fmaf(r.dzdy, fy, fmaf(r.dzdx, float(Px - r.X0), r.z0));
-                    z = fminf(fmaxf(z, 0.0f), 1.0f);
-                    const uint32_t zb = fbits(z) & 0x7fffffffu;
-                    const unsigned long long w =
-                        shadow ? (unsigned long long)zb : ((unsigned long long)zb << 32) | (unsigned long long)(r.id + 1u);
-                    atomicMax(&row[x], w);
-                }
-#pragma unroll
-                for (int k = 0; k < 3; ++k) e[k] += int64_t(ed.A[k]) * 256;
-            }
+            // the whole triangle's box (sub-pixel units) bounds every |E| over it
+            const int32_t wx = max(ed.X[0], max(ed.X[1], ed.X[2])) - min(ed.X[0], min(ed.X[1], ed.X[2]));
+            const int32_t wy = max(ed.Y[0], max(ed.Y[1], ed.Y[2])) - min(ed.Y[0], min(ed.Y[1], ed.Y[2]));
+            if (wx < 64 * 256 && wy < 64 * 256)
+                raster_row<int32_t, S>(r, ed, ox, oy, y, bx0, bx1, shadow, tile);
+            else
+                raster_row<int64_t, S>(r, ed, ox, oy, y, bx0, bx1, shadow, tile);
         }
         __syncthreads();
     }
@@ -600,8 +630,9 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
         const int x = ox + p % T, y = oy + p / T;
         if (x < int(V.width) && y < int(V.height)) {
             const size_t o = size_t(y) * V.width + x;
-            if (shadow) reinterpret_cast<uint32_t*>(V.out)[o] = uint32_t(tile[p]);
-            else reinterpret_cast<unsigned long long*>(V.out)[o] = tile[p];
+            const unsigned long long w = tile[(p / T) * S + p % T];
+            if (shadow) reinterpret_cast<uint32_t*>(V.out)[o] = uint32_t(w);
+            else reinterpret_cast<unsigned long long*>(V.out)[o] = w;
         }
     }
 }
@@ -626,17 +657,18 @@ hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
                            g.objects, g.px, g.py, g.pz, g.xv);
     if (g.n_tblocks)
         hipLaunchKernelGGL(k_setup, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.draws0, g.draws1,
-                           g.indices, g.xv, g.recs, g.clipq, g.counters, g.bin_counts, g.block_valid, g.shift);
+                           g.indices, g.xv, g.recs, g.clipq, g.counters, g.bin_counts, g.pair_slots, g.big_pairs,
+                           g.block_valid, g.shift);
     hipLaunchKernelGGL(k_clip, dim3(g.clip_blocks, g.n_views), dim3(256), 0, s, g.views, g.draws0, g.draws1,
-                       g.indices, g.xv, g.clipq, g.recs, g.counters, g.bin_counts, g.shift);
+                       g.indices, g.xv, g.clipq, g.recs, g.counters, g.bin_counts, g.big_pairs, g.shift);
     return hipGetLastError();
 }
 
 hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, b.bin_offsets, b.cursor,
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, b.bin_offsets,
                        b.block_valid, b.n_tblocks, b.list_cap, b.totals);
     hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.recs, b.counters,
-                       b.cursor, b.lists, b.list_cap, b.shift);
+                       b.bin_offsets, b.pair_slots, b.big_pairs, b.lists, b.list_cap, b.shift);
     return hipGetLastError();
 }
 

@@ -123,68 +123,95 @@ __device__ inline float cmp_tap(const float* map, int S, float u, float v, float
            (1 - ax) * ay * (ref >= t01 ? 1.f : 0.f) + ax * ay * (ref >= t11 ? 1.f : 0.f);
 }
 
-struct VOut {
+// Varyings of basicVertex (data.slang:10-22) plus csm.near, interpolated at the pixel.
+struct Varyings {
     f3 normal, world_normal, view_dir, light_dir, tangent, bitangent;
     float4 color;
     float u, v, sigma;
+    float su, sv;  // whole periods removed from u, v (see below)
     float4 csm;
 };
 
-__device__ VOut vertex_main(const rsy_scene_data& sd, const rsy_position& vin, const float* gd,
-                            bool has_material, uint32_t normal_index, const float* cascade_m) {
-    VOut o;
-    float4 nt = make_float4(vin.tangents[0], vin.tangents[1], vin.tangents[2], vin.tangents[3]);
-    if (sd.ignore_asset_tangent_sign == 1) nt.w = 1.f;
-    if (sd.flip_tangents[0] == 1) nt.x = -nt.x;
-    if (sd.flip_tangents[1] == 1) nt.y = -nt.y;
-    if (sd.flip_tangents[2] == 1) nt.z = -nt.z;
-    if (sd.flip_tangents[3] == 1) nt.w = -nt.w;
+// vertexMain (basic.slang:15-119) evaluated at the pixel.  Every output is affine in the vertex
+// inputs except the bitangent (a cross product per vertex) and, with tangent space off, the
+// normalised world normal, so those two are formed per vertex and interpolated, and the rest is
+// formed once from the interpolated position, normal, tangent, colour and uv: the same values as
+// running vertexMain per vertex and interpolating its outputs.
+__device__ Varyings pixel_varyings(const rsy_scene_data& sd, const rsy_position* const v[3], const float* gd,
+                                   bool has_material, uint32_t normal_index, const float* cascade_m, float w0,
+                                   float w1, float w2, bool shift_uv) {
+    Varyings o;
+    const float w[3] = {w0, w1, w2};
     bool tangent_space = sd.tangent_space_enabled == 1;
     if (has_material && !valid_index(normal_index)) tangent_space = false;
-    const f3 vpos = mk(vin.vertex[0], vin.vertex[1], vin.vertex[2]);
-    const f3 vnorm = mk(vin.normal[0], vin.normal[1], vin.normal[2]);
-    f3 normal = vnorm, tangent = mk(nt.x, nt.y, nt.z);
-    f3 bitangent = cross(normal, tangent) * nt.w;
-    if (sd.ensure_orthogonal_bitangent == 1) {
-        const f3 np = bitangent * dot(normal, bitangent);
-        const f3 on = normalize(normal - np);
-        const f3 tp = np - bitangent * dot(tangent, bitangent);
-        const f3 ot = normalize(tangent - tp);
-        bitangent = cross(on, ot) * nt.w;
-    }
     const float* W = gd;
     const float* toObj = gd + 16;
     const float* nmat = gd + 32;
-    const float4 world = mul4(W, make_float4(vpos.x, vpos.y, vpos.z, 1.f));
+    f3 P = mk(0.f, 0.f, 0.f), N = P, Tn = P, B = P, Nw = P;
+    float sigma = 0.f;
+    float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const rsy_position& vin = *v[k];
+        float4 nt = make_float4(vin.tangents[0], vin.tangents[1], vin.tangents[2], vin.tangents[3]);
+        if (sd.ignore_asset_tangent_sign == 1) nt.w = 1.f;
+        if (sd.flip_tangents[0] == 1) nt.x = -nt.x;
+        if (sd.flip_tangents[1] == 1) nt.y = -nt.y;
+        if (sd.flip_tangents[2] == 1) nt.z = -nt.z;
+        if (sd.flip_tangents[3] == 1) nt.w = -nt.w;
+        const f3 n = mk(vin.normal[0], vin.normal[1], vin.normal[2]);
+        const f3 t = mk(nt.x, nt.y, nt.z);
+        f3 bt = cross(n, t) * nt.w;
+        if (sd.ensure_orthogonal_bitangent == 1) {
+            const f3 np = bt * dot(n, bt);
+            const f3 on = normalize(n - np);
+            const f3 tp = np - bt * dot(t, bt);
+            const f3 ot = normalize(t - tp);
+            bt = cross(on, ot) * nt.w;
+        }
+        P = P + mk(vin.vertex[0], vin.vertex[1], vin.vertex[2]) * w[k];
+        N = N + n * w[k];
+        Tn = Tn + t * w[k];
+        B = B + bt * w[k];
+        sigma += nt.w * w[k];
+        col.x += vin.color[0] * w[k];
+        col.y += vin.color[1] * w[k];
+        col.z += vin.color[2] * w[k];
+        col.w += vin.color[3] * w[k];
+        if (!tangent_space) Nw = Nw + normalize(mul3(nmat, n)) * w[k];
+    }
+    const float4 world = mul4(W, make_float4(P.x, P.y, P.z, 1.f));
     const float4 cam = make_float4(sd.camera_position[0], sd.camera_position[1], sd.camera_position[2], sd.camera_position[3]);
     const float4 sun = make_float4(sd.sun_position[0], sd.sun_position[1], sd.sun_position[2], sd.sun_position[3]);
-    f3 view_dir = xyz(cam) - xyz(world);
-    f3 light_dir = xyz(sun) - xyz(world);
     if (tangent_space) {
-        view_dir = xyz(mul4(toObj, cam)) - vpos;
-        light_dir = xyz(mul4(toObj, sun)) - vpos;
+        o.normal = N;
+        o.view_dir = xyz(mul4(toObj, cam)) - P;
+        o.light_dir = xyz(mul4(toObj, sun)) - P;
     } else {
-        normal = normalize(mul3(nmat, vnorm));
-        tangent = normalize(mul3(nmat, mk(nt.x, nt.y, nt.z)));
-        tangent = normalize(tangent - normal * dot(normal, tangent));
+        o.normal = Nw;
+        o.view_dir = xyz(cam) - xyz(world);
+        o.light_dir = xyz(sun) - xyz(world);
     }
-    if (sd.flip_lights[0] == 1) light_dir.x = -light_dir.x;
-    if (sd.flip_lights[1] == 1) light_dir.y = -light_dir.y;
-    if (sd.flip_lights[2] == 1) light_dir.z = -light_dir.z;
-    o.normal = normal;
-    o.world_normal = mul3(nmat, vnorm);
-    o.color = make_float4(vin.color[0], vin.color[1], vin.color[2], vin.color[3]);
-    o.tangent = mk(nt.x, nt.y, nt.z);
-    o.bitangent = bitangent;
-    o.view_dir = view_dir;
-    o.light_dir = light_dir;
-    o.sigma = nt.w;
-    o.u = vin.texture_coordinates[0];
-    o.v = vin.texture_coordinates[1];
+    if (sd.flip_lights[0] == 1) o.light_dir.x = -o.light_dir.x;
+    if (sd.flip_lights[1] == 1) o.light_dir.y = -o.light_dir.y;
+    if (sd.flip_lights[2] == 1) o.light_dir.z = -o.light_dir.z;
+    o.world_normal = mul3(nmat, N);
+    o.tangent = Tn;
+    o.bitangent = B;
+    o.color = col;
+    o.sigma = sigma;
+    // Texture coordinates relative to vertex 0 with a whole even number of periods removed: exact
+    // for REPEAT and MIRRORED_REPEAT and keeps |uv| small, so the FP32 texel address keeps its
+    // sub-texel precision at large uv.  Clamp-to-edge samplers keep the raw value.
+    const float u0 = v[0]->texture_coordinates[0], v0c = v[0]->texture_coordinates[1];
+    o.su = shift_uv ? 2.0f * floorf(u0 * 0.5f) : 0.f;
+    o.sv = shift_uv ? 2.0f * floorf(v0c * 0.5f) : 0.f;
+    o.u = (u0 - o.su) + w1 * (v[1]->texture_coordinates[0] - u0) + w2 * (v[2]->texture_coordinates[0] - u0);
+    o.v = (v0c - o.sv) + w1 * (v[1]->texture_coordinates[1] - v0c) + w2 * (v[2]->texture_coordinates[1] - v0c);
     o.csm = make_float4(0.f, 0.f, 0.f, 0.f);
     if (cascade_m) {
-        const float4 s = mul4(cascade_m, world);
-        o.csm = make_float4(0.5f * s.x + 0.5f * s.w, 0.5f * s.y + 0.5f * s.w, s.z, s.w);
+        const float4 s4 = mul4(cascade_m, world);
+        o.csm = make_float4(0.5f * s4.x + 0.5f * s4.w, 0.5f * s4.y + 0.5f * s4.w, s4.z, s4.w);  // ndc_to_tc
     }
     return o;
 }
@@ -197,12 +224,6 @@ __device__ inline bool uses_clamp(const rsy_gpu_material* md, const rsy_sampler*
     for (int k = 0; k < 4; ++k)
         if (idx[k] < n_samplers) clamp |= samplers[idx[k]].wrap_s == 0 || samplers[idx[k]].wrap_t == 0;
     return clamp;
-}
-
-__device__ inline f3 lerp3(f3 a, f3 b, f3 c, float w0, float w1, float w2) { return a * w0 + b * w1 + c * w2; }
-__device__ inline float4 lerp4(float4 a, float4 b, float4 c, float w0, float w1, float w2) {
-    return make_float4(a.x * w0 + b.x * w1 + c.x * w2, a.y * w0 + b.y * w1 + c.y * w2, a.z * w0 + b.z * w1 + c.z * w2,
-                       a.w * w0 + b.w * w1 + c.w * w2);
 }
 
 __device__ inline void mr_from(int channels, float4 s, float& rough, float& metal) {
@@ -306,29 +327,14 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
         }
     const float* cm = cascade >= 0 ? L.cascades[cascade].m : nullptr;
     const uint32_t nti = has_material ? md->normal_sampled_image_index : RSY_ABSENT;
-    const VOut v0 = vertex_main(sd, verts[dr.vtx_base + i0], gd, has_material, nti, cm);
-    const VOut v1 = vertex_main(sd, verts[dr.vtx_base + i1], gd, has_material, nti, cm);
-    const VOut v2 = vertex_main(sd, verts[dr.vtx_base + i2], gd, has_material, nti, cm);
-    const float w0 = lam.x, w1 = lam.y, w2 = lam.z;
-    const f3 bn = lerp3(v0.normal, v1.normal, v2.normal, w0, w1, w2);
-    const f3 bwn = lerp3(v0.world_normal, v1.world_normal, v2.world_normal, w0, w1, w2);
-    const f3 bview = lerp3(v0.view_dir, v1.view_dir, v2.view_dir, w0, w1, w2);
-    const f3 blight = lerp3(v0.light_dir, v1.light_dir, v2.light_dir, w0, w1, w2);
-    const f3 btan = lerp3(v0.tangent, v1.tangent, v2.tangent, w0, w1, w2);
-    const f3 bbit = lerp3(v0.bitangent, v1.bitangent, v2.bitangent, w0, w1, w2);
-    const float4 bcol = lerp4(v0.color, v1.color, v2.color, w0, w1, w2);
-    // Texture coordinates relative to vertex 0 with a whole even number of periods removed: exact
-    // for REPEAT and MIRRORED_REPEAT and keeps |uv| small, so the FP32 texel address keeps its
-    // sub-texel precision at large uv.  Clamp-to-edge samplers keep the raw value.
-    float su = 0.f, sv = 0.f;
-    if (has_material && !uses_clamp(md, samplers, L.n_samplers)) {
-        su = 2.0f * floorf(v0.u * 0.5f);
-        sv = 2.0f * floorf(v0.v * 0.5f);
-    }
-    const float tu = (v0.u - su) + w1 * (v1.u - v0.u) + w2 * (v2.u - v0.u);
-    const float tv = (v0.v - sv) + w1 * (v1.v - v0.v) + w2 * (v2.v - v0.v);
-    const float sigma = v0.sigma * w0 + v1.sigma * w1 + v2.sigma * w2;
-    const float4 csm = lerp4(v0.csm, v1.csm, v2.csm, w0, w1, w2);
+    const rsy_position* const vtx[3] = {&verts[dr.vtx_base + i0], &verts[dr.vtx_base + i1], &verts[dr.vtx_base + i2]};
+    const bool shift_uv = has_material && !uses_clamp(md, samplers, L.n_samplers);
+    const Varyings bv = pixel_varyings(sd, vtx, gd, has_material, nti, cm, lam.x, lam.y, lam.z, shift_uv);
+    const f3 bn = bv.normal, bwn = bv.world_normal, bview = bv.view_dir, blight = bv.light_dir;
+    const f3 btan = bv.tangent, bbit = bv.bitangent;
+    const float4 bcol = bv.color;
+    const float tu = bv.u, tv = bv.v, su = bv.su, sv = bv.sv, sigma = bv.sigma;
+    const float4 csm = bv.csm;
 
     const f3 ambient = mk(sd.ambient_color[0], sd.ambient_color[1], sd.ambient_color[2]);
     float4 out = bcol;
