@@ -318,6 +318,79 @@ rsy_result rsy_set_debug_flags(rsy_ctx* ctx, uint32_t flags);
 rsy_result rsy_autotune(rsy_ctx* ctx, const rsy_frame_desc* frame, uint32_t n_iters, uint32_t* chosen_tile);
 
 /* ---------------------------------------------------------------------------------------------
+ * Neural material decoder (no counterpart in the reference: an alternative to sampling the three
+ * material maps of basic.slang:172-221, judged against the plain sampler).
+ *
+ * Each eligible material gets an 8-channel latent grid of latent_size^2 cells; one MLP shared by
+ * all materials maps [bilinear latent (8), 1-|2ax-1|, 1-|2ay-1|] -> 32 ReLU -> 7 outputs clamped to
+ * [0,1]: linear albedo rgb, normal-map r, g, metallic, roughness -- the plain sampler's values for
+ * those maps.  Eligible: RGBA8 opaque albedo, two-channel normal and metallic-roughness maps,
+ * bilinear REPEAT samplers.  Training (Adam on the GPU) regresses the decoder onto plain-sampler
+ * values at random points; evaluation decodes every albedo texel centre, computes the PSNR over
+ * the 7 channels (peak 1) and activates the materials at or above psnr_threshold.  Shade mode
+ * RSY_SHADE_NEURAL decodes active materials (FP16 latents, FP16 MFMA with FP32 accumulation) and
+ * samples the plain maps for every other material.
+ * ------------------------------------------------------------------------------------------- */
+typedef enum rsy_neural_init_mode {
+    RSY_NEURAL_INIT_FROM_MAPS = 0, /* latent channels 0..6 = plain values at cell centres, MLP = identity */
+    RSY_NEURAL_INIT_RANDOM = 1     /* uniform latents in [-0.1, 0.1], random MLP */
+} rsy_neural_init_mode;
+
+typedef struct rsy_neural_config {
+    uint32_t latent_size;    /* cells per side: a power of two in [4, 4096] */
+    uint32_t batch;          /* samples per step, a multiple of 256 */
+    uint32_t init;           /* rsy_neural_init_mode */
+    uint32_t region;         /* training region side in cells: a power of two in [2, 32], <= latent_size */
+    float lr_mlp;            /* Adam learning rates */
+    float lr_latent;
+    float beta1, beta2, eps;
+    float psnr_threshold;    /* dB */
+    uint64_t seed;
+} rsy_neural_config;
+
+typedef struct rsy_neural_stats {
+    uint32_t n_materials;    /* materials of the context */
+    uint32_t n_eligible;     /* materials with a latent grid */
+    uint32_t n_active;       /* materials decoded when rendering (PSNR >= threshold) */
+    uint32_t steps;          /* Adam steps run so far */
+    float train_ms;          /* device time of the last rsy_neural_train call */
+    float eval_ms;           /* device time of the last rsy_neural_evaluate call */
+    float last_loss;         /* mean squared error of the last step */
+    float reserved;
+} rsy_neural_stats;
+
+/* Fills the defaults: latent 256, batch 65536, init from maps, region 16, lr 1e-3 / 1e-3,
+ * betas 0.9 / 0.999, eps 1e-8, threshold 40 dB, seed 1. */
+void rsy_neural_default_config(rsy_neural_config* cfg);
+/* Allocates and initialises the decoder for the context's current materials (rsy_set_materials
+ * first).  Re-initialising discards any trained state. */
+rsy_result rsy_neural_init(rsy_ctx* ctx, const rsy_neural_config* cfg);
+/* Runs n_steps Adam steps on the GPU.  loss_history (nullable) receives each step's mean squared
+ * error.  Invalidates the last evaluation. */
+rsy_result rsy_neural_train(rsy_ctx* ctx, uint32_t n_steps, float* loss_history);
+/* Exports the FP16 inference parameters, measures every eligible material's PSNR and sets the
+ * active set.  psnr (nullable, n_materials entries) receives the PSNR, NaN for ineligible ones. */
+rsy_result rsy_neural_evaluate(rsy_ctx* ctx, float* psnr, uint32_t n_materials);
+rsy_result rsy_neural_get_stats(rsy_ctx* ctx, rsy_neural_stats* out);
+/* Overrides the active set after an evaluation (active[m] = 0 forces the plain maps; 1 is only
+ * accepted for eligible materials). */
+rsy_result rsy_neural_set_active(rsy_ctx* ctx, const uint32_t* active, uint32_t n_materials);
+/* FP32 master parameters (checkpoint / restore): latents [eligible][L][L][8] in material order,
+ * MLP [583] = W1 [32][10], b1 [32], W2 [7][32], b2 [7]. */
+rsy_result rsy_neural_get_params(rsy_ctx* ctx, float* latents, uint64_t n_latents, float* mlp, uint32_t n_mlp);
+rsy_result rsy_neural_set_params(rsy_ctx* ctx, const float* latents, uint64_t n_latents, const float* mlp,
+                                 uint32_t n_mlp);
+/* Exact loss and gradients of the training objective on a given batch (the training kernels run
+ * over the whole grid with the caller's samples; no update): material[n], uv[n][2], target[n][7].
+ * Requires latent_size <= 32.  grad_mlp [583], grad_latents as in rsy_neural_get_params. */
+rsy_result rsy_neural_gradients(rsy_ctx* ctx, const uint32_t* material, const float* uv, const float* target,
+                                uint32_t n, float* loss, float* grad_mlp, float* grad_latents, uint64_t n_latents);
+/* Decoder outputs (inference path, after rsy_neural_evaluate) or, with targets != 0, the plain
+ * sampler's values that training regresses onto, at material[n], uv[n][2] -> out[n][7]. */
+rsy_result rsy_neural_decode(rsy_ctx* ctx, const uint32_t* material, const float* uv, uint32_t n, uint32_t targets,
+                             float* out);
+
+/* ---------------------------------------------------------------------------------------------
  * Input producers: closed-form restatements of the reference's camera, light and node math.
  * All matrices are column-major float[16] (glm value_ptr order).
  * ------------------------------------------------------------------------------------------- */

@@ -36,6 +36,11 @@ class ShadeIn(C.Structure):
                 ("view", C.POINTER(abi.CameraView))]
 
 
+class Neural(C.Structure):
+    _fields_ = [("latent_size", C.c_uint32), ("n_materials", C.c_uint32), ("slot_of", C.c_void_p),
+                ("active", C.c_void_p), ("latents", C.c_void_p), ("mlp", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -55,6 +60,17 @@ def lib():
         L.orc_shade_view.restype = C.c_int
         L.orc_shade_view.argtypes = [C.POINTER(Geometry), C.POINTER(ShadeIn), C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_shade_view_neural.restype = C.c_int
+        L.orc_shade_view_neural.argtypes = [C.POINTER(Geometry), C.POINTER(ShadeIn), C.POINTER(Neural), C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_nm_decode.restype = C.c_int
+        L.orc_nm_decode.argtypes = [C.POINTER(Neural), C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
+        L.orc_nm_gradients.restype = C.c_int
+        L.orc_nm_gradients.argtypes = [C.POINTER(Neural), C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                       C.POINTER(C.c_double), C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_sample_texture.restype = C.c_int
+        L.orc_sample_texture.argtypes = [C.POINTER(abi.TextureDesc), C.POINTER(abi.Sampler), C.c_double, C.c_double,
+                                         C.POINTER(C.c_double)]
         L.orc_srgb_to_linear.restype = C.c_double
         L.orc_srgb_to_linear.argtypes = [C.c_uint32]
         for name in ("orc_ggx_d", "orc_schlick_ggx", "orc_schlick_fresnel"):
@@ -114,9 +130,59 @@ def tile_pairs(scene, view_projection, width, height, shadow, raster_state, tile
                                           tile, n_threads or threads()))
 
 
+class NeuralParams:
+    """Decoder parameters for the oracle: slot_of / active per material, latents [slots, L, L, 8] and
+    mlp [583] as the float values to evaluate (e.g. FP16-rounded for the inference path)."""
+
+    def __init__(self, slot_of, latents, mlp, active=None):
+        self.slot_of = np.ascontiguousarray(slot_of, dtype=np.uint32)
+        self.latents = np.ascontiguousarray(latents, dtype=np.float32)
+        self.mlp = np.ascontiguousarray(mlp, dtype=np.float32)
+        self.active = None if active is None else np.ascontiguousarray(active, dtype=np.uint32)
+        self.struct = Neural(self.latents.shape[1], self.slot_of.size, self.slot_of.ctypes.data,
+                             self.active.ctypes.data if self.active is not None else None,
+                             self.latents.ctypes.data, self.mlp.ctypes.data)
+
+
+def nm_decode(params, material, uv):
+    m = np.ascontiguousarray(material, dtype=np.uint32)
+    uv = np.ascontiguousarray(uv, dtype=np.float32)
+    out = np.zeros((m.size, 7), dtype=np.float64)
+    rc = lib().orc_nm_decode(C.byref(params.struct), m.ctypes.data, uv.ctypes.data, m.size, out.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"orc_nm_decode failed: {rc}")
+    return out
+
+
+def nm_gradients(params, material, uv, target):
+    """(loss, grad_mlp [583], grad_latents like params.latents, min |pre-activation| per sample)."""
+    m = np.ascontiguousarray(material, dtype=np.uint32)
+    uv = np.ascontiguousarray(uv, dtype=np.float32)
+    t = np.ascontiguousarray(target, dtype=np.float32)
+    g_mlp = np.zeros(583, dtype=np.float64)
+    g_lat = np.zeros(params.latents.shape, dtype=np.float64)
+    margin = np.zeros(m.size, dtype=np.float64)
+    loss = C.c_double()
+    rc = lib().orc_nm_gradients(C.byref(params.struct), params.latents.shape[0], m.ctypes.data, uv.ctypes.data,
+                                t.ctypes.data, m.size, C.byref(loss), g_mlp.ctypes.data, g_lat.ctypes.data,
+                                margin.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"orc_nm_gradients failed: {rc}")
+    return loss.value, g_mlp, g_lat, margin
+
+
+def sample_texture(desc, sampler, u, v):
+    out = (C.c_double * 4)()
+    rc = lib().orc_sample_texture(C.byref(desc), C.byref(sampler) if sampler is not None else None, u, v, out)
+    if rc != 0:
+        raise RuntimeError(f"orc_sample_texture failed: {rc}")
+    return np.array(out[:])
+
+
 def shade_view(scene, view, cascades, shadow_maps, vis, scene_data=None, raster_state=None, override_shadow=None,
-               n_threads=None):
-    """FP64 shading of one camera view.  Returns (rgba float64 [h,w,4], shadow factor [h,w], margin [h,w])."""
+               n_threads=None, neural=None):
+    """FP64 shading of one camera view.  Returns (rgba float64 [h,w,4], shadow factor [h,w], margin [h,w]).
+    With `neural` (NeuralParams) the maps of the active materials are replaced by the decoder."""
     s = scene.s
     g = geometry(scene, False)
     maps = [np.ascontiguousarray(m, dtype=np.float32) for m in shadow_maps]
@@ -137,9 +203,14 @@ def shade_view(scene, view, cascades, shadow_maps, vis, scene_data=None, raster_
     ov = None
     if override_shadow is not None:
         ov = np.ascontiguousarray(override_shadow, dtype=np.float32)
-    rc = lib().orc_shade_view(C.byref(g), C.byref(si), vis.ctypes.data, rgba.ctypes.data, shadow.ctypes.data,
-                              margin.ctypes.data, ov.ctypes.data if ov is not None else None,
-                              n_threads or threads())
+    if neural is not None:
+        rc = lib().orc_shade_view_neural(C.byref(g), C.byref(si), C.byref(neural.struct), vis.ctypes.data,
+                                         rgba.ctypes.data, shadow.ctypes.data, margin.ctypes.data,
+                                         ov.ctypes.data if ov is not None else None, n_threads or threads())
+    else:
+        rc = lib().orc_shade_view(C.byref(g), C.byref(si), vis.ctypes.data, rgba.ctypes.data, shadow.ctypes.data,
+                                  margin.ctypes.data, ov.ctypes.data if ov is not None else None,
+                                  n_threads or threads())
     if rc != 0:
         raise RuntimeError(f"orc_shade_view failed: {rc}")
     return rgba, shadow, margin

@@ -90,6 +90,31 @@ typedef struct orc_shade_in {
 int orc_shade_view(const orc_geometry* g, const orc_shade_in* in, const uint64_t* vis, double* out_rgba,
                    double* out_shadow, double* out_margin, const float* override_shadow, int n_threads);
 
+/* Neural material decoder parameters (see neural.cpp for the definition). */
+typedef struct orc_neural {
+    uint32_t latent_size;
+    uint32_t n_materials;
+    const uint32_t* slot_of; /* per material: latent grid slot, 0xFFFFFFFF = none */
+    const uint32_t* active;  /* per material: 1 = decoded, 0 = plain maps (nullable: all slots decoded) */
+    const float* latents;    /* [slots][L][L][8] */
+    const float* mlp;        /* [583]: W1 [32][10], b1 [32], W2 [7][32], b2 [7] */
+} orc_neural;
+
+/* orc_shade_view with the maps of every active material replaced by the decoder's outputs. */
+int orc_shade_view_neural(const orc_geometry* g, const orc_shade_in* in, const orc_neural* nm, const uint64_t* vis,
+                          double* out_rgba, double* out_shadow, double* out_margin, const float* override_shadow,
+                          int n_threads);
+/* Decoder outputs at material[n], uv[n][2] -> out[n][7]. */
+int orc_nm_decode(const orc_neural* nm, const uint32_t* material, const float* uv, uint32_t n, double* out);
+/* Loss and gradients of the training objective (mean squared error over samples and channels, no
+ * clamp) for samples material[n], uv[n][2], target[n][7].  grad_mlp [583]; grad_latents
+ * [n_slots][L][L][8]; min_abs_preact (nullable) [n]: min_j |z_j| of each sample's hidden layer. */
+int orc_nm_gradients(const orc_neural* nm, uint32_t n_slots, const uint32_t* material, const float* uv,
+                     const float* target, uint32_t n, double* loss, double* grad_mlp, double* grad_latents,
+                     double* min_abs_preact);
+/* The plain sampler (bilinear or nearest, wrap modes, sRGB decode before filtering) at (u, v). */
+int orc_sample_texture(const rsy_texture_desc* tex, const rsy_sampler* sampler, double u, double v, double out[4]);
+
 /* Exact sRGB EOTF (basic.slang:121-128 form) of an 8-bit code. */
 double orc_srgb_to_linear(uint32_t code);
 

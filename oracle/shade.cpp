@@ -16,6 +16,7 @@
 
 #include "fixed_point.h"
 #include "oracle.h"
+#include "neural_oracle.h"
 
 namespace {
 
@@ -154,6 +155,7 @@ struct VOut {  // BasicVertexStageOutput (data.slang:3-22) + CSM
 struct Ctx {
     const orc_geometry* g;
     const orc_shade_in* in;
+    const orc_neural* nm = nullptr;
     std::vector<uint32_t> tri_base;
     std::vector<Tex> tex;
 };
@@ -360,30 +362,51 @@ void shade_pixel(const Ctx& c, uint32_t px, uint32_t py, uint64_t vis, double ou
         out_color = {md->color[0], md->color[1], md->color[2], md->color[3]};
         image_color = out_color;
         alpha = std::min(double(md->color[3]), alpha);
-        const uint32_t nti = md->normal_sampled_image_index;
-        if (sd.normal_maps_enabled == 1 && valid_index(nti) && nti < c.tex.size()) {
-            const rsy_sampler* smp = md->normal_sampler_index < in.n_samplers ? &in.samplers[md->normal_sampler_index] : nullptr;
-            const D4 sm = sample(c.tex[nti], smp, bv.u, bv.v);
-            D3 m{sm.x * 2 - 1, sm.y * 2 - 1, sm.z * 2 - 1};
-            if (c.tex[nti].channels == 2) m.z = std::sqrt(std::max(0.0, 1.0 - m.x * m.x - m.y * m.y));
-            const D3 n = normalize(bv.normal);
-            const D3 t = normalize(bv.tangent - n * dot(bv.tangent, n));
-            const D3 bb = normalize(cross(bv.normal, bv.tangent) * bv.sigma);
-            light_normal = normalize(t * m.x + bb * m.y + n * m.z);
-        }
-        const uint32_t cti = md->color_sampled_image_index;
-        if (valid_index(cti) && cti < c.tex.size()) {
-            const rsy_sampler* smp = md->color_sampler_index < in.n_samplers ? &in.samplers[md->color_sampler_index] : nullptr;
-            image_color = sample(c.tex[cti], smp, bv.u, bv.v);
-            alpha = std::min(image_color.w, alpha);
-        }
-        const uint32_t mixti = md->mixmap_sampled_image_index, metti = md->metallic_sampled_image_index;
-        if (valid_index(mixti) && mixti < c.tex.size()) {
-            const rsy_sampler* smp = md->mixmap_sampler_index < in.n_samplers ? &in.samplers[md->mixmap_sampler_index] : nullptr;
-            mr_from(c.tex[mixti], sample(c.tex[mixti], smp, bv.u, bv.v), sampled_rough, sampled_metal);
-        } else if (valid_index(metti) && metti < c.tex.size()) {
-            const rsy_sampler* smp = md->metallic_sampler_index < in.n_samplers ? &in.samplers[md->metallic_sampler_index] : nullptr;
-            mr_from(c.tex[metti], sample(c.tex[metti], smp, bv.u, bv.v), sampled_rough, sampled_metal);
+        const bool decoded = c.nm && dr.material_index < c.nm->n_materials &&
+                             c.nm->slot_of[dr.material_index] != 0xffffffffu &&
+                             (!c.nm->active || c.nm->active[dr.material_index]);
+        if (decoded) {
+            // the decoder's outputs stand in for the three samples (two-channel normal map)
+            double dec[7];
+            orc::nm_decode_point(c.nm, dr.material_index, bv.u, bv.v, dec);
+            if (sd.normal_maps_enabled == 1) {
+                D3 m{dec[3] * 2 - 1, dec[4] * 2 - 1, 0.0};
+                m.z = std::sqrt(std::max(0.0, 1.0 - m.x * m.x - m.y * m.y));
+                const D3 n = normalize(bv.normal);
+                const D3 t = normalize(bv.tangent - n * dot(bv.tangent, n));
+                const D3 bb = normalize(cross(bv.normal, bv.tangent) * bv.sigma);
+                light_normal = normalize(t * m.x + bb * m.y + n * m.z);
+            }
+            image_color = {dec[0], dec[1], dec[2], 1.0};
+            alpha = std::min(1.0, alpha);
+            sampled_metal = dec[5];
+            sampled_rough = dec[6];
+        } else {
+            const uint32_t nti = md->normal_sampled_image_index;
+            if (sd.normal_maps_enabled == 1 && valid_index(nti) && nti < c.tex.size()) {
+                const rsy_sampler* smp = md->normal_sampler_index < in.n_samplers ? &in.samplers[md->normal_sampler_index] : nullptr;
+                const D4 sm = sample(c.tex[nti], smp, bv.u, bv.v);
+                D3 m{sm.x * 2 - 1, sm.y * 2 - 1, sm.z * 2 - 1};
+                if (c.tex[nti].channels == 2) m.z = std::sqrt(std::max(0.0, 1.0 - m.x * m.x - m.y * m.y));
+                const D3 n = normalize(bv.normal);
+                const D3 t = normalize(bv.tangent - n * dot(bv.tangent, n));
+                const D3 bb = normalize(cross(bv.normal, bv.tangent) * bv.sigma);
+                light_normal = normalize(t * m.x + bb * m.y + n * m.z);
+            }
+            const uint32_t cti = md->color_sampled_image_index;
+            if (valid_index(cti) && cti < c.tex.size()) {
+                const rsy_sampler* smp = md->color_sampler_index < in.n_samplers ? &in.samplers[md->color_sampler_index] : nullptr;
+                image_color = sample(c.tex[cti], smp, bv.u, bv.v);
+                alpha = std::min(image_color.w, alpha);
+            }
+            const uint32_t mixti = md->mixmap_sampled_image_index, metti = md->metallic_sampled_image_index;
+            if (valid_index(mixti) && mixti < c.tex.size()) {
+                const rsy_sampler* smp = md->mixmap_sampler_index < in.n_samplers ? &in.samplers[md->mixmap_sampler_index] : nullptr;
+                mr_from(c.tex[mixti], sample(c.tex[mixti], smp, bv.u, bv.v), sampled_rough, sampled_metal);
+            } else if (valid_index(metti) && metti < c.tex.size()) {
+                const rsy_sampler* smp = md->metallic_sampler_index < in.n_samplers ? &in.samplers[md->metallic_sampler_index] : nullptr;
+                mr_from(c.tex[metti], sample(c.tex[metti], smp, bv.u, bv.v), sampled_rough, sampled_metal);
+            }
         }
         // neither map present: the reference reads an invalid descriptor (basic.slang:211-213);
         // here roughness and metallic stay 0.
@@ -473,12 +496,14 @@ double orc_pcf(const float* map, uint32_t size, double u, double v, double ref, 
     return r;
 }
 
-int orc_shade_view(const orc_geometry* g, const orc_shade_in* in, const uint64_t* vis, double* out_rgba,
-                   double* out_shadow, double* out_margin, const float* override_shadow, int n_threads) {
+static int shade_view(const orc_geometry* g, const orc_shade_in* in, const orc_neural* nm, const uint64_t* vis,
+                      double* out_rgba, double* out_shadow, double* out_margin, const float* override_shadow,
+                      int n_threads) {
     if (!g || !in || !vis || !out_rgba || !in->scene || !in->raster || !in->view) return RSY_INVALID_ARGUMENT;
     Ctx c;
     c.g = g;
     c.in = in;
+    c.nm = nm;
     uint32_t base = 0;
     for (uint32_t d = 0; d < g->n_draws; ++d) {
         c.tri_base.push_back(base);
@@ -503,6 +528,30 @@ int orc_shade_view(const orc_geometry* g, const orc_shade_in* in, const uint64_t
     std::vector<std::thread> th;
     for (int t = 0; t < nt; ++t) th.emplace_back(work, t);
     for (auto& x : th) x.join();
+    return RSY_OK;
+}
+
+int orc_shade_view(const orc_geometry* g, const orc_shade_in* in, const uint64_t* vis, double* out_rgba,
+                   double* out_shadow, double* out_margin, const float* override_shadow, int n_threads) {
+    return shade_view(g, in, nullptr, vis, out_rgba, out_shadow, out_margin, override_shadow, n_threads);
+}
+
+int orc_shade_view_neural(const orc_geometry* g, const orc_shade_in* in, const orc_neural* nm, const uint64_t* vis,
+                          double* out_rgba, double* out_shadow, double* out_margin, const float* override_shadow,
+                          int n_threads) {
+    if (!nm || !nm->slot_of || !nm->latents || !nm->mlp) return RSY_INVALID_ARGUMENT;
+    return shade_view(g, in, nm, vis, out_rgba, out_shadow, out_margin, override_shadow, n_threads);
+}
+
+int orc_sample_texture(const rsy_texture_desc* tex, const rsy_sampler* sampler, double u, double v, double out[4]) {
+    if (!tex || !out || !tex->pixels) return RSY_INVALID_ARGUMENT;
+    const Tex t{tex, (tex->format == RSY_TEX_RG8_UNORM || tex->format == RSY_TEX_RG8_SRGB) ? 2 : 4,
+                tex->format == RSY_TEX_RGBA8_SRGB || tex->format == RSY_TEX_RG8_SRGB};
+    const D4 r = sample(t, sampler, u, v);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    out[3] = r.w;
     return RSY_OK;
 }
 

@@ -143,6 +143,24 @@ class Scene(C.Structure):
                 ("default_shadow_size", C.c_uint32), ("workload", C.c_uint32)]
 
 
+class NeuralConfig(C.Structure):
+    _fields_ = [("latent_size", C.c_uint32), ("batch", C.c_uint32), ("init", C.c_uint32), ("region", C.c_uint32),
+                ("lr_mlp", C.c_float), ("lr_latent", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("eps", C.c_float), ("psnr_threshold", C.c_float), ("seed", C.c_uint64)]
+
+
+class NeuralStats(C.Structure):
+    _fields_ = [("n_materials", C.c_uint32), ("n_eligible", C.c_uint32), ("n_active", C.c_uint32),
+                ("steps", C.c_uint32), ("train_ms", C.c_float), ("eval_ms", C.c_float), ("last_loss", C.c_float),
+                ("reserved", C.c_float)]
+
+
+NEURAL_INIT_FROM_MAPS = 0
+NEURAL_INIT_RANDOM = 1
+NEURAL_PARAMS = 583   # W1 [32][10], b1 [32], W2 [7][32], b2 [7]
+NEURAL_CHANNELS = 8
+NEURAL_OUTPUTS = 7
+
 EXPECTED_SIZES = {Position: 64, GpuMaterial: 64, ObjectRecord: 164, SceneData: 408, Draw: 48, MeshRange: 24,
                   Sampler: 8}
 for _t, _n in EXPECTED_SIZES.items():
@@ -173,6 +191,17 @@ PROTOTYPES = [
     ("rsy_get_device_outputs", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     ("rsy_set_debug_flags", C.c_int, [C.c_void_p, C.c_uint32]),
     ("rsy_autotune", C.c_int, [C.c_void_p, C.POINTER(FrameDesc), C.c_uint32, C.POINTER(C.c_uint32)]),
+    ("rsy_neural_default_config", None, [C.POINTER(NeuralConfig)]),
+    ("rsy_neural_init", C.c_int, [C.c_void_p, C.POINTER(NeuralConfig)]),
+    ("rsy_neural_train", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    ("rsy_neural_evaluate", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    ("rsy_neural_get_stats", C.c_int, [C.c_void_p, C.POINTER(NeuralStats)]),
+    ("rsy_neural_set_active", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    ("rsy_neural_get_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("rsy_neural_set_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32]),
+    ("rsy_neural_gradients", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                       C.POINTER(C.c_float), C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("rsy_neural_decode", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
     ("rsy_camera_projection", None, [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_float)]),
     ("rsy_camera_view_matrix", None, [C.POINTER(C.c_float), C.c_float, C.c_float, C.POINTER(C.c_float)]),
     ("rsy_light_matrices", None, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float),

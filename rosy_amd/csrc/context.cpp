@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "context.h"
 #include "frame_types.h"
 #include "host_math.h"
 #include "kernels.h"
@@ -23,207 +24,6 @@
 using namespace rsy;
 
 namespace {
-
-constexpr uint32_t kMaxViewExtent = 8192;
-constexpr uint32_t kMaxViews = 16;
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), cap(o.cap) {
-        o.p = nullptr;
-        o.cap = 0;
-    }
-    DevBuf& operator=(DevBuf&& o) noexcept {
-        if (this != &o) {
-            release();
-            p = o.p;
-            cap = o.cap;
-            o.p = nullptr;
-            o.cap = 0;
-        }
-        return *this;
-    }
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    // Grows (never shrinks); contents are not preserved.
-    hipError_t ensure(size_t bytes) {
-        if (bytes <= cap && p) return hipSuccess;
-        release();
-        const size_t want = std::max<size_t>(bytes, 256);
-        hipError_t e = hipMalloc(&p, want);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return e;
-        }
-        cap = want;
-        return hipSuccess;
-    }
-    template <typename T>
-    T* as() const {
-        return static_cast<T*>(p);
-    }
-};
-
-struct PinnedBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    PinnedBuf() = default;
-    PinnedBuf(const PinnedBuf&) = delete;
-    PinnedBuf& operator=(const PinnedBuf&) = delete;
-    ~PinnedBuf() {
-        if (p) (void)hipHostFree(p);
-    }
-    hipError_t ensure(size_t bytes) {
-        if (bytes <= cap && p) return hipSuccess;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-        hipError_t e = hipHostMalloc(&p, std::max<size_t>(bytes, 256), hipHostMallocDefault);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return e;
-        }
-        cap = std::max<size_t>(bytes, 256);
-        return hipSuccess;
-    }
-};
-
-struct FrameViewDesc {
-    bool shadow;
-    uint32_t width, height;
-    float vp[16];
-    // camera only
-    float camera_position[4];
-    // shadow only
-    float split_far;
-};
-
-struct Layout {
-    // key
-    std::vector<FrameViewDesc> views;
-    uint32_t tile = 0;
-    uint64_t lists_version = 0;
-    bool shade = false;
-    // derived
-    std::vector<ViewGPU> vg;
-    std::vector<BlockEntry> vblocks, tblocks;
-    uint32_t total_bins = 0;
-    uint64_t total_xv = 0;
-    uint32_t rec_total = 0, clip_total = 0, big_total = 0;
-    uint32_t n_shadow = 0, n_camera = 0;
-    bool valid = false;
-};
-
-bool same_views(const std::vector<FrameViewDesc>& a, const std::vector<FrameViewDesc>& b) {
-    if (a.size() != b.size()) return false;
-    for (size_t i = 0; i < a.size(); ++i)
-        if (a[i].shadow != b[i].shadow || a[i].width != b[i].width || a[i].height != b[i].height) return false;
-    return true;
-}
-
-uint32_t ilog2(uint32_t v) {
-    uint32_t r = 0;
-    while ((1u << (r + 1)) <= v) ++r;
-    return r;
-}
-
-}  // namespace
-
-struct rsy_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-    hipEvent_t ev_stage[5] = {};
-    std::string err;
-    uint32_t debug_flags = 0;
-
-    // geometry
-    DevBuf vertices, px, py, pz, indices;
-    uint64_t n_vertices = 0, n_indices = 0;
-    std::vector<rsy_mesh_range> meshes;
-    bool has_geometry = false;
-    // objects + surfaces
-    DevBuf objects;
-    std::vector<rsy_object_record> h_objects;
-    std::vector<rsy_draw> surfaces;
-    bool has_objects = false;
-    // materials
-    DevBuf materials, tex_table, texels, samplers, srgb_lut;
-    uint32_t n_materials = 0, n_textures = 0, n_samplers = 0;
-    std::vector<uint32_t> tex_channels;
-    // draw lists: 0 = shadow (all), 1 = opaque (!blended), 2 = blended (kept, not rasterised)
-    std::vector<DrawGPU> lists[3];
-    std::vector<uint32_t> list_index[3];  // surface indices
-    uint32_t list_tris[3] = {0, 0, 0};
-    uint64_t list_verts[3] = {0, 0, 0};
-    DevBuf lists_d[2];
-    uint64_t lists_version = 0;
-    bool lists_dirty = true;
-    // per-frame uniforms
-    rsy_scene_data sd{};
-    rsy_raster_state rs{};
-    bool has_scene = false;
-    DevBuf scene_d;
-
-    // frame buffers
-    Layout layout;
-    DevBuf views_d, xv, recs, clipq, counters, bin_counts, bin_offsets, pair_slots, big_pairs, block_valid, totals,
-        lists_mem, vblocks_d, tblocks_d, shade_views_d, shade_cascades_d, xv_bases_d;
-    uint32_t list_cap = 0;
-    uint32_t rec_scale = 1;  // grows on overflow
-    std::vector<DevBuf> vis_out, color_out, shadow_factor_out, shadow_maps;
-    PinnedBuf staging, readback_counters;
-    std::vector<ShadeView> h_shade_views;
-    std::vector<ShadeCascade> h_shade_cascades;
-    std::vector<uint32_t> h_xv_bases;
-    uint32_t shade_blocks = 0;
-
-    // last frame (for re-running after an overflow)
-    std::vector<FrameViewDesc> last_views;
-    uint32_t last_shade = 0;
-    bool frame_pending = false;
-    bool frame_validated = false;
-    rsy_frame_stats stats{};
-    uint32_t autotuned_tile = 0;
-
-    // graph cache
-    hipGraphExec_t graph_exec = nullptr;
-    bool graph_valid = false;
-
-    hipError_t last = hipSuccess;
-};
-
-namespace {
-
-rsy_result fail(rsy_ctx* c, rsy_result r, const std::string& msg) {
-    if (c) c->err = msg;
-    return r;
-}
-
-rsy_result hip_fail(rsy_ctx* c, hipError_t e, const char* what) {
-    if (c) c->err = std::string(what) + ": " + hipGetErrorString(e);
-    return e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation ? RSY_ALLOCATION_FAILURE : RSY_GRAPHICS_FRAME_FAILURE;
-}
-
-#define HIP_TRY(ctx, expr, what)                    \
-    do {                                            \
-        hipError_t e_ = (expr);                     \
-        if (e_ != hipSuccess) return hip_fail(ctx, e_, what); \
-    } while (0)
-
-void invalidate_graph(rsy_ctx* c) {
-    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-    c->graph_exec = nullptr;
-    c->graph_valid = false;
-}
 
 // Rebuilds the per-list draw tables (Graphics.cpp:4716-4735: shadow = every surface,
 // opaque = !blended, blended = blended) from the surfaces and the mesh table.
@@ -279,7 +79,8 @@ rsy_result resolve_views(rsy_ctx* c, const rsy_frame_desc* f, std::vector<FrameV
     views.clear();
     shade = f ? f->shade_mode : RSY_SHADE_PLAIN;
     if (shade > RSY_SHADE_NEURAL) return fail(c, RSY_INVALID_ARGUMENT, "unknown shade mode");
-    if (shade == RSY_SHADE_NEURAL) return fail(c, RSY_INVALID_STATE, "neural material decoder not loaded");
+    if (shade == RSY_SHADE_NEURAL && !neural_ready(c))
+        return fail(c, RSY_INVALID_STATE, "material decoder not trained and evaluated (rsy_neural_evaluate)");
     // shadow cascades first (the reference records the shadow pass before the main pass)
     if (f && f->n_cascades) {
         if (!f->cascades) return fail(c, RSY_INVALID_ARGUMENT, "cascades == NULL");
@@ -332,7 +133,7 @@ uint32_t choose_tile(const rsy_ctx* c) {
 }
 
 // Computes bin, vertex, record and clip-queue layout of a frame and (re)allocates its buffers.
-rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint32_t tile, bool shade) {
+rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint32_t tile, uint32_t shade) {
     Layout& L = c->layout;
     const bool reuse = L.valid && same_views(L.views, views) && L.tile == tile && L.lists_version == c->lists_version &&
                        L.shade == shade;
@@ -609,6 +410,7 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
         sh.srgb_lut = c->srgb_lut.as<float>();
         sh.xv = g.xv;
         sh.xv_bases = c->xv_bases_d.as<uint32_t>();
+        if (L.shade == RSY_SHADE_NEURAL) neural_shade_args(c, sh);
         if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[3], s), "event");
         HIP_TRY(c, launch_shade(sh, s), "shade kernel");
     } else if (timing) {
@@ -670,7 +472,7 @@ bool collect_stats(rsy_ctx* c) {
 }
 
 rsy_result render_now(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint32_t shade) {
-    rsy_result r = build_layout(c, views, choose_tile(c), shade != RSY_SHADE_NONE);
+    rsy_result r = build_layout(c, views, choose_tile(c), shade);
     if (r != RSY_OK) return r;
     r = upload_frame(c);
     if (r != RSY_OK) return r;
@@ -871,6 +673,21 @@ rsy_result rsy_set_materials(rsy_ctx* c, const rsy_gpu_material* materials, uint
     c->n_samplers = n_samplers;
     c->tex_channels.clear();
     for (auto& t : table) c->tex_channels.push_back(t.channels);
+    c->h_materials.assign(materials, materials + n_materials);
+    c->h_tex_table = table;
+    c->h_samplers.assign(samplers, samplers + n_samplers);
+    c->tex_opaque.assign(n_textures, 0);
+    for (uint32_t t = 0; t < n_textures; ++t) {
+        if (table[t].channels != 4) continue;
+        const uint8_t* px = static_cast<const uint8_t*>(textures[t].pixels);
+        const uint64_t n = uint64_t(table[t].width) * table[t].height;
+        bool opaque = true;
+        for (uint64_t i = 0; i < n && opaque; ++i) opaque = px[4 * i + 3] == 255;
+        c->tex_opaque[t] = opaque ? 1 : 0;
+    }
+    // a decoder trained on the previous materials no longer applies
+    destroy_neural(c->neural);
+    c->neural = nullptr;
     invalidate_graph(c);
     return RSY_OK;
 }

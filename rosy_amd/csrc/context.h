@@ -1,0 +1,232 @@
+// Context internals shared by the frame orchestration (context.cpp) and the material decoder's
+// host side (neural_ctx.cpp).  Not part of the C ABI: callers see rsy_ctx as an opaque handle.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "frame_types.h"
+#include "kernels.h"
+#include "rosy_amd.h"
+
+namespace rsy {
+
+constexpr uint32_t kMaxViewExtent = 8192;
+constexpr uint32_t kMaxViews = 16;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), cap(o.cap) {
+        o.p = nullptr;
+        o.cap = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            cap = o.cap;
+            o.p = nullptr;
+            o.cap = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    // Grows (never shrinks); contents are not preserved.
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        release();
+        const size_t want = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        cap = want;
+        return hipSuccess;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipHostMalloc(&p, std::max<size_t>(bytes, 256), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        cap = std::max<size_t>(bytes, 256);
+        return hipSuccess;
+    }
+};
+
+struct FrameViewDesc {
+    bool shadow;
+    uint32_t width, height;
+    float vp[16];
+    // camera only
+    float camera_position[4];
+    // shadow only
+    float split_far;
+};
+
+struct Layout {
+    // key
+    std::vector<FrameViewDesc> views;
+    uint32_t tile = 0;
+    uint64_t lists_version = 0;
+    uint32_t shade = 0;  // rsy_shade_mode
+    // derived
+    std::vector<ViewGPU> vg;
+    std::vector<BlockEntry> vblocks, tblocks;
+    uint32_t total_bins = 0;
+    uint64_t total_xv = 0;
+    uint32_t rec_total = 0, clip_total = 0, big_total = 0;
+    uint32_t n_shadow = 0, n_camera = 0;
+    bool valid = false;
+};
+
+inline bool same_views(const std::vector<FrameViewDesc>& a, const std::vector<FrameViewDesc>& b) {
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); ++i)
+        if (a[i].shadow != b[i].shadow || a[i].width != b[i].width || a[i].height != b[i].height) return false;
+    return true;
+}
+
+inline uint32_t ilog2(uint32_t v) {
+    uint32_t r = 0;
+    while ((1u << (r + 1)) <= v) ++r;
+    return r;
+}
+
+struct NeuralState;  // material decoder (neural_ctx.cpp)
+void destroy_neural(NeuralState* n);
+
+}  // namespace rsy
+
+struct rsy_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    hipEvent_t ev_stage[5] = {};
+    std::string err;
+    uint32_t debug_flags = 0;
+
+    // geometry
+    rsy::DevBuf vertices, px, py, pz, indices;
+    uint64_t n_vertices = 0, n_indices = 0;
+    std::vector<rsy_mesh_range> meshes;
+    bool has_geometry = false;
+    // objects + surfaces
+    rsy::DevBuf objects;
+    std::vector<rsy_object_record> h_objects;
+    std::vector<rsy_draw> surfaces;
+    bool has_objects = false;
+    // materials
+    rsy::DevBuf materials, tex_table, texels, samplers, srgb_lut;
+    uint32_t n_materials = 0, n_textures = 0, n_samplers = 0;
+    std::vector<uint32_t> tex_channels;
+    std::vector<rsy_gpu_material> h_materials;
+    std::vector<rsy::TexGPU> h_tex_table;
+    std::vector<rsy_sampler> h_samplers;
+    std::vector<uint8_t> tex_opaque;  // four-channel texture whose alpha is 255 everywhere
+    // draw lists: 0 = shadow (all), 1 = opaque (!blended), 2 = blended (kept, not rasterised)
+    std::vector<rsy::DrawGPU> lists[3];
+    std::vector<uint32_t> list_index[3];  // surface indices
+    uint32_t list_tris[3] = {0, 0, 0};
+    uint64_t list_verts[3] = {0, 0, 0};
+    rsy::DevBuf lists_d[2];
+    uint64_t lists_version = 0;
+    bool lists_dirty = true;
+    // per-frame uniforms
+    rsy_scene_data sd{};
+    rsy_raster_state rs{};
+    bool has_scene = false;
+    rsy::DevBuf scene_d;
+
+    // frame buffers
+    rsy::Layout layout;
+    rsy::DevBuf views_d, xv, recs, clipq, counters, bin_counts, bin_offsets, pair_slots, big_pairs, block_valid, totals,
+        lists_mem, vblocks_d, tblocks_d, shade_views_d, shade_cascades_d, xv_bases_d;
+    uint32_t list_cap = 0;
+    uint32_t rec_scale = 1;  // grows on overflow
+    std::vector<rsy::DevBuf> vis_out, color_out, shadow_factor_out, shadow_maps;
+    rsy::PinnedBuf staging, readback_counters;
+    std::vector<rsy::ShadeView> h_shade_views;
+    std::vector<rsy::ShadeCascade> h_shade_cascades;
+    std::vector<uint32_t> h_xv_bases;
+    uint32_t shade_blocks = 0;
+
+    // last frame (for re-running after an overflow)
+    std::vector<rsy::FrameViewDesc> last_views;
+    uint32_t last_shade = 0;
+    bool frame_pending = false;
+    bool frame_validated = false;
+    rsy_frame_stats stats{};
+    uint32_t autotuned_tile = 0;
+
+    // graph cache
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_valid = false;
+
+    hipError_t last = hipSuccess;
+
+    // material decoder state (null until rsy_neural_init)
+    rsy::NeuralState* neural = nullptr;
+    ~rsy_ctx() { rsy::destroy_neural(neural); }
+};
+
+namespace rsy {
+
+inline rsy_result fail(rsy_ctx* c, rsy_result r, const std::string& msg) {
+    if (c) c->err = msg;
+    return r;
+}
+
+inline rsy_result hip_fail(rsy_ctx* c, hipError_t e, const char* what) {
+    if (c) c->err = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation ? RSY_ALLOCATION_FAILURE : RSY_GRAPHICS_FRAME_FAILURE;
+}
+
+#define HIP_TRY(ctx, expr, what)                    \
+    do {                                            \
+        hipError_t e_ = (expr);                     \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, what); \
+    } while (0)
+
+inline void invalidate_graph(rsy_ctx* c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    c->graph_exec = nullptr;
+    c->graph_valid = false;
+}
+
+// Material decoder hooks used by the frame path (neural_ctx.cpp).
+bool neural_ready(const rsy_ctx* c);                     // evaluated: RSY_SHADE_NEURAL may render
+void neural_shade_args(const rsy_ctx* c, ShadeLaunch& sh);  // decoder inputs of the shading kernel
+
+}  // namespace rsy

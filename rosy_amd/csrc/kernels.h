@@ -108,6 +108,86 @@ struct ShadeLaunch {
     const float* srgb_lut;      // 256 entries
     const float4* xv;           // transformed vertices of all views
     const uint32_t* xv_bases;   // per shading view: first transformed vertex of its raster view
+    // material decoder (shade mode NEURAL only)
+    const uint32_t* nm_slot;    // per material: latent grid slot or 0xFFFFFFFF
+    const uint32_t* nm_active;  // per material: 1 = decoded by the network, 0 = plain textures
+    const uint4* nm_lat16;      // FP16 latent grids, L * L cells of 8 halves per slot
+    const void* nm_weights;     // NmWeights
+    uint32_t nm_L;
+    uint32_t neural;            // 1: launch the decoder-capable shading kernel
+};
+
+// One material-decoder training step (or, in explicit mode, the exact gradient of a given batch).
+struct NmTrainLaunch {
+    float* lat;      // [slots][L][L][8] FP32 latents
+    float* lat_m;    // Adam moments of the latents
+    float* lat_v;
+    float* mlp;      // [584] FP32 MLP parameters (W1, b1, W2, b2)
+    float* mlp_m;
+    float* mlp_v;
+    float* partial;  // [n_wg][584]: per-workgroup MLP gradient sums, loss sum at index 583
+    uint32_t* step;  // device step counter: completed Adam steps
+    float* loss_out; // loss per step, index step - loss_base (nullable)
+    uint32_t loss_base, loss_cap;
+    const void* slots;  // NmSlot
+    uint32_t n_slots;
+    const TexGPU* textures;
+    const uint8_t* texels;
+    const float* srgb_lut;
+    uint32_t L, log2L;
+    uint32_t log2R;   // region side 2^log2R cells
+    uint32_t n_wg;    // workgroups per step
+    uint32_t spw;     // samples per workgroup
+    float inv_norm;   // 1 / (batch * 7)
+    float lr_mlp, lr_lat, beta1, beta2, eps;
+    uint64_t seed;
+    // explicit-batch mode: one workgroup per slot over its whole grid
+    const float* ex_uv;          // [n][2]
+    const float* ex_target;      // [n][7]
+    const uint32_t* ex_offsets;  // [slots + 1]
+    float* ex_grad_lat;          // [slots][L][L][8]
+    float* ex_grad_mlp;          // [584]: MLP gradient, loss at 583
+};
+
+struct NmInitLaunch {
+    float* lat;
+    uint32_t n_slots, L;
+    uint32_t random;  // 0: latents from the plain maps (channels 0..6), 1: uniform noise
+    uint64_t seed;
+    const void* slots;
+    const TexGPU* textures;
+    const uint8_t* texels;
+    const float* srgb_lut;
+};
+
+// PSNR evaluation: every texel centre of each slot's albedo map, decoded vs plain.
+struct NmEvalLaunch {
+    const uint4* lat16;
+    const void* weights;  // NmWeights
+    uint32_t L;
+    const void* slots;
+    uint32_t n_slots;
+    const uint32_t* block_base;  // [slots + 1] first 16x16-texel block of each slot
+    const TexGPU* textures;
+    const uint8_t* texels;
+    const float* srgb_lut;
+    float* partial;  // per block: sum of squared errors over its texels and 7 channels
+};
+
+// Decode (or sample the plain maps) at explicit (slot, uv) points.
+struct NmPointLaunch {
+    const uint4* lat16;
+    const void* weights;
+    uint32_t L;
+    const void* slots;
+    const uint32_t* slot;  // [n]
+    const float* uv;       // [n][2]
+    uint32_t n;
+    uint32_t targets;      // 1: plain-map targets instead of the decoder
+    const TexGPU* textures;
+    const uint8_t* texels;
+    const float* srgb_lut;
+    float* out;  // [n][7]
 };
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
@@ -115,5 +195,11 @@ hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s);
 hipError_t launch_binning(const BinLaunch& b, hipStream_t s);
 hipError_t launch_raster(const RasterLaunch& r, hipStream_t s);
 hipError_t launch_shade(const ShadeLaunch& l, hipStream_t s);
+hipError_t launch_nm_init(const NmInitLaunch& l, hipStream_t s);
+hipError_t launch_nm_step(const NmTrainLaunch& l, bool explicit_batch, hipStream_t s);
+hipError_t launch_nm_export(const float* lat, uint4* lat16, uint64_t cells, hipStream_t s);
+hipError_t launch_nm_export_weights(const float* mlp, void* weights, hipStream_t s);
+hipError_t launch_nm_eval(const NmEvalLaunch& l, uint32_t n_blocks, hipStream_t s);
+hipError_t launch_nm_points(const NmPointLaunch& l, hipStream_t s);
 
 }  // namespace rsy

@@ -13,7 +13,9 @@
 #include "fixed_point.h"
 #include "frame_types.h"
 #include "kernels.h"
+#include "neural.h"
 #include "rosy_amd.h"
+#include "texture.h"
 
 namespace rsy {
 namespace {
@@ -48,66 +50,6 @@ __device__ inline f3 mul3(const float* m, f3 v) {
 __device__ inline f3 xyz(float4 v) { return {v.x, v.y, v.z}; }
 
 __device__ inline bool valid_index(uint32_t i) { return i < 4294967295u; }
-
-__device__ inline int wrap_coord(int i, int n, uint32_t mode) {
-    if (mode == 0) return min(max(i, 0), n - 1);
-    if (mode == 1) {
-        const int period = 2 * n;
-        int m = i % period;
-        m = m < 0 ? m + period : m;
-        return m < n ? m : period - 1 - m;
-    }
-    int m = i % n;
-    return m < 0 ? m + n : m;
-}
-
-struct TexRef {
-    const uint8_t* base;
-    int w, h, ch, srgb;
-};
-
-__device__ inline float4 texel(const TexRef& t, int x, int y, const float* lut) {
-    const size_t o = size_t(y) * t.w + x;
-    float4 c = make_float4(0.f, 0.f, 0.f, 1.f);
-    if (t.ch == 4) {
-        const uint32_t p = reinterpret_cast<const uint32_t*>(t.base)[o];
-        const uint32_t r = p & 0xffu, g = (p >> 8) & 0xffu, b = (p >> 16) & 0xffu, a = p >> 24;
-        c.x = t.srgb ? lut[r] : float(r) * (1.0f / 255.0f);
-        c.y = t.srgb ? lut[g] : float(g) * (1.0f / 255.0f);
-        c.z = t.srgb ? lut[b] : float(b) * (1.0f / 255.0f);
-        c.w = float(a) * (1.0f / 255.0f);
-    } else {
-        const uint32_t p = reinterpret_cast<const uint16_t*>(t.base)[o];
-        const uint32_t r = p & 0xffu, g = p >> 8;
-        c.x = t.srgb ? lut[r] : float(r) * (1.0f / 255.0f);
-        c.y = t.srgb ? lut[g] : float(g) * (1.0f / 255.0f);
-    }
-    return c;
-}
-
-__device__ float4 sample_tex(const TexRef& t, const rsy_sampler* s, float u, float v, const float* lut) {
-    const uint32_t ws = s ? s->wrap_s : 2u, wt = s ? s->wrap_t : 2u;
-    const bool nearest = s && s->mag_filter <= 2;
-    if (nearest) {
-        const int x = wrap_coord(int(floorf(u * t.w)), t.w, ws), y = wrap_coord(int(floorf(v * t.h)), t.h, wt);
-        return texel(t, x, y, lut);
-    }
-    const float x = u * float(t.w) - 0.5f, y = v * float(t.h) - 0.5f;
-    const float fx0 = floorf(x), fy0 = floorf(y);
-    const float ax = x - fx0, ay = y - fy0;
-    const int x0 = int(fx0), y0 = int(fy0);
-    const int xa = wrap_coord(x0, t.w, ws), xb = wrap_coord(x0 + 1, t.w, ws);
-    const int ya = wrap_coord(y0, t.h, wt), yb = wrap_coord(y0 + 1, t.h, wt);
-    const float4 c00 = texel(t, xa, ya, lut), c10 = texel(t, xb, ya, lut);
-    const float4 c01 = texel(t, xa, yb, lut), c11 = texel(t, xb, yb, lut);
-    const float w00 = (1 - ax) * (1 - ay), w10 = ax * (1 - ay), w01 = (1 - ax) * ay, w11 = ax * ay;
-    float4 r;
-    r.x = w00 * c00.x + w10 * c10.x + w01 * c01.x + w11 * c11.x;
-    r.y = w00 * c00.y + w10 * c10.y + w01 * c01.y + w11 * c11.y;
-    r.z = w00 * c00.z + w10 * c10.z + w01 * c01.z + w11 * c11.z;
-    r.w = w00 * c00.w + w10 * c10.w + w01 * c01.w + w11 * c11.w;
-    return r;
-}
 
 __device__ inline float cmp_tap(const float* map, int S, float u, float v, float ref) {
     const float x = u * float(S) - 0.5f, y = v * float(S) - 0.5f;
@@ -248,31 +190,17 @@ __device__ inline float schlick_ggx(float ndv, float r) {
     return ndv / (ndv * (1.f - k) + k);
 }
 
-}  // namespace
+// Visible triangle of a covered pixel: its draw, vertex indices, barycentrics and cascade.
+struct PixelGeom {
+    uint32_t draw;        // index in the opaque draw list
+    uint32_t i0, i1, i2;  // mesh-local vertex indices
+    f3 lam;               // perspective-correct barycentrics at the pixel centre
+    int cascade;
+};
 
-__global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
-    __shared__ float lut[256];
-    __shared__ uint32_t tri_base[kLdsDraws];
-    lut[threadIdx.x] = L.srgb_lut[threadIdx.x];
-    const uint32_t nd_lds = L.n_draws < kLdsDraws ? L.n_draws : kLdsDraws;
-    for (uint32_t i = threadIdx.x; i < nd_lds; i += 256) tri_base[i] = L.draws[i].tri_base;
-    __syncthreads();
-
-    uint32_t vi = 0;
-    while (vi + 1 < L.n_views && L.views[vi + 1].block_base <= blockIdx.x) ++vi;
-    const ShadeView& V = L.views[vi];
-    const uint32_t lb = blockIdx.x - V.block_base;
-    const uint32_t x = (lb % V.blocks_x) * 16 + (threadIdx.x & 15);
-    const uint32_t y = (lb / V.blocks_x) * 16 + (threadIdx.x >> 4);
-    if (x >= V.width || y >= V.height) return;
-    const size_t o = size_t(y) * V.width + x;
-    const unsigned long long vis = V.vis[o];
-    if (vis == 0ull) {
-        V.color[o] = make_float4(L.clear_color[0], L.clear_color[1], L.clear_color[2], L.clear_color[3]);
-        if (V.shadow_factor) V.shadow_factor[o] = 1.f;
-        return;
-    }
-    const rsy_scene_data& sd = *static_cast<const rsy_scene_data*>(L.scene);
+__device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView& V, const uint32_t* tri_base,
+                                           uint32_t vi, uint32_t x, uint32_t y, unsigned long long vis) {
+    PixelGeom pg;
     const uint32_t tri = uint32_t(vis & 0xffffffffull) - 1u;
     // draw lookup: last draw whose tri_base <= tri
     uint32_t lo = 0, hi = L.n_draws;
@@ -281,18 +209,15 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
         const uint32_t b = mid < kLdsDraws ? tri_base[mid] : L.draws[mid].tri_base;
         if (b <= tri) lo = mid; else hi = mid;
     }
+    pg.draw = lo;
     const DrawGPU dr = L.draws[lo];
     const uint32_t local = tri - dr.tri_base;
-    const float* gd = L.objects + size_t(dr.object) * 41u;
-    const bool has_material = dr.material < L.n_materials;
-    const rsy_gpu_material* mats = static_cast<const rsy_gpu_material*>(L.materials);
-    const rsy_gpu_material* md = has_material ? &mats[dr.material] : nullptr;
-    const rsy_position* verts = static_cast<const rsy_position*>(L.vertices);
-    const rsy_sampler* samplers = static_cast<const rsy_sampler*>(L.samplers);
     const uint32_t* ip = L.indices + dr.first_index + 3u * local;
-    const uint32_t i0 = ip[0], i1 = ip[1], i2 = ip[2];
+    pg.i0 = ip[0];
+    pg.i1 = ip[1];
+    pg.i2 = ip[2];
     const float4* xv = L.xv + L.xv_bases[vi] + dr.xv_off;
-    const float4 c0 = xv[i0], c1 = xv[i1], c2 = xv[i2];
+    const float4 c0 = xv[pg.i0], c1 = xv[pg.i1], c2 = xv[pg.i2];
 
     // Perspective-correct barycentrics at the pixel centre: from the exact integer edge functions
     // of the snapped triangle (the one the raster pass covered) divided by each vertex's w, or, for
@@ -325,6 +250,27 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
             cascade = int(k);
             break;
         }
+    pg.lam = lam;
+    pg.cascade = cascade;
+    return pg;
+}
+
+// fragmentMain (basic.slang:137-344) for one covered pixel.  When `use_dec` is set, the material
+// decoder's outputs `dec` replace the albedo, normal and metallic-roughness samples of the plain
+// textures.
+__device__ inline void shade_pixel(const ShadeLaunch& L, const ShadeView& V, const float* lut, size_t o,
+                                   const PixelGeom& pg, bool use_dec, const float (&dec)[kNmOut]) {
+    const rsy_scene_data& sd = *static_cast<const rsy_scene_data*>(L.scene);
+    const DrawGPU dr = L.draws[pg.draw];
+    const float* gd = L.objects + size_t(dr.object) * 41u;
+    const bool has_material = dr.material < L.n_materials;
+    const rsy_gpu_material* mats = static_cast<const rsy_gpu_material*>(L.materials);
+    const rsy_gpu_material* md = has_material ? &mats[dr.material] : nullptr;
+    const rsy_position* verts = static_cast<const rsy_position*>(L.vertices);
+    const rsy_sampler* samplers = static_cast<const rsy_sampler*>(L.samplers);
+    const uint32_t i0 = pg.i0, i1 = pg.i1, i2 = pg.i2;
+    const f3 lam = pg.lam;
+    const int cascade = pg.cascade;
     const float* cm = cascade >= 0 ? L.cascades[cascade].m : nullptr;
     const uint32_t nti = has_material ? md->normal_sampled_image_index : RSY_ABSENT;
     const rsy_position* const vtx[3] = {&verts[dr.vtx_base + i0], &verts[dr.vtx_base + i1], &verts[dr.vtx_base + i2]};
@@ -347,35 +293,50 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
         out = make_float4(md->color[0], md->color[1], md->color[2], md->color[3]);
         image = out;
         alpha = fminf(md->color[3], alpha);
-        if (sd.normal_maps_enabled == 1 && valid_index(nti) && nti < L.n_textures) {
-            const TexGPU tg = L.textures[nti];
-            const TexRef t{L.texels + tg.offset, int(tg.width), int(tg.height), int(tg.channels), int(tg.srgb)};
-            const rsy_sampler* smp = md->normal_sampler_index < L.n_samplers ? &samplers[md->normal_sampler_index] : nullptr;
-            const float4 s = sample_tex(t, smp, tu, tv, lut);
-            f3 m = mk(s.x * 2.f - 1.f, s.y * 2.f - 1.f, s.z * 2.f - 1.f);
-            if (t.ch == 2) m.z = sqrtf(fmaxf(0.f, 1.f - m.x * m.x - m.y * m.y));
-            const f3 n = normalize(bn);
-            const f3 tt = normalize(btan - n * dot(btan, n));
-            const f3 bb = normalize(cross(bn, btan) * sigma);
-            light_normal = normalize(tt * m.x + bb * m.y + n * m.z);
-        }
-        const uint32_t cti = md->color_sampled_image_index;
-        if (valid_index(cti) && cti < L.n_textures) {
-            const TexGPU tg = L.textures[cti];
-            const TexRef t{L.texels + tg.offset, int(tg.width), int(tg.height), int(tg.channels), int(tg.srgb)};
-            const rsy_sampler* smp = md->color_sampler_index < L.n_samplers ? &samplers[md->color_sampler_index] : nullptr;
-            image = sample_tex(t, smp, tu, tv, lut);
-            alpha = fminf(image.w, alpha);
-        }
-        const uint32_t mixti = md->mixmap_sampled_image_index, metti = md->metallic_sampled_image_index;
-        const bool use_mix = valid_index(mixti) && mixti < L.n_textures;
-        const uint32_t mti = use_mix ? mixti : metti;
-        const uint32_t msi = use_mix ? md->mixmap_sampler_index : md->metallic_sampler_index;
-        if (valid_index(mti) && mti < L.n_textures) {
-            const TexGPU tg = L.textures[mti];
-            const TexRef t{L.texels + tg.offset, int(tg.width), int(tg.height), int(tg.channels), int(tg.srgb)};
-            const rsy_sampler* smp = msi < L.n_samplers ? &samplers[msi] : nullptr;
-            mr_from(t.ch, sample_tex(t, smp, tu, tv, lut), srough, smetal);
+        if (use_dec) {
+            if (sd.normal_maps_enabled == 1) {
+                f3 m = mk(dec[3] * 2.f - 1.f, dec[4] * 2.f - 1.f, 0.f);
+                m.z = sqrtf(fmaxf(0.f, 1.f - m.x * m.x - m.y * m.y));  // two-channel normal map
+                const f3 n = normalize(bn);
+                const f3 tt = normalize(btan - n * dot(btan, n));
+                const f3 bb = normalize(cross(bn, btan) * sigma);
+                light_normal = normalize(tt * m.x + bb * m.y + n * m.z);
+            }
+            image = make_float4(dec[0], dec[1], dec[2], 1.f);
+            alpha = fminf(1.f, alpha);
+            smetal = dec[5];
+            srough = dec[6];
+        } else {
+            if (sd.normal_maps_enabled == 1 && valid_index(nti) && nti < L.n_textures) {
+                const TexGPU tg = L.textures[nti];
+                const TexRef t{L.texels + tg.offset, int(tg.width), int(tg.height), int(tg.channels), int(tg.srgb)};
+                const rsy_sampler* smp = md->normal_sampler_index < L.n_samplers ? &samplers[md->normal_sampler_index] : nullptr;
+                const float4 s = sample_tex(t, smp, tu, tv, lut);
+                f3 m = mk(s.x * 2.f - 1.f, s.y * 2.f - 1.f, s.z * 2.f - 1.f);
+                if (t.ch == 2) m.z = sqrtf(fmaxf(0.f, 1.f - m.x * m.x - m.y * m.y));
+                const f3 n = normalize(bn);
+                const f3 tt = normalize(btan - n * dot(btan, n));
+                const f3 bb = normalize(cross(bn, btan) * sigma);
+                light_normal = normalize(tt * m.x + bb * m.y + n * m.z);
+            }
+            const uint32_t cti = md->color_sampled_image_index;
+            if (valid_index(cti) && cti < L.n_textures) {
+                const TexGPU tg = L.textures[cti];
+                const TexRef t{L.texels + tg.offset, int(tg.width), int(tg.height), int(tg.channels), int(tg.srgb)};
+                const rsy_sampler* smp = md->color_sampler_index < L.n_samplers ? &samplers[md->color_sampler_index] : nullptr;
+                image = sample_tex(t, smp, tu, tv, lut);
+                alpha = fminf(image.w, alpha);
+            }
+            const uint32_t mixti = md->mixmap_sampled_image_index, metti = md->metallic_sampled_image_index;
+            const bool use_mix = valid_index(mixti) && mixti < L.n_textures;
+            const uint32_t mti = use_mix ? mixti : metti;
+            const uint32_t msi = use_mix ? md->mixmap_sampler_index : md->metallic_sampler_index;
+            if (valid_index(mti) && mti < L.n_textures) {
+                const TexGPU tg = L.textures[mti];
+                const TexRef t{L.texels + tg.offset, int(tg.width), int(tg.height), int(tg.channels), int(tg.srgb)};
+                const rsy_sampler* smp = msi < L.n_samplers ? &samplers[msi] : nullptr;
+                mr_from(t.ch, sample_tex(t, smp, tu, tv, lut), srough, smetal);
+            }
         }
     }
     const f3 Vd = normalize(bview);
@@ -458,9 +419,76 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
     if (V.shadow_factor) V.shadow_factor[o] = shadow;
 }
 
+
+}  // namespace
+
+template <bool kNeural>
+__global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
+    __shared__ float lut[256];
+    __shared__ uint32_t tri_base[kLdsDraws];
+    lut[threadIdx.x] = L.srgb_lut[threadIdx.x];
+    const uint32_t nd_lds = L.n_draws < kLdsDraws ? L.n_draws : kLdsDraws;
+    for (uint32_t i = threadIdx.x; i < nd_lds; i += 256) tri_base[i] = L.draws[i].tri_base;
+    __syncthreads();
+
+    uint32_t vi = 0;
+    while (vi + 1 < L.n_views && L.views[vi + 1].block_base <= blockIdx.x) ++vi;
+    const ShadeView& V = L.views[vi];
+    const uint32_t lb = blockIdx.x - V.block_base;
+    const uint32_t x = (lb % V.blocks_x) * 16 + (threadIdx.x & 15);
+    const uint32_t y = (lb / V.blocks_x) * 16 + (threadIdx.x >> 4);
+    const bool inside = x < V.width && y < V.height;
+    const size_t o = size_t(y) * V.width + x;
+    if constexpr (!kNeural) {
+        if (!inside) return;
+        const unsigned long long vis = V.vis[o];
+        if (vis == 0ull) {
+            V.color[o] = make_float4(L.clear_color[0], L.clear_color[1], L.clear_color[2], L.clear_color[3]);
+            if (V.shadow_factor) V.shadow_factor[o] = 1.f;
+            return;
+        }
+        const float none[kNmOut] = {};
+        shade_pixel(L, V, lut, o, pixel_geometry(L, V, tri_base, vi, x, y, vis), false, none);
+    } else {
+        // Every lane stays live up to the wave-wide decoder MLP; lanes without a decoded material
+        // feed zero features and ignore the result.
+        const unsigned long long vis = inside ? V.vis[o] : 0ull;
+        PixelGeom pg{};
+        bool decode = false;
+        float f[kNmFeat] = {};
+        if (vis != 0ull) {
+            pg = pixel_geometry(L, V, tri_base, vi, x, y, vis);
+            const DrawGPU dr = L.draws[pg.draw];
+            if (dr.material < L.n_materials && L.nm_active[dr.material]) {
+                decode = true;
+                const rsy_position* verts = static_cast<const rsy_position*>(L.vertices);
+                const float* t0 = verts[dr.vtx_base + pg.i0].texture_coordinates;
+                const float* t1 = verts[dr.vtx_base + pg.i1].texture_coordinates;
+                const float* t2 = verts[dr.vtx_base + pg.i2].texture_coordinates;
+                // relative to vertex 0 with whole even periods removed (the grid repeats with period 1)
+                const float u = (t0[0] - 2.0f * floorf(t0[0] * 0.5f)) + pg.lam.y * (t1[0] - t0[0]) + pg.lam.z * (t2[0] - t0[0]);
+                const float v = (t0[1] - 2.0f * floorf(t0[1] * 0.5f)) + pg.lam.y * (t1[1] - t0[1]) + pg.lam.z * (t2[1] - t0[1]);
+                nm_features(L.nm_lat16 + size_t(L.nm_slot[dr.material]) * L.nm_L * L.nm_L, L.nm_L, u, v, f);
+            }
+        }
+        float dec[kNmOut] = {};
+        if (__builtin_amdgcn_ballot_w64(decode) != 0ull) nm_mlp_wave(static_cast<const NmWeights*>(L.nm_weights), f, dec);
+        if (!inside) return;
+        if (vis == 0ull) {
+            V.color[o] = make_float4(L.clear_color[0], L.clear_color[1], L.clear_color[2], L.clear_color[3]);
+            if (V.shadow_factor) V.shadow_factor[o] = 1.f;
+            return;
+        }
+        shade_pixel(L, V, lut, o, pg, decode, dec);
+    }
+}
+
 hipError_t launch_shade(const ShadeLaunch& l, hipStream_t s) {
     if (l.total_blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_shade, dim3(l.total_blocks), dim3(256), 0, s, l);
+    if (l.neural)
+        hipLaunchKernelGGL(k_shade<true>, dim3(l.total_blocks), dim3(256), 0, s, l);
+    else
+        hipLaunchKernelGGL(k_shade<false>, dim3(l.total_blocks), dim3(256), 0, s, l);
     return hipGetLastError();
 }
 

@@ -1,0 +1,73 @@
+// Texture sampling shared by the shading and material-decoder kernels: mip 0, nearest or bilinear,
+// CLAMP_TO_EDGE / MIRRORED_REPEAT / REPEAT addressing, sRGB decode before filtering through a
+// 256-entry table (Vulkan's filtering order; samplers Graphics.cpp:3421-3432, formats 4000-4015).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "rosy_amd.h"
+
+namespace rsy {
+
+__device__ inline int wrap_coord(int i, int n, uint32_t mode) {
+    if (mode == 0) return min(max(i, 0), n - 1);
+    if (mode == 1) {
+        const int period = 2 * n;
+        int m = i % period;
+        m = m < 0 ? m + period : m;
+        return m < n ? m : period - 1 - m;
+    }
+    int m = i % n;
+    return m < 0 ? m + n : m;
+}
+
+struct TexRef {
+    const uint8_t* base;
+    int w, h, ch, srgb;
+};
+
+__device__ inline float4 texel(const TexRef& t, int x, int y, const float* lut) {
+    const size_t o = size_t(y) * t.w + x;
+    float4 c = make_float4(0.f, 0.f, 0.f, 1.f);
+    if (t.ch == 4) {
+        const uint32_t p = reinterpret_cast<const uint32_t*>(t.base)[o];
+        const uint32_t r = p & 0xffu, g = (p >> 8) & 0xffu, b = (p >> 16) & 0xffu, a = p >> 24;
+        c.x = t.srgb ? lut[r] : float(r) * (1.0f / 255.0f);
+        c.y = t.srgb ? lut[g] : float(g) * (1.0f / 255.0f);
+        c.z = t.srgb ? lut[b] : float(b) * (1.0f / 255.0f);
+        c.w = float(a) * (1.0f / 255.0f);
+    } else {
+        const uint32_t p = reinterpret_cast<const uint16_t*>(t.base)[o];
+        const uint32_t r = p & 0xffu, g = p >> 8;
+        c.x = t.srgb ? lut[r] : float(r) * (1.0f / 255.0f);
+        c.y = t.srgb ? lut[g] : float(g) * (1.0f / 255.0f);
+    }
+    return c;
+}
+
+__device__ inline float4 sample_tex(const TexRef& t, const rsy_sampler* s, float u, float v, const float* lut) {
+    const uint32_t ws = s ? s->wrap_s : 2u, wt = s ? s->wrap_t : 2u;
+    const bool nearest = s && s->mag_filter <= 2;
+    if (nearest) {
+        const int x = wrap_coord(int(floorf(u * t.w)), t.w, ws), y = wrap_coord(int(floorf(v * t.h)), t.h, wt);
+        return texel(t, x, y, lut);
+    }
+    const float x = u * float(t.w) - 0.5f, y = v * float(t.h) - 0.5f;
+    const float fx0 = floorf(x), fy0 = floorf(y);
+    const float ax = x - fx0, ay = y - fy0;
+    const int x0 = int(fx0), y0 = int(fy0);
+    const int xa = wrap_coord(x0, t.w, ws), xb = wrap_coord(x0 + 1, t.w, ws);
+    const int ya = wrap_coord(y0, t.h, wt), yb = wrap_coord(y0 + 1, t.h, wt);
+    const float4 c00 = texel(t, xa, ya, lut), c10 = texel(t, xb, ya, lut);
+    const float4 c01 = texel(t, xa, yb, lut), c11 = texel(t, xb, yb, lut);
+    const float w00 = (1 - ax) * (1 - ay), w10 = ax * (1 - ay), w01 = (1 - ax) * ay, w11 = ax * ay;
+    float4 r;
+    r.x = w00 * c00.x + w10 * c10.x + w01 * c01.x + w11 * c11.x;
+    r.y = w00 * c00.y + w10 * c10.y + w01 * c01.y + w11 * c11.y;
+    r.z = w00 * c00.z + w10 * c10.z + w01 * c01.z + w11 * c11.z;
+    r.w = w00 * c00.w + w10 * c10.w + w01 * c01.w + w11 * c11.w;
+    return r;
+}
+
+}  // namespace rsy
