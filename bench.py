@@ -1,18 +1,27 @@
 #!/usr/bin/env python3
-"""Frame benchmark of the raster-and-shade path on one MI355X.
+"""Frame benchmark of the raster-and-shade path on one MI355X: plain vs neural-material shading.
 
-Workload: 120 procedural meshes, 262,144 triangles, 1920x1080 camera view, 16 materials with
-2048^2 RGBA8 albedo + RG8 normal + RG8 metallic-roughness maps, one directional light with a 2048^2
-shadow map (seeded, synthetic).  A step is one whole frame: shadow pass, main visibility pass and
-deferred Cook-Torrance + PCF shading, replayed from a captured HIP graph with every input resident
-in HBM.  `value` is device milliseconds per frame (HIP events around the timed replays).
+Workload (seeded, synthetic): 120 procedural meshes, 262,144 triangles, one 1920x1080 camera view
+and a 2048^2 shadow map; 64 materials whose RGBA8 albedo, RG8 normal and RG8 metallic-roughness
+maps are 1024^2 (value noise + stripes + blobs).  The neural material decoder (256^2 x 8-channel
+FP16 latent grid per material + one 10-32-7 MLP) is trained on the GPU first (Adam, lr 1e-3 with
+cosine decay, 20,000 steps of 65,536 samples), its PSNR measured per material, and every material
+at or above 40 dB decoded; the rest keep their plain maps.
+
+A step is one whole frame: shadow pass, main visibility pass and deferred Cook-Torrance + PCF
+shading, replayed from a captured HIP graph with every input resident in HBM.  `value` is the
+neural-material frame in device milliseconds (HIP events around the timed replays); the plain frame
+is timed the same way beside it.
 
 Also reported:
-  roofline     the dominant kernel's algorithmic bytes per launch / its average duration, measured
-               live with HIP events on the context's stream, against the 8 TB/s HBM3E peak;
-  cpu_baseline the CPU oracle (fixed-point raster + FP64 shading, std::thread over the host cores)
-               on the same frame, repeated for a bounded time;
-  max_abs_err  shaded colour vs that FP64 CPU frame (oracle re-run with the GPU's shadow factors).
+  roofline      the dominant kernel (the tile raster): its algorithmic bytes per launch / its average
+                duration measured live with HIP events on the context's stream, against 8 TB/s;
+  decoder       MFMA FLOPs of the decoder per frame / the neural shading kernel's duration, against
+                the 2.5 PFLOP/s dense FP16 peak (the shading kernel is not MFMA-bound);
+  cpu_baseline  the CPU oracle (fixed-point raster + FP64 shading, std::thread over the host cores)
+                on the same plain frame, repeated for a bounded time;
+  max_abs_err   shaded colour of both forms vs the FP64 CPU oracle (re-run with the GPU's shadow
+                factors; for the neural form with the same FP16 decoder parameters).
 """
 import argparse
 import json
@@ -25,10 +34,23 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms/frame @1080p,262k tris: plain vs neural-material shading @PSNR>=40dB"
 HBM_PEAK_GBS = 8000.0
+MFMA_FP16_PEAK_TFLOPS = 2500.0
+DECODER_FLOPS_PER_PIXEL = 2 * (10 * 32 + 32 * 7)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def stage_medians(r, frame, n=10):
+    import numpy as np
+    from rosy_amd import abi
+    r.set_debug_flags(abi.DEBUG_STAGE_TIMING | abi.DEBUG_SHADOW_FACTOR)
+    st = []
+    for _ in range(n):
+        r.render(frame)
+        st.append(list(r.stats().stage_ms)[:4])
+    return np.median(np.array(st), axis=0)
 
 
 def main():
@@ -37,6 +59,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -51,58 +74,86 @@ def main():
     from rosy_amd import abi
 
     t_gen = time.time()
-    scene = rosy_amd.Scene(1, seed=args.seed)
-    log(f"scene generated in {time.time() - t_gen:.1f}s: {scene.n_triangles} triangles")
+    scene = rosy_amd.Scene(3, seed=args.seed)
+    log(f"scene generated in {time.time() - t_gen:.1f}s: {scene.n_triangles} triangles, "
+        f"{scene.s.n_materials} materials")
     r = rosy_amd.Renderer(device=0)
     r.load(scene)
-    frame = scene.frame_desc(abi.SHADE_PLAIN)
-    tile = r.autotune(frame, 10)
+    plain = scene.frame_desc(abi.SHADE_PLAIN)
+    neural = scene.frame_desc(abi.SHADE_NEURAL)
+    tile = r.autotune(plain, 10)
     log(f"autotuned bin size {tile}")
 
-    # per-kernel durations: eager frames with stage events around the raster and shading kernels
-    r.set_debug_flags(abi.DEBUG_STAGE_TIMING | abi.DEBUG_SHADOW_FACTOR)
-    stages = []
-    for _ in range(10):
-        r.render(frame)
-        st = r.stats()
-        stages.append(list(st.stage_ms)[:4])
-    stages = np.median(np.array(stages), axis=0)
-    st = r.stats()
+    # decoder: train on the GPU, measure PSNR, activate the materials at >= 40 dB
+    r.neural_init(decay_steps=args.train_steps)
+    psnr0 = r.neural_evaluate()
+    loss = r.neural_train(args.train_steps)
+    psnr = r.neural_evaluate()
+    nst = r.neural_stats()
+    log(f"decoder: {args.train_steps} steps in {nst.train_ms:.0f} ms, loss {loss[0]:.2e} -> {loss[-1]:.2e}, "
+        f"PSNR min/median {psnr.min():.2f}/{np.median(psnr):.2f} dB (untrained {psnr0.min():.2f}/"
+        f"{np.median(psnr0):.2f}), {nst.n_active}/{nst.n_materials} decoded")
+
+    # per-kernel durations of both forms (eager frames, HIP events between the stages)
+    st_plain = stage_medians(r, plain)
+    st_neural = stage_medians(r, neural)
+    stp = r.stats()
     vis = r.visibility(0)
     shadow = r.shadow_map(0)
-    colour = r.color(0)
-    sfac = r.shadow_factor(0)
+    col_neural = r.color(0)
+    sf_neural = r.shadow_factor(0)
+    r.render(plain)
+    col_plain = r.color(0)
+    sf_plain = r.shadow_factor(0)
     r.set_debug_flags(0)
 
-    # timed region: warmup replays, then exactly `steps` replays of the captured frame graph
-    r.render_graph(frame, args.warmup)
+    # timed region: warmup replays, then exactly `steps` replays of each captured frame graph
+    r.render_graph(plain, args.warmup)
+    r.render_graph(plain, args.steps)
+    plain_ms = float(r.stats().gpu_ms)
+    r.render_graph(neural, args.warmup)
     t0 = time.perf_counter()
-    r.render_graph(frame, args.steps)  # synchronises at the end
+    r.render_graph(neural, args.steps)  # synchronises at the end
     wall = (time.perf_counter() - t0) * 1e3 / args.steps
-    gst = r.stats()
-    ms = float(gst.gpu_ms)
-    log(f"graph frame: {ms:.4f} ms device, {wall:.4f} ms wall")
+    ms = float(r.stats().gpu_ms)
+    log(f"graph frame: neural {ms:.4f} ms, plain {plain_ms:.4f} ms (device); neural wall {wall:.4f} ms")
 
     # raster kernel: bin counts + offsets, list entries and 48-byte records per (triangle, bin)
     # pair, one visibility word (8 B) per camera pixel and one depth (4 B) per shadow texel
     cam = scene.camera_views()[0]
     cas = scene.cascades()[0]
-    bins = sum(((v.width + tile - 1) // tile) * ((v.height + tile - 1) // tile) for v in (cam,)) + \
-        ((cas.size + tile - 1) // tile) ** 2
-    raster_bytes = bins * 8 + int(st.tile_pairs) * (4 + 48) + cam.width * cam.height * 8 + cas.size * cas.size * 4
-    raster_ms = float(stages[2])
-    shade_ms = float(stages[3])
+    bins = ((cam.width + tile - 1) // tile) * ((cam.height + tile - 1) // tile) + ((cas.size + tile - 1) // tile) ** 2
+    raster_bytes = bins * 8 + int(stp.tile_pairs) * (4 + 48) + cam.width * cam.height * 8 + cas.size * cas.size * 4
+    raster_ms = float(st_neural[2])
     achieved = raster_bytes / (raster_ms * 1e-3) / 1e9
     roofline = {"kernel": "k_raster (tile raster, camera + shadow views in one launch)", "bound": "hbm",
                 "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "algorithmic_bytes": raster_bytes, "avg_launch_ms": round(raster_ms, 5)}
 
-    cpu = None
-    err = None
-    if not args.no_cpu:
-        import numpy as np
+    # decoder MFMA work: decoded pixels x 1088 FLOPs over the neural shading kernel
+    active = np.zeros(scene.s.n_materials + 1, bool)
+    r_active = psnr >= 40.0
+    active[:scene.s.n_materials] = r_active
+    surf = [d for d in scene.surfaces() if not d.blended]
+    tri_mat = np.concatenate([np.full(d.index_count // 3, min(int(d.material_index), scene.s.n_materials))
+                              for d in surf])
+    covered = vis != 0
+    tri = (vis[covered] & 0xffffffff).astype(np.int64) - 1
+    decoded_px = int(active[tri_mat[tri]].sum())
+    dec_tflops = decoded_px * DECODER_FLOPS_PER_PIXEL / (float(st_neural[3]) * 1e-3) / 1e12
+    decoder = {"decoded_pixels": decoded_px, "flops_per_pixel": DECODER_FLOPS_PER_PIXEL,
+               "achieved_tflops": round(dec_tflops, 3), "peak_tflops": MFMA_FP16_PEAK_TFLOPS,
+               "mfma_frac": round(dec_tflops / MFMA_FP16_PEAK_TFLOPS, 6),
+               "train_ms": round(float(nst.train_ms), 1), "train_steps": args.train_steps,
+               "psnr_min": round(float(psnr.min()), 2), "psnr_median": round(float(np.median(psnr)), 2),
+               "psnr_untrained_median": round(float(np.median(psnr0)), 2),
+               "materials_decoded": int(nst.n_active), "materials": int(nst.n_materials)}
 
+    cpu = None
+    err_plain = err_neural = None
+    bit_exact = None
+    if not args.no_cpu:
         import oracle
         threads = oracle.threads()
         rs = scene.s.raster_state
@@ -111,16 +162,27 @@ def main():
         while True:
             sm, _ = oracle.raster_view(scene, cas.light_view_projection[:], cas.size, cas.size, True, rs, threads)
             ov, _ = oracle.raster_view(scene, cam.view_projection[:], cam.width, cam.height, False, rs, threads)
-            ref, _, _ = oracle.shade_view(scene, cam, [cas], [sm], ov, n_threads=threads)
+            oracle.shade_view(scene, cam, [cas], [sm], ov, n_threads=threads)
             n += 1
             if time.perf_counter() - t0 >= args.cpu_seconds or n >= 50:
                 break
         cpu_ms = (time.perf_counter() - t0) * 1e3 / n
         cpu = {"value": round(cpu_ms, 2), "unit": "ms/frame", "cores": threads, "kind": "port",
-               "sample": f"{n} whole frames of the same workload (shadow raster + camera raster + FP64 shading)"}
-        ref2, _, _ = oracle.shade_view(scene, cam, [cas], [sm], ov, override_shadow=sfac, n_threads=threads)
-        err = float(np.abs(colour.astype(np.float64) - ref2).max())
+               "sample": f"{n} whole plain frames of the same workload (shadow raster + camera raster + FP64 "
+                         f"shading)"}
         bit_exact = bool(np.array_equal(ov, vis) and np.array_equal(sm.view(np.uint32), shadow.view(np.uint32)))
+        ref, _, _ = oracle.shade_view(scene, cam, [cas], [sm], ov, override_shadow=sf_plain, n_threads=threads)
+        err_plain = float(np.abs(col_plain.astype(np.float64) - ref).max())
+        lat, mlp = r.neural_params()
+        q = mlp.astype(np.float16).astype(np.float32)
+        q[576:583] = mlp[576:583]
+        params = oracle.NeuralParams(np.arange(scene.s.n_materials), lat.astype(np.float16).astype(np.float32), q,
+                                     r_active.astype(np.uint32))
+        ref, _, _ = oracle.shade_view(scene, cam, [cas], [sm], ov, override_shadow=sf_neural, n_threads=threads,
+                                      neural=params)
+        err_neural = float(np.abs(col_neural.astype(np.float64) - ref).max())
+    frame_psnr = float(10 * np.log10(1.0 / np.mean((col_neural[covered][:, :3].astype(np.float64) -
+                                                    col_plain[covered][:, :3]) ** 2)))
     out = {
         "metric": METRIC,
         "value": round(ms, 5),
@@ -134,20 +196,25 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "scene-scale: 120 meshes, 262,144 triangles, 1920x1080, 2048^2 shadow map, "
-                               "16 materials x 2048^2 RGBA8+RG8+RG8, plain shading",
+        "config": {"workload": "decoder config: 120 meshes, 262,144 triangles, 1920x1080, 2048^2 shadow map, "
+                               "64 materials x 1024^2 RGBA8+RG8+RG8 (noise+stripes+blobs), neural-material shading "
+                               "(256^2 x 8 FP16 latents + 10-32-7 MLP, FP16 MFMA), plain-map fallback below 40 dB",
                    "seed": args.seed, "bin_size": tile},
+        "plain_ms_per_frame": round(plain_ms, 5),
+        "shade_ms": {"plain": round(float(st_plain[3]), 5), "neural": round(float(st_neural[3]), 5)},
+        "shading_speedup": round(float(st_plain[3]) / float(st_neural[3]), 3),
         "roofline": roofline,
+        "decoder": decoder,
         "cpu_baseline": cpu,
         "wall_ms_per_step": round(wall, 5),
         "mtris_per_s": round(2 * scene.n_triangles / (ms * 1e-3) / 1e6, 1),
-        "stage_ms": {"geometry": round(float(stages[0]), 5), "binning": round(float(stages[1]), 5),
-                     "raster": round(raster_ms, 5), "shade": round(shade_ms, 5)},
-        "tile_pairs": int(st.tile_pairs),
-        "max_abs_err_vs_fp64": err,
+        "stage_ms": {"geometry": round(float(st_neural[0]), 5), "binning": round(float(st_neural[1]), 5),
+                     "raster": round(raster_ms, 5), "shade": round(float(st_neural[3]), 5)},
+        "tile_pairs": int(stp.tile_pairs),
+        "max_abs_err_vs_fp64": {"plain": err_plain, "neural": err_neural},
+        "frame_psnr_neural_vs_plain_db": round(frame_psnr, 2),
+        "bit_exact_vs_oracle": bit_exact,
     }
-    if cpu is not None:
-        out["bit_exact_vs_oracle"] = bit_exact
     print(json.dumps(out), flush=True)
     r.close()
     scene.close()

@@ -345,6 +345,8 @@ typedef struct rsy_neural_config {
     float lr_latent;
     float beta1, beta2, eps;
     float psnr_threshold;    /* dB */
+    uint32_t decay_steps;    /* cosine learning-rate decay to 0 over this many steps (0: constant) */
+    uint32_t reserved;
     uint64_t seed;
 } rsy_neural_config;
 
@@ -360,7 +362,7 @@ typedef struct rsy_neural_stats {
 } rsy_neural_stats;
 
 /* Fills the defaults: latent 256, batch 65536, init from maps, region 16, lr 1e-3 / 1e-3,
- * betas 0.9 / 0.999, eps 1e-8, threshold 40 dB, seed 1. */
+ * betas 0.9 / 0.999, eps 1e-8, threshold 40 dB, no decay, seed 1. */
 void rsy_neural_default_config(rsy_neural_config* cfg);
 /* Allocates and initialises the decoder for the context's current materials (rsy_set_materials
  * first).  Re-initialising discards any trained state. */

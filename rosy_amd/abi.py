@@ -146,7 +146,8 @@ class Scene(C.Structure):
 class NeuralConfig(C.Structure):
     _fields_ = [("latent_size", C.c_uint32), ("batch", C.c_uint32), ("init", C.c_uint32), ("region", C.c_uint32),
                 ("lr_mlp", C.c_float), ("lr_latent", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
-                ("eps", C.c_float), ("psnr_threshold", C.c_float), ("seed", C.c_uint64)]
+                ("eps", C.c_float), ("psnr_threshold", C.c_float), ("decay_steps", C.c_uint32),
+                ("reserved", C.c_uint32), ("seed", C.c_uint64)]
 
 
 class NeuralStats(C.Structure):

@@ -140,6 +140,7 @@ struct NmTrainLaunch {
     uint32_t spw;     // samples per workgroup
     float inv_norm;   // 1 / (batch * 7)
     float lr_mlp, lr_lat, beta1, beta2, eps;
+    uint32_t decay_steps;  // cosine decay of both learning rates to 0 over this many steps (0: constant)
     uint64_t seed;
     // explicit-batch mode: one workgroup per slot over its whole grid
     const float* ex_uv;          // [n][2]

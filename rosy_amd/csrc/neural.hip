@@ -3,15 +3,16 @@
 //
 // Training.  A step draws `batch` samples (u, v) and regresses the decoder onto the plain
 // sampler's values there (mean squared error over the 7 channels), then applies one Adam update
-// (lr, beta1, beta2, eps) to the latents it touched and to the shared MLP.  Samples are drawn per
+// (lr with an optional cosine decay, beta1, beta2, eps) to the latents it touched and to the shared MLP.  Samples are drawn per
 // workgroup inside a region of 2^log2R x 2^log2R latent cells of one material, positioned so that
 // every bilinear footprint stays inside the region: the region's latents and their gradients live
 // in LDS (gradients accumulate with LDS atomics, never with global ones), and the workgroup applies
 // Adam to its region itself.  The regions of one step are distinct cells of one partition of every
 // grid (the partition shifts by a random offset each step), so no two workgroups touch the same
 // latent.  The MLP gradient is reduced per workgroup from per-sample records in LDS and summed over
-// workgroups in a fixed order by a one-block kernel, which also applies Adam to the MLP and advances
-// the device-side step counter (so a captured graph of steps replays as consecutive steps).
+// workgroups in a fixed order by one wave per parameter, which also applies Adam to it; a last
+// one-thread kernel advances the device-side step counter (so a captured graph of steps replays as
+// consecutive steps).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -35,6 +36,13 @@ __device__ inline uint64_t mix64(uint64_t z) {  // SplitMix64 finaliser
 }
 
 __device__ inline float unit24(uint32_t x) { return float(x >> 8) * (1.0f / 16777216.0f); }  // [0, 1)
+
+// Learning-rate factor of step `step`: 0.5 (1 + cos(pi step / decay)) until `decay`, then 0.
+__device__ inline float lr_factor(uint32_t step, uint32_t decay) {
+    if (decay == 0) return 1.f;
+    if (step >= decay) return 0.f;
+    return 0.5f * (1.f + cospif(float(step) / float(decay)));
+}
 
 __device__ inline float block_sum(float v, float* red) {
 #pragma unroll
@@ -209,17 +217,18 @@ __global__ __launch_bounds__(256) void k_nm_step(NmTrainLaunch T) {
         __syncthreads();
     }
     const float loss_sum = block_sum(loss, red_s);
-    float* P = T.partial + size_t(blockIdx.x) * kNmParamsPad;
+    // partial sums, parameter-major: partial[g * n_wg + workgroup]
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const uint32_t g = tid + 256u * a;
-        if (g < uint32_t(kNmParams)) P[g] = acc[a];
+        if (g < uint32_t(kNmParams)) T.partial[size_t(g) * T.n_wg + blockIdx.x] = acc[a];
     }
-    if (tid == 0) P[kNmParams] = loss_sum;
+    if (tid == 0) T.partial[size_t(kNmParams) * T.n_wg + blockIdx.x] = loss_sum;
 
     // latents of the region: gradient out (explicit mode) or Adam
     const float t = float(step + 1);
     const float bc1 = 1.f - powf(T.beta1, t), bc2 = 1.f - powf(T.beta2, t);
+    const float lr = T.lr_lat * lr_factor(step, T.decay_steps);
     for (uint32_t i = tid; i < n_el; i += 256) {
         const uint32_t c = i & 7u, cell = i >> 3, cx = cell % R, cy = cell / R;
         const size_t g = grid_base + (size_t((ry + cy) & Lm) * L + ((rx + cx) & Lm)) * kNmCh + c;
@@ -231,38 +240,44 @@ __global__ __launch_bounds__(256) void k_nm_step(NmTrainLaunch T) {
             const float v = T.beta2 * T.lat_v[g] + (1.f - T.beta2) * gr * gr;
             T.lat_m[g] = m;
             T.lat_v[g] = v;
-            T.lat[g] = lat_s[i] - T.lr_lat * (m / bc1) / (sqrtf(v / bc2) + T.eps);
+            T.lat[g] = lat_s[i] - lr * (m / bc1) / (sqrtf(v / bc2) + T.eps);
         }
     }
 }
 
-// Sums the per-workgroup MLP gradients in workgroup order, then Adam on the MLP (or, in explicit
-// mode, writes the summed gradient), records the step's loss and advances the step counter.
+// One wave per MLP parameter (and one for the loss): sums that parameter's per-workgroup partials
+// (lane-strided, then a fixed-order wave reduction), then Adam (or, in explicit mode, writes the
+// summed gradient) and records the step's loss.
 template <bool kExplicit>
 __global__ __launch_bounds__(256) void k_nm_mlp(NmTrainLaunch T) {
-    const uint32_t tid = threadIdx.x;
+    const uint32_t g = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (g >= uint32_t(kNmParamsPad)) return;
     const uint32_t step = kExplicit ? 0u : *T.step;
-    const float t = float(step + 1);
-    const float bc1 = 1.f - powf(T.beta1, t), bc2 = 1.f - powf(T.beta2, t);
-    for (uint32_t g = tid; g < uint32_t(kNmParamsPad); g += 256) {
-        float s = 0.f;
-        for (uint32_t b = 0; b < T.n_wg; ++b) s += T.partial[size_t(b) * kNmParamsPad + g];
-        if (kExplicit) {
-            T.ex_grad_mlp[g] = s;
-        } else if (g < uint32_t(kNmParams)) {
-            const float m = T.beta1 * T.mlp_m[g] + (1.f - T.beta1) * s;
-            const float v = T.beta2 * T.mlp_v[g] + (1.f - T.beta2) * s * s;
-            T.mlp_m[g] = m;
-            T.mlp_v[g] = v;
-            T.mlp[g] -= T.lr_mlp * (m / bc1) / (sqrtf(v / bc2) + T.eps);
-        } else if (T.loss_out && step >= T.loss_base && step - T.loss_base < T.loss_cap) {
-            T.loss_out[step - T.loss_base] = s * T.inv_norm;  // mean squared error of the step
-        }
+    const float* P = T.partial + size_t(g) * T.n_wg;
+    float s = 0.f;
+    for (uint32_t b = lane; b < T.n_wg; b += 64) s += P[b];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane != 0) return;
+    if (kExplicit) {
+        T.ex_grad_mlp[g] = s;
+    } else if (g < uint32_t(kNmParams)) {
+        const float t = float(step + 1);
+        const float bc1 = 1.f - powf(T.beta1, t), bc2 = 1.f - powf(T.beta2, t);
+        const float lr = T.lr_mlp * lr_factor(step, T.decay_steps);
+        const float m = T.beta1 * T.mlp_m[g] + (1.f - T.beta1) * s;
+        const float v = T.beta2 * T.mlp_v[g] + (1.f - T.beta2) * s * s;
+        T.mlp_m[g] = m;
+        T.mlp_v[g] = v;
+        T.mlp[g] -= lr * (m / bc1) / (sqrtf(v / bc2) + T.eps);
+    } else if (T.loss_out && step >= T.loss_base && step - T.loss_base < T.loss_cap) {
+        T.loss_out[step - T.loss_base] = s * T.inv_norm;  // mean squared error of the step
     }
-    if (!kExplicit) {
-        __syncthreads();
-        if (tid == 0) *T.step = step + 1;
-    }
+}
+
+// Advances the device step counter once every reader of this step has finished.
+__global__ void k_nm_advance(uint32_t* step) {
+    if (threadIdx.x == 0) *step += 1u;
 }
 
 __global__ __launch_bounds__(256) void k_nm_init(NmInitLaunch I) {
@@ -386,11 +401,12 @@ hipError_t launch_nm_step(const NmTrainLaunch& l, bool explicit_batch, hipStream
     if (explicit_batch) {
         if (l.L > uint32_t(kMaxR)) return hipErrorInvalidValue;
         hipLaunchKernelGGL(k_nm_step<true>, dim3(l.n_slots), dim3(256), 0, s, l);
-        hipLaunchKernelGGL(k_nm_mlp<true>, dim3(1), dim3(256), 0, s, l);
+        hipLaunchKernelGGL(k_nm_mlp<true>, dim3(kNmParamsPad / 4), dim3(256), 0, s, l);
     } else {
         if ((1u << l.log2R) > uint32_t(kMaxR) || l.log2R > l.log2L || l.log2R < 1) return hipErrorInvalidValue;
         hipLaunchKernelGGL(k_nm_step<false>, dim3(l.n_wg), dim3(256), 0, s, l);
-        hipLaunchKernelGGL(k_nm_mlp<false>, dim3(1), dim3(256), 0, s, l);
+        hipLaunchKernelGGL(k_nm_mlp<false>, dim3(kNmParamsPad / 4), dim3(256), 0, s, l);
+        hipLaunchKernelGGL(k_nm_advance, dim3(1), dim3(64), 0, s, l.step);
     }
     return hipGetLastError();
 }

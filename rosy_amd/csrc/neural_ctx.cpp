@@ -144,6 +144,7 @@ NmTrainLaunch train_args(rsy_ctx* c, NeuralState* n) {
     T.beta1 = n->cfg.beta1;
     T.beta2 = n->cfg.beta2;
     T.eps = n->cfg.eps;
+    T.decay_steps = n->cfg.decay_steps;
     T.seed = n->cfg.seed;
     return T;
 }
