@@ -62,11 +62,25 @@ def main():
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (5 times): the reference kernel that "
+                         "calibrates the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this frame path is built and measured on a single MI355X "
               "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe_mb:
+        import torch
+        n = args.copy_probe_mb << 20
+        a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "copies": 5}), flush=True)
+        return
 
     import numpy as np
 

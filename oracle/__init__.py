@@ -57,6 +57,9 @@ def lib():
         L.orc_count_tile_pairs.restype = C.c_uint64
         L.orc_count_tile_pairs.argtypes = [C.POINTER(Geometry), C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int,
                                            C.POINTER(abi.RasterState), C.c_uint32, C.c_int]
+        L.orc_tile_counts.restype = C.c_int
+        L.orc_tile_counts.argtypes = [C.POINTER(Geometry), C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int,
+                                      C.POINTER(abi.RasterState), C.c_uint32, C.c_void_p, C.c_void_p, C.c_int]
         L.orc_shade_view.restype = C.c_int
         L.orc_shade_view.argtypes = [C.POINTER(Geometry), C.POINTER(ShadeIn), C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p, C.c_int]
@@ -177,6 +180,20 @@ def sample_texture(desc, sampler, u, v):
     if rc != 0:
         raise RuntimeError(f"orc_sample_texture failed: {rc}")
     return np.array(out[:])
+
+
+def tile_counts(scene, view_projection, width, height, shadow, raster_state, tile, n_threads=None):
+    """(records per bin, box pixels per bin), each shaped (ceil(h / tile), ceil(w / tile))."""
+    g = geometry(scene, shadow)
+    vp = (C.c_float * 16)(*view_projection)
+    shape = ((height + tile - 1) // tile, (width + tile - 1) // tile)
+    out = np.zeros(shape, dtype=np.uint32)
+    area = np.zeros(shape, dtype=np.uint64)
+    rc = lib().orc_tile_counts(C.byref(g), vp, width, height, 1 if shadow else 0, C.byref(raster_state), tile,
+                               out.ctypes.data, area.ctypes.data, n_threads or threads())
+    if rc != 0:
+        raise RuntimeError(f"orc_tile_counts failed: {rc}")
+    return out, area
 
 
 def shade_view(scene, view, cascades, shadow_maps, vis, scene_data=None, raster_state=None, override_shadow=None,

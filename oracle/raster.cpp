@@ -317,4 +317,28 @@ uint64_t orc_count_tile_pairs(const orc_geometry* g, const float view_projection
     return pairs;
 }
 
+int orc_tile_counts(const orc_geometry* g, const float view_projection[16], uint32_t width, uint32_t height,
+                    int is_shadow, const rsy_raster_state* rs, uint32_t tile, uint32_t* counts, uint64_t* areas,
+                    int n_threads) {
+    if (!g || !rs || tile == 0 || !counts) return RSY_INVALID_ARGUMENT;
+    const Params p = make_params(width, height, is_shadow, rs);
+    std::vector<Rec> recs;
+    if (build_records(g, view_projection, p, recs, nullptr, n_threads) != RSY_OK) return RSY_ERROR;
+    const uint32_t tx = (width + tile - 1) / tile, ty = (height + tile - 1) / tile;
+    std::fill(counts, counts + size_t(tx) * ty, 0u);
+    if (areas) std::fill(areas, areas + size_t(tx) * ty, uint64_t(0));
+    const int32_t T = int32_t(tile);
+    for (const Rec& r : recs)
+        for (int32_t y = r.py0 / T; y <= r.py1 / T; ++y)
+            for (int32_t x = r.px0 / T; x <= r.px1 / T; ++x) {
+                ++counts[size_t(y) * tx + x];
+                if (areas) {
+                    const int32_t w = std::min(r.px1, x * T + T - 1) - std::max(r.px0, x * T) + 1;
+                    const int32_t h = std::min(r.py1, y * T + T - 1) - std::max(r.py0, y * T) + 1;
+                    areas[size_t(y) * tx + x] += uint64_t(w) * uint64_t(h);
+                }
+            }
+    return RSY_OK;
+}
+
 }  // extern "C"

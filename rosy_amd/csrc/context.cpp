@@ -212,7 +212,8 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     HIP_TRY(c, c->pair_slots.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * 16), "allocate bin slots");
     HIP_TRY(c, c->big_pairs.ensure(size_t(std::max<uint32_t>(1, L.big_total)) * 16), "allocate bin pairs");
     HIP_TRY(c, c->block_valid.ensure(std::max<size_t>(1, L.tblocks.size()) * 4), "allocate statistics");
-    HIP_TRY(c, c->totals.ensure(16), "allocate statistics");
+    HIP_TRY(c, c->totals.ensure(32), "allocate statistics");
+    HIP_TRY(c, c->split_bins.ensure(size_t(std::max<uint32_t>(1, L.total_bins)) * 4), "allocate bins");
     HIP_TRY(c, c->vblocks_d.ensure(std::max<size_t>(1, L.vblocks.size()) * sizeof(BlockEntry)), "allocate blocks");
     HIP_TRY(c, c->tblocks_d.ensure(std::max<size_t>(1, L.tblocks.size()) * sizeof(BlockEntry)), "allocate blocks");
     if (!L.vblocks.empty())
@@ -228,6 +229,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         HIP_TRY(c, c->lists_mem.ensure(size_t(want_pairs) * 4), "allocate bin lists");
         c->list_cap = uint32_t(want_pairs);
     }
+    HIP_TRY(c, c->items.ensure(size_t(raster_items_cap(L.total_bins, c->list_cap)) * sizeof(uint2)), "allocate work items");
     // outputs
     c->vis_out.resize(L.n_camera);
     c->color_out.resize(L.n_camera);
@@ -368,6 +370,9 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     b.list_cap = c->list_cap;
     b.shift = shift;
     b.scatter_blocks = 512;
+    b.items = c->items.as<uint2>();
+    b.items_cap = raster_items_cap(L.total_bins, c->list_cap);
+    b.split_bins = c->split_bins.as<uint32_t>();
     HIP_TRY(c, launch_binning(b, s), "binning kernels");
     if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[1], s), "event");
 
@@ -382,6 +387,9 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     r.list_cap = c->list_cap;
     r.recs = g.recs;
     r.rec_cap_total = L.rec_total;
+    r.items = b.items;
+    r.items_cap = b.items_cap;
+    r.totals = b.totals;
     HIP_TRY(c, launch_raster(r, s), "raster kernel");
     if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[2], s), "event");
 
@@ -510,6 +518,8 @@ rsy_result finish_frame(rsy_ctx* c) {
             if (want <= c->list_cap) return fail(c, RSY_OVERFLOW, "bin lists exceed 2^32 entries");
             HIP_TRY(c, c->lists_mem.ensure(size_t(want) * 4), "grow bin lists");
             c->list_cap = uint32_t(want);
+            HIP_TRY(c, c->items.ensure(size_t(raster_items_cap(c->layout.total_bins, c->list_cap)) * sizeof(uint2)),
+                    "grow work items");
             invalidate_graph(c);
         }
         rsy_result r = render_now(c, c->last_views, c->last_shade);

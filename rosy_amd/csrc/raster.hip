@@ -13,10 +13,10 @@
 //   k_clip       per clipped tri    near/far/guard-band Sutherland-Hodgman + fan, one wave each
 //   k_scan       all bins           single-workgroup exclusive prefix sum (+ per-frame statistics)
 //   k_scatter    per record         bin lists, slots handed out by the same ballot aggregation
-//   k_raster     per (view, bin)    one workgroup per 32x32 or 64x64 bin, the bin's depth/ID tile in
-//                                   LDS: small triangles walk their own pixels, larger ones are
-//                                   covered by the whole workgroup; LDS atomic max resolves depth and
-//                                   the bin is written to HBM once (the clear never touches HBM)
+//   k_raster     per work item      one workgroup per bin (or per 256 records of a crowded bin), the
+//                                   bin's depth/ID tile in LDS: every record's rows become tasks
+//                                   spread over the workgroup; LDS atomic max resolves depth and the
+//                                   tile goes to HBM once (crowded bins merge with device atomic max)
 // Every floating-point step is compiled without contraction; the CPU oracle restates the same
 // rules, so visibility and depth are bit-exact by construction.
 #include <hip/hip_runtime.h>
@@ -439,29 +439,73 @@ __global__ __launch_bounds__(256) void k_clip(const ViewGPU* views, const DrawGP
 constexpr uint32_t kScanThreads = 1024;
 constexpr uint32_t kScanPerThread = 4;
 
+// Raster work items: a bin with at most kChunkRecs records is one item (its tile is stored
+// whole); a larger bin is split into one item per kChunkRecs records, and its items merge into
+// the output with device atomic max after k_scatter has zeroed the bin's pixels.  Max is order
+// independent, so the result is the same whichever item finishes first.  The table holds the split
+// bins' items first (padded to a multiple of 8), then one item per remaining bin in bin order.
+constexpr uint32_t kChunkRecs = 256;
+constexpr uint32_t kNoBin = 0xffffffffu;
+
 __global__ __launch_bounds__(1024) void k_scan(const uint32_t* counts, uint32_t n, uint32_t* offsets,
                                                const uint32_t* block_valid, uint32_t n_blocks,
-                                               uint32_t list_cap, uint32_t* totals) {
+                                               uint32_t list_cap, uint32_t* totals, uint2* items,
+                                               uint32_t items_cap, uint32_t* split_bins) {
     __shared__ uint32_t lds[16];
-    uint32_t carry = 0;
+    uint32_t carry = 0, carry_items = 0, carry_split = 0;
     for (uint32_t base = 0; base < n; base += kScanThreads * kScanPerThread) {
         const uint32_t i0 = base + threadIdx.x * kScanPerThread;
-        uint32_t v[kScanPerThread];
-        uint32_t s = 0;
+        uint32_t v[kScanPerThread], sp[kScanPerThread];
+        uint32_t s = 0, si = 0, sb = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kScanPerThread; ++k) {
             v[k] = i0 + k < n ? counts[i0 + k] : 0u;
+            sp[k] = v[k] > kChunkRecs ? (v[k] + kChunkRecs - 1) / kChunkRecs : 0u;
             s += v[k];
+            si += sp[k];
+            sb += sp[k] ? 1u : 0u;
         }
-        uint32_t total;
+        uint32_t total, total_items, total_split;
         uint32_t run = carry + block_exclusive_scan(s, lds, total);
+        uint32_t run_items = carry_items + block_exclusive_scan(si, lds, total_items);
+        uint32_t run_split = carry_split + block_exclusive_scan(sb, lds, total_split);
 #pragma unroll
         for (uint32_t k = 0; k < kScanPerThread; ++k)
             if (i0 + k < n) {
                 offsets[i0 + k] = run;
                 run += v[k];
+                if (sp[k]) {
+                    split_bins[run_split++] = i0 + k;
+                    for (uint32_t c = 0; c < sp[k]; ++c, ++run_items)
+                        if (run_items < items_cap) items[run_items] = make_uint2(i0 + k, c);
+                }
             }
         carry += total;
+        carry_items += total_items;
+        carry_split += total_split;
+    }
+    const uint32_t split_region = (carry_items + 7u) & ~7u;
+    for (uint32_t i = carry_items + threadIdx.x; i < split_region && i < items_cap; i += kScanThreads)
+        items[i] = make_uint2(kNoBin, 0u);
+    uint32_t carry_whole = 0;
+    for (uint32_t base = 0; base < n; base += kScanThreads * kScanPerThread) {
+        const uint32_t i0 = base + threadIdx.x * kScanPerThread;
+        uint32_t w[kScanPerThread];
+        uint32_t sw = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPerThread; ++k) {
+            w[k] = (i0 + k < n && counts[i0 + k] <= kChunkRecs) ? 1u : 0u;
+            sw += w[k];
+        }
+        uint32_t total_whole;
+        uint32_t run = split_region + carry_whole + block_exclusive_scan(sw, lds, total_whole);
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPerThread; ++k)
+            if (w[k]) {
+                if (run < items_cap) items[run] = make_uint2(i0 + k, 0u);
+                ++run;
+            }
+        carry_whole += total_whole;
     }
     uint32_t vs = 0;
     for (uint32_t i = threadIdx.x; i < n_blocks; i += kScanThreads) vs += block_valid[i];
@@ -470,16 +514,21 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t* counts, uint32_t 
     if (threadIdx.x == 0) {
         totals[0] = carry;
         totals[1] = total_valid;
-        totals[2] = carry > list_cap ? 1u : 0u;
+        totals[2] = (carry > list_cap || split_region + carry_whole > items_cap) ? 1u : 0u;
+        totals[3] = split_region;
+        totals[4] = carry_whole;
+        totals[5] = carry_split;
     }
 }
 
 // Bin lists without atomics: every (record, bin) pair already holds its slot inside the bin (from
-// k_setup's block table, or the big-pair list), so the entry goes to bin_offsets[bin] + slot.
-__global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, const TriRec* recs, const uint32_t* counters,
-                                                 const uint32_t* bin_offsets, const uint32_t* pair_slots,
-                                                 const uint4* big_pairs, uint32_t* lists, uint32_t list_cap,
-                                                 uint32_t shift) {
+// k_setup's block table, or the big-pair list), so the entry goes to bin_offsets[bin] + slot.  The
+// same launch zeroes the output pixels of the split bins, which k_raster then merges into.
+__global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t n_views, const TriRec* recs,
+                                                 const uint32_t* counters, const uint32_t* bin_offsets,
+                                                 const uint32_t* pair_slots, const uint4* big_pairs, uint32_t* lists,
+                                                 uint32_t list_cap, uint32_t shift, const uint32_t* totals,
+                                                 const uint32_t* split_bins) {
     const uint32_t view = blockIdx.y;
     const ViewGPU& V = views[view];
     const uint32_t nbig = min(counters[view * kCtrPerView + kCtrBig], V.big_cap);
@@ -509,21 +558,41 @@ __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, const Tri
             if (pos < list_cap) lists[pos] = bp.z;
         }
     }
+    // zero the split bins' pixels
+    const uint32_t T = 1u << shift, px = T * T;
+    const uint64_t work = uint64_t(totals[5]) * px;
+    const uint64_t stride = uint64_t(gridDim.x) * gridDim.y * blockDim.x;
+    for (uint64_t w = (uint64_t(blockIdx.y) * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x; w < work; w += stride) {
+        const uint32_t bin = split_bins[w / px], p = uint32_t(w % px);
+        uint32_t v = 0;
+        while (v + 1 < n_views && views[v + 1].tile_base <= bin) ++v;
+        const ViewGPU& W = views[v];
+        const uint32_t local = bin - W.tile_base;
+        const uint32_t x = (local % W.tiles_x) * T + p % T, y = (local / W.tiles_x) * T + p / T;
+        if (x < W.width && y < W.height) {
+            const size_t o = size_t(y) * W.width + x;
+            if (W.is_shadow) reinterpret_cast<uint32_t*>(W.out)[o] = 0u;
+            else reinterpret_cast<unsigned long long*>(W.out)[o] = 0ull;
+        }
+    }
 }
 
 // One row of a record inside the bin: E_k(x) for the three edges advances by 256 * A_k per pixel
 // (exact integer steps; EdgeT = int32_t when the triangle's box is under 64 pixels on each side, so
 // every |E| < 2^30, else int64_t), depth is the FP32 plane rule of setup, and covered pixels merge
-// into the LDS row with atomic max.
+// into the LDS row with atomic max.  The top-left rule is folded into the edge value:
+// F_k = E_k - 1 + tl_k, so "E_k > 0, or E_k == 0 on a top-left edge" is F_k >= 0 for integer E_k,
+// and a pixel is covered when (F_0 | F_1 | F_2) has a clear sign bit -- one test, no branches per
+// edge.
 template <typename EdgeT, int STRIDE>
 __device__ inline void raster_row(const TriRec& r, const Edges& ed, int ox, int oy, int y, int bx0, int bx1,
                                   bool shadow, unsigned long long* tile) {
     const int32_t Py = (oy + y) * 256 + 128;
     int32_t Px = (ox + bx0) * 256 + 128;
-    EdgeT e[3], step[3];
+    EdgeT f[3], step[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        e[k] = EdgeT(ed.A[k]) * EdgeT(Px - ed.X[k]) + EdgeT(ed.B[k]) * EdgeT(Py - ed.Y[k]);
+        f[k] = EdgeT(ed.A[k]) * EdgeT(Px - ed.X[k]) + EdgeT(ed.B[k]) * EdgeT(Py - ed.Y[k]) - EdgeT(ed.tl[k] ? 0 : 1);
         step[k] = EdgeT(ed.A[k]) * EdgeT(256);
     }
     const float fy = float(Py - r.Y0);
@@ -531,17 +600,14 @@ __device__ inline void raster_row(const TriRec& r, const Edges& ed, int ox, int 
     const int sh = shadow ? 0 : 32;
     unsigned long long* row = tile + y * STRIDE;
     for (int x = bx0; x <= bx1; ++x, Px += 256) {
-        bool inside = true;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) inside &= e[k] > 0 || (e[k] == 0 && ed.tl[k]);
-        if (inside) {
+        if ((f[0] | f[1] | f[2]) >= 0) {
             float z = fmaf(r.dzdy, fy, fmaf(r.dzdx, float(Px - r.X0), r.z0));
             z = fminf(fmaxf(z, 0.0f), 1.0f);
             const uint32_t zb = fbits(z) & 0x7fffffffu;
             atomicMax(&row[x], ((unsigned long long)zb << sh) | idw);
         }
 #pragma unroll
-        for (int k = 0; k < 3; ++k) e[k] += step[k];
+        for (int k = 0; k < 3; ++k) f[k] += step[k];
     }
 }
 
@@ -554,10 +620,10 @@ __device__ inline void raster_row(const TriRec& r, const Edges& ed, int ox, int 
 // the later triangle wins ties).  Workgroups are mapped so that neighbouring bins run on the same
 // XCD (workgroups are dealt round-robin over the 8 XCDs, each with its own L2).
 template <int T>
-__global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n_views, uint32_t total_bins,
-                                                const uint32_t* bin_counts, const uint32_t* bin_offsets,
-                                                const uint32_t* lists, uint32_t list_cap, const TriRec* recs,
-                                                uint64_t rec_cap_total) {
+__global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n_views, const uint32_t* bin_counts,
+                                                const uint32_t* bin_offsets, const uint32_t* lists, uint32_t list_cap,
+                                                const TriRec* recs, uint64_t rec_cap_total, const uint2* items,
+                                                uint32_t items_cap, const uint32_t* totals) {
     constexpr int S = T + 1;  // padded row stride (64-bit words)
     __shared__ unsigned long long tile[T * S];
     __shared__ TriRec chunk_recs[256];
@@ -565,20 +631,27 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
     __shared__ uint32_t row_start[257];   // exclusive prefix of row counts
     __shared__ uint32_t scan_lds[4];
 
-    const uint32_t groups = (total_bins + 7) / 8;
-    const uint32_t bin = (blockIdx.x % 8) * groups + blockIdx.x / 8;
-    if (bin >= total_bins) return;
+    // split items first (the heaviest work starts first); workgroups are dealt round-robin over the
+    // 8 XCDs, so consecutive items -- neighbouring bins -- spread the dense regions of a view over
+    // every XCD
+    const uint32_t item = blockIdx.x;
+    if (item >= totals[3] + totals[4] || item >= items_cap) return;
+    const uint2 it = items[item];
+    if (it.x == kNoBin) return;
+    const uint32_t bin = it.x;
     uint32_t view = 0;
     while (view + 1 < n_views && views[view + 1].tile_base <= bin) ++view;
     const ViewGPU& V = views[view];
     const uint32_t local_bin = bin - V.tile_base;
     const int ox = int(local_bin % V.tiles_x) * T, oy = int(local_bin / V.tiles_x) * T;
     const bool shadow = V.is_shadow != 0;
+    const uint32_t count = bin_counts[bin];
+    const bool split = count > kChunkRecs;
 
     for (int p = threadIdx.x; p < T * S; p += 256) tile[p] = 0ull;
 
-    const uint32_t off = bin_offsets[bin];
-    const uint32_t end = min(off + bin_counts[bin], list_cap);
+    const uint32_t off = bin_offsets[bin] + it.y * kChunkRecs;
+    const uint32_t end = min(min(bin_offsets[bin] + count, off + kChunkRecs), list_cap);
     for (uint32_t base = off; base < end; base += 256) {
         const uint32_t i = base + threadIdx.x;
         uint32_t rows = 0;
@@ -631,8 +704,13 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
         if (x < int(V.width) && y < int(V.height)) {
             const size_t o = size_t(y) * V.width + x;
             const unsigned long long w = tile[(p / T) * S + p % T];
-            if (shadow) reinterpret_cast<uint32_t*>(V.out)[o] = uint32_t(w);
-            else reinterpret_cast<unsigned long long*>(V.out)[o] = w;
+            if (!split) {
+                if (shadow) reinterpret_cast<uint32_t*>(V.out)[o] = uint32_t(w);
+                else reinterpret_cast<unsigned long long*>(V.out)[o] = w;
+            } else if (w != 0ull) {
+                if (shadow) atomicMax(reinterpret_cast<uint32_t*>(V.out) + o, uint32_t(w));
+                else atomicMax(reinterpret_cast<unsigned long long*>(V.out) + o, w);
+            }
         }
     }
 }
@@ -666,26 +744,34 @@ hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
 
 hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, b.bin_offsets,
-                       b.block_valid, b.n_tblocks, b.list_cap, b.totals);
-    hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.recs, b.counters,
-                       b.bin_offsets, b.pair_slots, b.big_pairs, b.lists, b.list_cap, b.shift);
+                       b.block_valid, b.n_tblocks, b.list_cap, b.totals, b.items, b.items_cap, b.split_bins);
+    hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.n_views, b.recs,
+                       b.counters, b.bin_offsets, b.pair_slots, b.big_pairs, b.lists, b.list_cap, b.shift, b.totals,
+                       b.split_bins);
     return hipGetLastError();
 }
 
 hipError_t launch_raster(const RasterLaunch& r, hipStream_t s) {
-    const uint32_t grid = ((r.total_bins + 7) / 8) * 8;
+    // one workgroup per work item; the table size bounds the item count (extra workgroups exit)
+    const uint32_t grid = r.items_cap;
     switch (r.tile) {
         case 32:
-            hipLaunchKernelGGL(k_raster<32>, dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.total_bins,
-                               r.bin_counts, r.bin_offsets, r.lists, r.list_cap, r.recs, r.rec_cap_total);
+            hipLaunchKernelGGL(k_raster<32>, dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.bin_counts,
+                               r.bin_offsets, r.lists, r.list_cap, r.recs, r.rec_cap_total, r.items, r.items_cap,
+                               r.totals);
             break;
         case 64:
-            hipLaunchKernelGGL(k_raster<64>, dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.total_bins,
-                               r.bin_counts, r.bin_offsets, r.lists, r.list_cap, r.recs, r.rec_cap_total);
+            hipLaunchKernelGGL(k_raster<64>, dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.bin_counts,
+                               r.bin_offsets, r.lists, r.list_cap, r.recs, r.rec_cap_total, r.items, r.items_cap,
+                               r.totals);
             break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+uint32_t raster_items_cap(uint32_t total_bins, uint32_t list_cap) {
+    return total_bins + (list_cap + kChunkRecs - 1) / kChunkRecs + 8;
 }
 
 }  // namespace rsy
