@@ -188,12 +188,13 @@ def tile_counts(scene, view_projection, width, height, shadow, raster_state, til
     vp = (C.c_float * 16)(*view_projection)
     shape = ((height + tile - 1) // tile, (width + tile - 1) // tile)
     out = np.zeros(shape, dtype=np.uint32)
-    area = np.zeros(shape, dtype=np.uint64)
+    area = np.zeros(shape[0] * shape[1] + 2, dtype=np.uint64)
     rc = lib().orc_tile_counts(C.byref(g), vp, width, height, 1 if shadow else 0, C.byref(raster_state), tile,
                                out.ctypes.data, area.ctypes.data, n_threads or threads())
     if rc != 0:
         raise RuntimeError(f"orc_tile_counts failed: {rc}")
-    return out, area
+    tile_counts.wide = (int(area[-2]), int(area[-1]))  # records over 4 bins, their pairs
+    return out, area[:-2].reshape(shape)
 
 
 def shade_view(scene, view, cascades, shadow_maps, vis, scene_data=None, raster_state=None, override_shadow=None,

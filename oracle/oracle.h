@@ -69,7 +69,8 @@ uint64_t orc_count_tile_pairs(const orc_geometry* g, const float view_projection
                               int n_threads);
 
 /* Records per tile x tile bin (row-major, ceil(w/tile) x ceil(h/tile)) for the same binning, and
- * (nullable) the pixels of the records' boxes inside each bin. */
+ * (nullable) the pixels of the records' boxes inside each bin, followed by two totals: records
+ * touching more than 4 bins and their (record, bin) pairs. */
 int orc_tile_counts(const orc_geometry* g, const float view_projection[16], uint32_t width, uint32_t height,
                     int is_shadow, const rsy_raster_state* rs, uint32_t tile, uint32_t* counts, uint64_t* areas,
                     int n_threads);

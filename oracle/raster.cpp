@@ -326,9 +326,11 @@ int orc_tile_counts(const orc_geometry* g, const float view_projection[16], uint
     if (build_records(g, view_projection, p, recs, nullptr, n_threads) != RSY_OK) return RSY_ERROR;
     const uint32_t tx = (width + tile - 1) / tile, ty = (height + tile - 1) / tile;
     std::fill(counts, counts + size_t(tx) * ty, 0u);
-    if (areas) std::fill(areas, areas + size_t(tx) * ty, uint64_t(0));
+    if (areas) std::fill(areas, areas + size_t(tx) * ty + 2, uint64_t(0));
     const int32_t T = int32_t(tile);
-    for (const Rec& r : recs)
+    for (const Rec& r : recs) {
+        const uint64_t nb = uint64_t(r.px1 / T - r.px0 / T + 1) * uint64_t(r.py1 / T - r.py0 / T + 1);
+        if (nb > 4 && areas) areas[size_t(tx) * ty] += 1, areas[size_t(tx) * ty + 1] += nb;
         for (int32_t y = r.py0 / T; y <= r.py1 / T; ++y)
             for (int32_t x = r.px0 / T; x <= r.px1 / T; ++x) {
                 ++counts[size_t(y) * tx + x];
@@ -338,6 +340,7 @@ int orc_tile_counts(const orc_geometry* g, const float view_projection[16], uint
                     areas[size_t(y) * tx + x] += uint64_t(w) * uint64_t(h);
                 }
             }
+    }
     return RSY_OK;
 }
 

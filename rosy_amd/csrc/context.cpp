@@ -214,6 +214,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     HIP_TRY(c, c->block_valid.ensure(std::max<size_t>(1, L.tblocks.size()) * 4), "allocate statistics");
     HIP_TRY(c, c->totals.ensure(32), "allocate statistics");
     HIP_TRY(c, c->split_bins.ensure(size_t(std::max<uint32_t>(1, L.total_bins)) * 4), "allocate bins");
+    HIP_TRY(c, c->whole_items.ensure(size_t(std::max<uint32_t>(1, L.total_bins)) * 4), "allocate bins");
     HIP_TRY(c, c->vblocks_d.ensure(std::max<size_t>(1, L.vblocks.size()) * sizeof(BlockEntry)), "allocate blocks");
     HIP_TRY(c, c->tblocks_d.ensure(std::max<size_t>(1, L.tblocks.size()) * sizeof(BlockEntry)), "allocate blocks");
     if (!L.vblocks.empty())
@@ -369,9 +370,12 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     b.lists = c->lists_mem.as<uint32_t>();
     b.list_cap = c->list_cap;
     b.shift = shift;
-    b.scatter_blocks = 512;
-    b.items = c->items.as<uint2>();
+    uint32_t max_tris = 0;
+    for (const auto& v : L.vg) max_tris = std::max(max_tris, v.tri_count);
+    b.scatter_blocks = std::max<uint32_t>(64, std::min<uint32_t>(4096, (max_tris + 255) / 256));
+    b.split_items = c->items.as<uint2>();
     b.items_cap = raster_items_cap(L.total_bins, c->list_cap);
+    b.whole_items = c->whole_items.as<uint32_t>();
     b.split_bins = c->split_bins.as<uint32_t>();
     HIP_TRY(c, launch_binning(b, s), "binning kernels");
     if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[1], s), "event");
@@ -387,8 +391,9 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     r.list_cap = c->list_cap;
     r.recs = g.recs;
     r.rec_cap_total = L.rec_total;
-    r.items = b.items;
+    r.split_items = b.split_items;
     r.items_cap = b.items_cap;
+    r.whole_items = b.whole_items;
     r.totals = b.totals;
     HIP_TRY(c, launch_raster(r, s), "raster kernel");
     if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[2], s), "event");

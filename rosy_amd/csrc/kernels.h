@@ -44,16 +44,17 @@ struct BinLaunch {
     const uint32_t* block_valid;
     uint32_t n_tblocks;
     uint32_t* totals;  // [0] (record, bin) pairs, [1] records set up, [2] bin lists or items overflowed,
-                       // [3] split-item region size, [4] whole-bin items, [5] split bins
+                       // [3] split items, [4] whole-bin items, [5] split bins
     const TriRec* recs;
     const uint32_t* counters;
     uint32_t* lists;
     uint32_t list_cap;
     uint32_t shift;
     uint32_t scatter_blocks;
-    uint2* items;          // raster work items (bin, chunk)
-    uint32_t items_cap;
-    uint32_t* split_bins;  // bins split over several items
+    uint2* split_items;     // raster work items of crowded bins: (bin, chunk)
+    uint32_t items_cap;     // split_items capacity
+    uint32_t* whole_items;  // bins rasterised by one workgroup (total_bins entries)
+    uint32_t* split_bins;   // bins split over several items
 };
 
 struct RasterLaunch {
@@ -67,8 +68,9 @@ struct RasterLaunch {
     uint32_t list_cap;
     const TriRec* recs;
     uint64_t rec_cap_total;
-    const uint2* items;
+    const uint2* split_items;
     uint32_t items_cap;
+    const uint32_t* whole_items;
     const uint32_t* totals;
 };
 
@@ -202,7 +204,7 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
 hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s);
 hipError_t launch_binning(const BinLaunch& b, hipStream_t s);
 hipError_t launch_raster(const RasterLaunch& r, hipStream_t s);
-uint32_t raster_items_cap(uint32_t total_bins, uint32_t list_cap);  // work-item table entries
+uint32_t raster_items_cap(uint32_t total_bins, uint32_t list_cap);  // split work-item table entries
 hipError_t launch_shade(const ShadeLaunch& l, hipStream_t s);
 hipError_t launch_nm_init(const NmInitLaunch& l, hipStream_t s);
 hipError_t launch_nm_step(const NmTrainLaunch& l, bool explicit_batch, hipStream_t s);

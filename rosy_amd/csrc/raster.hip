@@ -230,27 +230,47 @@ __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, con
 }
 
 // Bins of a record with at most kSmallBins bins go through the block's LDS table in k_setup; the
-// others (and every clipper record) are binned by the whole wave with one global atomic per bin,
-// their (bin, slot, record) triples appended to the view's big-pair list.
+// others (and every clipper record) are binned by the whole wave: a wave prefix over their bin
+// counts numbers the wave's (record, bin) pairs, one atomic reserves that many entries of the view's
+// big-pair list, and every lane takes pairs in turn (its record found by a binary search over the
+// prefix with lane shuffles), with one global atomic per pair for the slot inside the bin.
 __device__ void bin_wide(bool active, const TriRec& r, uint32_t ri, const ViewGPU& V, uint32_t shift,
                          uint32_t* bin_counts, uint4* big_pairs, uint32_t* ctr) {
     int tx0 = 0, ty0 = 0, tx1 = -1, ty1 = -1;
     if (active) rec_tiles(r, shift, tx0, ty0, tx1, ty1);
-    uint64_t todo = __ballot(active);
-    while (todo) {
-        const int l = __ffsll((unsigned long long)todo) - 1;
-        todo &= todo - 1;
-        const int btx0 = __builtin_amdgcn_readlane(tx0, l), bty0 = __builtin_amdgcn_readlane(ty0, l);
-        const int btx1 = __builtin_amdgcn_readlane(tx1, l), bty1 = __builtin_amdgcn_readlane(ty1, l);
-        const uint32_t bri = uint32_t(__builtin_amdgcn_readlane(int(ri), l));
-        const int bw = btx1 - btx0 + 1, bn = bw * (bty1 - bty0 + 1);
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(&ctr[kCtrBig], uint32_t(bn));
-        base = uint32_t(__builtin_amdgcn_readlane(int(base), 0));
-        for (int k = int(lane_id()); k < bn; k += 64) {
-            const uint32_t key = V.tile_base + uint32_t((bty0 + k / bw) * int(V.tiles_x) + btx0 + k % bw);
+    const int bw = tx1 - tx0 + 1;
+    const uint32_t bn = active ? uint32_t(bw * (ty1 - ty0 + 1)) : 0u;
+    const uint32_t lane = lane_id();
+    uint32_t incl = bn;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off);
+        if (int(lane) >= off) incl += y;
+    }
+    const uint32_t total = uint32_t(__builtin_amdgcn_readlane(int(incl), 63));
+    if (total == 0) return;
+    const uint32_t excl = incl - bn;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&ctr[kCtrBig], total);
+    base = uint32_t(__builtin_amdgcn_readlane(int(base), 0));
+    for (uint32_t p0 = 0; p0 < total; p0 += 64) {
+        const uint32_t p = p0 + lane;
+        // owner: the last lane whose exclusive prefix is <= p (lanes without pairs share the
+        // prefix of the next lane that has some, and come before it)
+        int lo = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1) {
+            const uint32_t e = uint32_t(__shfl(int(excl), lo + step));
+            if (e <= p) lo += step;
+        }
+        const uint32_t o_excl = uint32_t(__shfl(int(excl), lo));
+        const int o_tx0 = __shfl(tx0, lo), o_ty0 = __shfl(ty0, lo), o_bw = __shfl(bw, lo);
+        const uint32_t o_ri = uint32_t(__shfl(int(ri), lo));
+        if (p < total) {
+            const int k = int(p - o_excl);
+            const uint32_t key = V.tile_base + uint32_t((o_ty0 + k / o_bw) * int(V.tiles_x) + o_tx0 + k % o_bw);
             const uint32_t local = atomicAdd(&bin_counts[key], 1u);
-            if (base + uint32_t(k) < V.big_cap) big_pairs[V.big_base + base + k] = make_uint4(key, local, bri, 0u);
+            if (base + p < V.big_cap) big_pairs[V.big_base + base + p] = make_uint4(key, local, o_ri, 0u);
             else atomicOr(&ctr[kCtrOverflow], 4u);
         }
     }
@@ -442,82 +462,97 @@ constexpr uint32_t kScanPerThread = 4;
 // Raster work items: a bin with at most kChunkRecs records is one item (its tile is stored
 // whole); a larger bin is split into one item per kChunkRecs records, and its items merge into
 // the output with device atomic max after k_scatter has zeroed the bin's pixels.  Max is order
-// independent, so the result is the same whichever item finishes first.  The table holds the split
-// bins' items first (padded to a multiple of 8), then one item per remaining bin in bin order.
+// independent, so the result is the same whichever item finishes first.  Split items and whole
+// bins go to two tables (split_items, whole_items) in one pass.
 constexpr uint32_t kChunkRecs = 256;
-constexpr uint32_t kNoBin = 0xffffffffu;
 
+__device__ inline uint64_t block_exclusive_scan64(uint64_t v, uint64_t* lds_waves, uint64_t& total) {
+    uint64_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t y = __shfl_up(x, off);
+        if (int(lane_id()) >= off) x += y;
+    }
+    const uint32_t wave = threadIdx.x / 64, nwaves = blockDim.x / 64;
+    if (lane_id() == 63) lds_waves[wave] = x;
+    __syncthreads();
+    uint64_t wave_off = 0, sum = 0;
+    for (uint32_t w = 0; w < nwaves; ++w) {
+        const uint64_t t = lds_waves[w];
+        if (w < wave) wave_off += t;
+        sum += t;
+    }
+    __syncthreads();
+    total = sum;
+    return wave_off + x - v;
+}
+
+// One pass over the bins: bin offsets (exclusive prefix of the counts) and the work-item tables.
+// Two block scans per pass: a 64-bit one of (record count | split items << 32) -- the counts of a
+// frame stay below 2^32, so the low half never carries -- and a 32-bit one of (split bins | whole
+// bins << 16), both at most 4096 per pass.
 __global__ __launch_bounds__(1024) void k_scan(const uint32_t* counts, uint32_t n, uint32_t* offsets,
                                                const uint32_t* block_valid, uint32_t n_blocks,
-                                               uint32_t list_cap, uint32_t* totals, uint2* items,
-                                               uint32_t items_cap, uint32_t* split_bins) {
+                                               uint32_t list_cap, uint32_t* totals, uint2* split_items,
+                                               uint32_t* whole_items, uint32_t items_cap, uint32_t* split_bins) {
     __shared__ uint32_t lds[16];
-    uint32_t carry = 0, carry_items = 0, carry_split = 0;
+    __shared__ uint64_t lds64[16];
+    static_assert(kScanThreads * kScanPerThread <= 65535, "bin fields of the 32-bit scan");
+    uint64_t carry = 0;
+    uint32_t c_split = 0, c_whole = 0;
+    uint32_t vs = 0;
+    for (uint32_t i = threadIdx.x; i < n_blocks; i += kScanThreads) vs += block_valid[i];
     for (uint32_t base = 0; base < n; base += kScanThreads * kScanPerThread) {
         const uint32_t i0 = base + threadIdx.x * kScanPerThread;
-        uint32_t v[kScanPerThread], sp[kScanPerThread];
-        uint32_t s = 0, si = 0, sb = 0;
+        uint32_t v[kScanPerThread];
+        uint64_t a = 0;
+        uint32_t b = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kScanPerThread; ++k) {
             v[k] = i0 + k < n ? counts[i0 + k] : 0u;
-            sp[k] = v[k] > kChunkRecs ? (v[k] + kChunkRecs - 1) / kChunkRecs : 0u;
-            s += v[k];
-            si += sp[k];
-            sb += sp[k] ? 1u : 0u;
+            if (i0 + k < n) {
+                if (v[k] > kChunkRecs) {
+                    a += uint64_t(v[k]) | (uint64_t((v[k] + kChunkRecs - 1) / kChunkRecs) << 32);
+                    b += 1u;
+                } else {
+                    a += v[k];
+                    b += 1u << 16;
+                }
+            }
         }
-        uint32_t total, total_items, total_split;
-        uint32_t run = carry + block_exclusive_scan(s, lds, total);
-        uint32_t run_items = carry_items + block_exclusive_scan(si, lds, total_items);
-        uint32_t run_split = carry_split + block_exclusive_scan(sb, lds, total_split);
+        uint64_t ta;
+        uint32_t tb;
+        const uint64_t ea = carry + block_exclusive_scan64(a, lds64, ta);
+        const uint32_t eb = block_exclusive_scan(b, lds, tb);
+        uint32_t run = uint32_t(ea), r_items = uint32_t(ea >> 32);
+        uint32_t r_split = c_split + (eb & 0xffffu), r_whole = c_whole + (eb >> 16);
 #pragma unroll
         for (uint32_t k = 0; k < kScanPerThread; ++k)
             if (i0 + k < n) {
                 offsets[i0 + k] = run;
                 run += v[k];
-                if (sp[k]) {
-                    split_bins[run_split++] = i0 + k;
-                    for (uint32_t c = 0; c < sp[k]; ++c, ++run_items)
-                        if (run_items < items_cap) items[run_items] = make_uint2(i0 + k, c);
+                if (v[k] > kChunkRecs) {
+                    split_bins[r_split++] = i0 + k;
+                    const uint32_t ni = (v[k] + kChunkRecs - 1) / kChunkRecs;
+                    for (uint32_t c = 0; c < ni; ++c, ++r_items)
+                        if (r_items < items_cap) split_items[r_items] = make_uint2(i0 + k, c);
+                } else {
+                    whole_items[r_whole++] = i0 + k;
                 }
             }
-        carry += total;
-        carry_items += total_items;
-        carry_split += total_split;
+        carry += ta;
+        c_split += tb & 0xffffu;
+        c_whole += tb >> 16;
     }
-    const uint32_t split_region = (carry_items + 7u) & ~7u;
-    for (uint32_t i = carry_items + threadIdx.x; i < split_region && i < items_cap; i += kScanThreads)
-        items[i] = make_uint2(kNoBin, 0u);
-    uint32_t carry_whole = 0;
-    for (uint32_t base = 0; base < n; base += kScanThreads * kScanPerThread) {
-        const uint32_t i0 = base + threadIdx.x * kScanPerThread;
-        uint32_t w[kScanPerThread];
-        uint32_t sw = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < kScanPerThread; ++k) {
-            w[k] = (i0 + k < n && counts[i0 + k] <= kChunkRecs) ? 1u : 0u;
-            sw += w[k];
-        }
-        uint32_t total_whole;
-        uint32_t run = split_region + carry_whole + block_exclusive_scan(sw, lds, total_whole);
-#pragma unroll
-        for (uint32_t k = 0; k < kScanPerThread; ++k)
-            if (w[k]) {
-                if (run < items_cap) items[run] = make_uint2(i0 + k, 0u);
-                ++run;
-            }
-        carry_whole += total_whole;
-    }
-    uint32_t vs = 0;
-    for (uint32_t i = threadIdx.x; i < n_blocks; i += kScanThreads) vs += block_valid[i];
     uint32_t total_valid;
     block_exclusive_scan(vs, lds, total_valid);
     if (threadIdx.x == 0) {
-        totals[0] = carry;
+        const uint32_t pairs = uint32_t(carry), n_items = uint32_t(carry >> 32);
+        totals[0] = pairs;
         totals[1] = total_valid;
-        totals[2] = (carry > list_cap || split_region + carry_whole > items_cap) ? 1u : 0u;
-        totals[3] = split_region;
-        totals[4] = carry_whole;
-        totals[5] = carry_split;
+        totals[2] = (pairs > list_cap || n_items > items_cap) ? 1u : 0u;
+        totals[3] = min(n_items, items_cap);
+        totals[4] = c_whole;
+        totals[5] = c_split;
     }
 }
 
@@ -622,8 +657,8 @@ __device__ inline void raster_row(const TriRec& r, const Edges& ed, int ox, int 
 template <int T>
 __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n_views, const uint32_t* bin_counts,
                                                 const uint32_t* bin_offsets, const uint32_t* lists, uint32_t list_cap,
-                                                const TriRec* recs, uint64_t rec_cap_total, const uint2* items,
-                                                uint32_t items_cap, const uint32_t* totals) {
+                                                const TriRec* recs, uint64_t rec_cap_total, const uint2* split_items,
+                                                const uint32_t* whole_items, const uint32_t* totals) {
     constexpr int S = T + 1;  // padded row stride (64-bit words)
     __shared__ unsigned long long tile[T * S];
     __shared__ TriRec chunk_recs[256];
@@ -634,10 +669,14 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
     // split items first (the heaviest work starts first); workgroups are dealt round-robin over the
     // 8 XCDs, so consecutive items -- neighbouring bins -- spread the dense regions of a view over
     // every XCD
-    const uint32_t item = blockIdx.x;
-    if (item >= totals[3] + totals[4] || item >= items_cap) return;
-    const uint2 it = items[item];
-    if (it.x == kNoBin) return;
+    const uint32_t n_split_items = totals[3];
+    uint2 it;
+    if (blockIdx.x < n_split_items) {
+        it = split_items[blockIdx.x];
+    } else {
+        if (blockIdx.x - n_split_items >= totals[4]) return;
+        it = make_uint2(whole_items[blockIdx.x - n_split_items], 0u);
+    }
     const uint32_t bin = it.x;
     uint32_t view = 0;
     while (view + 1 < n_views && views[view + 1].tile_base <= bin) ++view;
@@ -744,7 +783,8 @@ hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
 
 hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, b.bin_offsets,
-                       b.block_valid, b.n_tblocks, b.list_cap, b.totals, b.items, b.items_cap, b.split_bins);
+                       b.block_valid, b.n_tblocks, b.list_cap, b.totals, b.split_items, b.whole_items, b.items_cap,
+                       b.split_bins);
     hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.n_views, b.recs,
                        b.counters, b.bin_offsets, b.pair_slots, b.big_pairs, b.lists, b.list_cap, b.shift, b.totals,
                        b.split_bins);
@@ -752,18 +792,19 @@ hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
 }
 
 hipError_t launch_raster(const RasterLaunch& r, hipStream_t s) {
-    // one workgroup per work item; the table size bounds the item count (extra workgroups exit)
-    const uint32_t grid = r.items_cap;
+    // one workgroup per work item: split items first, then whole bins; the split-item table size
+    // plus the bin count bounds the item count (extra workgroups exit)
+    const uint32_t grid = r.items_cap + r.total_bins;
     switch (r.tile) {
         case 32:
             hipLaunchKernelGGL(k_raster<32>, dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.bin_counts,
-                               r.bin_offsets, r.lists, r.list_cap, r.recs, r.rec_cap_total, r.items, r.items_cap,
-                               r.totals);
+                               r.bin_offsets, r.lists, r.list_cap, r.recs, r.rec_cap_total, r.split_items,
+                               r.whole_items, r.totals);
             break;
         case 64:
             hipLaunchKernelGGL(k_raster<64>, dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.bin_counts,
-                               r.bin_offsets, r.lists, r.list_cap, r.recs, r.rec_cap_total, r.items, r.items_cap,
-                               r.totals);
+                               r.bin_offsets, r.lists, r.list_cap, r.recs, r.rec_cap_total, r.split_items,
+                               r.whole_items, r.totals);
             break;
         default: return hipErrorInvalidValue;
     }
@@ -771,7 +812,7 @@ hipError_t launch_raster(const RasterLaunch& r, hipStream_t s) {
 }
 
 uint32_t raster_items_cap(uint32_t total_bins, uint32_t list_cap) {
-    return total_bins + (list_cap + kChunkRecs - 1) / kChunkRecs + 8;
+    return total_bins + (list_cap + kChunkRecs - 1) / kChunkRecs;
 }
 
 }  // namespace rsy
