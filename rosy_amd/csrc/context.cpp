@@ -49,6 +49,10 @@ rsy_result rebuild_lists(rsy_ctx* c) {
         d.vtx_count = m.num_vertices;
         d.object = uint32_t(obj);
         d.material = s.material_index < 0xFFFFFFFFull ? uint32_t(s.material_index) : 0xFFFFFFFFu;
+        for (int k = 0; k < 3; ++k) {
+            d.bmin[k] = c->mesh_bounds[size_t(s.mesh_index) * 6 + k];
+            d.bmax[k] = c->mesh_bounds[size_t(s.mesh_index) * 6 + 3 + k];
+        }
         const int lists_of[2] = {0, s.blended ? 2 : 1};
         for (int l : lists_of) {
             d.tri_base = c->list_tris[l];
@@ -154,6 +158,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     L.total_xv = 0;
     L.rec_total = L.clip_total = L.big_total = 0;
     L.n_shadow = L.n_camera = 0;
+    uint32_t cull_total = 0;
     const uint32_t shift = ilog2(tile);
     for (uint32_t vi = 0; vi < views.size(); ++vi) {
         const FrameViewDesc& f = views[vi];
@@ -168,6 +173,8 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         g.list = f.shadow ? 0 : 1;
         const auto& list = c->lists[g.list];
         g.n_draws = uint32_t(list.size());
+        g.cull_base = cull_total;
+        cull_total += g.n_draws;
         g.xv_base = uint32_t(L.total_xv);
         const uint32_t tris = c->list_tris[g.list];
         const uint32_t clip_cap = std::max<uint32_t>(4096, tris / 8) * c->rec_scale;
@@ -207,6 +214,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     HIP_TRY(c, c->recs.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * sizeof(TriRec)), "allocate records");
     HIP_TRY(c, c->clipq.ensure(size_t(std::max<uint32_t>(1, L.clip_total)) * sizeof(uint2)), "allocate clip queue");
     HIP_TRY(c, c->counters.ensure(nv * kCtrPerView * sizeof(uint32_t)), "allocate counters");
+    HIP_TRY(c, c->draw_culled.ensure(size_t(std::max<uint32_t>(1, cull_total)) * 4), "allocate culling flags");
     HIP_TRY(c, c->bin_counts.ensure(size_t(L.total_bins) * 4), "allocate bins");
     HIP_TRY(c, c->bin_offsets.ensure(size_t(L.total_bins) * 4), "allocate bins");
     HIP_TRY(c, c->pair_slots.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * 16), "allocate bin slots");
@@ -348,8 +356,9 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     g.pair_slots = c->pair_slots.as<uint32_t>();
     g.big_pairs = c->big_pairs.as<uint4>();
     g.block_valid = c->block_valid.as<uint32_t>();
+    g.draw_culled = c->draw_culled.as<uint32_t>();
     g.shift = shift;
-    g.clip_blocks = 64;
+    g.clip_blocks = 128;
     g.total_bins = L.total_bins;
     HIP_TRY(c, launch_geometry(g, s), "geometry kernels");
     if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[0], s), "event");
@@ -641,6 +650,22 @@ rsy_result rsy_set_geometry(rsy_ctx* c, const rsy_position* vertices, uint64_t n
     c->n_vertices = n_vertices;
     c->n_indices = n_indices;
     c->meshes.assign(meshes, meshes + n_meshes);
+    c->mesh_bounds.assign(size_t(n_meshes) * 6, 0.f);
+    for (uint32_t m = 0; m < n_meshes; ++m) {
+        const uint64_t vb = meshes[m].vertex_buffer_offset / sizeof(rsy_position);
+        float* b = &c->mesh_bounds[size_t(m) * 6];
+        for (int k = 0; k < 3; ++k) {
+            b[k] = INFINITY;
+            b[3 + k] = -INFINITY;
+        }
+        for (uint64_t v = vb; v < vb + meshes[m].num_vertices; ++v)
+            for (int k = 0; k < 3; ++k) {
+                b[k] = std::min(b[k], vertices[v].vertex[k]);
+                b[3 + k] = std::max(b[3 + k], vertices[v].vertex[k]);
+            }
+        if (meshes[m].num_vertices == 0)
+            for (int k = 0; k < 6; ++k) b[k] = 0.f;  // empty mesh: any box (no triangles to cull)
+    }
     c->has_geometry = true;
     c->lists_dirty = true;
     c->layout.valid = false;

@@ -141,6 +141,7 @@ struct rsy_ctx {
     rsy::DevBuf vertices, px, py, pz, indices;
     uint64_t n_vertices = 0, n_indices = 0;
     std::vector<rsy_mesh_range> meshes;
+    std::vector<float> mesh_bounds;  // per mesh: object-space min xyz, max xyz of its vertices
     bool has_geometry = false;
     // objects + surfaces
     rsy::DevBuf objects;
@@ -172,7 +173,8 @@ struct rsy_ctx {
     // frame buffers
     rsy::Layout layout;
     rsy::DevBuf views_d, xv, recs, clipq, counters, bin_counts, bin_offsets, pair_slots, big_pairs, block_valid, totals,
-        lists_mem, vblocks_d, tblocks_d, shade_views_d, shade_cascades_d, xv_bases_d, items, split_bins, whole_items;
+        lists_mem, vblocks_d, tblocks_d, shade_views_d, shade_cascades_d, xv_bases_d, items, split_bins, whole_items,
+        draw_culled;
     uint32_t list_cap = 0;
     uint32_t rec_scale = 1;  // grows on overflow
     std::vector<rsy::DevBuf> vis_out, color_out, shadow_factor_out, shadow_maps;

@@ -27,6 +27,7 @@ struct DrawGPU {
     uint32_t xv_off;       // offset of this draw's transformed vertices inside a view's block
     uint32_t object;       // graphic_objects_offset + graphics_object_index
     uint32_t material;     // material index or 0xFFFFFFFF
+    float bmin[3], bmax[3];  // object-space box of the mesh's vertices (view culling)
 };
 
 // Per-view counters: records appended by the clipper, queued clipped triangles, entries of the
@@ -47,7 +48,8 @@ struct ViewGPU {
     uint32_t big_base, big_cap;  // (bin, slot, record) entries of records with many bins
     uint32_t cull, reverse, bias;
     float bias_const, bias_clamp, bias_slope;
-    uint32_t pad[3];
+    uint32_t cull_base;     // first per-draw culling flag of this view
+    uint32_t pad[2];
     uint64_t out;           // device address: u64 visibility (camera) or u32 depth bits (shadow)
     float view_projection[16];
 };

@@ -28,6 +28,7 @@ struct GeometryLaunch {
     uint32_t* pair_slots;   // per record slot: its slot inside each of its (at most 4) bins
     uint4* big_pairs;       // (bin, slot, record) of records with more bins
     uint32_t* block_valid;  // records set up per setup block
+    uint32_t* draw_culled;  // per (view, draw): 1 when the draw's box is outside the view
     uint32_t shift;  // log2(tile)
     uint32_t clip_blocks;
     uint32_t total_bins;

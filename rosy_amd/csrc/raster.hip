@@ -200,15 +200,50 @@ __global__ void k_clear2(uint32_t* a, uint32_t na, uint32_t* b, uint32_t nb) {
     }
 }
 
+// True when every corner of the object-space box lies outside one clip plane by more than the
+// rounding of the transform (|error| <= a few ulp of sum |m_rj p_j| per coordinate; the margin takes
+// 2^-18 of that sum).  A vertex of the mesh is a convex combination of the corners and its own sum
+// is no larger, so all its FP32 plane distances are negative too: every triangle of the draw would
+// be rejected by the per-triangle test, and skipping the draw changes nothing.
+__device__ inline bool box_outside(const float* m, const float* bmin, const float* bmax) {
+    bool any_plane = false;
+#pragma unroll
+    for (int plane = 0; plane < 6; ++plane) {
+        bool all_out = true;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float x = (k & 1) ? bmax[0] : bmin[0], y = (k & 2) ? bmax[1] : bmin[1];
+            const float z = (k & 4) ? bmax[2] : bmin[2];
+            const float4 c = transform(m, x, y, z);
+            float sr[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                sr[r] = fabsf(m[r] * x) + fabsf(m[4 + r] * y) + fabsf(m[8 + r] * z) + fabsf(m[12 + r]);
+            float bound;
+            switch (plane) {
+                case 0: bound = sr[3] + sr[2]; break;
+                case 1: bound = sr[2]; break;
+                case 2: case 3: bound = sr[0] + 4.0f * sr[3]; break;
+                default: bound = sr[1] + 4.0f * sr[3]; break;
+            }
+            all_out &= plane_dist(c, plane) < -bound * 0x1.0p-18f;
+        }
+        any_plane |= all_out;
+    }
+    return any_plane;
+}
+
 // clip = (VP * W) * p per (view, vertex).  Every block covers vertices of one (view, draw); its
 // MVP = views[v].view_projection * objects[draw.object].transform is formed once in LDS by the fmaf
 // chain in k order, then each thread reads its vertex from the SoA x/y/z streams (coalesced 12 B)
-// and writes the clip position (coalesced 16 B).
+// and writes the clip position (coalesced 16 B).  A draw whose box is outside the view is skipped,
+// and its first block records that for k_setup.
 __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, const ViewGPU* views,
                                                    const DrawGPU* draws0, const DrawGPU* draws1,
                                                    const float* objects, const float* px, const float* py,
-                                                   const float* pz, float4* xv) {
+                                                   const float* pz, float4* xv, uint32_t* draw_culled) {
     __shared__ float m[16];
+    __shared__ uint32_t culled;
     const BlockEntry e = blocks[blockIdx.x];
     const ViewGPU& V = views[e.view];
     const DrawGPU& dr = (V.list ? draws1 : draws0)[e.draw];
@@ -223,6 +258,12 @@ __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, con
         m[threadIdx.x] = acc;
     }
     __syncthreads();
+    if (threadIdx.x == 0) {
+        culled = box_outside(m, dr.bmin, dr.bmax) ? 1u : 0u;
+        if (e.start == 0) draw_culled[V.cull_base + e.draw] = culled;
+    }
+    __syncthreads();
+    if (culled) return;
     const uint32_t v = e.start + threadIdx.x;
     if (v >= dr.vtx_count) return;
     const uint32_t g = dr.vtx_base + v;
@@ -283,7 +324,7 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
                                                const DrawGPU* draws1, const uint32_t* indices, const float4* xv,
                                                TriRec* recs, uint2* clipq, uint32_t* counters,
                                                uint32_t* bin_counts, uint32_t* pair_slots, uint4* big_pairs,
-                                               uint32_t* block_valid, uint32_t shift) {
+                                               uint32_t* block_valid, const uint32_t* draw_culled, uint32_t shift) {
     __shared__ uint32_t hkey[kHash];
     __shared__ uint32_t hcount[kHash];
     __shared__ uint32_t hbase[kHash];
@@ -297,6 +338,12 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     const DrawGPU& dr = (V.list ? draws1 : draws0)[e.draw];
     uint32_t* ctr = counters + e.view * kCtrPerView;
     const uint32_t t = e.start + threadIdx.x;
+    if (draw_culled[V.cull_base + e.draw]) {  // the whole draw is outside the view: empty slots only
+        if (t < dr.tri_count)
+            *reinterpret_cast<uint2*>(&recs[V.rec_base + dr.tri_base + t].bbx) = make_uint2(kInvalidBox, kInvalidBox);
+        if (threadIdx.x == 0) block_valid[blockIdx.x] = 0u;
+        return;
+    }
     const bool valid = t < dr.tri_count;
     int status = 0;  // 0 reject, 1 accept, 2 clip
     float4 c[3];
@@ -372,40 +419,48 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     if (threadIdx.x == 0) block_valid[blockIdx.x] = wave_valid[0] + wave_valid[1] + wave_valid[2] + wave_valid[3];
 }
 
-// One wave per queued triangle: lane 0 clips and sets up the fan; the wave counts the bins.
-__global__ __launch_bounds__(256) void k_clip(const ViewGPU* views, const DrawGPU* draws0, const DrawGPU* draws1,
-                                              const uint32_t* indices, const float4* xv, const uint2* clipq,
-                                              TriRec* recs, uint32_t* counters, uint32_t* bin_counts,
-                                              uint4* big_pairs, uint32_t shift) {
+// One thread per queued triangle: Sutherland-Hodgman against the near, far and guard-band planes
+// in that order (intersections from the inside vertex), the polygon kept in the thread's slice of
+// LDS, then the fan (v0, vk, vk+1) set up as records; record slots are handed out per wave and
+// the fan triangles are binned by the whole wave.
+constexpr int kClipThreads = 128;
+constexpr int kClipMax = 9;  // a triangle cut by 6 planes has at most 9 vertices
+
+__global__ __launch_bounds__(kClipThreads) void k_clip(const ViewGPU* views, const DrawGPU* draws0,
+                                                       const DrawGPU* draws1, const uint32_t* indices,
+                                                       const float4* xv, const uint2* clipq, TriRec* recs,
+                                                       uint32_t* counters, uint32_t* bin_counts, uint4* big_pairs,
+                                                       uint32_t shift) {
+    __shared__ float4 poly[kClipThreads][2][kClipMax];
     const uint32_t view = blockIdx.y;
     const ViewGPU& V = views[view];
     uint32_t* ctr = counters + view * kCtrPerView;
     const uint32_t n = min(ctr[kCtrClip], V.clip_cap);
-    const uint32_t waves_per_block = blockDim.x / 64;
-    const uint32_t wid = blockIdx.x * waves_per_block + threadIdx.x / 64;
-    const uint32_t stride = gridDim.x * waves_per_block;
-    for (uint32_t q = wid; q < n; q += stride) {
-        TriRec out[7];
-        uint32_t out_slot[7];
-        int nout = 0;
-        if (lane_id() == 0) {
+    for (uint32_t base = blockIdx.x * kClipThreads; base < n; base += gridDim.x * kClipThreads) {
+        const uint32_t q = base + threadIdx.x;
+        const bool active = q < n;
+        float4* A = poly[threadIdx.x][0];
+        float4* B = poly[threadIdx.x][1];
+        int cnt = 0;
+        uint32_t id = 0;
+        if (active) {
             const uint2 ent = clipq[V.clip_base + q];
             const DrawGPU& dr = (V.list ? draws1 : draws0)[ent.x];
             const uint32_t* ip = indices + dr.first_index + 3u * ent.y;
-            const float4* base = xv + size_t(V.xv_base) + dr.xv_off;
-            float4 a[12], b[12];
-            a[0] = base[ip[0]];
-            a[1] = base[ip[1]];
-            a[2] = base[ip[2]];
-            int cnt = 3;
+            const float4* vb = xv + size_t(V.xv_base) + dr.xv_off;
+            id = dr.tri_base + ent.y;
+            A[0] = vb[ip[0]];
+            A[1] = vb[ip[1]];
+            A[2] = vb[ip[2]];
+            cnt = 3;
             for (int plane = 0; plane < 6 && cnt >= 3; ++plane) {
                 int m = 0;
                 for (int i = 0; i < cnt; ++i) {
-                    const float4 cur = a[i];
-                    const float4 nxt = a[(i + 1) % cnt];
+                    const float4 cur = A[i];
+                    const float4 nxt = A[i + 1 < cnt ? i + 1 : 0];
                     const float dc = plane_dist(cur, plane), dn = plane_dist(nxt, plane);
                     const bool ic = dc >= 0.f, in_ = dn >= 0.f;
-                    if (ic) b[m++] = cur;
+                    if (ic) B[m++] = cur;
                     if (ic != in_) {
                         const float4 pi = ic ? cur : nxt;
                         const float4 po = ic ? nxt : cur;
@@ -416,39 +471,39 @@ __global__ __launch_bounds__(256) void k_clip(const ViewGPU* views, const DrawGP
                         qv.y = pi.y + tt * (po.y - pi.y);
                         qv.z = pi.z + tt * (po.z - pi.z);
                         qv.w = pi.w + tt * (po.w - pi.w);
-                        b[m++] = qv;
+                        B[m++] = qv;
                     }
                 }
                 cnt = m;
-                for (int i = 0; i < cnt; ++i) a[i] = b[i];
+                float4* t = A;
+                A = B;
+                B = t;
             }
             bool ok = cnt >= 3;
-            for (int i = 0; i < cnt && ok; ++i) ok = a[i].w > 0.f;
-            if (ok) {
-                for (int k = 1; k + 1 < cnt && nout < 7; ++k) {
-                    const float4 tv[3] = {a[0], a[k], a[k + 1]};
-                    TriRec r;
-                    if (setup_tri(tv, dr.tri_base + ent.y, V, r)) {
-                        const uint32_t slot = atomicAdd(&ctr[kCtrRecords], 1u);
-                        if (V.tri_count + slot < V.rec_cap) {
-                            recs[size_t(V.rec_base) + V.tri_count + slot] = r;
-                            out_slot[nout] = V.rec_base + V.tri_count + slot;
-                            out[nout++] = r;
-                        } else {
-                            atomicOr(&ctr[kCtrOverflow], 2u);
-                        }
-                    }
+            for (int i = 0; i < cnt && ok; ++i) ok = A[i].w > 0.f;
+            if (!ok) cnt = 0;
+        }
+        const uint32_t nclipped = uint32_t(__popcll(__ballot(active)));
+        if (lane_id() == 0 && nclipped) atomicAdd(&ctr[kCtrClipped], nclipped);
+        for (int k = 1; k <= kClipMax - 2; ++k) {  // fan triangle k: (v0, vk, vk+1)
+            TriRec r;
+            bool has = false;
+            if (k + 1 < cnt) {
+                const float4 tv[3] = {A[0], A[k], A[k + 1]};
+                has = setup_tri(tv, id, V, r);
+            }
+            const uint32_t slot = wave_append(has, &ctr[kCtrRecords]);
+            uint32_t ri = 0;
+            if (has) {
+                if (V.tri_count + slot < V.rec_cap) {
+                    ri = V.rec_base + V.tri_count + slot;
+                    recs[ri] = r;
+                } else {
+                    atomicOr(&ctr[kCtrOverflow], 2u);
+                    has = false;
                 }
             }
-            atomicAdd(&ctr[kCtrClipped], 1u);
-        }
-        nout = __shfl(nout, 0);
-        for (int k = 0; k < nout; ++k) {
-            TriRec r;
-            r.bbx = __shfl(out[k].bbx, 0);
-            r.bby = __shfl(out[k].bby, 0);
-            const uint32_t ri = uint32_t(__shfl(int(out_slot[k]), 0));
-            bin_wide(lane_id() == 0, r, ri, V, shift, bin_counts, big_pairs, ctr);
+            bin_wide(has, r, ri, V, shift, bin_counts, big_pairs, ctr);
         }
     }
 }
@@ -771,12 +826,12 @@ hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
                        g.n_views * kCtrPerView, g.bin_counts, g.total_bins);
     if (g.n_vblocks)
         hipLaunchKernelGGL(k_transform, dim3(g.n_vblocks), dim3(256), 0, s, g.vblocks, g.views, g.draws0, g.draws1,
-                           g.objects, g.px, g.py, g.pz, g.xv);
+                           g.objects, g.px, g.py, g.pz, g.xv, g.draw_culled);
     if (g.n_tblocks)
         hipLaunchKernelGGL(k_setup, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.draws0, g.draws1,
                            g.indices, g.xv, g.recs, g.clipq, g.counters, g.bin_counts, g.pair_slots, g.big_pairs,
-                           g.block_valid, g.shift);
-    hipLaunchKernelGGL(k_clip, dim3(g.clip_blocks, g.n_views), dim3(256), 0, s, g.views, g.draws0, g.draws1,
+                           g.block_valid, g.draw_culled, g.shift);
+    hipLaunchKernelGGL(k_clip, dim3(g.clip_blocks, g.n_views), dim3(kClipThreads), 0, s, g.views, g.draws0, g.draws1,
                        g.indices, g.xv, g.clipq, g.recs, g.counters, g.bin_counts, g.big_pairs, g.shift);
     return hipGetLastError();
 }
