@@ -223,6 +223,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     HIP_TRY(c, c->totals.ensure(32), "allocate statistics");
     HIP_TRY(c, c->split_bins.ensure(size_t(std::max<uint32_t>(1, L.total_bins)) * 4), "allocate bins");
     HIP_TRY(c, c->whole_items.ensure(size_t(std::max<uint32_t>(1, L.total_bins)) * 4), "allocate bins");
+    HIP_TRY(c, c->scan_partial.ensure((size_t(L.total_bins) / 4096 + 1) * kScanPartialBytes), "allocate bins");
     HIP_TRY(c, c->vblocks_d.ensure(std::max<size_t>(1, L.vblocks.size()) * sizeof(BlockEntry)), "allocate blocks");
     HIP_TRY(c, c->tblocks_d.ensure(std::max<size_t>(1, L.tblocks.size()) * sizeof(BlockEntry)), "allocate blocks");
     if (!L.vblocks.empty())
@@ -382,6 +383,7 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     uint32_t max_tris = 0;
     for (const auto& v : L.vg) max_tris = std::max(max_tris, v.tri_count);
     b.scatter_blocks = std::max<uint32_t>(64, std::min<uint32_t>(4096, (max_tris + 255) / 256));
+    b.scan_partial = c->scan_partial.p;
     b.split_items = c->items.as<uint2>();
     b.items_cap = raster_items_cap(L.total_bins, c->list_cap);
     b.whole_items = c->whole_items.as<uint32_t>();

@@ -174,7 +174,7 @@ struct rsy_ctx {
     rsy::Layout layout;
     rsy::DevBuf views_d, xv, recs, clipq, counters, bin_counts, bin_offsets, pair_slots, big_pairs, block_valid, totals,
         lists_mem, vblocks_d, tblocks_d, shade_views_d, shade_cascades_d, xv_bases_d, items, split_bins, whole_items,
-        draw_culled;
+        draw_culled, scan_partial;
     uint32_t list_cap = 0;
     uint32_t rec_scale = 1;  // grows on overflow
     std::vector<rsy::DevBuf> vis_out, color_out, shadow_factor_out, shadow_maps;

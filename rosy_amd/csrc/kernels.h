@@ -52,6 +52,7 @@ struct BinLaunch {
     uint32_t list_cap;
     uint32_t shift;
     uint32_t scatter_blocks;
+    void* scan_partial;     // 16 bytes per 4096 bins
     uint2* split_items;     // raster work items of crowded bins: (bin, chunk)
     uint32_t items_cap;     // split_items capacity
     uint32_t* whole_items;  // bins rasterised by one workgroup (total_bins entries)
@@ -204,6 +205,7 @@ struct NmPointLaunch {
 hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
 hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s);
 hipError_t launch_binning(const BinLaunch& b, hipStream_t s);
+constexpr uint32_t kScanPartialBytes = 16;  // per 4096 bins
 hipError_t launch_raster(const RasterLaunch& r, hipStream_t s);
 uint32_t raster_items_cap(uint32_t total_bins, uint32_t list_cap);  // split work-item table entries
 hipError_t launch_shade(const ShadeLaunch& l, hipStream_t s);

@@ -541,74 +541,122 @@ __device__ inline uint64_t block_exclusive_scan64(uint64_t v, uint64_t* lds_wave
     return wave_off + x - v;
 }
 
-// One pass over the bins: bin offsets (exclusive prefix of the counts) and the work-item tables.
-// Two block scans per pass: a 64-bit one of (record count | split items << 32) -- the counts of a
-// frame stay below 2^32, so the low half never carries -- and a 32-bit one of (split bins | whole
-// bins << 16), both at most 4096 per pass.
-__global__ __launch_bounds__(1024) void k_scan(const uint32_t* counts, uint32_t n, uint32_t* offsets,
-                                               const uint32_t* block_valid, uint32_t n_blocks,
-                                               uint32_t list_cap, uint32_t* totals, uint2* split_items,
-                                               uint32_t* whole_items, uint32_t items_cap, uint32_t* split_bins) {
+// Bin offsets (exclusive prefix of the counts) and the work-item tables, in three launches: block
+// partial sums, one scan of the partials (plus the frame totals), and each block's final scan.
+// Each block covers kScanChunk bins with two block scans: a 64-bit one of (record count | split
+// items << 32) -- the counts of a frame stay below 2^32, so the low half never carries -- and a
+// 32-bit one of (split bins | whole bins << 16), both at most kScanChunk per block.
+constexpr uint32_t kScanChunk = kScanThreads * kScanPerThread;
+static_assert(kScanChunk <= 65535, "bin fields of the 32-bit scan");
+
+struct ScanPartial {
+    uint64_t a;
+    uint32_t b, pad;
+};
+
+__device__ inline void bin_fields(uint32_t v, uint64_t& a, uint32_t& b) {
+    if (v > kChunkRecs) {
+        a += uint64_t(v) | (uint64_t((v + kChunkRecs - 1) / kChunkRecs) << 32);
+        b += 1u;
+    } else {
+        a += v;
+        b += 1u << 16;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_partial(const uint32_t* counts, uint32_t n, ScanPartial* partial) {
     __shared__ uint32_t lds[16];
     __shared__ uint64_t lds64[16];
-    static_assert(kScanThreads * kScanPerThread <= 65535, "bin fields of the 32-bit scan");
-    uint64_t carry = 0;
-    uint32_t c_split = 0, c_whole = 0;
+    const uint32_t i0 = blockIdx.x * kScanChunk + threadIdx.x * kScanPerThread;
+    uint64_t a = 0;
+    uint32_t b = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPerThread; ++k)
+        if (i0 + k < n) bin_fields(counts[i0 + k], a, b);
+    uint64_t ta;
+    uint32_t tb;
+    block_exclusive_scan64(a, lds64, ta);
+    block_exclusive_scan(b, lds, tb);
+    if (threadIdx.x == 0) partial[blockIdx.x] = ScanPartial{ta, tb, 0u};
+}
+
+// One block: exclusive scan of the block partials (in place, as the blocks' bases) and the totals.
+__global__ __launch_bounds__(1024) void k_scan_top(ScanPartial* partial, uint32_t n_parts,
+                                                   const uint32_t* block_valid, uint32_t n_blocks,
+                                                   uint32_t list_cap, uint32_t items_cap, uint32_t* totals) {
+    __shared__ uint32_t lds[16];
+    __shared__ uint64_t lds64[16];
+    uint64_t carry_a = 0;
+    uint32_t carry_s = 0, carry_w = 0;
+    for (uint32_t base = 0; base < n_parts; base += kScanThreads) {
+        const uint32_t i = base + threadIdx.x;
+        const ScanPartial p = i < n_parts ? partial[i] : ScanPartial{0ull, 0u, 0u};
+        // split / whole bin counts of a block fit 16 bits each, their sums over blocks may not
+        uint64_t ta, ts, tw;
+        const uint64_t ea = block_exclusive_scan64(p.a, lds64, ta);
+        const uint64_t es = block_exclusive_scan64(p.b & 0xffffu, lds64, ts);
+        const uint64_t ew = block_exclusive_scan64(p.b >> 16, lds64, tw);
+        __syncthreads();
+        if (i < n_parts) {
+            partial[i].a = carry_a + ea;
+            partial[i].b = carry_s + uint32_t(es);
+            partial[i].pad = carry_w + uint32_t(ew);
+        }
+        carry_a += ta;
+        carry_s += uint32_t(ts);
+        carry_w += uint32_t(tw);
+    }
     uint32_t vs = 0;
     for (uint32_t i = threadIdx.x; i < n_blocks; i += kScanThreads) vs += block_valid[i];
-    for (uint32_t base = 0; base < n; base += kScanThreads * kScanPerThread) {
-        const uint32_t i0 = base + threadIdx.x * kScanPerThread;
-        uint32_t v[kScanPerThread];
-        uint64_t a = 0;
-        uint32_t b = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < kScanPerThread; ++k) {
-            v[k] = i0 + k < n ? counts[i0 + k] : 0u;
-            if (i0 + k < n) {
-                if (v[k] > kChunkRecs) {
-                    a += uint64_t(v[k]) | (uint64_t((v[k] + kChunkRecs - 1) / kChunkRecs) << 32);
-                    b += 1u;
-                } else {
-                    a += v[k];
-                    b += 1u << 16;
-                }
-            }
-        }
-        uint64_t ta;
-        uint32_t tb;
-        const uint64_t ea = carry + block_exclusive_scan64(a, lds64, ta);
-        const uint32_t eb = block_exclusive_scan(b, lds, tb);
-        uint32_t run = uint32_t(ea), r_items = uint32_t(ea >> 32);
-        uint32_t r_split = c_split + (eb & 0xffffu), r_whole = c_whole + (eb >> 16);
-#pragma unroll
-        for (uint32_t k = 0; k < kScanPerThread; ++k)
-            if (i0 + k < n) {
-                offsets[i0 + k] = run;
-                run += v[k];
-                if (v[k] > kChunkRecs) {
-                    split_bins[r_split++] = i0 + k;
-                    const uint32_t ni = (v[k] + kChunkRecs - 1) / kChunkRecs;
-                    for (uint32_t c = 0; c < ni; ++c, ++r_items)
-                        if (r_items < items_cap) split_items[r_items] = make_uint2(i0 + k, c);
-                } else {
-                    whole_items[r_whole++] = i0 + k;
-                }
-            }
-        carry += ta;
-        c_split += tb & 0xffffu;
-        c_whole += tb >> 16;
-    }
     uint32_t total_valid;
     block_exclusive_scan(vs, lds, total_valid);
     if (threadIdx.x == 0) {
-        const uint32_t pairs = uint32_t(carry), n_items = uint32_t(carry >> 32);
+        const uint32_t pairs = uint32_t(carry_a), n_items = uint32_t(carry_a >> 32);
         totals[0] = pairs;
         totals[1] = total_valid;
         totals[2] = (pairs > list_cap || n_items > items_cap) ? 1u : 0u;
         totals[3] = min(n_items, items_cap);
-        totals[4] = c_whole;
-        totals[5] = c_split;
+        totals[4] = carry_w;
+        totals[5] = carry_s;
     }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_final(const uint32_t* counts, uint32_t n, uint32_t* offsets,
+                                                     const ScanPartial* partial, uint2* split_items,
+                                                     uint32_t* whole_items, uint32_t items_cap,
+                                                     uint32_t* split_bins) {
+    __shared__ uint32_t lds[16];
+    __shared__ uint64_t lds64[16];
+    const uint32_t i0 = blockIdx.x * kScanChunk + threadIdx.x * kScanPerThread;
+    uint32_t v[kScanPerThread];
+    uint64_t a = 0;
+    uint32_t b = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPerThread; ++k) {
+        v[k] = i0 + k < n ? counts[i0 + k] : 0u;
+        if (i0 + k < n) bin_fields(v[k], a, b);
+    }
+    uint64_t ta;
+    uint32_t tb;
+    const ScanPartial base = partial[blockIdx.x];
+    const uint64_t ea = base.a + block_exclusive_scan64(a, lds64, ta);
+    const uint32_t eb = block_exclusive_scan(b, lds, tb);
+    uint32_t run = uint32_t(ea), r_items = uint32_t(ea >> 32);
+    uint32_t r_split = base.b + (eb & 0xffffu), r_whole = base.pad + (eb >> 16);
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPerThread; ++k)
+        if (i0 + k < n) {
+            offsets[i0 + k] = run;
+            run += v[k];
+            if (v[k] > kChunkRecs) {
+                split_bins[r_split++] = i0 + k;
+                const uint32_t ni = (v[k] + kChunkRecs - 1) / kChunkRecs;
+                for (uint32_t c = 0; c < ni; ++c, ++r_items)
+                    if (r_items < items_cap) split_items[r_items] = make_uint2(i0 + k, c);
+            } else {
+                whole_items[r_whole++] = i0 + k;
+            }
+        }
 }
 
 // Bin lists without atomics: every (record, bin) pair already holds its slot inside the bin (from
@@ -837,9 +885,13 @@ hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
 }
 
 hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, b.bin_offsets,
-                       b.block_valid, b.n_tblocks, b.list_cap, b.totals, b.split_items, b.whole_items, b.items_cap,
-                       b.split_bins);
+    const uint32_t parts = (b.total_bins + kScanChunk - 1) / kScanChunk;
+    ScanPartial* partial = reinterpret_cast<ScanPartial*>(b.scan_partial);
+    hipLaunchKernelGGL(k_scan_partial, dim3(parts), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, partial);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, partial, parts, b.block_valid, b.n_tblocks,
+                       b.list_cap, b.items_cap, b.totals);
+    hipLaunchKernelGGL(k_scan_final, dim3(parts), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, b.bin_offsets,
+                       partial, b.split_items, b.whole_items, b.items_cap, b.split_bins);
     hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.n_views, b.recs,
                        b.counters, b.bin_offsets, b.pair_slots, b.big_pairs, b.lists, b.list_cap, b.shift, b.totals,
                        b.split_bins);
