@@ -317,7 +317,9 @@ __device__ void bin_wide(bool active, const TriRec& r, uint32_t ri, const ViewGP
     }
 }
 
-constexpr uint32_t kHash = 2048;  // LDS bin table of one setup block (256 threads x 4 bins at most)
+// LDS bin table of one setup block: at most 256 x kSmallBins = 1024 distinct keys, so a probe always
+// finds its key or an empty entry.
+constexpr uint32_t kHash = 1024;
 constexpr uint32_t kEmpty = 0xffffffffu;
 
 __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const ViewGPU* views, const DrawGPU* draws0,
@@ -326,8 +328,7 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
                                                uint32_t* bin_counts, uint32_t* pair_slots, uint4* big_pairs,
                                                uint32_t* block_valid, const uint32_t* draw_culled, uint32_t shift) {
     __shared__ uint32_t hkey[kHash];
-    __shared__ uint32_t hcount[kHash];
-    __shared__ uint32_t hbase[kHash];
+    __shared__ uint32_t hcount[kHash];  // per key: records of the block, then the block's first slot
     __shared__ uint32_t wave_valid[4];
     for (uint32_t i = threadIdx.x; i < kHash; i += 256) {
         hkey[i] = kEmpty;
@@ -388,7 +389,7 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
         rank[k] = 0u;
         if (small && k < n) {
             const uint32_t key = V.tile_base + uint32_t((ty0 + k / bw) * int(V.tiles_x) + tx0 + k % bw);
-            uint32_t h = (key * 2654435761u) >> 21;  // 11-bit multiplicative hash
+            uint32_t h = (key * 2654435761u) >> 22;  // 10-bit multiplicative hash
             while (true) {
                 const uint32_t prev = atomicCAS(&hkey[h], kEmpty, key);
                 if (prev == kEmpty || prev == key) break;
@@ -401,14 +402,14 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     __syncthreads();
     // one global atomic per distinct bin of the block: the block's first slot in that bin
     for (uint32_t i = threadIdx.x; i < kHash; i += 256)
-        if (hkey[i] != kEmpty) hbase[i] = atomicAdd(&bin_counts[hkey[i]], hcount[i]);
+        if (hkey[i] != kEmpty) hcount[i] = atomicAdd(&bin_counts[hkey[i]], hcount[i]);
     __syncthreads();
     if (small) {
         uint4 out = make_uint4(0u, 0u, 0u, 0u);
-        out.x = hbase[hidx[0]] + rank[0];
-        if (n > 1) out.y = hbase[hidx[1]] + rank[1];
-        if (n > 2) out.z = hbase[hidx[2]] + rank[2];
-        if (n > 3) out.w = hbase[hidx[3]] + rank[3];
+        out.x = hcount[hidx[0]] + rank[0];
+        if (n > 1) out.y = hcount[hidx[1]] + rank[1];
+        if (n > 2) out.z = hcount[hidx[2]] + rank[2];
+        if (n > 3) out.w = hcount[hidx[3]] + rank[3];
         reinterpret_cast<uint4*>(pair_slots)[slot] = out;
     }
     // records with many bins: the whole wave spreads their bins
