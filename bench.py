@@ -249,7 +249,8 @@ WORKLOADS = {
     2: "batched views: 1,048,576 triangles (106 meshes), 2560x1440 camera + 4 cascades 2048^2 (splits 5/15/40/100), "
        "16 materials x 2048^2 maps, plain shading, one launch per stage over all 5 views",
     4: "combined: 2,097,152 triangles (116 meshes), cameras 3840x2160 + 1920x1080 + 1280x720 + 640x360 + 4 cascades "
-       "2048^2, 64 materials x 1024^2 maps, neural-material shading (FP16 decoder) with plain fallback below 40 dB",
+       "2048^2, 64 materials x 1024^2 maps, neural-material shading (FP8 E4M3 decoder weights, FP32 accumulation) "
+       "with plain fallback below 40 dB",
 }
 
 
@@ -273,13 +274,13 @@ def run_views(args):
     frame = plain
     dec = None
     if args.workload == 4:
-        r.neural_init(decay_steps=args.train_steps)
+        r.neural_init(decay_steps=args.train_steps, weights=abi.NEURAL_WEIGHTS_FP8)
         r.neural_train(args.train_steps)
         psnr = r.neural_evaluate()
         nst = r.neural_stats()
         dec = {"psnr_min": round(float(psnr.min()), 2), "psnr_median": round(float(np.median(psnr)), 2),
                "materials_decoded": int(nst.n_active), "materials": int(nst.n_materials),
-               "train_ms": round(float(nst.train_ms), 1), "weights": "fp16"}
+               "train_ms": round(float(nst.train_ms), 1), "weights": "fp8 e4m3 (per-layer scale)"}
         frame = scene.frame_desc(abi.SHADE_NEURAL)
     st = stage_medians(r, frame)
     fst = r.stats()

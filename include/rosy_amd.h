@@ -336,6 +336,11 @@ typedef enum rsy_neural_init_mode {
     RSY_NEURAL_INIT_RANDOM = 1     /* uniform latents in [-0.1, 0.1], random MLP */
 } rsy_neural_init_mode;
 
+typedef enum rsy_neural_weights {
+    RSY_NEURAL_WEIGHTS_FP16 = 0, /* FP16 weights */
+    RSY_NEURAL_WEIGHTS_FP8 = 1   /* OCP FP8 E4M3 weights with one FP32 scale per layer (b2 stays FP32) */
+} rsy_neural_weights;
+
 typedef struct rsy_neural_config {
     uint32_t latent_size;    /* cells per side: a power of two in [4, 4096] */
     uint32_t batch;          /* samples per step, a multiple of 256 */
@@ -346,7 +351,7 @@ typedef struct rsy_neural_config {
     float beta1, beta2, eps;
     float psnr_threshold;    /* dB */
     uint32_t decay_steps;    /* cosine learning-rate decay to 0 over this many steps (0: constant) */
-    uint32_t reserved;
+    uint32_t weights;        /* rsy_neural_weights: storage of the inference MLP weights */
     uint64_t seed;
 } rsy_neural_config;
 
@@ -387,6 +392,12 @@ rsy_result rsy_neural_set_params(rsy_ctx* ctx, const float* latents, uint64_t n_
  * Requires latent_size <= 32.  grad_mlp [583], grad_latents as in rsy_neural_get_params. */
 rsy_result rsy_neural_gradients(rsy_ctx* ctx, const uint32_t* material, const float* uv, const float* target,
                                 uint32_t n, float* loss, float* grad_mlp, float* grad_latents, uint64_t n_latents);
+/* Switches the inference weight storage (rsy_neural_weights); takes effect at the next
+ * rsy_neural_evaluate (neural frames are refused until then). */
+rsy_result rsy_neural_set_weights(rsy_ctx* ctx, uint32_t weights);
+/* The MLP the inference path evaluates, as FP32 values in the rsy_neural_get_params layout: FP16- or
+ * FP8-rounded (times the layer scale) W1, b1, W2 and FP32 b2.  After rsy_neural_evaluate. */
+rsy_result rsy_neural_get_inference_mlp(rsy_ctx* ctx, float* mlp, uint32_t n_mlp);
 /* Decoder outputs (inference path, after rsy_neural_evaluate) or, with targets != 0, the plain
  * sampler's values that training regresses onto, at material[n], uv[n][2] -> out[n][7]. */
 rsy_result rsy_neural_decode(rsy_ctx* ctx, const uint32_t* material, const float* uv, uint32_t n, uint32_t targets,

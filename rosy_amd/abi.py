@@ -147,7 +147,7 @@ class NeuralConfig(C.Structure):
     _fields_ = [("latent_size", C.c_uint32), ("batch", C.c_uint32), ("init", C.c_uint32), ("region", C.c_uint32),
                 ("lr_mlp", C.c_float), ("lr_latent", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
                 ("eps", C.c_float), ("psnr_threshold", C.c_float), ("decay_steps", C.c_uint32),
-                ("reserved", C.c_uint32), ("seed", C.c_uint64)]
+                ("weights", C.c_uint32), ("seed", C.c_uint64)]
 
 
 class NeuralStats(C.Structure):
@@ -156,6 +156,8 @@ class NeuralStats(C.Structure):
                 ("reserved", C.c_float)]
 
 
+NEURAL_WEIGHTS_FP16 = 0
+NEURAL_WEIGHTS_FP8 = 1
 NEURAL_INIT_FROM_MAPS = 0
 NEURAL_INIT_RANDOM = 1
 NEURAL_PARAMS = 583   # W1 [32][10], b1 [32], W2 [7][32], b2 [7]
@@ -202,6 +204,8 @@ PROTOTYPES = [
     ("rsy_neural_set_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32]),
     ("rsy_neural_gradients", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                        C.POINTER(C.c_float), C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("rsy_neural_set_weights", C.c_int, [C.c_void_p, C.c_uint32]),
+    ("rsy_neural_get_inference_mlp", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     ("rsy_neural_decode", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
     ("rsy_camera_projection", None, [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_float)]),
     ("rsy_camera_view_matrix", None, [C.POINTER(C.c_float), C.c_float, C.c_float, C.POINTER(C.c_float)]),

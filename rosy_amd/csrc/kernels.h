@@ -212,7 +212,8 @@ hipError_t launch_shade(const ShadeLaunch& l, hipStream_t s);
 hipError_t launch_nm_init(const NmInitLaunch& l, hipStream_t s);
 hipError_t launch_nm_step(const NmTrainLaunch& l, bool explicit_batch, hipStream_t s);
 hipError_t launch_nm_export(const float* lat, uint4* lat16, uint64_t cells, hipStream_t s);
-hipError_t launch_nm_export_weights(const float* mlp, void* weights, hipStream_t s);
+hipError_t launch_nm_export_weights(const float* mlp, void* weights, uint32_t fp8, hipStream_t s);
+hipError_t launch_nm_unpack_weights(const void* weights, float* mlp, hipStream_t s);
 hipError_t launch_nm_eval(const NmEvalLaunch& l, uint32_t n_blocks, hipStream_t s);
 hipError_t launch_nm_points(const NmPointLaunch& l, hipStream_t s);
 

@@ -36,11 +36,14 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 // MFMA operands of the MLP, one 8-byte entry per lane.  Layer 1 multiplies W1' = [W1 | b1 | 0]
 // (32 x 16) by the feature columns; entry [s][l] holds W1'[l % 32][8s + 4(l / 32) + 0..3].
 // Layer 2 multiplies W2' (32 x 32, rows 7..31 zero) by the hidden columns; entry [s][l] holds
-// W2'[l % 32][8s + 4(l / 32) + 0..3].
+// W2'[l % 32][8s + 4(l / 32) + 0..3].  With FP8 weights the entries hold the E4M3 codes' values
+// (exact in FP16) and scale[] the per-layer factor that turns them back into weights.
 struct NmWeights {
     h4 w1[2][64];
     h4 w2[4][64];
     float b2[8];
+    float scale[2];  // layer-1 / layer-2 accumulator scale: 1 for FP16 weights, the FP8 scale otherwise
+    float pad[2];
 };
 static_assert(sizeof(NmWeights) == kNmWeightsBytes, "NmWeights layout");
 
@@ -148,6 +151,7 @@ __device__ inline void nm_mlp_wave(const NmWeights* __restrict__ W, const float 
     other_h[1] = nm_swap_halves(hi ? qh[2] : zero);
     other_l[1] = nm_swap_halves(hi ? ql[2] : zero);
     const h4 a10 = W->w1[0][lane], a11 = W->w1[1][lane];
+    const float s1 = W->scale[0], s2 = W->scale[1];
     h4 a2[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) a2[s] = W->w2[s][lane];
@@ -164,8 +168,8 @@ __device__ inline void nm_mlp_wave(const NmWeights* __restrict__ W, const float 
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             h4 hh, hl;
-            nm_split4(fmaxf(acc[4 * s + 0], 0.f), fmaxf(acc[4 * s + 1], 0.f), fmaxf(acc[4 * s + 2], 0.f),
-                      fmaxf(acc[4 * s + 3], 0.f), hh, hl);
+            nm_split4(fmaxf(acc[4 * s + 0] * s1, 0.f), fmaxf(acc[4 * s + 1] * s1, 0.f),
+                      fmaxf(acc[4 * s + 2] * s1, 0.f), fmaxf(acc[4 * s + 3] * s1, 0.f), hh, hl);
             acc2 = __builtin_amdgcn_mfma_f32_32x32x8f16(a2[s], hh, acc2, 0, 0, 0);
             acc2 = __builtin_amdgcn_mfma_f32_32x32x8f16(a2[s], hl, acc2, 0, 0, 0);
         }
@@ -187,7 +191,7 @@ __device__ inline void nm_mlp_wave(const NmWeights* __restrict__ W, const float 
         if (r < 3) out[4 + r] = hi4;
     }
 #pragma unroll
-    for (int c = 0; c < kNmOut; ++c) out[c] = fminf(1.f, fmaxf(0.f, out[c] + W->b2[c]));
+    for (int c = 0; c < kNmOut; ++c) out[c] = fminf(1.f, fmaxf(0.f, out[c] * s2 + W->b2[c]));
 }
 
 }  // namespace rsy

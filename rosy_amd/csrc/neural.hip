@@ -361,31 +361,77 @@ __global__ __launch_bounds__(256) void k_nm_points(NmPointLaunch P) {
 }
 
 // MFMA-operand form of the MLP (see NmWeights): one wave, lane l writes its own operand entries.
-__global__ __launch_bounds__(64) void k_nm_export_weights(const float* p, NmWeights* W) {
+// FP16: every weight rounded to FP16, scales 1.  FP8: each layer's weights (layer 1 with b1 folded
+// in) divided by max|w| / 448 and rounded to OCP E4M3 (round to nearest even); the codes' values go
+// into the FP16 operands exactly and the scale multiplies the layer's accumulator.
+__device__ inline float layer_weight(const float* p, int layer, uint32_t i, uint32_t k) {
+    if (layer == 0) return k < uint32_t(kNmIn) ? p[kNmW1 + i * kNmIn + k] : (k == uint32_t(kNmIn) ? p[kNmB1 + i] : 0.f);
+    return i < uint32_t(kNmOut) ? p[kNmW2 + i * kNmHidden + k] : 0.f;
+}
+
+__device__ inline float fp8_round(float x) {  // x / scale already within the E4M3 range
+    const int packed = __builtin_amdgcn_cvt_pk_fp8_f32(x, 0.f, 0, false);
+    return __builtin_amdgcn_cvt_f32_fp8(packed, 0);
+}
+
+__global__ __launch_bounds__(64) void k_nm_export_weights(const float* p, NmWeights* W, uint32_t fp8) {
     const uint32_t l = threadIdx.x, i = l % 32, kq = 4 * (l / 32);
+    float scale[2] = {1.f, 1.f};
+    if (fp8) {
+        for (int layer = 0; layer < 2; ++layer) {
+            float m = 0.f;
+            for (uint32_t e = l; e < 32u * 32u; e += 64) m = fmaxf(m, fabsf(layer_weight(p, layer, e / 32, e % 32)));
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+            scale[layer] = m > 0.f ? m / 448.f : 1.f;
+        }
+    }
     for (int s = 0; s < 2; ++s) {
         float w[4];
         for (int e = 0; e < 4; ++e) {
-            const uint32_t k = 8 * s + kq + e;
-            w[e] = k < uint32_t(kNmIn) ? p[kNmW1 + i * kNmIn + k] : (k == uint32_t(kNmIn) ? p[kNmB1 + i] : 0.f);
+            const float v = layer_weight(p, 0, i, 8 * s + kq + e);
+            w[e] = fp8 ? fp8_round(v / scale[0]) : v;
         }
         W->w1[s][l] = h4{_Float16(w[0]), _Float16(w[1]), _Float16(w[2]), _Float16(w[3])};
     }
     for (int s = 0; s < 4; ++s) {
         float w[4];
         for (int e = 0; e < 4; ++e) {
-            const uint32_t k = 8 * s + kq + e;
-            w[e] = i < uint32_t(kNmOut) ? p[kNmW2 + i * kNmHidden + k] : 0.f;
+            const float v = layer_weight(p, 1, i, 8 * s + kq + e);
+            w[e] = fp8 ? fp8_round(v / scale[1]) : v;
         }
         W->w2[s][l] = h4{_Float16(w[0]), _Float16(w[1]), _Float16(w[2]), _Float16(w[3])};
     }
     if (l < 8) W->b2[l] = l < uint32_t(kNmOut) ? p[kNmB2 + l] : 0.f;
+    if (l < 2) W->scale[l] = scale[l];
+    if (l < 2) W->pad[l] = 0.f;
+}
+
+// The inference MLP back in the parameter layout (FP32 values: operand value times layer scale).
+__global__ __launch_bounds__(64) void k_nm_unpack_weights(const NmWeights* W, float* p) {
+    const uint32_t l = threadIdx.x, i = l % 32, kq = 4 * (l / 32);
+    for (int s = 0; s < 2; ++s)
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t k = 8 * s + kq + e;
+            const float v = float(W->w1[s][l][e]) * W->scale[0];
+            if (k < uint32_t(kNmIn)) p[kNmW1 + i * kNmIn + k] = v;
+            else if (k == uint32_t(kNmIn)) p[kNmB1 + i] = v;
+        }
+    for (int s = 0; s < 4; ++s)
+        for (int e = 0; e < 4; ++e)
+            if (i < uint32_t(kNmOut)) p[kNmW2 + i * kNmHidden + 8 * s + kq + e] = float(W->w2[s][l][e]) * W->scale[1];
+    if (l < uint32_t(kNmOut)) p[kNmB2 + l] = W->b2[l];
 }
 
 }  // namespace
 
-hipError_t launch_nm_export_weights(const float* mlp, void* weights, hipStream_t s) {
-    hipLaunchKernelGGL(k_nm_export_weights, dim3(1), dim3(64), 0, s, mlp, static_cast<NmWeights*>(weights));
+hipError_t launch_nm_export_weights(const float* mlp, void* weights, uint32_t fp8, hipStream_t s) {
+    hipLaunchKernelGGL(k_nm_export_weights, dim3(1), dim3(64), 0, s, mlp, static_cast<NmWeights*>(weights), fp8);
+    return hipGetLastError();
+}
+
+hipError_t launch_nm_unpack_weights(const void* weights, float* mlp, hipStream_t s) {
+    hipLaunchKernelGGL(k_nm_unpack_weights, dim3(1), dim3(64), 0, s, static_cast<const NmWeights*>(weights), mlp);
     return hipGetLastError();
 }
 

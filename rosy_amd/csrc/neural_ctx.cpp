@@ -186,6 +186,7 @@ rsy_result rsy_neural_init(rsy_ctx* c, const rsy_neural_config* cfg) {
     if (!pow2(cfg->region) || cfg->region < 2 || cfg->region > 32 || cfg->region > cfg->latent_size)
         return fail(c, RSY_INVALID_ARGUMENT, "region must be a power of two in [2, min(32, latent_size)]");
     if (cfg->init > RSY_NEURAL_INIT_RANDOM) return fail(c, RSY_INVALID_ARGUMENT, "unknown init mode");
+    if (cfg->weights > RSY_NEURAL_WEIGHTS_FP8) return fail(c, RSY_INVALID_ARGUMENT, "unknown weight format");
     if (!(cfg->lr_mlp >= 0.f) || !(cfg->lr_latent >= 0.f) || !(cfg->beta1 >= 0.f && cfg->beta1 < 1.f) ||
         !(cfg->beta2 >= 0.f && cfg->beta2 < 1.f) || !(cfg->eps > 0.f))
         return fail(c, RSY_INVALID_ARGUMENT, "invalid Adam hyper-parameters");
@@ -330,7 +331,8 @@ rsy_result rsy_neural_evaluate(rsy_ctx* c, float* psnr, uint32_t n_materials) {
     const uint32_t ns = uint32_t(n->slots.size());
     HIP_TRY(c, hipEventRecord(n->ev0, s), "event");
     HIP_TRY(c, launch_nm_export(n->lat.as<float>(), n->lat16.as<uint4>(), uint64_t(ns) * n->L * n->L, s), "export latents");
-    HIP_TRY(c, launch_nm_export_weights(n->mlp.as<float>(), n->weights_d.p, s), "export weights");
+    HIP_TRY(c, launch_nm_export_weights(n->mlp.as<float>(), n->weights_d.p, n->cfg.weights == RSY_NEURAL_WEIGHTS_FP8 ? 1u : 0u, s),
+            "export weights");
     NmEvalLaunch E{};
     E.lat16 = n->lat16.as<uint4>();
     E.weights = n->weights_d.p;
@@ -485,6 +487,29 @@ rsy_result rsy_neural_gradients(rsy_ctx* c, const uint32_t* material, const floa
     if (loss) *loss = g[kNmParams] * T.inv_norm;
     if (grad_latents && n_latents)
         HIP_TRY(c, hipMemcpy(grad_latents, n->tmp[3].p, n_latents * 4, hipMemcpyDeviceToHost), "read gradients");
+    return RSY_OK;
+}
+
+rsy_result rsy_neural_set_weights(rsy_ctx* c, uint32_t weights) {
+    rsy_result r = need_neural(c);
+    if (r != RSY_OK) return r;
+    if (weights > RSY_NEURAL_WEIGHTS_FP8) return fail(c, RSY_INVALID_ARGUMENT, "unknown weight format");
+    c->neural->cfg.weights = weights;
+    c->neural->evaluated = false;
+    return RSY_OK;
+}
+
+rsy_result rsy_neural_get_inference_mlp(rsy_ctx* c, float* mlp, uint32_t n_mlp) {
+    rsy_result r = need_neural(c);
+    if (r != RSY_OK) return r;
+    NeuralState* n = c->neural;
+    if (!mlp || n_mlp != uint32_t(kNmParams)) return fail(c, RSY_INVALID_ARGUMENT, "mlp array must hold 583 values");
+    if (!n->evaluated) return fail(c, RSY_INVALID_STATE, "rsy_neural_evaluate has not been called");
+    HIP_TRY(c, hipSetDevice(c->device), "set device");
+    HIP_TRY(c, n->tmp[4].ensure(kNmParamsPad * 4), "allocate");
+    HIP_TRY(c, launch_nm_unpack_weights(n->weights_d.p, n->tmp[4].as<float>(), c->stream), "unpack weights");
+    HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
+    HIP_TRY(c, hipMemcpy(mlp, n->tmp[4].p, kNmParams * 4, hipMemcpyDeviceToHost), "read mlp");
     return RSY_OK;
 }
 

@@ -46,12 +46,49 @@ def f16(x):
 
 
 def inference_params(r, active=None):
-    """Oracle parameters equal to what the GPU decodes with: FP16 latents, FP16 W1 / b1 / W2, FP32 b2."""
-    lat, mlp = r.neural_params()
-    q = f16(mlp)
-    q[576:583] = mlp[576:583]
+    """Oracle parameters equal to what the GPU decodes with: FP16 latents and the inference MLP
+    (FP16 or scaled FP8 W1 / b1 / W2, FP32 b2)."""
+    lat, _ = r.neural_params()
     n = r.scene.s.n_materials
-    return oracle.NeuralParams(np.arange(n), f16(lat), q, active)
+    return oracle.NeuralParams(np.arange(n), f16(lat), r.neural_inference_mlp(), active)
+
+
+def e4m3_table():
+    """All finite non-negative OCP FP8 E4M3 values with their codes (bias 7, max 448)."""
+    vals, codes = [], []
+    for code in range(0x7f):  # 0x7f is NaN
+        e, m = code >> 3, code & 7
+        vals.append(m * 2.0 ** -9 if e == 0 else (1 + m / 8) * 2.0 ** (e - 7))
+        codes.append(code)
+    return np.array(vals), np.array(codes)
+
+
+def e4m3_round(x):
+    """Round to nearest E4M3 value, ties to the even code."""
+    vals, codes = e4m3_table()
+    a = np.abs(np.asarray(x, np.float64))
+    idx = np.searchsorted(vals, a)
+    lo = np.clip(idx - 1, 0, len(vals) - 1)
+    hi = np.clip(idx, 0, len(vals) - 1)
+    dl, dh = a - vals[lo], vals[hi] - a
+    pick = np.where(dl < dh, lo, np.where(dh < dl, hi, np.where(codes[lo] % 2 == 0, lo, hi)))
+    return np.sign(x) * vals[pick]
+
+
+def reference_inference_mlp(mlp, fp8):
+    """The inference MLP restated independently: FP16 rounding, or per-layer E4M3 with scale max/448."""
+    mlp = np.asarray(mlp, np.float32)
+    out = mlp.astype(np.float64).copy()
+    layers = [np.concatenate([mlp[0:320], mlp[320:352]]), mlp[352:576]]
+    spans = [(0, 352), (352, 576)]
+    for w, (a, b) in zip(layers, spans):
+        if fp8:
+            scale = np.float32(np.abs(w).max()) / np.float32(448.0)
+            q = e4m3_round((mlp[a:b] / scale).astype(np.float32))
+            out[a:b] = (q.astype(np.float32) * scale).astype(np.float32)
+        else:
+            out[a:b] = f16(mlp[a:b])
+    return out.astype(np.float32)
 
 
 def maps(scene, m):
@@ -128,6 +165,40 @@ def test_inference_matches_fp64_decoder_of_fp16_parameters(renderer, scene):
     err = np.abs(got - want).max()
     print(f"decoder max abs error vs FP64 (FP16 parameters): {err:.2e}")
     assert err < DECODE_TOL
+
+
+@pytest.mark.parametrize("fp8", [False, True])
+def test_inference_weights_are_the_stated_rounding(renderer, scene, fp8):
+    renderer.neural_init(latent_size=16, weights=abi.NEURAL_WEIGHTS_FP8 if fp8 else abi.NEURAL_WEIGHTS_FP16)
+    renderer.neural_train(20)
+    renderer.neural_evaluate()
+    _, mlp = renderer.neural_params()
+    got = renderer.neural_inference_mlp()
+    want = reference_inference_mlp(mlp, fp8)
+    assert np.array_equal(got, want), np.abs(got - want).max()
+    if fp8:
+        assert len(np.unique(np.abs(got[0:352]))) <= 127  # at most the E4M3 magnitudes, times one scale
+
+
+def test_fp8_decoder_matches_fp64_and_reports_its_psnr(renderer, scene):
+    renderer.neural_init(latent_size=32, psnr_threshold=0.0)
+    renderer.neural_train(60)
+    p16 = renderer.neural_evaluate()
+    renderer.neural_set_weights(abi.NEURAL_WEIGHTS_FP8)
+    p8 = renderer.neural_evaluate()
+    print("PSNR FP16", p16, "FP8", p8)
+    assert (p8 <= p16 + 0.5).all() and (p8 > 20).all()
+    rng = np.random.default_rng(13)
+    material = rng.integers(0, 3, 3000)
+    uv = rng.uniform(-2, 3, (3000, 2)).astype(np.float32)
+    got = renderer.neural_decode(material, uv)
+    want = oracle.nm_decode(inference_params(renderer), material, uv)
+    assert np.abs(got - want).max() < DECODE_TOL
+    vis, sm, col, sf, cv, cs = shaded(renderer, scene, abi.SHADE_NEURAL)
+    ref, _, _ = oracle.shade_view(scene, cv, [cs], [sm], vis, neural=inference_params(renderer), override_shadow=sf)
+    err = np.abs(col - ref).max()
+    print(f"FP8 neural frame colour max abs error vs FP64: {err:.2e}")
+    assert err <= NEURAL_TOL
 
 
 def test_reported_psnr_matches_decoded_texel_centres(renderer, scene):
