@@ -16,6 +16,8 @@ is timed the same way beside it.
 Also reported:
   roofline      the dominant kernel (the tile raster): its algorithmic bytes per launch / its average
                 duration measured live with HIP events on the context's stream, against 8 TB/s;
+                `traffic` is its measured HBM bytes per launch from the committed rocprofv3
+                FETCH_SIZE / WRITE_SIZE passes (profiles/k_raster_pmc_decoder_1080p.json);
   decoder       MFMA FLOPs of the decoder per frame / the neural shading kernel's duration, against
                 the 2.5 PFLOP/s dense FP16 peak (the shading kernel is not MFMA-bound);
   cpu_baseline  the CPU oracle (fixed-point raster + FP64 shading, std::thread over the host cores)
@@ -40,6 +42,41 @@ DECODER_FLOPS_PER_PIXEL = 2 * (10 * 32 + 32 * 7)
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def choose_tile(r, scene, frame, tile):
+    """Bin size of the run: the autotuner's pick (tile 0) or the given size, fixed in the raster state."""
+    if tile == 0:
+        tile = r.autotune(frame, 10)
+        log(f"autotuned bin size {tile}")
+        return tile
+    import ctypes as C
+
+    from rosy_amd import abi
+    rs = abi.RasterState()
+    C.memmove(C.byref(rs), C.byref(scene.s.raster_state), C.sizeof(rs))
+    rs.tile_size = tile
+    r.update(scene.s.scene_data, rs)
+    log(f"bin size {tile} (fixed)")
+    return tile
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_raster_pmc_decoder_1080p.json")
+
+
+def measured_traffic(tile):
+    """HBM bytes per k_raster launch from the committed FETCH_SIZE / WRITE_SIZE passes of this workload
+    (each counter scaled by what it reads for a plain device copy of known size), or None when no
+    measurement of this bin size is committed."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("tile") != tile:
+        return None
+    return int(round(1024 * (t["fetch_kib_per_dispatch"] * t["fetch_scale"] +
+                             t["write_kib_per_dispatch"] * t["write_scale"])))
 
 
 def stage_medians(r, frame, n=10):
@@ -67,6 +104,8 @@ def main():
                          "2: 1M triangles, 2560x1440 + 4 cascades; 4: 2M triangles, 4 views + 4 cascades + decoder")
     ap.add_argument("--check", action="store_true",
                     help="workloads 1, 2, 4: compare every view's visibility buffer / shadow map with the oracle")
+    ap.add_argument("--tile", type=int, default=0, choices=[0, 32, 64],
+                    help="bin size in pixels; 0 (default): the autotuner picks it")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 times): the reference kernel that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters")
@@ -104,8 +143,7 @@ def main():
     r.load(scene)
     plain = scene.frame_desc(abi.SHADE_PLAIN)
     neural = scene.frame_desc(abi.SHADE_NEURAL)
-    tile = r.autotune(plain, 10)
-    log(f"autotuned bin size {tile}")
+    tile = choose_tile(r, scene, plain, args.tile)
 
     # decoder: train on the GPU, measure PSNR, activate the materials at >= 40 dB
     r.neural_init(decay_steps=args.train_steps)
@@ -151,7 +189,7 @@ def main():
     achieved = raster_bytes / (raster_ms * 1e-3) / 1e9
     roofline = {"kernel": "k_raster (tile raster, camera + shadow views in one launch)", "bound": "hbm",
                 "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(tile),
                 "algorithmic_bytes": raster_bytes, "avg_launch_ms": round(raster_ms, 5)}
 
     # decoder MFMA work: decoded pixels x 1088 FLOPs over the neural shading kernel
@@ -270,7 +308,7 @@ def run_views(args):
     r = rosy_amd.Renderer(device=0)
     r.load(scene)
     plain = scene.frame_desc(abi.SHADE_PLAIN)
-    tile = r.autotune(plain, 10)
+    tile = choose_tile(r, scene, plain, args.tile)
     frame = plain
     dec = None
     if args.workload == 4:
