@@ -221,8 +221,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     HIP_TRY(c, c->big_pairs.ensure(size_t(std::max<uint32_t>(1, L.big_total)) * 16), "allocate bin pairs");
     HIP_TRY(c, c->block_valid.ensure(std::max<size_t>(1, L.tblocks.size()) * 4), "allocate statistics");
     HIP_TRY(c, c->totals.ensure(32), "allocate statistics");
-    HIP_TRY(c, c->split_bins.ensure(size_t(std::max<uint32_t>(1, L.total_bins)) * 4), "allocate bins");
-    HIP_TRY(c, c->whole_items.ensure(size_t(std::max<uint32_t>(1, L.total_bins)) * 4), "allocate bins");
+    HIP_TRY(c, c->zero_bins.ensure(size_t(std::max<uint32_t>(1, L.total_bins)) * 4), "allocate bins");
     HIP_TRY(c, c->scan_partial.ensure((size_t(L.total_bins) / 4096 + 1) * kScanPartialBytes), "allocate bins");
     HIP_TRY(c, c->vblocks_d.ensure(std::max<size_t>(1, L.vblocks.size()) * sizeof(BlockEntry)), "allocate blocks");
     HIP_TRY(c, c->tblocks_d.ensure(std::max<size_t>(1, L.tblocks.size()) * sizeof(BlockEntry)), "allocate blocks");
@@ -239,7 +238,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         HIP_TRY(c, c->lists_mem.ensure(size_t(want_pairs) * 4), "allocate bin lists");
         c->list_cap = uint32_t(want_pairs);
     }
-    HIP_TRY(c, c->items.ensure(size_t(raster_items_cap(L.total_bins, c->list_cap)) * sizeof(uint2)), "allocate work items");
+    HIP_TRY(c, c->items.ensure(size_t(raster_items_cap(L.total_bins, c->list_cap)) * sizeof(uint4)), "allocate work items");
     // outputs
     c->vis_out.resize(L.n_camera);
     c->color_out.resize(L.n_camera);
@@ -293,6 +292,7 @@ rsy_result upload_frame(rsy_ctx* c) {
             sv.block_base = blocks;
             blocks += sv.blocks_x * ((vg.height + 15) / 16);
             sv.vis = reinterpret_cast<const unsigned long long*>(vg.out);
+            sv.recs = c->recs.as<TriRec>() + vg.rec_base;
             sv.color = c->color_out[ci].as<float4>();
             sv.shadow_factor = (c->debug_flags & RSY_DEBUG_SHADOW_FACTOR) ? c->shadow_factor_out[ci].as<float>() : nullptr;
             std::memcpy(sv.view_projection, vg.view_projection, sizeof(sv.view_projection));
@@ -384,10 +384,9 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     for (const auto& v : L.vg) max_tris = std::max(max_tris, v.tri_count);
     b.scatter_blocks = std::max<uint32_t>(64, std::min<uint32_t>(4096, (max_tris + 255) / 256));
     b.scan_partial = c->scan_partial.p;
-    b.split_items = c->items.as<uint2>();
+    b.items = c->items.as<uint4>();
     b.items_cap = raster_items_cap(L.total_bins, c->list_cap);
-    b.whole_items = c->whole_items.as<uint32_t>();
-    b.split_bins = c->split_bins.as<uint32_t>();
+    b.zero_bins = c->zero_bins.as<uint32_t>();
     HIP_TRY(c, launch_binning(b, s), "binning kernels");
     if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[1], s), "event");
 
@@ -396,15 +395,12 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     r.n_views = nv;
     r.total_bins = L.total_bins;
     r.tile = L.tile;
-    r.bin_counts = g.bin_counts;
-    r.bin_offsets = b.bin_offsets;
     r.lists = b.lists;
     r.list_cap = c->list_cap;
     r.recs = g.recs;
     r.rec_cap_total = L.rec_total;
-    r.split_items = b.split_items;
+    r.items = b.items;
     r.items_cap = b.items_cap;
-    r.whole_items = b.whole_items;
     r.totals = b.totals;
     HIP_TRY(c, launch_raster(r, s), "raster kernel");
     if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[2], s), "event");
@@ -534,7 +530,7 @@ rsy_result finish_frame(rsy_ctx* c) {
             if (want <= c->list_cap) return fail(c, RSY_OVERFLOW, "bin lists exceed 2^32 entries");
             HIP_TRY(c, c->lists_mem.ensure(size_t(want) * 4), "grow bin lists");
             c->list_cap = uint32_t(want);
-            HIP_TRY(c, c->items.ensure(size_t(raster_items_cap(c->layout.total_bins, c->list_cap)) * sizeof(uint2)),
+            HIP_TRY(c, c->items.ensure(size_t(raster_items_cap(c->layout.total_bins, c->list_cap)) * sizeof(uint4)),
                     "grow work items");
             invalidate_graph(c);
         }

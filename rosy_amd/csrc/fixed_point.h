@@ -1,6 +1,6 @@
-// Fixed-point raster rules used by the raster kernels and, to recover the snapped triangle, by the
-// shading kernel.  Contraction is switched off inside each function so that every FP32 operation
-// rounds where the source says, whatever the translation unit's -ffp-contract setting.
+// Fixed-point raster rules used by the raster kernels.  Contraction is switched off inside each
+// function so that every FP32 operation rounds where the source says, whatever the translation
+// unit's -ffp-contract setting.
 #pragma once
 #include <hip/hip_runtime.h>
 

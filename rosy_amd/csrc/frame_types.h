@@ -11,11 +11,17 @@ struct alignas(16) TriRec {
     float z0;                // depth plane at (X0, Y0) (depth bias folded in for shadow views)
     float dzdx;              // per sub-pixel unit
     float dzdy;
-    uint32_t id;             // triangle id in the view's draw list
-    uint32_t bbx;            // px0 | px1 << 16 (inclusive pixel bounds)
+    uint32_t id;             // triangle id in the view's draw list | kRecSwapped
+    uint32_t bbx;            // px0 | px1 << 16 (inclusive pixel bounds); kInvalidBox: no record
     uint32_t bby;            // py0 | py1 << 16
 };
 static_assert(sizeof(TriRec) == 48, "TriRec is 48 bytes");
+
+// Setup stores every record clockwise in pixel space (y down): a triangle whose snapped vertices
+// were counter-clockwise has vertices 1 and 2 exchanged, and its id carries this bit.
+constexpr uint32_t kRecSwapped = 0x80000000u;
+constexpr uint32_t kRecIdMask = 0x7fffffffu;
+constexpr uint32_t kInvalidBox = 0xffffffffu;  // bbx of a triangle slot that produced no record
 
 // Per-draw data of one draw list (shadow or opaque), in list order.
 struct DrawGPU {

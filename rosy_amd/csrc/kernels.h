@@ -45,7 +45,7 @@ struct BinLaunch {
     const uint32_t* block_valid;
     uint32_t n_tblocks;
     uint32_t* totals;  // [0] (record, bin) pairs, [1] records set up, [2] bin lists or items overflowed,
-                       // [3] split items, [4] whole-bin items, [5] split bins
+                       // [3] raster work items, [4] items of split bins, [5] bins to clear
     const TriRec* recs;
     const uint32_t* counters;
     uint32_t* lists;
@@ -53,10 +53,9 @@ struct BinLaunch {
     uint32_t shift;
     uint32_t scatter_blocks;
     void* scan_partial;     // 16 bytes per 4096 bins
-    uint2* split_items;     // raster work items of crowded bins: (bin, chunk)
-    uint32_t items_cap;     // split_items capacity
-    uint32_t* whole_items;  // bins rasterised by one workgroup (total_bins entries)
-    uint32_t* split_bins;   // bins split over several items
+    uint4* items;           // raster work items (bin, first list entry, records, split): split bins' first
+    uint32_t items_cap;     // items capacity
+    uint32_t* zero_bins;    // bins whose pixels the scatter clears: empty bins and split bins
 };
 
 struct RasterLaunch {
@@ -64,15 +63,12 @@ struct RasterLaunch {
     uint32_t n_views;
     uint32_t total_bins;
     uint32_t tile;
-    const uint32_t* bin_counts;
-    const uint32_t* bin_offsets;
     const uint32_t* lists;
     uint32_t list_cap;
     const TriRec* recs;
     uint64_t rec_cap_total;
-    const uint2* split_items;
+    const uint4* items;
     uint32_t items_cap;
-    const uint32_t* whole_items;
     const uint32_t* totals;
 };
 
@@ -82,6 +78,7 @@ struct ShadeView {
     uint32_t block_base;  // first shading block of this view
     uint32_t blocks_x;
     const unsigned long long* vis;
+    const TriRec* recs;    // the raster view's record slots, indexed by triangle id
     float4* color;
     float* shadow_factor;  // nullable debug output
     float view_projection[16];
@@ -207,7 +204,7 @@ hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s);
 hipError_t launch_binning(const BinLaunch& b, hipStream_t s);
 constexpr uint32_t kScanPartialBytes = 16;  // per 4096 bins
 hipError_t launch_raster(const RasterLaunch& r, hipStream_t s);
-uint32_t raster_items_cap(uint32_t total_bins, uint32_t list_cap);  // split work-item table entries
+uint32_t raster_items_cap(uint32_t total_bins, uint32_t list_cap);  // work-item table entries
 hipError_t launch_shade(const ShadeLaunch& l, hipStream_t s);
 hipError_t launch_nm_init(const NmInitLaunch& l, hipStream_t s);
 hipError_t launch_nm_step(const NmTrainLaunch& l, bool explicit_batch, hipStream_t s);

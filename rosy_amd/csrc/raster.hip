@@ -7,16 +7,19 @@
 //   k_transform  per (view, vertex) clip = (VP * W) * p from SoA x/y/z (12 B in, 16 B out, coalesced);
 //                                   the block's VP * W is formed once in LDS (fmaf chain, k order)
 //   k_setup      per (view, tri)    clip test, snap, cull, bbox, depth plane, bias; the record goes
-//                                   to the triangle's own slot (no append); bin counts use one atomic
-//                                   per distinct bin per wave (ballot on the bin key), records that
+//                                   to the triangle's own slot (no append); bin counts use one device
+//                                   atomic per distinct bin per block (LDS hash table), records that
 //                                   span many bins are spread over the wave
-//   k_clip       per clipped tri    near/far/guard-band Sutherland-Hodgman + fan, one wave each
-//   k_scan       all bins           single-workgroup exclusive prefix sum (+ per-frame statistics)
-//   k_scatter    per record         bin lists, slots handed out by the same ballot aggregation
-//   k_raster     per work item      one workgroup per bin (or per 256 records of a crowded bin), the
-//                                   bin's depth/ID tile in LDS: every record's rows become tasks
-//                                   spread over the workgroup; LDS atomic max resolves depth and the
-//                                   tile goes to HBM once (crowded bins merge with device atomic max)
+//   k_clip       per clipped tri    near/far/guard-band Sutherland-Hodgman + fan, one thread each
+//   k_scan_*     all bins           exclusive prefix sum of the bin counts in three launches, the
+//                                   raster work items and the list of bins to clear
+//   k_scatter    per record         bin lists into the slots setup handed out (no atomics); clears
+//                                   the empty and split bins' pixels
+//   k_raster     per work item      one workgroup per non-empty bin (or per 128 records of a crowded
+//                                   bin), the bin's depth/ID tile in LDS: every record's rows become
+//                                   tasks spread over the workgroup; LDS atomic max resolves depth and
+//                                   the tile goes to HBM once (crowded bins merge with device atomic
+//                                   max)
 // Every floating-point step is compiled without contraction; the CPU oracle restates the same
 // rules, so visibility and depth are bit-exact by construction.
 #include <hip/hip_runtime.h>
@@ -88,7 +91,7 @@ __device__ bool setup_tri(const float4 v[3], uint32_t id, const ViewGPU& V, TriR
     r.z0 = z0;
     r.dzdx = dzdx;
     r.dzdy = dzdy;
-    r.id = id;
+    r.id = id | (area2 > 0 ? kRecSwapped : 0u);
     r.bbx = uint32_t(px0) | (uint32_t(px1) << 16);
     r.bby = uint32_t(py0) | (uint32_t(py1) << 16);
     return true;
@@ -123,44 +126,8 @@ __device__ inline void rec_tiles(const TriRec& r, uint32_t shift, int& tx0, int&
     ty1 = int(r.bby >> 16) >> shift;
 }
 
-constexpr uint32_t kInvalidBox = 0xffffffffu;  // bbx of a triangle slot that produced no record
 
 constexpr int kSmallBins = 4;  // records touching at most this many bins are binned in the block's LDS table
-
-// Edge functions of a record (inside >= 0 with ties broken by the top-left rule, y down).
-struct Edges {
-    int32_t X[3], Y[3], A[3], B[3];
-    bool tl[3];
-};
-
-__device__ inline void make_edges(const TriRec& r, Edges& e) {
-    e.X[0] = r.X0; e.X[1] = r.X1; e.X[2] = r.X2;
-    e.Y[0] = r.Y0; e.Y[1] = r.Y1; e.Y[2] = r.Y2;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const int f = k == 2 ? 0 : k + 1;
-        e.A[k] = e.Y[f] - e.Y[k];
-        e.B[k] = e.X[k] - e.X[f];
-        e.tl[k] = e.A[k] > 0 || (e.A[k] == 0 && e.B[k] > 0);
-    }
-}
-
-// Visibility word of pixel (x, y) for the record (depth bits << 32 | id + 1, or depth bits alone in a
-// shadow view), or 0 when the pixel centre is not covered.
-__device__ inline unsigned long long fragment(const TriRec& r, const Edges& e, int x, int y, bool shadow) {
-    const int32_t Px = x * 256 + 128, Py = y * 256 + 128;
-    bool inside = true;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const int64_t ev = int64_t(e.A[k]) * (Px - e.X[k]) + int64_t(e.B[k]) * (Py - e.Y[k]);
-        inside &= ev > 0 || (ev == 0 && e.tl[k]);
-    }
-    if (!inside) return 0ull;
-    float z = fmaf(r.dzdy, float(Py - r.Y0), fmaf(r.dzdx, float(Px - r.X0), r.z0));
-    z = fminf(fmaxf(z, 0.0f), 1.0f);
-    const uint32_t zb = fbits(z) & 0x7fffffffu;
-    return shadow ? (unsigned long long)zb : ((unsigned long long)zb << 32) | (unsigned long long)(r.id + 1u);
-}
 
 // Exclusive prefix sum over the workgroup (wave scans with shuffles, wave totals through LDS).
 // Every thread of the block must call it; it synchronises the block.
@@ -335,8 +302,8 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
         hcount[i] = 0u;
     }
     const BlockEntry e = blocks[blockIdx.x];
-    const ViewGPU& V = views[e.view];
-    const DrawGPU& dr = (V.list ? draws1 : draws0)[e.draw];
+    const ViewGPU V = views[e.view];
+    const DrawGPU dr = (V.list ? draws1 : draws0)[e.draw];
     uint32_t* ctr = counters + e.view * kCtrPerView;
     const uint32_t t = e.start + threadIdx.x;
     if (draw_culled[V.cull_base + e.draw]) {  // the whole draw is outside the view: empty slots only
@@ -434,7 +401,7 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(const ViewGPU* views, con
                                                        uint32_t shift) {
     __shared__ float4 poly[kClipThreads][2][kClipMax];
     const uint32_t view = blockIdx.y;
-    const ViewGPU& V = views[view];
+    const ViewGPU V = views[view];
     uint32_t* ctr = counters + view * kCtrPerView;
     const uint32_t n = min(ctr[kCtrClip], V.clip_cap);
     for (uint32_t base = blockIdx.x * kClipThreads; base < n; base += gridDim.x * kClipThreads) {
@@ -509,18 +476,18 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(const ViewGPU* views, con
     }
 }
 
-// Exclusive prefix sum of the bin counts in one workgroup (the bin count is a few tens of
-// thousands), plus the frame statistics: total (record, bin) pairs, records set up, and the bin-list
-// overflow flag.
+// Bin scan geometry: each scan block covers kScanPerThread bins per thread.
 constexpr uint32_t kScanThreads = 1024;
 constexpr uint32_t kScanPerThread = 4;
 
-// Raster work items: a bin with at most kChunkRecs records is one item (its tile is stored
-// whole); a larger bin is split into one item per kChunkRecs records, and its items merge into
-// the output with device atomic max after k_scatter has zeroed the bin's pixels.  Max is order
-// independent, so the result is the same whichever item finishes first.  Split items and whole
-// bins go to two tables (split_items, whole_items) in one pass.
-constexpr uint32_t kChunkRecs = 256;
+// Raster work items: a bin with 1..kItemRecs records is one item (its tile is stored whole); a
+// larger bin is split into one item per kItemRecs records, and its items merge into the output
+// with device atomic max after k_scatter has zeroed the bin's pixels.  Max is order independent,
+// so the result is the same whichever item finishes first.  An empty bin has no item: k_scatter
+// clears its pixels.  One table holds every item as (bin, first list entry, records, split), the
+// split bins' items first.
+constexpr uint32_t kItemRecs = 256;   // records per raster work item (one per thread)
+constexpr uint32_t kChunkRecs = 128;  // records staged in LDS at a time
 
 __device__ inline uint64_t block_exclusive_scan64(uint64_t v, uint64_t* lds_waves, uint64_t& total) {
     uint64_t x = v;
@@ -546,7 +513,7 @@ __device__ inline uint64_t block_exclusive_scan64(uint64_t v, uint64_t* lds_wave
 // partial sums, one scan of the partials (plus the frame totals), and each block's final scan.
 // Each block covers kScanChunk bins with two block scans: a 64-bit one of (record count | split
 // items << 32) -- the counts of a frame stay below 2^32, so the low half never carries -- and a
-// 32-bit one of (split bins | whole bins << 16), both at most kScanChunk per block.
+// 32-bit one of (bins to clear | whole bins << 16), both at most kScanChunk per block.
 constexpr uint32_t kScanChunk = kScanThreads * kScanPerThread;
 static_assert(kScanChunk <= 65535, "bin fields of the 32-bit scan");
 
@@ -556,12 +523,12 @@ struct ScanPartial {
 };
 
 __device__ inline void bin_fields(uint32_t v, uint64_t& a, uint32_t& b) {
-    if (v > kChunkRecs) {
-        a += uint64_t(v) | (uint64_t((v + kChunkRecs - 1) / kChunkRecs) << 32);
+    if (v > kItemRecs) {
+        a += uint64_t(v) | (uint64_t((v + kItemRecs - 1) / kItemRecs) << 32);
         b += 1u;
     } else {
         a += v;
-        b += 1u << 16;
+        b += v == 0 ? 1u : 1u << 16;
     }
 }
 
@@ -592,7 +559,7 @@ __global__ __launch_bounds__(1024) void k_scan_top(ScanPartial* partial, uint32_
     for (uint32_t base = 0; base < n_parts; base += kScanThreads) {
         const uint32_t i = base + threadIdx.x;
         const ScanPartial p = i < n_parts ? partial[i] : ScanPartial{0ull, 0u, 0u};
-        // split / whole bin counts of a block fit 16 bits each, their sums over blocks may not
+        // cleared / whole bin counts of a block fit 16 bits each, their sums over blocks may not
         uint64_t ta, ts, tw;
         const uint64_t ea = block_exclusive_scan64(p.a, lds64, ta);
         const uint64_t es = block_exclusive_scan64(p.b & 0xffffu, lds64, ts);
@@ -612,20 +579,20 @@ __global__ __launch_bounds__(1024) void k_scan_top(ScanPartial* partial, uint32_
     uint32_t total_valid;
     block_exclusive_scan(vs, lds, total_valid);
     if (threadIdx.x == 0) {
-        const uint32_t pairs = uint32_t(carry_a), n_items = uint32_t(carry_a >> 32);
+        const uint32_t pairs = uint32_t(carry_a), n_split = uint32_t(carry_a >> 32);
+        const uint64_t n_items = uint64_t(n_split) + carry_w;
         totals[0] = pairs;
         totals[1] = total_valid;
         totals[2] = (pairs > list_cap || n_items > items_cap) ? 1u : 0u;
-        totals[3] = min(n_items, items_cap);
-        totals[4] = carry_w;
+        totals[3] = uint32_t(min<uint64_t>(n_items, items_cap));
+        totals[4] = n_split;
         totals[5] = carry_s;
     }
 }
 
 __global__ __launch_bounds__(1024) void k_scan_final(const uint32_t* counts, uint32_t n, uint32_t* offsets,
-                                                     const ScanPartial* partial, uint2* split_items,
-                                                     uint32_t* whole_items, uint32_t items_cap,
-                                                     uint32_t* split_bins) {
+                                                     const ScanPartial* partial, const uint32_t* totals,
+                                                     uint4* items, uint32_t items_cap, uint32_t* zero_bins) {
     __shared__ uint32_t lds[16];
     __shared__ uint64_t lds64[16];
     const uint32_t i0 = blockIdx.x * kScanChunk + threadIdx.x * kScanPerThread;
@@ -643,33 +610,39 @@ __global__ __launch_bounds__(1024) void k_scan_final(const uint32_t* counts, uin
     const uint64_t ea = base.a + block_exclusive_scan64(a, lds64, ta);
     const uint32_t eb = block_exclusive_scan(b, lds, tb);
     uint32_t run = uint32_t(ea), r_items = uint32_t(ea >> 32);
-    uint32_t r_split = base.b + (eb & 0xffffu), r_whole = base.pad + (eb >> 16);
+    uint32_t r_zero = base.b + (eb & 0xffffu);
+    uint64_t r_whole = uint64_t(totals[4]) + base.pad + (eb >> 16);  // whole bins follow every split item
 #pragma unroll
     for (uint32_t k = 0; k < kScanPerThread; ++k)
         if (i0 + k < n) {
             offsets[i0 + k] = run;
-            run += v[k];
-            if (v[k] > kChunkRecs) {
-                split_bins[r_split++] = i0 + k;
-                const uint32_t ni = (v[k] + kChunkRecs - 1) / kChunkRecs;
+            if (v[k] > kItemRecs) {
+                zero_bins[r_zero++] = i0 + k;
+                const uint32_t ni = (v[k] + kItemRecs - 1) / kItemRecs;
                 for (uint32_t c = 0; c < ni; ++c, ++r_items)
-                    if (r_items < items_cap) split_items[r_items] = make_uint2(i0 + k, c);
+                    if (r_items < items_cap)
+                        items[r_items] = make_uint4(i0 + k, run + c * kItemRecs, min(kItemRecs, v[k] - c * kItemRecs), 1u);
+            } else if (v[k] == 0) {
+                zero_bins[r_zero++] = i0 + k;
             } else {
-                whole_items[r_whole++] = i0 + k;
+                if (r_whole < items_cap) items[r_whole] = make_uint4(i0 + k, run, v[k], 0u);
+                ++r_whole;
             }
+            run += v[k];
         }
 }
 
 // Bin lists without atomics: every (record, bin) pair already holds its slot inside the bin (from
 // k_setup's block table, or the big-pair list), so the entry goes to bin_offsets[bin] + slot.  The
-// same launch zeroes the output pixels of the split bins, which k_raster then merges into.
+// same launch zeroes the output pixels of the empty bins (which get no raster work item) and of
+// the split bins (which k_raster then merges into).
 __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t n_views, const TriRec* recs,
                                                  const uint32_t* counters, const uint32_t* bin_offsets,
                                                  const uint32_t* pair_slots, const uint4* big_pairs, uint32_t* lists,
                                                  uint32_t list_cap, uint32_t shift, const uint32_t* totals,
-                                                 const uint32_t* split_bins) {
+                                                 const uint32_t* zero_bins) {
     const uint32_t view = blockIdx.y;
-    const ViewGPU& V = views[view];
+    const ViewGPU V = views[view];
     const uint32_t nbig = min(counters[view * kCtrPerView + kCtrBig], V.big_cap);
     const uint32_t n = V.tri_count + nbig;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -697,162 +670,211 @@ __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t 
             if (pos < list_cap) lists[pos] = bp.z;
         }
     }
-    // zero the split bins' pixels
-    const uint32_t T = 1u << shift, px = T * T;
-    const uint64_t work = uint64_t(totals[5]) * px;
-    const uint64_t stride = uint64_t(gridDim.x) * gridDim.y * blockDim.x;
-    for (uint64_t w = (uint64_t(blockIdx.y) * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x; w < work; w += stride) {
-        const uint32_t bin = split_bins[w / px], p = uint32_t(w % px);
+    // zero the pixels of the empty and split bins: one block per bin at a time (the bin, its view
+    // and its origin are block-uniform), consecutive threads on consecutive pixels of a bin row
+    const uint32_t T = 1u << shift;
+    const uint32_t n_zero = totals[5];
+    const uint32_t n_blocks = gridDim.x * gridDim.y;
+    for (uint32_t zb = blockIdx.y * gridDim.x + blockIdx.x; zb < n_zero; zb += n_blocks) {
+        const uint32_t bin = zero_bins[zb];
         uint32_t v = 0;
         while (v + 1 < n_views && views[v + 1].tile_base <= bin) ++v;
         const ViewGPU& W = views[v];
-        const uint32_t local = bin - W.tile_base;
-        const uint32_t x = (local % W.tiles_x) * T + p % T, y = (local / W.tiles_x) * T + p / T;
-        if (x < W.width && y < W.height) {
-            const size_t o = size_t(y) * W.width + x;
-            if (W.is_shadow) reinterpret_cast<uint32_t*>(W.out)[o] = 0u;
-            else reinterpret_cast<unsigned long long*>(W.out)[o] = 0ull;
+        // view fields copied out: the stores below could alias the view table for the compiler
+        const uint32_t width = W.width, height = W.height, local = bin - W.tile_base;
+        const uint32_t x0 = (local % W.tiles_x) * T, y0 = (local / W.tiles_x) * T;
+        const bool shadow = W.is_shadow != 0;
+        const uint64_t out = W.out;
+        for (uint32_t p = threadIdx.x; p < T * T; p += blockDim.x) {
+            const uint32_t x = x0 + (p & (T - 1u)), y = y0 + (p >> shift);
+            if (x < width && y < height) {
+                const size_t o = size_t(y) * width + x;
+                if (shadow) reinterpret_cast<uint32_t*>(out)[o] = 0u;
+                else reinterpret_cast<unsigned long long*>(out)[o] = 0ull;
+            }
         }
     }
 }
 
+// Per-record raster flags, computed once when a record is staged for a bin: bit k (k < 3) is set when
+// edge k is not a top-left edge (its value must exceed 0: F_k = E_k - 1), bit 3 when the edge
+// values need 64 bits.
+constexpr uint32_t kMetaWide = 8u;
+
+__device__ inline uint32_t record_meta(const TriRec& r) {
+    const int32_t X[3] = {r.X0, r.X1, r.X2}, Y[3] = {r.Y0, r.Y1, r.Y2};
+    uint32_t meta = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int f = k == 2 ? 0 : k + 1;
+        const int32_t A = Y[f] - Y[k], B = X[k] - X[f];
+        const bool tl = A > 0 || (A == 0 && B > 0);
+        meta |= tl ? 0u : (1u << k);
+    }
+    // the whole triangle's box (sub-pixel units) bounds every |E| over it: under 64 pixels on each
+    // side every |E| < 2^30
+    const int32_t wx = max(X[0], max(X[1], X[2])) - min(X[0], min(X[1], X[2]));
+    const int32_t wy = max(Y[0], max(Y[1], Y[2])) - min(Y[0], min(Y[1], Y[2]));
+    if (wx >= 64 * 256 || wy >= 64 * 256) meta |= kMetaWide;
+    return meta;
+}
+
 // One row of a record inside the bin: E_k(x) for the three edges advances by 256 * A_k per pixel
-// (exact integer steps; EdgeT = int32_t when the triangle's box is under 64 pixels on each side, so
-// every |E| < 2^30, else int64_t), depth is the FP32 plane rule of setup, and covered pixels merge
-// into the LDS row with atomic max.  The top-left rule is folded into the edge value:
-// F_k = E_k - 1 + tl_k, so "E_k > 0, or E_k == 0 on a top-left edge" is F_k >= 0 for integer E_k,
-// and a pixel is covered when (F_0 | F_1 | F_2) has a clear sign bit -- one test, no branches per
-// edge.
-template <typename EdgeT, int STRIDE>
-__device__ inline void raster_row(const TriRec& r, const Edges& ed, int ox, int oy, int y, int bx0, int bx1,
-                                  bool shadow, unsigned long long* tile) {
+// (exact integer steps; EdgeT = int32_t unless the record is wide), depth is the FP32 plane rule of
+// setup, and covered pixels merge into the LDS row with atomic max (64-bit depth | id + 1 words in
+// a camera view, 32-bit depth words in a shadow view).  The top-left rule is folded into the edge
+// value: F_k = E_k - 1 + tl_k, so "E_k > 0, or E_k == 0 on a top-left edge" is F_k >= 0 for
+// integer E_k, and a pixel is covered when (F_0 | F_1 | F_2) has a clear sign bit -- one test, no
+// branches per edge.
+template <typename EdgeT, bool kShadow, int STRIDE, typename Word>
+__device__ inline void raster_row(const TriRec& r, uint32_t meta, int ox, int oy, int y, int bx0, int bx1,
+                                  Word* tile) {
+    const int32_t X[3] = {r.X0, r.X1, r.X2}, Y[3] = {r.Y0, r.Y1, r.Y2};
     const int32_t Py = (oy + y) * 256 + 128;
     int32_t Px = (ox + bx0) * 256 + 128;
     EdgeT f[3], step[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        f[k] = EdgeT(ed.A[k]) * EdgeT(Px - ed.X[k]) + EdgeT(ed.B[k]) * EdgeT(Py - ed.Y[k]) - EdgeT(ed.tl[k] ? 0 : 1);
-        step[k] = EdgeT(ed.A[k]) * EdgeT(256);
+        const int n = k == 2 ? 0 : k + 1;
+        const int32_t A = Y[n] - Y[k], B = X[k] - X[n];
+        f[k] = EdgeT(A) * EdgeT(Px - X[k]) + EdgeT(B) * EdgeT(Py - Y[k]) - EdgeT((meta >> k) & 1u);
+        step[k] = EdgeT(A) * EdgeT(256);
     }
     const float fy = float(Py - r.Y0);
-    const unsigned long long idw = shadow ? 0ull : (unsigned long long)(r.id + 1u);
-    const int sh = shadow ? 0 : 32;
-    unsigned long long* row = tile + y * STRIDE;
+    const uint32_t idw = (r.id & kRecIdMask) + 1u;
+    Word* row = tile + y * STRIDE;
     for (int x = bx0; x <= bx1; ++x, Px += 256) {
         if ((f[0] | f[1] | f[2]) >= 0) {
             float z = fmaf(r.dzdy, fy, fmaf(r.dzdx, float(Px - r.X0), r.z0));
             z = fminf(fmaxf(z, 0.0f), 1.0f);
             const uint32_t zb = fbits(z) & 0x7fffffffu;
-            atomicMax(&row[x], ((unsigned long long)zb << sh) | idw);
+            if constexpr (kShadow) atomicMax(&row[x], zb);
+            else atomicMax(&row[x], ((unsigned long long)zb << 32) | idw);
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) f[k] += step[k];
     }
 }
 
-// Bin raster: one workgroup per bin of T x T pixels, the bin's visibility (or depth) tile in LDS
-// (rows padded to T + 1 words so that lanes on the same column of different rows fall in different
-// banks).  The bin list is taken 256 records at a time; each record's box is clipped to the bin and
-// its rows become tasks: a prefix sum over the rows numbers them, and every thread takes tasks
-// (record, row) in turn, so small and large triangles spread evenly over the workgroup.  Fragments
-// merge with LDS atomic max (depth bits above triangle id + 1: nearer depth wins under reverse-Z,
-// the later triangle wins ties).  Workgroups are mapped so that neighbouring bins run on the same
-// XCD (workgroups are dealt round-robin over the 8 XCDs, each with its own L2).
+// The records of one work item through the bin's LDS tile.  Every thread loads one record of the
+// item (its box clipped to the bin, its raster flags computed) in one round of loads; the records
+// then go through LDS kChunkRecs at a time, each chunk staged by its own threads.  A staged
+// record's rows become tasks (a prefix sum numbers them and each record writes its index into the
+// task-owner table), and every thread takes tasks in turn, so small and large triangles spread
+// evenly over the workgroup.
+template <int T, bool kShadow, typename Word>
+__device__ inline void raster_item(Word* tile, TriRec* chunk_recs, uint8_t* owner, uint32_t* row_start,
+                                   uint32_t* scan_lds, const uint32_t* lists, const TriRec* recs,
+                                   uint64_t rec_cap_total, uint32_t off, uint32_t end, int ox, int oy) {
+    static_assert(kItemRecs == 256 && kItemRecs % kChunkRecs == 0, "one record per thread");
+    constexpr int S = T + 1;
+    const uint32_t i = off + threadIdx.x;
+    TriRec r{};
+    uint32_t my_rows = 0;
+    if (i < end) {
+        const uint32_t ri = lists[i];
+        if (ri < rec_cap_total) {
+            r = recs[ri];
+            const int bx0 = max(int(r.bbx & 0xffffu), ox) - ox, bx1 = min(int(r.bbx >> 16), ox + T - 1) - ox;
+            const int by0 = max(int(r.bby & 0xffffu), oy) - oy, by1 = min(int(r.bby >> 16), oy + T - 1) - oy;
+            if (bx0 <= bx1 && by0 <= by1) {
+                my_rows = uint32_t(by1 - by0 + 1);
+                // the staged copy keeps the clipped box in bbx and the raster flags in bby
+                r.bbx = uint32_t(bx0) | (uint32_t(bx1) << 8) | (uint32_t(by0) << 16);
+                r.bby = record_meta(r);
+            }
+        }
+    }
+    const uint32_t n = end > off ? end - off : 0u;  // at most kItemRecs
+    const uint32_t slot = threadIdx.x % kChunkRecs;
+    for (uint32_t c = 0; c * kChunkRecs < n; ++c) {
+        const bool stager = threadIdx.x / kChunkRecs == c;
+        const uint32_t rows = stager ? my_rows : 0u;
+        if (rows) chunk_recs[slot] = r;
+        uint32_t total_rows;
+        const uint32_t first = block_exclusive_scan(rows, scan_lds, total_rows);  // synchronises
+        if (stager) row_start[slot] = first;
+        for (uint32_t j = 0; j < rows; ++j) owner[first + j] = uint8_t(slot);
+        __syncthreads();
+        for (uint32_t task = threadIdx.x; task < total_rows; task += 256) {
+            const uint32_t k = owner[task];
+            const TriRec q = chunk_recs[k];
+            const int bx0 = int(q.bbx & 0xffu), bx1 = int((q.bbx >> 8) & 0xffu);
+            const int y = int((q.bbx >> 16) & 0xffu) + int(task - row_start[k]);
+            if (q.bby & kMetaWide)
+                raster_row<int64_t, kShadow, S>(q, q.bby, ox, oy, y, bx0, bx1, tile);
+            else
+                raster_row<int32_t, kShadow, S>(q, q.bby, ox, oy, y, bx0, bx1, tile);
+        }
+        __syncthreads();
+    }
+}
+
+// Bin raster: one workgroup per work item (a whole non-empty bin, or up to kItemRecs records of a
+// crowded bin), the bin's tile in LDS (rows padded to T + 1 words so that lanes on the same column of
+// different rows fall in different banks): 64-bit visibility words (depth bits above triangle id
+// + 1: nearer depth wins under reverse-Z, the later triangle wins ties) in a camera view, 32-bit
+// depth words in a shadow view.  A whole bin stores its tile; the items of a split bin merge into
+// the cleared bin with global atomic max.  Workgroups are dealt round-robin over the 8 XCDs, and
+// consecutive items are neighbouring bins, so the dense regions of a view spread over every XCD.
 template <int T>
-__global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n_views, const uint32_t* bin_counts,
-                                                const uint32_t* bin_offsets, const uint32_t* lists, uint32_t list_cap,
-                                                const TriRec* recs, uint64_t rec_cap_total, const uint2* split_items,
-                                                const uint32_t* whole_items, const uint32_t* totals) {
-    constexpr int S = T + 1;  // padded row stride (64-bit words)
+__global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n_views, const uint32_t* lists,
+                                                uint32_t list_cap, const TriRec* recs, uint64_t rec_cap_total,
+                                                const uint4* items, const uint32_t* totals) {
+    constexpr int S = T + 1;  // padded row stride (words)
     __shared__ unsigned long long tile[T * S];
-    __shared__ TriRec chunk_recs[256];
-    __shared__ uint32_t chunk_box[256];   // clipped box: x0 | x1 << 8 | y0 << 16 | y1 << 24 (bin-local)
-    __shared__ uint32_t row_start[257];   // exclusive prefix of row counts
+    __shared__ TriRec chunk_recs[kChunkRecs];
+    __shared__ uint8_t owner[kChunkRecs * T];  // task (row) -> record of the chunk
+    __shared__ uint32_t row_start[kChunkRecs];  // exclusive prefix of row counts
     __shared__ uint32_t scan_lds[4];
 
-    // split items first (the heaviest work starts first); workgroups are dealt round-robin over the
-    // 8 XCDs, so consecutive items -- neighbouring bins -- spread the dense regions of a view over
-    // every XCD
-    const uint32_t n_split_items = totals[3];
-    uint2 it;
-    if (blockIdx.x < n_split_items) {
-        it = split_items[blockIdx.x];
-    } else {
-        if (blockIdx.x - n_split_items >= totals[4]) return;
-        it = make_uint2(whole_items[blockIdx.x - n_split_items], 0u);
-    }
+    // the grid covers the item table's capacity: workgroups past the frame's item count exit (both
+    // loads are issued together)
+    const uint32_t n_items = totals[3];
+    const uint4 it = items[blockIdx.x];
+    if (blockIdx.x >= n_items) return;
     const uint32_t bin = it.x;
     uint32_t view = 0;
     while (view + 1 < n_views && views[view + 1].tile_base <= bin) ++view;
-    const ViewGPU& V = views[view];
+    const ViewGPU V = views[view];
     const uint32_t local_bin = bin - V.tile_base;
     const int ox = int(local_bin % V.tiles_x) * T, oy = int(local_bin / V.tiles_x) * T;
-    const bool shadow = V.is_shadow != 0;
-    const uint32_t count = bin_counts[bin];
-    const bool split = count > kChunkRecs;
+    // view fields copied out: the output stores could alias the view table for the compiler
+    const int width = int(V.width), height = int(V.height);
+    const uint64_t out_addr = V.out;
+    const bool split = it.w != 0u;
+    const uint32_t off = it.y;
+    const uint32_t end = uint32_t(min(uint64_t(off) + it.z, uint64_t(list_cap)));
 
-    for (int p = threadIdx.x; p < T * S; p += 256) tile[p] = 0ull;
-
-    const uint32_t off = bin_offsets[bin] + it.y * kChunkRecs;
-    const uint32_t end = min(min(bin_offsets[bin] + count, off + kChunkRecs), list_cap);
-    for (uint32_t base = off; base < end; base += 256) {
-        const uint32_t i = base + threadIdx.x;
-        uint32_t rows = 0;
-        if (i < end) {
-            const uint32_t ri = lists[i];
-            if (ri < rec_cap_total) {
-                const TriRec r = recs[ri];
-                const int bx0 = max(int(r.bbx & 0xffffu), ox) - ox, bx1 = min(int(r.bbx >> 16), ox + T - 1) - ox;
-                const int by0 = max(int(r.bby & 0xffffu), oy) - oy, by1 = min(int(r.bby >> 16), oy + T - 1) - oy;
-                if (bx0 <= bx1 && by0 <= by1) {
-                    rows = uint32_t(by1 - by0 + 1);
-                    chunk_recs[threadIdx.x] = r;
-                    chunk_box[threadIdx.x] = uint32_t(bx0) | (uint32_t(bx1) << 8) | (uint32_t(by0) << 16) |
-                                             (uint32_t(by1) << 24);
-                }
+    if (V.is_shadow) {
+        uint32_t* t32 = reinterpret_cast<uint32_t*>(tile);
+        for (int p = threadIdx.x; p < T * S; p += 256) t32[p] = 0u;
+        raster_item<T, true>(t32, chunk_recs, owner, row_start, scan_lds, lists, recs, rec_cap_total, off, end, ox,
+                             oy);
+        __syncthreads();  // tile complete (and cleared, for an item without records)
+        uint32_t* out = reinterpret_cast<uint32_t*>(out_addr);
+        for (int p = threadIdx.x; p < T * T; p += 256) {
+            const int x = ox + p % T, y = oy + p / T;
+            if (x < width && y < height) {
+                const uint32_t w = t32[(p / T) * S + p % T];
+                const size_t o = size_t(y) * width + x;
+                if (!split) out[o] = w;
+                else if (w != 0u) atomicMax(out + o, w);
             }
         }
-        uint32_t total_rows;
-        const uint32_t first = block_exclusive_scan(rows, scan_lds, total_rows);  // synchronises
-        row_start[threadIdx.x] = first;
-        if (threadIdx.x == 255) row_start[256] = first + rows;
+    } else {
+        for (int p = threadIdx.x; p < T * S; p += 256) tile[p] = 0ull;
+        raster_item<T, false>(tile, chunk_recs, owner, row_start, scan_lds, lists, recs, rec_cap_total, off, end, ox,
+                              oy);
         __syncthreads();
-        for (uint32_t task = threadIdx.x; task < total_rows; task += 256) {
-            // the record whose rows contain `task`: the last record whose first row is <= task
-            uint32_t lo = 0, hi = 256;
-#pragma unroll
-            for (int it = 0; it < 8; ++it) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (row_start[mid] <= task) lo = mid; else hi = mid;
-            }
-            const TriRec r = chunk_recs[lo];
-            const uint32_t box = chunk_box[lo];
-            const int bx0 = int(box & 0xffu), bx1 = int((box >> 8) & 0xffu);
-            const int y = int((box >> 16) & 0xffu) + int(task - row_start[lo]);
-            Edges ed;
-            make_edges(r, ed);
-            // the whole triangle's box (sub-pixel units) bounds every |E| over it
-            const int32_t wx = max(ed.X[0], max(ed.X[1], ed.X[2])) - min(ed.X[0], min(ed.X[1], ed.X[2]));
-            const int32_t wy = max(ed.Y[0], max(ed.Y[1], ed.Y[2])) - min(ed.Y[0], min(ed.Y[1], ed.Y[2]));
-            if (wx < 64 * 256 && wy < 64 * 256)
-                raster_row<int32_t, S>(r, ed, ox, oy, y, bx0, bx1, shadow, tile);
-            else
-                raster_row<int64_t, S>(r, ed, ox, oy, y, bx0, bx1, shadow, tile);
-        }
-        __syncthreads();
-    }
-    __syncthreads();
-    for (int p = threadIdx.x; p < T * T; p += 256) {
-        const int x = ox + p % T, y = oy + p / T;
-        if (x < int(V.width) && y < int(V.height)) {
-            const size_t o = size_t(y) * V.width + x;
-            const unsigned long long w = tile[(p / T) * S + p % T];
-            if (!split) {
-                if (shadow) reinterpret_cast<uint32_t*>(V.out)[o] = uint32_t(w);
-                else reinterpret_cast<unsigned long long*>(V.out)[o] = w;
-            } else if (w != 0ull) {
-                if (shadow) atomicMax(reinterpret_cast<uint32_t*>(V.out) + o, uint32_t(w));
-                else atomicMax(reinterpret_cast<unsigned long long*>(V.out) + o, w);
+        unsigned long long* out = reinterpret_cast<unsigned long long*>(out_addr);
+        for (int p = threadIdx.x; p < T * T; p += 256) {
+            const int x = ox + p % T, y = oy + p / T;
+            if (x < width && y < height) {
+                const unsigned long long w = tile[(p / T) * S + p % T];
+                const size_t o = size_t(y) * width + x;
+                if (!split) out[o] = w;
+                else if (w != 0ull) atomicMax(out + o, w);
             }
         }
     }
@@ -892,27 +914,25 @@ hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, partial, parts, b.block_valid, b.n_tblocks,
                        b.list_cap, b.items_cap, b.totals);
     hipLaunchKernelGGL(k_scan_final, dim3(parts), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, b.bin_offsets,
-                       partial, b.split_items, b.whole_items, b.items_cap, b.split_bins);
+                       partial, b.totals, b.items, b.items_cap, b.zero_bins);
     hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.n_views, b.recs,
                        b.counters, b.bin_offsets, b.pair_slots, b.big_pairs, b.lists, b.list_cap, b.shift, b.totals,
-                       b.split_bins);
+                       b.zero_bins);
     return hipGetLastError();
 }
 
 hipError_t launch_raster(const RasterLaunch& r, hipStream_t s) {
-    // one workgroup per work item: split items first, then whole bins; the split-item table size
-    // plus the bin count bounds the item count (extra workgroups exit)
-    const uint32_t grid = r.items_cap + r.total_bins;
+    // one workgroup per work-item slot (extra workgroups exit)
+    const uint32_t grid = r.items_cap;
+    if (grid == 0) return hipSuccess;
     switch (r.tile) {
         case 32:
-            hipLaunchKernelGGL(k_raster<32>, dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.bin_counts,
-                               r.bin_offsets, r.lists, r.list_cap, r.recs, r.rec_cap_total, r.split_items,
-                               r.whole_items, r.totals);
+            hipLaunchKernelGGL(k_raster<32>, dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.lists, r.list_cap,
+                               r.recs, r.rec_cap_total, r.items, r.totals);
             break;
         case 64:
-            hipLaunchKernelGGL(k_raster<64>, dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.bin_counts,
-                               r.bin_offsets, r.lists, r.list_cap, r.recs, r.rec_cap_total, r.split_items,
-                               r.whole_items, r.totals);
+            hipLaunchKernelGGL(k_raster<64>, dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.lists, r.list_cap,
+                               r.recs, r.rec_cap_total, r.items, r.totals);
             break;
         default: return hipErrorInvalidValue;
     }
@@ -920,7 +940,7 @@ hipError_t launch_raster(const RasterLaunch& r, hipStream_t s) {
 }
 
 uint32_t raster_items_cap(uint32_t total_bins, uint32_t list_cap) {
-    return total_bins + (list_cap + kChunkRecs - 1) / kChunkRecs;
+    return total_bins + (list_cap + kItemRecs - 1) / kItemRecs;
 }
 
 }  // namespace rsy

@@ -217,25 +217,31 @@ __device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView
     pg.i1 = ip[1];
     pg.i2 = ip[2];
     const float4* xv = L.xv + L.xv_bases[vi] + dr.xv_off;
-    const float4 c0 = xv[pg.i0], c1 = xv[pg.i1], c2 = xv[pg.i2];
+    const float w0 = xv[pg.i0].w, w1 = xv[pg.i1].w, w2 = xv[pg.i2].w;
 
-    // Perspective-correct barycentrics at the pixel centre: from the exact integer edge functions
-    // of the snapped triangle (the one the raster pass covered) divided by each vertex's w, or, for
-    // a triangle that needed clipping, from b ~ cross(x - u w, y - v w) over the clip positions.
+    // Perspective-correct barycentrics at the pixel centre.  A triangle that needed no clipping has
+    // its record in the triangle's slot: the exact integer edge functions of the snapped vertices
+    // the raster pass covered, each over its vertex's w (as e_k * w_i * w_j, normalised below).  The
+    // record may hold vertices 1 and 2 exchanged, which negates all three edge functions and swaps
+    // the last two.  A clipped triangle's slot is empty; its barycentrics come from
+    // b ~ cross(x - u w, y - v w) over the clip positions.
     f3 lam;
-    if (all_inside(c0, c1, c2)) {
-        const float hw = 0.5f * float(V.width), hh = 0.5f * float(V.height);
-        int32_t X0, Y0, X1, Y1, X2, Y2;
-        float zs;
-        snap(c0, hw, hh, X0, Y0, zs);
-        snap(c1, hw, hh, X1, Y1, zs);
-        snap(c2, hw, hh, X2, Y2, zs);
+    const uint4* rq = reinterpret_cast<const uint4*>(V.recs + tri);
+    const uint4 q2 = rq[2];  // dzdy, id, bbx, bby
+    if (q2.z != kInvalidBox) {
+        const uint4 q0 = rq[0];
+        const uint2 q1 = *reinterpret_cast<const uint2*>(rq + 1);
+        const int32_t X0 = int32_t(q0.x), Y0 = int32_t(q0.y), X1 = int32_t(q0.z), Y1 = int32_t(q0.w);
+        const int32_t X2 = int32_t(q1.x), Y2 = int32_t(q1.y);
         const int32_t Px = int32_t(x) * 256 + 128, Py = int32_t(y) * 256 + 128;
-        const int64_t e0 = int64_t(X1 - Px) * (Y2 - Py) - int64_t(X2 - Px) * (Y1 - Py);
-        const int64_t e1 = int64_t(X2 - Px) * (Y0 - Py) - int64_t(X0 - Px) * (Y2 - Py);
-        const int64_t e2 = int64_t(X0 - Px) * (Y1 - Py) - int64_t(X1 - Px) * (Y0 - Py);
-        lam = mk(float(e0) / c0.w, float(e1) / c1.w, float(e2) / c2.w);
+        const float e0 = float(int64_t(X1 - Px) * (Y2 - Py) - int64_t(X2 - Px) * (Y1 - Py));
+        const float e1 = float(int64_t(X2 - Px) * (Y0 - Py) - int64_t(X0 - Px) * (Y2 - Py));
+        const float e2 = float(int64_t(X0 - Px) * (Y1 - Py) - int64_t(X1 - Px) * (Y0 - Py));
+        const bool swapped = (q2.y & kRecSwapped) != 0u;
+        const float b1 = swapped ? e2 : e1, b2 = swapped ? e1 : e2;
+        lam = mk(e0 * (w1 * w2), b1 * (w0 * w2), b2 * (w0 * w1));
     } else {
+        const float4 c0 = xv[pg.i0], c1 = xv[pg.i1], c2 = xv[pg.i2];
         const float u = (float(x) + 0.5f) * (2.0f / float(V.width)) - 1.0f;
         const float v = (float(y) + 0.5f) * (2.0f / float(V.height)) - 1.0f;
         const f3 a = mk(c0.x - u * c0.w, c1.x - u * c1.w, c2.x - u * c2.w);
@@ -243,7 +249,7 @@ __device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView
         lam = cross(a, b);
     }
     lam = lam * (1.0f / (lam.x + lam.y + lam.z));
-    const float view_depth = c0.w * lam.x + c1.w * lam.y + c2.w * lam.z;
+    const float view_depth = w0 * lam.x + w1 * lam.y + w2 * lam.z;
     int cascade = -1;
     for (uint32_t k = 0; k < L.n_cascades; ++k)
         if (view_depth <= L.cascades[k].split_far) {
