@@ -45,7 +45,8 @@ struct BinLaunch {
     const uint32_t* block_valid;
     uint32_t n_tblocks;
     uint32_t* totals;  // [0] (record, bin) pairs, [1] records set up, [2] bin lists or items overflowed,
-                       // [3] raster work items, [4] items of split bins, [5] bins to clear
+                       // [3] raster work items, [4] first mid-class item, [5] bins to clear,
+                       // [6] first split item, [7] first light-class item
     const TriRec* recs;
     const uint32_t* counters;
     uint32_t* lists;
@@ -202,7 +203,7 @@ struct NmPointLaunch {
 hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
 hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s);
 hipError_t launch_binning(const BinLaunch& b, hipStream_t s);
-constexpr uint32_t kScanPartialBytes = 16;  // per 4096 bins
+constexpr uint32_t kScanPartialBytes = 32;  // per 4096 bins
 hipError_t launch_raster(const RasterLaunch& r, hipStream_t s);
 uint32_t raster_items_cap(uint32_t total_bins, uint32_t list_cap);  // work-item table entries
 hipError_t launch_shade(const ShadeLaunch& l, hipStream_t s);

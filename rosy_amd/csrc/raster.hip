@@ -484,10 +484,14 @@ constexpr uint32_t kScanPerThread = 4;
 // larger bin is split into one item per kItemRecs records, and its items merge into the output
 // with device atomic max after k_scatter has zeroed the bin's pixels.  Max is order independent,
 // so the result is the same whichever item finishes first.  An empty bin has no item: k_scatter
-// clears its pixels.  One table holds every item as (bin, first list entry, records, split), the
-// split bins' items first.
+// clears its pixels.  One table holds every item as (bin, first list entry, records, split),
+// ordered so that the longest items start first and the short ones fill the end of the launch:
+// whole bins with at least kHeavyRecs records, then those with at least kMidRecs, then the split
+// bins' items (each a share of a crowded bin), then the remaining whole bins.
 constexpr uint32_t kItemRecs = 256;   // records per raster work item (one per thread)
 constexpr uint32_t kChunkRecs = 128;  // records staged in LDS at a time
+constexpr uint32_t kHeavyRecs = 128;
+constexpr uint32_t kMidRecs = 48;
 
 __device__ inline uint64_t block_exclusive_scan64(uint64_t v, uint64_t* lds_waves, uint64_t& total) {
     uint64_t x = v;
@@ -509,26 +513,38 @@ __device__ inline uint64_t block_exclusive_scan64(uint64_t v, uint64_t* lds_wave
     return wave_off + x - v;
 }
 
-// Bin offsets (exclusive prefix of the counts) and the work-item tables, in three launches: block
+// Bin offsets (exclusive prefix of the counts) and the work-item table, in three launches: block
 // partial sums, one scan of the partials (plus the frame totals), and each block's final scan.
-// Each block covers kScanChunk bins with two block scans: a 64-bit one of (record count | split
-// items << 32) -- the counts of a frame stay below 2^32, so the low half never carries -- and a
-// 32-bit one of (bins to clear | whole bins << 16), both at most kScanChunk per block.
+// Each block covers kScanChunk bins with two 64-bit block scans: one of (record count | split
+// items << 32) -- the counts of a frame stay below 2^32, so the low half never carries -- and one
+// of four 16-bit bin classes (bins to clear, heavy, mid and light whole bins), each at most
+// kScanChunk per block.
 constexpr uint32_t kScanChunk = kScanThreads * kScanPerThread;
-static_assert(kScanChunk <= 65535, "bin fields of the 32-bit scan");
+static_assert(kScanChunk <= 65535, "16-bit bin class fields");
 
 struct ScanPartial {
-    uint64_t a;
-    uint32_t b, pad;
+    uint64_t a;      // records | split items << 32
+    uint32_t f[4];   // bins to clear, heavy, mid, light whole bins
+    uint32_t pad[2];
 };
+static_assert(sizeof(ScanPartial) == kScanPartialBytes, "scan partial size");
 
-__device__ inline void bin_fields(uint32_t v, uint64_t& a, uint32_t& b) {
-    if (v > kItemRecs) {
+enum BinClass : uint32_t { kClassZero = 0, kClassHeavy = 1, kClassMid = 2, kClassLight = 3, kClassSplit = 4 };
+
+__device__ inline uint32_t bin_class(uint32_t v) {
+    if (v > kItemRecs) return kClassSplit;
+    if (v == 0) return kClassZero;
+    return v >= kHeavyRecs ? kClassHeavy : (v >= kMidRecs ? kClassMid : kClassLight);
+}
+
+__device__ inline void bin_fields(uint32_t v, uint64_t& a, uint64_t& b) {
+    const uint32_t cls = bin_class(v);
+    if (cls == kClassSplit) {
         a += uint64_t(v) | (uint64_t((v + kItemRecs - 1) / kItemRecs) << 32);
-        b += 1u;
+        b += 1u;  // a split bin is cleared too
     } else {
         a += v;
-        b += v == 0 ? 1u : 1u << 16;
+        b += uint64_t(1) << (16u * cls);
     }
 }
 
@@ -536,16 +552,19 @@ __global__ __launch_bounds__(1024) void k_scan_partial(const uint32_t* counts, u
     __shared__ uint32_t lds[16];
     __shared__ uint64_t lds64[16];
     const uint32_t i0 = blockIdx.x * kScanChunk + threadIdx.x * kScanPerThread;
-    uint64_t a = 0;
-    uint32_t b = 0;
+    uint64_t a = 0, b = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kScanPerThread; ++k)
         if (i0 + k < n) bin_fields(counts[i0 + k], a, b);
-    uint64_t ta;
-    uint32_t tb;
+    uint64_t ta, tb;
     block_exclusive_scan64(a, lds64, ta);
-    block_exclusive_scan(b, lds, tb);
-    if (threadIdx.x == 0) partial[blockIdx.x] = ScanPartial{ta, tb, 0u};
+    block_exclusive_scan64(b, lds64, tb);
+    if (threadIdx.x == 0) {
+        ScanPartial p{};
+        p.a = ta;
+        for (int k = 0; k < 4; ++k) p.f[k] = uint32_t(tb >> (16 * k)) & 0xffffu;
+        partial[blockIdx.x] = p;
+    }
 }
 
 // One block: exclusive scan of the block partials (in place, as the blocks' bases) and the totals.
@@ -554,25 +573,31 @@ __global__ __launch_bounds__(1024) void k_scan_top(ScanPartial* partial, uint32_
                                                    uint32_t list_cap, uint32_t items_cap, uint32_t* totals) {
     __shared__ uint32_t lds[16];
     __shared__ uint64_t lds64[16];
-    uint64_t carry_a = 0;
-    uint32_t carry_s = 0, carry_w = 0;
+    uint64_t carry_a = 0, carry_01 = 0, carry_23 = 0;
     for (uint32_t base = 0; base < n_parts; base += kScanThreads) {
         const uint32_t i = base + threadIdx.x;
-        const ScanPartial p = i < n_parts ? partial[i] : ScanPartial{0ull, 0u, 0u};
-        // cleared / whole bin counts of a block fit 16 bits each, their sums over blocks may not
-        uint64_t ta, ts, tw;
+        ScanPartial p{};
+        if (i < n_parts) p = partial[i];
+        // class counts of a block fit 16 bits, their sums over blocks 32 bits: two classes per scan
+        const uint64_t v01 = uint64_t(p.f[0]) | (uint64_t(p.f[1]) << 32);
+        const uint64_t v23 = uint64_t(p.f[2]) | (uint64_t(p.f[3]) << 32);
+        uint64_t ta, t01, t23;
         const uint64_t ea = block_exclusive_scan64(p.a, lds64, ta);
-        const uint64_t es = block_exclusive_scan64(p.b & 0xffffu, lds64, ts);
-        const uint64_t ew = block_exclusive_scan64(p.b >> 16, lds64, tw);
+        const uint64_t e01 = carry_01 + block_exclusive_scan64(v01, lds64, t01);
+        const uint64_t e23 = carry_23 + block_exclusive_scan64(v23, lds64, t23);
         __syncthreads();
         if (i < n_parts) {
-            partial[i].a = carry_a + ea;
-            partial[i].b = carry_s + uint32_t(es);
-            partial[i].pad = carry_w + uint32_t(ew);
+            ScanPartial q{};
+            q.a = carry_a + ea;
+            q.f[0] = uint32_t(e01);
+            q.f[1] = uint32_t(e01 >> 32);
+            q.f[2] = uint32_t(e23);
+            q.f[3] = uint32_t(e23 >> 32);
+            partial[i] = q;
         }
         carry_a += ta;
-        carry_s += uint32_t(ts);
-        carry_w += uint32_t(tw);
+        carry_01 += t01;
+        carry_23 += t23;
     }
     uint32_t vs = 0;
     for (uint32_t i = threadIdx.x; i < n_blocks; i += kScanThreads) vs += block_valid[i];
@@ -580,13 +605,17 @@ __global__ __launch_bounds__(1024) void k_scan_top(ScanPartial* partial, uint32_
     block_exclusive_scan(vs, lds, total_valid);
     if (threadIdx.x == 0) {
         const uint32_t pairs = uint32_t(carry_a), n_split = uint32_t(carry_a >> 32);
-        const uint64_t n_items = uint64_t(n_split) + carry_w;
+        const uint32_t n_zero = uint32_t(carry_01), n_heavy = uint32_t(carry_01 >> 32);
+        const uint32_t n_mid = uint32_t(carry_23), n_light = uint32_t(carry_23 >> 32);
+        const uint64_t n_items = uint64_t(n_heavy) + n_mid + n_split + n_light;
         totals[0] = pairs;
         totals[1] = total_valid;
         totals[2] = (pairs > list_cap || n_items > items_cap) ? 1u : 0u;
         totals[3] = uint32_t(min<uint64_t>(n_items, items_cap));
-        totals[4] = n_split;
-        totals[5] = carry_s;
+        totals[4] = n_heavy;                  // first mid item
+        totals[5] = n_zero;
+        totals[6] = n_heavy + n_mid;          // first split item
+        totals[7] = n_heavy + n_mid + n_split;  // first light item
     }
 }
 
@@ -597,36 +626,39 @@ __global__ __launch_bounds__(1024) void k_scan_final(const uint32_t* counts, uin
     __shared__ uint64_t lds64[16];
     const uint32_t i0 = blockIdx.x * kScanChunk + threadIdx.x * kScanPerThread;
     uint32_t v[kScanPerThread];
-    uint64_t a = 0;
-    uint32_t b = 0;
+    uint64_t a = 0, b = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kScanPerThread; ++k) {
         v[k] = i0 + k < n ? counts[i0 + k] : 0u;
         if (i0 + k < n) bin_fields(v[k], a, b);
     }
-    uint64_t ta;
-    uint32_t tb;
+    uint64_t ta, tb;
     const ScanPartial base = partial[blockIdx.x];
     const uint64_t ea = base.a + block_exclusive_scan64(a, lds64, ta);
-    const uint32_t eb = block_exclusive_scan(b, lds, tb);
-    uint32_t run = uint32_t(ea), r_items = uint32_t(ea >> 32);
-    uint32_t r_zero = base.b + (eb & 0xffffu);
-    uint64_t r_whole = uint64_t(totals[4]) + base.pad + (eb >> 16);  // whole bins follow every split item
+    const uint64_t eb = block_exclusive_scan64(b, lds64, tb);
+    uint32_t run = uint32_t(ea);
+    uint64_t r_split = uint64_t(totals[6]) + uint32_t(ea >> 32);
+    uint32_t r_zero = base.f[0] + uint32_t(eb & 0xffffu);
+    // next item slot of each whole-bin class: the class's first item + earlier blocks + this block
+    uint64_t r_heavy = base.f[1] + ((eb >> 16) & 0xffffu);
+    uint64_t r_mid = uint64_t(totals[4]) + base.f[2] + ((eb >> 32) & 0xffffu);
+    uint64_t r_light = uint64_t(totals[7]) + base.f[3] + ((eb >> 48) & 0xffffu);
 #pragma unroll
     for (uint32_t k = 0; k < kScanPerThread; ++k)
         if (i0 + k < n) {
             offsets[i0 + k] = run;
-            if (v[k] > kItemRecs) {
+            const uint32_t cls = bin_class(v[k]);
+            if (cls == kClassSplit) {
                 zero_bins[r_zero++] = i0 + k;
                 const uint32_t ni = (v[k] + kItemRecs - 1) / kItemRecs;
-                for (uint32_t c = 0; c < ni; ++c, ++r_items)
-                    if (r_items < items_cap)
-                        items[r_items] = make_uint4(i0 + k, run + c * kItemRecs, min(kItemRecs, v[k] - c * kItemRecs), 1u);
-            } else if (v[k] == 0) {
+                for (uint32_t c = 0; c < ni; ++c, ++r_split)
+                    if (r_split < items_cap)
+                        items[r_split] = make_uint4(i0 + k, run + c * kItemRecs, min(kItemRecs, v[k] - c * kItemRecs), 1u);
+            } else if (cls == kClassZero) {
                 zero_bins[r_zero++] = i0 + k;
             } else {
-                if (r_whole < items_cap) items[r_whole] = make_uint4(i0 + k, run, v[k], 0u);
-                ++r_whole;
+                const uint64_t slot = cls == kClassHeavy ? r_heavy++ : (cls == kClassMid ? r_mid++ : r_light++);
+                if (slot < items_cap) items[slot] = make_uint4(i0 + k, run, v[k], 0u);
             }
             run += v[k];
         }
@@ -701,8 +733,8 @@ __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t 
 // values need 64 bits.
 constexpr uint32_t kMetaWide = 8u;
 
-__device__ inline uint32_t record_meta(const TriRec& r) {
-    const int32_t X[3] = {r.X0, r.X1, r.X2}, Y[3] = {r.Y0, r.Y1, r.Y2};
+__device__ inline uint32_t record_meta(int32_t X0, int32_t Y0, int32_t X1, int32_t Y1, int32_t X2, int32_t Y2) {
+    const int32_t X[3] = {X0, X1, X2}, Y[3] = {Y0, Y1, Y2};
     uint32_t meta = 0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -769,19 +801,25 @@ __device__ inline void raster_item(Word* tile, TriRec* chunk_recs, uint8_t* owne
     static_assert(kItemRecs == 256 && kItemRecs % kChunkRecs == 0, "one record per thread");
     constexpr int S = T + 1;
     const uint32_t i = off + threadIdx.x;
-    TriRec r{};
+    // the record as three 16-byte words (X0 Y0 X1 Y1 | X2 Y2 z0 dzdx | dzdy id bbx bby), kept in
+    // registers until its chunk is staged
+    uint4 w0 = make_uint4(0u, 0u, 0u, 0u), w1 = w0, w2 = w0;
     uint32_t my_rows = 0;
     if (i < end) {
         const uint32_t ri = lists[i];
         if (ri < rec_cap_total) {
-            r = recs[ri];
-            const int bx0 = max(int(r.bbx & 0xffffu), ox) - ox, bx1 = min(int(r.bbx >> 16), ox + T - 1) - ox;
-            const int by0 = max(int(r.bby & 0xffffu), oy) - oy, by1 = min(int(r.bby >> 16), oy + T - 1) - oy;
+            const uint4* src = reinterpret_cast<const uint4*>(recs + ri);
+            w0 = src[0];
+            w1 = src[1];
+            w2 = src[2];
+            const int bx0 = max(int(w2.z & 0xffffu), ox) - ox, bx1 = min(int(w2.z >> 16), ox + T - 1) - ox;
+            const int by0 = max(int(w2.w & 0xffffu), oy) - oy, by1 = min(int(w2.w >> 16), oy + T - 1) - oy;
             if (bx0 <= bx1 && by0 <= by1) {
                 my_rows = uint32_t(by1 - by0 + 1);
                 // the staged copy keeps the clipped box in bbx and the raster flags in bby
-                r.bbx = uint32_t(bx0) | (uint32_t(bx1) << 8) | (uint32_t(by0) << 16);
-                r.bby = record_meta(r);
+                w2.z = uint32_t(bx0) | (uint32_t(bx1) << 8) | (uint32_t(by0) << 16);
+                w2.w = record_meta(int32_t(w0.x), int32_t(w0.y), int32_t(w0.z), int32_t(w0.w), int32_t(w1.x),
+                                   int32_t(w1.y));
             }
         }
     }
@@ -790,7 +828,12 @@ __device__ inline void raster_item(Word* tile, TriRec* chunk_recs, uint8_t* owne
     for (uint32_t c = 0; c * kChunkRecs < n; ++c) {
         const bool stager = threadIdx.x / kChunkRecs == c;
         const uint32_t rows = stager ? my_rows : 0u;
-        if (rows) chunk_recs[slot] = r;
+        if (rows) {
+            uint4* dst = reinterpret_cast<uint4*>(chunk_recs + slot);
+            dst[0] = w0;
+            dst[1] = w1;
+            dst[2] = w2;
+        }
         uint32_t total_rows;
         const uint32_t first = block_exclusive_scan(rows, scan_lds, total_rows);  // synchronises
         if (stager) row_start[slot] = first;
