@@ -796,34 +796,22 @@ __device__ inline void raster_row(const TriRec& r, uint32_t meta, int ox, int oy
 // evenly over the workgroup.
 template <int T, bool kShadow, typename Word>
 __device__ inline void raster_item(Word* tile, TriRec* chunk_recs, uint8_t* owner, uint32_t* row_start,
-                                   uint32_t* scan_lds, const uint32_t* lists, const TriRec* recs,
-                                   uint64_t rec_cap_total, uint32_t off, uint32_t end, int ox, int oy) {
+                                   uint32_t* scan_lds, bool have, uint4 w0, uint4 w1, uint4 w2, uint32_t n, int ox,
+                                   int oy) {
     static_assert(kItemRecs == 256 && kItemRecs % kChunkRecs == 0, "one record per thread");
     constexpr int S = T + 1;
-    const uint32_t i = off + threadIdx.x;
-    // the record as three 16-byte words (X0 Y0 X1 Y1 | X2 Y2 z0 dzdx | dzdy id bbx bby), kept in
-    // registers until its chunk is staged
-    uint4 w0 = make_uint4(0u, 0u, 0u, 0u), w1 = w0, w2 = w0;
     uint32_t my_rows = 0;
-    if (i < end) {
-        const uint32_t ri = lists[i];
-        if (ri < rec_cap_total) {
-            const uint4* src = reinterpret_cast<const uint4*>(recs + ri);
-            w0 = src[0];
-            w1 = src[1];
-            w2 = src[2];
-            const int bx0 = max(int(w2.z & 0xffffu), ox) - ox, bx1 = min(int(w2.z >> 16), ox + T - 1) - ox;
-            const int by0 = max(int(w2.w & 0xffffu), oy) - oy, by1 = min(int(w2.w >> 16), oy + T - 1) - oy;
-            if (bx0 <= bx1 && by0 <= by1) {
-                my_rows = uint32_t(by1 - by0 + 1);
-                // the staged copy keeps the clipped box in bbx and the raster flags in bby
-                w2.z = uint32_t(bx0) | (uint32_t(bx1) << 8) | (uint32_t(by0) << 16);
-                w2.w = record_meta(int32_t(w0.x), int32_t(w0.y), int32_t(w0.z), int32_t(w0.w), int32_t(w1.x),
-                                   int32_t(w1.y));
-            }
+    if (have) {
+        const int bx0 = max(int(w2.z & 0xffffu), ox) - ox, bx1 = min(int(w2.z >> 16), ox + T - 1) - ox;
+        const int by0 = max(int(w2.w & 0xffffu), oy) - oy, by1 = min(int(w2.w >> 16), oy + T - 1) - oy;
+        if (bx0 <= bx1 && by0 <= by1) {
+            my_rows = uint32_t(by1 - by0 + 1);
+            // the staged copy keeps the clipped box in bbx and the raster flags in bby
+            w2.z = uint32_t(bx0) | (uint32_t(bx1) << 8) | (uint32_t(by0) << 16);
+            w2.w = record_meta(int32_t(w0.x), int32_t(w0.y), int32_t(w0.z), int32_t(w0.w), int32_t(w1.x),
+                               int32_t(w1.y));
         }
     }
-    const uint32_t n = end > off ? end - off : 0u;  // at most kItemRecs
     const uint32_t slot = threadIdx.x % kChunkRecs;
     for (uint32_t c = 0; c * kChunkRecs < n; ++c) {
         const bool stager = threadIdx.x / kChunkRecs == c;
@@ -876,6 +864,13 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
     const uint32_t n_items = totals[3];
     const uint4 it = items[blockIdx.x];
     if (blockIdx.x >= n_items) return;
+    // this thread's record of the item: the list entry, then the record itself as three 16-byte
+    // words (X0 Y0 X1 Y1 | X2 Y2 z0 dzdx | dzdy id bbx bby); the view lookup and the tile clear
+    // overlap these loads
+    const uint32_t off = it.y;
+    const uint32_t end = uint32_t(min(uint64_t(off) + it.z, uint64_t(list_cap)));
+    const uint32_t n = end > off ? end - off : 0u;  // at most kItemRecs
+    const uint32_t ri = threadIdx.x < n ? lists[off + threadIdx.x] : 0xffffffffu;
     const uint32_t bin = it.x;
     uint32_t view = 0;
     while (view + 1 < n_views && views[view + 1].tile_base <= bin) ++view;
@@ -886,14 +881,16 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
     const int width = int(V.width), height = int(V.height);
     const uint64_t out_addr = V.out;
     const bool split = it.w != 0u;
-    const uint32_t off = it.y;
-    const uint32_t end = uint32_t(min(uint64_t(off) + it.z, uint64_t(list_cap)));
+    // unconditional loads (record 0 stands in for a missing entry), so that the first wait for
+    // them comes after the tile clear
+    const bool have = ri < rec_cap_total;
+    const uint4* src = reinterpret_cast<const uint4*>(recs + (have ? ri : 0u));
+    const uint4 w0 = src[0], w1 = src[1], w2 = src[2];
 
     if (V.is_shadow) {
         uint32_t* t32 = reinterpret_cast<uint32_t*>(tile);
         for (int p = threadIdx.x; p < T * S; p += 256) t32[p] = 0u;
-        raster_item<T, true>(t32, chunk_recs, owner, row_start, scan_lds, lists, recs, rec_cap_total, off, end, ox,
-                             oy);
+        raster_item<T, true>(t32, chunk_recs, owner, row_start, scan_lds, have, w0, w1, w2, n, ox, oy);
         __syncthreads();  // tile complete (and cleared, for an item without records)
         uint32_t* out = reinterpret_cast<uint32_t*>(out_addr);
         for (int p = threadIdx.x; p < T * T; p += 256) {
@@ -907,8 +904,7 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
         }
     } else {
         for (int p = threadIdx.x; p < T * S; p += 256) tile[p] = 0ull;
-        raster_item<T, false>(tile, chunk_recs, owner, row_start, scan_lds, lists, recs, rec_cap_total, off, end, ox,
-                              oy);
+        raster_item<T, false>(tile, chunk_recs, owner, row_start, scan_lds, have, w0, w1, w2, n, ox, oy);
         __syncthreads();
         unsigned long long* out = reinterpret_cast<unsigned long long*>(out_addr);
         for (int p = threadIdx.x; p < T * T; p += 256) {
