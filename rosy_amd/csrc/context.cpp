@@ -218,6 +218,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     HIP_TRY(c, c->bin_counts.ensure(size_t(L.total_bins) * 4), "allocate bins");
     HIP_TRY(c, c->bin_offsets.ensure(size_t(L.total_bins) * 4), "allocate bins");
     HIP_TRY(c, c->pair_slots.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * 16), "allocate bin slots");
+    HIP_TRY(c, c->boxes.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * 8), "allocate record boxes");
     HIP_TRY(c, c->big_pairs.ensure(size_t(std::max<uint32_t>(1, L.big_total)) * 16), "allocate bin pairs");
     HIP_TRY(c, c->block_valid.ensure(std::max<size_t>(1, L.tblocks.size()) * 4), "allocate statistics");
     HIP_TRY(c, c->totals.ensure(32), "allocate statistics");
@@ -293,6 +294,7 @@ rsy_result upload_frame(rsy_ctx* c) {
             blocks += sv.blocks_x * ((vg.height + 15) / 16);
             sv.vis = reinterpret_cast<const unsigned long long*>(vg.out);
             sv.recs = c->recs.as<TriRec>() + vg.rec_base;
+            sv.boxes = c->boxes.as<uint2>() + vg.rec_base;
             sv.color = c->color_out[ci].as<float4>();
             sv.shadow_factor = (c->debug_flags & RSY_DEBUG_SHADOW_FACTOR) ? c->shadow_factor_out[ci].as<float>() : nullptr;
             std::memcpy(sv.view_projection, vg.view_projection, sizeof(sv.view_projection));
@@ -351,6 +353,7 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     g.indices = c->indices.as<uint32_t>();
     g.xv = c->xv.as<float4>();
     g.recs = c->recs.as<TriRec>();
+    g.boxes = c->boxes.as<uint2>();
     g.clipq = c->clipq.as<uint2>();
     g.counters = c->counters.as<uint32_t>();
     g.bin_counts = c->bin_counts.as<uint32_t>();
@@ -376,6 +379,7 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     b.n_tblocks = uint32_t(L.tblocks.size());
     b.totals = c->totals.as<uint32_t>();
     b.recs = g.recs;
+    b.boxes = g.boxes;
     b.counters = g.counters;
     b.lists = c->lists_mem.as<uint32_t>();
     b.list_cap = c->list_cap;

@@ -22,6 +22,7 @@ struct GeometryLaunch {
     const uint32_t* indices;
     float4* xv;
     TriRec* recs;
+    uint2* boxes;           // per record slot: the record's (bbx, bby), or kInvalidBox twice
     uint2* clipq;
     uint32_t* counters;
     uint32_t* bin_counts;
@@ -43,6 +44,7 @@ struct BinLaunch {
     const uint32_t* pair_slots;
     const uint4* big_pairs;
     const uint32_t* block_valid;
+    const uint2* boxes;
     uint32_t n_tblocks;
     uint32_t* totals;  // [0] (record, bin) pairs, [1] records set up, [2] bin lists or items overflowed,
                        // [3] raster work items, [4] first mid-class item, [5] bins to clear,
@@ -80,6 +82,7 @@ struct ShadeView {
     uint32_t blocks_x;
     const unsigned long long* vis;
     const TriRec* recs;    // the raster view's record slots, indexed by triangle id
+    const uint2* boxes;    // their boxes (kInvalidBox: no record, e.g. a clipped triangle)
     float4* color;
     float* shadow_factor;  // nullable debug output
     float view_projection[16];

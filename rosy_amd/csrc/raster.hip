@@ -291,7 +291,7 @@ constexpr uint32_t kEmpty = 0xffffffffu;
 
 __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const ViewGPU* views, const DrawGPU* draws0,
                                                const DrawGPU* draws1, const uint32_t* indices, const float4* xv,
-                                               TriRec* recs, uint2* clipq, uint32_t* counters,
+                                               TriRec* recs, uint2* boxes, uint2* clipq, uint32_t* counters,
                                                uint32_t* bin_counts, uint32_t* pair_slots, uint4* big_pairs,
                                                uint32_t* block_valid, const uint32_t* draw_culled, uint32_t shift) {
     __shared__ uint32_t hkey[kHash];
@@ -307,8 +307,7 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     uint32_t* ctr = counters + e.view * kCtrPerView;
     const uint32_t t = e.start + threadIdx.x;
     if (draw_culled[V.cull_base + e.draw]) {  // the whole draw is outside the view: empty slots only
-        if (t < dr.tri_count)
-            *reinterpret_cast<uint2*>(&recs[V.rec_base + dr.tri_base + t].bbx) = make_uint2(kInvalidBox, kInvalidBox);
+        if (t < dr.tri_count) boxes[V.rec_base + dr.tri_base + t] = make_uint2(kInvalidBox, kInvalidBox);
         if (threadIdx.x == 0) block_valid[blockIdx.x] = 0u;
         return;
     }
@@ -335,7 +334,7 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     const uint32_t slot = V.rec_base + dr.tri_base + t;  // every triangle owns one record slot
     if (valid) {
         if (has) recs[slot] = r;
-        else *reinterpret_cast<uint2*>(&recs[slot].bbx) = make_uint2(kInvalidBox, kInvalidBox);
+        boxes[slot] = has ? make_uint2(r.bbx, r.bby) : make_uint2(kInvalidBox, kInvalidBox);
     }
     const uint32_t cslot = wave_append(status == 2, &ctr[kCtrClip]);
     if (status == 2) {
@@ -668,7 +667,7 @@ __global__ __launch_bounds__(1024) void k_scan_final(const uint32_t* counts, uin
 // k_setup's block table, or the big-pair list), so the entry goes to bin_offsets[bin] + slot.  The
 // same launch zeroes the output pixels of the empty bins (which get no raster work item) and of
 // the split bins (which k_raster then merges into).
-__global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t n_views, const TriRec* recs,
+__global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t n_views, const uint2* boxes,
                                                  const uint32_t* counters, const uint32_t* bin_offsets,
                                                  const uint32_t* pair_slots, const uint4* big_pairs, uint32_t* lists,
                                                  uint32_t list_cap, uint32_t shift, const uint32_t* totals,
@@ -680,7 +679,7 @@ __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t 
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (i < V.tri_count) {
             const uint32_t ri = V.rec_base + i;
-            const uint2 bb = *reinterpret_cast<const uint2*>(&recs[ri].bbx);
+            const uint2 bb = boxes[ri];
             if (bb.x == kInvalidBox) continue;
             TriRec r;
             r.bbx = bb.x;
@@ -939,8 +938,8 @@ hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
                            g.objects, g.px, g.py, g.pz, g.xv, g.draw_culled);
     if (g.n_tblocks)
         hipLaunchKernelGGL(k_setup, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.draws0, g.draws1,
-                           g.indices, g.xv, g.recs, g.clipq, g.counters, g.bin_counts, g.pair_slots, g.big_pairs,
-                           g.block_valid, g.draw_culled, g.shift);
+                           g.indices, g.xv, g.recs, g.boxes, g.clipq, g.counters, g.bin_counts, g.pair_slots,
+                           g.big_pairs, g.block_valid, g.draw_culled, g.shift);
     hipLaunchKernelGGL(k_clip, dim3(g.clip_blocks, g.n_views), dim3(kClipThreads), 0, s, g.views, g.draws0, g.draws1,
                        g.indices, g.xv, g.clipq, g.recs, g.counters, g.bin_counts, g.big_pairs, g.shift);
     return hipGetLastError();
@@ -954,7 +953,7 @@ hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
                        b.list_cap, b.items_cap, b.totals);
     hipLaunchKernelGGL(k_scan_final, dim3(parts), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, b.bin_offsets,
                        partial, b.totals, b.items, b.items_cap, b.zero_bins);
-    hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.n_views, b.recs,
+    hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.n_views, b.boxes,
                        b.counters, b.bin_offsets, b.pair_slots, b.big_pairs, b.lists, b.list_cap, b.shift, b.totals,
                        b.zero_bins);
     return hipGetLastError();

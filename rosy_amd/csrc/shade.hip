@@ -220,24 +220,24 @@ __device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView
     const float w0 = xv[pg.i0].w, w1 = xv[pg.i1].w, w2 = xv[pg.i2].w;
 
     // Perspective-correct barycentrics at the pixel centre.  A triangle that needed no clipping has
-    // its record in the triangle's slot: the exact integer edge functions of the snapped vertices
-    // the raster pass covered, each over its vertex's w (as e_k * w_i * w_j, normalised below).  The
-    // record may hold vertices 1 and 2 exchanged, which negates all three edge functions and swaps
-    // the last two.  A clipped triangle's slot is empty; its barycentrics come from
-    // b ~ cross(x - u w, y - v w) over the clip positions.
+    // a record in the triangle's slot (its box says so): the exact integer edge functions of the
+    // snapped vertices the raster pass covered, each over its vertex's w (as e_k * w_i * w_j,
+    // normalised below).  The record may hold vertices 1 and 2 exchanged, which negates all three
+    // edge functions and swaps the last two.  A clipped triangle's slot is empty; its barycentrics
+    // come from b ~ cross(x - u w, y - v w) over the clip positions.
     f3 lam;
-    const uint4* rq = reinterpret_cast<const uint4*>(V.recs + tri);
-    const uint4 q2 = rq[2];  // dzdy, id, bbx, bby
-    if (q2.z != kInvalidBox) {
-        const uint4 q0 = rq[0];
-        const uint2 q1 = *reinterpret_cast<const uint2*>(rq + 1);
+    if (V.boxes[tri].x != kInvalidBox) {
+        const uint4* rq = reinterpret_cast<const uint4*>(V.recs + tri);
+        const uint4 q0 = rq[0];  // X0, Y0, X1, Y1
+        const uint2 q1 = *reinterpret_cast<const uint2*>(rq + 1);  // X2, Y2
+        const uint32_t id = reinterpret_cast<const uint32_t*>(rq + 2)[1];
         const int32_t X0 = int32_t(q0.x), Y0 = int32_t(q0.y), X1 = int32_t(q0.z), Y1 = int32_t(q0.w);
         const int32_t X2 = int32_t(q1.x), Y2 = int32_t(q1.y);
         const int32_t Px = int32_t(x) * 256 + 128, Py = int32_t(y) * 256 + 128;
         const float e0 = float(int64_t(X1 - Px) * (Y2 - Py) - int64_t(X2 - Px) * (Y1 - Py));
         const float e1 = float(int64_t(X2 - Px) * (Y0 - Py) - int64_t(X0 - Px) * (Y2 - Py));
         const float e2 = float(int64_t(X0 - Px) * (Y1 - Py) - int64_t(X1 - Px) * (Y0 - Py));
-        const bool swapped = (q2.y & kRecSwapped) != 0u;
+        const bool swapped = (id & kRecSwapped) != 0u;
         const float b1 = swapped ? e2 : e1, b2 = swapped ? e1 : e2;
         lam = mk(e0 * (w1 * w2), b1 * (w0 * w2), b2 * (w0 * w1));
     } else {
