@@ -54,12 +54,13 @@ __device__ inline uint32_t nm_lane() { return __builtin_amdgcn_mbcnt_hi(~0u, __b
 __device__ inline void nm_target(const uint8_t* texels, const TexGPU* textures, const NmSlot& s, float u, float v,
                                  const float* lut, float out[kNmOut]) {
     const TexGPU ta = textures[s.albedo], tn = textures[s.normal], tm = textures[s.mr];
+    const SamplerState repeat_linear{};
     const float4 a = sample_tex(TexRef{texels + ta.offset, int(ta.width), int(ta.height), int(ta.channels), int(ta.srgb)},
-                                nullptr, u, v, lut);
+                                repeat_linear, u, v, lut);
     const float4 n = sample_tex(TexRef{texels + tn.offset, int(tn.width), int(tn.height), int(tn.channels), int(tn.srgb)},
-                                nullptr, u, v, lut);
+                                repeat_linear, u, v, lut);
     const float4 m = sample_tex(TexRef{texels + tm.offset, int(tm.width), int(tm.height), int(tm.channels), int(tm.srgb)},
-                                nullptr, u, v, lut);
+                                repeat_linear, u, v, lut);
     out[0] = a.x;
     out[1] = a.y;
     out[2] = a.z;

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "fixed_point.h"
 #include "frame_types.h"
@@ -35,7 +36,8 @@ __device__ inline f3 cross(f3 a, f3 b) { return {a.y * b.z - a.z * b.y, a.z * b.
 __device__ inline f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
 __device__ inline float clamp01(float x) { return fminf(1.0f, fmaxf(0.0f, x)); }
 
-__device__ inline float4 mul4(const float* m, float4 v) {
+template <class MP>
+__device__ inline float4 mul4(MP m, float4 v) {
     float4 r;
     r.x = m[0] * v.x + m[4] * v.y + m[8] * v.z + m[12] * v.w;
     r.y = m[1] * v.x + m[5] * v.y + m[9] * v.z + m[13] * v.w;
@@ -43,7 +45,8 @@ __device__ inline float4 mul4(const float* m, float4 v) {
     r.w = m[3] * v.x + m[7] * v.y + m[11] * v.z + m[15] * v.w;
     return r;
 }
-__device__ inline f3 mul3(const float* m, f3 v) {
+template <class MP>
+__device__ inline f3 mul3(MP m, f3 v) {
     return {m[0] * v.x + m[3] * v.y + m[6] * v.z, m[1] * v.x + m[4] * v.y + m[7] * v.z,
             m[2] * v.x + m[5] * v.y + m[8] * v.z};
 }
@@ -79,16 +82,17 @@ struct Varyings {
 // normalised world normal, so those two are formed per vertex and interpolated, and the rest is
 // formed once from the interpolated position, normal, tangent, colour and uv: the same values as
 // running vertexMain per vertex and interpolating its outputs.
-__device__ Varyings pixel_varyings(const rsy_scene_data& sd, const rsy_position* const v[3], const float* gd,
+template <class GP>
+__device__ Varyings pixel_varyings(const rsy_scene_data& sd, const rsy_position* const v[3], GP gd,
                                    bool has_material, uint32_t normal_index, const float* cascade_m, float w0,
                                    float w1, float w2, bool shift_uv) {
     Varyings o;
     const float w[3] = {w0, w1, w2};
     bool tangent_space = sd.tangent_space_enabled == 1;
     if (has_material && !valid_index(normal_index)) tangent_space = false;
-    const float* W = gd;
-    const float* toObj = gd + 16;
-    const float* nmat = gd + 32;
+    const GP W = gd;
+    const GP toObj = gd + 16;
+    const GP nmat = gd + 32;
     f3 P = mk(0.f, 0.f, 0.f), N = P, Tn = P, B = P, Nw = P;
     float sigma = 0.f;
     float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -159,7 +163,8 @@ __device__ Varyings pixel_varyings(const rsy_scene_data& sd, const rsy_position*
 }
 
 // True when any sampler of the material clamps (the uv period shift is then not an identity).
-__device__ inline bool uses_clamp(const rsy_gpu_material* md, const rsy_sampler* samplers, uint32_t n_samplers) {
+template <class MP, class SP>
+__device__ inline bool uses_clamp(MP md, SP samplers, uint32_t n_samplers) {
     const uint32_t idx[4] = {md->color_sampler_index, md->normal_sampler_index, md->metallic_sampler_index,
                              md->mixmap_sampler_index};
     bool clamp = false;
@@ -261,19 +266,47 @@ __device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView
     return pg;
 }
 
+// Access to the per-draw records (draw, object, material, sampler, texture) of a shading pass:
+// plain global pointers, or, when every live lane of the wave shades the same draw, pointers into
+// the constant address space -- the records are read-only while the kernel runs -- so that the
+// wave-uniform loads become scalar loads.
+struct LaneRecords {
+    template <class T> using P = const T*;
+    template <class T> __device__ static P<T> ptr(const T* p) { return p; }
+    template <class T> __device__ static T load(P<T> p) { return *p; }
+};
+struct WaveRecords {
+    template <class T> using P = const __attribute__((address_space(4))) T*;
+    template <class T> __device__ static P<T> ptr(const T* p) { return (P<T>)(p); }
+    // a record copied word by word (a struct copy would need a generic-address-space source)
+    template <class T> __device__ static T load(P<T> p) {
+        static_assert(sizeof(T) % 4 == 0, "whole words");
+        T v;
+        uint32_t* d = reinterpret_cast<uint32_t*>(&v);
+        const P<uint32_t> w = (P<uint32_t>)(p);
+#pragma unroll
+        for (uint32_t k = 0; k < sizeof(T) / 4; ++k) d[k] = w[k];
+        return v;
+    }
+};
+
 // fragmentMain (basic.slang:137-344) for one covered pixel.  When `use_dec` is set, the material
 // decoder's outputs `dec` replace the albedo, normal and metallic-roughness samples of the plain
-// textures.
+// textures.  kUniformDraw: every live lane of the wave shades the same draw.
+template <bool kUniformDraw>
 __device__ inline void shade_pixel(const ShadeLaunch& L, const ShadeView& V, const float* lut, size_t o,
                                    const PixelGeom& pg, bool use_dec, const float (&dec)[kNmOut]) {
+    using R = std::conditional_t<kUniformDraw, WaveRecords, LaneRecords>;
     const rsy_scene_data& sd = *static_cast<const rsy_scene_data*>(L.scene);
-    const DrawGPU dr = L.draws[pg.draw];
-    const float* gd = L.objects + size_t(dr.object) * 41u;
+    const uint32_t draw = kUniformDraw ? uint32_t(__builtin_amdgcn_readfirstlane(int(pg.draw))) : pg.draw;
+    const DrawGPU dr = R::load(R::ptr(L.draws) + draw);
+    const auto gd = R::ptr(L.objects) + size_t(dr.object) * 41u;
     const bool has_material = dr.material < L.n_materials;
-    const rsy_gpu_material* mats = static_cast<const rsy_gpu_material*>(L.materials);
-    const rsy_gpu_material* md = has_material ? &mats[dr.material] : nullptr;
+    // dereferenced only when has_material
+    const auto md = R::ptr(static_cast<const rsy_gpu_material*>(L.materials)) + (has_material ? dr.material : 0u);
     const rsy_position* verts = static_cast<const rsy_position*>(L.vertices);
-    const rsy_sampler* samplers = static_cast<const rsy_sampler*>(L.samplers);
+    const auto samplers = R::ptr(static_cast<const rsy_sampler*>(L.samplers));
+    const auto textures = R::ptr(L.textures);
     const uint32_t i0 = pg.i0, i1 = pg.i1, i2 = pg.i2;
     const f3 lam = pg.lam;
     const int cascade = pg.cascade;
@@ -314,9 +347,10 @@ __device__ inline void shade_pixel(const ShadeLaunch& L, const ShadeView& V, con
             srough = dec[6];
         } else {
             if (sd.normal_maps_enabled == 1 && valid_index(nti) && nti < L.n_textures) {
-                const TexGPU tg = L.textures[nti];
+                const TexGPU tg = R::load(textures + nti);
                 const TexRef t{L.texels + tg.offset, int(tg.width), int(tg.height), int(tg.channels), int(tg.srgb)};
-                const rsy_sampler* smp = md->normal_sampler_index < L.n_samplers ? &samplers[md->normal_sampler_index] : nullptr;
+                const uint32_t si = md->normal_sampler_index;
+                const SamplerState smp = si < L.n_samplers ? sampler_state(R::load(samplers + si)) : SamplerState{};
                 const float4 s = sample_tex(t, smp, tu, tv, lut);
                 f3 m = mk(s.x * 2.f - 1.f, s.y * 2.f - 1.f, s.z * 2.f - 1.f);
                 if (t.ch == 2) m.z = sqrtf(fmaxf(0.f, 1.f - m.x * m.x - m.y * m.y));
@@ -327,9 +361,10 @@ __device__ inline void shade_pixel(const ShadeLaunch& L, const ShadeView& V, con
             }
             const uint32_t cti = md->color_sampled_image_index;
             if (valid_index(cti) && cti < L.n_textures) {
-                const TexGPU tg = L.textures[cti];
+                const TexGPU tg = R::load(textures + cti);
                 const TexRef t{L.texels + tg.offset, int(tg.width), int(tg.height), int(tg.channels), int(tg.srgb)};
-                const rsy_sampler* smp = md->color_sampler_index < L.n_samplers ? &samplers[md->color_sampler_index] : nullptr;
+                const uint32_t si = md->color_sampler_index;
+                const SamplerState smp = si < L.n_samplers ? sampler_state(R::load(samplers + si)) : SamplerState{};
                 image = sample_tex(t, smp, tu, tv, lut);
                 alpha = fminf(image.w, alpha);
             }
@@ -338,9 +373,9 @@ __device__ inline void shade_pixel(const ShadeLaunch& L, const ShadeView& V, con
             const uint32_t mti = use_mix ? mixti : metti;
             const uint32_t msi = use_mix ? md->mixmap_sampler_index : md->metallic_sampler_index;
             if (valid_index(mti) && mti < L.n_textures) {
-                const TexGPU tg = L.textures[mti];
+                const TexGPU tg = R::load(textures + mti);
                 const TexRef t{L.texels + tg.offset, int(tg.width), int(tg.height), int(tg.channels), int(tg.srgb)};
-                const rsy_sampler* smp = msi < L.n_samplers ? &samplers[msi] : nullptr;
+                const SamplerState smp = msi < L.n_samplers ? sampler_state(R::load(samplers + msi)) : SamplerState{};
                 mr_from(t.ch, sample_tex(t, smp, tu, tv, lut), srough, smetal);
             }
         }
@@ -428,6 +463,12 @@ __device__ inline void shade_pixel(const ShadeLaunch& L, const ShadeView& V, con
 
 }  // namespace
 
+// True when every live lane of the wave holds the same value.
+__device__ inline bool wave_uniform(uint32_t v) {
+    const uint32_t v0 = uint32_t(__builtin_amdgcn_readfirstlane(int(v)));
+    return __builtin_amdgcn_ballot_w64(v != v0) == 0ull;
+}
+
 template <bool kNeural>
 __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
     __shared__ float lut[256];
@@ -454,7 +495,9 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
             return;
         }
         const float none[kNmOut] = {};
-        shade_pixel(L, V, lut, o, pixel_geometry(L, V, tri_base, vi, x, y, vis), false, none);
+        const PixelGeom pg = pixel_geometry(L, V, tri_base, vi, x, y, vis);
+        if (wave_uniform(pg.draw)) shade_pixel<true>(L, V, lut, o, pg, false, none);
+        else shade_pixel<false>(L, V, lut, o, pg, false, none);
     } else {
         // Every lane stays live up to the wave-wide decoder MLP; lanes without a decoded material
         // feed zero features and ignore the result.
@@ -485,7 +528,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
             if (V.shadow_factor) V.shadow_factor[o] = 1.f;
             return;
         }
-        shade_pixel(L, V, lut, o, pg, decode, dec);
+        if (wave_uniform(pg.draw)) shade_pixel<true>(L, V, lut, o, pg, decode, dec);
+        else shade_pixel<false>(L, V, lut, o, pg, decode, dec);
     }
 }
 

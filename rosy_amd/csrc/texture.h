@@ -27,6 +27,22 @@ struct TexRef {
     int w, h, ch, srgb;
 };
 
+// Sampler state of a sample: addressing per axis (0 clamp-to-edge, 1 mirrored repeat, 2 repeat)
+// and nearest or bilinear filtering; the default is the REPEAT / bilinear sampler.
+struct SamplerState {
+    uint32_t wrap_s = 2u, wrap_t = 2u;
+    bool nearest = false;
+};
+
+template <class S>
+__device__ inline SamplerState sampler_state(const S& s) {
+    SamplerState r;
+    r.wrap_s = s.wrap_s;
+    r.wrap_t = s.wrap_t;
+    r.nearest = s.mag_filter <= 2;
+    return r;
+}
+
 __device__ inline float4 texel(const TexRef& t, int x, int y, const float* lut) {
     const size_t o = size_t(y) * t.w + x;
     float4 c = make_float4(0.f, 0.f, 0.f, 1.f);
@@ -46,10 +62,9 @@ __device__ inline float4 texel(const TexRef& t, int x, int y, const float* lut) 
     return c;
 }
 
-__device__ inline float4 sample_tex(const TexRef& t, const rsy_sampler* s, float u, float v, const float* lut) {
-    const uint32_t ws = s ? s->wrap_s : 2u, wt = s ? s->wrap_t : 2u;
-    const bool nearest = s && s->mag_filter <= 2;
-    if (nearest) {
+__device__ inline float4 sample_tex(const TexRef& t, const SamplerState& s, float u, float v, const float* lut) {
+    const uint32_t ws = s.wrap_s, wt = s.wrap_t;
+    if (s.nearest) {
         const int x = wrap_coord(int(floorf(u * t.w)), t.w, ws), y = wrap_coord(int(floorf(v * t.h)), t.h, wt);
         return texel(t, x, y, lut);
     }
