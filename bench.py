@@ -179,12 +179,14 @@ def main():
     ms = float(r.stats().gpu_ms)
     log(f"graph frame: neural {ms:.4f} ms, plain {plain_ms:.4f} ms (device); neural wall {wall:.4f} ms")
 
-    # raster kernel: bin counts + offsets, list entries and 48-byte records per (triangle, bin)
-    # pair, one visibility word (8 B) per camera pixel and one depth (4 B) per shadow texel
+    # raster kernel, algorithmic bytes with the implementation-independent unit costs: one tile-list
+    # entry read (4 B) per (triangle, bin) pair, one setup record read (32 B: three edge pairs, the
+    # 64-bit constant terms packed, the depth plane) per triangle set up, one visibility word (8 B)
+    # per camera pixel and one depth (4 B) per shadow texel
     cam = scene.camera_views()[0]
     cas = scene.cascades()[0]
-    bins = ((cam.width + tile - 1) // tile) * ((cam.height + tile - 1) // tile) + ((cas.size + tile - 1) // tile) ** 2
-    raster_bytes = bins * 8 + int(stp.tile_pairs) * (4 + 48) + cam.width * cam.height * 8 + cas.size * cas.size * 4
+    raster_bytes = (int(stp.tile_pairs) * 4 + int(stp.triangles_setup) * 32 + cam.width * cam.height * 8 +
+                    cas.size * cas.size * 4)
     raster_ms = float(st_neural[2])
     achieved = raster_bytes / (raster_ms * 1e-3) / 1e9
     roofline = {"kernel": "k_raster (tile raster, camera + shadow views in one launch)", "bound": "hbm",

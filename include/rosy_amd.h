@@ -423,6 +423,135 @@ void rsy_object_record_from_trs(const float translate[3], float scale, float yaw
 void rsy_mat4_mul(const float a[16], const float b[16], float c[16]);
 
 /* ---------------------------------------------------------------------------------------------
+ * .rsy asset files: the Packager's binary format (src/Asset/Asset.h:9-149, Asset.cpp:6-848),
+ * read and written on the host, and packed into the inputs of rsy_set_geometry and
+ * rsy_set_materials the way set_asset packs them (Graphics.cpp:3525-4689).
+ *
+ * Layout (little endian, the reference's structs with natural alignment): an 80-byte header
+ * (magic "RSYF" = 0x52535946, version 1, endianness 1, coordinate system float[16], root scene);
+ * six u64 counts (materials, samplers, scenes, nodes, images, meshes); the 68-byte materials; the
+ * 8-byte samplers; per scene a u64 node count and its u32 node indices; per node seven u64 sizes
+ * (1, 1, 1, 1, 1, children, name bytes) then translate float[3], scale, yaw, transform float[16],
+ * mesh id, u32 children and the name; per image two u64 sizes (1, name bytes), the u32 image type
+ * and the name; per mesh three u64 counts (positions, indices, surfaces) then the 64-byte
+ * positions, u32 indices and 36-byte surfaces.  A node's coordinate system and is_world_node are
+ * not part of the file (Asset.h:64-70).
+ *
+ * Errors follow asset::read / asset::write: RSY_OPEN_FAILED when a file cannot be opened,
+ * RSY_WRITE_FAILED when it cannot be written, RSY_READ_FAILED for a short file, a magic, version
+ * or endianness mismatch, node or image size fields other than 1, or counts that exceed the bytes
+ * left (nothing is allocated for them), RSY_INVALID_ARGUMENT for null pointers or a capacity too
+ * small.  Image pixels live outside the file (the engine loads them by name): the caller decodes
+ * them and passes them to rsy_set_materials in the asset's image order.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct rsy_asset rsy_asset;
+
+/* rosy_asset::material (Asset.h:26-46): 68 bytes with the compiler's padding made explicit. */
+typedef struct rsy_asset_material {
+    uint8_t double_sided;
+    uint8_t pad0[3];
+    float base_color_factor[4];
+    float metallic_factor;
+    float roughness_factor;
+    uint8_t alpha_mode;
+    uint8_t pad1[3];
+    float alpha_cutoff;
+    uint32_t color_image_index; /* RSY_ABSENT == not present, likewise below */
+    uint32_t color_sampler_index;
+    uint32_t normal_image_index;
+    uint32_t normal_sampler_index;
+    uint32_t metallic_image_index;
+    uint32_t metallic_sampler_index;
+    uint32_t mixmap_image_index;
+    uint32_t mixmap_sampler_index;
+} rsy_asset_material;
+
+/* rosy_asset::surface (Asset.h:56-63): 36 bytes. */
+typedef struct rsy_asset_surface {
+    uint32_t start_index;
+    uint32_t count;
+    uint32_t material;
+    float min_bounds[3];
+    float max_bounds[3];
+} rsy_asset_surface;
+
+/* rosy_asset::node (Asset.h:64-84) as stored. */
+typedef struct rsy_asset_node {
+    float world_translate[3];
+    float world_scale;
+    float world_yaw;
+    float transform[16];
+    uint32_t mesh_id; /* RSY_ABSENT == no mesh */
+    const uint32_t* child_nodes;
+    uint64_t n_child_nodes;
+    const char* name; /* not NUL-terminated */
+    uint64_t name_len;
+} rsy_asset_node;
+
+typedef struct rsy_asset_scene { /* rosy_asset::scene (Asset.h:86-89) */
+    const uint32_t* nodes;
+    uint64_t n_nodes;
+} rsy_asset_scene;
+
+typedef struct rsy_asset_image { /* rosy_asset::image (Asset.h:107-117) */
+    uint32_t image_type;         /* 0 color, 1 normal map, 2 metallic-roughness, 3 mixmap */
+    const char* name;
+    uint64_t name_len;
+} rsy_asset_image;
+
+typedef struct rsy_asset_mesh { /* rosy_asset::mesh (Asset.h:100-105) */
+    const rsy_position* positions;
+    uint64_t n_positions;
+    const uint32_t* indices; /* mesh-local */
+    uint64_t n_indices;
+    const rsy_asset_surface* surfaces;
+    uint64_t n_surfaces;
+} rsy_asset_mesh;
+
+/* An asset in memory (rosy_asset::asset, Asset.h:125-148), as borrowed arrays. */
+typedef struct rsy_asset_desc {
+    float coordinate_system[16];
+    uint32_t root_scene;
+    const rsy_asset_material* materials;
+    uint64_t n_materials;
+    const rsy_sampler* samplers;
+    uint64_t n_samplers;
+    const rsy_asset_scene* scenes;
+    uint64_t n_scenes;
+    const rsy_asset_node* nodes;
+    uint64_t n_nodes;
+    const rsy_asset_image* images;
+    uint64_t n_images;
+    const rsy_asset_mesh* meshes;
+    uint64_t n_meshes;
+} rsy_asset_desc;
+
+/* asset::write (Asset.cpp:39-403) into memory: with out == NULL only *size is written (the file
+ * size); else the file bytes go to out[0, capacity). */
+rsy_result rsy_asset_encode(const rsy_asset_desc* desc, void* out, uint64_t capacity, uint64_t* size);
+/* asset::write to a file. */
+rsy_result rsy_asset_write(const rsy_asset_desc* desc, const char* path);
+/* asset::read (Asset.cpp:405-848) from memory or a file; the asset owns copies of everything. */
+rsy_result rsy_asset_decode(const void* data, uint64_t size, rsy_asset** out_asset);
+rsy_result rsy_asset_read(const char* path, rsy_asset** out_asset);
+/* The asset's contents as a description whose arrays stay valid until rsy_asset_destroy. */
+rsy_result rsy_asset_describe(const rsy_asset* asset, rsy_asset_desc* out);
+void rsy_asset_destroy(rsy_asset* asset);
+/* Sizes of the packed geometry: vertices and indices over all meshes, and the mesh count. */
+rsy_result rsy_asset_geometry_size(const rsy_asset* asset, uint64_t* n_vertices, uint64_t* n_indices,
+                                   uint32_t* n_meshes);
+/* The global vertex buffer (meshes' positions back to back), index buffer (mesh-local indices back
+ * to back) and mesh table as set_asset builds them (Graphics.cpp:4310-4333): byte offset of each
+ * mesh's first vertex, its first index and index count. */
+rsy_result rsy_asset_pack_geometry(const rsy_asset* asset, rsy_position* vertices, uint64_t n_vertices,
+                                   uint32_t* indices, uint64_t n_indices, rsy_mesh_range* meshes, uint32_t n_meshes);
+/* gpu_material records as set_asset fills them (Graphics.cpp:4066-4152), with texture indices into
+ * the asset's image list and sampler indices into its sampler list: a map whose image index is
+ * out of range is absent, and a present map whose sampler index is out of range uses the default
+ * sampler (RSY_ABSENT here: REPEAT, linear). */
+rsy_result rsy_asset_pack_materials(const rsy_asset* asset, rsy_gpu_material* materials, uint32_t n_materials);
+
+/* ---------------------------------------------------------------------------------------------
  * Synthetic scene generator for the five workloads (seeded; host-only).
  * ------------------------------------------------------------------------------------------- */
 typedef struct rsy_scene_desc {
@@ -482,6 +611,8 @@ static_assert(sizeof(rsy_scene_data) == 408, "rsy_scene_data must be 408 bytes (
 static_assert(sizeof(rsy_draw) == 48, "rsy_draw must be 48 bytes");
 static_assert(sizeof(rsy_mesh_range) == 24, "rsy_mesh_range must be 24 bytes");
 static_assert(sizeof(rsy_sampler) == 8, "rsy_sampler must be 8 bytes (Asset.h:48-54)");
+static_assert(sizeof(rsy_asset_material) == 68, "rsy_asset_material must be 68 bytes (Asset.h:26-46)");
+static_assert(sizeof(rsy_asset_surface) == 36, "rsy_asset_surface must be 36 bytes (Asset.h:56-63)");
 #endif
 
 #endif /* ROSY_AMD_H */

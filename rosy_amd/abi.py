@@ -156,6 +156,55 @@ class NeuralStats(C.Structure):
                 ("reserved", C.c_float)]
 
 
+class AssetMaterial(C.Structure):
+    """rosy_asset::material (Asset.h:26-46), natural alignment: 68 bytes."""
+    _fields_ = [("double_sided", C.c_uint8), ("pad0", C.c_uint8 * 3), ("base_color_factor", F4),
+                ("metallic_factor", C.c_float), ("roughness_factor", C.c_float), ("alpha_mode", C.c_uint8),
+                ("pad1", C.c_uint8 * 3), ("alpha_cutoff", C.c_float),
+                ("color_image_index", C.c_uint32), ("color_sampler_index", C.c_uint32),
+                ("normal_image_index", C.c_uint32), ("normal_sampler_index", C.c_uint32),
+                ("metallic_image_index", C.c_uint32), ("metallic_sampler_index", C.c_uint32),
+                ("mixmap_image_index", C.c_uint32), ("mixmap_sampler_index", C.c_uint32)]
+
+
+class AssetSurface(C.Structure):
+    """rosy_asset::surface (Asset.h:56-63): 36 bytes."""
+    _fields_ = [("start_index", C.c_uint32), ("count", C.c_uint32), ("material", C.c_uint32),
+                ("min_bounds", C.c_float * 3), ("max_bounds", C.c_float * 3)]
+
+
+class AssetNode(C.Structure):
+    _fields_ = [("world_translate", C.c_float * 3), ("world_scale", C.c_float), ("world_yaw", C.c_float),
+                ("transform", F16), ("mesh_id", C.c_uint32), ("child_nodes", C.POINTER(C.c_uint32)),
+                ("n_child_nodes", C.c_uint64), ("name", C.c_void_p), ("name_len", C.c_uint64)]
+
+
+class AssetScene(C.Structure):
+    _fields_ = [("nodes", C.POINTER(C.c_uint32)), ("n_nodes", C.c_uint64)]
+
+
+class AssetImage(C.Structure):
+    _fields_ = [("image_type", C.c_uint32), ("name", C.c_void_p), ("name_len", C.c_uint64)]
+
+
+class AssetMesh(C.Structure):
+    _fields_ = [("positions", C.POINTER(Position)), ("n_positions", C.c_uint64),
+                ("indices", C.POINTER(C.c_uint32)), ("n_indices", C.c_uint64),
+                ("surfaces", C.POINTER(AssetSurface)), ("n_surfaces", C.c_uint64)]
+
+
+class AssetDesc(C.Structure):
+    _fields_ = [("coordinate_system", F16), ("root_scene", C.c_uint32),
+                ("materials", C.POINTER(AssetMaterial)), ("n_materials", C.c_uint64),
+                ("samplers", C.POINTER(Sampler)), ("n_samplers", C.c_uint64),
+                ("scenes", C.POINTER(AssetScene)), ("n_scenes", C.c_uint64),
+                ("nodes", C.POINTER(AssetNode)), ("n_nodes", C.c_uint64),
+                ("images", C.POINTER(AssetImage)), ("n_images", C.c_uint64),
+                ("meshes", C.POINTER(AssetMesh)), ("n_meshes", C.c_uint64)]
+
+
+RSY_OPEN_FAILED, RSY_WRITE_FAILED = 8, 9
+
 NEURAL_WEIGHTS_FP16 = 0
 NEURAL_WEIGHTS_FP8 = 1
 NEURAL_INIT_FROM_MAPS = 0
@@ -165,7 +214,7 @@ NEURAL_CHANNELS = 8
 NEURAL_OUTPUTS = 7
 
 EXPECTED_SIZES = {Position: 64, GpuMaterial: 64, ObjectRecord: 164, SceneData: 408, Draw: 48, MeshRange: 24,
-                  Sampler: 8}
+                  Sampler: 8, AssetMaterial: 68, AssetSurface: 36}
 for _t, _n in EXPECTED_SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)
 
@@ -213,6 +262,17 @@ PROTOTYPES = [
                                   C.POINTER(C.c_float)]),
     ("rsy_object_record_from_trs", None, [C.POINTER(C.c_float), C.c_float, C.c_float, C.POINTER(ObjectRecord)]),
     ("rsy_mat4_mul", None, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("rsy_asset_encode", C.c_int, [C.POINTER(AssetDesc), C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    ("rsy_asset_write", C.c_int, [C.POINTER(AssetDesc), C.c_char_p]),
+    ("rsy_asset_decode", C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("rsy_asset_read", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("rsy_asset_describe", C.c_int, [C.c_void_p, C.POINTER(AssetDesc)]),
+    ("rsy_asset_destroy", None, [C.c_void_p]),
+    ("rsy_asset_geometry_size", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                          C.POINTER(C.c_uint32)]),
+    ("rsy_asset_pack_geometry", C.c_int, [C.c_void_p, C.POINTER(Position), C.c_uint64, C.POINTER(C.c_uint32),
+                                          C.c_uint64, C.POINTER(MeshRange), C.c_uint32]),
+    ("rsy_asset_pack_materials", C.c_int, [C.c_void_p, C.POINTER(GpuMaterial), C.c_uint32]),
     ("rsy_scene_generate", C.c_int, [C.POINTER(SceneDesc), C.POINTER(C.POINTER(Scene))]),
     ("rsy_scene_free", None, [C.POINTER(Scene)]),
 ]
