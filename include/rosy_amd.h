@@ -313,6 +313,15 @@ rsy_result rsy_get_device_outputs(rsy_ctx* ctx, uint32_t view, const void** vis,
 #define RSY_DEBUG_STAGE_TIMING 2u  /* record HIP events between stages */
 rsy_result rsy_set_debug_flags(rsy_ctx* ctx, uint32_t flags);
 
+/* The output stage of the last shaded frame for one camera view (Graphics.cpp:1908, 5835-5880):
+ * its colour becomes the RGBA16F draw image (round to nearest even), which is blitted with a
+ * linear filter (vkCmdBlitImage2 mapping: destination texel centre (x + 0.5) * src / dst,
+ * clamp-to-edge) to a width x height B8G8R8A8_SRGB image (colour through the sRGB encode, alpha
+ * linear, both rounded to 8 bits).  Enqueued on the context stream; the readbacks synchronise. */
+rsy_result rsy_present(rsy_ctx* ctx, uint32_t view, uint32_t width, uint32_t height);
+rsy_result rsy_read_draw_image(rsy_ctx* ctx, uint16_t* out_rgba16f, uint64_t n_pixels);
+rsy_result rsy_read_present(rsy_ctx* ctx, uint8_t* out_bgra8, uint64_t n_pixels);
+
 /* Autotuner: replays the given frame n_iters times per candidate bin size (32, 64) and keeps the
  * fastest for later frames with tile_size == 0.  Writes the chosen size. */
 rsy_result rsy_autotune(rsy_ctx* ctx, const rsy_frame_desc* frame, uint32_t n_iters, uint32_t* chosen_tile);

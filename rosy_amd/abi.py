@@ -242,6 +242,9 @@ PROTOTYPES = [
     ("rsy_get_frame_stats", C.c_int, [C.c_void_p, C.POINTER(FrameStats)]),
     ("rsy_get_device_outputs", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     ("rsy_set_debug_flags", C.c_int, [C.c_void_p, C.c_uint32]),
+    ("rsy_present", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("rsy_read_draw_image", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("rsy_read_present", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     ("rsy_autotune", C.c_int, [C.c_void_p, C.POINTER(FrameDesc), C.c_uint32, C.POINTER(C.c_uint32)]),
     ("rsy_neural_default_config", None, [C.POINTER(NeuralConfig)]),
     ("rsy_neural_init", C.c_int, [C.c_void_p, C.POINTER(NeuralConfig)]),

@@ -892,6 +892,45 @@ rsy_result rsy_read_color(rsy_ctx* c, uint32_t view, float* out, uint64_t n_pixe
     return RSY_OK;
 }
 
+rsy_result rsy_present(rsy_ctx* c, uint32_t view, uint32_t width, uint32_t height) {
+    if (!c) return RSY_INVALID_ARGUMENT;
+    rsy_result r = rsy_synchronize(c);
+    if (r != RSY_OK) return r;
+    uint32_t idx;
+    if ((r = camera_index(c, view, idx)) != RSY_OK) return r;
+    if (!c->layout.shade) return fail(c, RSY_INVALID_STATE, "last frame was not shaded");
+    if (width == 0 || height == 0 || width > 16384 || height > 16384)
+        return fail(c, RSY_INVALID_ARGUMENT, "present extent must be 1..16384 on each side");
+    const ViewGPU* g = camera_vg(c, view);
+    HIP_TRY(c, c->draw_image.ensure(size_t(g->width) * g->height * 8), "allocate draw image");
+    HIP_TRY(c, c->present_image.ensure(size_t(width) * height * 4), "allocate present image");
+    HIP_TRY(c, launch_present(c->color_out[idx].as<float4>(), c->draw_image.as<uint2>(), g->width, g->height,
+                              c->present_image.as<uint32_t>(), width, height, c->stream), "present kernels");
+    c->draw_w = g->width;
+    c->draw_h = g->height;
+    c->present_w = width;
+    c->present_h = height;
+    return RSY_OK;
+}
+
+rsy_result rsy_read_draw_image(rsy_ctx* c, uint16_t* out, uint64_t n_pixels) {
+    if (!c || !out) return RSY_INVALID_ARGUMENT;
+    if (c->draw_w == 0) return fail(c, RSY_INVALID_STATE, "nothing presented");
+    if (n_pixels != uint64_t(c->draw_w) * c->draw_h) return fail(c, RSY_INVALID_ARGUMENT, "pixel count mismatch");
+    HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
+    HIP_TRY(c, hipMemcpy(out, c->draw_image.p, n_pixels * 8, hipMemcpyDeviceToHost), "read draw image");
+    return RSY_OK;
+}
+
+rsy_result rsy_read_present(rsy_ctx* c, uint8_t* out, uint64_t n_pixels) {
+    if (!c || !out) return RSY_INVALID_ARGUMENT;
+    if (c->present_w == 0) return fail(c, RSY_INVALID_STATE, "nothing presented");
+    if (n_pixels != uint64_t(c->present_w) * c->present_h) return fail(c, RSY_INVALID_ARGUMENT, "pixel count mismatch");
+    HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
+    HIP_TRY(c, hipMemcpy(out, c->present_image.p, n_pixels * 4, hipMemcpyDeviceToHost), "read present image");
+    return RSY_OK;
+}
+
 rsy_result rsy_read_shadow_factor(rsy_ctx* c, uint32_t view, float* out, uint64_t n_pixels) {
     if (!c || !out) return RSY_INVALID_ARGUMENT;
     rsy_result r = rsy_synchronize(c);

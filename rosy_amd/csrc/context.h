@@ -191,6 +191,9 @@ struct rsy_ctx {
     bool frame_validated = false;
     rsy_frame_stats stats{};
     uint32_t autotuned_tile = 0;
+    // output stage (rsy_present): the draw image of the presented view and the swapchain image
+    rsy::DevBuf draw_image, present_image;
+    uint32_t draw_w = 0, draw_h = 0, present_w = 0, present_h = 0;
 
     // graph cache
     hipGraphExec_t graph_exec = nullptr;

@@ -209,6 +209,9 @@ hipError_t launch_binning(const BinLaunch& b, hipStream_t s);
 constexpr uint32_t kScanPartialBytes = 32;  // per 4096 bins
 hipError_t launch_raster(const RasterLaunch& r, hipStream_t s);
 uint32_t raster_items_cap(uint32_t total_bins, uint32_t list_cap);  // work-item table entries
+// output stage: colour -> RGBA16F draw image (sw x sh) -> linear blit to B8G8R8A8_SRGB (dw x dh)
+hipError_t launch_present(const float4* color, uint2* draw, uint32_t sw, uint32_t sh, uint32_t* out, uint32_t dw,
+                          uint32_t dh, hipStream_t s);
 hipError_t launch_shade(const ShadeLaunch& l, hipStream_t s);
 hipError_t launch_nm_init(const NmInitLaunch& l, hipStream_t s);
 hipError_t launch_nm_step(const NmTrainLaunch& l, bool explicit_batch, hipStream_t s);
