@@ -57,6 +57,10 @@ def lib():
         L.orc_count_tile_pairs.restype = C.c_uint64
         L.orc_count_tile_pairs.argtypes = [C.POINTER(Geometry), C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int,
                                            C.POINTER(abi.RasterState), C.c_uint32, C.c_int]
+        L.orc_blend_layers.restype = C.c_int
+        L.orc_blend_layers.argtypes = [C.POINTER(Geometry), C.POINTER(C.c_float), C.c_uint32, C.c_uint32,
+                                       C.POINTER(abi.RasterState), C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                       C.c_int]
         L.orc_tile_counts.restype = C.c_int
         L.orc_tile_counts.argtypes = [C.POINTER(Geometry), C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int,
                                       C.POINTER(abi.RasterState), C.c_uint32, C.c_void_p, C.c_void_p, C.c_int]
@@ -90,10 +94,16 @@ def threads():
     return max(1, min(os.cpu_count() or 1, 16))
 
 
-def draw_list(scene, shadow):
-    """Shadow list = every surface; opaque list = surfaces with blended == 0 (Graphics.cpp:4716-4735)."""
+SHADOW_LIST, OPAQUE_LIST, BLENDED_LIST = 0, 1, 2
+
+
+def draw_list(scene, which):
+    """Shadow list = every surface; opaque list = surfaces with blended == 0; blended list = the
+    others, in surface order (Graphics.cpp:4716-4735).  `which` is a list number, or a bool for
+    the shadow (True) / opaque (False) list."""
+    which = (SHADOW_LIST if which else OPAQUE_LIST) if isinstance(which, bool) else which
     surf = scene.surfaces()
-    keep = [d for d in surf if shadow or not d.blended]
+    keep = [d for d in surf if which == SHADOW_LIST or bool(d.blended) == (which == BLENDED_LIST)]
     arr = (abi.Draw * max(1, len(keep)))()
     for i, d in enumerate(keep):
         arr[i] = d
@@ -198,11 +208,12 @@ def tile_counts(scene, view_projection, width, height, shadow, raster_state, til
 
 
 def shade_view(scene, view, cascades, shadow_maps, vis, scene_data=None, raster_state=None, override_shadow=None,
-               n_threads=None, neural=None):
+               n_threads=None, neural=None, which=OPAQUE_LIST):
     """FP64 shading of one camera view.  Returns (rgba float64 [h,w,4], shadow factor [h,w], margin [h,w]).
-    With `neural` (NeuralParams) the maps of the active materials are replaced by the decoder."""
+    With `neural` (NeuralParams) the maps of the active materials are replaced by the decoder.  The
+    visibility words name triangles of draw list `which`."""
     s = scene.s
-    g = geometry(scene, False)
+    g = geometry(scene, which)
     maps = [np.ascontiguousarray(m, dtype=np.float32) for m in shadow_maps]
     ptrs = (C.POINTER(C.c_float) * max(1, len(maps)))(*[m.ctypes.data_as(C.POINTER(C.c_float)) for m in maps])
     casc = (abi.ShadowCascade * max(1, len(cascades)))(*cascades)
@@ -232,3 +243,50 @@ def shade_view(scene, view, cascades, shadow_maps, vis, scene_data=None, raster_
     if rc != 0:
         raise RuntimeError(f"orc_shade_view failed: {rc}")
     return rgba, shadow, margin
+
+
+def blend_layers(scene, view, opaque_vis, max_layers=16, raster_state=None, n_threads=None):
+    """The blended list's fragments per pixel in primitive order: (layers [K, h, w] u64 visibility
+    words, count [h, w])."""
+    g = geometry(scene, BLENDED_LIST)
+    h, w = opaque_vis.shape
+    vp = (C.c_float * 16)(*view.view_projection[:])
+    rs = raster_state if raster_state is not None else scene.s.raster_state
+    layers = np.zeros((max_layers, h, w), dtype=np.uint64)
+    count = np.zeros((h, w), dtype=np.uint32)
+    ov = np.ascontiguousarray(opaque_vis, dtype=np.uint64)
+    rc = lib().orc_blend_layers(C.byref(g), vp, w, h, C.byref(rs), ov.ctypes.data, max_layers, layers.ctypes.data,
+                                count.ctypes.data, n_threads or threads())
+    if rc != 0:
+        raise RuntimeError(f"orc_blend_layers failed: {rc}")
+    return layers, count
+
+
+def blend_view(scene, view, cascades, shadow_maps, opaque_vis, opaque_rgba, max_layers=16, scene_data=None,
+               raster_state=None, n_threads=None):
+    """The blended pass over an opaque frame, FP64: each pixel's blended fragments in primitive
+    order are shaded (fragmentMain), discarded when alpha mode 1 puts alpha below the cutoff
+    (basic.slang:335-337), and blended as colour = src.rgb * src.a + dst.rgb * (1 - src.a),
+    alpha = src.a (Graphics.cpp:5657-5664)."""
+    layers, count = blend_layers(scene, view, opaque_vis, max_layers, raster_state, n_threads)
+    out = np.array(opaque_rgba, dtype=np.float64, copy=True)
+    draws, n = draw_list(scene, BLENDED_LIST)
+    tri_base = np.cumsum([0] + [draws[i].index_count // 3 for i in range(n)])
+    nm = scene.s.n_materials
+    modes = np.array([scene.s.materials[i].alpha_mode for i in range(nm)] + [0], dtype=np.int64)
+    cuts = np.array([scene.s.materials[i].alpha_cutoff for i in range(nm)] + [0.0], dtype=np.float64)
+    mat_of_draw = np.array([min(int(draws[i].material_index), nm) for i in range(n)] + [nm], dtype=np.int64)
+    for k in range(int(count.max()) if count.size else 0):
+        has = count > k
+        rgba, _, _ = shade_view(scene, view, cascades, shadow_maps, np.where(has, layers[k], 0), scene_data,
+                                raster_state, n_threads=n_threads, which=BLENDED_LIST)
+        tri = (layers[k] & 0xFFFFFFFF).astype(np.int64) - 1
+        d = np.where(has, np.searchsorted(tri_base, tri, side="right") - 1, n)
+        mat = mat_of_draw[np.clip(d, 0, n)]  # index nm: no material (never discarded)
+        mode, cut = modes[mat], cuts[mat]
+        a = rgba[..., 3]
+        keep = has & ~((mode == 1) & (a < cut))
+        src = rgba[..., :3] * a[..., None] + out[..., :3] * (1.0 - a[..., None])
+        out[..., :3] = np.where(keep[..., None], src, out[..., :3])
+        out[..., 3] = np.where(keep, a, out[..., 3])
+    return out, count

@@ -303,6 +303,77 @@ int orc_raster_view(const orc_geometry* g, const float view_projection[16], uint
     return RSY_OK;
 }
 
+int orc_blend_layers(const orc_geometry* g, const float view_projection[16], uint32_t width, uint32_t height,
+                     const rsy_raster_state* rs, const uint64_t* opaque_vis, uint32_t max_layers, uint64_t* layers,
+                     uint32_t* count, int n_threads) {
+    if (!g || !rs || !opaque_vis || !count || width == 0 || height == 0) return RSY_INVALID_ARGUMENT;
+    if (max_layers && !layers) return RSY_INVALID_ARGUMENT;
+    Params p = make_params(width, height, 0, rs);
+    p.cull = false;  // the blended pass draws both faces (Graphics.cpp:5675)
+    std::vector<Rec> recs;
+    int rc = build_records(g, view_projection, p, recs, nullptr, n_threads);
+    if (rc != RSY_OK) return rc;
+    const size_t np = size_t(width) * height;
+    std::memset(count, 0, sizeof(uint32_t) * np);
+    if (max_layers) std::memset(layers, 0, sizeof(uint64_t) * np * max_layers);
+    const int nt = std::max(1, n_threads);
+    std::vector<int> overflow(nt, 0);
+    auto work = [&](int t) {
+        const int32_t y0 = int32_t(uint64_t(height) * t / nt), y1 = int32_t(uint64_t(height) * (t + 1) / nt);
+        // the fragments of this thread's rows: (key, depth bits) per pixel
+        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> frags(size_t(std::max(0, y1 - y0)) * width);
+        for (const Rec& r : recs) {
+            const int32_t ya = std::max(r.py0, y0), yb = std::min(r.py1, y1 - 1);
+            if (ya > yb) continue;
+            int32_t A[3], B[3];
+            bool tl[3];
+            for (int i = 0; i < 3; ++i) {
+                const int j = (i + 1) % 3;
+                A[i] = r.Y[j] - r.Y[i];
+                B[i] = r.X[i] - r.X[j];
+                tl[i] = A[i] > 0 || (A[i] == 0 && B[i] > 0);
+            }
+            for (int32_t py = ya; py <= yb; ++py) {
+                const int32_t Py = py * 256 + 128;
+                for (int32_t px = r.px0; px <= r.px1; ++px) {
+                    const int32_t Px = px * 256 + 128;
+                    bool inside = true;
+                    for (int i = 0; i < 3; ++i) {
+                        const int64_t e = int64_t(A[i]) * (Px - r.X[i]) + int64_t(B[i]) * (Py - r.Y[i]);
+                        if (!(e > 0 || (e == 0 && tl[i]))) {
+                            inside = false;
+                            break;
+                        }
+                    }
+                    if (!inside) continue;
+                    float z = std::fmaf(r.dzdy, float(Py - r.Y[0]), std::fmaf(r.dzdx, float(Px - r.X[0]), r.z0));
+                    z = std::fmin(std::fmax(z, 0.0f), 1.0f);
+                    const uint32_t zb = fbits(z) & 0x7fffffffu;
+                    const size_t o = size_t(py) * width + px;
+                    if (zb < uint32_t(opaque_vis[o] >> 32)) continue;  // GEQUAL against the opaque depth
+                    frags[size_t(py - y0) * width + px].push_back({r.id + 1u, zb});
+                }
+            }
+        }
+        for (int32_t py = y0; py < y1; ++py)
+            for (uint32_t px = 0; px < width; ++px) {
+                auto& f = frags[size_t(py - y0) * width + px];
+                std::sort(f.begin(), f.end());  // primitive order
+                const size_t o = size_t(py) * width + px;
+                count[o] = uint32_t(f.size());
+                if (f.size() > max_layers) overflow[t] = 1;
+                for (size_t k = 0; k < f.size() && k < max_layers; ++k)
+                    layers[k * np + o] = (uint64_t(f[k].second) << 32) | f[k].first;
+            }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) th.emplace_back(work, t);
+    for (auto& x : th) x.join();
+    for (int o : overflow)
+        if (o) return RSY_OVERFLOW;
+    return RSY_OK;
+}
+
 uint64_t orc_count_tile_pairs(const orc_geometry* g, const float view_projection[16], uint32_t width,
                               uint32_t height, int is_shadow, const rsy_raster_state* rs, uint32_t tile,
                               int n_threads) {

@@ -66,7 +66,7 @@ rsy_result rebuild_lists(rsy_ctx* c) {
     for (int l = 0; l < 3; ++l)
         if (c->list_tris[l] >= 0x7fffffffu || c->list_verts[l] >= 0x7fffffffull)
             return fail(c, RSY_OVERFLOW, "draw list too large");
-    for (int l = 0; l < 2; ++l) {
+    for (int l = 0; l < 3; ++l) {
         const size_t bytes = std::max<size_t>(1, c->lists[l].size()) * sizeof(DrawGPU);
         HIP_TRY(c, c->lists_d[l].ensure(bytes), "allocate draw list");
         if (!c->lists[l].empty())
@@ -144,6 +144,8 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     L.views = views;  // matrices may change without a new layout
     if (reuse) {
         for (size_t i = 0; i < views.size(); ++i) std::memcpy(L.vg[i].view_projection, views[i].vp, sizeof(views[i].vp));
+        for (size_t i = views.size(); i < L.vg.size(); ++i)  // blend views follow their camera view
+            std::memcpy(L.vg[i].view_projection, views[L.vg[i].source].vp, sizeof(views[0].vp));
         return RSY_OK;
     }
     invalidate_graph(c);
@@ -155,13 +157,25 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     L.vblocks.clear();
     L.tblocks.clear();
     L.total_bins = 0;
+    L.blend_bins = 0;
     L.total_xv = 0;
     L.rec_total = L.clip_total = L.big_total = 0;
-    L.n_shadow = L.n_camera = 0;
+    L.n_shadow = L.n_camera = L.n_blend = 0;
     uint32_t cull_total = 0;
     const uint32_t shift = ilog2(tile);
-    for (uint32_t vi = 0; vi < views.size(); ++vi) {
-        const FrameViewDesc& f = views[vi];
+    // the requested views, then one blend view per camera view when the blended list is drawn
+    // (layer peeling of the blended list against that camera view's depth)
+    std::vector<uint32_t> order(views.size());
+    for (uint32_t vi = 0; vi < views.size(); ++vi) order[vi] = vi;
+    const bool blend = shade != RSY_SHADE_NONE && !c->lists[2].empty();
+    if (blend)
+        for (uint32_t vi = 0; vi < views.size(); ++vi)
+            if (!views[vi].shadow) order.push_back(vi);
+    L.first_blend = uint32_t(views.size());
+    for (uint32_t oi = 0; oi < order.size(); ++oi) {
+        const uint32_t vi = uint32_t(L.vg.size());
+        const FrameViewDesc& f = views[order[oi]];
+        const bool is_blend = oi >= views.size();
         ViewGPU g{};
         g.width = f.width;
         g.height = f.height;
@@ -170,7 +184,9 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         g.tiles_y = (f.height + tile - 1) >> shift;
         g.tile_base = L.total_bins;
         g.is_shadow = f.shadow ? 1 : 0;
-        g.list = f.shadow ? 0 : 1;
+        g.list = f.shadow ? 0 : (is_blend ? 2 : 1);
+        g.blend = is_blend ? 1 : 0;
+        g.source = order[oi];
         const auto& list = c->lists[g.list];
         g.n_draws = uint32_t(list.size());
         g.cull_base = cull_total;
@@ -185,7 +201,8 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         g.clip_cap = clip_cap;
         g.big_base = L.big_total;
         g.big_cap = std::max<uint32_t>(65536, tris / 4) * c->rec_scale;
-        g.cull = c->rs.cull_enabled ? 1 : 0;
+        // the blended pass draws both faces (VK_CULL_MODE_NONE, Graphics.cpp:5675)
+        g.cull = c->rs.cull_enabled && !is_blend ? 1 : 0;
         g.reverse = c->rs.reverse_winding_order_enabled ? 1 : 0;
         g.bias = c->rs.depth_bias_enabled ? 1 : 0;
         g.bias_const = c->rs.depth_bias_constant;
@@ -193,6 +210,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         g.bias_slope = c->rs.depth_bias_slope_factor;
         std::memcpy(g.view_projection, f.vp, sizeof(f.vp));
         L.total_bins += g.tiles_x * g.tiles_y;
+        if (is_blend) L.blend_bins += g.tiles_x * g.tiles_y;
         L.total_xv += c->list_verts[g.list];
         if (uint64_t(L.rec_total) + g.rec_cap >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "record buffer too large");
         L.rec_total += g.rec_cap;
@@ -203,12 +221,14 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
             for (uint32_t s = 0; s < list[d].vtx_count; s += 256) L.vblocks.push_back({vi, d, s, 0});
             for (uint32_t s = 0; s < list[d].tri_count; s += 256) L.tblocks.push_back({vi, d, s, 0});
         }
-        if (f.shadow) ++L.n_shadow; else ++L.n_camera;
+        if (is_blend) ++L.n_blend;
+        else if (f.shadow) ++L.n_shadow;
+        else ++L.n_camera;
         L.vg.push_back(g);
     }
     if (L.total_xv >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "transformed vertex buffer too large");
     // buffers
-    const size_t nv = views.size();
+    const size_t nv = L.vg.size();
     HIP_TRY(c, c->views_d.ensure(nv * sizeof(ViewGPU)), "allocate views");
     HIP_TRY(c, c->xv.ensure(size_t(std::max<uint64_t>(1, L.total_xv)) * sizeof(float4)), "allocate vertices");
     HIP_TRY(c, c->recs.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * sizeof(TriRec)), "allocate records");
@@ -245,10 +265,15 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     c->color_out.resize(L.n_camera);
     c->shadow_factor_out.resize(L.n_camera);
     c->shadow_maps.resize(L.n_shadow);
-    uint32_t ci = 0, si = 0;
+    c->blend_layers.resize(L.n_blend);
+    uint32_t ci = 0, si = 0, bi = 0;
     for (auto& g : L.vg) {
         const size_t px = size_t(g.width) * g.height;
-        if (g.is_shadow) {
+        if (g.blend) {
+            HIP_TRY(c, c->blend_layers[bi].ensure(px * 8), "allocate blend layer keys");
+            g.out = reinterpret_cast<uint64_t>(c->blend_layers[bi].p);
+            ++bi;
+        } else if (g.is_shadow) {
             HIP_TRY(c, c->shadow_maps[si].ensure(px * 4), "allocate shadow map");
             g.out = reinterpret_cast<uint64_t>(c->shadow_maps[si].p);
             ++si;
@@ -274,11 +299,23 @@ rsy_result upload_frame(rsy_ctx* c) {
     const uint32_t nv = uint32_t(L.vg.size());
     c->h_shade_views.clear();
     c->h_shade_cascades.clear();
-    c->h_xv_bases.clear();
+    c->h_blend_views.clear();
+    std::vector<uint32_t> camera_of(nv, 0);  // shading view index of each camera view
     uint32_t blocks = 0, ci = 0;
     for (uint32_t i = 0; i < nv; ++i) {
         const ViewGPU& vg = L.vg[i];
-        if (vg.is_shadow) {
+        if (vg.blend) {
+            // the blended pass of camera view vg.source: its keys, records and transformed vertices,
+            // shaded into that view's colour buffer
+            ShadeView sv = c->h_shade_views[camera_of[vg.source]];
+            sv.vis = nullptr;
+            sv.keys = reinterpret_cast<const uint32_t*>(vg.out);
+            sv.recs = c->recs.as<TriRec>() + vg.rec_base;
+            sv.boxes = c->boxes.as<uint2>() + vg.rec_base;
+            sv.xv_base = vg.xv_base;
+            sv.shadow_factor = nullptr;
+            c->h_blend_views.push_back(sv);
+        } else if (vg.is_shadow) {
             ShadeCascade sc{};
             sc.size = vg.width;
             sc.split_far = L.views[i].split_far;
@@ -293,38 +330,42 @@ rsy_result upload_frame(rsy_ctx* c) {
             sv.block_base = blocks;
             blocks += sv.blocks_x * ((vg.height + 15) / 16);
             sv.vis = reinterpret_cast<const unsigned long long*>(vg.out);
+            sv.keys = nullptr;
             sv.recs = c->recs.as<TriRec>() + vg.rec_base;
             sv.boxes = c->boxes.as<uint2>() + vg.rec_base;
+            sv.xv_base = vg.xv_base;
             sv.color = c->color_out[ci].as<float4>();
             sv.shadow_factor = (c->debug_flags & RSY_DEBUG_SHADOW_FACTOR) ? c->shadow_factor_out[ci].as<float>() : nullptr;
             std::memcpy(sv.view_projection, vg.view_projection, sizeof(sv.view_projection));
             std::memcpy(sv.camera_position, L.views[i].camera_position, sizeof(sv.camera_position));
+            camera_of[i] = uint32_t(c->h_shade_views.size());
             c->h_shade_views.push_back(sv);
-            c->h_xv_bases.push_back(vg.xv_base);
             ++ci;
         }
     }
     c->shade_blocks = blocks;
-    const size_t nsv = c->h_shade_views.size(), nsc = c->h_shade_cascades.size();
+    const size_t nsv = c->h_shade_views.size(), nsc = c->h_shade_cascades.size(), nbv = c->h_blend_views.size();
     const size_t a0 = 0, a1 = (nv * sizeof(ViewGPU) + 255) & ~size_t(255);
     const size_t a2 = a1 + ((nsv * sizeof(ShadeView) + 255) & ~size_t(255));
     const size_t a3 = a2 + ((nsc * sizeof(ShadeCascade) + 255) & ~size_t(255));
-    const size_t total = a3 + nsv * 4 + 256;
+    const size_t total = a3 + nbv * sizeof(ShadeView) + 256;
     HIP_TRY(c, c->shade_views_d.ensure(std::max<size_t>(1, nsv) * sizeof(ShadeView) + 64), "allocate shade views");
     HIP_TRY(c, c->shade_cascades_d.ensure(std::max<size_t>(1, nsc) * sizeof(ShadeCascade) + 64), "allocate cascades");
-    HIP_TRY(c, c->xv_bases_d.ensure(std::max<size_t>(1, nsv) * 4 + 64), "allocate xv bases");
+    HIP_TRY(c, c->blend_shade_views_d.ensure(std::max<size_t>(1, nbv) * sizeof(ShadeView) + 64), "allocate blend views");
     HIP_TRY(c, c->staging.ensure(total), "allocate staging");
     uint8_t* st = static_cast<uint8_t*>(c->staging.p);
     std::memcpy(st + a0, L.vg.data(), nv * sizeof(ViewGPU));
     if (nsv) std::memcpy(st + a1, c->h_shade_views.data(), nsv * sizeof(ShadeView));
     if (nsc) std::memcpy(st + a2, c->h_shade_cascades.data(), nsc * sizeof(ShadeCascade));
-    if (nsv) std::memcpy(st + a3, c->h_xv_bases.data(), nsv * 4);
+    if (nbv) std::memcpy(st + a3, c->h_blend_views.data(), nbv * sizeof(ShadeView));
     HIP_TRY(c, hipMemcpyAsync(c->views_d.p, st + a0, nv * sizeof(ViewGPU), hipMemcpyHostToDevice, s), "upload views");
     if (nsv)
         HIP_TRY(c, hipMemcpyAsync(c->shade_views_d.p, st + a1, nsv * sizeof(ShadeView), hipMemcpyHostToDevice, s), "upload");
     if (nsc)
         HIP_TRY(c, hipMemcpyAsync(c->shade_cascades_d.p, st + a2, nsc * sizeof(ShadeCascade), hipMemcpyHostToDevice, s), "upload");
-    if (nsv) HIP_TRY(c, hipMemcpyAsync(c->xv_bases_d.p, st + a3, nsv * 4, hipMemcpyHostToDevice, s), "upload");
+    if (nbv)
+        HIP_TRY(c, hipMemcpyAsync(c->blend_shade_views_d.p, st + a3, nbv * sizeof(ShadeView), hipMemcpyHostToDevice, s),
+                "upload");
     return RSY_OK;
 }
 
@@ -342,6 +383,7 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     g.n_views = nv;
     g.draws0 = c->lists_d[0].as<DrawGPU>();
     g.draws1 = c->lists_d[1].as<DrawGPU>();
+    g.draws2 = c->lists_d[2].as<DrawGPU>();
     g.objects = c->objects.as<float>();
     g.vblocks = c->vblocks_d.as<BlockEntry>();
     g.n_vblocks = uint32_t(L.vblocks.size());
@@ -433,10 +475,43 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
         std::memcpy(sh.clear_color, c->rs.clear_color, sizeof(sh.clear_color));
         sh.srgb_lut = c->srgb_lut.as<float>();
         sh.xv = g.xv;
-        sh.xv_bases = c->xv_bases_d.as<uint32_t>();
         if (L.shade == RSY_SHADE_NEURAL) neural_shade_args(c, sh);
         if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[3], s), "event");
         HIP_TRY(c, launch_shade(sh, s), "shade kernel");
+        if (L.n_blend) {
+            // the blended pass: peel the blended list layer by layer in primitive order and blend
+            // each layer over the shaded colour
+            const uint32_t passes = c->blend_passes;
+            HIP_TRY(c, c->blend_found.ensure(size_t(passes) * 4 + 64), "allocate blend counters");
+            HIP_TRY(c, launch_fill_u32(c->blend_found.as<uint32_t>(), passes, 0u, s), "clear blend counters");
+            PeelLaunch pl{};
+            pl.views = g.views;
+            pl.n_views = nv;
+            pl.first_blend = L.first_blend;
+            pl.blend_bins = L.blend_bins;
+            pl.tile = L.tile;
+            pl.bin_counts = g.bin_counts;
+            pl.bin_offsets = b.bin_offsets;
+            pl.lists = b.lists;
+            pl.list_cap = c->list_cap;
+            pl.recs = g.recs;
+            pl.rec_cap_total = L.rec_total;
+            pl.found = c->blend_found.as<uint32_t>();
+            pl.n_passes = passes;
+            pl.counters = g.counters;
+            ShadeLaunch sb = sh;
+            sb.views = c->blend_shade_views_d.as<ShadeView>();
+            sb.n_views = uint32_t(c->h_blend_views.size());
+            sb.draws = c->lists_d[2].as<DrawGPU>();
+            sb.n_draws = uint32_t(c->lists[2].size());
+            sb.neural = 0;
+            sb.blend_found = pl.found;
+            for (uint32_t pass = 0; pass < passes; ++pass) {
+                HIP_TRY(c, launch_raster_peel(pl, pass, s), "peel kernel");
+                sb.blend_pass = pass;
+                HIP_TRY(c, launch_shade_blend(sb, s), "blend kernel");
+            }
+        }
     } else if (timing) {
         HIP_TRY(c, hipEventRecord(c->ev_stage[3], s), "event");
     }
@@ -522,9 +597,18 @@ rsy_result finish_frame(rsy_ctx* c) {
         const uint32_t nv = uint32_t(c->layout.vg.size());
         const uint32_t total_pairs = static_cast<const uint32_t*>(c->readback_counters.p)[nv * kCtrPerView];
         const bool list_over = static_cast<const uint32_t*>(c->readback_counters.p)[nv * kCtrPerView + 2] != 0;
-        bool rec_over = false;
-        for (uint32_t v = 0; v < nv; ++v)
-            rec_over |= static_cast<const uint32_t*>(c->readback_counters.p)[v * kCtrPerView + kCtrOverflow] != 0;
+        bool rec_over = false, layers_over = false;
+        for (uint32_t v = 0; v < nv; ++v) {
+            const uint32_t ov = static_cast<const uint32_t*>(c->readback_counters.p)[v * kCtrPerView + kCtrOverflow];
+            rec_over |= (ov & ~kOverflowLayers) != 0;
+            layers_over |= (ov & kOverflowLayers) != 0;
+        }
+        if (layers_over) {
+            if (c->blend_passes >= kMaxBlendPasses)
+                return fail(c, RSY_OVERFLOW, "blended layers at a pixel exceed the peeling pass limit");
+            c->blend_passes = std::min(2 * c->blend_passes, kMaxBlendPasses);
+            invalidate_graph(c);
+        }
         if (rec_over) {
             c->rec_scale *= 2;
             c->layout.valid = false;

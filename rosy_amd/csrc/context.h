@@ -107,7 +107,8 @@ struct Layout {
     uint32_t total_bins = 0;
     uint64_t total_xv = 0;
     uint32_t rec_total = 0, clip_total = 0, big_total = 0;
-    uint32_t n_shadow = 0, n_camera = 0;
+    uint32_t n_shadow = 0, n_camera = 0, n_blend = 0;
+    uint32_t first_blend = 0, blend_bins = 0;  // first blend view, bins of all blend views
     bool valid = false;
 };
 
@@ -117,6 +118,8 @@ inline bool same_views(const std::vector<FrameViewDesc>& a, const std::vector<Fr
         if (a[i].shadow != b[i].shadow || a[i].width != b[i].width || a[i].height != b[i].height) return false;
     return true;
 }
+
+constexpr uint32_t kMaxBlendPasses = 64;  // blended layers a pixel may hold
 
 inline uint32_t ilog2(uint32_t v) {
     uint32_t r = 0;
@@ -161,7 +164,7 @@ struct rsy_ctx {
     std::vector<uint32_t> list_index[3];  // surface indices
     uint32_t list_tris[3] = {0, 0, 0};
     uint64_t list_verts[3] = {0, 0, 0};
-    rsy::DevBuf lists_d[2];
+    rsy::DevBuf lists_d[3];
     uint64_t lists_version = 0;
     bool lists_dirty = true;
     // per-frame uniforms
@@ -173,15 +176,19 @@ struct rsy_ctx {
     // frame buffers
     rsy::Layout layout;
     rsy::DevBuf views_d, xv, recs, boxes, clipq, counters, bin_counts, bin_offsets, pair_slots, big_pairs, block_valid, totals,
-        lists_mem, vblocks_d, tblocks_d, shade_views_d, shade_cascades_d, xv_bases_d, items, zero_bins,
+        lists_mem, vblocks_d, tblocks_d, shade_views_d, shade_cascades_d, items, zero_bins,
         draw_culled, scan_partial;
     uint32_t list_cap = 0;
     uint32_t rec_scale = 1;  // grows on overflow
     std::vector<rsy::DevBuf> vis_out, color_out, shadow_factor_out, shadow_maps;
+    std::vector<rsy::DevBuf> blend_layers;  // per blend view: layer keys, then last peeled keys
+    rsy::DevBuf blend_found;                // per peeling pass: pixels that found a layer
+    rsy::DevBuf blend_shade_views_d;
+    std::vector<rsy::ShadeView> h_blend_views;
+    uint32_t blend_passes = 4;              // peeling passes per frame (grows when a frame needs more)
     rsy::PinnedBuf staging, readback_counters;
     std::vector<rsy::ShadeView> h_shade_views;
     std::vector<rsy::ShadeCascade> h_shade_cascades;
-    std::vector<uint32_t> h_xv_bases;
     uint32_t shade_blocks = 0;
 
     // last frame (for re-running after an overflow)

@@ -39,13 +39,15 @@ struct DrawGPU {
 // Per-view counters: records appended by the clipper, queued clipped triangles, entries of the
 // big-pair list, overflow flags, clipped-triangle statistic.
 enum : uint32_t { kCtrRecords = 0, kCtrClip = 1, kCtrBig = 2, kCtrOverflow = 3, kCtrClipped = 4, kCtrPerView = 8 };
+// kCtrOverflow bits: 1 clip queue, 2 records, 4 big pairs, 8 blended layers beyond the frame's passes
+constexpr uint32_t kOverflowLayers = 8u;
 
 struct ViewGPU {
     uint32_t width, height;
     uint32_t tile, tiles_x, tiles_y;
     uint32_t tile_base;     // first bin of this view in the global bin arrays
     uint32_t is_shadow;
-    uint32_t list;          // 0 = shadow list, 1 = opaque list
+    uint32_t list;          // 0 = shadow list, 1 = opaque list, 2 = blended list
     uint32_t n_draws;
     uint32_t xv_base;       // first transformed vertex of this view
     uint32_t rec_base, rec_cap;  // triangle slots [rec_base, rec_base + tri_count), then clipper output
@@ -55,8 +57,10 @@ struct ViewGPU {
     uint32_t cull, reverse, bias;
     float bias_const, bias_clamp, bias_slope;
     uint32_t cull_base;     // first per-draw culling flag of this view
-    uint32_t pad[2];
-    uint64_t out;           // device address: u64 visibility (camera) or u32 depth bits (shadow)
+    uint32_t blend;         // 1: the blended list of camera view `source`, resolved by layer peeling
+    uint32_t source;
+    uint64_t out;           // device address: u64 visibility (camera), u32 depth bits (shadow), or for a
+                            // blend view the u32 layer keys followed by the u32 last peeled keys
     float view_projection[16];
 };
 

@@ -8,11 +8,18 @@
 
 namespace rsy {
 
+// The three draw lists of a frame, indexed by ViewGPU::list.
+struct DrawLists {
+    const DrawGPU* l[3];  // shadow, opaque, blended
+    __host__ __device__ const DrawGPU* of(uint32_t list) const { return l[list]; }
+};
+
 struct GeometryLaunch {
     const ViewGPU* views;
     uint32_t n_views;
     const DrawGPU* draws0;  // shadow list
     const DrawGPU* draws1;  // opaque list
+    const DrawGPU* draws2;  // blended list
     const float* objects;   // packed 164-byte object records
     const BlockEntry* vblocks;
     uint32_t n_vblocks;
@@ -75,14 +82,35 @@ struct RasterLaunch {
     const uint32_t* totals;
 };
 
+// Layer peeling of the blend views (one launch per pass).
+struct PeelLaunch {
+    const ViewGPU* views;
+    uint32_t n_views;
+    uint32_t first_blend;  // first blend view; blend views come last and own the last bins
+    uint32_t blend_bins;
+    uint32_t tile;
+    const uint32_t* bin_counts;
+    const uint32_t* bin_offsets;
+    const uint32_t* lists;
+    uint32_t list_cap;
+    const TriRec* recs;
+    uint64_t rec_cap_total;
+    uint32_t* found;  // per pass: pixels that found a layer
+    uint32_t n_passes;
+    uint32_t* counters;
+};
+
 // Per-pixel shading inputs of one camera view.
 struct ShadeView {
     uint32_t width, height;
     uint32_t block_base;  // first shading block of this view
     uint32_t blocks_x;
     const unsigned long long* vis;
+    const uint32_t* keys;  // blended pass: the peeled layer's key (triangle id + 1) per pixel, 0 = none
     const TriRec* recs;    // the raster view's record slots, indexed by triangle id
     const uint2* boxes;    // their boxes (kInvalidBox: no record, e.g. a clipped triangle)
+    uint32_t xv_base;      // first transformed vertex of the raster view
+    uint32_t pad;
     float4* color;
     float* shadow_factor;  // nullable debug output
     float view_projection[16];
@@ -119,7 +147,9 @@ struct ShadeLaunch {
     float clear_color[4];
     const float* srgb_lut;      // 256 entries
     const float4* xv;           // transformed vertices of all views
-    const uint32_t* xv_bases;   // per shading view: first transformed vertex of its raster view
+    // blended pass: pixels found by each peeling pass, and the pass this launch blends
+    const uint32_t* blend_found;
+    uint32_t blend_pass;
     // material decoder (shade mode NEURAL only)
     const uint32_t* nm_slot;    // per material: latent grid slot or 0xFFFFFFFF
     const uint32_t* nm_active;  // per material: 1 = decoded by the network, 0 = plain textures
@@ -209,10 +239,12 @@ hipError_t launch_binning(const BinLaunch& b, hipStream_t s);
 constexpr uint32_t kScanPartialBytes = 32;  // per 4096 bins
 hipError_t launch_raster(const RasterLaunch& r, hipStream_t s);
 uint32_t raster_items_cap(uint32_t total_bins, uint32_t list_cap);  // work-item table entries
+hipError_t launch_raster_peel(const PeelLaunch& p, uint32_t pass, hipStream_t s);
 // output stage: colour -> RGBA16F draw image (sw x sh) -> linear blit to B8G8R8A8_SRGB (dw x dh)
 hipError_t launch_present(const float4* color, uint2* draw, uint32_t sw, uint32_t sh, uint32_t* out, uint32_t dw,
                           uint32_t dh, hipStream_t s);
 hipError_t launch_shade(const ShadeLaunch& l, hipStream_t s);
+hipError_t launch_shade_blend(const ShadeLaunch& l, hipStream_t s);  // one pass of the blended pass
 hipError_t launch_nm_init(const NmInitLaunch& l, hipStream_t s);
 hipError_t launch_nm_step(const NmTrainLaunch& l, bool explicit_batch, hipStream_t s);
 hipError_t launch_nm_export(const float* lat, uint4* lat16, uint64_t cells, hipStream_t s);

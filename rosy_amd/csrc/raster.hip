@@ -205,15 +205,14 @@ __device__ inline bool box_outside(const float* m, const float* bmin, const floa
 // chain in k order, then each thread reads its vertex from the SoA x/y/z streams (coalesced 12 B)
 // and writes the clip position (coalesced 16 B).  A draw whose box is outside the view is skipped,
 // and its first block records that for k_setup.
-__global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, const ViewGPU* views,
-                                                   const DrawGPU* draws0, const DrawGPU* draws1,
+__global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, const ViewGPU* views, DrawLists lists,
                                                    const float* objects, const float* px, const float* py,
                                                    const float* pz, float4* xv, uint32_t* draw_culled) {
     __shared__ float m[16];
     __shared__ uint32_t culled;
     const BlockEntry e = blocks[blockIdx.x];
     const ViewGPU& V = views[e.view];
-    const DrawGPU& dr = (V.list ? draws1 : draws0)[e.draw];
+    const DrawGPU& dr = lists.of(V.list)[e.draw];
     if (threadIdx.x < 16) {
         const int col = threadIdx.x >> 2, row = threadIdx.x & 3;
         const float* a = V.view_projection;
@@ -289,9 +288,8 @@ __device__ void bin_wide(bool active, const TriRec& r, uint32_t ri, const ViewGP
 constexpr uint32_t kHash = 1024;
 constexpr uint32_t kEmpty = 0xffffffffu;
 
-__global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const ViewGPU* views, const DrawGPU* draws0,
-                                               const DrawGPU* draws1, const uint32_t* indices, const float4* xv,
-                                               TriRec* recs, uint2* boxes, uint2* clipq, uint32_t* counters,
+__global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const ViewGPU* views, DrawLists lists,
+                                               const uint32_t* indices, const float4* xv, TriRec* recs, uint2* boxes, uint2* clipq, uint32_t* counters,
                                                uint32_t* bin_counts, uint32_t* pair_slots, uint4* big_pairs,
                                                uint32_t* block_valid, const uint32_t* draw_culled, uint32_t shift) {
     __shared__ uint32_t hkey[kHash];
@@ -303,7 +301,7 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     }
     const BlockEntry e = blocks[blockIdx.x];
     const ViewGPU V = views[e.view];
-    const DrawGPU dr = (V.list ? draws1 : draws0)[e.draw];
+    const DrawGPU dr = lists.of(V.list)[e.draw];
     uint32_t* ctr = counters + e.view * kCtrPerView;
     const uint32_t t = e.start + threadIdx.x;
     if (draw_culled[V.cull_base + e.draw]) {  // the whole draw is outside the view: empty slots only
@@ -393,9 +391,8 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
 constexpr int kClipThreads = 128;
 constexpr int kClipMax = 9;  // a triangle cut by 6 planes has at most 9 vertices
 
-__global__ __launch_bounds__(kClipThreads) void k_clip(const ViewGPU* views, const DrawGPU* draws0,
-                                                       const DrawGPU* draws1, const uint32_t* indices,
-                                                       const float4* xv, const uint2* clipq, TriRec* recs,
+__global__ __launch_bounds__(kClipThreads) void k_clip(const ViewGPU* views, DrawLists lists,
+                                                       const uint32_t* indices, const float4* xv, const uint2* clipq, TriRec* recs,
                                                        uint32_t* counters, uint32_t* bin_counts, uint4* big_pairs,
                                                        uint32_t shift) {
     __shared__ float4 poly[kClipThreads][2][kClipMax];
@@ -412,7 +409,7 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(const ViewGPU* views, con
         uint32_t id = 0;
         if (active) {
             const uint2 ent = clipq[V.clip_base + q];
-            const DrawGPU& dr = (V.list ? draws1 : draws0)[ent.x];
+            const DrawGPU& dr = lists.of(V.list)[ent.x];
             const uint32_t* ip = indices + dr.first_index + 3u * ent.y;
             const float4* vb = xv + size_t(V.xv_base) + dr.xv_off;
             id = dr.tri_base + ent.y;
@@ -711,6 +708,7 @@ __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t 
         uint32_t v = 0;
         while (v + 1 < n_views && views[v + 1].tile_base <= bin) ++v;
         const ViewGPU& W = views[v];
+        if (W.blend) continue;  // k_raster_peel writes every pixel of a blend view
         // view fields copied out: the stores below could alias the view table for the compiler
         const uint32_t width = W.width, height = W.height, local = bin - W.tile_base;
         const uint32_t x0 = (local % W.tiles_x) * T, y0 = (local / W.tiles_x) * T;
@@ -750,6 +748,34 @@ __device__ inline uint32_t record_meta(int32_t X0, int32_t Y0, int32_t X1, int32
     return meta;
 }
 
+// Fragment operations of the bin raster on the LDS tile (rows padded to T + 1 words).  Each takes
+// a covered pixel's tile index, its 31-bit depth bits and the record's key (triangle id + 1).
+template <int T>
+struct ShadowFrag {  // shadow view: largest depth (GEQUAL under reverse-Z)
+    static constexpr int kStride = T + 1;
+    uint32_t* tile;
+    __device__ void operator()(int i, uint32_t zb, uint32_t) const { atomicMax(&tile[i], zb); }
+};
+template <int T>
+struct CameraFrag {  // camera view: depth bits above key, the largest word wins
+    static constexpr int kStride = T + 1;
+    unsigned long long* tile;
+    __device__ void operator()(int i, uint32_t zb, uint32_t key) const {
+        atomicMax(&tile[i], ((unsigned long long)zb << 32) | key);
+    }
+};
+template <int T>
+struct PeelFrag {  // blend view: the smallest key above the pixel's last peeled key among the
+                   // fragments that pass the depth test against the opaque depth
+    static constexpr int kStride = T + 1;
+    uint32_t* keys;
+    const uint32_t* opaque;  // depth bits of the camera view
+    const uint32_t* last;    // last peeled key (0: none yet)
+    __device__ void operator()(int i, uint32_t zb, uint32_t key) const {
+        if (zb >= opaque[i] && key > last[i]) atomicMin(&keys[i], key);
+    }
+};
+
 // One row of a record inside the bin: E_k(x) for the three edges advances by 256 * A_k per pixel
 // (exact integer steps; EdgeT = int32_t unless the record is wide), depth is the FP32 plane rule of
 // setup, and covered pixels merge into the LDS row with atomic max (64-bit depth | id + 1 words in
@@ -757,9 +783,9 @@ __device__ inline uint32_t record_meta(int32_t X0, int32_t Y0, int32_t X1, int32
 // value: F_k = E_k - 1 + tl_k, so "E_k > 0, or E_k == 0 on a top-left edge" is F_k >= 0 for
 // integer E_k, and a pixel is covered when (F_0 | F_1 | F_2) has a clear sign bit -- one test, no
 // branches per edge.
-template <typename EdgeT, bool kShadow, int STRIDE, typename Word>
+template <typename EdgeT, class Op>
 __device__ inline void raster_row(const TriRec& r, uint32_t meta, int ox, int oy, int y, int bx0, int bx1,
-                                  Word* tile) {
+                                  const Op& op) {
     const int32_t X[3] = {r.X0, r.X1, r.X2}, Y[3] = {r.Y0, r.Y1, r.Y2};
     const int32_t Py = (oy + y) * 256 + 128;
     int32_t Px = (ox + bx0) * 256 + 128;
@@ -772,15 +798,13 @@ __device__ inline void raster_row(const TriRec& r, uint32_t meta, int ox, int oy
         step[k] = EdgeT(A) * EdgeT(256);
     }
     const float fy = float(Py - r.Y0);
-    const uint32_t idw = (r.id & kRecIdMask) + 1u;
-    Word* row = tile + y * STRIDE;
+    const uint32_t key = (r.id & kRecIdMask) + 1u;
+    const int row = y * Op::kStride;
     for (int x = bx0; x <= bx1; ++x, Px += 256) {
         if ((f[0] | f[1] | f[2]) >= 0) {
             float z = fmaf(r.dzdy, fy, fmaf(r.dzdx, float(Px - r.X0), r.z0));
             z = fminf(fmaxf(z, 0.0f), 1.0f);
-            const uint32_t zb = fbits(z) & 0x7fffffffu;
-            if constexpr (kShadow) atomicMax(&row[x], zb);
-            else atomicMax(&row[x], ((unsigned long long)zb << 32) | idw);
+            op(row + x, fbits(z) & 0x7fffffffu, key);
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) f[k] += step[k];
@@ -793,12 +817,11 @@ __device__ inline void raster_row(const TriRec& r, uint32_t meta, int ox, int oy
 // record's rows become tasks (a prefix sum numbers them and each record writes its index into the
 // task-owner table), and every thread takes tasks in turn, so small and large triangles spread
 // evenly over the workgroup.
-template <int T, bool kShadow, typename Word>
-__device__ inline void raster_item(Word* tile, TriRec* chunk_recs, uint8_t* owner, uint32_t* row_start,
+template <int T, class Op>
+__device__ inline void raster_item(const Op& op, TriRec* chunk_recs, uint8_t* owner, uint32_t* row_start,
                                    uint32_t* scan_lds, bool have, uint4 w0, uint4 w1, uint4 w2, uint32_t n, int ox,
                                    int oy) {
     static_assert(kItemRecs == 256 && kItemRecs % kChunkRecs == 0, "one record per thread");
-    constexpr int S = T + 1;
     uint32_t my_rows = 0;
     if (have) {
         const int bx0 = max(int(w2.z & 0xffffu), ox) - ox, bx1 = min(int(w2.z >> 16), ox + T - 1) - ox;
@@ -832,9 +855,9 @@ __device__ inline void raster_item(Word* tile, TriRec* chunk_recs, uint8_t* owne
             const int bx0 = int(q.bbx & 0xffu), bx1 = int((q.bbx >> 8) & 0xffu);
             const int y = int((q.bbx >> 16) & 0xffu) + int(task - row_start[k]);
             if (q.bby & kMetaWide)
-                raster_row<int64_t, kShadow, S>(q, q.bby, ox, oy, y, bx0, bx1, tile);
+                raster_row<int64_t>(q, q.bby, ox, oy, y, bx0, bx1, op);
             else
-                raster_row<int32_t, kShadow, S>(q, q.bby, ox, oy, y, bx0, bx1, tile);
+                raster_row<int32_t>(q, q.bby, ox, oy, y, bx0, bx1, op);
         }
         __syncthreads();
     }
@@ -874,6 +897,7 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
     uint32_t view = 0;
     while (view + 1 < n_views && views[view + 1].tile_base <= bin) ++view;
     const ViewGPU V = views[view];
+    if (V.blend) return;  // blend views are resolved by k_raster_peel
     const uint32_t local_bin = bin - V.tile_base;
     const int ox = int(local_bin % V.tiles_x) * T, oy = int(local_bin / V.tiles_x) * T;
     // view fields copied out: the output stores could alias the view table for the compiler
@@ -889,7 +913,7 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
     if (V.is_shadow) {
         uint32_t* t32 = reinterpret_cast<uint32_t*>(tile);
         for (int p = threadIdx.x; p < T * S; p += 256) t32[p] = 0u;
-        raster_item<T, true>(t32, chunk_recs, owner, row_start, scan_lds, have, w0, w1, w2, n, ox, oy);
+        raster_item<T>(ShadowFrag<T>{t32}, chunk_recs, owner, row_start, scan_lds, have, w0, w1, w2, n, ox, oy);
         __syncthreads();  // tile complete (and cleared, for an item without records)
         uint32_t* out = reinterpret_cast<uint32_t*>(out_addr);
         for (int p = threadIdx.x; p < T * T; p += 256) {
@@ -903,7 +927,7 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
         }
     } else {
         for (int p = threadIdx.x; p < T * S; p += 256) tile[p] = 0ull;
-        raster_item<T, false>(tile, chunk_recs, owner, row_start, scan_lds, have, w0, w1, w2, n, ox, oy);
+        raster_item<T>(CameraFrag<T>{tile}, chunk_recs, owner, row_start, scan_lds, have, w0, w1, w2, n, ox, oy);
         __syncthreads();
         unsigned long long* out = reinterpret_cast<unsigned long long*>(out_addr);
         for (int p = threadIdx.x; p < T * T; p += 256) {
@@ -915,6 +939,82 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
                 else if (w != 0ull) atomicMax(out + o, w);
             }
         }
+    }
+}
+
+// Layer peeling of the blended list (the blended pass of Graphics.cpp:5655-5691: draw order, no
+// depth write, depth test GEQUAL against the opaque pass, no culling).  Pass k finds for every
+// pixel of a blend view the blended fragment with the smallest key (triangle id + 1 in the
+// blended list's draw order) above the key peeled in pass k - 1 whose depth is at least the opaque
+// depth; the shading pass then blends it.  Keys grow along the blended list, so the passes visit
+// each pixel's fragments in primitive order -- the order the reference's blending applies them.
+// One workgroup per bin of the blend views, every record of the bin in rounds of kItemRecs; each
+// pass writes every pixel (0: no fragment) and the pixel's last peeled key, and counts the pixels
+// that found one.  A pass after a pass that found nothing exits at once; a fragment found in the
+// last pass flags that the frame may hold more layers than it has passes.
+template <int T>
+__global__ __launch_bounds__(256) void k_raster_peel(const ViewGPU* views, uint32_t n_views, uint32_t first_blend,
+                                                     const uint32_t* bin_counts, const uint32_t* bin_offsets,
+                                                     const uint32_t* lists, uint32_t list_cap, const TriRec* recs,
+                                                     uint64_t rec_cap_total, uint32_t* found, uint32_t pass,
+                                                     uint32_t n_passes, uint32_t* counters) {
+    constexpr int S = T + 1;
+    __shared__ uint32_t keys[T * S], opaque[T * S], last[T * S];
+    __shared__ TriRec chunk_recs[kChunkRecs];
+    __shared__ uint8_t owner[kChunkRecs * T];
+    __shared__ uint32_t row_start[kChunkRecs];
+    __shared__ uint32_t scan_lds[4];
+    __shared__ uint32_t found_lds;
+    if (pass > 0 && found[pass - 1] == 0u) return;
+    const uint32_t bin = views[first_blend].tile_base + blockIdx.x;
+    uint32_t view = first_blend;
+    while (view + 1 < n_views && views[view + 1].tile_base <= bin) ++view;
+    const ViewGPU V = views[view];
+    const uint32_t local_bin = bin - V.tile_base;
+    const int ox = int(local_bin % V.tiles_x) * T, oy = int(local_bin / V.tiles_x) * T;
+    const int width = int(V.width), height = int(V.height);
+    uint32_t* key_out = reinterpret_cast<uint32_t*>(V.out);
+    uint32_t* last_out = key_out + size_t(width) * height;
+    const unsigned long long* vis = reinterpret_cast<const unsigned long long*>(views[V.source].out);
+    for (int p = threadIdx.x; p < T * T; p += 256) {
+        const int x = ox + p % T, y = oy + p / T, i = (p / T) * S + p % T;
+        const bool in = x < width && y < height;
+        const size_t o = size_t(y) * width + x;
+        opaque[i] = in ? uint32_t(vis[o] >> 32) : 0xffffffffu;
+        last[i] = in && pass > 0 ? last_out[o] : 0u;
+        keys[i] = 0xffffffffu;
+    }
+    if (threadIdx.x == 0) found_lds = 0u;
+    const uint32_t off = bin_offsets[bin];
+    const uint32_t end = uint32_t(min(uint64_t(off) + bin_counts[bin], uint64_t(list_cap)));
+    __syncthreads();
+    const PeelFrag<T> op{keys, opaque, last};
+    for (uint32_t base = off; base < end; base += kItemRecs) {
+        const uint32_t n = min(end - base, kItemRecs);
+        const uint32_t ri = threadIdx.x < n ? lists[base + threadIdx.x] : 0xffffffffu;
+        const bool have = ri < rec_cap_total;
+        const uint4* src = reinterpret_cast<const uint4*>(recs + (have ? ri : 0u));
+        const uint4 w0 = src[0], w1 = src[1], w2 = src[2];
+        raster_item<T>(op, chunk_recs, owner, row_start, scan_lds, have, w0, w1, w2, n, ox, oy);
+    }
+    __syncthreads();
+    uint32_t mine = 0;
+    for (int p = threadIdx.x; p < T * T; p += 256) {
+        const int x = ox + p % T, y = oy + p / T, i = (p / T) * S + p % T;
+        if (x < width && y < height) {
+            const size_t o = size_t(y) * width + x;
+            const uint32_t k = keys[i];
+            const bool hit = k != 0xffffffffu;
+            key_out[o] = hit ? k : 0u;
+            last_out[o] = hit ? k : last[i];
+            mine += hit ? 1u : 0u;
+        }
+    }
+    if (mine) atomicAdd(&found_lds, mine);
+    __syncthreads();
+    if (threadIdx.x == 0 && found_lds) {
+        atomicAdd(&found[pass], found_lds);
+        if (pass + 1 == n_passes) atomicOr(&counters[view * kCtrPerView + kCtrOverflow], kOverflowLayers);
     }
 }
 
@@ -930,17 +1030,18 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s) {
 
 hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
     if (g.n_views == 0) return hipSuccess;
+    const DrawLists lists{{g.draws0, g.draws1, g.draws2}};
     const uint32_t nclear = g.n_views * kCtrPerView + g.total_bins;
     hipLaunchKernelGGL(k_clear2, dim3(min((nclear + 255) / 256, 2048u)), dim3(256), 0, s, g.counters,
                        g.n_views * kCtrPerView, g.bin_counts, g.total_bins);
     if (g.n_vblocks)
-        hipLaunchKernelGGL(k_transform, dim3(g.n_vblocks), dim3(256), 0, s, g.vblocks, g.views, g.draws0, g.draws1,
+        hipLaunchKernelGGL(k_transform, dim3(g.n_vblocks), dim3(256), 0, s, g.vblocks, g.views, lists,
                            g.objects, g.px, g.py, g.pz, g.xv, g.draw_culled);
     if (g.n_tblocks)
-        hipLaunchKernelGGL(k_setup, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.draws0, g.draws1,
+        hipLaunchKernelGGL(k_setup, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, lists,
                            g.indices, g.xv, g.recs, g.boxes, g.clipq, g.counters, g.bin_counts, g.pair_slots,
                            g.big_pairs, g.block_valid, g.draw_culled, g.shift);
-    hipLaunchKernelGGL(k_clip, dim3(g.clip_blocks, g.n_views), dim3(kClipThreads), 0, s, g.views, g.draws0, g.draws1,
+    hipLaunchKernelGGL(k_clip, dim3(g.clip_blocks, g.n_views), dim3(kClipThreads), 0, s, g.views, lists,
                        g.indices, g.xv, g.clipq, g.recs, g.counters, g.bin_counts, g.big_pairs, g.shift);
     return hipGetLastError();
 }
@@ -971,6 +1072,24 @@ hipError_t launch_raster(const RasterLaunch& r, hipStream_t s) {
         case 64:
             hipLaunchKernelGGL(k_raster<64>, dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.lists, r.list_cap,
                                r.recs, r.rec_cap_total, r.items, r.totals);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_raster_peel(const PeelLaunch& p, uint32_t pass, hipStream_t s) {
+    if (p.blend_bins == 0) return hipSuccess;
+    switch (p.tile) {
+        case 32:
+            hipLaunchKernelGGL(k_raster_peel<32>, dim3(p.blend_bins), dim3(256), 0, s, p.views, p.n_views,
+                               p.first_blend, p.bin_counts, p.bin_offsets, p.lists, p.list_cap, p.recs,
+                               p.rec_cap_total, p.found, pass, p.n_passes, p.counters);
+            break;
+        case 64:
+            hipLaunchKernelGGL(k_raster_peel<64>, dim3(p.blend_bins), dim3(256), 0, s, p.views, p.n_views,
+                               p.first_blend, p.bin_counts, p.bin_offsets, p.lists, p.list_cap, p.recs,
+                               p.rec_cap_total, p.found, pass, p.n_passes, p.counters);
             break;
         default: return hipErrorInvalidValue;
     }

@@ -204,7 +204,7 @@ struct PixelGeom {
 };
 
 __device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView& V, const uint32_t* tri_base,
-                                           uint32_t vi, uint32_t x, uint32_t y, unsigned long long vis) {
+                                           uint32_t x, uint32_t y, unsigned long long vis) {
     PixelGeom pg;
     const uint32_t tri = uint32_t(vis & 0xffffffffull) - 1u;
     // draw lookup: last draw whose tri_base <= tri
@@ -221,7 +221,7 @@ __device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView
     pg.i0 = ip[0];
     pg.i1 = ip[1];
     pg.i2 = ip[2];
-    const float4* xv = L.xv + L.xv_bases[vi] + dr.xv_off;
+    const float4* xv = L.xv + V.xv_base + dr.xv_off;
     const float w0 = xv[pg.i0].w, w1 = xv[pg.i1].w, w2 = xv[pg.i2].w;
 
     // Perspective-correct barycentrics at the pixel centre.  A triangle that needed no clipping has
@@ -290,11 +290,19 @@ struct WaveRecords {
     }
 };
 
+// Result of fragmentMain for one pixel: the colour, the PCF factor, and whether the fragment is
+// discarded (alpha mask below its cutoff, basic.slang:335-337).
+struct Fragment {
+    float4 color;
+    float shadow;
+    bool discard;
+};
+
 // fragmentMain (basic.slang:137-344) for one covered pixel.  When `use_dec` is set, the material
 // decoder's outputs `dec` replace the albedo, normal and metallic-roughness samples of the plain
 // textures.  kUniformDraw: every live lane of the wave shades the same draw.
 template <bool kUniformDraw>
-__device__ inline void shade_pixel(const ShadeLaunch& L, const ShadeView& V, const float* lut, size_t o,
+__device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V, const float* lut, size_t o,
                                    const PixelGeom& pg, bool use_dec, const float (&dec)[kNmOut]) {
     using R = std::conditional_t<kUniformDraw, WaveRecords, LaneRecords>;
     const rsy_scene_data& sd = *static_cast<const rsy_scene_data*>(L.scene);
@@ -455,9 +463,17 @@ __device__ inline void shade_pixel(const ShadeLaunch& L, const ShadeView& V, con
             out = make_float4(fmaxf(amb.x, sh.x), fmaxf(amb.y, sh.y), fmaxf(amb.z, sh.z), alpha);
         }
     }
-    if (has_material) out.w = alpha;
-    V.color[o] = out;
-    if (V.shadow_factor) V.shadow_factor[o] = shadow;
+    bool discard = false;
+    if (has_material) {
+        discard = md->alpha_mode == 1u && alpha < md->alpha_cutoff;
+        out.w = alpha;
+    }
+    return Fragment{out, shadow, discard};
+}
+
+__device__ inline void store_fragment(const ShadeView& V, size_t o, const Fragment& f) {
+    V.color[o] = f.color;
+    if (V.shadow_factor) V.shadow_factor[o] = f.shadow;
 }
 
 
@@ -495,9 +511,9 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
             return;
         }
         const float none[kNmOut] = {};
-        const PixelGeom pg = pixel_geometry(L, V, tri_base, vi, x, y, vis);
-        if (wave_uniform(pg.draw)) shade_pixel<true>(L, V, lut, o, pg, false, none);
-        else shade_pixel<false>(L, V, lut, o, pg, false, none);
+        const PixelGeom pg = pixel_geometry(L, V, tri_base, x, y, vis);
+        store_fragment(V, o, wave_uniform(pg.draw) ? shade_pixel<true>(L, V, lut, o, pg, false, none)
+                                                   : shade_pixel<false>(L, V, lut, o, pg, false, none));
     } else {
         // Every lane stays live up to the wave-wide decoder MLP; lanes without a decoded material
         // feed zero features and ignore the result.
@@ -506,7 +522,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
         bool decode = false;
         float f[kNmFeat] = {};
         if (vis != 0ull) {
-            pg = pixel_geometry(L, V, tri_base, vi, x, y, vis);
+            pg = pixel_geometry(L, V, tri_base, x, y, vis);
             const DrawGPU dr = L.draws[pg.draw];
             if (dr.material < L.n_materials && L.nm_active[dr.material]) {
                 decode = true;
@@ -528,9 +544,48 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
             if (V.shadow_factor) V.shadow_factor[o] = 1.f;
             return;
         }
-        if (wave_uniform(pg.draw)) shade_pixel<true>(L, V, lut, o, pg, decode, dec);
-        else shade_pixel<false>(L, V, lut, o, pg, decode, dec);
+        store_fragment(V, o, wave_uniform(pg.draw) ? shade_pixel<true>(L, V, lut, o, pg, decode, dec)
+                                                   : shade_pixel<false>(L, V, lut, o, pg, decode, dec));
     }
+}
+
+// One pass of the blended pass: every pixel whose peeling pass found a layer shades that blended
+// fragment with the plain material maps (fragmentMain, basic.slang:137-344) and, unless the alpha
+// mask discards it, blends it over the colour with the reference's blend state (Graphics.cpp:
+// 5657-5664): colour = src.rgb * src.a + dst.rgb * (1 - src.a), alpha = src.a.
+__global__ __launch_bounds__(256) void k_shade_blend(ShadeLaunch L) {
+    __shared__ float lut[256];
+    __shared__ uint32_t tri_base[kLdsDraws];
+    if (L.blend_found[L.blend_pass] == 0u) return;  // no pixel has a layer in this pass
+    lut[threadIdx.x] = L.srgb_lut[threadIdx.x];
+    const uint32_t nd_lds = L.n_draws < kLdsDraws ? L.n_draws : kLdsDraws;
+    for (uint32_t i = threadIdx.x; i < nd_lds; i += 256) tri_base[i] = L.draws[i].tri_base;
+    __syncthreads();
+
+    uint32_t vi = 0;
+    while (vi + 1 < L.n_views && L.views[vi + 1].block_base <= blockIdx.x) ++vi;
+    const ShadeView& V = L.views[vi];
+    const uint32_t lb = blockIdx.x - V.block_base;
+    const uint32_t x = (lb % V.blocks_x) * 16 + (threadIdx.x & 15);
+    const uint32_t y = (lb / V.blocks_x) * 16 + (threadIdx.x >> 4);
+    if (x >= V.width || y >= V.height) return;
+    const size_t o = size_t(y) * V.width + x;
+    const uint32_t key = V.keys[o];
+    if (key == 0u) return;
+    const float none[kNmOut] = {};
+    const PixelGeom pg = pixel_geometry(L, V, tri_base, x, y, (unsigned long long)key);
+    const Fragment f = shade_pixel<false>(L, V, lut, o, pg, false, none);
+    if (f.discard) return;
+    const float4 dst = V.color[o];
+    const float a = f.color.w;
+    V.color[o] = make_float4(f.color.x * a + dst.x * (1.f - a), f.color.y * a + dst.y * (1.f - a),
+                             f.color.z * a + dst.z * (1.f - a), a);
+}
+
+hipError_t launch_shade_blend(const ShadeLaunch& l, hipStream_t s) {
+    if (l.total_blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_shade_blend, dim3(l.total_blocks), dim3(256), 0, s, l);
+    return hipGetLastError();
 }
 
 hipError_t launch_shade(const ShadeLaunch& l, hipStream_t s) {

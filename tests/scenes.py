@@ -23,7 +23,8 @@ def ndc_from_subpixel(X, extent):
 class CustomScene:
     def __init__(self, positions, triangles, width, height, draws=None, view_projection=None,
                  shadow_projection=None, shadow_size=64, normals=None, uvs=None, tangents=None,
-                 textures=None, material=True, objects=None, object_of_draw=None, blended=None):
+                 textures=None, material=True, objects=None, object_of_draw=None, blended=None,
+                 materials=None, material_of_draw=None):
         pos = np.asarray(positions, dtype=np.float32).reshape(-1, 3)
         tri = np.asarray(triangles, dtype=np.uint32).reshape(-1, 3)
         nv = len(pos)
@@ -58,7 +59,7 @@ class CustomScene:
             d.mesh_index = 0
             d.graphic_objects_offset = 0
             d.graphics_object_index = object_of_draw[i] if object_of_draw else 0
-            d.material_index = 0 if material else 0xFFFFFFFF
+            d.material_index = (material_of_draw[i] if material_of_draw else 0) if material else 0xFFFFFFFF
             d.index_count = 3 * count
             d.start_index = 3 * first_tri
             d.blended = 1 if blended and blended[i] else 0
@@ -74,15 +75,23 @@ class CustomScene:
             buf = C.create_string_buffer(bytes(data), len(data))
             self._tex_pixels.append(buf)
             self._tex[i] = abi.TextureDesc(w, h, fmt, 0, C.cast(buf, C.c_void_p))
-        self._mat = (abi.GpuMaterial * 1)()
-        m = self._mat[0]
-        m.color[:] = [1.0, 1.0, 1.0, 1.0]
-        m.metallic_factor = 1.0
-        m.roughness_factor = 1.0
-        m.color_sampled_image_index, m.color_sampler_index = 0, 0
-        m.normal_sampled_image_index, m.normal_sampler_index = 1, 0
-        m.metallic_sampled_image_index, m.metallic_sampler_index = 2, 0
-        m.mixmap_sampled_image_index = m.mixmap_sampler_index = abi.RSY_ABSENT
+        # materials: each a dict of overrides of the default (white, textures 0 / 1 / 2, opaque)
+        mats = materials if materials is not None else [{}]
+        self._mat = (abi.GpuMaterial * len(mats))()
+        for k, over in enumerate(mats):
+            m = self._mat[k]
+            m.color[:] = [1.0, 1.0, 1.0, 1.0]
+            m.metallic_factor = 1.0
+            m.roughness_factor = 1.0
+            m.color_sampled_image_index, m.color_sampler_index = 0, 0
+            m.normal_sampled_image_index, m.normal_sampler_index = 1, 0
+            m.metallic_sampled_image_index, m.metallic_sampler_index = 2, 0
+            m.mixmap_sampled_image_index = m.mixmap_sampler_index = abi.RSY_ABSENT
+            for key, val in over.items():
+                if key == "color":
+                    m.color[:] = [float(x) for x in val]
+                else:
+                    setattr(m, key, val)
         self._smp = (abi.Sampler * 1)()
         self._smp[0] = abi.Sampler(4, 4, 2, 2)
         vp = view_projection if view_projection is not None else IDENTITY
@@ -111,7 +120,7 @@ class CustomScene:
         s.surfaces = C.cast(self._draws, C.POINTER(abi.Draw))
         s.n_surfaces = len(draws)
         s.materials = C.cast(self._mat, C.POINTER(abi.GpuMaterial))
-        s.n_materials = 1
+        s.n_materials = len(mats)
         s.textures = C.cast(self._tex, C.POINTER(abi.TextureDesc))
         s.n_textures = len(texs)
         s.samplers = C.cast(self._smp, C.POINTER(abi.Sampler))

@@ -1,0 +1,60 @@
+"""The blended pass on the GPU against the FP64 oracle (oracle.blend_view): blended surfaces in draw
+order over the opaque frame -- GEQUAL against the opaque depth, no depth write, both faces, alpha
+mask discard, SRC_ALPHA / ONE_MINUS_SRC_ALPHA -- within 1e-3 max abs, at two view sizes.  The scene
+holds up to six layers at a pixel, more than the first frame's peeling passes, so the pass count
+grows on the way; the visibility buffer stays the opaque one, bit-exact, and graph replays give the
+eager frame."""
+import numpy as np
+import pytest
+
+import oracle
+import rosy_amd
+from rosy_amd import abi
+from blend_scene import blend_scene
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-3
+
+
+@pytest.mark.parametrize("size", [64, 200])
+def test_blended_pass_matches_the_oracle(size):
+    s = blend_scene(size)
+    r = rosy_amd.Renderer(device=0)
+    r.load(s)
+    frame = s.frame_desc(abi.SHADE_PLAIN)
+    r.render(frame)
+    vis, col = r.visibility(0), r.color(0).astype(np.float64)
+    st = r.stats()
+    cam = s.camera_views()[0]
+    want_vis, _ = oracle.raster_view(s, cam.view_projection[:], size, size, False, s.s.raster_state)
+    assert np.array_equal(vis, want_vis)
+    maps = [r.shadow_map(0)]
+    base, _, _ = oracle.shade_view(s, cam, s.cascades(), maps, vis)
+    want, count = oracle.blend_view(s, cam, s.cascades(), maps, vis, base)
+    assert count.max() >= 5 and st.overflowed == 0
+    err = float(np.abs(col - want).max())
+    print(f"blended pass {size}x{size}: max abs err {err:.3e}, layers up to {count.max()}")
+    assert err <= TOL
+    # the blended pass changed the pixels that hold a layer and no others
+    changed = np.any(np.abs(col - base) > 1e-6, axis=-1)
+    assert not changed[count == 0].any() and changed[count > 0].mean() > 0.5
+    r.render_graph(frame, 2)
+    assert np.array_equal(r.color(0), col.astype(np.float32))
+    r.close()
+
+
+def test_frames_without_blended_surfaces_are_unchanged():
+    s = blend_scene(64)
+    for i in range(s.s.n_surfaces):
+        s.s.surfaces[i].blended = 0
+    r = rosy_amd.Renderer(device=0)
+    r.load(s)
+    r.render(s.frame_desc(abi.SHADE_PLAIN))
+    vis = r.visibility(0)
+    cam = s.camera_views()[0]
+    want_vis, _ = oracle.raster_view(s, cam.view_projection[:], 64, 64, False, s.s.raster_state)
+    assert np.array_equal(vis, want_vis)
+    base, _, _ = oracle.shade_view(s, cam, s.cascades(), [r.shadow_map(0)], vis)
+    assert float(np.abs(r.color(0) - base).max()) <= TOL
+    r.close()
