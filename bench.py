@@ -86,8 +86,46 @@ def stage_medians(r, frame, n=10):
     st = []
     for _ in range(n):
         r.render(frame)
-        st.append(list(r.stats().stage_ms)[:4])
+        st.append(list(r.stats().stage_ms)[:5])
     return np.median(np.array(st), axis=0)
+
+
+def present_timing(r, width, height, n=10):
+    """Output stage of view 0 to a width x height B8G8R8A8_SRGB image: median device ms of the draw
+    image conversion + blit (HIP events around the two kernels) and its algorithmic bytes (16 B
+    colour read + 8 B draw image write + 8 B draw image read per source pixel, 4 B per destination
+    pixel)."""
+    import numpy as np
+    from rosy_amd import abi
+    r.set_debug_flags(abi.DEBUG_STAGE_TIMING)
+    t = []
+    for _ in range(n):
+        r.present(0, width, height, readback=False)
+        t.append(float(r.stats().stage_ms[5]))
+    r.set_debug_flags(0)
+    w, h = r.views[0]
+    ms = float(np.median(t))
+    nbytes = w * h * 32 + width * height * 4
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"extent": f"{w}x{h} -> {width}x{height}", "ms": round(ms, 5), "algorithmic_bytes": nbytes,
+            "achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def rsy_decode_timing(scene, seconds=2.0):
+    """The workload written as a .rsy file (rosy_amd.asset.from_scene) and decoded by rsy_asset_decode
+    from host memory, repeated for a bounded time: MB of file and host GB/s (one thread)."""
+    from rosy_amd import asset
+    data = asset.encode(asset.from_scene(scene))
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        asset.open_bytes(data).close()
+        n += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    ms = (time.perf_counter() - t0) * 1e3 / n
+    return {"file_mb": round(len(data) / 1e6, 2), "ms": round(ms, 3), "host_gbs": round(len(data) / (ms * 1e-3) / 1e9, 2),
+            "decodes": n}
 
 
 def main():
@@ -99,11 +137,13 @@ def main():
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", type=int, default=3, choices=[1, 2, 3, 4],
+    ap.add_argument("--workload", type=int, default=3, choices=[1, 2, 3, 4, 5],
                     help="3 (default): the decoder workload the metric is quoted on; 1: scene-scale plain; "
-                         "2: 1M triangles, 2560x1440 + 4 cascades; 4: 2M triangles, 4 views + 4 cascades + decoder")
+                         "2: 1M triangles, 2560x1440 + 4 cascades; 4: 2M triangles, 4 views + 4 cascades + decoder; "
+                         "5: the decoder scene with 16 translucent materials (blended pass), plain shading")
     ap.add_argument("--check", action="store_true",
-                    help="workloads 1, 2, 4: compare every view's visibility buffer / shadow map with the oracle")
+                    help="workloads 1, 2, 4, 5: compare every view's visibility buffer / shadow map with the oracle "
+                         "(5: also the blended frame's colour with the FP64 oracle)")
     ap.add_argument("--tile", type=int, default=0, choices=[0, 32, 64],
                     help="bin size in pixels; 0 (default): the autotuner picks it")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -178,6 +218,11 @@ def main():
     wall = (time.perf_counter() - t0) * 1e3 / args.steps
     ms = float(r.stats().gpu_ms)
     log(f"graph frame: neural {ms:.4f} ms, plain {plain_ms:.4f} ms (device); neural wall {wall:.4f} ms")
+
+    # output stage of the neural frame (draw image + sRGB blit) and the .rsy reader on this workload
+    output_stage = [present_timing(r, 1920, 1080), present_timing(r, 2560, 1440)]
+    rsy_decode = rsy_decode_timing(scene)
+    log(f"output stage {output_stage}; .rsy decode {rsy_decode}")
 
     # raster kernel, algorithmic bytes with the implementation-independent unit costs: one tile-list
     # entry read (4 B) per (triangle, bin) pair, one setup record read (32 B: three edge pairs, the
@@ -268,6 +313,8 @@ def main():
         "shading_speedup": round(float(st_plain[3]) / float(st_neural[3]), 3),
         "roofline": roofline,
         "decoder": decoder,
+        "output_stage": output_stage,
+        "rsy_decode": rsy_decode,
         "cpu_baseline": cpu,
         "wall_ms_per_step": round(wall, 5),
         "mtris_per_s": round(2 * scene.n_triangles / (ms * 1e-3) / 1e6, 1),
@@ -291,6 +338,9 @@ WORKLOADS = {
     4: "combined: 2,097,152 triangles (116 meshes), cameras 3840x2160 + 1920x1080 + 1280x720 + 640x360 + 4 cascades "
        "2048^2, 64 materials x 1024^2 maps, neural-material shading (FP8 E4M3 decoder weights, FP32 accumulation) "
        "with plain fallback below 40 dB",
+    5: "blended: the decoder scene (120 meshes, 262,144 triangles, 1920x1080, 2048^2 shadow map, 64 materials x "
+       "1024^2 maps) with materials 0-15 translucent (8 BLEND at alpha 0.7 x a sin-product texture alpha, 8 MASK at "
+       "cutoff 0.5) drawn by the blended pass in primitive order, plain shading",
 }
 
 
@@ -303,7 +353,8 @@ def run_views(args):
     from rosy_amd import abi
 
     t_gen = time.time()
-    scene = rosy_amd.Scene(args.workload, seed=args.seed)
+    scene = (rosy_amd.Scene(3, seed=args.seed, n_translucent=16) if args.workload == 5 else
+             rosy_amd.Scene(args.workload, seed=args.seed))
     cams, cascs = scene.camera_views(), scene.cascades()
     log(f"workload {args.workload}: {scene.n_triangles} triangles, {len(cams)} camera views, {len(cascs)} cascades "
         f"(generated in {time.time() - t_gen:.1f}s)")
@@ -325,6 +376,7 @@ def run_views(args):
     st = stage_medians(r, frame)
     fst = r.stats()
     exact = None
+    blend_check = None
     if args.check:
         import oracle
         rs = scene.s.raster_state
@@ -336,6 +388,14 @@ def run_views(args):
             o, _ = oracle.raster_view(scene, cv.view_projection[:], cv.width, cv.height, False, rs)
             exact &= bool(np.array_equal(r.visibility(vi), o))
         log(f"visibility + shadow maps bit-exact vs oracle: {exact}")
+        if args.workload == 5:
+            vis = r.visibility(0)
+            maps = [r.shadow_map(0)]
+            base, _, _ = oracle.shade_view(scene, cams[0], cascs, maps, vis)
+            want, count = oracle.blend_view(scene, cams[0], cascs, maps, vis, base, max_layers=64)
+            blend_check = {"max_abs_err_vs_fp64": float(np.abs(r.color(0).astype(np.float64) - want).max()),
+                           "pixels_with_layers": int((count > 0).sum()), "max_layers": int(count.max())}
+            log(f"blended frame vs FP64 oracle: {blend_check}")
     r.set_debug_flags(0)
     r.render_graph(plain, args.warmup)
     r.render_graph(plain, args.steps)
@@ -352,9 +412,11 @@ def run_views(args):
            "mtris_per_s": round(views * scene.n_triangles / (ms * 1e-3) / 1e6, 1),
            "pixels_per_frame": int(sum(c.width * c.height for c in cams) + sum(c.size * c.size for c in cascs)),
            "stage_ms": {"geometry": round(float(st[0]), 5), "binning": round(float(st[1]), 5),
-                        "raster": round(float(st[2]), 5), "shade": round(float(st[3]), 5)},
+                        "raster": round(float(st[2]), 5), "shade": round(float(st[3]), 5),
+                        "blended_pass": round(float(st[4]), 5)},
            "tile_pairs": int(fst.tile_pairs), "triangles_setup": int(fst.triangles_setup),
-           "triangles_clipped": int(fst.triangles_clipped), "decoder": dec, "bit_exact_vs_oracle": exact}
+           "triangles_clipped": int(fst.triangles_clipped), "decoder": dec, "bit_exact_vs_oracle": exact,
+           "blended": blend_check}
     print(json.dumps(out), flush=True)
     r.close()
     scene.close()

@@ -235,7 +235,9 @@ typedef struct rsy_frame_stats {
     float gpu_ms;               /* device time of the last frame (HIP events) */
     float stage_ms[8];          /* with RSY_DEBUG_STAGE_TIMING (HIP events on the context stream):
                                    [0] geometry (mvp, transform, setup, clip), [1] binning (scan,
-                                   scatter), [2] tile raster kernel, [3] shading kernel */
+                                   scatter), [2] tile raster kernel, [3] shading kernel, [4] blended
+                                   pass (peel + blend kernels), [5] the last rsy_present (draw image
+                                   + blit) */
 } rsy_frame_stats;
 
 typedef struct rsy_ctx rsy_ctx;
@@ -575,7 +577,8 @@ typedef struct rsy_scene_desc {
     uint32_t n_meshes;
     uint32_t n_materials;
     uint32_t texture_size;
-    uint32_t reserved2;
+    uint32_t n_translucent;   /* materials [0, n) translucent (default 0): even BLEND, odd MASK, with a
+                                 sin(8 pi u) sin(8 pi v) albedo alpha; their surfaces go to the blended list */
 } rsy_scene_desc;
 
 typedef struct rsy_scene {

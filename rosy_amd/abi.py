@@ -127,7 +127,7 @@ class SceneDesc(C.Structure):
     _fields_ = [("workload", C.c_uint32), ("reserved", C.c_uint32), ("seed", C.c_uint64), ("width", C.c_uint32),
                 ("height", C.c_uint32), ("shadow_size", C.c_uint32), ("total_triangles", C.c_uint32),
                 ("n_meshes", C.c_uint32), ("n_materials", C.c_uint32), ("texture_size", C.c_uint32),
-                ("reserved2", C.c_uint32)]
+                ("n_translucent", C.c_uint32)]
 
 
 class Scene(C.Structure):

@@ -478,6 +478,7 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
         if (L.shade == RSY_SHADE_NEURAL) neural_shade_args(c, sh);
         if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[3], s), "event");
         HIP_TRY(c, launch_shade(sh, s), "shade kernel");
+        if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[5], s), "event");
         if (L.n_blend) {
             // the blended pass: peel the blended list layer by layer in primitive order and blend
             // each layer over the shaded colour
@@ -514,6 +515,7 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
         }
     } else if (timing) {
         HIP_TRY(c, hipEventRecord(c->ev_stage[3], s), "event");
+        HIP_TRY(c, hipEventRecord(c->ev_stage[5], s), "event");
     }
     if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[4], s), "event");
     if (!capturing) {
@@ -564,7 +566,8 @@ bool collect_stats(rsy_ctx* c) {
         if (hipEventElapsedTime(&t, c->ev_begin, c->ev_stage[0]) == hipSuccess) st.stage_ms[0] = t;
         if (hipEventElapsedTime(&t, c->ev_stage[0], c->ev_stage[1]) == hipSuccess) st.stage_ms[1] = t;
         if (hipEventElapsedTime(&t, c->ev_stage[1], c->ev_stage[2]) == hipSuccess) st.stage_ms[2] = t;
-        if (hipEventElapsedTime(&t, c->ev_stage[3], c->ev_stage[4]) == hipSuccess) st.stage_ms[3] = t;
+        if (hipEventElapsedTime(&t, c->ev_stage[3], c->ev_stage[5]) == hipSuccess) st.stage_ms[3] = t;
+        if (hipEventElapsedTime(&t, c->ev_stage[5], c->ev_stage[4]) == hipSuccess) st.stage_ms[4] = t;
     }
     c->stats = st;
     return overflow;
@@ -988,8 +991,12 @@ rsy_result rsy_present(rsy_ctx* c, uint32_t view, uint32_t width, uint32_t heigh
     const ViewGPU* g = camera_vg(c, view);
     HIP_TRY(c, c->draw_image.ensure(size_t(g->width) * g->height * 8), "allocate draw image");
     HIP_TRY(c, c->present_image.ensure(size_t(width) * height * 4), "allocate present image");
+    const bool timing = (c->debug_flags & RSY_DEBUG_STAGE_TIMING) != 0;
+    if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[6], c->stream), "event");
     HIP_TRY(c, launch_present(c->color_out[idx].as<float4>(), c->draw_image.as<uint2>(), g->width, g->height,
                               c->present_image.as<uint32_t>(), width, height, c->stream), "present kernels");
+    if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[7], c->stream), "event");
+    c->present_timed = timing;
     c->draw_w = g->width;
     c->draw_h = g->height;
     c->present_w = width;
@@ -1043,6 +1050,11 @@ rsy_result rsy_get_frame_stats(rsy_ctx* c, rsy_frame_stats* out) {
     if (!c || !out) return RSY_INVALID_ARGUMENT;
     rsy_result r = rsy_synchronize(c);
     if (r != RSY_OK) return r;
+    if (c->present_timed) {
+        float t = 0.f;
+        HIP_TRY(c, hipEventSynchronize(c->ev_stage[7]), "synchronize");
+        if (hipEventElapsedTime(&t, c->ev_stage[6], c->ev_stage[7]) == hipSuccess) c->stats.stage_ms[5] = t;
+    }
     *out = c->stats;
     return RSY_OK;
 }

@@ -136,7 +136,8 @@ struct rsy_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-    hipEvent_t ev_stage[5] = {};
+    hipEvent_t ev_stage[8] = {};  // [5] end of shading, [6]/[7] around rsy_present
+    bool present_timed = false;
     std::string err;
     uint32_t debug_flags = 0;
 

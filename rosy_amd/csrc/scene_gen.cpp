@@ -313,7 +313,9 @@ struct Blob {
     double u, v, r, strength;
 };
 
-void make_material_textures(MaterialTextures& out, int size, Rng rng, bool stripes_and_blobs) {
+// `cutout`: the albedo alpha is 0.5 + 0.5 sin(8 pi u) sin(8 pi v) (translucent materials: holes
+// under an alpha mask, a varying alpha under blending); otherwise opaque (255).
+void make_material_textures(MaterialTextures& out, int size, Rng rng, bool stripes_and_blobs, bool cutout) {
     const double hue = rng.uniform(), sat = rng.range(0.2, 0.8);
     const bool metal = rng.uniform() < 0.35;
     const int base = std::max(2, size / 128);
@@ -367,7 +369,8 @@ void make_material_textures(MaterialTextures& out, int size, Rng rng, bool strip
                     const double lin = base_rgb[c] * lum * (1.0 + 0.4 * tint * (c == 0 ? 1 : c == 1 ? -0.5 : -1));
                     out.albedo[o * 4 + c] = to_byte(srgb_encode(lin));
                 }
-                out.albedo[o * 4 + 3] = 255;
+                out.albedo[o * 4 + 3] =
+                    cutout ? to_byte(0.5 + 0.5 * std::sin(8 * kPi * u) * std::sin(8 * kPi * v)) : uint8_t(255);
                 const double h0 = height_n.at(u, v), hx = height_n.at(u + eps, v), hy = height_n.at(u, v + eps);
                 const double nx = std::max(-0.6, std::min(0.6, (h0 - hx) * size * 0.012));
                 const double ny = std::max(-0.6, std::min(0.6, (h0 - hy) * size * 0.012));
@@ -678,7 +681,8 @@ rsy_result rsy_scene_generate(const rsy_scene_desc* desc, rsy_scene** out_scene)
         s->tex_store.resize(nm);
         s->v_samplers.push_back(rsy_sampler{4, 4, 2, 2});  // LINEAR / REPEAT
         for (uint32_t m = 0; m < nm; ++m) {
-            make_material_textures(s->tex_store[m], int(wl.tex_size), tex_rng.fork(m + 1), wl.stripes);
+            const bool translucent = m < desc->n_translucent;
+            make_material_textures(s->tex_store[m], int(wl.tex_size), tex_rng.fork(m + 1), wl.stripes, translucent);
             const uint32_t base = uint32_t(s->v_textures.size());
             s->v_textures.push_back({wl.tex_size, wl.tex_size, RSY_TEX_RGBA8_SRGB, 0, s->tex_store[m].albedo.data()});
             s->v_textures.push_back({wl.tex_size, wl.tex_size, RSY_TEX_RG8_UNORM, 0, s->tex_store[m].normal.data()});
@@ -689,6 +693,12 @@ rsy_result rsy_scene_generate(const rsy_scene_desc* desc, rsy_scene** out_scene)
             g.roughness_factor = 1.f;
             g.alpha_cutoff = 0.f;
             g.alpha_mode = 0;
+            if (translucent) {
+                // even: BLEND with a 0.7 alpha factor; odd: MASK at cutoff 0.5 (Level.cpp's alpha modes)
+                g.alpha_mode = m % 2 == 0 ? 2 : 1;
+                g.color[3] = m % 2 == 0 ? 0.7f : 1.f;
+                g.alpha_cutoff = m % 2 == 0 ? 0.f : 0.5f;
+            }
             g.color_sampled_image_index = base;
             g.color_sampler_index = 0;
             g.normal_sampled_image_index = base + 1;

@@ -6,7 +6,7 @@ import pytest
 
 import rosy_amd
 from rosy_amd import asset
-from asset_scene import scene_asset
+from rosy_amd.asset import from_scene as scene_asset
 
 
 @pytest.mark.parametrize("workload", [0, 1])

@@ -10,7 +10,7 @@ import pytest
 import oracle
 import rosy_amd
 from rosy_amd import abi, asset
-from asset_scene import scene_asset
+from rosy_amd.asset import from_scene as scene_asset
 
 pytestmark = pytest.mark.gpu
 

@@ -58,3 +58,32 @@ def test_frames_without_blended_surfaces_are_unchanged():
     base, _, _ = oracle.shade_view(s, cam, s.cascades(), [r.shadow_map(0)], vis)
     assert float(np.abs(r.color(0) - base).max()) <= TOL
     r.close()
+
+
+def test_generated_workload_with_translucent_materials():
+    """The decoder workload's generator with 16 translucent materials (BLEND and MASK over a varying
+    texture alpha) at 320x180: opaque visibility and shadow map bit-exact, the blended frame within
+    the tolerance of the FP64 oracle, in eager frames and graph replays."""
+    s = rosy_amd.Scene(3, seed=3, width=320, height=180, shadow_size=512, total_triangles=16384, n_meshes=30,
+                       texture_size=64, n_translucent=16)
+    r = rosy_amd.Renderer(device=0)
+    r.load(s)
+    frame = s.frame_desc(abi.SHADE_PLAIN)
+    r.render(frame)
+    cam, cas = s.camera_views()[0], s.cascades()
+    rs = s.s.raster_state
+    vis = r.visibility(0)
+    want_vis, _ = oracle.raster_view(s, cam.view_projection[:], cam.width, cam.height, False, rs)
+    want_sm, _ = oracle.raster_view(s, cas[0].light_view_projection[:], cas[0].size, cas[0].size, True, rs)
+    assert np.array_equal(vis, want_vis)
+    assert np.array_equal(r.shadow_map(0).view(np.uint32), want_sm.view(np.uint32))
+    base, _, _ = oracle.shade_view(s, cam, cas, [want_sm], vis)
+    want, count = oracle.blend_view(s, cam, cas, [want_sm], vis, base, max_layers=64)
+    col = r.color(0)
+    err = float(np.abs(col.astype(np.float64) - want).max())
+    print(f"generated blended frame: max abs err {err:.3e}, layers up to {count.max()}, "
+          f"{(count > 0).mean():.3f} of pixels")
+    assert count.max() >= 4 and err <= TOL
+    r.render_graph(frame, 3)
+    assert np.array_equal(r.color(0), col)
+    r.close()

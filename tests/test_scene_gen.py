@@ -1,5 +1,7 @@
 """The seeded workload generator: exact triangle totals, closed meshes, outward winding,
 determinism and the layouts the C ABI expects."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -77,3 +79,30 @@ def test_scene_data_defaults_follow_level_cpp():
     rs = sc.s.raster_state
     assert rs.cull_enabled == 1 and rs.depth_bias_enabled == 1
     assert abs(rs.depth_bias_constant + 21.882) < 1e-5 and abs(rs.depth_bias_slope_factor + 3) < 1e-6
+
+
+def test_translucent_materials_and_their_alpha():
+    kw = dict(width=160, height=90, shadow_size=256, total_triangles=8192, n_meshes=24, texture_size=32)
+    sc = rosy_amd.Scene(3, seed=4, n_translucent=6, **kw)
+    opaque = rosy_amd.Scene(3, seed=4, **kw)
+    mats = [sc.s.materials[i] for i in range(sc.s.n_materials)]
+    assert [m.alpha_mode for m in mats[:8]] == [2, 1, 2, 1, 2, 1, 0, 0]
+    assert [m.color[3] for m in mats[:3]] == pytest.approx([0.7, 1.0, 0.7])
+    assert [m.alpha_cutoff for m in mats[:3]] == pytest.approx([0.0, 0.5, 0.0])
+    # surfaces of translucent materials go to the blended list; geometry is unchanged
+    assert all(d.blended == (d.material_index < 6) for d in sc.surfaces())
+    assert np.array_equal(sc.indices, opaque.indices) and sc.vertices.tobytes() == opaque.vertices.tobytes()
+    def albedo(scene, m):
+        t = scene.s.textures[mats[m].color_sampled_image_index]
+        assert (t.width, t.height, t.format) == (32, 32, abi.TEX_RGBA8_SRGB)
+        return np.ctypeslib.as_array(C.cast(t.pixels, C.POINTER(C.c_uint8)), shape=(32, 32, 4)).copy()
+
+    for m in (0, 1, 6):
+        a = albedo(sc, m)
+        assert np.array_equal(a[..., :3], albedo(opaque, m)[..., :3])
+        if m == 6:
+            assert (a[..., 3] == 255).all()
+        else:
+            u = (np.arange(32) + 0.5) / 32
+            want = np.round(np.clip(0.5 + 0.5 * np.outer(np.sin(8 * np.pi * u), np.sin(8 * np.pi * u)), 0, 1) * 255)
+            assert np.array_equal(a[..., 3], want.astype(np.uint8))
