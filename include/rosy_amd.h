@@ -236,8 +236,7 @@ typedef struct rsy_frame_stats {
     float stage_ms[8];          /* with RSY_DEBUG_STAGE_TIMING (HIP events on the context stream):
                                    [0] geometry (mvp, transform, setup, clip), [1] binning (scan,
                                    scatter), [2] tile raster kernel, [3] shading kernel, [4] blended
-                                   pass (peel + blend kernels), [5] the last rsy_present (draw image
-                                   + blit) */
+                                   pass, [5] the last rsy_present (draw image + blit) */
 } rsy_frame_stats;
 
 typedef struct rsy_ctx rsy_ctx;

@@ -1,0 +1,188 @@
+// Bin-raster building blocks shared by the raster kernels (raster.hip) and the fused blended pass
+// (shade.hip): the workgroup scan, the per-record raster flags, the fragment operations on an LDS
+// tile, one row of a record, and one work item's records through the tile.  Device code only.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "frame_types.h"
+
+namespace rsy {
+
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+
+__device__ inline uint32_t fbits(float f) { return __float_as_uint(f); }
+
+constexpr uint32_t kItemRecs = 256;   // records per raster work item (one per thread)
+constexpr uint32_t kChunkRecs = 128;  // records staged in LDS at a time
+
+// Exclusive prefix sum over the workgroup (wave scans with shuffles, wave totals through LDS).
+// Every thread of the block must call it; it synchronises the block.
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds_waves, uint32_t& total) {
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (int(lane_id()) >= off) x += y;
+    }
+    const uint32_t wave = threadIdx.x / 64, nwaves = blockDim.x / 64;
+    if (lane_id() == 63) lds_waves[wave] = x;
+    __syncthreads();
+    uint32_t wave_off = 0, sum = 0;
+    for (uint32_t w = 0; w < nwaves; ++w) {
+        const uint32_t s = lds_waves[w];
+        if (w < wave) wave_off += s;
+        sum += s;
+    }
+    __syncthreads();
+    total = sum;
+    return wave_off + x - v;
+}
+
+// Per-record raster flags, computed once when a record is staged for a bin: bit k (k < 3) is set when
+// edge k is not a top-left edge (its value must exceed 0: F_k = E_k - 1), bit 3 when the edge
+// values need 64 bits.
+constexpr uint32_t kMetaWide = 8u;
+
+__device__ inline uint32_t record_meta(int32_t X0, int32_t Y0, int32_t X1, int32_t Y1, int32_t X2, int32_t Y2) {
+    const int32_t X[3] = {X0, X1, X2}, Y[3] = {Y0, Y1, Y2};
+    uint32_t meta = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int f = k == 2 ? 0 : k + 1;
+        const int32_t A = Y[f] - Y[k], B = X[k] - X[f];
+        const bool tl = A > 0 || (A == 0 && B > 0);
+        meta |= tl ? 0u : (1u << k);
+    }
+    // the whole triangle's box (sub-pixel units) bounds every |E| over it: under 64 pixels on each
+    // side every |E| < 2^30
+    const int32_t wx = max(X[0], max(X[1], X[2])) - min(X[0], min(X[1], X[2]));
+    const int32_t wy = max(Y[0], max(Y[1], Y[2])) - min(Y[0], min(Y[1], Y[2]));
+    if (wx >= 64 * 256 || wy >= 64 * 256) meta |= kMetaWide;
+    return meta;
+}
+
+// Fragment operations of the bin raster on the LDS tile (rows padded to T + 1 words).  Each takes
+// a covered pixel's tile index, its 31-bit depth bits and the record's key (triangle id + 1).
+template <int T>
+struct ShadowFrag {  // shadow view: largest depth (GEQUAL under reverse-Z)
+    static constexpr int kStride = T + 1;
+    uint32_t* tile;
+    __device__ void operator()(int i, uint32_t zb, uint32_t) const { atomicMax(&tile[i], zb); }
+};
+template <int T>
+struct CameraFrag {  // camera view: depth bits above key, the largest word wins
+    static constexpr int kStride = T + 1;
+    unsigned long long* tile;
+    __device__ void operator()(int i, uint32_t zb, uint32_t key) const {
+        atomicMax(&tile[i], ((unsigned long long)zb << 32) | key);
+    }
+};
+template <int T>
+struct CountFrag {  // blend view: count the fragments that pass the depth test against the opaque depth
+    static constexpr int kStride = T + 1;
+    uint32_t* count;
+    const uint32_t* opaque;  // depth bits of the camera view
+    __device__ void operator()(int i, uint32_t zb, uint32_t) const {
+        if (zb >= opaque[i]) atomicAdd(&count[i], 1u);
+    }
+};
+template <int T>
+struct FillFrag {  // blend view: append the key of each passing fragment to its pixel's list
+    static constexpr int kStride = T + 1;
+    uint32_t* cursor;        // next free entry of the pixel's list
+    const uint32_t* opaque;
+    uint32_t* keys;          // the bin's lists (LDS or global memory)
+    __device__ void operator()(int i, uint32_t zb, uint32_t key) const {
+        if (zb >= opaque[i]) keys[atomicAdd(&cursor[i], 1u)] = key;
+    }
+};
+
+// One row of a record inside the bin: E_k(x) for the three edges advances by 256 * A_k per pixel
+// (exact integer steps; EdgeT = int32_t unless the record is wide), depth is the FP32 plane rule of
+// setup, and covered pixels merge into the LDS row with atomic max (64-bit depth | id + 1 words in
+// a camera view, 32-bit depth words in a shadow view).  The top-left rule is folded into the edge
+// value: F_k = E_k - 1 + tl_k, so "E_k > 0, or E_k == 0 on a top-left edge" is F_k >= 0 for
+// integer E_k, and a pixel is covered when (F_0 | F_1 | F_2) has a clear sign bit -- one test, no
+// branches per edge.
+template <typename EdgeT, class Op>
+__device__ inline void raster_row(const TriRec& r, uint32_t meta, int ox, int oy, int y, int bx0, int bx1,
+                                  const Op& op) {
+    const int32_t X[3] = {r.X0, r.X1, r.X2}, Y[3] = {r.Y0, r.Y1, r.Y2};
+    const int32_t Py = (oy + y) * 256 + 128;
+    int32_t Px = (ox + bx0) * 256 + 128;
+    EdgeT f[3], step[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int n = k == 2 ? 0 : k + 1;
+        const int32_t A = Y[n] - Y[k], B = X[k] - X[n];
+        f[k] = EdgeT(A) * EdgeT(Px - X[k]) + EdgeT(B) * EdgeT(Py - Y[k]) - EdgeT((meta >> k) & 1u);
+        step[k] = EdgeT(A) * EdgeT(256);
+    }
+    const float fy = float(Py - r.Y0);
+    const uint32_t key = (r.id & kRecIdMask) + 1u;
+    const int row = y * Op::kStride;
+    for (int x = bx0; x <= bx1; ++x, Px += 256) {
+        if ((f[0] | f[1] | f[2]) >= 0) {
+            float z = fmaf(r.dzdy, fy, fmaf(r.dzdx, float(Px - r.X0), r.z0));
+            z = fminf(fmaxf(z, 0.0f), 1.0f);
+            op(row + x, fbits(z) & 0x7fffffffu, key);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) f[k] += step[k];
+    }
+}
+
+// The records of one work item through the bin's LDS tile.  Every thread loads one record of the
+// item (its box clipped to the bin, its raster flags computed) in one round of loads; the records
+// then go through LDS kChunkRecs at a time, each chunk staged by its own threads.  A staged
+// record's rows become tasks (a prefix sum numbers them and each record writes its index into the
+// task-owner table), and every thread takes tasks in turn, so small and large triangles spread
+// evenly over the workgroup.
+template <int T, class Op>
+__device__ inline void raster_item(const Op& op, TriRec* chunk_recs, uint8_t* owner, uint32_t* row_start,
+                                   uint32_t* scan_lds, bool have, uint4 w0, uint4 w1, uint4 w2, uint32_t n, int ox,
+                                   int oy) {
+    static_assert(kItemRecs == 256 && kItemRecs % kChunkRecs == 0, "one record per thread");
+    uint32_t my_rows = 0;
+    if (have) {
+        const int bx0 = max(int(w2.z & 0xffffu), ox) - ox, bx1 = min(int(w2.z >> 16), ox + T - 1) - ox;
+        const int by0 = max(int(w2.w & 0xffffu), oy) - oy, by1 = min(int(w2.w >> 16), oy + T - 1) - oy;
+        if (bx0 <= bx1 && by0 <= by1) {
+            my_rows = uint32_t(by1 - by0 + 1);
+            // the staged copy keeps the clipped box in bbx and the raster flags in bby
+            w2.z = uint32_t(bx0) | (uint32_t(bx1) << 8) | (uint32_t(by0) << 16);
+            w2.w = record_meta(int32_t(w0.x), int32_t(w0.y), int32_t(w0.z), int32_t(w0.w), int32_t(w1.x),
+                               int32_t(w1.y));
+        }
+    }
+    const uint32_t slot = threadIdx.x % kChunkRecs;
+    for (uint32_t c = 0; c * kChunkRecs < n; ++c) {
+        const bool stager = threadIdx.x / kChunkRecs == c;
+        const uint32_t rows = stager ? my_rows : 0u;
+        if (rows) {
+            uint4* dst = reinterpret_cast<uint4*>(chunk_recs + slot);
+            dst[0] = w0;
+            dst[1] = w1;
+            dst[2] = w2;
+        }
+        uint32_t total_rows;
+        const uint32_t first = block_exclusive_scan(rows, scan_lds, total_rows);  // synchronises
+        if (stager) row_start[slot] = first;
+        for (uint32_t j = 0; j < rows; ++j) owner[first + j] = uint8_t(slot);
+        __syncthreads();
+        for (uint32_t task = threadIdx.x; task < total_rows; task += 256) {
+            const uint32_t k = owner[task];
+            const TriRec q = chunk_recs[k];
+            const int bx0 = int(q.bbx & 0xffu), bx1 = int((q.bbx >> 8) & 0xffu);
+            const int y = int((q.bbx >> 16) & 0xffu) + int(task - row_start[k]);
+            if (q.bby & kMetaWide)
+                raster_row<int64_t>(q, q.bby, ox, oy, y, bx0, bx1, op);
+            else
+                raster_row<int32_t>(q, q.bby, ox, oy, y, bx0, bx1, op);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace rsy

@@ -164,7 +164,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     uint32_t cull_total = 0;
     const uint32_t shift = ilog2(tile);
     // the requested views, then one blend view per camera view when the blended list is drawn
-    // (layer peeling of the blended list against that camera view's depth)
+    // (the blended list drawn over that camera view)
     std::vector<uint32_t> order(views.size());
     for (uint32_t vi = 0; vi < views.size(); ++vi) order[vi] = vi;
     const bool blend = shade != RSY_SHADE_NONE && !c->lists[2].empty();
@@ -265,13 +265,15 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     c->color_out.resize(L.n_camera);
     c->shadow_factor_out.resize(L.n_camera);
     c->shadow_maps.resize(L.n_shadow);
-    c->blend_layers.resize(L.n_blend);
+    c->blend_lists.resize(L.n_blend);
     uint32_t ci = 0, si = 0, bi = 0;
+    uint64_t blend_px = 0;
     for (auto& g : L.vg) {
         const size_t px = size_t(g.width) * g.height;
         if (g.blend) {
-            HIP_TRY(c, c->blend_layers[bi].ensure(px * 8), "allocate blend layer keys");
-            g.out = reinterpret_cast<uint64_t>(c->blend_layers[bi].p);
+            HIP_TRY(c, c->blend_lists[bi].ensure(px * 8), "allocate blended fragment lists");
+            g.out = reinterpret_cast<uint64_t>(c->blend_lists[bi].p);
+            blend_px += px;
             ++bi;
         } else if (g.is_shadow) {
             HIP_TRY(c, c->shadow_maps[si].ensure(px * 4), "allocate shadow map");
@@ -284,6 +286,14 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
             g.out = reinterpret_cast<uint64_t>(c->vis_out[ci].p);
             ++ci;
         }
+    }
+    if (L.n_blend) {
+        // blended fragments: start at two per pixel of the camera views (grows on overflow)
+        const uint64_t want = std::min<uint64_t>(std::max<uint64_t>(c->frag_cap, 2 * blend_px + 4096), 0xfffffff0ull);
+        HIP_TRY(c, c->frags.ensure(size_t(want) * 16), "allocate blended fragments");
+        HIP_TRY(c, c->frag_keys.ensure(size_t(want) * 4), "allocate blended fragment keys");
+        HIP_TRY(c, c->frag_color.ensure(size_t(want) * 16), "allocate blended fragment colours");
+        c->frag_cap = uint32_t(want);
     }
     HIP_TRY(c, c->readback_counters.ensure(nv * kCtrPerView * 4 + 64), "allocate readback");
     L.valid = true;
@@ -305,11 +315,10 @@ rsy_result upload_frame(rsy_ctx* c) {
     for (uint32_t i = 0; i < nv; ++i) {
         const ViewGPU& vg = L.vg[i];
         if (vg.blend) {
-            // the blended pass of camera view vg.source: its keys, records and transformed vertices,
-            // shaded into that view's colour buffer
+            // the blended pass of camera view vg.source: its records and transformed vertices, shaded
+            // into that view's colour buffer
             ShadeView sv = c->h_shade_views[camera_of[vg.source]];
             sv.vis = nullptr;
-            sv.keys = reinterpret_cast<const uint32_t*>(vg.out);
             sv.recs = c->recs.as<TriRec>() + vg.rec_base;
             sv.boxes = c->boxes.as<uint2>() + vg.rec_base;
             sv.xv_base = vg.xv_base;
@@ -330,7 +339,6 @@ rsy_result upload_frame(rsy_ctx* c) {
             sv.block_base = blocks;
             blocks += sv.blocks_x * ((vg.height + 15) / 16);
             sv.vis = reinterpret_cast<const unsigned long long*>(vg.out);
-            sv.keys = nullptr;
             sv.recs = c->recs.as<TriRec>() + vg.rec_base;
             sv.boxes = c->boxes.as<uint2>() + vg.rec_base;
             sv.xv_base = vg.xv_base;
@@ -476,42 +484,49 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
         sh.srgb_lut = c->srgb_lut.as<float>();
         sh.xv = g.xv;
         if (L.shade == RSY_SHADE_NEURAL) neural_shade_args(c, sh);
-        if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[3], s), "event");
-        HIP_TRY(c, launch_shade(sh, s), "shade kernel");
-        if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[5], s), "event");
+        // the blended pass: the blended list of every camera view in primitive order
+        BlendLaunch bl{};
+        ShadeLaunch sb = sh;
         if (L.n_blend) {
-            // the blended pass: peel the blended list layer by layer in primitive order and blend
-            // each layer over the shaded colour
-            const uint32_t passes = c->blend_passes;
-            HIP_TRY(c, c->blend_found.ensure(size_t(passes) * 4 + 64), "allocate blend counters");
-            HIP_TRY(c, launch_fill_u32(c->blend_found.as<uint32_t>(), passes, 0u, s), "clear blend counters");
-            PeelLaunch pl{};
-            pl.views = g.views;
-            pl.n_views = nv;
-            pl.first_blend = L.first_blend;
-            pl.blend_bins = L.blend_bins;
-            pl.tile = L.tile;
-            pl.bin_counts = g.bin_counts;
-            pl.bin_offsets = b.bin_offsets;
-            pl.lists = b.lists;
-            pl.list_cap = c->list_cap;
-            pl.recs = g.recs;
-            pl.rec_cap_total = L.rec_total;
-            pl.found = c->blend_found.as<uint32_t>();
-            pl.n_passes = passes;
-            pl.counters = g.counters;
-            ShadeLaunch sb = sh;
+            bl.views = g.views;
+            bl.n_views = nv;
+            bl.first_blend = L.first_blend;
+            bl.blend_bins = L.blend_bins;
+            bl.tile = L.tile;
+            bl.bin_counts = g.bin_counts;
+            bl.bin_offsets = b.bin_offsets;
+            bl.lists = b.lists;
+            bl.list_cap = c->list_cap;
+            bl.recs = g.recs;
+            bl.rec_cap_total = L.rec_total;
+            bl.counters = g.counters;
+            bl.frags = c->frags.as<uint4>();
+            bl.frag_keys = c->frag_keys.as<uint32_t>();
+            bl.frag_color = c->frag_color.as<float4>();
+            bl.frag_cap = c->frag_cap;
             sb.views = c->blend_shade_views_d.as<ShadeView>();
             sb.n_views = uint32_t(c->h_blend_views.size());
             sb.draws = c->lists_d[2].as<DrawGPU>();
             sb.n_draws = uint32_t(c->lists[2].size());
             sb.neural = 0;
-            sb.blend_found = pl.found;
-            for (uint32_t pass = 0; pass < passes; ++pass) {
-                HIP_TRY(c, launch_raster_peel(pl, pass, s), "peel kernel");
-                sb.blend_pass = pass;
-                HIP_TRY(c, launch_shade_blend(sb, s), "blend kernel");
-            }
+        }
+        // The blended fragments need only the visibility buffers and shadow maps, so outside stage
+        // timing they are collected and shaded on the side stream while the opaque shading runs;
+        // the blending waits for both.
+        const bool side = L.n_blend && !timing;
+        if (side) {
+            HIP_TRY(c, hipEventRecord(c->ev_fork, s), "event");
+            HIP_TRY(c, hipStreamWaitEvent(c->side_stream, c->ev_fork, 0), "fork");
+            HIP_TRY(c, launch_blend_fragments(bl, sb, c->side_stream), "blended fragment kernels");
+            HIP_TRY(c, hipEventRecord(c->ev_join, c->side_stream), "event");
+        }
+        if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[3], s), "event");
+        HIP_TRY(c, launch_shade(sh, s), "shade kernel");
+        if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[5], s), "event");
+        if (L.n_blend) {
+            if (side) HIP_TRY(c, hipStreamWaitEvent(s, c->ev_join, 0), "join");
+            else HIP_TRY(c, launch_blend_fragments(bl, sb, s), "blended fragment kernels");
+            HIP_TRY(c, launch_blend_resolve(bl, sb, s), "blend kernel");
         }
     } else if (timing) {
         HIP_TRY(c, hipEventRecord(c->ev_stage[3], s), "event");
@@ -600,17 +615,20 @@ rsy_result finish_frame(rsy_ctx* c) {
         const uint32_t nv = uint32_t(c->layout.vg.size());
         const uint32_t total_pairs = static_cast<const uint32_t*>(c->readback_counters.p)[nv * kCtrPerView];
         const bool list_over = static_cast<const uint32_t*>(c->readback_counters.p)[nv * kCtrPerView + 2] != 0;
-        bool rec_over = false, layers_over = false;
+        bool rec_over = false, frags_over = false;
         for (uint32_t v = 0; v < nv; ++v) {
             const uint32_t ov = static_cast<const uint32_t*>(c->readback_counters.p)[v * kCtrPerView + kCtrOverflow];
-            rec_over |= (ov & ~kOverflowLayers) != 0;
-            layers_over |= (ov & kOverflowLayers) != 0;
+            rec_over |= (ov & ~kOverflowFrags) != 0;
+            frags_over |= (ov & kOverflowFrags) != 0;
         }
-        if (layers_over) {
-            if (c->blend_passes >= kMaxBlendPasses)
-                return fail(c, RSY_OVERFLOW, "blended layers at a pixel exceed the peeling pass limit");
-            c->blend_passes = std::min(2 * c->blend_passes, kMaxBlendPasses);
-            invalidate_graph(c);
+        if (frags_over) {
+            // the blended fragments of the frame, measured by the collection's counter
+            const uint32_t n = static_cast<const uint32_t*>(
+                c->readback_counters.p)[c->layout.first_blend * kCtrPerView + kCtrFrags];
+            const uint64_t want = std::min<uint64_t>(uint64_t(n) + n / 4 + 4096, 0xfffffff0ull);
+            if (want <= c->frag_cap) return fail(c, RSY_OVERFLOW, "blended fragments exceed 2^32");
+            c->frag_cap = uint32_t(want);
+            c->layout.valid = false;
         }
         if (rec_over) {
             c->rec_scale *= 2;
@@ -661,7 +679,10 @@ rsy_result rsy_create(const rsy_config* cfg, rsy_ctx** out_ctx) {
     if (!c) return RSY_ALLOCATION_FAILURE;
     c->device = dev;
     if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess) {
+        hipStreamCreateWithFlags(&c->side_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
         delete c;
         return RSY_GRAPHICS_INIT_FAILURE;
     }
@@ -690,11 +711,15 @@ void rsy_destroy(rsy_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     invalidate_graph(ctx);
+    if (ctx->side_stream) (void)hipStreamSynchronize(ctx->side_stream);
     if (ctx->ev_begin) (void)hipEventDestroy(ctx->ev_begin);
     if (ctx->ev_end) (void)hipEventDestroy(ctx->ev_end);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     for (auto& e : ctx->ev_stage)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->side_stream) (void)hipStreamDestroy(ctx->side_stream);
     delete ctx;
 }
 

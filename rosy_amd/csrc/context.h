@@ -119,8 +119,6 @@ inline bool same_views(const std::vector<FrameViewDesc>& a, const std::vector<Fr
     return true;
 }
 
-constexpr uint32_t kMaxBlendPasses = 64;  // blended layers a pixel may hold
-
 inline uint32_t ilog2(uint32_t v) {
     uint32_t r = 0;
     while ((1u << (r + 1)) <= v) ++r;
@@ -135,7 +133,9 @@ void destroy_neural(NeuralState* n);
 struct rsy_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side_stream = nullptr;  // the blended fragments beside the opaque shading (joined each frame)
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev_stage[8] = {};  // [5] end of shading, [6]/[7] around rsy_present
     bool present_timed = false;
     std::string err;
@@ -160,7 +160,7 @@ struct rsy_ctx {
     std::vector<rsy::TexGPU> h_tex_table;
     std::vector<rsy_sampler> h_samplers;
     std::vector<uint8_t> tex_opaque;  // four-channel texture whose alpha is 255 everywhere
-    // draw lists: 0 = shadow (all), 1 = opaque (!blended), 2 = blended (kept, not rasterised)
+    // draw lists: 0 = shadow (all), 1 = opaque (!blended), 2 = blended (the blended pass)
     std::vector<rsy::DrawGPU> lists[3];
     std::vector<uint32_t> list_index[3];  // surface indices
     uint32_t list_tris[3] = {0, 0, 0};
@@ -182,11 +182,11 @@ struct rsy_ctx {
     uint32_t list_cap = 0;
     uint32_t rec_scale = 1;  // grows on overflow
     std::vector<rsy::DevBuf> vis_out, color_out, shadow_factor_out, shadow_maps;
-    std::vector<rsy::DevBuf> blend_layers;  // per blend view: layer keys, then last peeled keys
-    rsy::DevBuf blend_found;                // per peeling pass: pixels that found a layer
+    std::vector<rsy::DevBuf> blend_lists;  // per blend view: per pixel (first fragment, count)
+    rsy::DevBuf frags, frag_keys, frag_color;  // the blended pass's fragments
+    uint32_t frag_cap = 0;
     rsy::DevBuf blend_shade_views_d;
     std::vector<rsy::ShadeView> h_blend_views;
-    uint32_t blend_passes = 4;              // peeling passes per frame (grows when a frame needs more)
     rsy::PinnedBuf staging, readback_counters;
     std::vector<rsy::ShadeView> h_shade_views;
     std::vector<rsy::ShadeCascade> h_shade_cascades;

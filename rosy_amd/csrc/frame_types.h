@@ -38,9 +38,12 @@ struct DrawGPU {
 
 // Per-view counters: records appended by the clipper, queued clipped triangles, entries of the
 // big-pair list, overflow flags, clipped-triangle statistic.
-enum : uint32_t { kCtrRecords = 0, kCtrClip = 1, kCtrBig = 2, kCtrOverflow = 3, kCtrClipped = 4, kCtrPerView = 8 };
-// kCtrOverflow bits: 1 clip queue, 2 records, 4 big pairs, 8 blended layers beyond the frame's passes
-constexpr uint32_t kOverflowLayers = 8u;
+// The first blend view's kCtrFrags slot counts the blended pass's fragments of the frame.
+enum : uint32_t {
+    kCtrRecords = 0, kCtrClip = 1, kCtrBig = 2, kCtrOverflow = 3, kCtrClipped = 4, kCtrFrags = 5, kCtrPerView = 8
+};
+// kCtrOverflow bits: 1 clip queue, 2 records, 4 big pairs, 8 blended fragments
+constexpr uint32_t kOverflowFrags = 8u;
 
 struct ViewGPU {
     uint32_t width, height;
@@ -57,10 +60,10 @@ struct ViewGPU {
     uint32_t cull, reverse, bias;
     float bias_const, bias_clamp, bias_slope;
     uint32_t cull_base;     // first per-draw culling flag of this view
-    uint32_t blend;         // 1: the blended list of camera view `source`, resolved by layer peeling
+    uint32_t blend;         // 1: the blended list of camera view `source`, drawn by the blended pass
     uint32_t source;
     uint64_t out;           // device address: u64 visibility (camera), u32 depth bits (shadow), or for a
-                            // blend view the u32 layer keys followed by the u32 last peeled keys
+                            // blend view its pixels' fragment lists (uint2: first fragment, count)
     float view_projection[16];
 };
 

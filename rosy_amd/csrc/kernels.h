@@ -82,8 +82,8 @@ struct RasterLaunch {
     const uint32_t* totals;
 };
 
-// Layer peeling of the blend views (one launch per pass).
-struct PeelLaunch {
+// The blended pass over the bins of the blend views (one launch).
+struct BlendLaunch {
     const ViewGPU* views;
     uint32_t n_views;
     uint32_t first_blend;  // first blend view; blend views come last and own the last bins
@@ -95,9 +95,11 @@ struct PeelLaunch {
     uint32_t list_cap;
     const TriRec* recs;
     uint64_t rec_cap_total;
-    uint32_t* found;  // per pass: pixels that found a layer
-    uint32_t n_passes;
-    uint32_t* counters;
+    uint32_t* counters;     // per-view counters (fragment count, overflow flags)
+    uint4* frags;           // per fragment: key, x | y << 16, blend view - first_blend, colour index
+    uint32_t* frag_keys;    // a bin's lists when they exceed the LDS budget
+    float4* frag_color;     // per fragment: shaded colour (discarded: kDiscarded in w)
+    uint32_t frag_cap;
 };
 
 // Per-pixel shading inputs of one camera view.
@@ -106,7 +108,6 @@ struct ShadeView {
     uint32_t block_base;  // first shading block of this view
     uint32_t blocks_x;
     const unsigned long long* vis;
-    const uint32_t* keys;  // blended pass: the peeled layer's key (triangle id + 1) per pixel, 0 = none
     const TriRec* recs;    // the raster view's record slots, indexed by triangle id
     const uint2* boxes;    // their boxes (kInvalidBox: no record, e.g. a clipped triangle)
     uint32_t xv_base;      // first transformed vertex of the raster view
@@ -147,9 +148,6 @@ struct ShadeLaunch {
     float clear_color[4];
     const float* srgb_lut;      // 256 entries
     const float4* xv;           // transformed vertices of all views
-    // blended pass: pixels found by each peeling pass, and the pass this launch blends
-    const uint32_t* blend_found;
-    uint32_t blend_pass;
     // material decoder (shade mode NEURAL only)
     const uint32_t* nm_slot;    // per material: latent grid slot or 0xFFFFFFFF
     const uint32_t* nm_active;  // per material: 1 = decoded by the network, 0 = plain textures
@@ -239,12 +237,14 @@ hipError_t launch_binning(const BinLaunch& b, hipStream_t s);
 constexpr uint32_t kScanPartialBytes = 32;  // per 4096 bins
 hipError_t launch_raster(const RasterLaunch& r, hipStream_t s);
 uint32_t raster_items_cap(uint32_t total_bins, uint32_t list_cap);  // work-item table entries
-hipError_t launch_raster_peel(const PeelLaunch& p, uint32_t pass, hipStream_t s);
 // output stage: colour -> RGBA16F draw image (sw x sh) -> linear blit to B8G8R8A8_SRGB (dw x dh)
 hipError_t launch_present(const float4* color, uint2* draw, uint32_t sw, uint32_t sh, uint32_t* out, uint32_t dw,
                           uint32_t dh, hipStream_t s);
 hipError_t launch_shade(const ShadeLaunch& l, hipStream_t s);
-hipError_t launch_shade_blend(const ShadeLaunch& l, hipStream_t s);  // one pass of the blended pass
+// the blended pass: fragment lists and their shading (needs the visibility buffers and shadow maps),
+// then the blending over the shaded colour
+hipError_t launch_blend_fragments(const BlendLaunch& b, const ShadeLaunch& l, hipStream_t s);
+hipError_t launch_blend_resolve(const BlendLaunch& b, const ShadeLaunch& l, hipStream_t s);
 hipError_t launch_nm_init(const NmInitLaunch& l, hipStream_t s);
 hipError_t launch_nm_step(const NmTrainLaunch& l, bool explicit_batch, hipStream_t s);
 hipError_t launch_nm_export(const float* lat, uint4* lat16, uint64_t cells, hipStream_t s);

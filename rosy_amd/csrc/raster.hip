@@ -26,16 +26,13 @@
 
 #include <cstdint>
 
+#include "bin_raster.h"
 #include "fixed_point.h"
 #include "frame_types.h"
 #include "kernels.h"
 
 namespace rsy {
 namespace {
-
-__device__ inline uint32_t lane_id() { return __lane_id(); }
-
-__device__ inline uint32_t fbits(float f) { return __float_as_uint(f); }
 
 // Snap, cull and set up one triangle.  Same rules, in the same operation order, as the oracle.
 __device__ bool setup_tri(const float4 v[3], uint32_t id, const ViewGPU& V, TriRec& r) {
@@ -126,30 +123,7 @@ __device__ inline void rec_tiles(const TriRec& r, uint32_t shift, int& tx0, int&
     ty1 = int(r.bby >> 16) >> shift;
 }
 
-
 constexpr int kSmallBins = 4;  // records touching at most this many bins are binned in the block's LDS table
-
-// Exclusive prefix sum over the workgroup (wave scans with shuffles, wave totals through LDS).
-// Every thread of the block must call it; it synchronises the block.
-__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds_waves, uint32_t& total) {
-    uint32_t x = v;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off);
-        if (int(lane_id()) >= off) x += y;
-    }
-    const uint32_t wave = threadIdx.x / 64, nwaves = blockDim.x / 64;
-    if (lane_id() == 63) lds_waves[wave] = x;
-    __syncthreads();
-    uint32_t wave_off = 0, sum = 0;
-    for (uint32_t w = 0; w < nwaves; ++w) {
-        const uint32_t s = lds_waves[w];
-        if (w < wave) wave_off += s;
-        sum += s;
-    }
-    __syncthreads();
-    total = sum;
-    return wave_off + x - v;
-}
 
 }  // namespace
 
@@ -484,8 +458,6 @@ constexpr uint32_t kScanPerThread = 4;
 // ordered so that the longest items start first and the short ones fill the end of the launch:
 // whole bins with at least kHeavyRecs records, then those with at least kMidRecs, then the split
 // bins' items (each a share of a crowded bin), then the remaining whole bins.
-constexpr uint32_t kItemRecs = 256;   // records per raster work item (one per thread)
-constexpr uint32_t kChunkRecs = 128;  // records staged in LDS at a time
 constexpr uint32_t kHeavyRecs = 128;
 constexpr uint32_t kMidRecs = 48;
 
@@ -708,7 +680,7 @@ __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t 
         uint32_t v = 0;
         while (v + 1 < n_views && views[v + 1].tile_base <= bin) ++v;
         const ViewGPU& W = views[v];
-        if (W.blend) continue;  // k_raster_peel writes every pixel of a blend view
+        if (W.blend) continue;  // a blend view has no output of its own
         // view fields copied out: the stores below could alias the view table for the compiler
         const uint32_t width = W.width, height = W.height, local = bin - W.tile_base;
         const uint32_t x0 = (local % W.tiles_x) * T, y0 = (local / W.tiles_x) * T;
@@ -722,144 +694,6 @@ __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t 
                 else reinterpret_cast<unsigned long long*>(out)[o] = 0ull;
             }
         }
-    }
-}
-
-// Per-record raster flags, computed once when a record is staged for a bin: bit k (k < 3) is set when
-// edge k is not a top-left edge (its value must exceed 0: F_k = E_k - 1), bit 3 when the edge
-// values need 64 bits.
-constexpr uint32_t kMetaWide = 8u;
-
-__device__ inline uint32_t record_meta(int32_t X0, int32_t Y0, int32_t X1, int32_t Y1, int32_t X2, int32_t Y2) {
-    const int32_t X[3] = {X0, X1, X2}, Y[3] = {Y0, Y1, Y2};
-    uint32_t meta = 0;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const int f = k == 2 ? 0 : k + 1;
-        const int32_t A = Y[f] - Y[k], B = X[k] - X[f];
-        const bool tl = A > 0 || (A == 0 && B > 0);
-        meta |= tl ? 0u : (1u << k);
-    }
-    // the whole triangle's box (sub-pixel units) bounds every |E| over it: under 64 pixels on each
-    // side every |E| < 2^30
-    const int32_t wx = max(X[0], max(X[1], X[2])) - min(X[0], min(X[1], X[2]));
-    const int32_t wy = max(Y[0], max(Y[1], Y[2])) - min(Y[0], min(Y[1], Y[2]));
-    if (wx >= 64 * 256 || wy >= 64 * 256) meta |= kMetaWide;
-    return meta;
-}
-
-// Fragment operations of the bin raster on the LDS tile (rows padded to T + 1 words).  Each takes
-// a covered pixel's tile index, its 31-bit depth bits and the record's key (triangle id + 1).
-template <int T>
-struct ShadowFrag {  // shadow view: largest depth (GEQUAL under reverse-Z)
-    static constexpr int kStride = T + 1;
-    uint32_t* tile;
-    __device__ void operator()(int i, uint32_t zb, uint32_t) const { atomicMax(&tile[i], zb); }
-};
-template <int T>
-struct CameraFrag {  // camera view: depth bits above key, the largest word wins
-    static constexpr int kStride = T + 1;
-    unsigned long long* tile;
-    __device__ void operator()(int i, uint32_t zb, uint32_t key) const {
-        atomicMax(&tile[i], ((unsigned long long)zb << 32) | key);
-    }
-};
-template <int T>
-struct PeelFrag {  // blend view: the smallest key above the pixel's last peeled key among the
-                   // fragments that pass the depth test against the opaque depth
-    static constexpr int kStride = T + 1;
-    uint32_t* keys;
-    const uint32_t* opaque;  // depth bits of the camera view
-    const uint32_t* last;    // last peeled key (0: none yet)
-    __device__ void operator()(int i, uint32_t zb, uint32_t key) const {
-        if (zb >= opaque[i] && key > last[i]) atomicMin(&keys[i], key);
-    }
-};
-
-// One row of a record inside the bin: E_k(x) for the three edges advances by 256 * A_k per pixel
-// (exact integer steps; EdgeT = int32_t unless the record is wide), depth is the FP32 plane rule of
-// setup, and covered pixels merge into the LDS row with atomic max (64-bit depth | id + 1 words in
-// a camera view, 32-bit depth words in a shadow view).  The top-left rule is folded into the edge
-// value: F_k = E_k - 1 + tl_k, so "E_k > 0, or E_k == 0 on a top-left edge" is F_k >= 0 for
-// integer E_k, and a pixel is covered when (F_0 | F_1 | F_2) has a clear sign bit -- one test, no
-// branches per edge.
-template <typename EdgeT, class Op>
-__device__ inline void raster_row(const TriRec& r, uint32_t meta, int ox, int oy, int y, int bx0, int bx1,
-                                  const Op& op) {
-    const int32_t X[3] = {r.X0, r.X1, r.X2}, Y[3] = {r.Y0, r.Y1, r.Y2};
-    const int32_t Py = (oy + y) * 256 + 128;
-    int32_t Px = (ox + bx0) * 256 + 128;
-    EdgeT f[3], step[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const int n = k == 2 ? 0 : k + 1;
-        const int32_t A = Y[n] - Y[k], B = X[k] - X[n];
-        f[k] = EdgeT(A) * EdgeT(Px - X[k]) + EdgeT(B) * EdgeT(Py - Y[k]) - EdgeT((meta >> k) & 1u);
-        step[k] = EdgeT(A) * EdgeT(256);
-    }
-    const float fy = float(Py - r.Y0);
-    const uint32_t key = (r.id & kRecIdMask) + 1u;
-    const int row = y * Op::kStride;
-    for (int x = bx0; x <= bx1; ++x, Px += 256) {
-        if ((f[0] | f[1] | f[2]) >= 0) {
-            float z = fmaf(r.dzdy, fy, fmaf(r.dzdx, float(Px - r.X0), r.z0));
-            z = fminf(fmaxf(z, 0.0f), 1.0f);
-            op(row + x, fbits(z) & 0x7fffffffu, key);
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) f[k] += step[k];
-    }
-}
-
-// The records of one work item through the bin's LDS tile.  Every thread loads one record of the
-// item (its box clipped to the bin, its raster flags computed) in one round of loads; the records
-// then go through LDS kChunkRecs at a time, each chunk staged by its own threads.  A staged
-// record's rows become tasks (a prefix sum numbers them and each record writes its index into the
-// task-owner table), and every thread takes tasks in turn, so small and large triangles spread
-// evenly over the workgroup.
-template <int T, class Op>
-__device__ inline void raster_item(const Op& op, TriRec* chunk_recs, uint8_t* owner, uint32_t* row_start,
-                                   uint32_t* scan_lds, bool have, uint4 w0, uint4 w1, uint4 w2, uint32_t n, int ox,
-                                   int oy) {
-    static_assert(kItemRecs == 256 && kItemRecs % kChunkRecs == 0, "one record per thread");
-    uint32_t my_rows = 0;
-    if (have) {
-        const int bx0 = max(int(w2.z & 0xffffu), ox) - ox, bx1 = min(int(w2.z >> 16), ox + T - 1) - ox;
-        const int by0 = max(int(w2.w & 0xffffu), oy) - oy, by1 = min(int(w2.w >> 16), oy + T - 1) - oy;
-        if (bx0 <= bx1 && by0 <= by1) {
-            my_rows = uint32_t(by1 - by0 + 1);
-            // the staged copy keeps the clipped box in bbx and the raster flags in bby
-            w2.z = uint32_t(bx0) | (uint32_t(bx1) << 8) | (uint32_t(by0) << 16);
-            w2.w = record_meta(int32_t(w0.x), int32_t(w0.y), int32_t(w0.z), int32_t(w0.w), int32_t(w1.x),
-                               int32_t(w1.y));
-        }
-    }
-    const uint32_t slot = threadIdx.x % kChunkRecs;
-    for (uint32_t c = 0; c * kChunkRecs < n; ++c) {
-        const bool stager = threadIdx.x / kChunkRecs == c;
-        const uint32_t rows = stager ? my_rows : 0u;
-        if (rows) {
-            uint4* dst = reinterpret_cast<uint4*>(chunk_recs + slot);
-            dst[0] = w0;
-            dst[1] = w1;
-            dst[2] = w2;
-        }
-        uint32_t total_rows;
-        const uint32_t first = block_exclusive_scan(rows, scan_lds, total_rows);  // synchronises
-        if (stager) row_start[slot] = first;
-        for (uint32_t j = 0; j < rows; ++j) owner[first + j] = uint8_t(slot);
-        __syncthreads();
-        for (uint32_t task = threadIdx.x; task < total_rows; task += 256) {
-            const uint32_t k = owner[task];
-            const TriRec q = chunk_recs[k];
-            const int bx0 = int(q.bbx & 0xffu), bx1 = int((q.bbx >> 8) & 0xffu);
-            const int y = int((q.bbx >> 16) & 0xffu) + int(task - row_start[k]);
-            if (q.bby & kMetaWide)
-                raster_row<int64_t>(q, q.bby, ox, oy, y, bx0, bx1, op);
-            else
-                raster_row<int32_t>(q, q.bby, ox, oy, y, bx0, bx1, op);
-        }
-        __syncthreads();
     }
 }
 
@@ -897,7 +731,7 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
     uint32_t view = 0;
     while (view + 1 < n_views && views[view + 1].tile_base <= bin) ++view;
     const ViewGPU V = views[view];
-    if (V.blend) return;  // blend views are resolved by k_raster_peel
+    if (V.blend) return;  // blend views are drawn by the blended pass (k_blend)
     const uint32_t local_bin = bin - V.tile_base;
     const int ox = int(local_bin % V.tiles_x) * T, oy = int(local_bin / V.tiles_x) * T;
     // view fields copied out: the output stores could alias the view table for the compiler
@@ -939,82 +773,6 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
                 else if (w != 0ull) atomicMax(out + o, w);
             }
         }
-    }
-}
-
-// Layer peeling of the blended list (the blended pass of Graphics.cpp:5655-5691: draw order, no
-// depth write, depth test GEQUAL against the opaque pass, no culling).  Pass k finds for every
-// pixel of a blend view the blended fragment with the smallest key (triangle id + 1 in the
-// blended list's draw order) above the key peeled in pass k - 1 whose depth is at least the opaque
-// depth; the shading pass then blends it.  Keys grow along the blended list, so the passes visit
-// each pixel's fragments in primitive order -- the order the reference's blending applies them.
-// One workgroup per bin of the blend views, every record of the bin in rounds of kItemRecs; each
-// pass writes every pixel (0: no fragment) and the pixel's last peeled key, and counts the pixels
-// that found one.  A pass after a pass that found nothing exits at once; a fragment found in the
-// last pass flags that the frame may hold more layers than it has passes.
-template <int T>
-__global__ __launch_bounds__(256) void k_raster_peel(const ViewGPU* views, uint32_t n_views, uint32_t first_blend,
-                                                     const uint32_t* bin_counts, const uint32_t* bin_offsets,
-                                                     const uint32_t* lists, uint32_t list_cap, const TriRec* recs,
-                                                     uint64_t rec_cap_total, uint32_t* found, uint32_t pass,
-                                                     uint32_t n_passes, uint32_t* counters) {
-    constexpr int S = T + 1;
-    __shared__ uint32_t keys[T * S], opaque[T * S], last[T * S];
-    __shared__ TriRec chunk_recs[kChunkRecs];
-    __shared__ uint8_t owner[kChunkRecs * T];
-    __shared__ uint32_t row_start[kChunkRecs];
-    __shared__ uint32_t scan_lds[4];
-    __shared__ uint32_t found_lds;
-    if (pass > 0 && found[pass - 1] == 0u) return;
-    const uint32_t bin = views[first_blend].tile_base + blockIdx.x;
-    uint32_t view = first_blend;
-    while (view + 1 < n_views && views[view + 1].tile_base <= bin) ++view;
-    const ViewGPU V = views[view];
-    const uint32_t local_bin = bin - V.tile_base;
-    const int ox = int(local_bin % V.tiles_x) * T, oy = int(local_bin / V.tiles_x) * T;
-    const int width = int(V.width), height = int(V.height);
-    uint32_t* key_out = reinterpret_cast<uint32_t*>(V.out);
-    uint32_t* last_out = key_out + size_t(width) * height;
-    const unsigned long long* vis = reinterpret_cast<const unsigned long long*>(views[V.source].out);
-    for (int p = threadIdx.x; p < T * T; p += 256) {
-        const int x = ox + p % T, y = oy + p / T, i = (p / T) * S + p % T;
-        const bool in = x < width && y < height;
-        const size_t o = size_t(y) * width + x;
-        opaque[i] = in ? uint32_t(vis[o] >> 32) : 0xffffffffu;
-        last[i] = in && pass > 0 ? last_out[o] : 0u;
-        keys[i] = 0xffffffffu;
-    }
-    if (threadIdx.x == 0) found_lds = 0u;
-    const uint32_t off = bin_offsets[bin];
-    const uint32_t end = uint32_t(min(uint64_t(off) + bin_counts[bin], uint64_t(list_cap)));
-    __syncthreads();
-    const PeelFrag<T> op{keys, opaque, last};
-    for (uint32_t base = off; base < end; base += kItemRecs) {
-        const uint32_t n = min(end - base, kItemRecs);
-        const uint32_t ri = threadIdx.x < n ? lists[base + threadIdx.x] : 0xffffffffu;
-        const bool have = ri < rec_cap_total;
-        const uint4* src = reinterpret_cast<const uint4*>(recs + (have ? ri : 0u));
-        const uint4 w0 = src[0], w1 = src[1], w2 = src[2];
-        raster_item<T>(op, chunk_recs, owner, row_start, scan_lds, have, w0, w1, w2, n, ox, oy);
-    }
-    __syncthreads();
-    uint32_t mine = 0;
-    for (int p = threadIdx.x; p < T * T; p += 256) {
-        const int x = ox + p % T, y = oy + p / T, i = (p / T) * S + p % T;
-        if (x < width && y < height) {
-            const size_t o = size_t(y) * width + x;
-            const uint32_t k = keys[i];
-            const bool hit = k != 0xffffffffu;
-            key_out[o] = hit ? k : 0u;
-            last_out[o] = hit ? k : last[i];
-            mine += hit ? 1u : 0u;
-        }
-    }
-    if (mine) atomicAdd(&found_lds, mine);
-    __syncthreads();
-    if (threadIdx.x == 0 && found_lds) {
-        atomicAdd(&found[pass], found_lds);
-        if (pass + 1 == n_passes) atomicOr(&counters[view * kCtrPerView + kCtrOverflow], kOverflowLayers);
     }
 }
 
@@ -1072,24 +830,6 @@ hipError_t launch_raster(const RasterLaunch& r, hipStream_t s) {
         case 64:
             hipLaunchKernelGGL(k_raster<64>, dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.lists, r.list_cap,
                                r.recs, r.rec_cap_total, r.items, r.totals);
-            break;
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_raster_peel(const PeelLaunch& p, uint32_t pass, hipStream_t s) {
-    if (p.blend_bins == 0) return hipSuccess;
-    switch (p.tile) {
-        case 32:
-            hipLaunchKernelGGL(k_raster_peel<32>, dim3(p.blend_bins), dim3(256), 0, s, p.views, p.n_views,
-                               p.first_blend, p.bin_counts, p.bin_offsets, p.lists, p.list_cap, p.recs,
-                               p.rec_cap_total, p.found, pass, p.n_passes, p.counters);
-            break;
-        case 64:
-            hipLaunchKernelGGL(k_raster_peel<64>, dim3(p.blend_bins), dim3(256), 0, s, p.views, p.n_views,
-                               p.first_blend, p.bin_counts, p.bin_offsets, p.lists, p.list_cap, p.recs,
-                               p.rec_cap_total, p.found, pass, p.n_passes, p.counters);
             break;
         default: return hipErrorInvalidValue;
     }

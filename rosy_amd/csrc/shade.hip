@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <type_traits>
 
+#include "bin_raster.h"
 #include "fixed_point.h"
 #include "frame_types.h"
 #include "kernels.h"
@@ -549,42 +550,187 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
     }
 }
 
-// One pass of the blended pass: every pixel whose peeling pass found a layer shades that blended
-// fragment with the plain material maps (fragmentMain, basic.slang:137-344) and, unless the alpha
-// mask discards it, blends it over the colour with the reference's blend state (Graphics.cpp:
-// 5657-5664): colour = src.rgb * src.a + dst.rgb * (1 - src.a), alpha = src.a.
-__global__ __launch_bounds__(256) void k_shade_blend(ShadeLaunch L) {
+// The blended pass (Graphics.cpp:5655-5691): the blended draw list in primitive order over the
+// opaque frame, depth test GEQUAL against the opaque depth without depth writes, no culling, each
+// fragment shaded with the plain material maps (fragmentMain, basic.slang:137-344), discarded under
+// the alpha mask (basic.slang:335-337) and otherwise blended with the reference's blend state
+// (Graphics.cpp:5657-5664): colour = src.rgb * src.a + dst.rgb * (1 - src.a), alpha = src.a.
+//
+// Three launches.  k_blend_collect builds every pixel's list of passing fragments, sorted by key
+// (triangle id + 1 in the blended list, so primitive order); k_blend_shade shades all fragments of
+// the frame in parallel; k_blend_resolve blends each pixel's list in order over its colour.
+
+// Shaded colour of a fragment the alpha mask discards: all bits of w set (a NaN that shading does
+// not produce).
+constexpr uint32_t kDiscarded = 0xffffffffu;
+
+// One workgroup per 16x16 quarter (T = 32) or sixteenth (T = 64) of a bin of a blend view, so that
+// a crowded bin spreads over several workgroups.  Two raster sweeps over the bin's blended records:
+// the first counts each pixel's passing fragments in LDS, a prefix sum turns the counts into list
+// offsets (the workgroup takes its range of the frame's fragment array with one device atomic),
+// and the second writes each fragment's key into its pixel's list.  The lists stay in LDS when
+// they fit (global memory otherwise), are sorted per pixel (insertion sort; lists are short), and
+// go out as fragment records plus per-pixel (first fragment, count) entries.  A workgroup whose
+// fragments do not fit the frame's fragment array writes empty lists and flags the overflow (the
+// frame is re-run with a larger array).
+constexpr int kBlendSub = 16;  // sub-tile edge of the fragment collection
+
+template <int T>
+__global__ __launch_bounds__(256) void k_blend_collect(const BlendLaunch B) {
+    constexpr int U = kBlendSub, S = U + 1, Q = (T / U) * (T / U);
+    static_assert(U * U == 256, "one pixel per thread");
+    constexpr uint32_t kLdsFrags = 8192u;
+    __shared__ uint32_t opaque[U * S], count[U * S], cursor[U * S];
+    __shared__ uint32_t lds_keys[kLdsFrags];
+    __shared__ TriRec chunk_recs[kChunkRecs];
+    __shared__ uint8_t owner[kChunkRecs * U];
+    __shared__ uint32_t row_start[kChunkRecs];
+    __shared__ uint32_t scan_lds[4];
+    __shared__ uint32_t base_lds;
+
+    const uint32_t bin = B.views[B.first_blend].tile_base + blockIdx.x / Q;
+    const uint32_t off = B.bin_offsets[bin];
+    const uint32_t end = uint32_t(min(uint64_t(off) + B.bin_counts[bin], uint64_t(B.list_cap)));
+    if (end <= off) return;  // no blended triangle touches this bin
+    uint32_t view = B.first_blend;
+    while (view + 1 < B.n_views && B.views[view + 1].tile_base <= bin) ++view;
+    const ViewGPU& V = B.views[view];
+    const uint32_t local_bin = bin - V.tile_base, q = blockIdx.x % Q;
+    const int ox = int(local_bin % V.tiles_x) * T + int(q % (T / U)) * U;
+    const int oy = int(local_bin / V.tiles_x) * T + int(q / (T / U)) * U;
+    const int width = int(V.width), height = int(V.height);
+    if (ox >= width || oy >= height) return;  // a sub-tile past the view's edge
+    const unsigned long long* vis = reinterpret_cast<const unsigned long long*>(B.views[V.source].out);
+    uint2* pixel_lists = reinterpret_cast<uint2*>(V.out);
+    const uint32_t sview = view - B.first_blend;
+
+    // thread t owns pixel t of the sub-tile (row-major)
+    const int px = ox + int(threadIdx.x) % U, py = oy + int(threadIdx.x) / U;
+    const int pi = (int(threadIdx.x) / U) * S + int(threadIdx.x) % U;
+    const bool inside = px < width && py < height;
+    opaque[pi] = inside ? uint32_t(vis[size_t(py) * width + px] >> 32) : 0xffffffffu;
+    count[pi] = 0u;
+    __syncthreads();
+    auto sweep = [&](const auto& op) {
+        for (uint32_t base = off; base < end; base += kItemRecs) {
+            const uint32_t n = min(end - base, kItemRecs);
+            const uint32_t ri = threadIdx.x < n ? B.lists[base + threadIdx.x] : 0xffffffffu;
+            const bool have = ri < B.rec_cap_total;
+            const uint4* src = reinterpret_cast<const uint4*>(B.recs + (have ? ri : 0u));
+            const uint4 w0 = src[0], w1 = src[1], w2 = src[2];
+            raster_item<U>(op, chunk_recs, owner, row_start, scan_lds, have, w0, w1, w2, n, ox, oy);
+        }
+    };
+    sweep(CountFrag<U>{count, opaque});  // ends on a barrier
+
+    const uint32_t n = count[pi];
+    uint32_t total;
+    const uint32_t first = block_exclusive_scan(n, scan_lds, total);
+    cursor[pi] = first;
+    if (threadIdx.x == 0) base_lds = atomicAdd(&B.counters[B.first_blend * kCtrPerView + kCtrFrags], total);
+    __syncthreads();
+    const uint32_t base = base_lds;
+    if (uint64_t(base) + total > B.frag_cap) {
+        // out of fragment space: empty lists for this sub-tile, empty records for its reserved range
+        if (threadIdx.x == 0) atomicOr(&B.counters[view * kCtrPerView + kCtrOverflow], kOverflowFrags);
+        for (uint32_t j = base + threadIdx.x; j < B.frag_cap; j += 256) B.frags[j] = make_uint4(0u, 0u, 0u, 0u);
+        if (inside) pixel_lists[size_t(py) * width + px] = make_uint2(0u, 0u);
+        return;
+    }
+    uint32_t* keys = total <= kLdsFrags ? lds_keys : B.frag_keys + base;
+    sweep(FillFrag<U>{cursor, opaque, keys});  // ends on a barrier
+
+    uint32_t* l = keys + first;
+    for (uint32_t a = 1; a < n; ++a) {
+        const uint32_t v = l[a];
+        uint32_t b = a;
+        for (; b > 0 && l[b - 1] > v; --b) l[b] = l[b - 1];
+        l[b] = v;
+    }
+    if (inside) pixel_lists[size_t(py) * width + px] = make_uint2(base + first, n);
+    // fragment records layer by layer (every pixel's first fragment, then every second one, ...),
+    // so that a wave of the shading kernel holds neighbouring pixels of one layer; each record
+    // carries the index of its colour in the per-pixel lists
+    const uint32_t xy = uint32_t(px) | (uint32_t(py) << 16);
+    uint32_t layer_base = base;
+    for (uint32_t a = 0;; ++a) {
+        uint32_t in_layer;
+        const uint32_t rank = block_exclusive_scan(a < n ? 1u : 0u, scan_lds, in_layer);
+        if (in_layer == 0u) break;
+        if (a < n) B.frags[layer_base + rank] = make_uint4(l[a], xy, sview, base + first + a);
+        layer_base += in_layer;
+    }
+}
+
+// Every fragment of the frame shaded at its pixel centre (a grid-stride loop over the count the
+// collection left), its colour stored at the index its record names.  Records of sub-tiles that ran
+// out of space have key 0 and are skipped.
+__global__ __launch_bounds__(256) void k_blend_shade(const BlendLaunch B, const ShadeLaunch L) {
     __shared__ float lut[256];
     __shared__ uint32_t tri_base[kLdsDraws];
-    if (L.blend_found[L.blend_pass] == 0u) return;  // no pixel has a layer in this pass
     lut[threadIdx.x] = L.srgb_lut[threadIdx.x];
     const uint32_t nd_lds = L.n_draws < kLdsDraws ? L.n_draws : kLdsDraws;
     for (uint32_t i = threadIdx.x; i < nd_lds; i += 256) tri_base[i] = L.draws[i].tri_base;
     __syncthreads();
-
-    uint32_t vi = 0;
-    while (vi + 1 < L.n_views && L.views[vi + 1].block_base <= blockIdx.x) ++vi;
-    const ShadeView& V = L.views[vi];
-    const uint32_t lb = blockIdx.x - V.block_base;
-    const uint32_t x = (lb % V.blocks_x) * 16 + (threadIdx.x & 15);
-    const uint32_t y = (lb / V.blocks_x) * 16 + (threadIdx.x >> 4);
-    if (x >= V.width || y >= V.height) return;
-    const size_t o = size_t(y) * V.width + x;
-    const uint32_t key = V.keys[o];
-    if (key == 0u) return;
+    const uint32_t n = min(B.counters[B.first_blend * kCtrPerView + kCtrFrags], B.frag_cap);
     const float none[kNmOut] = {};
-    const PixelGeom pg = pixel_geometry(L, V, tri_base, x, y, (unsigned long long)key);
-    const Fragment f = shade_pixel<false>(L, V, lut, o, pg, false, none);
-    if (f.discard) return;
-    const float4 dst = V.color[o];
-    const float a = f.color.w;
-    V.color[o] = make_float4(f.color.x * a + dst.x * (1.f - a), f.color.y * a + dst.y * (1.f - a),
-                             f.color.z * a + dst.z * (1.f - a), a);
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) {
+        const uint4 f = B.frags[j];
+        if (f.x == 0u) continue;
+        const ShadeView& SV = L.views[f.z];
+        const uint32_t x = f.y & 0xffffu, y = f.y >> 16;
+        const size_t o = size_t(y) * SV.width + x;
+        const PixelGeom pg = pixel_geometry(L, SV, tri_base, x, y, (unsigned long long)f.x);
+        const Fragment fr = wave_uniform(pg.draw) ? shade_pixel<true>(L, SV, lut, o, pg, false, none)
+                                                  : shade_pixel<false>(L, SV, lut, o, pg, false, none);
+        B.frag_color[f.w] = fr.discard ? make_float4(0.f, 0.f, 0.f, __uint_as_float(kDiscarded)) : fr.color;
+    }
 }
 
-hipError_t launch_shade_blend(const ShadeLaunch& l, hipStream_t s) {
-    if (l.total_blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_shade_blend, dim3(l.total_blocks), dim3(256), 0, s, l);
+// Each pixel of a blend view whose bin holds blended records blends its sorted fragments in order.
+template <int T>
+__global__ __launch_bounds__(256) void k_blend_resolve(const BlendLaunch B, const ShadeLaunch L) {
+    uint32_t vi = 0;
+    while (vi + 1 < L.n_views && L.views[vi + 1].block_base <= blockIdx.x) ++vi;
+    const ShadeView& SV = L.views[vi];
+    const ViewGPU& V = B.views[B.first_blend + vi];
+    const uint32_t lb = blockIdx.x - SV.block_base;
+    const uint32_t x = (lb % SV.blocks_x) * 16 + (threadIdx.x & 15);
+    const uint32_t y = (lb / SV.blocks_x) * 16 + (threadIdx.x >> 4);
+    if (x >= SV.width || y >= SV.height) return;
+    if (B.bin_counts[V.tile_base + (y / T) * V.tiles_x + x / T] == 0u) return;  // no list written
+    const size_t o = size_t(y) * SV.width + x;
+    const uint2 l = reinterpret_cast<const uint2*>(V.out)[o];
+    if (l.y == 0u) return;
+    float4 c = SV.color[o];
+    for (uint32_t k = 0; k < l.y; ++k) {
+        const float4 f = B.frag_color[l.x + k];
+        if (__float_as_uint(f.w) == kDiscarded) continue;
+        const float a = f.w;
+        c = make_float4(f.x * a + c.x * (1.f - a), f.y * a + c.y * (1.f - a), f.z * a + c.z * (1.f - a), a);
+    }
+    SV.color[o] = c;
+}
+
+hipError_t launch_blend_fragments(const BlendLaunch& b, const ShadeLaunch& l, hipStream_t s) {
+    if (b.blend_bins == 0 || l.total_blocks == 0) return hipSuccess;
+    const uint32_t shade_blocks = max(1u, min(2048u, (b.frag_cap + 255) / 256));
+    switch (b.tile) {
+        case 32: hipLaunchKernelGGL(k_blend_collect<32>, dim3(b.blend_bins * 4), dim3(256), 0, s, b); break;
+        case 64: hipLaunchKernelGGL(k_blend_collect<64>, dim3(b.blend_bins * 16), dim3(256), 0, s, b); break;
+        default: return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(k_blend_shade, dim3(shade_blocks), dim3(256), 0, s, b, l);
+    return hipGetLastError();
+}
+
+hipError_t launch_blend_resolve(const BlendLaunch& b, const ShadeLaunch& l, hipStream_t s) {
+    if (b.blend_bins == 0 || l.total_blocks == 0) return hipSuccess;
+    switch (b.tile) {
+        case 32: hipLaunchKernelGGL(k_blend_resolve<32>, dim3(l.total_blocks), dim3(256), 0, s, b, l); break;
+        case 64: hipLaunchKernelGGL(k_blend_resolve<64>, dim3(l.total_blocks), dim3(256), 0, s, b, l); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -1,9 +1,8 @@
 """The blended pass on the GPU against the FP64 oracle (oracle.blend_view): blended surfaces in draw
 order over the opaque frame -- GEQUAL against the opaque depth, no depth write, both faces, alpha
 mask discard, SRC_ALPHA / ONE_MINUS_SRC_ALPHA -- within 1e-3 max abs, at two view sizes.  The scene
-holds up to six layers at a pixel, more than the first frame's peeling passes, so the pass count
-grows on the way; the visibility buffer stays the opaque one, bit-exact, and graph replays give the
-eager frame."""
+holds up to six layers at a pixel; the visibility buffer stays the opaque one, bit-exact, and graph
+replays give the eager frame."""
 import numpy as np
 import pytest
 
@@ -86,4 +85,28 @@ def test_generated_workload_with_translucent_materials():
     assert count.max() >= 4 and err <= TOL
     r.render_graph(frame, 3)
     assert np.array_equal(r.color(0), col)
+    r.close()
+
+
+def test_deep_stacks_grow_the_fragment_array():
+    """Forty blended quads over the whole view: a 16x16 collection sub-tile holds 10,240 fragments,
+    more than its LDS lists take (they go through global memory), and the frame more than the first
+    fragment array holds, so the frame is re-run with a larger one; colours still match the oracle."""
+    quads = [(-0.8, 0.8, -0.8, 0.8, 0.30, 0, False)]
+    quads += [(-1.0, 1.0, -1.0, 1.0, 0.35 + 0.015 * k, 1 if k % 2 else 3, True) for k in range(40)]
+    s = blend_scene(96, quads=quads)
+    r = rosy_amd.Renderer(device=0)
+    r.load(s)
+    frame = s.frame_desc(abi.SHADE_PLAIN)
+    r.render(frame)
+    vis, col = r.visibility(0), r.color(0).astype(np.float64)
+    cam = s.camera_views()[0]
+    base, _, _ = oracle.shade_view(s, cam, s.cascades(), [r.shadow_map(0)], vis)
+    want, count = oracle.blend_view(s, cam, s.cascades(), [r.shadow_map(0)], vis, base, max_layers=48)
+    assert count.min() == 40 and r.stats().overflowed == 0
+    err = float(np.abs(col - want).max())
+    print(f"40-deep blended stacks: max abs err {err:.3e}")
+    assert err <= TOL
+    r.render_graph(frame, 2)
+    assert np.array_equal(r.color(0), col.astype(np.float32))
     r.close()
