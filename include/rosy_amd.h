@@ -176,8 +176,14 @@ typedef struct rsy_raster_state {
     float depth_bias_clamp;
     float depth_bias_slope_factor;
     uint32_t pcf_mode;  /* rsy_pcf_mode */
-    uint32_t tile_size; /* raster bin size in pixels: 32 or 64; 0 = autotuned */
+    uint32_t tile_size; /* raster bin size in pixels: 32 or 64; 0 = autotuned (32 with 4 samples) */
     float clear_color[4];
+    /* Samples per pixel of the camera views: 1 (0 = 1) or 4.  The reference's main pass renders
+     * into a 4-sample colour attachment resolved by averaging (Graphics.cpp:1794-1808, 5566-5575);
+     * with 4 samples, coverage and depth are taken at the standard sample locations and the
+     * fragment shader runs once per pixel and primitive.  Shadow views are single-sampled. */
+    uint32_t msaa_samples;
+    uint32_t reserved;
 } rsy_raster_state;
 
 /* A camera view rendered into a visibility buffer and shaded. */
@@ -296,9 +302,12 @@ rsy_result rsy_synchronize(rsy_ctx* ctx);
  * ------------------------------------------------------------------------------------------- */
 
 /* Visibility word per pixel: (float depth bits << 32) | (triangle id + 1); 0 = background.
- * Triangle ids count triangles over the opaque list in order. */
+ * Triangle ids count triangles over the opaque list in order.  With 4 samples the buffer holds
+ * one plane per sample (n_pixels = 4 * width * height; sample s of pixel o at s * width * height
+ * + o). */
 rsy_result rsy_read_visibility(rsy_ctx* ctx, uint32_t view, uint64_t* out, uint64_t n_pixels);
-/* Linear RGBA32F colour per pixel (the reference's draw image before resolve/blit). */
+/* Linear RGBA32F colour per pixel (the reference's draw image before the blit; with 4 samples the
+ * resolved average of the samples). */
 rsy_result rsy_read_color(rsy_ctx* ctx, uint32_t view, float* out_rgba, uint64_t n_pixels);
 /* Shadow-map depth per texel (float, 0 = clear). */
 rsy_result rsy_read_shadow_map(rsy_ctx* ctx, uint32_t cascade, float* out, uint64_t n_texels);

@@ -54,6 +54,14 @@ def lib():
         L.orc_raster_view.argtypes = [C.POINTER(Geometry), C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int,
                                       C.POINTER(abi.RasterState), C.c_void_p, C.c_void_p, C.POINTER(RasterStats),
                                       C.c_int]
+        L.orc_raster_view_at.restype = C.c_int
+        L.orc_raster_view_at.argtypes = [C.POINTER(Geometry), C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int,
+                                         C.POINTER(abi.RasterState), C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                         C.POINTER(RasterStats), C.c_int]
+        L.orc_blend_layers_at.restype = C.c_int
+        L.orc_blend_layers_at.argtypes = [C.POINTER(Geometry), C.POINTER(C.c_float), C.c_uint32, C.c_uint32,
+                                          C.POINTER(abi.RasterState), C.c_int32, C.c_int32, C.c_void_p, C.c_uint32,
+                                          C.c_void_p, C.c_void_p, C.c_int]
         L.orc_count_tile_pairs.restype = C.c_uint64
         L.orc_count_tile_pairs.argtypes = [C.POINTER(Geometry), C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int,
                                            C.POINTER(abi.RasterState), C.c_uint32, C.c_int]
@@ -119,21 +127,36 @@ def geometry(scene, shadow):
     return g
 
 
-def raster_view(scene, view_projection, width, height, shadow, raster_state, n_threads=None):
+# The standard sample locations of a 4-sample image (Vulkan: (0.375, 0.125), (0.875, 0.375),
+# (0.125, 0.625), (0.625, 0.875) of the pixel), in 1/256 pixel.
+MSAA4_SAMPLES = ((96, 32), (224, 96), (32, 160), (160, 224))
+
+
+def raster_view(scene, view_projection, width, height, shadow, raster_state, n_threads=None, sample=None):
+    """One view's visibility (camera) or depth (shadow); coverage and depth at the pixel centre, or at
+    `sample` = (x, y) in 1/256 pixel."""
     g = geometry(scene, shadow)
     vp = (C.c_float * 16)(*view_projection)
     st = RasterStats()
+    sx, sy = sample if sample is not None else (128, 128)
     if shadow:
         out = np.zeros((height, width), dtype=np.float32)
-        rc = lib().orc_raster_view(C.byref(g), vp, width, height, 1, C.byref(raster_state), None,
-                                   out.ctypes.data, C.byref(st), n_threads or threads())
+        rc = lib().orc_raster_view_at(C.byref(g), vp, width, height, 1, C.byref(raster_state), sx, sy, None,
+                                      out.ctypes.data, C.byref(st), n_threads or threads())
     else:
         out = np.zeros((height, width), dtype=np.uint64)
-        rc = lib().orc_raster_view(C.byref(g), vp, width, height, 0, C.byref(raster_state), out.ctypes.data,
-                                   None, C.byref(st), n_threads or threads())
+        rc = lib().orc_raster_view_at(C.byref(g), vp, width, height, 0, C.byref(raster_state), sx, sy,
+                                      out.ctypes.data, None, C.byref(st), n_threads or threads())
     if rc != 0:
         raise RuntimeError(f"orc_raster_view failed: {rc}")
     return out, st
+
+
+def raster_view_msaa(scene, view_projection, width, height, raster_state, n_threads=None):
+    """The four sample planes [4, h, w] of a 4-sample camera view: each the single-sample raster at
+    that sample location."""
+    return np.stack([raster_view(scene, view_projection, width, height, False, raster_state, n_threads, smp)[0]
+                     for smp in MSAA4_SAMPLES])
 
 
 def tile_pairs(scene, view_projection, width, height, shadow, raster_state, tile, n_threads=None):
@@ -245,9 +268,10 @@ def shade_view(scene, view, cascades, shadow_maps, vis, scene_data=None, raster_
     return rgba, shadow, margin
 
 
-def blend_layers(scene, view, opaque_vis, max_layers=16, raster_state=None, n_threads=None):
+def blend_layers(scene, view, opaque_vis, max_layers=16, raster_state=None, n_threads=None, sample=None):
     """The blended list's fragments per pixel in primitive order: (layers [K, h, w] u64 visibility
-    words, count [h, w])."""
+    words, count [h, w]); at the pixel centre, or at `sample` = (x, y) in 1/256 pixel against that
+    sample plane's opaque visibility."""
     g = geometry(scene, BLENDED_LIST)
     h, w = opaque_vis.shape
     vp = (C.c_float * 16)(*view.view_projection[:])
@@ -255,20 +279,22 @@ def blend_layers(scene, view, opaque_vis, max_layers=16, raster_state=None, n_th
     layers = np.zeros((max_layers, h, w), dtype=np.uint64)
     count = np.zeros((h, w), dtype=np.uint32)
     ov = np.ascontiguousarray(opaque_vis, dtype=np.uint64)
-    rc = lib().orc_blend_layers(C.byref(g), vp, w, h, C.byref(rs), ov.ctypes.data, max_layers, layers.ctypes.data,
-                                count.ctypes.data, n_threads or threads())
+    sx, sy = sample if sample is not None else (128, 128)
+    rc = lib().orc_blend_layers_at(C.byref(g), vp, w, h, C.byref(rs), sx, sy, ov.ctypes.data, max_layers,
+                                   layers.ctypes.data, count.ctypes.data, n_threads or threads())
     if rc != 0:
         raise RuntimeError(f"orc_blend_layers failed: {rc}")
     return layers, count
 
 
 def blend_view(scene, view, cascades, shadow_maps, opaque_vis, opaque_rgba, max_layers=16, scene_data=None,
-               raster_state=None, n_threads=None):
+               raster_state=None, n_threads=None, sample=None):
     """The blended pass over an opaque frame, FP64: each pixel's blended fragments in primitive
-    order are shaded (fragmentMain), discarded when alpha mode 1 puts alpha below the cutoff
-    (basic.slang:335-337), and blended as colour = src.rgb * src.a + dst.rgb * (1 - src.a),
-    alpha = src.a (Graphics.cpp:5657-5664)."""
-    layers, count = blend_layers(scene, view, opaque_vis, max_layers, raster_state, n_threads)
+    order are shaded (fragmentMain, at the pixel centre), discarded when alpha mode 1 puts alpha
+    below the cutoff (basic.slang:335-337), and blended as colour = src.rgb * src.a + dst.rgb *
+    (1 - src.a), alpha = src.a (Graphics.cpp:5657-5664).  With `sample`, the fragments are those
+    covering that sample location and passing the depth test there (one sample plane)."""
+    layers, count = blend_layers(scene, view, opaque_vis, max_layers, raster_state, n_threads, sample)
     out = np.array(opaque_rgba, dtype=np.float64, copy=True)
     draws, n = draw_list(scene, BLENDED_LIST)
     tri_base = np.cumsum([0] + [draws[i].index_count // 3 for i in range(n)])
@@ -290,3 +316,28 @@ def blend_view(scene, view, cascades, shadow_maps, opaque_vis, opaque_rgba, max_
         out[..., :3] = np.where(keep[..., None], src, out[..., :3])
         out[..., 3] = np.where(keep, a, out[..., 3])
     return out, count
+
+
+def msaa_view(scene, view, cascades, shadow_maps, vis_planes, scene_data=None, raster_state=None, n_threads=None,
+              neural=None, blended=False, max_layers=16):
+    """A 4-sample camera view shaded and resolved, FP64 (the main pass of Graphics.cpp:5566-5575:
+    a 4-sample colour attachment resolved with VK_RESOLVE_MODE_AVERAGE_BIT).  The fragment shader
+    runs once per pixel and primitive at the pixel centre (no sample shading), so sample s holds the
+    shading of its plane's primitive at the centre (the clear colour where uncovered); with
+    `blended` the blended pass then runs per sample plane.  Returns (resolved rgba [h, w, 4],
+    sample colours [4, h, w, 4], blended layer counts per plane [4, h, w] or None, the smallest
+    shadow-compare margin of the opaque shading over the planes [h, w])."""
+    rs = raster_state if raster_state is not None else scene.s.raster_state
+    samples, counts, margins = [], [], []
+    for s, smp in enumerate(MSAA4_SAMPLES):
+        rgba, _, margin = shade_view(scene, view, cascades, shadow_maps, vis_planes[s], scene_data, rs,
+                                     n_threads=n_threads, neural=neural)
+        margins.append(margin)
+        if blended:
+            rgba, count = blend_view(scene, view, cascades, shadow_maps, vis_planes[s], rgba, max_layers, scene_data,
+                                     rs, n_threads, sample=smp)
+            counts.append(count)
+        samples.append(rgba)
+    samples = np.stack(samples)
+    return samples.mean(axis=0), samples, (np.stack(counts) if blended else None), np.min(margins, axis=0)
+

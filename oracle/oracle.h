@@ -62,6 +62,12 @@ int orc_raster_view(const orc_geometry* g, const float view_projection[16], uint
                     int is_shadow, const rsy_raster_state* rs, uint64_t* out_vis, float* out_depth,
                     orc_raster_stats* stats, int n_threads);
 
+/* The same with coverage and depth taken at the point (sample_x, sample_y) / 256 inside each pixel
+ * instead of its centre (128, 128): one sample plane of a multisampled view. */
+int orc_raster_view_at(const orc_geometry* g, const float view_projection[16], uint32_t width, uint32_t height,
+                       int is_shadow, const rsy_raster_state* rs, int32_t sample_x, int32_t sample_y,
+                       uint64_t* out_vis, float* out_depth, orc_raster_stats* stats, int n_threads);
+
 /* The blended pass's fragments at every pixel of a camera view (Graphics.cpp:5655-5691: no culling,
  * no depth write, GEQUAL against the opaque depth of opaque_vis), in primitive order: layer k of
  * pixel o is layers[k * w*h + o] = (depth bits << 32) | (triangle id + 1) in the view's draw list,
@@ -69,6 +75,12 @@ int orc_raster_view(const orc_geometry* g, const float view_projection[16], uint
 int orc_blend_layers(const orc_geometry* g, const float view_projection[16], uint32_t width, uint32_t height,
                      const rsy_raster_state* rs, const uint64_t* opaque_vis, uint32_t max_layers, uint64_t* layers,
                      uint32_t* count, int n_threads);
+
+/* The same at the sample point (sample_x, sample_y) / 256 of each pixel, against that sample
+ * plane's opaque visibility. */
+int orc_blend_layers_at(const orc_geometry* g, const float view_projection[16], uint32_t width, uint32_t height,
+                        const rsy_raster_state* rs, int32_t sample_x, int32_t sample_y, const uint64_t* opaque_vis,
+                        uint32_t max_layers, uint64_t* layers, uint32_t* count, int n_threads);
 
 /* Number of (record, tile) pairs when records are binned by their pixel bounding boxes into
  * tile x tile bins (the bin traffic term of the roofline). */

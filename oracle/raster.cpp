@@ -38,6 +38,7 @@ struct Params {
     uint32_t width, height;
     bool shadow, cull, reverse, bias;
     float bias_const, bias_clamp, bias_slope;
+    int32_t sx = 128, sy = 128;  // the sample point inside a pixel, in 1/256 pixel (centre: 128, 128)
 };
 
 // Snap, cull and set up one (sub-)triangle; returns false when it produces no fragments.
@@ -58,10 +59,11 @@ bool setup_tri(const C4 v[3], uint32_t id, const Params& p, Rec& r) {
     }
     const int32_t minx = std::min({X[0], X[1], X[2]}), maxx = std::max({X[0], X[1], X[2]});
     const int32_t miny = std::min({Y[0], Y[1], Y[2]}), maxy = std::max({Y[0], Y[1], Y[2]});
-    r.px0 = std::max<int32_t>(0, (minx - 128 + 255) >> 8);
-    r.px1 = std::min<int32_t>(int32_t(p.width) - 1, (maxx - 128) >> 8);
-    r.py0 = std::max<int32_t>(0, (miny - 128 + 255) >> 8);
-    r.py1 = std::min<int32_t>(int32_t(p.height) - 1, (maxy - 128) >> 8);
+    // pixels whose sample point lies inside the vertices' box
+    r.px0 = std::max<int32_t>(0, (minx - p.sx + 255) >> 8);
+    r.px1 = std::min<int32_t>(int32_t(p.width) - 1, (maxx - p.sx) >> 8);
+    r.py0 = std::max<int32_t>(0, (miny - p.sy + 255) >> 8);
+    r.py1 = std::min<int32_t>(int32_t(p.height) - 1, (maxy - p.sy) >> 8);
     if (r.px0 > r.px1 || r.py0 > r.py1) return false;
     const float fx1 = float(X[1] - X[0]), fy1 = float(Y[1] - Y[0]);
     const float fx2 = float(X[2] - X[0]), fy2 = float(Y[2] - Y[0]);
@@ -240,9 +242,19 @@ extern "C" {
 int orc_raster_view(const orc_geometry* g, const float view_projection[16], uint32_t width, uint32_t height,
                     int is_shadow, const rsy_raster_state* rs, uint64_t* out_vis, float* out_depth,
                     orc_raster_stats* stats, int n_threads) {
+    return orc_raster_view_at(g, view_projection, width, height, is_shadow, rs, 128, 128, out_vis, out_depth, stats,
+                              n_threads);
+}
+
+int orc_raster_view_at(const orc_geometry* g, const float view_projection[16], uint32_t width, uint32_t height,
+                       int is_shadow, const rsy_raster_state* rs, int32_t sample_x, int32_t sample_y,
+                       uint64_t* out_vis, float* out_depth, orc_raster_stats* stats, int n_threads) {
     if (!g || !rs || width == 0 || height == 0) return RSY_INVALID_ARGUMENT;
     if (is_shadow ? !out_depth : !out_vis) return RSY_INVALID_ARGUMENT;
-    const Params p = make_params(width, height, is_shadow, rs);
+    if (sample_x < 0 || sample_x > 255 || sample_y < 0 || sample_y > 255) return RSY_INVALID_ARGUMENT;
+    Params p = make_params(width, height, is_shadow, rs);
+    p.sx = sample_x;
+    p.sy = sample_y;
     std::vector<Rec> recs;
     int rc = build_records(g, view_projection, p, recs, stats, n_threads);
     if (rc != RSY_OK) return rc;
@@ -265,9 +277,9 @@ int orc_raster_view(const orc_geometry* g, const float view_projection[16], uint
                 tl[i] = A[i] > 0 || (A[i] == 0 && B[i] > 0);
             }
             for (int32_t py = ya; py <= yb; ++py) {
-                const int32_t Py = py * 256 + 128;
+                const int32_t Py = py * 256 + p.sy;
                 for (int32_t px = r.px0; px <= r.px1; ++px) {
-                    const int32_t Px = px * 256 + 128;
+                    const int32_t Px = px * 256 + p.sx;
                     bool inside = true;
                     for (int i = 0; i < 3; ++i) {
                         const int64_t e = int64_t(A[i]) * (Px - r.X[i]) + int64_t(B[i]) * (Py - r.Y[i]);
@@ -306,9 +318,19 @@ int orc_raster_view(const orc_geometry* g, const float view_projection[16], uint
 int orc_blend_layers(const orc_geometry* g, const float view_projection[16], uint32_t width, uint32_t height,
                      const rsy_raster_state* rs, const uint64_t* opaque_vis, uint32_t max_layers, uint64_t* layers,
                      uint32_t* count, int n_threads) {
+    return orc_blend_layers_at(g, view_projection, width, height, rs, 128, 128, opaque_vis, max_layers, layers, count,
+                               n_threads);
+}
+
+int orc_blend_layers_at(const orc_geometry* g, const float view_projection[16], uint32_t width, uint32_t height,
+                        const rsy_raster_state* rs, int32_t sample_x, int32_t sample_y, const uint64_t* opaque_vis,
+                        uint32_t max_layers, uint64_t* layers, uint32_t* count, int n_threads) {
     if (!g || !rs || !opaque_vis || !count || width == 0 || height == 0) return RSY_INVALID_ARGUMENT;
     if (max_layers && !layers) return RSY_INVALID_ARGUMENT;
+    if (sample_x < 0 || sample_x > 255 || sample_y < 0 || sample_y > 255) return RSY_INVALID_ARGUMENT;
     Params p = make_params(width, height, 0, rs);
+    p.sx = sample_x;
+    p.sy = sample_y;
     p.cull = false;  // the blended pass draws both faces (Graphics.cpp:5675)
     std::vector<Rec> recs;
     int rc = build_records(g, view_projection, p, recs, nullptr, n_threads);
@@ -334,9 +356,9 @@ int orc_blend_layers(const orc_geometry* g, const float view_projection[16], uin
                 tl[i] = A[i] > 0 || (A[i] == 0 && B[i] > 0);
             }
             for (int32_t py = ya; py <= yb; ++py) {
-                const int32_t Py = py * 256 + 128;
+                const int32_t Py = py * 256 + p.sy;
                 for (int32_t px = r.px0; px <= r.px1; ++px) {
-                    const int32_t Px = px * 256 + 128;
+                    const int32_t Px = px * 256 + p.sx;
                     bool inside = true;
                     for (int i = 0; i < 3; ++i) {
                         const int64_t e = int64_t(A[i]) * (Px - r.X[i]) + int64_t(B[i]) * (Py - r.Y[i]);

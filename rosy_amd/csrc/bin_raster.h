@@ -55,7 +55,8 @@ __device__ inline uint32_t record_meta(int32_t X0, int32_t Y0, int32_t X1, int32
         meta |= tl ? 0u : (1u << k);
     }
     // the whole triangle's box (sub-pixel units) bounds every |E| over it: under 64 pixels on each
-    // side every |E| < 2^30
+    // side every |E| < 2^30 at pixel centres inside the box, and < 2^31 at the pixels and sample
+    // points a record's box reaches (at most 192 sub-pixel units beyond the vertices)
     const int32_t wx = max(X[0], max(X[1], X[2])) - min(X[0], min(X[1], X[2]));
     const int32_t wy = max(Y[0], max(Y[1], Y[2])) - min(Y[0], min(Y[1], Y[2]));
     if (wx >= 64 * 256 || wy >= 64 * 256) meta |= kMetaWide;
@@ -66,13 +67,13 @@ __device__ inline uint32_t record_meta(int32_t X0, int32_t Y0, int32_t X1, int32
 // a covered pixel's tile index, its 31-bit depth bits and the record's key (triangle id + 1).
 template <int T>
 struct ShadowFrag {  // shadow view: largest depth (GEQUAL under reverse-Z)
-    static constexpr int kStride = T + 1;
+    static constexpr int kStride = T + 1, kSamples = 1;
     uint32_t* tile;
     __device__ void operator()(int i, uint32_t zb, uint32_t) const { atomicMax(&tile[i], zb); }
 };
 template <int T>
 struct CameraFrag {  // camera view: depth bits above key, the largest word wins
-    static constexpr int kStride = T + 1;
+    static constexpr int kStride = T + 1, kSamples = 1;
     unsigned long long* tile;
     __device__ void operator()(int i, uint32_t zb, uint32_t key) const {
         atomicMax(&tile[i], ((unsigned long long)zb << 32) | key);
@@ -80,7 +81,7 @@ struct CameraFrag {  // camera view: depth bits above key, the largest word wins
 };
 template <int T>
 struct CountFrag {  // blend view: count the fragments that pass the depth test against the opaque depth
-    static constexpr int kStride = T + 1;
+    static constexpr int kStride = T + 1, kSamples = 1;
     uint32_t* count;
     const uint32_t* opaque;  // depth bits of the camera view
     __device__ void operator()(int i, uint32_t zb, uint32_t) const {
@@ -89,12 +90,53 @@ struct CountFrag {  // blend view: count the fragments that pass the depth test 
 };
 template <int T>
 struct FillFrag {  // blend view: append the key of each passing fragment to its pixel's list
-    static constexpr int kStride = T + 1;
+    static constexpr int kStride = T + 1, kSamples = 1;
     uint32_t* cursor;        // next free entry of the pixel's list
     const uint32_t* opaque;
     uint32_t* keys;          // the bin's lists (LDS or global memory)
     __device__ void operator()(int i, uint32_t zb, uint32_t key) const {
         if (zb >= opaque[i]) keys[atomicAdd(&cursor[i], 1u)] = key;
+    }
+};
+
+// The same operations on 4-sample tiles: one call per covered pixel with the mask of covered samples
+// and each sample's depth bits; sample planes lie T * (T + 1) words apart.
+template <int T>
+struct CameraFragMS {
+    static constexpr int kStride = T + 1, kSamples = 4, kPlane = T * (T + 1);
+    unsigned long long* tile;
+    __device__ void operator()(int i, uint32_t mask, const uint32_t (&zb)[4], uint32_t key) const {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            if (mask & (1u << s)) atomicMax(&tile[s * kPlane + i], ((unsigned long long)zb[s] << 32) | key);
+    }
+};
+template <int T>
+__device__ inline uint32_t passing_samples(const uint32_t* opaque, int i, uint32_t mask, const uint32_t (&zb)[4]) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+        if ((mask & (1u << s)) && zb[s] >= opaque[s * T * (T + 1) + i]) m |= 1u << s;
+    return m;
+}
+template <int T>
+struct CountFragMS {  // blend view: count the primitives with a sample passing the depth test
+    static constexpr int kStride = T + 1, kSamples = 4;
+    uint32_t* count;
+    const uint32_t* opaque;  // depth bits of the camera view's sample planes
+    __device__ void operator()(int i, uint32_t mask, const uint32_t (&zb)[4], uint32_t) const {
+        if (passing_samples<T>(opaque, i, mask, zb)) atomicAdd(&count[i], 1u);
+    }
+};
+template <int T>
+struct FillFragMS {  // blend view: append key | passing-sample mask << 28 to the pixel's list
+    static constexpr int kStride = T + 1, kSamples = 4;
+    uint32_t* cursor;
+    const uint32_t* opaque;
+    uint32_t* keys;
+    __device__ void operator()(int i, uint32_t mask, const uint32_t (&zb)[4], uint32_t key) const {
+        const uint32_t m = passing_samples<T>(opaque, i, mask, zb);
+        if (m) keys[atomicAdd(&cursor[i], 1u)] = key | (m << 28);
     }
 };
 
@@ -128,6 +170,47 @@ __device__ inline void raster_row(const TriRec& r, uint32_t meta, int ox, int oy
             z = fminf(fmaxf(z, 0.0f), 1.0f);
             op(row + x, fbits(z) & 0x7fffffffu, key);
         }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) f[k] += step[k];
+    }
+}
+
+// One row of a record at the 4 sample locations of each pixel: the centre's edge values plus a fixed
+// offset per sample and edge (exact integers), the depth plane at each covered sample.
+template <typename EdgeT, class Op>
+__device__ inline void raster_row_ms(const TriRec& r, uint32_t meta, int ox, int oy, int y, int bx0, int bx1,
+                                     const Op& op) {
+    const int32_t X[3] = {r.X0, r.X1, r.X2}, Y[3] = {r.Y0, r.Y1, r.Y2};
+    const int32_t Py = (oy + y) * 256 + 128;
+    int32_t Px = (ox + bx0) * 256 + 128;
+    EdgeT f[3], step[3], d[4][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int n = k == 2 ? 0 : k + 1;
+        const int32_t A = Y[n] - Y[k], B = X[k] - X[n];
+        f[k] = EdgeT(A) * EdgeT(Px - X[k]) + EdgeT(B) * EdgeT(Py - Y[k]) - EdgeT((meta >> k) & 1u);
+        step[k] = EdgeT(A) * EdgeT(256);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) d[s][k] = EdgeT(A) * EdgeT(msaa_x(s) - 128) + EdgeT(B) * EdgeT(msaa_y(s) - 128);
+    }
+    float fy[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) fy[s] = float(Py + msaa_y(s) - 128 - r.Y0);
+    const uint32_t key = (r.id & kRecIdMask) + 1u;
+    const int row = y * Op::kStride;
+    for (int x = bx0; x <= bx1; ++x, Px += 256) {
+        uint32_t mask = 0, zb[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            zb[s] = 0u;
+            if (((f[0] + d[s][0]) | (f[1] + d[s][1]) | (f[2] + d[s][2])) >= 0) {
+                float z = fmaf(r.dzdy, fy[s], fmaf(r.dzdx, float(Px + msaa_x(s) - 128 - r.X0), r.z0));
+                z = fminf(fmaxf(z, 0.0f), 1.0f);
+                zb[s] = fbits(z) & 0x7fffffffu;
+                mask |= 1u << s;
+            }
+        }
+        if (mask) op(row + x, mask, zb, key);
 #pragma unroll
         for (int k = 0; k < 3; ++k) f[k] += step[k];
     }
@@ -176,10 +259,17 @@ __device__ inline void raster_item(const Op& op, TriRec* chunk_recs, uint8_t* ow
             const TriRec q = chunk_recs[k];
             const int bx0 = int(q.bbx & 0xffu), bx1 = int((q.bbx >> 8) & 0xffu);
             const int y = int((q.bbx >> 16) & 0xffu) + int(task - row_start[k]);
-            if (q.bby & kMetaWide)
-                raster_row<int64_t>(q, q.bby, ox, oy, y, bx0, bx1, op);
-            else
-                raster_row<int32_t>(q, q.bby, ox, oy, y, bx0, bx1, op);
+            if constexpr (Op::kSamples == 4) {
+                if (q.bby & kMetaWide)
+                    raster_row_ms<int64_t>(q, q.bby, ox, oy, y, bx0, bx1, op);
+                else
+                    raster_row_ms<int32_t>(q, q.bby, ox, oy, y, bx0, bx1, op);
+            } else {
+                if (q.bby & kMetaWide)
+                    raster_row<int64_t>(q, q.bby, ox, oy, y, bx0, bx1, op);
+                else
+                    raster_row<int32_t>(q, q.bby, ox, oy, y, bx0, bx1, op);
+            }
         }
         __syncthreads();
     }

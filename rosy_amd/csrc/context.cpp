@@ -131,7 +131,10 @@ rsy_result resolve_views(rsy_ctx* c, const rsy_frame_desc* f, std::vector<FrameV
     return RSY_OK;
 }
 
+uint32_t msaa_of(const rsy_ctx* c) { return c->rs.msaa_samples == 4 ? 4u : 1u; }
+
 uint32_t choose_tile(const rsy_ctx* c) {
+    if (msaa_of(c) == 4) return 32u;  // the 4-sample tile of a 64-pixel bin exceeds the LDS budget
     if (c->rs.tile_size == 32 || c->rs.tile_size == 64) return c->rs.tile_size;
     return c->autotuned_tile ? c->autotuned_tile : 32u;
 }
@@ -168,6 +171,8 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     std::vector<uint32_t> order(views.size());
     for (uint32_t vi = 0; vi < views.size(); ++vi) order[vi] = vi;
     const bool blend = shade != RSY_SHADE_NONE && !c->lists[2].empty();
+    if (blend && msaa_of(c) == 4 && c->list_tris[2] >= (1u << 28))
+        return fail(c, RSY_INVALID_ARGUMENT, "4-sample blended pass: the blended list must hold fewer than 2^28 triangles");
     if (blend)
         for (uint32_t vi = 0; vi < views.size(); ++vi)
             if (!views[vi].shadow) order.push_back(vi);
@@ -187,6 +192,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         g.list = f.shadow ? 0 : (is_blend ? 2 : 1);
         g.blend = is_blend ? 1 : 0;
         g.source = order[oi];
+        g.samples = f.shadow ? 1u : msaa_of(c);
         const auto& list = c->lists[g.list];
         g.n_draws = uint32_t(list.size());
         g.cull_base = cull_total;
@@ -264,6 +270,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     c->vis_out.resize(L.n_camera);
     c->color_out.resize(L.n_camera);
     c->shadow_factor_out.resize(L.n_camera);
+    c->sample_color.resize(L.n_camera);
     c->shadow_maps.resize(L.n_shadow);
     c->blend_lists.resize(L.n_blend);
     uint32_t ci = 0, si = 0, bi = 0;
@@ -280,8 +287,10 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
             g.out = reinterpret_cast<uint64_t>(c->shadow_maps[si].p);
             ++si;
         } else {
-            HIP_TRY(c, c->vis_out[ci].ensure(px * 8), "allocate visibility buffer");
+            HIP_TRY(c, c->vis_out[ci].ensure(px * 8 * g.samples), "allocate visibility buffer");
             HIP_TRY(c, c->color_out[ci].ensure(px * 16), "allocate colour buffer");
+            if (g.samples == 4 && L.n_blend)
+                HIP_TRY(c, c->sample_color[ci].ensure(px * 16 * 4), "allocate sample colours");
             HIP_TRY(c, c->shadow_factor_out[ci].ensure(px * 4), "allocate shadow factor");
             g.out = reinterpret_cast<uint64_t>(c->vis_out[ci].p);
             ++ci;
@@ -293,6 +302,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         HIP_TRY(c, c->frags.ensure(size_t(want) * 16), "allocate blended fragments");
         HIP_TRY(c, c->frag_keys.ensure(size_t(want) * 4), "allocate blended fragment keys");
         HIP_TRY(c, c->frag_color.ensure(size_t(want) * 16), "allocate blended fragment colours");
+        HIP_TRY(c, c->frag_mask.ensure(size_t(want)), "allocate blended fragment samples");
         c->frag_cap = uint32_t(want);
     }
     HIP_TRY(c, c->readback_counters.ensure(nv * kCtrPerView * 4 + 64), "allocate readback");
@@ -342,7 +352,9 @@ rsy_result upload_frame(rsy_ctx* c) {
             sv.recs = c->recs.as<TriRec>() + vg.rec_base;
             sv.boxes = c->boxes.as<uint2>() + vg.rec_base;
             sv.xv_base = vg.xv_base;
+            sv.samples = vg.samples;
             sv.color = c->color_out[ci].as<float4>();
+            sv.sample_color = vg.samples == 4 && L.n_blend ? c->sample_color[ci].as<float4>() : nullptr;
             sv.shadow_factor = (c->debug_flags & RSY_DEBUG_SHADOW_FACTOR) ? c->shadow_factor_out[ci].as<float>() : nullptr;
             std::memcpy(sv.view_projection, vg.view_projection, sizeof(sv.view_projection));
             std::memcpy(sv.camera_position, L.views[i].camera_position, sizeof(sv.camera_position));
@@ -456,6 +468,7 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     r.items = b.items;
     r.items_cap = b.items_cap;
     r.totals = b.totals;
+    r.samples = msaa_of(c);
     HIP_TRY(c, launch_raster(r, s), "raster kernel");
     if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[2], s), "event");
 
@@ -483,6 +496,7 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
         std::memcpy(sh.clear_color, c->rs.clear_color, sizeof(sh.clear_color));
         sh.srgb_lut = c->srgb_lut.as<float>();
         sh.xv = g.xv;
+        sh.samples = msaa_of(c);
         if (L.shade == RSY_SHADE_NEURAL) neural_shade_args(c, sh);
         // the blended pass: the blended list of every camera view in primitive order
         BlendLaunch bl{};
@@ -503,7 +517,9 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
             bl.frags = c->frags.as<uint4>();
             bl.frag_keys = c->frag_keys.as<uint32_t>();
             bl.frag_color = c->frag_color.as<float4>();
+            bl.frag_mask = c->frag_mask.as<uint8_t>();
             bl.frag_cap = c->frag_cap;
+            bl.samples = msaa_of(c);
             sb.views = c->blend_shade_views_d.as<ShadeView>();
             sb.n_views = uint32_t(c->h_blend_views.size());
             sb.draws = c->lists_d[2].as<DrawGPU>();
@@ -886,6 +902,10 @@ rsy_result rsy_update(rsy_ctx* c, const rsy_scene_data* sd, const rsy_raster_sta
         if (state->tile_size != 0 && state->tile_size != 32 && state->tile_size != 64)
             return fail(c, RSY_INVALID_ARGUMENT, "tile_size must be 0, 32 or 64");
         if (state->pcf_mode > RSY_PCF_GRID_3X3) return fail(c, RSY_INVALID_ARGUMENT, "unknown pcf mode");
+        if (state->msaa_samples > 1 && state->msaa_samples != 4)
+            return fail(c, RSY_INVALID_ARGUMENT, "msaa_samples must be 0, 1 or 4");
+        if (state->msaa_samples == 4 && state->tile_size == 64)
+            return fail(c, RSY_INVALID_ARGUMENT, "4-sample frames use 32-pixel bins");
         const bool raster_changed = state->cull_enabled != c->rs.cull_enabled ||
                                     state->reverse_winding_order_enabled != c->rs.reverse_winding_order_enabled ||
                                     state->depth_bias_enabled != c->rs.depth_bias_enabled ||
@@ -893,6 +913,7 @@ rsy_result rsy_update(rsy_ctx* c, const rsy_scene_data* sd, const rsy_raster_sta
                                     state->depth_bias_clamp != c->rs.depth_bias_clamp ||
                                     state->depth_bias_slope_factor != c->rs.depth_bias_slope_factor ||
                                     state->tile_size != c->rs.tile_size || state->pcf_mode != c->rs.pcf_mode ||
+                                    (state->msaa_samples == 4) != (c->rs.msaa_samples == 4) ||
                                     std::memcmp(state->clear_color, c->rs.clear_color, sizeof(c->rs.clear_color)) != 0;
         c->rs = *state;
         if (raster_changed) c->layout.valid = false;
@@ -986,7 +1007,8 @@ rsy_result rsy_read_visibility(rsy_ctx* c, uint32_t view, uint64_t* out, uint64_
     uint32_t idx;
     if ((r = camera_index(c, view, idx)) != RSY_OK) return r;
     const ViewGPU* g = camera_vg(c, view);
-    if (n_pixels != uint64_t(g->width) * g->height) return fail(c, RSY_INVALID_ARGUMENT, "pixel count mismatch");
+    if (n_pixels != uint64_t(g->width) * g->height * g->samples)
+        return fail(c, RSY_INVALID_ARGUMENT, "pixel count mismatch (width * height * samples)");
     HIP_TRY(c, hipMemcpy(out, c->vis_out[idx].p, n_pixels * 8, hipMemcpyDeviceToHost), "read visibility");
     return RSY_OK;
 }
@@ -1107,6 +1129,7 @@ rsy_result rsy_autotune(rsy_ctx* c, const rsy_frame_desc* frame, uint32_t n_iter
     float best_ms = 1e30f;
     uint32_t best = 32;
     for (uint32_t t : {32u, 64u}) {
+        if (t == 64 && msaa_of(c) == 4) continue;  // 4-sample frames use 32-pixel bins
         c->rs.tile_size = t;
         c->layout.valid = false;
         rsy_result r = rsy_render_graph(c, frame, std::max<uint32_t>(1, n_iters));

@@ -183,7 +183,8 @@ struct rsy_ctx {
     uint32_t rec_scale = 1;  // grows on overflow
     std::vector<rsy::DevBuf> vis_out, color_out, shadow_factor_out, shadow_maps;
     std::vector<rsy::DevBuf> blend_lists;  // per blend view: per pixel (first fragment, count)
-    rsy::DevBuf frags, frag_keys, frag_color;  // the blended pass's fragments
+    rsy::DevBuf frags, frag_keys, frag_color, frag_mask;  // the blended pass's fragments
+    std::vector<rsy::DevBuf> sample_color;  // per camera view, 4 samples with a blended pass: colour planes
     uint32_t frag_cap = 0;
     rsy::DevBuf blend_shade_views_d;
     std::vector<rsy::ShadeView> h_blend_views;

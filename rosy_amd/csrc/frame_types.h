@@ -1,5 +1,7 @@
 // Device-side records shared by the host orchestration (context.cpp) and the kernels.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace rsy {
@@ -62,8 +64,11 @@ struct ViewGPU {
     uint32_t cull_base;     // first per-draw culling flag of this view
     uint32_t blend;         // 1: the blended list of camera view `source`, drawn by the blended pass
     uint32_t source;
-    uint64_t out;           // device address: u64 visibility (camera), u32 depth bits (shadow), or for a
-                            // blend view its pixels' fragment lists (uint2: first fragment, count)
+    uint32_t samples;       // 1, or 4 for the camera and blend views of a 4-sample frame
+    uint32_t pad;
+    uint64_t out;           // device address: u64 visibility (camera; one plane per sample), u32 depth
+                            // bits (shadow), or for a blend view its pixels' fragment lists (uint2:
+                            // first fragment, count)
     float view_projection[16];
 };
 
@@ -77,5 +82,10 @@ struct TexGPU {
     uint32_t channels;  // 2 or 4
     uint32_t srgb;
 };
+
+// The standard locations of a 4-sample pixel, (0.375, 0.125), (0.875, 0.375), (0.125, 0.625) and
+// (0.625, 0.875), in 1/256 pixel.
+__host__ __device__ constexpr int32_t msaa_x(int s) { return s == 0 ? 96 : s == 1 ? 224 : s == 2 ? 32 : 160; }
+__host__ __device__ constexpr int32_t msaa_y(int s) { return 32 + 64 * s; }
 
 }  // namespace rsy

@@ -80,6 +80,7 @@ struct RasterLaunch {
     const uint4* items;
     uint32_t items_cap;
     const uint32_t* totals;
+    uint32_t samples;  // of the camera views (1 or 4)
 };
 
 // The blended pass over the bins of the blend views (one launch).
@@ -99,7 +100,9 @@ struct BlendLaunch {
     uint4* frags;           // per fragment: key, x | y << 16, blend view - first_blend, colour index
     uint32_t* frag_keys;    // a bin's lists when they exceed the LDS budget
     float4* frag_color;     // per fragment: shaded colour (discarded: kDiscarded in w)
+    uint8_t* frag_mask;     // 4 samples: per fragment, the samples it covers and passes the depth test at
     uint32_t frag_cap;
+    uint32_t samples;       // samples per pixel of the camera views (1 or 4)
 };
 
 // Per-pixel shading inputs of one camera view.
@@ -111,8 +114,9 @@ struct ShadeView {
     const TriRec* recs;    // the raster view's record slots, indexed by triangle id
     const uint2* boxes;    // their boxes (kInvalidBox: no record, e.g. a clipped triangle)
     uint32_t xv_base;      // first transformed vertex of the raster view
-    uint32_t pad;
+    uint32_t samples;      // 1, or 4: vis holds one plane per sample and colour is their resolve
     float4* color;
+    float4* sample_color;  // 4 samples: per-sample colour planes for the blended pass (nullable)
     float* shadow_factor;  // nullable debug output
     float view_projection[16];
     float camera_position[4];
@@ -148,6 +152,7 @@ struct ShadeLaunch {
     float clear_color[4];
     const float* srgb_lut;      // 256 entries
     const float4* xv;           // transformed vertices of all views
+    uint32_t samples;           // samples per pixel of the camera views (1 or 4)
     // material decoder (shade mode NEURAL only)
     const uint32_t* nm_slot;    // per material: latent grid slot or 0xFFFFFFFF
     const uint32_t* nm_active;  // per material: 1 = decoded by the network, 0 = plain textures

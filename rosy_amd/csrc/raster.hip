@@ -54,10 +54,13 @@ __device__ bool setup_tri(const float4 v[3], uint32_t id, const ViewGPU& V, TriR
     }
     const int32_t minx = min(X[0], min(X[1], X[2])), maxx = max(X[0], max(X[1], X[2]));
     const int32_t miny = min(Y[0], min(Y[1], Y[2])), maxy = max(Y[0], max(Y[1], Y[2]));
-    const int32_t px0 = max(0, (minx - 128 + 255) >> 8);
-    const int32_t px1 = min(int32_t(V.width) - 1, (maxx - 128) >> 8);
-    const int32_t py0 = max(0, (miny - 128 + 255) >> 8);
-    const int32_t py1 = min(int32_t(V.height) - 1, (maxy - 128) >> 8);
+    // pixels with a sample point inside the vertices' box: the centre (128, 128), or with 4 samples
+    // anywhere in [32, 224] on each axis
+    const int32_t lo = V.samples == 4 ? 32 : 128, hi = V.samples == 4 ? 224 : 128;
+    const int32_t px0 = max(0, (minx - hi + 255) >> 8);
+    const int32_t px1 = min(int32_t(V.width) - 1, (maxx - lo) >> 8);
+    const int32_t py0 = max(0, (miny - hi + 255) >> 8);
+    const int32_t py1 = min(int32_t(V.height) - 1, (maxy - lo) >> 8);
     if (px0 > px1 || py0 > py1) return false;
     const float fx1 = float(X[1] - X[0]), fy1 = float(Y[1] - Y[0]);
     const float fx2 = float(X[2] - X[0]), fy2 = float(Y[2] - Y[0]);
@@ -685,13 +688,16 @@ __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t 
         const uint32_t width = W.width, height = W.height, local = bin - W.tile_base;
         const uint32_t x0 = (local % W.tiles_x) * T, y0 = (local / W.tiles_x) * T;
         const bool shadow = W.is_shadow != 0;
+        const uint32_t planes = W.samples;
         const uint64_t out = W.out;
+        const size_t plane = size_t(width) * height;
         for (uint32_t p = threadIdx.x; p < T * T; p += blockDim.x) {
             const uint32_t x = x0 + (p & (T - 1u)), y = y0 + (p >> shift);
             if (x < width && y < height) {
                 const size_t o = size_t(y) * width + x;
                 if (shadow) reinterpret_cast<uint32_t*>(out)[o] = 0u;
-                else reinterpret_cast<unsigned long long*>(out)[o] = 0ull;
+                else
+                    for (uint32_t s = 0; s < planes; ++s) reinterpret_cast<unsigned long long*>(out)[s * plane + o] = 0ull;
             }
         }
     }
@@ -704,12 +710,12 @@ __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t 
 // depth words in a shadow view.  A whole bin stores its tile; the items of a split bin merge into
 // the cleared bin with global atomic max.  Workgroups are dealt round-robin over the 8 XCDs, and
 // consecutive items are neighbouring bins, so the dense regions of a view spread over every XCD.
-template <int T>
+template <int T, int N>
 __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n_views, const uint32_t* lists,
                                                 uint32_t list_cap, const TriRec* recs, uint64_t rec_cap_total,
                                                 const uint4* items, const uint32_t* totals) {
     constexpr int S = T + 1;  // padded row stride (words)
-    __shared__ unsigned long long tile[T * S];
+    __shared__ unsigned long long tile[N * T * S];  // camera views: one plane per sample
     __shared__ TriRec chunk_recs[kChunkRecs];
     __shared__ uint8_t owner[kChunkRecs * T];  // task (row) -> record of the chunk
     __shared__ uint32_t row_start[kChunkRecs];  // exclusive prefix of row counts
@@ -760,17 +766,24 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
             }
         }
     } else {
-        for (int p = threadIdx.x; p < T * S; p += 256) tile[p] = 0ull;
-        raster_item<T>(CameraFrag<T>{tile}, chunk_recs, owner, row_start, scan_lds, have, w0, w1, w2, n, ox, oy);
+        for (int p = threadIdx.x; p < N * T * S; p += 256) tile[p] = 0ull;
+        if constexpr (N == 4)
+            raster_item<T>(CameraFragMS<T>{tile}, chunk_recs, owner, row_start, scan_lds, have, w0, w1, w2, n, ox, oy);
+        else
+            raster_item<T>(CameraFrag<T>{tile}, chunk_recs, owner, row_start, scan_lds, have, w0, w1, w2, n, ox, oy);
         __syncthreads();
         unsigned long long* out = reinterpret_cast<unsigned long long*>(out_addr);
+        const size_t plane = size_t(width) * height;
         for (int p = threadIdx.x; p < T * T; p += 256) {
             const int x = ox + p % T, y = oy + p / T;
             if (x < width && y < height) {
-                const unsigned long long w = tile[(p / T) * S + p % T];
                 const size_t o = size_t(y) * width + x;
-                if (!split) out[o] = w;
-                else if (w != 0ull) atomicMax(out + o, w);
+#pragma unroll
+                for (int s = 0; s < N; ++s) {
+                    const unsigned long long w = tile[s * T * S + (p / T) * S + p % T];
+                    if (!split) out[s * plane + o] = w;
+                    else if (w != 0ull) atomicMax(out + s * plane + o, w);
+                }
             }
         }
     }
@@ -822,17 +835,14 @@ hipError_t launch_raster(const RasterLaunch& r, hipStream_t s) {
     // one workgroup per work-item slot (extra workgroups exit)
     const uint32_t grid = r.items_cap;
     if (grid == 0) return hipSuccess;
-    switch (r.tile) {
-        case 32:
-            hipLaunchKernelGGL(k_raster<32>, dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.lists, r.list_cap,
-                               r.recs, r.rec_cap_total, r.items, r.totals);
-            break;
-        case 64:
-            hipLaunchKernelGGL(k_raster<64>, dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.lists, r.list_cap,
-                               r.recs, r.rec_cap_total, r.items, r.totals);
-            break;
-        default: return hipErrorInvalidValue;
-    }
+#define RSY_RASTER(T, N)                                                                                         \
+    hipLaunchKernelGGL((k_raster<T, N>), dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.lists, r.list_cap, \
+                       r.recs, r.rec_cap_total, r.items, r.totals)
+    if (r.tile == 32 && r.samples == 1) RSY_RASTER(32, 1);
+    else if (r.tile == 64 && r.samples == 1) RSY_RASTER(64, 1);
+    else if (r.tile == 32 && r.samples == 4) RSY_RASTER(32, 4);
+    else return hipErrorInvalidValue;  // 4-sample frames use 32-pixel bins
+#undef RSY_RASTER
     return hipGetLastError();
 }
 

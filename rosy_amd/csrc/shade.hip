@@ -550,6 +550,92 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
     }
 }
 
+// The 4-sample main pass (Graphics.cpp:5566-5575: a 4-sample colour attachment resolved with
+// VK_RESOLVE_MODE_AVERAGE_BIT).  Without sample shading the fragment shader runs once per pixel and
+// primitive, at the pixel centre, and its colour goes to every sample the primitive won; samples
+// no primitive covers keep the clear colour.  Each pixel shades its distinct primitives in turn
+// (the wave loops while any lane has one left, so the decoder's wave-wide MLP sees every lane) and
+// writes the average of its four samples; with a blended pass to follow it also keeps the sample
+// colours.  The debug shadow factor is that of sample 0's primitive.
+template <bool kNeural>
+__global__ __launch_bounds__(256) void k_shade_ms(ShadeLaunch L) {
+    __shared__ float lut[256];
+    __shared__ uint32_t tri_base[kLdsDraws];
+    lut[threadIdx.x] = L.srgb_lut[threadIdx.x];
+    const uint32_t nd_lds = L.n_draws < kLdsDraws ? L.n_draws : kLdsDraws;
+    for (uint32_t i = threadIdx.x; i < nd_lds; i += 256) tri_base[i] = L.draws[i].tri_base;
+    __syncthreads();
+
+    uint32_t vi = 0;
+    while (vi + 1 < L.n_views && L.views[vi + 1].block_base <= blockIdx.x) ++vi;
+    const ShadeView& V = L.views[vi];
+    const uint32_t lb = blockIdx.x - V.block_base;
+    const uint32_t x = (lb % V.blocks_x) * 16 + (threadIdx.x & 15);
+    const uint32_t y = (lb / V.blocks_x) * 16 + (threadIdx.x >> 4);
+    const bool inside = x < V.width && y < V.height;
+    const size_t o = size_t(y) * V.width + x, plane = size_t(V.width) * V.height;
+    unsigned long long v[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) v[s] = inside ? V.vis[s * plane + o] : 0ull;
+    const uint32_t k0 = uint32_t(v[0]), k1 = uint32_t(v[1]), k2 = uint32_t(v[2]), k3 = uint32_t(v[3]);
+    // bit s: sample s is the first to hold its primitive (or the background)
+    uint32_t todo = inside ? 1u | (k1 != k0 ? 2u : 0u) | (k2 != k0 && k2 != k1 ? 4u : 0u) |
+                                 (k3 != k0 && k3 != k1 && k3 != k2 ? 8u : 0u)
+                           : 0u;
+    const float4 clear = make_float4(L.clear_color[0], L.clear_color[1], L.clear_color[2], L.clear_color[3]);
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    float shadow0 = 1.f;
+    while (__builtin_amdgcn_ballot_w64(todo != 0u) != 0ull) {
+        const bool live = todo != 0u;
+        const int s = live ? __builtin_ctz(todo) : 0;
+        todo &= todo - 1u;
+        const unsigned long long w = s == 0 ? v[0] : s == 1 ? v[1] : s == 2 ? v[2] : v[3];
+        const uint32_t key = uint32_t(w);
+        const uint32_t same = (k0 == key ? 1u : 0u) | (k1 == key ? 2u : 0u) | (k2 == key ? 4u : 0u) | (k3 == key ? 8u : 0u);
+        const bool covered = live && key != 0u;
+        PixelGeom pg{};
+        if (covered) pg = pixel_geometry(L, V, tri_base, x, y, w);
+        float dec[kNmOut] = {};
+        bool decode = false;
+        if constexpr (kNeural) {
+            float f[kNmFeat] = {};
+            if (covered) {
+                const DrawGPU dr = L.draws[pg.draw];
+                if (dr.material < L.n_materials && L.nm_active[dr.material]) {
+                    decode = true;
+                    const rsy_position* verts = static_cast<const rsy_position*>(L.vertices);
+                    const float* t0 = verts[dr.vtx_base + pg.i0].texture_coordinates;
+                    const float* t1 = verts[dr.vtx_base + pg.i1].texture_coordinates;
+                    const float* t2 = verts[dr.vtx_base + pg.i2].texture_coordinates;
+                    const float u = (t0[0] - 2.0f * floorf(t0[0] * 0.5f)) + pg.lam.y * (t1[0] - t0[0]) + pg.lam.z * (t2[0] - t0[0]);
+                    const float vv = (t0[1] - 2.0f * floorf(t0[1] * 0.5f)) + pg.lam.y * (t1[1] - t0[1]) + pg.lam.z * (t2[1] - t0[1]);
+                    nm_features(L.nm_lat16 + size_t(L.nm_slot[dr.material]) * L.nm_L * L.nm_L, L.nm_L, u, vv, f);
+                }
+            }
+            if (__builtin_amdgcn_ballot_w64(decode) != 0ull) nm_mlp_wave(static_cast<const NmWeights*>(L.nm_weights), f, dec);
+        }
+        if (!live) continue;
+        float4 c = clear;
+        float sh = 1.f;
+        if (covered) {
+            const Fragment fr = wave_uniform(pg.draw) ? shade_pixel<true>(L, V, lut, o, pg, decode, dec)
+                                                      : shade_pixel<false>(L, V, lut, o, pg, decode, dec);
+            c = fr.color;
+            sh = fr.shadow;
+        }
+        const float n = float(__builtin_popcount(same));
+        acc = make_float4(acc.x + c.x * n, acc.y + c.y * n, acc.z + c.z * n, acc.w + c.w * n);
+        if (s == 0) shadow0 = sh;
+        if (V.sample_color)
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (same & (1u << t)) V.sample_color[t * plane + o] = c;
+    }
+    if (!inside) return;
+    V.color[o] = make_float4(acc.x * 0.25f, acc.y * 0.25f, acc.z * 0.25f, acc.w * 0.25f);
+    if (V.shadow_factor) V.shadow_factor[o] = shadow0;
+}
+
 // The blended pass (Graphics.cpp:5655-5691): the blended draw list in primitive order over the
 // opaque frame, depth test GEQUAL against the opaque depth without depth writes, no culling, each
 // fragment shaded with the plain material maps (fragmentMain, basic.slang:137-344), discarded under
@@ -575,12 +661,14 @@ constexpr uint32_t kDiscarded = 0xffffffffu;
 // frame is re-run with a larger array).
 constexpr int kBlendSub = 16;  // sub-tile edge of the fragment collection
 
-template <int T>
+template <int T, int N>
 __global__ __launch_bounds__(256) void k_blend_collect(const BlendLaunch B) {
     constexpr int U = kBlendSub, S = U + 1, Q = (T / U) * (T / U);
     static_assert(U * U == 256, "one pixel per thread");
     constexpr uint32_t kLdsFrags = 8192u;
-    __shared__ uint32_t opaque[U * S], count[U * S], cursor[U * S];
+    // list entries: the key, and with 4 samples the passing-sample mask in bits 28-31
+    constexpr uint32_t kKeyMask = N == 4 ? 0x0fffffffu : 0xffffffffu;
+    __shared__ uint32_t opaque[N * U * S], count[U * S], cursor[U * S];
     __shared__ uint32_t lds_keys[kLdsFrags];
     __shared__ TriRec chunk_recs[kChunkRecs];
     __shared__ uint8_t owner[kChunkRecs * U];
@@ -608,7 +696,10 @@ __global__ __launch_bounds__(256) void k_blend_collect(const BlendLaunch B) {
     const int px = ox + int(threadIdx.x) % U, py = oy + int(threadIdx.x) / U;
     const int pi = (int(threadIdx.x) / U) * S + int(threadIdx.x) % U;
     const bool inside = px < width && py < height;
-    opaque[pi] = inside ? uint32_t(vis[size_t(py) * width + px] >> 32) : 0xffffffffu;
+    const size_t plane = size_t(width) * height;
+#pragma unroll
+    for (int s = 0; s < N; ++s)
+        opaque[s * U * S + pi] = inside ? uint32_t(vis[s * plane + size_t(py) * width + px] >> 32) : 0xffffffffu;
     count[pi] = 0u;
     __syncthreads();
     auto sweep = [&](const auto& op) {
@@ -621,7 +712,8 @@ __global__ __launch_bounds__(256) void k_blend_collect(const BlendLaunch B) {
             raster_item<U>(op, chunk_recs, owner, row_start, scan_lds, have, w0, w1, w2, n, ox, oy);
         }
     };
-    sweep(CountFrag<U>{count, opaque});  // ends on a barrier
+    if constexpr (N == 4) sweep(CountFragMS<U>{count, opaque});  // ends on a barrier
+    else sweep(CountFrag<U>{count, opaque});
 
     const uint32_t n = count[pi];
     uint32_t total;
@@ -638,15 +730,18 @@ __global__ __launch_bounds__(256) void k_blend_collect(const BlendLaunch B) {
         return;
     }
     uint32_t* keys = total <= kLdsFrags ? lds_keys : B.frag_keys + base;
-    sweep(FillFrag<U>{cursor, opaque, keys});  // ends on a barrier
+    if constexpr (N == 4) sweep(FillFragMS<U>{cursor, opaque, keys});  // ends on a barrier
+    else sweep(FillFrag<U>{cursor, opaque, keys});
 
     uint32_t* l = keys + first;
     for (uint32_t a = 1; a < n; ++a) {
         const uint32_t v = l[a];
         uint32_t b = a;
-        for (; b > 0 && l[b - 1] > v; --b) l[b] = l[b - 1];
+        for (; b > 0 && (l[b - 1] & kKeyMask) > (v & kKeyMask); --b) l[b] = l[b - 1];
         l[b] = v;
     }
+    if constexpr (N == 4)
+        for (uint32_t a = 0; a < n; ++a) B.frag_mask[base + first + a] = uint8_t(l[a] >> 28);
     if (inside) pixel_lists[size_t(py) * width + px] = make_uint2(base + first, n);
     // fragment records layer by layer (every pixel's first fragment, then every second one, ...),
     // so that a wave of the shading kernel holds neighbouring pixels of one layer; each record
@@ -657,7 +752,7 @@ __global__ __launch_bounds__(256) void k_blend_collect(const BlendLaunch B) {
         uint32_t in_layer;
         const uint32_t rank = block_exclusive_scan(a < n ? 1u : 0u, scan_lds, in_layer);
         if (in_layer == 0u) break;
-        if (a < n) B.frags[layer_base + rank] = make_uint4(l[a], xy, sview, base + first + a);
+        if (a < n) B.frags[layer_base + rank] = make_uint4(l[a] & kKeyMask, xy, sview, base + first + a);
         layer_base += in_layer;
     }
 }
@@ -687,8 +782,15 @@ __global__ __launch_bounds__(256) void k_blend_shade(const BlendLaunch B, const 
     }
 }
 
-// Each pixel of a blend view whose bin holds blended records blends its sorted fragments in order.
-template <int T>
+// Each pixel of a blend view whose bin holds blended records blends its sorted fragments in order;
+// with 4 samples each fragment goes into the samples it passed at, and the pixel's colour becomes
+// the average of its samples again.
+__device__ inline float4 blend_over(float4 dst, float4 f) {
+    const float a = f.w;
+    return make_float4(f.x * a + dst.x * (1.f - a), f.y * a + dst.y * (1.f - a), f.z * a + dst.z * (1.f - a), a);
+}
+
+template <int T, int N>
 __global__ __launch_bounds__(256) void k_blend_resolve(const BlendLaunch B, const ShadeLaunch L) {
     uint32_t vi = 0;
     while (vi + 1 < L.n_views && L.views[vi + 1].block_base <= blockIdx.x) ++vi;
@@ -698,44 +800,77 @@ __global__ __launch_bounds__(256) void k_blend_resolve(const BlendLaunch B, cons
     const uint32_t x = (lb % SV.blocks_x) * 16 + (threadIdx.x & 15);
     const uint32_t y = (lb / SV.blocks_x) * 16 + (threadIdx.x >> 4);
     if (x >= SV.width || y >= SV.height) return;
-    if (B.bin_counts[V.tile_base + (y / T) * V.tiles_x + x / T] == 0u) return;  // no list written
+    // the collection wrote this pixel's list only when its bin has records inside the bin lists'
+    // capacity (a frame that overflowed them is re-run); the same test here
+    const uint32_t bin = V.tile_base + (y / T) * V.tiles_x + x / T;
+    const uint32_t off = B.bin_offsets[bin];
+    if (uint64_t(off) + B.bin_counts[bin] <= off || off >= B.list_cap) return;
     const size_t o = size_t(y) * SV.width + x;
     const uint2 l = reinterpret_cast<const uint2*>(V.out)[o];
-    if (l.y == 0u) return;
-    float4 c = SV.color[o];
-    for (uint32_t k = 0; k < l.y; ++k) {
-        const float4 f = B.frag_color[l.x + k];
-        if (__float_as_uint(f.w) == kDiscarded) continue;
-        const float a = f.w;
-        c = make_float4(f.x * a + c.x * (1.f - a), f.y * a + c.y * (1.f - a), f.z * a + c.z * (1.f - a), a);
+    if (l.y == 0u || uint64_t(l.x) + l.y > B.frag_cap) return;
+    if constexpr (N == 4) {
+        const size_t plane = size_t(SV.width) * SV.height;
+        float4 c[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) c[s] = SV.sample_color[s * plane + o];
+        for (uint32_t k = 0; k < l.y; ++k) {
+            const float4 f = B.frag_color[l.x + k];
+            if (__float_as_uint(f.w) == kDiscarded) continue;
+            const uint32_t m = B.frag_mask[l.x + k];
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                if (m & (1u << s)) c[s] = blend_over(c[s], f);
+        }
+        SV.color[o] = make_float4((c[0].x + c[1].x + c[2].x + c[3].x) * 0.25f, (c[0].y + c[1].y + c[2].y + c[3].y) * 0.25f,
+                                  (c[0].z + c[1].z + c[2].z + c[3].z) * 0.25f, (c[0].w + c[1].w + c[2].w + c[3].w) * 0.25f);
+    } else {
+        float4 c = SV.color[o];
+        for (uint32_t k = 0; k < l.y; ++k) {
+            const float4 f = B.frag_color[l.x + k];
+            if (__float_as_uint(f.w) == kDiscarded) continue;
+            c = blend_over(c, f);
+        }
+        SV.color[o] = c;
     }
-    SV.color[o] = c;
 }
 
 hipError_t launch_blend_fragments(const BlendLaunch& b, const ShadeLaunch& l, hipStream_t s) {
     if (b.blend_bins == 0 || l.total_blocks == 0) return hipSuccess;
     const uint32_t shade_blocks = max(1u, min(2048u, (b.frag_cap + 255) / 256));
-    switch (b.tile) {
-        case 32: hipLaunchKernelGGL(k_blend_collect<32>, dim3(b.blend_bins * 4), dim3(256), 0, s, b); break;
-        case 64: hipLaunchKernelGGL(k_blend_collect<64>, dim3(b.blend_bins * 16), dim3(256), 0, s, b); break;
-        default: return hipErrorInvalidValue;
-    }
+    if (b.tile == 32 && b.samples == 1)
+        hipLaunchKernelGGL((k_blend_collect<32, 1>), dim3(b.blend_bins * 4), dim3(256), 0, s, b);
+    else if (b.tile == 64 && b.samples == 1)
+        hipLaunchKernelGGL((k_blend_collect<64, 1>), dim3(b.blend_bins * 16), dim3(256), 0, s, b);
+    else if (b.tile == 32 && b.samples == 4)
+        hipLaunchKernelGGL((k_blend_collect<32, 4>), dim3(b.blend_bins * 4), dim3(256), 0, s, b);
+    else
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_blend_shade, dim3(shade_blocks), dim3(256), 0, s, b, l);
     return hipGetLastError();
 }
 
 hipError_t launch_blend_resolve(const BlendLaunch& b, const ShadeLaunch& l, hipStream_t s) {
     if (b.blend_bins == 0 || l.total_blocks == 0) return hipSuccess;
-    switch (b.tile) {
-        case 32: hipLaunchKernelGGL(k_blend_resolve<32>, dim3(l.total_blocks), dim3(256), 0, s, b, l); break;
-        case 64: hipLaunchKernelGGL(k_blend_resolve<64>, dim3(l.total_blocks), dim3(256), 0, s, b, l); break;
-        default: return hipErrorInvalidValue;
-    }
+    if (b.tile == 32 && b.samples == 1)
+        hipLaunchKernelGGL((k_blend_resolve<32, 1>), dim3(l.total_blocks), dim3(256), 0, s, b, l);
+    else if (b.tile == 64 && b.samples == 1)
+        hipLaunchKernelGGL((k_blend_resolve<64, 1>), dim3(l.total_blocks), dim3(256), 0, s, b, l);
+    else if (b.tile == 32 && b.samples == 4)
+        hipLaunchKernelGGL((k_blend_resolve<32, 4>), dim3(l.total_blocks), dim3(256), 0, s, b, l);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
 hipError_t launch_shade(const ShadeLaunch& l, hipStream_t s) {
     if (l.total_blocks == 0) return hipSuccess;
+    if (l.samples == 4) {
+        if (l.neural)
+            hipLaunchKernelGGL(k_shade_ms<true>, dim3(l.total_blocks), dim3(256), 0, s, l);
+        else
+            hipLaunchKernelGGL(k_shade_ms<false>, dim3(l.total_blocks), dim3(256), 0, s, l);
+        return hipGetLastError();
+    }
     if (l.neural)
         hipLaunchKernelGGL(k_shade<true>, dim3(l.total_blocks), dim3(256), 0, s, l);
     else

@@ -34,8 +34,9 @@ def raster(scene, rs=None, shadow=False, w=W, h=H):
     return out, st
 
 
-def brute_force_mask(pts):
-    """Exact coverage of pixel centres by the triangle with integer sub-pixel vertices (y down)."""
+def brute_force_mask(pts, sample=(128, 128)):
+    """Exact coverage of each pixel's sample point (the centre by default) by the triangle with
+    integer sub-pixel vertices (y down)."""
     (x0, y0), (x1, y1), (x2, y2) = [(Fr(X), Fr(Y)) for X, Y in pts]
     area = (x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0)
     mask = np.zeros((H, W), dtype=bool)
@@ -53,7 +54,7 @@ def brute_force_mask(pts):
 
     for py in range(H):
         for px in range(W):
-            P = (Fr(px * 256 + 128), Fr(py * 256 + 128))
+            P = (Fr(px * 256 + sample[0]), Fr(py * 256 + sample[1]))
             ok = True
             for i in range(3):
                 a, b, c = V[(i + 1) % 3], V[(i + 2) % 3], V[i]
