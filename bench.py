@@ -111,6 +111,40 @@ def present_timing(r, width, height, n=10):
             "achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def msaa_timing(r, scene, plain, neural, args, check):
+    """Graph-replayed frames with 4 samples per camera pixel (bin size 32), both shading forms, and
+    the stage medians of the neural one; with `check`, every sample plane against the oracle."""
+    import ctypes as C
+
+    import numpy as np
+    from rosy_amd import abi
+    rs = abi.RasterState()
+    C.memmove(C.byref(rs), C.byref(scene.s.raster_state), C.sizeof(rs))
+    rs.msaa_samples = 4
+    rs.tile_size = 32
+    r.update(scene.s.scene_data, rs)
+    st = stage_medians(r, neural)
+    exact = None
+    if check:
+        import oracle
+        cam = scene.camera_views()[0]
+        want = oracle.raster_view_msaa(scene, cam.view_projection[:], cam.width, cam.height, rs, oracle.threads())
+        exact = bool(np.array_equal(r.visibility(0), want))
+    r.set_debug_flags(0)
+    out = {}
+    for name, frame in (("plain", plain), ("neural", neural)):
+        r.render_graph(frame, args.warmup)
+        r.render_graph(frame, args.steps)
+        out[name + "_ms_per_frame"] = round(float(r.stats().gpu_ms), 5)
+    out["stage_ms_neural"] = {"geometry": round(float(st[0]), 5), "binning": round(float(st[1]), 5),
+                              "raster": round(float(st[2]), 5), "shade": round(float(st[3]), 5)}
+    out["sample_planes_bit_exact_vs_oracle"] = exact
+    rs.msaa_samples = 0
+    rs.tile_size = scene.s.raster_state.tile_size
+    r.update(scene.s.scene_data, rs)
+    return out
+
+
 def rsy_decode_timing(scene, seconds=2.0):
     """The workload written as a .rsy file (rosy_amd.asset.from_scene) and decoded by rsy_asset_decode
     from host memory, repeated for a bounded time: MB of file and host GB/s (one thread)."""
@@ -146,6 +180,9 @@ def main():
                          "(5: also the blended frame's colour with the FP64 oracle)")
     ap.add_argument("--tile", type=int, default=0, choices=[0, 32, 64],
                     help="bin size in pixels; 0 (default): the autotuner picks it")
+    ap.add_argument("--msaa", type=int, default=1, choices=[1, 4],
+                    help="workloads 1, 2, 4, 5: samples per camera pixel (4: the reference's 4-sample main pass, "
+                         "bin size 32)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 times): the reference kernel that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters")
@@ -223,6 +260,10 @@ def main():
     output_stage = [present_timing(r, 1920, 1080), present_timing(r, 2560, 1440)]
     rsy_decode = rsy_decode_timing(scene)
     log(f"output stage {output_stage}; .rsy decode {rsy_decode}")
+
+    # the same frames with the reference's 4-sample main pass
+    msaa = msaa_timing(r, scene, plain, neural, args, check=not args.no_cpu)
+    log(f"4-sample frames {msaa}")
 
     # raster kernel, algorithmic bytes with the implementation-independent unit costs: one tile-list
     # entry read (4 B) per (triangle, bin) pair, one setup record read (32 B: three edge pairs, the
@@ -314,6 +355,7 @@ def main():
         "roofline": roofline,
         "decoder": decoder,
         "output_stage": output_stage,
+        "msaa4": msaa,
         "rsy_decode": rsy_decode,
         "cpu_baseline": cpu,
         "wall_ms_per_step": round(wall, 5),
@@ -361,7 +403,16 @@ def run_views(args):
     r = rosy_amd.Renderer(device=0)
     r.load(scene)
     plain = scene.frame_desc(abi.SHADE_PLAIN)
-    tile = choose_tile(r, scene, plain, args.tile)
+    if args.msaa == 4:
+        import ctypes as C
+        rs = abi.RasterState()
+        C.memmove(C.byref(rs), C.byref(scene.s.raster_state), C.sizeof(rs))
+        rs.msaa_samples, rs.tile_size = 4, 32
+        r.update(scene.s.scene_data, rs)
+        tile = 32
+    else:
+        tile = choose_tile(r, scene, plain, args.tile)
+    rs_used = scene.s.raster_state if args.msaa == 1 else rs
     frame = plain
     dec = None
     if args.workload == 4:
@@ -379,20 +430,28 @@ def run_views(args):
     blend_check = None
     if args.check:
         import oracle
-        rs = scene.s.raster_state
+        rs = rs_used
         exact = True
         for ci, cs in enumerate(cascs):
             o, _ = oracle.raster_view(scene, cs.light_view_projection[:], cs.size, cs.size, True, rs)
             exact &= bool(np.array_equal(r.shadow_map(ci).view(np.uint32), o.view(np.uint32)))
         for vi, cv in enumerate(cams):
-            o, _ = oracle.raster_view(scene, cv.view_projection[:], cv.width, cv.height, False, rs)
+            if args.msaa == 4:
+                o = oracle.raster_view_msaa(scene, cv.view_projection[:], cv.width, cv.height, rs)
+            else:
+                o, _ = oracle.raster_view(scene, cv.view_projection[:], cv.width, cv.height, False, rs)
             exact &= bool(np.array_equal(r.visibility(vi), o))
         log(f"visibility + shadow maps bit-exact vs oracle: {exact}")
         if args.workload == 5:
             vis = r.visibility(0)
             maps = [r.shadow_map(0)]
-            base, _, _ = oracle.shade_view(scene, cams[0], cascs, maps, vis)
-            want, count = oracle.blend_view(scene, cams[0], cascs, maps, vis, base, max_layers=64)
+            if args.msaa == 4:
+                want, _, counts, _ = oracle.msaa_view(scene, cams[0], cascs, maps, vis, raster_state=rs, blended=True,
+                                                      max_layers=64)
+                count = counts.max(axis=0)
+            else:
+                base, _, _ = oracle.shade_view(scene, cams[0], cascs, maps, vis)
+                want, count = oracle.blend_view(scene, cams[0], cascs, maps, vis, base, max_layers=64)
             blend_check = {"max_abs_err_vs_fp64": float(np.abs(r.color(0).astype(np.float64) - want).max()),
                            "pixels_with_layers": int((count > 0).sum()), "max_layers": int(count.max())}
             log(f"blended frame vs FP64 oracle: {blend_check}")
@@ -407,7 +466,8 @@ def run_views(args):
     out = {"metric": "ms/frame (all views of the frame)", "value": round(ms, 5), "unit": "ms/frame", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": False,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": WORKLOADS[args.workload], "seed": args.seed, "bin_size": tile},
+           "config": {"workload": WORKLOADS[args.workload], "seed": args.seed, "bin_size": tile,
+                      "samples_per_pixel": args.msaa},
            "plain_ms_per_frame": round(plain_ms, 5),
            "mtris_per_s": round(views * scene.n_triangles / (ms * 1e-3) / 1e6, 1),
            "pixels_per_frame": int(sum(c.width * c.height for c in cams) + sum(c.size * c.size for c in cascs)),

@@ -144,21 +144,33 @@ typedef struct rsy_sampler {
 
 #pragma pack(pop)
 
-/* Texture formats.  The reference's BC7 images are decoded formats here (uncompressed, mip 0):
- * color and metallic/mixmap maps are sRGB-encoded (Graphics.cpp:3632-3647), normal maps UNORM. */
+/* Texture formats.  Colour, metallic-roughness and mixmap images are sRGB-encoded, normal maps
+ * UNORM (Graphics.cpp:3632-3647).  The reference's images are BC7 (VK_FORMAT_BC7_SRGB_BLOCK /
+ * VK_FORMAT_BC7_UNORM_BLOCK): BC7 data is decoded on the GPU when it is uploaded and kept as RGBA8
+ * texels. */
 typedef enum rsy_texture_format {
     RSY_TEX_RGBA8_SRGB = 0,
     RSY_TEX_RGBA8_UNORM = 1,
     RSY_TEX_RG8_UNORM = 2,
-    RSY_TEX_RG8_SRGB = 3
+    RSY_TEX_RG8_SRGB = 3,
+    RSY_TEX_BC7_SRGB = 4,  /* 16-byte blocks of 4x4 texels, ceil(w/4) x ceil(h/4) per level */
+    RSY_TEX_BC7_UNORM = 5
 } rsy_texture_format;
 
+/* mip_levels: 0 or 1 = level 0 only; n = n levels in `pixels`, level 0 first, level l being
+ * max(1, width >> l) x max(1, height >> l) (the DDS layout of the reference's images, whose chains
+ * run down to 1 x 1); RSY_MIPS_BUILD = the library builds the full chain from level 0 with the
+ * reference's packager filter (Packager.cpp:216-308: 2x2 box on linear, alpha-premultiplied colour
+ * for sRGB images; on the stored values otherwise).  Sampled with the reference's samplers
+ * (VK_SAMPLER_MIPMAP_MODE_LINEAR, no LOD clamp: Graphics.cpp:4002-4012): trilinear filtering at
+ * the level of detail of the pixel's texture-coordinate derivatives. */
+#define RSY_MIPS_BUILD 0xFFFFFFFFu
 typedef struct rsy_texture_desc {
     uint32_t width;
     uint32_t height;
-    uint32_t format; /* rsy_texture_format */
-    uint32_t reserved;
-    const void* pixels; /* width*height*bytes_per_texel, row-major, row 0 = v 0 */
+    uint32_t format;     /* rsy_texture_format */
+    uint32_t mip_levels;
+    const void* pixels;  /* level 0: width*height*bytes_per_texel (or BC7 blocks), row-major, row 0 = v 0 */
 } rsy_texture_desc;
 
 /* Fixed-function state the reference sets with dynamic state (Graphics.cpp:4988-5045, 5226-5231)
@@ -311,6 +323,10 @@ rsy_result rsy_read_visibility(rsy_ctx* ctx, uint32_t view, uint64_t* out, uint6
 rsy_result rsy_read_color(rsy_ctx* ctx, uint32_t view, float* out_rgba, uint64_t n_pixels);
 /* Shadow-map depth per texel (float, 0 = clear). */
 rsy_result rsy_read_shadow_map(rsy_ctx* ctx, uint32_t cascade, float* out, uint64_t n_texels);
+/* Texels of one mip level of texture `texture` as the sampler reads them (BC7 decoded to RGBA8,
+ * built levels included): max(1, w >> level) * max(1, h >> level) * channels bytes (4 for RGBA8 and
+ * BC7, 2 for RG8).  Counterpart of inspecting the uploaded VkImage; for validation. */
+rsy_result rsy_read_texture(rsy_ctx* ctx, uint32_t texture, uint32_t level, uint8_t* out, uint64_t n_bytes);
 /* Per-pixel shadow factor written when RSY_DEBUG_SHADOW_FACTOR is enabled (else zeros). */
 rsy_result rsy_read_shadow_factor(rsy_ctx* ctx, uint32_t view, float* out, uint64_t n_pixels);
 rsy_result rsy_get_frame_stats(rsy_ctx* ctx, rsy_frame_stats* out);

@@ -62,6 +62,10 @@ def lib():
         L.orc_blend_layers_at.argtypes = [C.POINTER(Geometry), C.POINTER(C.c_float), C.c_uint32, C.c_uint32,
                                           C.POINTER(abi.RasterState), C.c_int32, C.c_int32, C.c_void_p, C.c_uint32,
                                           C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_bc7_decode.restype = C.c_int
+        L.orc_bc7_decode.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+        L.orc_bc7_tables.restype = C.c_int
+        L.orc_bc7_tables.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_count_tile_pairs.restype = C.c_uint64
         L.orc_count_tile_pairs.argtypes = [C.POINTER(Geometry), C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int,
                                            C.POINTER(abi.RasterState), C.c_uint32, C.c_int]
@@ -340,4 +344,22 @@ def msaa_view(scene, view, cascades, shadow_maps, vis_planes, scene_data=None, r
         samples.append(rgba)
     samples = np.stack(samples)
     return samples.mean(axis=0), samples, (np.stack(counts) if blended else None), np.min(margins, axis=0)
+
+
+def bc7_decode(blocks, width, height):
+    """BC7 blocks (bytes, ceil(w/4) * ceil(h/4) * 16) decoded to RGBA8 [h, w, 4]."""
+    b = np.frombuffer(bytes(blocks), np.uint8)
+    assert b.size == ((width + 3) // 4) * ((height + 3) // 4) * 16
+    out = np.zeros((height, width, 4), np.uint8)
+    rc = lib().orc_bc7_decode(b.ctypes.data, width, height, out.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"orc_bc7_decode failed: {rc}")
+    return out
+
+
+def bc7_tables():
+    """(two-subset [64, 16], three-subset [64, 16], anchors [3, 64]) as the decoder uses them."""
+    two, three, anchors = np.zeros((64, 16), np.uint8), np.zeros((64, 16), np.uint8), np.zeros((3, 64), np.uint8)
+    lib().orc_bc7_tables(two.ctypes.data, three.ctypes.data, anchors.ctypes.data)
+    return two, three, anchors
 

@@ -153,6 +153,14 @@ double orc_schlick_fresnel(double v_dot_h, double f0);       /* lighting.slang:5
  * RSY_PCF_REFERENCE_5TAP, 3x3 one-texel grid for RSY_PCF_GRID_3X3). */
 double orc_pcf(const float* map, uint32_t size, double u, double v, double ref, uint32_t mode, double* margin);
 
+/* BC7 blocks of a width x height image (ceil(w/4) x ceil(h/4) blocks of 16 bytes, row-major)
+ * decoded to RGBA8 (R first); texels past the image's edge are dropped. */
+int orc_bc7_decode(const uint8_t* blocks, uint32_t width, uint32_t height, uint8_t* rgba);
+/* The partition tables as subset per texel (64 x 16 two-subset, 64 x 16 three-subset) and the
+ * anchor texels (64 of subset 1 in two-subset partitions, then 64 of subset 1 and 64 of subset 2
+ * in three-subset ones). */
+int orc_bc7_tables(uint8_t* two, uint8_t* three, uint8_t* anchors);
+
 #ifdef __cplusplus
 }
 #endif

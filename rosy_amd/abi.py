@@ -24,7 +24,8 @@ RSY_ABI_VERSION = 1
 SHADE_NONE, SHADE_PLAIN, SHADE_NEURAL = 0, 1, 2
 PCF_REFERENCE_5TAP, PCF_GRID_3X3 = 0, 1
 DEBUG_SHADOW_FACTOR, DEBUG_STAGE_TIMING = 1, 2
-TEX_RGBA8_SRGB, TEX_RGBA8_UNORM, TEX_RG8_UNORM, TEX_RG8_SRGB = 0, 1, 2, 3
+TEX_RGBA8_SRGB, TEX_RGBA8_UNORM, TEX_RG8_UNORM, TEX_RG8_SRGB, TEX_BC7_SRGB, TEX_BC7_UNORM = 0, 1, 2, 3, 4, 5
+MIPS_BUILD = 0xFFFFFFFF
 
 F16 = C.c_float * 16
 F4 = C.c_float * 4
@@ -85,7 +86,7 @@ class Sampler(C.Structure):
 
 
 class TextureDesc(C.Structure):
-    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32), ("reserved", C.c_uint32),
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32), ("mip_levels", C.c_uint32),
                 ("pixels", C.c_void_p)]
 
 
@@ -239,6 +240,7 @@ PROTOTYPES = [
     ("rsy_read_visibility", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
     ("rsy_read_color", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
     ("rsy_read_shadow_map", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
+    ("rsy_read_texture", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint64]),
     ("rsy_read_shadow_factor", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
     ("rsy_get_frame_stats", C.c_int, [C.c_void_p, C.POINTER(FrameStats)]),
     ("rsy_get_device_outputs", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),

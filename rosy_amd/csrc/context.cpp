@@ -810,14 +810,24 @@ rsy_result rsy_set_materials(rsy_ctx* c, const rsy_gpu_material* materials, uint
     if ((n_materials && !materials) || (n_textures && !textures) || (n_samplers && !samplers))
         return fail(c, RSY_INVALID_ARGUMENT, "null material pointer");
     std::vector<TexGPU> table(n_textures);
-    uint64_t pool = 0;
+    uint64_t pool = 0, staging = 0;
     for (uint32_t t = 0; t < n_textures; ++t) {
         const rsy_texture_desc& d = textures[t];
-        if (d.width == 0 || d.height == 0 || d.width > 16384 || d.height > 16384 || !d.pixels || d.format > RSY_TEX_RG8_SRGB)
+        if (d.width == 0 || d.height == 0 || d.width > 16384 || d.height > 16384 || !d.pixels || d.format > RSY_TEX_BC7_UNORM)
             return fail(c, RSY_INVALID_ARGUMENT, "invalid texture description");
+        const uint32_t full = ilog2(std::max(d.width, d.height)) + 1;  // levels down to 1 x 1
+        const bool build = d.mip_levels == RSY_MIPS_BUILD;
+        const uint32_t given = build ? 1u : std::max(1u, d.mip_levels);
+        if (given > full) return fail(c, RSY_INVALID_ARGUMENT, "more mip levels than the image has");
+        const bool bc7 = d.format == RSY_TEX_BC7_SRGB || d.format == RSY_TEX_BC7_UNORM;
         const uint32_t ch = (d.format == RSY_TEX_RG8_UNORM || d.format == RSY_TEX_RG8_SRGB) ? 2 : 4;
-        table[t] = {pool, d.width, d.height, ch, (d.format == RSY_TEX_RGBA8_SRGB || d.format == RSY_TEX_RG8_SRGB) ? 1u : 0u};
-        pool += (uint64_t(d.width) * d.height * ch + 255) & ~uint64_t(255);
+        const uint32_t srgb = d.format == RSY_TEX_RGBA8_SRGB || d.format == RSY_TEX_RG8_SRGB || d.format == RSY_TEX_BC7_SRGB;
+        table[t] = {pool, d.width, d.height, ch, srgb, build ? full : given, 0u};
+        pool += (mip_offset(d.width, d.height, ch, table[t].levels) + 255) & ~uint64_t(255);
+        if (bc7)
+            for (uint32_t l = 0; l < given; ++l)
+                staging = std::max<uint64_t>(staging, uint64_t((mip_extent(d.width, l) + 3) / 4) *
+                                                          ((mip_extent(d.height, l) + 3) / 4) * 16);
     }
     auto check_img = [&](uint32_t i) { return i == RSY_ABSENT || i < n_textures; };
     for (uint32_t m = 0; m < n_materials; ++m) {
@@ -834,9 +844,37 @@ rsy_result rsy_set_materials(rsy_ctx* c, const rsy_gpu_material* materials, uint
     HIP_TRY(c, c->samplers.ensure(std::max<size_t>(1, n_samplers) * sizeof(rsy_sampler)), "allocate samplers");
     if (n_materials) HIP_TRY(c, hipMemcpy(c->materials.p, materials, n_materials * sizeof(rsy_gpu_material), hipMemcpyHostToDevice), "upload materials");
     if (n_textures) HIP_TRY(c, hipMemcpy(c->tex_table.p, table.data(), n_textures * sizeof(TexGPU), hipMemcpyHostToDevice), "upload textures");
-    for (uint32_t t = 0; t < n_textures; ++t)
-        HIP_TRY(c, hipMemcpy(c->texels.as<uint8_t>() + table[t].offset, textures[t].pixels,
-                             uint64_t(table[t].width) * table[t].height * table[t].channels, hipMemcpyHostToDevice), "upload texels");
+    if (staging) HIP_TRY(c, c->tex_staging.ensure(staging), "allocate texture staging");
+    for (uint32_t t = 0; t < n_textures; ++t) {
+        // the given levels (BC7 decoded on the GPU), then the built ones, each from the one above
+        const rsy_texture_desc& d = textures[t];
+        const TexGPU& g = table[t];
+        const bool bc7 = d.format == RSY_TEX_BC7_SRGB || d.format == RSY_TEX_BC7_UNORM;
+        const uint32_t given = d.mip_levels == RSY_MIPS_BUILD ? 1u : std::max(1u, d.mip_levels);
+        const uint8_t* src = static_cast<const uint8_t*>(d.pixels);
+        uint8_t* base = c->texels.as<uint8_t>() + g.offset;
+        for (uint32_t l = 0; l < given; ++l) {
+            const uint32_t w = mip_extent(g.width, l), h = mip_extent(g.height, l);
+            uint8_t* dst = base + mip_offset(g.width, g.height, g.channels, l);
+            if (bc7) {
+                const uint64_t n = uint64_t((w + 3) / 4) * ((h + 3) / 4) * 16;
+                HIP_TRY(c, hipMemcpy(c->tex_staging.p, src, n, hipMemcpyHostToDevice), "upload BC7 blocks");
+                HIP_TRY(c, launch_bc7_decode(c->tex_staging.p, w, h, reinterpret_cast<uint32_t*>(dst), c->stream),
+                        "BC7 decode");
+                HIP_TRY(c, hipStreamSynchronize(c->stream), "BC7 decode");
+                src += n;
+            } else {
+                const uint64_t n = uint64_t(w) * h * g.channels;
+                HIP_TRY(c, hipMemcpy(dst, src, n, hipMemcpyHostToDevice), "upload texels");
+                src += n;
+            }
+        }
+        for (uint32_t l = given; l < g.levels; ++l)
+            HIP_TRY(c, launch_build_mip(base + mip_offset(g.width, g.height, g.channels, l - 1), mip_extent(g.width, l - 1),
+                                        mip_extent(g.height, l - 1), base + mip_offset(g.width, g.height, g.channels, l),
+                                        g.channels, g.srgb, c->stream), "build mip level");
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
     if (n_samplers) HIP_TRY(c, hipMemcpy(c->samplers.p, samplers, n_samplers * sizeof(rsy_sampler), hipMemcpyHostToDevice), "upload samplers");
     c->n_materials = n_materials;
     c->n_textures = n_textures;
@@ -847,10 +885,19 @@ rsy_result rsy_set_materials(rsy_ctx* c, const rsy_gpu_material* materials, uint
     c->h_tex_table = table;
     c->h_samplers.assign(samplers, samplers + n_samplers);
     c->tex_opaque.assign(n_textures, 0);
+    c->any_mips = false;
+    std::vector<uint8_t> decoded;
     for (uint32_t t = 0; t < n_textures; ++t) {
+        c->any_mips |= table[t].levels > 1;
         if (table[t].channels != 4) continue;
-        const uint8_t* px = static_cast<const uint8_t*>(textures[t].pixels);
         const uint64_t n = uint64_t(table[t].width) * table[t].height;
+        const uint8_t* px = static_cast<const uint8_t*>(textures[t].pixels);
+        if (textures[t].format == RSY_TEX_BC7_SRGB || textures[t].format == RSY_TEX_BC7_UNORM) {
+            decoded.resize(n * 4);  // level 0 as decoded
+            HIP_TRY(c, hipMemcpy(decoded.data(), c->texels.as<uint8_t>() + table[t].offset, n * 4, hipMemcpyDeviceToHost),
+                    "read decoded texels");
+            px = decoded.data();
+        }
         bool opaque = true;
         for (uint64_t i = 0; i < n && opaque; ++i) opaque = px[4 * i + 3] == 255;
         c->tex_opaque[t] = opaque ? 1 : 0;
@@ -1090,6 +1137,20 @@ rsy_result rsy_read_shadow_map(rsy_ctx* c, uint32_t cascade, float* out, uint64_
     const ViewGPU& g = c->layout.vg[cascade];  // shadow views come first
     if (n_texels != uint64_t(g.width) * g.height) return fail(c, RSY_INVALID_ARGUMENT, "texel count mismatch");
     HIP_TRY(c, hipMemcpy(out, c->shadow_maps[cascade].p, n_texels * 4, hipMemcpyDeviceToHost), "read shadow map");
+    return RSY_OK;
+}
+
+rsy_result rsy_read_texture(rsy_ctx* c, uint32_t texture, uint32_t level, uint8_t* out, uint64_t n_bytes) {
+    if (!c || !out) return RSY_INVALID_ARGUMENT;
+    if (texture >= c->n_textures) return fail(c, RSY_INVALID_ARGUMENT, "texture index out of range");
+    const TexGPU& g = c->h_tex_table[texture];
+    if (level >= g.levels) return fail(c, RSY_INVALID_ARGUMENT, "mip level out of range");
+    const uint64_t n = uint64_t(mip_extent(g.width, level)) * mip_extent(g.height, level) * g.channels;
+    if (n_bytes != n) return fail(c, RSY_INVALID_ARGUMENT, "byte count mismatch");
+    HIP_TRY(c, hipSetDevice(c->device), "set device");
+    HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
+    HIP_TRY(c, hipMemcpy(out, c->texels.as<uint8_t>() + g.offset + mip_offset(g.width, g.height, g.channels, level), n,
+                         hipMemcpyDeviceToHost), "read texture");
     return RSY_OK;
 }
 

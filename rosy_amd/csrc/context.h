@@ -160,6 +160,8 @@ struct rsy_ctx {
     std::vector<rsy::TexGPU> h_tex_table;
     std::vector<rsy_sampler> h_samplers;
     std::vector<uint8_t> tex_opaque;  // four-channel texture whose alpha is 255 everywhere
+    rsy::DevBuf tex_staging;          // BC7 blocks of one level on their way to the decoder
+    bool any_mips = false;            // some texture has more than one level (LOD sampling)
     // draw lists: 0 = shadow (all), 1 = opaque (!blended), 2 = blended (the blended pass)
     std::vector<rsy::DrawGPU> lists[3];
     std::vector<uint32_t> list_index[3];  // surface indices

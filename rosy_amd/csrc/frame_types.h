@@ -77,11 +77,23 @@ struct BlockEntry {  // work of one 256-thread block of the transform / setup ke
 };
 
 struct TexGPU {
-    uint64_t offset;  // bytes into the texel pool
+    uint64_t offset;  // bytes into the texel pool (level 0; the levels follow one another)
     uint32_t width, height;
     uint32_t channels;  // 2 or 4
     uint32_t srgb;
+    uint32_t levels;    // mip levels (1: level 0 only)
+    uint32_t pad;
 };
+
+// Size of mip level l of a width x height image.
+__host__ __device__ inline uint32_t mip_extent(uint32_t e, uint32_t l) { return e >> l ? e >> l : 1u; }
+
+// Bytes into a texture's texels of mip level l.
+__host__ __device__ inline uint64_t mip_offset(uint32_t w, uint32_t h, uint32_t ch, uint32_t l) {
+    uint64_t o = 0;
+    for (uint32_t k = 0; k < l; ++k) o += uint64_t(mip_extent(w, k)) * mip_extent(h, k) * ch;
+    return o;
+}
 
 // The standard locations of a 4-sample pixel, (0.375, 0.125), (0.875, 0.375), (0.125, 0.625) and
 // (0.625, 0.875), in 1/256 pixel.

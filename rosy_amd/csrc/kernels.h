@@ -236,6 +236,10 @@ struct NmPointLaunch {
     float* out;  // [n][7]
 };
 
+// texture upload: BC7 blocks of a width x height image into RGBA8 texels; mip level l + 1 from level l
+hipError_t launch_bc7_decode(const void* blocks, uint32_t width, uint32_t height, uint32_t* rgba, hipStream_t s);
+hipError_t launch_build_mip(const uint8_t* src, uint32_t src_w, uint32_t src_h, uint8_t* dst, uint32_t channels,
+                            uint32_t srgb, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
 hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s);
 hipError_t launch_binning(const BinLaunch& b, hipStream_t s);
