@@ -591,7 +591,7 @@ rsy_result rsy_asset_pack_materials(const rsy_asset* asset, rsy_gpu_material* ma
  * ------------------------------------------------------------------------------------------- */
 typedef struct rsy_scene_desc {
     uint32_t workload;        /* 0 sphere, 1 scene-scale, 2 batched views, 3 learned decoder, 4 combined */
-    uint32_t reserved;
+    uint32_t texture_mips;    /* 1: every texture gets its full mip chain (RSY_MIPS_BUILD); 0: level 0 only */
     uint64_t seed;
     /* overrides, 0 = the workload's default */
     uint32_t width;

@@ -248,7 +248,8 @@ def shade_view(scene, view, cascades, shadow_maps, vis, scene_data=None, raster_
     rs = raster_state if raster_state is not None else s.raster_state
     cv = abi.CameraView()
     C.memmove(C.byref(cv), C.byref(view), C.sizeof(abi.CameraView))
-    si = ShadeIn(s.materials, s.n_materials, s.n_textures, s.textures, s.samplers, s.n_samplers, len(cascades),
+    si = ShadeIn(s.materials, s.n_materials, s.n_textures, stored_textures(scene), s.samplers, s.n_samplers,
+                 len(cascades),
                  C.cast(casc, C.POINTER(abi.ShadowCascade)), C.cast(ptrs, C.POINTER(C.POINTER(C.c_float))),
                  C.pointer(sd), C.pointer(rs), C.pointer(cv))
     h, w = vis.shape
@@ -362,4 +363,44 @@ def bc7_tables():
     two, three, anchors = np.zeros((64, 16), np.uint8), np.zeros((64, 16), np.uint8), np.zeros((3, 64), np.uint8)
     lib().orc_bc7_tables(two.ctypes.data, three.ctypes.data, anchors.ctypes.data)
     return two, three, anchors
+
+
+def stored_textures(scene):
+    """The scene's textures as the sampler reads them, for the oracle's shading: BC7 images decoded
+    by the oracle's decoder and chains to build (mip_levels = MIPS_BUILD) built by the oracle's
+    packager filter (oracle/textures.py), as RGBA8 / RG8 with every level given.  Cached on the
+    scene."""
+    cached = getattr(scene, "_oracle_textures", None)
+    if cached is not None:
+        return cached[0]
+    from . import textures as tx
+    s = scene.s
+    n = s.n_textures
+    out = (abi.TextureDesc * max(1, n))()
+    keep = []
+    for i in range(n):
+        d = s.textures[i]
+        bc7 = d.format in (abi.TEX_BC7_SRGB, abi.TEX_BC7_UNORM)
+        build = d.mip_levels == abi.MIPS_BUILD
+        if not bc7 and not build:
+            out[i] = d
+            continue
+        ch = 2 if d.format in (abi.TEX_RG8_UNORM, abi.TEX_RG8_SRGB) else 4
+        srgb = d.format in (abi.TEX_RGBA8_SRGB, abi.TEX_RG8_SRGB, abi.TEX_BC7_SRGB)
+        given = 1 if build else max(1, d.mip_levels)
+        levels, at = [], 0
+        for l in range(given):
+            w, h = max(1, d.width >> l), max(1, d.height >> l)
+            nbytes = ((w + 3) // 4) * ((h + 3) // 4) * 16 if bc7 else w * h * ch
+            raw = C.string_at(d.pixels + at, nbytes)
+            at += nbytes
+            levels.append(bc7_decode(raw, w, h) if bc7 else np.frombuffer(raw, np.uint8).reshape(h, w, ch))
+        if build:
+            levels = tx.mip_chain(levels[0], srgb)
+        data = np.concatenate([np.ascontiguousarray(v).reshape(-1) for v in levels])
+        keep.append(data)
+        fmt = d.format if not bc7 else (abi.TEX_RGBA8_SRGB if srgb else abi.TEX_RGBA8_UNORM)
+        out[i] = abi.TextureDesc(d.width, d.height, fmt, len(levels), data.ctypes.data)
+    scene._oracle_textures = (out, keep)
+    return out
 

@@ -55,10 +55,22 @@ bool valid_index(uint32_t i) { return i < 4294967295u; }  // validDescriptorInde
 
 double srgb_eotf(double c) { return c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4); }
 
+// A texture level: texels of level `level` of the description (levels stored one after another
+// from level 0, as mip_levels says; 0 or 1 = level 0 only).
 struct Tex {
     const rsy_texture_desc* d;
     int channels;
     bool srgb;
+    int level = 0;
+    int levels() const { return d->mip_levels > 1 ? int(d->mip_levels) : 1; }
+    int width() const { return std::max(1, int(d->width) >> level); }
+    int height() const { return std::max(1, int(d->height) >> level); }
+    const uint8_t* texels() const {
+        size_t o = 0;
+        for (int l = 0; l < level; ++l)
+            o += size_t(std::max(1, int(d->width) >> l)) * std::max(1, int(d->height) >> l) * channels;
+        return static_cast<const uint8_t*>(d->pixels) + o;
+    }
 };
 
 int wrap_coord(int i, int n, uint16_t mode) {
@@ -72,7 +84,7 @@ int wrap_coord(int i, int n, uint16_t mode) {
 }
 
 D4 texel(const Tex& t, int x, int y) {
-    const uint8_t* p = static_cast<const uint8_t*>(t.d->pixels) + (size_t(y) * t.d->width + x) * t.channels;
+    const uint8_t* p = t.texels() + (size_t(y) * t.width() + x) * t.channels;
     D4 c{0, 0, 0, 1};
     double* o = &c.x;
     for (int k = 0; k < t.channels; ++k) {
@@ -82,10 +94,10 @@ D4 texel(const Tex& t, int x, int y) {
     return c;
 }
 
-D4 sample(const Tex& t, const rsy_sampler* s, double u, double v) {
+D4 sample(const Tex& t, const rsy_sampler* s, double u, double v, bool minify = false) {
     const uint16_t ws = s ? s->wrap_s : 2, wt = s ? s->wrap_t : 2;
-    const bool nearest = s && s->mag_filter <= 2;
-    const int W = int(t.d->width), H = int(t.d->height);
+    const bool nearest = s && (minify ? s->min_filter : s->mag_filter) <= 2;
+    const int W = t.width(), H = t.height();
     if (nearest) {
         const int x = wrap_coord(int(std::floor(u * W)), W, ws), y = wrap_coord(int(std::floor(v * H)), H, wt);
         return texel(t, x, y);
@@ -106,6 +118,27 @@ D4 sample(const Tex& t, const rsy_sampler* s, double u, double v) {
     for (int k = 0; k < 4; ++k)
         o[k] = (1 - ax) * (1 - ay) * p00[k] + ax * (1 - ay) * p10[k] + (1 - ax) * ay * p01[k] + ax * ay * p11[k];
     return r;
+}
+
+// Vulkan's level of detail with mipmapMode LINEAR (the asset samplers, Graphics.cpp:4002-4012; no
+// bias, no clamp): rho_x = |(du/dx W, dv/dx H)|, rho_y likewise, lambda = log2(max(rho_x, rho_y));
+// lambda <= 0 magnifies from level 0, otherwise d = min(lambda, levels - 1) blends levels floor(d)
+// and floor(d) + 1 (the last level past it) by frac(d) with the minification filter.
+D4 sample_lod(const Tex& t, const rsy_sampler* s, double u, double v, double dudx, double dvdx, double dudy,
+              double dvdy) {
+    if (t.levels() <= 1) return sample(t, s, u, v);
+    const double W = t.d->width, H = t.d->height;
+    const double rho = std::max(std::hypot(dudx * W, dvdx * H), std::hypot(dudy * W, dvdy * H));
+    const double lambda = std::log2(rho);
+    if (!(lambda > 0.0)) return sample(t, s, u, v);
+    const double d = std::min(lambda, double(t.levels() - 1));
+    const int lo = int(std::floor(d)), hi = std::min(lo + 1, t.levels() - 1);
+    const double f = d - std::floor(d);
+    Tex a = t, b = t;
+    a.level = lo;
+    b.level = hi;
+    const D4 ca = sample(a, s, u, v, true), cb = sample(b, s, u, v, true);
+    return {ca.x + (cb.x - ca.x) * f, ca.y + (cb.y - ca.y) * f, ca.z + (cb.z - ca.z) * f, ca.w + (cb.w - ca.w) * f};
 }
 
 // One bilinear compare tap (SampleCmpLevelZero, GREATER_OR_EQUAL, CLAMP_TO_EDGE).
@@ -298,7 +331,9 @@ void shade_pixel(const Ctx& c, uint32_t px, uint32_t py, uint64_t vis, double ou
         clip[k] = mul4(view.view_projection, mul4(gd.transform, p));
         c32[k] = orc::transform(mvp, vin[k]->vertex);
     }
-    double w3[3];
+    // With them, their derivatives along the pixel's x and y for the texture level of detail:
+    // both forms are w3 = n / S with n linear in the pixel position over w, so dw3 = (dn - w3 dS) / S.
+    double w3[3], dx3[3], dy3[3];
     if (orc::all_inside(c32)) {
         int32_t X[3], Y[3];
         float zs;
@@ -312,16 +347,42 @@ void shade_pixel(const Ctx& c, uint32_t px, uint32_t py, uint64_t vis, double ou
         w3[0] = f0 / s;
         w3[1] = f1 / s;
         w3[2] = f2 / s;
+        // e_k is linear in (Px, Py); one pixel is 256 sub-pixel units
+        const double gx[3] = {256.0 * (Y[1] - Y[2]) / clip[0].w, 256.0 * (Y[2] - Y[0]) / clip[1].w,
+                              256.0 * (Y[0] - Y[1]) / clip[2].w};
+        const double gy[3] = {256.0 * (X[2] - X[1]) / clip[0].w, 256.0 * (X[0] - X[2]) / clip[1].w,
+                              256.0 * (X[1] - X[0]) / clip[2].w};
+        const double sx = gx[0] + gx[1] + gx[2], sy = gy[0] + gy[1] + gy[2];
+        for (int k = 0; k < 3; ++k) {
+            dx3[k] = (gx[k] - w3[k] * sx) / s;
+            dy3[k] = (gy[k] - w3[k] * sy) / s;
+        }
     } else {
         const double u = (px + 0.5) / view.width * 2.0 - 1.0, v = (py + 0.5) / view.height * 2.0 - 1.0;
         const D3 a{clip[0].x - u * clip[0].w, clip[1].x - u * clip[1].w, clip[2].x - u * clip[2].w};
         const D3 b{clip[0].y - v * clip[0].w, clip[1].y - v * clip[1].w, clip[2].y - v * clip[2].w};
-        D3 lam = cross(a, b);
-        lam = lam * (1.0 / (lam.x + lam.y + lam.z));
-        w3[0] = lam.x;
-        w3[1] = lam.y;
-        w3[2] = lam.z;
+        const D3 n = cross(a, b);
+        const double s = n.x + n.y + n.z;
+        // du/dx = 2 / width, dv/dy = 2 / height
+        const D3 da{-clip[0].w * 2.0 / view.width, -clip[1].w * 2.0 / view.width, -clip[2].w * 2.0 / view.width};
+        const D3 db{-clip[0].w * 2.0 / view.height, -clip[1].w * 2.0 / view.height, -clip[2].w * 2.0 / view.height};
+        const D3 nx = cross(da, b), ny = cross(a, db);
+        const double n3[3] = {n.x, n.y, n.z}, nx3[3] = {nx.x, nx.y, nx.z}, ny3[3] = {ny.x, ny.y, ny.z};
+        for (int k = 0; k < 3; ++k) w3[k] = n3[k] / s;
+        for (int k = 0; k < 3; ++k) {
+            dx3[k] = (nx3[k] - w3[k] * (nx.x + nx.y + nx.z)) / s;
+            dy3[k] = (ny3[k] - w3[k] * (ny.x + ny.y + ny.z)) / s;
+        }
     }
+    double duv[4] = {0, 0, 0, 0};  // du/dx, dv/dx, du/dy, dv/dy
+    for (int k = 0; k < 3; ++k) {
+        const double tu = vin[k]->texture_coordinates[0], tv = vin[k]->texture_coordinates[1];
+        duv[0] += dx3[k] * tu;
+        duv[1] += dx3[k] * tv;
+        duv[2] += dy3[k] * tu;
+        duv[3] += dy3[k] * tv;
+    }
+
     const double view_depth = clip[0].w * w3[0] + clip[1].w * w3[1] + clip[2].w * w3[2];
 
     // cascade selection by view depth against the split distances
@@ -385,7 +446,7 @@ void shade_pixel(const Ctx& c, uint32_t px, uint32_t py, uint64_t vis, double ou
             const uint32_t nti = md->normal_sampled_image_index;
             if (sd.normal_maps_enabled == 1 && valid_index(nti) && nti < c.tex.size()) {
                 const rsy_sampler* smp = md->normal_sampler_index < in.n_samplers ? &in.samplers[md->normal_sampler_index] : nullptr;
-                const D4 sm = sample(c.tex[nti], smp, bv.u, bv.v);
+                const D4 sm = sample_lod(c.tex[nti], smp, bv.u, bv.v, duv[0], duv[1], duv[2], duv[3]);
                 D3 m{sm.x * 2 - 1, sm.y * 2 - 1, sm.z * 2 - 1};
                 if (c.tex[nti].channels == 2) m.z = std::sqrt(std::max(0.0, 1.0 - m.x * m.x - m.y * m.y));
                 const D3 n = normalize(bv.normal);
@@ -396,16 +457,18 @@ void shade_pixel(const Ctx& c, uint32_t px, uint32_t py, uint64_t vis, double ou
             const uint32_t cti = md->color_sampled_image_index;
             if (valid_index(cti) && cti < c.tex.size()) {
                 const rsy_sampler* smp = md->color_sampler_index < in.n_samplers ? &in.samplers[md->color_sampler_index] : nullptr;
-                image_color = sample(c.tex[cti], smp, bv.u, bv.v);
+                image_color = sample_lod(c.tex[cti], smp, bv.u, bv.v, duv[0], duv[1], duv[2], duv[3]);
                 alpha = std::min(image_color.w, alpha);
             }
             const uint32_t mixti = md->mixmap_sampled_image_index, metti = md->metallic_sampled_image_index;
             if (valid_index(mixti) && mixti < c.tex.size()) {
                 const rsy_sampler* smp = md->mixmap_sampler_index < in.n_samplers ? &in.samplers[md->mixmap_sampler_index] : nullptr;
-                mr_from(c.tex[mixti], sample(c.tex[mixti], smp, bv.u, bv.v), sampled_rough, sampled_metal);
+                mr_from(c.tex[mixti], sample_lod(c.tex[mixti], smp, bv.u, bv.v, duv[0], duv[1], duv[2], duv[3]), sampled_rough,
+                        sampled_metal);
             } else if (valid_index(metti) && metti < c.tex.size()) {
                 const rsy_sampler* smp = md->metallic_sampler_index < in.n_samplers ? &in.samplers[md->metallic_sampler_index] : nullptr;
-                mr_from(c.tex[metti], sample(c.tex[metti], smp, bv.u, bv.v), sampled_rough, sampled_metal);
+                mr_from(c.tex[metti], sample_lod(c.tex[metti], smp, bv.u, bv.v, duv[0], duv[1], duv[2], duv[3]), sampled_rough,
+                        sampled_metal);
             }
         }
         // neither map present: the reference reads an invalid descriptor (basic.slang:211-213);
@@ -510,7 +573,10 @@ static int shade_view(const orc_geometry* g, const orc_shade_in* in, const orc_n
         base += g->draws[d].index_count / 3;
     }
     for (uint32_t t = 0; t < in->n_textures; ++t) {
+        // the oracle samples stored texels: BC7 images and built chains come decoded / built, with
+        // their levels given
         const rsy_texture_desc& d = in->textures[t];
+        if (d.format > RSY_TEX_RG8_SRGB || d.mip_levels == RSY_MIPS_BUILD) return RSY_INVALID_ARGUMENT;
         Tex x{&d, (d.format == RSY_TEX_RG8_UNORM || d.format == RSY_TEX_RG8_SRGB) ? 2 : 4,
               d.format == RSY_TEX_RGBA8_SRGB || d.format == RSY_TEX_RG8_SRGB};
         c.tex.push_back(x);

@@ -126,7 +126,7 @@ class FrameStats(C.Structure):
 
 
 class SceneDesc(C.Structure):
-    _fields_ = [("workload", C.c_uint32), ("reserved", C.c_uint32), ("seed", C.c_uint64), ("width", C.c_uint32),
+    _fields_ = [("workload", C.c_uint32), ("texture_mips", C.c_uint32), ("seed", C.c_uint64), ("width", C.c_uint32),
                 ("height", C.c_uint32), ("shadow_size", C.c_uint32), ("total_triangles", C.c_uint32),
                 ("n_meshes", C.c_uint32), ("n_materials", C.c_uint32), ("texture_size", C.c_uint32),
                 ("n_translucent", C.c_uint32)]

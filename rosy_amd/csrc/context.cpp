@@ -497,6 +497,7 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
         sh.srgb_lut = c->srgb_lut.as<float>();
         sh.xv = g.xv;
         sh.samples = msaa_of(c);
+        sh.lod = c->any_mips ? 1u : 0u;
         if (L.shade == RSY_SHADE_NEURAL) neural_shade_args(c, sh);
         // the blended pass: the blended list of every camera view in primitive order
         BlendLaunch bl{};

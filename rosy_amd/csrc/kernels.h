@@ -153,6 +153,7 @@ struct ShadeLaunch {
     const float* srgb_lut;      // 256 entries
     const float4* xv;           // transformed vertices of all views
     uint32_t samples;           // samples per pixel of the camera views (1 or 4)
+    uint32_t lod;               // 1: some texture has mip levels (trilinear sampling at the pixel's LOD)
     // material decoder (shade mode NEURAL only)
     const uint32_t* nm_slot;    // per material: latent grid slot or 0xFFFFFFFF
     const uint32_t* nm_active;  // per material: 1 = decoded by the network, 0 = plain textures

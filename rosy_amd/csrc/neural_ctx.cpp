@@ -67,14 +67,16 @@ uint64_t splitmix(uint64_t& s) {
 }
 float uniform(uint64_t& s, float lo, float hi) { return lo + (hi - lo) * float(double(splitmix(s) >> 11) * 0x1.0p-53); }
 
-// A material is decodable when its three maps are the formats the decoder reproduces and every
-// one of them is sampled bilinearly with REPEAT addressing (the sampler state the training
-// targets use): opaque four-channel albedo, two-channel normal map, two-channel metallic-roughness
-// (mixmap when present, as basic.slang:198-221 prefers it).
+// A material is decodable when its three maps are the formats the decoder reproduces, single-level
+// (the decoder reproduces level 0 and has no level of detail), and every one of them is sampled
+// bilinearly with REPEAT addressing (the sampler state the training targets use): opaque
+// four-channel albedo, two-channel normal map, two-channel metallic-roughness (mixmap when present,
+// as basic.slang:198-221 prefers it).
 bool eligible(const rsy_ctx* c, uint32_t m, NmSlot& out) {
     const rsy_gpu_material& md = c->h_materials[m];
     auto usable = [&](uint32_t img, uint32_t smp, uint32_t channels) {
         if (img == RSY_ABSENT || img >= c->n_textures || c->h_tex_table[img].channels != channels) return false;
+        if (c->h_tex_table[img].levels > 1) return false;
         if (smp < c->n_samplers) {
             const rsy_sampler& s = c->h_samplers[smp];
             if (s.mag_filter <= 2 || s.wrap_s != 2 || s.wrap_t != 2) return false;

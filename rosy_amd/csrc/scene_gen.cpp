@@ -684,9 +684,10 @@ rsy_result rsy_scene_generate(const rsy_scene_desc* desc, rsy_scene** out_scene)
             const bool translucent = m < desc->n_translucent;
             make_material_textures(s->tex_store[m], int(wl.tex_size), tex_rng.fork(m + 1), wl.stripes, translucent);
             const uint32_t base = uint32_t(s->v_textures.size());
-            s->v_textures.push_back({wl.tex_size, wl.tex_size, RSY_TEX_RGBA8_SRGB, 0, s->tex_store[m].albedo.data()});
-            s->v_textures.push_back({wl.tex_size, wl.tex_size, RSY_TEX_RG8_UNORM, 0, s->tex_store[m].normal.data()});
-            s->v_textures.push_back({wl.tex_size, wl.tex_size, RSY_TEX_RG8_SRGB, 0, s->tex_store[m].mr.data()});
+            const uint32_t mips = desc->texture_mips ? RSY_MIPS_BUILD : 0u;
+            s->v_textures.push_back({wl.tex_size, wl.tex_size, RSY_TEX_RGBA8_SRGB, mips, s->tex_store[m].albedo.data()});
+            s->v_textures.push_back({wl.tex_size, wl.tex_size, RSY_TEX_RG8_UNORM, mips, s->tex_store[m].normal.data()});
+            s->v_textures.push_back({wl.tex_size, wl.tex_size, RSY_TEX_RG8_SRGB, mips, s->tex_store[m].mr.data()});
             rsy_gpu_material g{};
             g.color[0] = g.color[1] = g.color[2] = g.color[3] = 1.f;
             g.metallic_factor = 1.f;

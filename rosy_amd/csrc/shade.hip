@@ -201,9 +201,13 @@ struct PixelGeom {
     uint32_t draw;        // index in the opaque draw list
     uint32_t i0, i1, i2;  // mesh-local vertex indices
     f3 lam;               // perspective-correct barycentrics at the pixel centre
+    f3 dlx, dly;          // with kLod: their derivatives along the pixel's x and y
     int cascade;
 };
 
+// kLod: also the barycentrics' screen derivatives (for the texture level of detail), exact for the
+// same rational form: lam = n / S with n linear over w, so d(lam) = (dn - lam dS) / S.
+template <bool kLod = false>
 __device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView& V, const uint32_t* tri_base,
                                            uint32_t x, uint32_t y, unsigned long long vis) {
     PixelGeom pg;
@@ -246,6 +250,14 @@ __device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView
         const bool swapped = (id & kRecSwapped) != 0u;
         const float b1 = swapped ? e2 : e1, b2 = swapped ? e1 : e2;
         lam = mk(e0 * (w1 * w2), b1 * (w0 * w2), b2 * (w0 * w1));
+        if constexpr (kLod) {
+            // d e_k per pixel step (256 sub-pixel units), vertices in the record's order
+            const float ex0 = 256.0f * float(Y1 - Y2), ey0 = 256.0f * float(X2 - X1);
+            const float ex1 = 256.0f * float(Y2 - Y0), ey1 = 256.0f * float(X0 - X2);
+            const float ex2 = 256.0f * float(Y0 - Y1), ey2 = 256.0f * float(X1 - X0);
+            pg.dlx = mk(ex0 * (w1 * w2), (swapped ? ex2 : ex1) * (w0 * w2), (swapped ? ex1 : ex2) * (w0 * w1));
+            pg.dly = mk(ey0 * (w1 * w2), (swapped ? ey2 : ey1) * (w0 * w2), (swapped ? ey1 : ey2) * (w0 * w1));
+        }
     } else {
         const float4 c0 = xv[pg.i0], c1 = xv[pg.i1], c2 = xv[pg.i2];
         const float u = (float(x) + 0.5f) * (2.0f / float(V.width)) - 1.0f;
@@ -253,8 +265,19 @@ __device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView
         const f3 a = mk(c0.x - u * c0.w, c1.x - u * c1.w, c2.x - u * c2.w);
         const f3 b = mk(c0.y - v * c0.w, c1.y - v * c1.w, c2.y - v * c2.w);
         lam = cross(a, b);
+        if constexpr (kLod) {
+            const float sx = -2.0f / float(V.width), sy = -2.0f / float(V.height);
+            pg.dlx = cross(mk(c0.w * sx, c1.w * sx, c2.w * sx), b);
+            pg.dly = cross(a, mk(c0.w * sy, c1.w * sy, c2.w * sy));
+        }
     }
-    lam = lam * (1.0f / (lam.x + lam.y + lam.z));
+    const float inv_s = 1.0f / (lam.x + lam.y + lam.z);
+    lam = lam * inv_s;
+    if constexpr (kLod) {
+        const float dsx = pg.dlx.x + pg.dlx.y + pg.dlx.z, dsy = pg.dly.x + pg.dly.y + pg.dly.z;
+        pg.dlx = (pg.dlx - lam * dsx) * inv_s;
+        pg.dly = (pg.dly - lam * dsy) * inv_s;
+    }
     const float view_depth = w0 * lam.x + w1 * lam.y + w2 * lam.z;
     int cascade = -1;
     for (uint32_t k = 0; k < L.n_cascades; ++k)
@@ -302,7 +325,7 @@ struct Fragment {
 // fragmentMain (basic.slang:137-344) for one covered pixel.  When `use_dec` is set, the material
 // decoder's outputs `dec` replace the albedo, normal and metallic-roughness samples of the plain
 // textures.  kUniformDraw: every live lane of the wave shades the same draw.
-template <bool kUniformDraw>
+template <bool kUniformDraw, bool kLod = false>
 __device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V, const float* lut, size_t o,
                                    const PixelGeom& pg, bool use_dec, const float (&dec)[kNmOut]) {
     using R = std::conditional_t<kUniformDraw, WaveRecords, LaneRecords>;
@@ -328,6 +351,23 @@ __device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V,
     const f3 btan = bv.tangent, bbit = bv.bitangent;
     const float4 bcol = bv.color;
     const float tu = bv.u, tv = bv.v, su = bv.su, sv = bv.sv, sigma = bv.sigma;
+    // kLod: texture-coordinate derivatives along the pixel's x and y
+    float dudx = 0.f, dvdx = 0.f, dudy = 0.f, dvdy = 0.f;
+    if constexpr (kLod) {
+        const float* t0 = vtx[0]->texture_coordinates;
+        const float* t1 = vtx[1]->texture_coordinates;
+        const float* t2 = vtx[2]->texture_coordinates;
+        dudx = pg.dlx.x * t0[0] + pg.dlx.y * t1[0] + pg.dlx.z * t2[0];
+        dvdx = pg.dlx.x * t0[1] + pg.dlx.y * t1[1] + pg.dlx.z * t2[1];
+        dudy = pg.dly.x * t0[0] + pg.dly.y * t1[0] + pg.dly.z * t2[0];
+        dvdy = pg.dly.x * t0[1] + pg.dly.y * t1[1] + pg.dly.z * t2[1];
+    }
+    auto sample = [&](const TexGPU& tg, const SamplerState& smp) {
+        const TexRef t{L.texels + tg.offset, int(tg.width), int(tg.height), int(tg.channels), int(tg.srgb),
+                       int(tg.levels)};
+        if constexpr (kLod) return sample_tex_lod(t, smp, tu, tv, dudx, dvdx, dudy, dvdy, lut);
+        else return sample_tex(t, smp, tu, tv, lut);
+    };
     const float4 csm = bv.csm;
 
     const f3 ambient = mk(sd.ambient_color[0], sd.ambient_color[1], sd.ambient_color[2]);
@@ -357,12 +397,11 @@ __device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V,
         } else {
             if (sd.normal_maps_enabled == 1 && valid_index(nti) && nti < L.n_textures) {
                 const TexGPU tg = R::load(textures + nti);
-                const TexRef t{L.texels + tg.offset, int(tg.width), int(tg.height), int(tg.channels), int(tg.srgb)};
                 const uint32_t si = md->normal_sampler_index;
                 const SamplerState smp = si < L.n_samplers ? sampler_state(R::load(samplers + si)) : SamplerState{};
-                const float4 s = sample_tex(t, smp, tu, tv, lut);
+                const float4 s = sample(tg, smp);
                 f3 m = mk(s.x * 2.f - 1.f, s.y * 2.f - 1.f, s.z * 2.f - 1.f);
-                if (t.ch == 2) m.z = sqrtf(fmaxf(0.f, 1.f - m.x * m.x - m.y * m.y));
+                if (tg.channels == 2) m.z = sqrtf(fmaxf(0.f, 1.f - m.x * m.x - m.y * m.y));
                 const f3 n = normalize(bn);
                 const f3 tt = normalize(btan - n * dot(btan, n));
                 const f3 bb = normalize(cross(bn, btan) * sigma);
@@ -371,10 +410,9 @@ __device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V,
             const uint32_t cti = md->color_sampled_image_index;
             if (valid_index(cti) && cti < L.n_textures) {
                 const TexGPU tg = R::load(textures + cti);
-                const TexRef t{L.texels + tg.offset, int(tg.width), int(tg.height), int(tg.channels), int(tg.srgb)};
                 const uint32_t si = md->color_sampler_index;
                 const SamplerState smp = si < L.n_samplers ? sampler_state(R::load(samplers + si)) : SamplerState{};
-                image = sample_tex(t, smp, tu, tv, lut);
+                image = sample(tg, smp);
                 alpha = fminf(image.w, alpha);
             }
             const uint32_t mixti = md->mixmap_sampled_image_index, metti = md->metallic_sampled_image_index;
@@ -383,9 +421,8 @@ __device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V,
             const uint32_t msi = use_mix ? md->mixmap_sampler_index : md->metallic_sampler_index;
             if (valid_index(mti) && mti < L.n_textures) {
                 const TexGPU tg = R::load(textures + mti);
-                const TexRef t{L.texels + tg.offset, int(tg.width), int(tg.height), int(tg.channels), int(tg.srgb)};
                 const SamplerState smp = msi < L.n_samplers ? sampler_state(R::load(samplers + msi)) : SamplerState{};
-                mr_from(t.ch, sample_tex(t, smp, tu, tv, lut), srough, smetal);
+                mr_from(int(tg.channels), sample(tg, smp), srough, smetal);
             }
         }
     }
@@ -486,7 +523,7 @@ __device__ inline bool wave_uniform(uint32_t v) {
     return __builtin_amdgcn_ballot_w64(v != v0) == 0ull;
 }
 
-template <bool kNeural>
+template <bool kNeural, bool kLod>
 __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
     __shared__ float lut[256];
     __shared__ uint32_t tri_base[kLdsDraws];
@@ -512,9 +549,9 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
             return;
         }
         const float none[kNmOut] = {};
-        const PixelGeom pg = pixel_geometry(L, V, tri_base, x, y, vis);
-        store_fragment(V, o, wave_uniform(pg.draw) ? shade_pixel<true>(L, V, lut, o, pg, false, none)
-                                                   : shade_pixel<false>(L, V, lut, o, pg, false, none));
+        const PixelGeom pg = pixel_geometry<kLod>(L, V, tri_base, x, y, vis);
+        store_fragment(V, o, wave_uniform(pg.draw) ? shade_pixel<true, kLod>(L, V, lut, o, pg, false, none)
+                                                   : shade_pixel<false, kLod>(L, V, lut, o, pg, false, none));
     } else {
         // Every lane stays live up to the wave-wide decoder MLP; lanes without a decoded material
         // feed zero features and ignore the result.
@@ -523,7 +560,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
         bool decode = false;
         float f[kNmFeat] = {};
         if (vis != 0ull) {
-            pg = pixel_geometry(L, V, tri_base, x, y, vis);
+            pg = pixel_geometry<kLod>(L, V, tri_base, x, y, vis);
             const DrawGPU dr = L.draws[pg.draw];
             if (dr.material < L.n_materials && L.nm_active[dr.material]) {
                 decode = true;
@@ -545,8 +582,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
             if (V.shadow_factor) V.shadow_factor[o] = 1.f;
             return;
         }
-        store_fragment(V, o, wave_uniform(pg.draw) ? shade_pixel<true>(L, V, lut, o, pg, decode, dec)
-                                                   : shade_pixel<false>(L, V, lut, o, pg, decode, dec));
+        store_fragment(V, o, wave_uniform(pg.draw) ? shade_pixel<true, kLod>(L, V, lut, o, pg, decode, dec)
+                                                   : shade_pixel<false, kLod>(L, V, lut, o, pg, decode, dec));
     }
 }
 
@@ -557,7 +594,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
 // (the wave loops while any lane has one left, so the decoder's wave-wide MLP sees every lane) and
 // writes the average of its four samples; with a blended pass to follow it also keeps the sample
 // colours.  The debug shadow factor is that of sample 0's primitive.
-template <bool kNeural>
+template <bool kNeural, bool kLod>
 __global__ __launch_bounds__(256) void k_shade_ms(ShadeLaunch L) {
     __shared__ float lut[256];
     __shared__ uint32_t tri_base[kLdsDraws];
@@ -594,7 +631,7 @@ __global__ __launch_bounds__(256) void k_shade_ms(ShadeLaunch L) {
         const uint32_t same = (k0 == key ? 1u : 0u) | (k1 == key ? 2u : 0u) | (k2 == key ? 4u : 0u) | (k3 == key ? 8u : 0u);
         const bool covered = live && key != 0u;
         PixelGeom pg{};
-        if (covered) pg = pixel_geometry(L, V, tri_base, x, y, w);
+        if (covered) pg = pixel_geometry<kLod>(L, V, tri_base, x, y, w);
         float dec[kNmOut] = {};
         bool decode = false;
         if constexpr (kNeural) {
@@ -618,8 +655,8 @@ __global__ __launch_bounds__(256) void k_shade_ms(ShadeLaunch L) {
         float4 c = clear;
         float sh = 1.f;
         if (covered) {
-            const Fragment fr = wave_uniform(pg.draw) ? shade_pixel<true>(L, V, lut, o, pg, decode, dec)
-                                                      : shade_pixel<false>(L, V, lut, o, pg, decode, dec);
+            const Fragment fr = wave_uniform(pg.draw) ? shade_pixel<true, kLod>(L, V, lut, o, pg, decode, dec)
+                                                      : shade_pixel<false, kLod>(L, V, lut, o, pg, decode, dec);
             c = fr.color;
             sh = fr.shadow;
         }
@@ -760,6 +797,7 @@ __global__ __launch_bounds__(256) void k_blend_collect(const BlendLaunch B) {
 // Every fragment of the frame shaded at its pixel centre (a grid-stride loop over the count the
 // collection left), its colour stored at the index its record names.  Records of sub-tiles that ran
 // out of space have key 0 and are skipped.
+template <bool kLod>
 __global__ __launch_bounds__(256) void k_blend_shade(const BlendLaunch B, const ShadeLaunch L) {
     __shared__ float lut[256];
     __shared__ uint32_t tri_base[kLdsDraws];
@@ -775,9 +813,9 @@ __global__ __launch_bounds__(256) void k_blend_shade(const BlendLaunch B, const 
         const ShadeView& SV = L.views[f.z];
         const uint32_t x = f.y & 0xffffu, y = f.y >> 16;
         const size_t o = size_t(y) * SV.width + x;
-        const PixelGeom pg = pixel_geometry(L, SV, tri_base, x, y, (unsigned long long)f.x);
-        const Fragment fr = wave_uniform(pg.draw) ? shade_pixel<true>(L, SV, lut, o, pg, false, none)
-                                                  : shade_pixel<false>(L, SV, lut, o, pg, false, none);
+        const PixelGeom pg = pixel_geometry<kLod>(L, SV, tri_base, x, y, (unsigned long long)f.x);
+        const Fragment fr = wave_uniform(pg.draw) ? shade_pixel<true, kLod>(L, SV, lut, o, pg, false, none)
+                                                  : shade_pixel<false, kLod>(L, SV, lut, o, pg, false, none);
         B.frag_color[f.w] = fr.discard ? make_float4(0.f, 0.f, 0.f, __uint_as_float(kDiscarded)) : fr.color;
     }
 }
@@ -845,7 +883,10 @@ hipError_t launch_blend_fragments(const BlendLaunch& b, const ShadeLaunch& l, hi
         hipLaunchKernelGGL((k_blend_collect<32, 4>), dim3(b.blend_bins * 4), dim3(256), 0, s, b);
     else
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_blend_shade, dim3(shade_blocks), dim3(256), 0, s, b, l);
+    if (l.lod)
+        hipLaunchKernelGGL(k_blend_shade<true>, dim3(shade_blocks), dim3(256), 0, s, b, l);
+    else
+        hipLaunchKernelGGL(k_blend_shade<false>, dim3(shade_blocks), dim3(256), 0, s, b, l);
     return hipGetLastError();
 }
 
@@ -864,17 +905,17 @@ hipError_t launch_blend_resolve(const BlendLaunch& b, const ShadeLaunch& l, hipS
 
 hipError_t launch_shade(const ShadeLaunch& l, hipStream_t s) {
     if (l.total_blocks == 0) return hipSuccess;
+    // kernels by samples per pixel, decoder, and whether some texture has mip levels
+#define RSY_SHADE(K, N, LOD) hipLaunchKernelGGL((K<N, LOD>), dim3(l.total_blocks), dim3(256), 0, s, l)
+    const bool lod = l.lod != 0, neural = l.neural != 0;
     if (l.samples == 4) {
-        if (l.neural)
-            hipLaunchKernelGGL(k_shade_ms<true>, dim3(l.total_blocks), dim3(256), 0, s, l);
-        else
-            hipLaunchKernelGGL(k_shade_ms<false>, dim3(l.total_blocks), dim3(256), 0, s, l);
-        return hipGetLastError();
+        if (neural) { if (lod) RSY_SHADE(k_shade_ms, true, true); else RSY_SHADE(k_shade_ms, true, false); }
+        else { if (lod) RSY_SHADE(k_shade_ms, false, true); else RSY_SHADE(k_shade_ms, false, false); }
+    } else {
+        if (neural) { if (lod) RSY_SHADE(k_shade, true, true); else RSY_SHADE(k_shade, true, false); }
+        else { if (lod) RSY_SHADE(k_shade, false, true); else RSY_SHADE(k_shade, false, false); }
     }
-    if (l.neural)
-        hipLaunchKernelGGL(k_shade<true>, dim3(l.total_blocks), dim3(256), 0, s, l);
-    else
-        hipLaunchKernelGGL(k_shade<false>, dim3(l.total_blocks), dim3(256), 0, s, l);
+#undef RSY_SHADE
     return hipGetLastError();
 }
 

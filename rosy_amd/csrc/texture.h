@@ -1,11 +1,14 @@
-// Texture sampling shared by the shading and material-decoder kernels: mip 0, nearest or bilinear,
+// Texture sampling shared by the shading and material-decoder kernels: nearest or bilinear,
 // CLAMP_TO_EDGE / MIRRORED_REPEAT / REPEAT addressing, sRGB decode before filtering through a
 // 256-entry table (Vulkan's filtering order; samplers Graphics.cpp:3421-3432, formats 4000-4015).
+// Textures with mip levels are sampled trilinearly at the level of detail of the pixel's texture
+// coordinate derivatives (the asset samplers' VK_SAMPLER_MIPMAP_MODE_LINEAR, Graphics.cpp:4002-4012).
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 
+#include "frame_types.h"
 #include "rosy_amd.h"
 
 namespace rsy {
@@ -25,13 +28,25 @@ __device__ inline int wrap_coord(int i, int n, uint32_t mode) {
 struct TexRef {
     const uint8_t* base;
     int w, h, ch, srgb;
+    int levels = 1;
 };
+
+// Mip level l of a texture as a texture of its own.
+__device__ inline TexRef level_of(const TexRef& t, int l) {
+    TexRef r = t;
+    r.base = t.base + mip_offset(uint32_t(t.w), uint32_t(t.h), uint32_t(t.ch), uint32_t(l));
+    r.w = int(mip_extent(uint32_t(t.w), uint32_t(l)));
+    r.h = int(mip_extent(uint32_t(t.h), uint32_t(l)));
+    r.levels = 1;
+    return r;
+}
 
 // Sampler state of a sample: addressing per axis (0 clamp-to-edge, 1 mirrored repeat, 2 repeat)
 // and nearest or bilinear filtering; the default is the REPEAT / bilinear sampler.
 struct SamplerState {
     uint32_t wrap_s = 2u, wrap_t = 2u;
-    bool nearest = false;
+    bool nearest = false;      // magnification filter (and the filter of single-level textures)
+    bool min_nearest = false;  // minification filter of textures with mip levels
 };
 
 template <class S>
@@ -40,6 +55,7 @@ __device__ inline SamplerState sampler_state(const S& s) {
     r.wrap_s = s.wrap_s;
     r.wrap_t = s.wrap_t;
     r.nearest = s.mag_filter <= 2;
+    r.min_nearest = s.min_filter <= 2;
     return r;
 }
 
@@ -83,6 +99,32 @@ __device__ inline float4 sample_tex(const TexRef& t, const SamplerState& s, floa
     r.z = w00 * c00.z + w10 * c10.z + w01 * c01.z + w11 * c11.z;
     r.w = w00 * c00.w + w10 * c10.w + w01 * c01.w + w11 * c11.w;
     return r;
+}
+
+// Trilinear sampling (Vulkan's level-of-detail and mipmapping rules with mipmapMode LINEAR, no LOD
+// bias or clamp): rho = the larger of |d(uv)/dx| and |d(uv)/dy| in level-0 texels, lambda =
+// log2(rho); lambda <= 0 samples level 0 with the magnification filter, otherwise d = min(lambda,
+// levels - 1) blends levels floor(d) and floor(d) + 1 (the last level when past it) by frac(d) with
+// the minification filter.  Single-level textures keep the plain sample.
+__device__ inline float4 sample_tex_lod(const TexRef& t, const SamplerState& s, float u, float v, float dudx, float dvdx,
+                                        float dudy, float dvdy, const float* lut) {
+    if (t.levels <= 1) return sample_tex(t, s, u, v, lut);
+    const float W = float(t.w), H = float(t.h);
+    const float rx = sqrtf(dudx * W * (dudx * W) + dvdx * H * (dvdx * H));
+    const float ry = sqrtf(dudy * W * (dudy * W) + dvdy * H * (dvdy * H));
+    const float lambda = log2f(fmaxf(rx, ry));
+    if (!(lambda > 0.0f)) return sample_tex(level_of(t, 0), s, u, v, lut);
+    SamplerState m = s;
+    m.nearest = s.min_nearest;
+    const float d = fminf(lambda, float(t.levels - 1));
+    const float fl = floorf(d);
+    const int dl = int(fl), dh = min(dl + 1, t.levels - 1);
+    const float delta = d - fl;
+    const float4 a = sample_tex(level_of(t, dl), m, u, v, lut);
+    if (delta == 0.0f || dh == dl) return a;
+    const float4 b = sample_tex(level_of(t, dh), m, u, v, lut);
+    return make_float4(a.x + (b.x - a.x) * delta, a.y + (b.y - a.y) * delta, a.z + (b.z - a.z) * delta,
+                       a.w + (b.w - a.w) * delta);
 }
 
 }  // namespace rsy

@@ -71,10 +71,12 @@ class CustomScene:
             (1, 1, abi.TEX_RG8_SRGB, bytes([0, 230])),
         ]
         self._tex = (abi.TextureDesc * len(texs))()
-        for i, (w, h, fmt, data) in enumerate(texs):
+        for i, tex in enumerate(texs):
+            w, h, fmt, data = tex[:4]
+            levels = tex[4] if len(tex) > 4 else 0  # mip_levels: 0 / 1 one level, n given, MIPS_BUILD
             buf = C.create_string_buffer(bytes(data), len(data))
             self._tex_pixels.append(buf)
-            self._tex[i] = abi.TextureDesc(w, h, fmt, 0, C.cast(buf, C.c_void_p))
+            self._tex[i] = abi.TextureDesc(w, h, fmt, levels, C.cast(buf, C.c_void_p))
         # materials: each a dict of overrides of the default (white, textures 0 / 1 / 2, opaque)
         mats = materials if materials is not None else [{}]
         self._mat = (abi.GpuMaterial * len(mats))()

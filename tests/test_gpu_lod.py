@@ -1,0 +1,80 @@
+"""Texture level of detail on the GPU through the C ABI, against the FP64 oracle: trilinear sampling
+at the pixel's texture-coordinate derivatives (the asset samplers' VK_SAMPLER_MIPMAP_MODE_LINEAR,
+Graphics.cpp:4002-4012) with given, built and BC7 mip chains, on affine and perspective quads and on
+a generated scene with full chains; colour within 1e-3 (max abs) of the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+import rosy_amd
+from rosy_amd import abi
+from lod_scene import affine_quad, level_coded_texture, receding_quad
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-3
+TIE = 1e-5
+
+
+def render_and_compare(scene, shadows=False):
+    r = rosy_amd.Renderer(device=0)
+    r.load(scene)
+    r.render(scene.frame_desc(abi.SHADE_PLAIN))
+    cam = scene.camera_views()[0]
+    vis = r.visibility(0)
+    want_vis, _ = oracle.raster_view(scene, cam.view_projection[:], cam.width, cam.height, False,
+                                     scene.s.raster_state)
+    assert np.array_equal(vis, want_vis)
+    maps = [r.shadow_map(i) for i in range(len(scene.cascades()))]
+    ref, _, margin = oracle.shade_view(scene, cam, scene.cascades(), maps, vis)
+    col = r.color(0).astype(np.float64)
+    ok = margin >= TIE
+    err = float(np.abs(col - ref)[ok].max())
+    r.close()
+    return err, col, ref, ok
+
+
+@pytest.mark.parametrize("su,sv", [(3.0, 3.0), (0.25, 0.25), (2.0, 8.0), (40.0, 40.0)])
+def test_level_coded_affine_quads(su, sv):
+    err, *_ = render_and_compare(affine_quad(32, su, sv, level_coded_texture()))
+    print(f"affine {su}x{sv}: {err:.2e}")
+    assert err <= TOL
+
+
+def test_level_coded_receding_plane():
+    err, col, ref, _ = render_and_compare(receding_quad(48, 1.0, level_coded_texture()))
+    print(f"receding plane: {err:.2e}")
+    assert err <= TOL and np.ptp(ref[..., 0]) > 0.1  # the level of detail varies over the view
+
+
+def test_built_chain_of_a_noise_texture():
+    rng = np.random.default_rng(3)
+    img = rng.integers(0, 256, (128, 128, 4), dtype=np.uint8)
+    img[..., 3] = 255
+    tex = (128, 128, abi.TEX_RGBA8_SRGB, img.tobytes(), abi.MIPS_BUILD)
+    err, *_ = render_and_compare(receding_quad(64, 3.0, tex))
+    print(f"built chain: {err:.2e}")
+    assert err <= TOL
+
+
+def test_bc7_chain_with_given_levels():
+    rng = np.random.default_rng(4)
+    w = h = 64
+    blocks = b""
+    for l in range(7):
+        n = max(1, (max(1, w >> l) + 3) // 4) * max(1, (max(1, h >> l) + 3) // 4)
+        b = rng.integers(0, 256, (n, 16), dtype=np.uint8)
+        b[:, 0] = (b[:, 0] & 0x80) | 0x40  # mode 6, alpha from the block
+        blocks += b.tobytes()
+    tex = (w, h, abi.TEX_BC7_SRGB, blocks, 7)
+    err, *_ = render_and_compare(receding_quad(48, 2.0, tex))
+    print(f"BC7 chain: {err:.2e}")
+    assert err <= TOL
+
+
+def test_generated_scene_with_mip_chains():
+    s = rosy_amd.Scene(1, seed=4, width=256, height=144, shadow_size=512, total_triangles=20000, n_meshes=24,
+                       texture_size=64, texture_mips=1)
+    err, _, _, ok = render_and_compare(s)
+    print(f"generated scene with chains: {err:.2e} ({1 - ok.mean():.4f} of pixels on a shadow tie)")
+    assert err <= TOL and ok.mean() > 0.99
