@@ -145,6 +145,54 @@ def msaa_timing(r, scene, plain, neural, args, check):
     return out
 
 
+def texture_timing(args):
+    """BC7 images with full chains uploaded through rsy_set_materials (64 x 1024^2, random blocks of
+    every mode: host-to-device copy plus GPU decode, wall clock), and the plain decoder-workload frame
+    with every texture's chain built (trilinear sampling at the pixel's level of detail)."""
+    import ctypes as C
+
+    import numpy as np
+    import rosy_amd
+    from rosy_amd import abi
+    rng = np.random.default_rng(11)
+    size, n_tex = 1024, 64
+    levels = int(np.log2(size)) + 1
+    nblocks = sum(max(1, (size >> l) // 4) ** 2 for l in range(levels))
+    blocks = rng.integers(0, 256, (nblocks, 16), dtype=np.uint8)
+    blocks[:, 0] = np.left_shift(1, rng.integers(0, 8, nblocks)).astype(np.uint8) | (blocks[:, 0] & 0x80)
+    data = blocks.tobytes()
+    descs = (abi.TextureDesc * n_tex)()
+    for i in range(n_tex):
+        descs[i] = abi.TextureDesc(size, size, abi.TEX_BC7_SRGB, levels, C.cast(C.c_char_p(data), C.c_void_p))
+    small = rosy_amd.Scene(0, seed=1)
+    r = rosy_amd.Renderer(device=0)
+    r.load(small)
+    lib = rosy_amd.lib()
+    lib.rsy_set_materials(r.ctx, None, 0, descs, n_tex, None, 0)  # warm-up: allocations
+    t0 = time.perf_counter()
+    rc = lib.rsy_set_materials(r.ctx, None, 0, descs, n_tex, None, 0)
+    upload_ms = (time.perf_counter() - t0) * 1e3
+    assert rc == abi.RSY_OK
+    r.close()
+    small.close()
+    out = {"bc7_upload": {"textures": n_tex, "extent": size, "levels": levels, "mb": round(n_tex * len(data) / 1e6, 1),
+                          "ms": round(upload_ms, 2), "gbs": round(n_tex * len(data) / (upload_ms * 1e-3) / 1e9, 2),
+                          "note": "wall clock of rsy_set_materials: host-to-device copy and GPU decode"}}
+    scene = rosy_amd.Scene(3, seed=args.seed, texture_mips=1)
+    r = rosy_amd.Renderer(device=0)
+    r.load(scene)
+    plain = scene.frame_desc(abi.SHADE_PLAIN)
+    st = stage_medians(r, plain)
+    r.set_debug_flags(0)
+    r.render_graph(plain, args.warmup)
+    r.render_graph(plain, args.steps)
+    out["mip_chains"] = {"plain_ms_per_frame": round(float(r.stats().gpu_ms), 5), "shade_ms": round(float(st[3]), 5),
+                         "workload": "decoder workload scene, every 1024^2 map with its built chain (11 levels)"}
+    r.close()
+    scene.close()
+    return out
+
+
 def rsy_decode_timing(scene, seconds=2.0):
     """The workload written as a .rsy file (rosy_amd.asset.from_scene) and decoded by rsy_asset_decode
     from host memory, repeated for a bounded time: MB of file and host GB/s (one thread)."""
@@ -264,6 +312,9 @@ def main():
     # the same frames with the reference's 4-sample main pass
     msaa = msaa_timing(r, scene, plain, neural, args, check=not args.no_cpu)
     log(f"4-sample frames {msaa}")
+    # the texture pipeline: BC7 upload with full chains, and the plain frame with mip chains
+    texture_pipeline = texture_timing(args)
+    log(f"texture pipeline {texture_pipeline}")
 
     # raster kernel, algorithmic bytes with the implementation-independent unit costs: one tile-list
     # entry read (4 B) per (triangle, bin) pair, one setup record read (32 B: three edge pairs, the
@@ -356,6 +407,7 @@ def main():
         "decoder": decoder,
         "output_stage": output_stage,
         "msaa4": msaa,
+        "texture_pipeline": texture_pipeline,
         "rsy_decode": rsy_decode,
         "cpu_baseline": cpu,
         "wall_ms_per_step": round(wall, 5),

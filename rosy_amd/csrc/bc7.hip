@@ -63,19 +63,14 @@ __constant__ uint8_t kWeights2[4] = {0, 21, 43, 64};
 __constant__ uint8_t kWeights3[8] = {0, 9, 18, 27, 37, 46, 55, 64};
 __constant__ uint8_t kWeights4[16] = {0, 4, 9, 13, 17, 21, 26, 30, 34, 38, 43, 47, 51, 55, 60, 64};
 
-// Bits of the block from bit `pos` upwards (n <= 8 here; no field crosses more than two words).
-struct BitReader {
-    uint32_t w[4];
-    uint32_t pos = 0;
-    __device__ uint32_t get(uint32_t n) {
-        if (n == 0) return 0u;
-        const uint32_t i = pos >> 5, s = pos & 31u;
-        uint64_t v = uint64_t(w[i]) >> s;
-        if (s + n > 32u && i < 3u) v |= uint64_t(w[i + 1]) << (32u - s);
-        pos += n;
-        return uint32_t(v) & ((1u << n) - 1u);
-    }
-};
+// n <= 8 bits of the block from bit position `pos` (fields never cross more than one word boundary).
+__device__ inline uint32_t bits_at(const uint32_t (&w)[4], uint32_t pos, uint32_t n) {
+    if (n == 0) return 0u;
+    const uint32_t i = pos >> 5, sft = pos & 31u;
+    uint64_t v = uint64_t(w[i]) >> sft;
+    if (sft + n > 32u && i < 3u) v |= uint64_t(w[i + 1]) << (32u - sft);
+    return uint32_t(v) & ((1u << n) - 1u);
+}
 
 __device__ inline uint32_t widen(uint32_t v, uint32_t bits) {  // replicate the top bits downwards
     v <<= 8u - bits;
@@ -94,101 +89,92 @@ __device__ inline uint32_t subset_of(uint32_t ns, uint32_t part, uint32_t i) {
     return 0u;
 }
 
-__device__ inline bool is_anchor(uint32_t ns, uint32_t part, uint32_t i) {
-    if (i == 0) return true;
-    if (ns == 2) return i == kAnchor2[part];
-    if (ns == 3) return i == kAnchor3a[part] || i == kAnchor3b[part];
-    return false;
+// Anchor texels before texel i (texel 0 always is one) and whether texel i is one.
+__device__ inline void anchors_before(uint32_t ns, uint32_t part, uint32_t i, uint32_t& before, bool& is) {
+    uint32_t a1 = 16u, a2 = 16u;  // 16: no such anchor
+    if (ns == 2) a1 = kAnchor2[part];
+    if (ns == 3) {
+        a1 = kAnchor3a[part];
+        a2 = kAnchor3b[part];
+    }
+    before = (i > 0 ? 1u : 0u) + (a1 < i ? 1u : 0u) + (a2 < i ? 1u : 0u);
+    is = i == 0 || i == a1 || i == a2;
 }
 
-// One block into 16 RGBA8 texels (texel i = row i / 4, column i % 4), R in the low byte.
-__device__ void decode_block(uint4 blk, uint32_t out[16]) {
-    BitReader br;
-    br.w[0] = blk.x;
-    br.w[1] = blk.y;
-    br.w[2] = blk.z;
-    br.w[3] = blk.w;
+// Texel i (row i / 4, column i % 4) of a block as RGBA8, R in the low byte.  Every field's bit
+// position follows from the mode: header, colour endpoints (channel, subset, endpoint), alpha
+// endpoints, p-bits, then the index sets, where each anchor texel's index is one bit short.
+__device__ uint32_t decode_texel(const uint4 blk, uint32_t i) {
+    const uint32_t w[4] = {blk.x, blk.y, blk.z, blk.w};
     const uint32_t b0 = blk.x & 0xffu;
-    if (b0 == 0u) {
-        for (int i = 0; i < 16; ++i) out[i] = 0u;
-        return;
-    }
+    if (b0 == 0u) return 0u;  // no mode bit: transparent black
     const uint32_t mode = uint32_t(__builtin_ctz(b0));
-    br.pos = mode + 1u;
     const Bc7Mode M = kModes[mode];
-    const uint32_t part = br.get(M.pb), rot = br.get(M.rb), isel = br.get(M.isb);
-    uint32_t ep[3][2][4];  // [subset][endpoint][channel]
-    for (uint32_t c = 0; c < 3; ++c)
-        for (uint32_t s = 0; s < M.ns; ++s)
-            for (uint32_t e = 0; e < 2; ++e) ep[s][e][c] = br.get(M.cb);
-    for (uint32_t s = 0; s < M.ns; ++s)
-        for (uint32_t e = 0; e < 2; ++e) ep[s][e][3] = M.ab ? br.get(M.ab) : 255u;
-    uint32_t cbits = M.cb, abits = M.ab;
-    if (M.epb || M.spb) {
-        for (uint32_t s = 0; s < M.ns; ++s) {
-            uint32_t p0, p1;
-            if (M.epb) {
-                p0 = br.get(1);
-                p1 = br.get(1);
-            } else {
-                p0 = p1 = br.get(1);
-            }
-            for (uint32_t c = 0; c < 4; ++c) {
-                if (c == 3 && !M.ab) continue;
-                ep[s][0][c] = (ep[s][0][c] << 1) | p0;
-                ep[s][1][c] = (ep[s][1][c] << 1) | p1;
-            }
+    uint32_t pos = mode + 1u;
+    const uint32_t part = bits_at(w, pos, M.pb);
+    pos += M.pb;
+    const uint32_t rot = bits_at(w, pos, M.rb);
+    pos += M.rb;
+    const uint32_t isel = bits_at(w, pos, M.isb);
+    pos += M.isb;
+    const uint32_t ns = M.ns, s = subset_of(ns, part, i);
+    const uint32_t colour_at = pos, alpha_at = colour_at + 6u * ns * M.cb, pbit_at = alpha_at + 2u * ns * M.ab;
+    const uint32_t index_at = pbit_at + (M.epb ? 2u * ns : M.spb ? ns : 0u);
+    uint32_t e[2][4];
+    const bool has_p = M.epb || M.spb;
+    for (uint32_t k = 0; k < 2; ++k) {
+        const uint32_t p = M.epb ? bits_at(w, pbit_at + 2u * s + k, 1) : M.spb ? bits_at(w, pbit_at + s, 1) : 0u;
+        for (uint32_t c = 0; c < 3; ++c) {
+            const uint32_t v = bits_at(w, colour_at + ((c * ns + s) * 2u + k) * M.cb, M.cb);
+            e[k][c] = has_p ? widen((v << 1) | p, M.cb + 1u) : widen(v, M.cb);
         }
-        cbits += 1;
-        if (M.ab) abits += 1;
+        if (M.ab) {
+            const uint32_t v = bits_at(w, alpha_at + (s * 2u + k) * M.ab, M.ab);
+            e[k][3] = has_p ? widen((v << 1) | p, M.ab + 1u) : widen(v, M.ab);
+        } else {
+            e[k][3] = 255u;
+        }
     }
-    for (uint32_t s = 0; s < M.ns; ++s)
-        for (uint32_t e = 0; e < 2; ++e) {
-            for (uint32_t c = 0; c < 3; ++c) ep[s][e][c] = widen(ep[s][e][c], cbits);
-            if (M.ab) ep[s][e][3] = widen(ep[s][e][3], abits);
+    uint32_t before;
+    bool anchor;
+    anchors_before(ns, part, i, before, anchor);
+    const uint32_t idx = bits_at(w, index_at + i * M.ib - before, M.ib - (anchor ? 1u : 0u));
+    uint32_t ci = idx, cb = M.ib, ai = idx, ab = M.ib;
+    if (M.ib2) {
+        // modes 4 and 5: a second index set (one subset: only texel 0 is an anchor) for alpha, or
+        // for colour when the selection bit is set
+        const uint32_t at2 = index_at + 16u * M.ib - 1u;
+        const uint32_t idx2 = bits_at(w, at2 + i * M.ib2 - (i > 0 ? 1u : 0u), M.ib2 - (i == 0 ? 1u : 0u));
+        if (isel) {
+            ci = idx2;
+            cb = M.ib2;
+        } else {
+            ai = idx2;
+            ab = M.ib2;
         }
-    uint32_t idx[16], idx2[16];
-    for (uint32_t i = 0; i < 16; ++i) idx[i] = br.get(M.ib - (is_anchor(M.ns, part, i) ? 1u : 0u));
-    for (uint32_t i = 0; i < 16; ++i) idx2[i] = M.ib2 ? br.get(M.ib2 - (i == 0 ? 1u : 0u)) : 0u;
-    for (uint32_t i = 0; i < 16; ++i) {
-        const uint32_t s = subset_of(M.ns, part, i);
-        // modes 4 and 5: colour and alpha take separate index sets (swapped by the selection bit)
-        uint32_t ci = idx[i], cb = M.ib, ai = idx[i], ab = M.ib;
-        if (M.ib2) {
-            if (isel) {
-                ci = idx2[i];
-                cb = M.ib2;
-            } else {
-                ai = idx2[i];
-                ab = M.ib2;
-            }
-        }
-        uint32_t r = lerp8(ep[s][0][0], ep[s][1][0], weight(cb, ci));
-        uint32_t g = lerp8(ep[s][0][1], ep[s][1][1], weight(cb, ci));
-        uint32_t b = lerp8(ep[s][0][2], ep[s][1][2], weight(cb, ci));
-        uint32_t a = lerp8(ep[s][0][3], ep[s][1][3], weight(ab, ai));
-        uint32_t t;
-        if (rot == 1) { t = a; a = r; r = t; }
-        else if (rot == 2) { t = a; a = g; g = t; }
-        else if (rot == 3) { t = a; a = b; b = t; }
-        out[i] = r | (g << 8) | (b << 16) | (a << 24);
     }
+    uint32_t r = lerp8(e[0][0], e[1][0], weight(cb, ci));
+    uint32_t g = lerp8(e[0][1], e[1][1], weight(cb, ci));
+    uint32_t b = lerp8(e[0][2], e[1][2], weight(cb, ci));
+    uint32_t a = lerp8(e[0][3], e[1][3], weight(ab, ai));
+    uint32_t t;
+    if (rot == 1) { t = a; a = r; r = t; }
+    else if (rot == 2) { t = a; a = g; g = t; }
+    else if (rot == 3) { t = a; a = b; b = t; }
+    return r | (g << 8) | (b << 16) | (a << 24);
 }
 
-// One thread per block; texels outside the image (its width or height not a multiple of 4) are
-// dropped.
+// One thread per texel, 16 consecutive threads per block; texels outside the image (its width or
+// height not a multiple of 4) are not written.
 __global__ __launch_bounds__(256) void k_bc7_decode(const uint4* blocks, uint32_t width, uint32_t height,
                                                     uint32_t* rgba) {
     const uint32_t bw = (width + 3) / 4, bh = (height + 3) / 4;
-    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t b = gid >> 4, i = gid & 15u;
     if (b >= bw * bh) return;
-    uint32_t t[16];
-    decode_block(blocks[b], t);
-    const uint32_t x0 = (b % bw) * 4, y0 = (b / bw) * 4;
-    for (uint32_t i = 0; i < 16; ++i) {
-        const uint32_t x = x0 + i % 4, y = y0 + i / 4;
-        if (x < width && y < height) rgba[size_t(y) * width + x] = t[i];
-    }
+    const uint32_t x = (b % bw) * 4 + i % 4, y = (b / bw) * 4 + i / 4;
+    const uint32_t texel = decode_texel(blocks[b], i);
+    if (x < width && y < height) rgba[size_t(y) * width + x] = texel;
 }
 
 // One texel of mip level l + 1 from the 2x2 texels of level l above it (clamped to the level's
@@ -244,10 +230,10 @@ hipError_t launch_build_mip(const uint8_t* src, uint32_t src_w, uint32_t src_h, 
 }
 
 hipError_t launch_bc7_decode(const void* blocks, uint32_t width, uint32_t height, uint32_t* rgba, hipStream_t s) {
-    const uint32_t n = ((width + 3) / 4) * ((height + 3) / 4);
+    const uint64_t n = uint64_t((width + 3) / 4) * ((height + 3) / 4) * 16;  // texels of whole blocks
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_bc7_decode, dim3((n + 255) / 256), dim3(256), 0, s, static_cast<const uint4*>(blocks),
-                       width, height, rgba);
+    hipLaunchKernelGGL(k_bc7_decode, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const uint4*>(blocks), width, height, rgba);
     return hipGetLastError();
 }
 

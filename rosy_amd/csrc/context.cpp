@@ -827,8 +827,7 @@ rsy_result rsy_set_materials(rsy_ctx* c, const rsy_gpu_material* materials, uint
         pool += (mip_offset(d.width, d.height, ch, table[t].levels) + 255) & ~uint64_t(255);
         if (bc7)
             for (uint32_t l = 0; l < given; ++l)
-                staging = std::max<uint64_t>(staging, uint64_t((mip_extent(d.width, l) + 3) / 4) *
-                                                          ((mip_extent(d.height, l) + 3) / 4) * 16);
+                staging += uint64_t((mip_extent(d.width, l) + 3) / 4) * ((mip_extent(d.height, l) + 3) / 4) * 16;
     }
     auto check_img = [&](uint32_t i) { return i == RSY_ABSENT || i < n_textures; };
     for (uint32_t m = 0; m < n_materials; ++m) {
@@ -846,6 +845,7 @@ rsy_result rsy_set_materials(rsy_ctx* c, const rsy_gpu_material* materials, uint
     if (n_materials) HIP_TRY(c, hipMemcpy(c->materials.p, materials, n_materials * sizeof(rsy_gpu_material), hipMemcpyHostToDevice), "upload materials");
     if (n_textures) HIP_TRY(c, hipMemcpy(c->tex_table.p, table.data(), n_textures * sizeof(TexGPU), hipMemcpyHostToDevice), "upload textures");
     if (staging) HIP_TRY(c, c->tex_staging.ensure(staging), "allocate texture staging");
+    uint64_t staged = 0;  // every BC7 image's blocks go to their own part of the staging buffer
     for (uint32_t t = 0; t < n_textures; ++t) {
         // the given levels (BC7 decoded on the GPU), then the built ones, each from the one above
         const rsy_texture_desc& d = textures[t];
@@ -854,16 +854,21 @@ rsy_result rsy_set_materials(rsy_ctx* c, const rsy_gpu_material* materials, uint
         const uint32_t given = d.mip_levels == RSY_MIPS_BUILD ? 1u : std::max(1u, d.mip_levels);
         const uint8_t* src = static_cast<const uint8_t*>(d.pixels);
         uint8_t* base = c->texels.as<uint8_t>() + g.offset;
+        if (bc7) {
+            uint64_t n = 0;
+            for (uint32_t l = 0; l < given; ++l)
+                n += uint64_t((mip_extent(g.width, l) + 3) / 4) * ((mip_extent(g.height, l) + 3) / 4) * 16;
+            HIP_TRY(c, hipMemcpyAsync(c->tex_staging.as<uint8_t>() + staged, src, n, hipMemcpyHostToDevice, c->stream),
+                    "upload BC7 blocks");
+        }
         for (uint32_t l = 0; l < given; ++l) {
             const uint32_t w = mip_extent(g.width, l), h = mip_extent(g.height, l);
             uint8_t* dst = base + mip_offset(g.width, g.height, g.channels, l);
             if (bc7) {
                 const uint64_t n = uint64_t((w + 3) / 4) * ((h + 3) / 4) * 16;
-                HIP_TRY(c, hipMemcpy(c->tex_staging.p, src, n, hipMemcpyHostToDevice), "upload BC7 blocks");
-                HIP_TRY(c, launch_bc7_decode(c->tex_staging.p, w, h, reinterpret_cast<uint32_t*>(dst), c->stream),
-                        "BC7 decode");
-                HIP_TRY(c, hipStreamSynchronize(c->stream), "BC7 decode");
-                src += n;
+                HIP_TRY(c, launch_bc7_decode(c->tex_staging.as<uint8_t>() + staged, w, h, reinterpret_cast<uint32_t*>(dst),
+                                             c->stream), "BC7 decode");
+                staged += n;
             } else {
                 const uint64_t n = uint64_t(w) * h * g.channels;
                 HIP_TRY(c, hipMemcpy(dst, src, n, hipMemcpyHostToDevice), "upload texels");

@@ -590,25 +590,30 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
 // The 4-sample main pass (Graphics.cpp:5566-5575: a 4-sample colour attachment resolved with
 // VK_RESOLVE_MODE_AVERAGE_BIT).  Without sample shading the fragment shader runs once per pixel and
 // primitive, at the pixel centre, and its colour goes to every sample the primitive won; samples
-// no primitive covers keep the clear colour.  Each pixel shades its distinct primitives in turn
-// (the wave loops while any lane has one left, so the decoder's wave-wide MLP sees every lane) and
-// writes the average of its four samples; with a blended pass to follow it also keeps the sample
-// colours.  The debug shadow factor is that of sample 0's primitive.
+// no primitive covers keep the clear colour.  The block's (pixel, distinct primitive) pairs are
+// numbered by a prefix sum and shaded as one list by all 256 threads (so a wave is not held back
+// by its edge pixels), their colours kept in LDS; each pixel then writes the average of its four
+// samples (and, with a blended pass to follow, the sample colours).  The debug shadow factor is
+// that of sample 0's primitive.
 template <bool kNeural, bool kLod>
 __global__ __launch_bounds__(256) void k_shade_ms(ShadeLaunch L) {
     __shared__ float lut[256];
     __shared__ uint32_t tri_base[kLdsDraws];
+    __shared__ uint16_t jobs[1024];       // pixel (thread) << 2 | first sample of the primitive
+    __shared__ float4 job_color[1024];
+    __shared__ float shadow0[256];
+    __shared__ uint32_t keys[4][256];     // the pixel's sample keys, kept for the resolve
+    __shared__ uint32_t scan_lds[4];
     lut[threadIdx.x] = L.srgb_lut[threadIdx.x];
     const uint32_t nd_lds = L.n_draws < kLdsDraws ? L.n_draws : kLdsDraws;
     for (uint32_t i = threadIdx.x; i < nd_lds; i += 256) tri_base[i] = L.draws[i].tri_base;
-    __syncthreads();
 
     uint32_t vi = 0;
     while (vi + 1 < L.n_views && L.views[vi + 1].block_base <= blockIdx.x) ++vi;
     const ShadeView& V = L.views[vi];
     const uint32_t lb = blockIdx.x - V.block_base;
-    const uint32_t x = (lb % V.blocks_x) * 16 + (threadIdx.x & 15);
-    const uint32_t y = (lb / V.blocks_x) * 16 + (threadIdx.x >> 4);
+    const uint32_t bx = (lb % V.blocks_x) * 16, by = (lb / V.blocks_x) * 16;
+    const uint32_t x = bx + (threadIdx.x & 15), y = by + (threadIdx.x >> 4);
     const bool inside = x < V.width && y < V.height;
     const size_t o = size_t(y) * V.width + x, plane = size_t(V.width) * V.height;
     unsigned long long v[4];
@@ -616,22 +621,36 @@ __global__ __launch_bounds__(256) void k_shade_ms(ShadeLaunch L) {
     for (int s = 0; s < 4; ++s) v[s] = inside ? V.vis[s * plane + o] : 0ull;
     const uint32_t k0 = uint32_t(v[0]), k1 = uint32_t(v[1]), k2 = uint32_t(v[2]), k3 = uint32_t(v[3]);
     // bit s: sample s is the first to hold its primitive (or the background)
-    uint32_t todo = inside ? 1u | (k1 != k0 ? 2u : 0u) | (k2 != k0 && k2 != k1 ? 4u : 0u) |
-                                 (k3 != k0 && k3 != k1 && k3 != k2 ? 8u : 0u)
-                           : 0u;
+    const uint32_t todo = inside ? 1u | (k1 != k0 ? 2u : 0u) | (k2 != k0 && k2 != k1 ? 4u : 0u) |
+                                       (k3 != k0 && k3 != k1 && k3 != k2 ? 8u : 0u)
+                                 : 0u;
+    keys[0][threadIdx.x] = k0;
+    keys[1][threadIdx.x] = k1;
+    keys[2][threadIdx.x] = k2;
+    keys[3][threadIdx.x] = k3;
+    uint32_t total;
+    const uint32_t first = block_exclusive_scan(uint32_t(__builtin_popcount(todo)), scan_lds, total);
+    {
+        uint32_t m = todo, at = first;
+        while (m) {
+            jobs[at++] = uint16_t((threadIdx.x << 2) | uint32_t(__builtin_ctz(m)));
+            m &= m - 1u;
+        }
+    }
+    __syncthreads();
     const float4 clear = make_float4(L.clear_color[0], L.clear_color[1], L.clear_color[2], L.clear_color[3]);
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    float shadow0 = 1.f;
-    while (__builtin_amdgcn_ballot_w64(todo != 0u) != 0ull) {
-        const bool live = todo != 0u;
-        const int s = live ? __builtin_ctz(todo) : 0;
-        todo &= todo - 1u;
-        const unsigned long long w = s == 0 ? v[0] : s == 1 ? v[1] : s == 2 ? v[2] : v[3];
-        const uint32_t key = uint32_t(w);
-        const uint32_t same = (k0 == key ? 1u : 0u) | (k1 == key ? 2u : 0u) | (k2 == key ? 4u : 0u) | (k3 == key ? 8u : 0u);
-        const bool covered = live && key != 0u;
+    const uint32_t rounds = (total + 255u) / 256u;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t j = r * 256u + threadIdx.x;
+        const bool live = j < total;
+        const uint32_t job = live ? jobs[j] : 0u;
+        const uint32_t t = job >> 2, smp = job & 3u;
+        const uint32_t jx = bx + (t & 15u), jy = by + (t >> 4);
+        const size_t jo = size_t(jy) * V.width + jx;
+        const unsigned long long w = live ? V.vis[smp * plane + jo] : 0ull;
+        const bool covered = live && uint32_t(w) != 0u;
         PixelGeom pg{};
-        if (covered) pg = pixel_geometry<kLod>(L, V, tri_base, x, y, w);
+        if (covered) pg = pixel_geometry<kLod>(L, V, tri_base, jx, jy, w);
         float dec[kNmOut] = {};
         bool decode = false;
         if constexpr (kNeural) {
@@ -655,22 +674,37 @@ __global__ __launch_bounds__(256) void k_shade_ms(ShadeLaunch L) {
         float4 c = clear;
         float sh = 1.f;
         if (covered) {
-            const Fragment fr = wave_uniform(pg.draw) ? shade_pixel<true, kLod>(L, V, lut, o, pg, decode, dec)
-                                                      : shade_pixel<false, kLod>(L, V, lut, o, pg, decode, dec);
+            const Fragment fr = wave_uniform(pg.draw) ? shade_pixel<true, kLod>(L, V, lut, jo, pg, decode, dec)
+                                                      : shade_pixel<false, kLod>(L, V, lut, jo, pg, decode, dec);
             c = fr.color;
             sh = fr.shadow;
         }
+        job_color[j] = c;
+        if (smp == 0u) shadow0[t] = sh;
+    }
+    __syncthreads();
+    if (!inside) return;
+    // the average of the four samples: each primitive's colour times the samples it holds
+    const uint32_t q0 = keys[0][threadIdx.x], q1 = keys[1][threadIdx.x], q2 = keys[2][threadIdx.x],
+                   q3 = keys[3][threadIdx.x];
+    uint32_t m = 1u | (q1 != q0 ? 2u : 0u) | (q2 != q0 && q2 != q1 ? 4u : 0u) | (q3 != q0 && q3 != q1 && q3 != q2 ? 8u : 0u);
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t at = first;
+    while (m) {
+        const int s = __builtin_ctz(m);
+        m &= m - 1u;
+        const uint32_t key = s == 0 ? q0 : s == 1 ? q1 : s == 2 ? q2 : q3;
+        const uint32_t same = (q0 == key ? 1u : 0u) | (q1 == key ? 2u : 0u) | (q2 == key ? 4u : 0u) | (q3 == key ? 8u : 0u);
+        const float4 c = job_color[at++];
         const float n = float(__builtin_popcount(same));
         acc = make_float4(acc.x + c.x * n, acc.y + c.y * n, acc.z + c.z * n, acc.w + c.w * n);
-        if (s == 0) shadow0 = sh;
         if (V.sample_color)
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
-                if (same & (1u << t)) V.sample_color[t * plane + o] = c;
+            for (int q = 0; q < 4; ++q)
+                if (same & (1u << q)) V.sample_color[q * plane + o] = c;
     }
-    if (!inside) return;
     V.color[o] = make_float4(acc.x * 0.25f, acc.y * 0.25f, acc.z * 0.25f, acc.w * 0.25f);
-    if (V.shadow_factor) V.shadow_factor[o] = shadow0;
+    if (V.shadow_factor) V.shadow_factor[o] = shadow0[threadIdx.x];
 }
 
 // The blended pass (Graphics.cpp:5655-5691): the blended draw list in primitive order over the
