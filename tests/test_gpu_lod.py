@@ -78,3 +78,35 @@ def test_generated_scene_with_mip_chains():
     err, _, _, ok = render_and_compare(s)
     print(f"generated scene with chains: {err:.2e} ({1 - ok.mean():.4f} of pixels on a shadow tie)")
     assert err <= TOL and ok.mean() > 0.99
+
+
+def test_chains_with_four_samples_and_blending():
+    """Mip chains under the 4-sample pass with translucent materials: the 4-sample shading and the
+    blended fragments sample at the level of detail too."""
+    import ctypes as C
+    s = rosy_amd.Scene(3, seed=5, width=192, height=112, shadow_size=512, total_triangles=12000, n_meshes=20,
+                       texture_size=64, n_translucent=8, texture_mips=1)
+    rs = abi.RasterState()
+    C.memmove(C.byref(rs), C.byref(s.s.raster_state), C.sizeof(rs))
+    rs.msaa_samples = 4
+    r = rosy_amd.Renderer(device=0)
+    r.load(s)
+    r.update(s.s.scene_data, rs)
+    r.render(s.frame_desc(abi.SHADE_PLAIN))
+    cam = s.camera_views()[0]
+    vis = r.visibility(0)
+    assert np.array_equal(vis, oracle.raster_view_msaa(s, cam.view_projection[:], cam.width, cam.height, rs))
+    maps = [r.shadow_map(0)]
+    want, _, counts, margin = oracle.msaa_view(s, cam, s.cascades(), maps, vis, raster_state=rs, blended=True,
+                                               max_layers=32)
+    ok = margin >= TIE
+    err = float(np.abs(r.color(0).astype(np.float64) - want)[ok].max())
+    print(f"chains + 4 samples + blending: {err:.2e}, layers up to {counts.max()}")
+    assert err <= TOL and counts.max() >= 2 and ok.mean() > 0.99
+    # the decoder takes no material with chains: the neural frame is the plain one
+    r.neural_init(latent_size=16, psnr_threshold=0.0)
+    r.neural_evaluate()
+    assert r.neural_stats().n_active == 0
+    r.render(s.frame_desc(abi.SHADE_NEURAL))
+    assert float(np.abs(r.color(0).astype(np.float64) - want)[ok].max()) <= TOL
+    r.close()
