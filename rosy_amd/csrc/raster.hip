@@ -5,14 +5,16 @@
 // pipeline.  Device-scope atomics are the scarce resource on MI355X, so every stage is laid out to
 // issue few of them:
 //   k_transform  per (view, vertex) clip = (VP * W) * p from SoA x/y/z (12 B in, 16 B out, coalesced);
-//                                   the block's VP * W is formed once in LDS (fmaf chain, k order)
+//                                   the block's VP * W is formed once in LDS (fmaf chain, k order); the
+//                                   grid also zeroes the frame's counters and bin counts
 //   k_setup      per (view, tri)    clip test, snap, cull, bbox, depth plane, bias; the record goes
 //                                   to the triangle's own slot (no append); bin counts use one device
 //                                   atomic per distinct bin per block (LDS hash table), records that
 //                                   span many bins are spread over the wave
 //   k_clip       per clipped tri    near/far/guard-band Sutherland-Hodgman + fan, one thread each
-//   k_scan_*     all bins           exclusive prefix sum of the bin counts in three launches, the
-//                                   raster work items and the list of bins to clear
+//   k_scan_*     all bins           exclusive prefix sum of the bin counts in two launches (block
+//                                   partials, then every block's offsets from them), the raster work
+//                                   items and the list of bins to clear
 //   k_scatter    per record         bin lists into the slots setup handed out (no atomics); clears
 //                                   the empty and split bins' pixels
 //   k_raster     per work item      one workgroup per non-empty bin (or per 128 records of a crowded
@@ -182,11 +184,18 @@ __device__ inline bool box_outside(const float* m, const float* bmin, const floa
 // chain in k order, then each thread reads its vertex from the SoA x/y/z streams (coalesced 12 B)
 // and writes the clip position (coalesced 16 B).  A draw whose box is outside the view is skipped,
 // and its first block records that for k_setup.
+// Also zeroes the frame's counters and bin counts (`clear` words at `clear_a` then at `clear_b`), spread
+// over the grid, before the setup kernel counts into them.
 __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, const ViewGPU* views, DrawLists lists,
                                                    const float* objects, const float* px, const float* py,
-                                                   const float* pz, float4* xv, uint32_t* draw_culled) {
+                                                   const float* pz, float4* xv, uint32_t* draw_culled,
+                                                   uint32_t* clear_a, uint32_t na, uint32_t* clear_b, uint32_t nb) {
     __shared__ float m[16];
     __shared__ uint32_t culled;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < na + nb; i += gridDim.x * 256) {
+        if (i < na) clear_a[i] = 0u;
+        else clear_b[i - na] = 0u;
+    }
     const BlockEntry e = blocks[blockIdx.x];
     const ViewGPU& V = views[e.view];
     const DrawGPU& dr = lists.of(V.list)[e.draw];
@@ -538,63 +547,63 @@ __global__ __launch_bounds__(1024) void k_scan_partial(const uint32_t* counts, u
     }
 }
 
-// One block: exclusive scan of the block partials (in place, as the blocks' bases) and the totals.
-__global__ __launch_bounds__(1024) void k_scan_top(ScanPartial* partial, uint32_t n_parts,
-                                                   const uint32_t* block_valid, uint32_t n_blocks,
-                                                   uint32_t list_cap, uint32_t items_cap, uint32_t* totals) {
+// Every block scans the block partials itself (there are few: one per kScanChunk bins) for its
+// base and the totals; block 0 writes the totals.  Then the block's bins get their list offsets,
+// raster work items and clear-list entries.
+__global__ __launch_bounds__(1024) void k_scan_final(const uint32_t* counts, uint32_t n, uint32_t* offsets,
+                                                     const ScanPartial* partial, uint32_t n_parts,
+                                                     const uint32_t* block_valid, uint32_t n_blocks,
+                                                     uint32_t list_cap, uint32_t* totals, uint4* items,
+                                                     uint32_t items_cap, uint32_t* zero_bins) {
     __shared__ uint32_t lds[16];
     __shared__ uint64_t lds64[16];
+    __shared__ ScanPartial base_lds;
     uint64_t carry_a = 0, carry_01 = 0, carry_23 = 0;
-    for (uint32_t base = 0; base < n_parts; base += kScanThreads) {
-        const uint32_t i = base + threadIdx.x;
+    for (uint32_t at = 0; at < n_parts; at += kScanThreads) {
+        const uint32_t i = at + threadIdx.x;
         ScanPartial p{};
         if (i < n_parts) p = partial[i];
         // class counts of a block fit 16 bits, their sums over blocks 32 bits: two classes per scan
         const uint64_t v01 = uint64_t(p.f[0]) | (uint64_t(p.f[1]) << 32);
         const uint64_t v23 = uint64_t(p.f[2]) | (uint64_t(p.f[3]) << 32);
         uint64_t ta, t01, t23;
-        const uint64_t ea = block_exclusive_scan64(p.a, lds64, ta);
+        const uint64_t ea = carry_a + block_exclusive_scan64(p.a, lds64, ta);
         const uint64_t e01 = carry_01 + block_exclusive_scan64(v01, lds64, t01);
         const uint64_t e23 = carry_23 + block_exclusive_scan64(v23, lds64, t23);
-        __syncthreads();
-        if (i < n_parts) {
+        if (i == blockIdx.x) {
             ScanPartial q{};
-            q.a = carry_a + ea;
+            q.a = ea;
             q.f[0] = uint32_t(e01);
             q.f[1] = uint32_t(e01 >> 32);
             q.f[2] = uint32_t(e23);
             q.f[3] = uint32_t(e23 >> 32);
-            partial[i] = q;
+            base_lds = q;
         }
         carry_a += ta;
         carry_01 += t01;
         carry_23 += t23;
     }
-    uint32_t vs = 0;
-    for (uint32_t i = threadIdx.x; i < n_blocks; i += kScanThreads) vs += block_valid[i];
-    uint32_t total_valid;
-    block_exclusive_scan(vs, lds, total_valid);
-    if (threadIdx.x == 0) {
-        const uint32_t pairs = uint32_t(carry_a), n_split = uint32_t(carry_a >> 32);
-        const uint32_t n_zero = uint32_t(carry_01), n_heavy = uint32_t(carry_01 >> 32);
-        const uint32_t n_mid = uint32_t(carry_23), n_light = uint32_t(carry_23 >> 32);
-        const uint64_t n_items = uint64_t(n_heavy) + n_mid + n_split + n_light;
-        totals[0] = pairs;
-        totals[1] = total_valid;
-        totals[2] = (pairs > list_cap || n_items > items_cap) ? 1u : 0u;
-        totals[3] = uint32_t(min<uint64_t>(n_items, items_cap));
-        totals[4] = n_heavy;                  // first mid item
-        totals[5] = n_zero;
-        totals[6] = n_heavy + n_mid;          // first split item
-        totals[7] = n_heavy + n_mid + n_split;  // first light item
+    const uint32_t n_split = uint32_t(carry_a >> 32);
+    const uint32_t n_heavy = uint32_t(carry_01 >> 32), n_mid = uint32_t(carry_23);
+    if (blockIdx.x == 0) {
+        uint32_t vs = 0;
+        for (uint32_t i = threadIdx.x; i < n_blocks; i += kScanThreads) vs += block_valid[i];
+        uint32_t total_valid;
+        block_exclusive_scan(vs, lds, total_valid);
+        if (threadIdx.x == 0) {
+            const uint32_t pairs = uint32_t(carry_a), n_zero = uint32_t(carry_01), n_light = uint32_t(carry_23 >> 32);
+            const uint64_t n_items = uint64_t(n_heavy) + n_mid + n_split + n_light;
+            totals[0] = pairs;
+            totals[1] = total_valid;
+            totals[2] = (pairs > list_cap || n_items > items_cap) ? 1u : 0u;
+            totals[3] = uint32_t(min<uint64_t>(n_items, items_cap));
+            totals[4] = n_heavy;                    // first mid item
+            totals[5] = n_zero;
+            totals[6] = n_heavy + n_mid;            // first split item
+            totals[7] = n_heavy + n_mid + n_split;  // first light item
+        }
     }
-}
-
-__global__ __launch_bounds__(1024) void k_scan_final(const uint32_t* counts, uint32_t n, uint32_t* offsets,
-                                                     const ScanPartial* partial, const uint32_t* totals,
-                                                     uint4* items, uint32_t items_cap, uint32_t* zero_bins) {
-    __shared__ uint32_t lds[16];
-    __shared__ uint64_t lds64[16];
+    __syncthreads();
     const uint32_t i0 = blockIdx.x * kScanChunk + threadIdx.x * kScanPerThread;
     uint32_t v[kScanPerThread];
     uint64_t a = 0, b = 0;
@@ -604,16 +613,16 @@ __global__ __launch_bounds__(1024) void k_scan_final(const uint32_t* counts, uin
         if (i0 + k < n) bin_fields(v[k], a, b);
     }
     uint64_t ta, tb;
-    const ScanPartial base = partial[blockIdx.x];
+    const ScanPartial base = base_lds;
     const uint64_t ea = base.a + block_exclusive_scan64(a, lds64, ta);
     const uint64_t eb = block_exclusive_scan64(b, lds64, tb);
     uint32_t run = uint32_t(ea);
-    uint64_t r_split = uint64_t(totals[6]) + uint32_t(ea >> 32);
+    uint64_t r_split = uint64_t(n_heavy) + n_mid + uint32_t(ea >> 32);
     uint32_t r_zero = base.f[0] + uint32_t(eb & 0xffffu);
     // next item slot of each whole-bin class: the class's first item + earlier blocks + this block
     uint64_t r_heavy = base.f[1] + ((eb >> 16) & 0xffffu);
-    uint64_t r_mid = uint64_t(totals[4]) + base.f[2] + ((eb >> 32) & 0xffffu);
-    uint64_t r_light = uint64_t(totals[7]) + base.f[3] + ((eb >> 48) & 0xffffu);
+    uint64_t r_mid = uint64_t(n_heavy) + base.f[2] + ((eb >> 32) & 0xffffu);
+    uint64_t r_light = uint64_t(n_heavy) + n_mid + n_split + base.f[3] + ((eb >> 48) & 0xffffu);
 #pragma unroll
     for (uint32_t k = 0; k < kScanPerThread; ++k)
         if (i0 + k < n) {
@@ -802,12 +811,15 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s) {
 hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
     if (g.n_views == 0) return hipSuccess;
     const DrawLists lists{{g.draws0, g.draws1, g.draws2}};
+    // the transform clears the counters and bin counts; without vertices a clear of its own
     const uint32_t nclear = g.n_views * kCtrPerView + g.total_bins;
-    hipLaunchKernelGGL(k_clear2, dim3(min((nclear + 255) / 256, 2048u)), dim3(256), 0, s, g.counters,
-                       g.n_views * kCtrPerView, g.bin_counts, g.total_bins);
     if (g.n_vblocks)
         hipLaunchKernelGGL(k_transform, dim3(g.n_vblocks), dim3(256), 0, s, g.vblocks, g.views, lists,
-                           g.objects, g.px, g.py, g.pz, g.xv, g.draw_culled);
+                           g.objects, g.px, g.py, g.pz, g.xv, g.draw_culled, g.counters, g.n_views * kCtrPerView,
+                           g.bin_counts, g.total_bins);
+    else
+        hipLaunchKernelGGL(k_clear2, dim3(min((nclear + 255) / 256, 2048u)), dim3(256), 0, s, g.counters,
+                           g.n_views * kCtrPerView, g.bin_counts, g.total_bins);
     if (g.n_tblocks)
         hipLaunchKernelGGL(k_setup, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, lists,
                            g.indices, g.xv, g.recs, g.boxes, g.clipq, g.counters, g.bin_counts, g.pair_slots,
@@ -821,10 +833,9 @@ hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
     const uint32_t parts = (b.total_bins + kScanChunk - 1) / kScanChunk;
     ScanPartial* partial = reinterpret_cast<ScanPartial*>(b.scan_partial);
     hipLaunchKernelGGL(k_scan_partial, dim3(parts), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, partial);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, partial, parts, b.block_valid, b.n_tblocks,
-                       b.list_cap, b.items_cap, b.totals);
     hipLaunchKernelGGL(k_scan_final, dim3(parts), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, b.bin_offsets,
-                       partial, b.totals, b.items, b.items_cap, b.zero_bins);
+                       partial, parts, b.block_valid, b.n_tblocks, b.list_cap, b.totals, b.items, b.items_cap,
+                       b.zero_bins);
     hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.n_views, b.boxes,
                        b.counters, b.bin_offsets, b.pair_slots, b.big_pairs, b.lists, b.list_cap, b.shift, b.totals,
                        b.zero_bins);
