@@ -145,6 +145,39 @@ def msaa_timing(r, scene, plain, neural, args, check):
     return out
 
 
+def copy_ceiling_gbs(mib=1024, reps=10):
+    """A measured HBM ceiling on this box: device-to-device copies of `mib` MiB (read + write bytes per
+    second, HIP events), the best of `reps`.  Through the HIP runtime the library itself uses."""
+    import ctypes as C
+
+    import rosy_amd
+    rosy_amd.lib()  # the library's HIP runtime is loaded: use the same one
+    hip = C.CDLL("libamdhip64.so.7")  # the soname the library links
+    n = mib << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), C.c_size_t(n)) or hip.hipMalloc(C.byref(dst), C.c_size_t(n)):
+        return None
+    hip.hipEventCreate(C.byref(e0))
+    hip.hipEventCreate(C.byref(e1))
+    hip.hipMemset(src, 1, C.c_size_t(n))
+    best = 0.0
+    for _ in range(reps):
+        hip.hipEventRecord(e0, None)
+        hip.hipMemcpyAsync(dst, src, C.c_size_t(n), 3, None)  # hipMemcpyDeviceToDevice
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = C.c_float()
+        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+        if ms.value > 0:
+            best = max(best, 2.0 * n / (ms.value * 1e-3) / 1e9)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return round(best, 1)
+
+
 def texture_timing(args):
     """BC7 images with full chains uploaded through rsy_set_materials (64 x 1024^2, random blocks of
     every mode: host-to-device copy plus GPU decode, wall clock), and the plain decoder-workload frame
@@ -329,7 +362,8 @@ def main():
     roofline = {"kernel": "k_raster (tile raster, camera + shadow views in one launch)", "bound": "hbm",
                 "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(tile),
-                "algorithmic_bytes": raster_bytes, "avg_launch_ms": round(raster_ms, 5)}
+                "algorithmic_bytes": raster_bytes, "avg_launch_ms": round(raster_ms, 5),
+                "measured_copy_gbs": copy_ceiling_gbs()}
 
     # decoder MFMA work: decoded pixels x 1088 FLOPs over the neural shading kernel
     active = np.zeros(scene.s.n_materials + 1, bool)
