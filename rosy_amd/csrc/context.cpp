@@ -245,6 +245,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     HIP_TRY(c, c->bin_offsets.ensure(size_t(L.total_bins) * 4), "allocate bins");
     HIP_TRY(c, c->pair_slots.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * 16), "allocate bin slots");
     HIP_TRY(c, c->boxes.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * 8), "allocate record boxes");
+    HIP_TRY(c, c->shade_recs.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * sizeof(ShadeRec)), "allocate shading records");
     HIP_TRY(c, c->big_pairs.ensure(size_t(std::max<uint32_t>(1, L.big_total)) * 16), "allocate bin pairs");
     HIP_TRY(c, c->block_valid.ensure(std::max<size_t>(1, L.tblocks.size()) * 4), "allocate statistics");
     HIP_TRY(c, c->totals.ensure(32), "allocate statistics");
@@ -330,7 +331,7 @@ rsy_result upload_frame(rsy_ctx* c) {
             ShadeView sv = c->h_shade_views[camera_of[vg.source]];
             sv.vis = nullptr;
             sv.recs = c->recs.as<TriRec>() + vg.rec_base;
-            sv.boxes = c->boxes.as<uint2>() + vg.rec_base;
+            sv.shade_recs = c->shade_recs.as<ShadeRec>() + vg.rec_base;
             sv.xv_base = vg.xv_base;
             sv.shadow_factor = nullptr;
             c->h_blend_views.push_back(sv);
@@ -350,7 +351,7 @@ rsy_result upload_frame(rsy_ctx* c) {
             blocks += sv.blocks_x * ((vg.height + 15) / 16);
             sv.vis = reinterpret_cast<const unsigned long long*>(vg.out);
             sv.recs = c->recs.as<TriRec>() + vg.rec_base;
-            sv.boxes = c->boxes.as<uint2>() + vg.rec_base;
+            sv.shade_recs = c->shade_recs.as<ShadeRec>() + vg.rec_base;
             sv.xv_base = vg.xv_base;
             sv.samples = vg.samples;
             sv.color = c->color_out[ci].as<float4>();
@@ -416,6 +417,7 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     g.xv = c->xv.as<float4>();
     g.recs = c->recs.as<TriRec>();
     g.boxes = c->boxes.as<uint2>();
+    g.shade_recs = c->shade_recs.as<ShadeRec>();
     g.clipq = c->clipq.as<uint2>();
     g.counters = c->counters.as<uint32_t>();
     g.bin_counts = c->bin_counts.as<uint32_t>();

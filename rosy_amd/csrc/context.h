@@ -178,7 +178,7 @@ struct rsy_ctx {
 
     // frame buffers
     rsy::Layout layout;
-    rsy::DevBuf views_d, xv, recs, boxes, clipq, counters, bin_counts, bin_offsets, pair_slots, big_pairs, block_valid, totals,
+    rsy::DevBuf views_d, xv, recs, boxes, shade_recs, clipq, counters, bin_counts, bin_offsets, pair_slots, big_pairs, block_valid, totals,
         lists_mem, vblocks_d, tblocks_d, shade_views_d, shade_cascades_d, items, zero_bins,
         draw_culled, scan_partial;
     uint32_t list_cap = 0;

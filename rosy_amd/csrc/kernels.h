@@ -30,6 +30,7 @@ struct GeometryLaunch {
     float4* xv;
     TriRec* recs;
     uint2* boxes;           // per record slot: the record's (bbx, bby), or kInvalidBox twice
+    ShadeRec* shade_recs;   // per record slot of a camera or blend view: its shading record
     uint2* clipq;
     uint32_t* counters;
     uint32_t* bin_counts;
@@ -112,7 +113,7 @@ struct ShadeView {
     uint32_t blocks_x;
     const unsigned long long* vis;
     const TriRec* recs;    // the raster view's record slots, indexed by triangle id
-    const uint2* boxes;    // their boxes (kInvalidBox: no record, e.g. a clipped triangle)
+    const ShadeRec* shade_recs;  // the raster view's shading records, indexed by triangle id
     uint32_t xv_base;      // first transformed vertex of the raster view
     uint32_t samples;      // 1, or 4: vis holds one plane per sample and colour is their resolve
     float4* color;

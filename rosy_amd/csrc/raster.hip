@@ -275,7 +275,8 @@ constexpr uint32_t kHash = 1024;
 constexpr uint32_t kEmpty = 0xffffffffu;
 
 __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const ViewGPU* views, DrawLists lists,
-                                               const uint32_t* indices, const float4* xv, TriRec* recs, uint2* boxes, uint2* clipq, uint32_t* counters,
+                                               const uint32_t* indices, const float4* xv, TriRec* recs, uint2* boxes,
+                                               ShadeRec* shade_recs, uint2* clipq, uint32_t* counters,
                                                uint32_t* bin_counts, uint32_t* pair_slots, uint4* big_pairs,
                                                uint32_t* block_valid, const uint32_t* draw_culled, uint32_t shift) {
     __shared__ uint32_t hkey[kHash];
@@ -298,12 +299,16 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     const bool valid = t < dr.tri_count;
     int status = 0;  // 0 reject, 1 accept, 2 clip
     float4 c[3];
+    uint32_t vi[3] = {0u, 0u, 0u};
     if (valid) {
         const uint32_t* ip = indices + dr.first_index + 3u * t;
+        vi[0] = ip[0];
+        vi[1] = ip[1];
+        vi[2] = ip[2];
         const float4* base = xv + size_t(V.xv_base) + dr.xv_off;
-        c[0] = base[ip[0]];
-        c[1] = base[ip[1]];
-        c[2] = base[ip[2]];
+        c[0] = base[vi[0]];
+        c[1] = base[vi[1]];
+        c[2] = base[vi[2]];
         bool reject = false;
 #pragma unroll
         for (int plane = 0; plane < 6; ++plane) {
@@ -319,6 +324,13 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     if (valid) {
         if (has) recs[slot] = r;
         boxes[slot] = has ? make_uint2(r.bbx, r.bby) : make_uint2(kInvalidBox, kInvalidBox);
+        // a camera or blend view's triangle that can be visible: its shading record
+        if (!V.is_shadow && status != 0) {
+            uint4* sr = reinterpret_cast<uint4*>(shade_recs + slot);
+            sr[0] = make_uint4(dr.vtx_base + vi[0], dr.vtx_base + vi[1], dr.vtx_base + vi[2],
+                               e.draw | (has ? 0u : kShadeNoRecord));
+            sr[1] = make_uint4(__float_as_uint(c[0].w), __float_as_uint(c[1].w), __float_as_uint(c[2].w), 0u);
+        }
     }
     const uint32_t cslot = wave_append(status == 2, &ctr[kCtrClip]);
     if (status == 2) {
@@ -822,7 +834,7 @@ hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
                            g.n_views * kCtrPerView, g.bin_counts, g.total_bins);
     if (g.n_tblocks)
         hipLaunchKernelGGL(k_setup, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, lists,
-                           g.indices, g.xv, g.recs, g.boxes, g.clipq, g.counters, g.bin_counts, g.pair_slots,
+                           g.indices, g.xv, g.recs, g.boxes, g.shade_recs, g.clipq, g.counters, g.bin_counts, g.pair_slots,
                            g.big_pairs, g.block_valid, g.draw_culled, g.shift);
     hipLaunchKernelGGL(k_clip, dim3(g.clip_blocks, g.n_views), dim3(kClipThreads), 0, s, g.views, lists,
                        g.indices, g.xv, g.clipq, g.recs, g.counters, g.bin_counts, g.big_pairs, g.shift);

@@ -22,8 +22,6 @@
 namespace rsy {
 namespace {
 
-constexpr uint32_t kLdsDraws = 2048;
-
 struct f3 {
     float x, y, z;
 };
@@ -198,8 +196,8 @@ __device__ inline float schlick_ggx(float ndv, float r) {
 
 // Visible triangle of a covered pixel: its draw, vertex indices, barycentrics and cascade.
 struct PixelGeom {
-    uint32_t draw;        // index in the opaque draw list
-    uint32_t i0, i1, i2;  // mesh-local vertex indices
+    uint32_t draw;        // index in the view's draw list
+    uint32_t v0, v1, v2;  // indices into the global vertex buffer
     f3 lam;               // perspective-correct barycentrics at the pixel centre
     f3 dlx, dly;          // with kLod: their derivatives along the pixel's x and y
     int cascade;
@@ -208,26 +206,23 @@ struct PixelGeom {
 // kLod: also the barycentrics' screen derivatives (for the texture level of detail), exact for the
 // same rational form: lam = n / S with n linear over w, so d(lam) = (dn - lam dS) / S.
 template <bool kLod = false>
-__device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView& V, const uint32_t* tri_base,
-                                           uint32_t x, uint32_t y, unsigned long long vis) {
+__device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView& V, uint32_t x, uint32_t y,
+                                           unsigned long long vis) {
     PixelGeom pg;
     const uint32_t tri = uint32_t(vis & 0xffffffffull) - 1u;
-    // draw lookup: last draw whose tri_base <= tri
-    uint32_t lo = 0, hi = L.n_draws;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        const uint32_t b = mid < kLdsDraws ? tri_base[mid] : L.draws[mid].tri_base;
-        if (b <= tri) lo = mid; else hi = mid;
-    }
-    pg.draw = lo;
-    const DrawGPU dr = L.draws[lo];
-    const uint32_t local = tri - dr.tri_base;
-    const uint32_t* ip = L.indices + dr.first_index + 3u * local;
-    pg.i0 = ip[0];
-    pg.i1 = ip[1];
-    pg.i2 = ip[2];
-    const float4* xv = L.xv + V.xv_base + dr.xv_off;
-    const float w0 = xv[pg.i0].w, w1 = xv[pg.i1].w, w2 = xv[pg.i2].w;
+    // the triangle's shading record (vertices, draw, clip w) and its raster record, loaded together
+    // (a clipped triangle's raster slot holds stale data, unused)
+    const uint4* sr = reinterpret_cast<const uint4*>(V.shade_recs + tri);
+    const uint4 sa = sr[0], sb = sr[1];
+    const uint4* rq = reinterpret_cast<const uint4*>(V.recs + tri);
+    const uint4 q0 = rq[0];  // X0, Y0, X1, Y1
+    const uint2 q1 = *reinterpret_cast<const uint2*>(rq + 1);  // X2, Y2
+    const uint32_t id = reinterpret_cast<const uint32_t*>(rq + 2)[1];
+    pg.draw = sa.w & ~kShadeNoRecord;
+    pg.v0 = sa.x;
+    pg.v1 = sa.y;
+    pg.v2 = sa.z;
+    const float w0 = __uint_as_float(sb.x), w1 = __uint_as_float(sb.y), w2 = __uint_as_float(sb.z);
 
     // Perspective-correct barycentrics at the pixel centre.  A triangle that needed no clipping has
     // a record in the triangle's slot (its box says so): the exact integer edge functions of the
@@ -236,11 +231,7 @@ __device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView
     // edge functions and swaps the last two.  A clipped triangle's slot is empty; its barycentrics
     // come from b ~ cross(x - u w, y - v w) over the clip positions.
     f3 lam;
-    if (V.boxes[tri].x != kInvalidBox) {
-        const uint4* rq = reinterpret_cast<const uint4*>(V.recs + tri);
-        const uint4 q0 = rq[0];  // X0, Y0, X1, Y1
-        const uint2 q1 = *reinterpret_cast<const uint2*>(rq + 1);  // X2, Y2
-        const uint32_t id = reinterpret_cast<const uint32_t*>(rq + 2)[1];
+    if ((sa.w & kShadeNoRecord) == 0u) {
         const int32_t X0 = int32_t(q0.x), Y0 = int32_t(q0.y), X1 = int32_t(q0.z), Y1 = int32_t(q0.w);
         const int32_t X2 = int32_t(q1.x), Y2 = int32_t(q1.y);
         const int32_t Px = int32_t(x) * 256 + 128, Py = int32_t(y) * 256 + 128;
@@ -259,7 +250,9 @@ __device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView
             pg.dly = mk(ey0 * (w1 * w2), (swapped ? ey2 : ey1) * (w0 * w2), (swapped ? ey1 : ey2) * (w0 * w1));
         }
     } else {
-        const float4 c0 = xv[pg.i0], c1 = xv[pg.i1], c2 = xv[pg.i2];
+        const DrawGPU dr = L.draws[pg.draw];
+        const float4* xv = L.xv + V.xv_base + dr.xv_off;
+        const float4 c0 = xv[pg.v0 - dr.vtx_base], c1 = xv[pg.v1 - dr.vtx_base], c2 = xv[pg.v2 - dr.vtx_base];
         const float u = (float(x) + 0.5f) * (2.0f / float(V.width)) - 1.0f;
         const float v = (float(y) + 0.5f) * (2.0f / float(V.height)) - 1.0f;
         const f3 a = mk(c0.x - u * c0.w, c1.x - u * c1.w, c2.x - u * c2.w);
@@ -339,12 +332,11 @@ __device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V,
     const rsy_position* verts = static_cast<const rsy_position*>(L.vertices);
     const auto samplers = R::ptr(static_cast<const rsy_sampler*>(L.samplers));
     const auto textures = R::ptr(L.textures);
-    const uint32_t i0 = pg.i0, i1 = pg.i1, i2 = pg.i2;
     const f3 lam = pg.lam;
     const int cascade = pg.cascade;
     const float* cm = cascade >= 0 ? L.cascades[cascade].m : nullptr;
     const uint32_t nti = has_material ? md->normal_sampled_image_index : RSY_ABSENT;
-    const rsy_position* const vtx[3] = {&verts[dr.vtx_base + i0], &verts[dr.vtx_base + i1], &verts[dr.vtx_base + i2]};
+    const rsy_position* const vtx[3] = {&verts[pg.v0], &verts[pg.v1], &verts[pg.v2]};
     const bool shift_uv = has_material && !uses_clamp(md, samplers, L.n_samplers);
     const Varyings bv = pixel_varyings(sd, vtx, gd, has_material, nti, cm, lam.x, lam.y, lam.z, shift_uv);
     const f3 bn = bv.normal, bwn = bv.world_normal, bview = bv.view_dir, blight = bv.light_dir;
@@ -526,10 +518,7 @@ __device__ inline bool wave_uniform(uint32_t v) {
 template <bool kNeural, bool kLod>
 __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
     __shared__ float lut[256];
-    __shared__ uint32_t tri_base[kLdsDraws];
     lut[threadIdx.x] = L.srgb_lut[threadIdx.x];
-    const uint32_t nd_lds = L.n_draws < kLdsDraws ? L.n_draws : kLdsDraws;
-    for (uint32_t i = threadIdx.x; i < nd_lds; i += 256) tri_base[i] = L.draws[i].tri_base;
     __syncthreads();
 
     uint32_t vi = 0;
@@ -549,7 +538,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
             return;
         }
         const float none[kNmOut] = {};
-        const PixelGeom pg = pixel_geometry<kLod>(L, V, tri_base, x, y, vis);
+        const PixelGeom pg = pixel_geometry<kLod>(L, V, x, y, vis);
         store_fragment(V, o, wave_uniform(pg.draw) ? shade_pixel<true, kLod>(L, V, lut, o, pg, false, none)
                                                    : shade_pixel<false, kLod>(L, V, lut, o, pg, false, none));
     } else {
@@ -560,14 +549,14 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
         bool decode = false;
         float f[kNmFeat] = {};
         if (vis != 0ull) {
-            pg = pixel_geometry<kLod>(L, V, tri_base, x, y, vis);
+            pg = pixel_geometry<kLod>(L, V, x, y, vis);
             const DrawGPU dr = L.draws[pg.draw];
             if (dr.material < L.n_materials && L.nm_active[dr.material]) {
                 decode = true;
                 const rsy_position* verts = static_cast<const rsy_position*>(L.vertices);
-                const float* t0 = verts[dr.vtx_base + pg.i0].texture_coordinates;
-                const float* t1 = verts[dr.vtx_base + pg.i1].texture_coordinates;
-                const float* t2 = verts[dr.vtx_base + pg.i2].texture_coordinates;
+                const float* t0 = verts[pg.v0].texture_coordinates;
+                const float* t1 = verts[pg.v1].texture_coordinates;
+                const float* t2 = verts[pg.v2].texture_coordinates;
                 // relative to vertex 0 with whole even periods removed (the grid repeats with period 1)
                 const float u = (t0[0] - 2.0f * floorf(t0[0] * 0.5f)) + pg.lam.y * (t1[0] - t0[0]) + pg.lam.z * (t2[0] - t0[0]);
                 const float v = (t0[1] - 2.0f * floorf(t0[1] * 0.5f)) + pg.lam.y * (t1[1] - t0[1]) + pg.lam.z * (t2[1] - t0[1]);
@@ -598,15 +587,12 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
 template <bool kNeural, bool kLod>
 __global__ __launch_bounds__(256) void k_shade_ms(ShadeLaunch L) {
     __shared__ float lut[256];
-    __shared__ uint32_t tri_base[kLdsDraws];
     __shared__ uint16_t jobs[1024];       // pixel (thread) << 2 | first sample of the primitive
     __shared__ float4 job_color[1024];
     __shared__ float shadow0[256];
     __shared__ uint32_t keys[4][256];     // the pixel's sample keys, kept for the resolve
     __shared__ uint32_t scan_lds[4];
     lut[threadIdx.x] = L.srgb_lut[threadIdx.x];
-    const uint32_t nd_lds = L.n_draws < kLdsDraws ? L.n_draws : kLdsDraws;
-    for (uint32_t i = threadIdx.x; i < nd_lds; i += 256) tri_base[i] = L.draws[i].tri_base;
 
     uint32_t vi = 0;
     while (vi + 1 < L.n_views && L.views[vi + 1].block_base <= blockIdx.x) ++vi;
@@ -650,7 +636,7 @@ __global__ __launch_bounds__(256) void k_shade_ms(ShadeLaunch L) {
         const unsigned long long w = live ? V.vis[smp * plane + jo] : 0ull;
         const bool covered = live && uint32_t(w) != 0u;
         PixelGeom pg{};
-        if (covered) pg = pixel_geometry<kLod>(L, V, tri_base, jx, jy, w);
+        if (covered) pg = pixel_geometry<kLod>(L, V, jx, jy, w);
         float dec[kNmOut] = {};
         bool decode = false;
         if constexpr (kNeural) {
@@ -660,9 +646,9 @@ __global__ __launch_bounds__(256) void k_shade_ms(ShadeLaunch L) {
                 if (dr.material < L.n_materials && L.nm_active[dr.material]) {
                     decode = true;
                     const rsy_position* verts = static_cast<const rsy_position*>(L.vertices);
-                    const float* t0 = verts[dr.vtx_base + pg.i0].texture_coordinates;
-                    const float* t1 = verts[dr.vtx_base + pg.i1].texture_coordinates;
-                    const float* t2 = verts[dr.vtx_base + pg.i2].texture_coordinates;
+                    const float* t0 = verts[pg.v0].texture_coordinates;
+                    const float* t1 = verts[pg.v1].texture_coordinates;
+                    const float* t2 = verts[pg.v2].texture_coordinates;
                     const float u = (t0[0] - 2.0f * floorf(t0[0] * 0.5f)) + pg.lam.y * (t1[0] - t0[0]) + pg.lam.z * (t2[0] - t0[0]);
                     const float vv = (t0[1] - 2.0f * floorf(t0[1] * 0.5f)) + pg.lam.y * (t1[1] - t0[1]) + pg.lam.z * (t2[1] - t0[1]);
                     nm_features(L.nm_lat16 + size_t(L.nm_slot[dr.material]) * L.nm_L * L.nm_L, L.nm_L, u, vv, f);
@@ -834,10 +820,7 @@ __global__ __launch_bounds__(256) void k_blend_collect(const BlendLaunch B) {
 template <bool kLod>
 __global__ __launch_bounds__(256) void k_blend_shade(const BlendLaunch B, const ShadeLaunch L) {
     __shared__ float lut[256];
-    __shared__ uint32_t tri_base[kLdsDraws];
     lut[threadIdx.x] = L.srgb_lut[threadIdx.x];
-    const uint32_t nd_lds = L.n_draws < kLdsDraws ? L.n_draws : kLdsDraws;
-    for (uint32_t i = threadIdx.x; i < nd_lds; i += 256) tri_base[i] = L.draws[i].tri_base;
     __syncthreads();
     const uint32_t n = min(B.counters[B.first_blend * kCtrPerView + kCtrFrags], B.frag_cap);
     const float none[kNmOut] = {};
@@ -847,7 +830,7 @@ __global__ __launch_bounds__(256) void k_blend_shade(const BlendLaunch B, const 
         const ShadeView& SV = L.views[f.z];
         const uint32_t x = f.y & 0xffffu, y = f.y >> 16;
         const size_t o = size_t(y) * SV.width + x;
-        const PixelGeom pg = pixel_geometry<kLod>(L, SV, tri_base, x, y, (unsigned long long)f.x);
+        const PixelGeom pg = pixel_geometry<kLod>(L, SV, x, y, (unsigned long long)f.x);
         const Fragment fr = wave_uniform(pg.draw) ? shade_pixel<true, kLod>(L, SV, lut, o, pg, false, none)
                                                   : shade_pixel<false, kLod>(L, SV, lut, o, pg, false, none);
         B.frag_color[f.w] = fr.discard ? make_float4(0.f, 0.f, 0.f, __uint_as_float(kDiscarded)) : fr.color;
