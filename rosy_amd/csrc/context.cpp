@@ -250,7 +250,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     HIP_TRY(c, c->block_valid.ensure(std::max<size_t>(1, L.tblocks.size()) * 4), "allocate statistics");
     HIP_TRY(c, c->totals.ensure(32), "allocate statistics");
     HIP_TRY(c, c->zero_bins.ensure(size_t(std::max<uint32_t>(1, L.total_bins)) * 4), "allocate bins");
-    HIP_TRY(c, c->scan_partial.ensure((size_t(L.total_bins) / 4096 + 1) * kScanPartialBytes), "allocate bins");
+    HIP_TRY(c, c->scan_partial.ensure((size_t(L.total_bins) / kScanBlockBins + 1) * kScanPartialBytes), "allocate bins");
     HIP_TRY(c, c->vblocks_d.ensure(std::max<size_t>(1, L.vblocks.size()) * sizeof(BlockEntry)), "allocate blocks");
     HIP_TRY(c, c->tblocks_d.ensure(std::max<size_t>(1, L.tblocks.size()) * sizeof(BlockEntry)), "allocate blocks");
     if (!L.vblocks.empty())

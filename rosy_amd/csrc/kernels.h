@@ -63,7 +63,7 @@ struct BinLaunch {
     uint32_t list_cap;
     uint32_t shift;
     uint32_t scatter_blocks;
-    void* scan_partial;     // 16 bytes per 4096 bins
+    void* scan_partial;     // kScanPartialBytes per scan block
     uint4* items;           // raster work items (bin, first list entry, records, split): split bins' first
     uint32_t items_cap;     // items capacity
     uint32_t* zero_bins;    // bins whose pixels the scatter clears: empty bins and split bins
@@ -245,7 +245,8 @@ hipError_t launch_build_mip(const uint8_t* src, uint32_t src_w, uint32_t src_h, 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
 hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s);
 hipError_t launch_binning(const BinLaunch& b, hipStream_t s);
-constexpr uint32_t kScanPartialBytes = 32;  // per 4096 bins
+constexpr uint32_t kScanBlockBins = 1024;    // bins per scan block
+constexpr uint32_t kScanPartialBytes = 32;  // per scan block
 hipError_t launch_raster(const RasterLaunch& r, hipStream_t s);
 uint32_t raster_items_cap(uint32_t total_bins, uint32_t list_cap);  // work-item table entries
 // output stage: colour -> RGBA16F draw image (sw x sh) -> linear blit to B8G8R8A8_SRGB (dw x dh)

@@ -471,7 +471,7 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(const ViewGPU* views, Dra
 }
 
 // Bin scan geometry: each scan block covers kScanPerThread bins per thread.
-constexpr uint32_t kScanThreads = 1024;
+constexpr uint32_t kScanThreads = 256;
 constexpr uint32_t kScanPerThread = 4;
 
 // Raster work items: a bin with 1..kItemRecs records is one item (its tile is stored whole); a
@@ -505,19 +505,21 @@ __device__ inline uint64_t block_exclusive_scan64(uint64_t v, uint64_t* lds_wave
     return wave_off + x - v;
 }
 
-// Bin offsets (exclusive prefix of the counts) and the work-item table, in three launches: block
-// partial sums, one scan of the partials (plus the frame totals), and each block's final scan.
+// Bin offsets (exclusive prefix of the counts) and the work-item table, in two launches: block
+// partial sums, then each block's final scan (which scans the partials itself for its base).
 // Each block covers kScanChunk bins with two 64-bit block scans: one of (record count | split
 // items << 32) -- the counts of a frame stay below 2^32, so the low half never carries -- and one
 // of four 16-bit bin classes (bins to clear, heavy, mid and light whole bins), each at most
 // kScanChunk per block.
 constexpr uint32_t kScanChunk = kScanThreads * kScanPerThread;
 static_assert(kScanChunk <= 65535, "16-bit bin class fields");
+static_assert(kScanChunk == kScanBlockBins, "scan partial buffer sizing");
 
 struct ScanPartial {
     uint64_t a;      // records | split items << 32
     uint32_t f[4];   // bins to clear, heavy, mid, light whole bins
-    uint32_t pad[2];
+    uint32_t valid;  // records set up by this block's share of the setup blocks (a frame statistic)
+    uint32_t pad;
 };
 static_assert(sizeof(ScanPartial) == kScanPartialBytes, "scan partial size");
 
@@ -540,7 +542,10 @@ __device__ inline void bin_fields(uint32_t v, uint64_t& a, uint64_t& b) {
     }
 }
 
-__global__ __launch_bounds__(1024) void k_scan_partial(const uint32_t* counts, uint32_t n, ScanPartial* partial) {
+// Block sums of the bin fields; each block also sums an equal share of the setup blocks' record
+// counts, so that no block walks the whole table.
+__global__ __launch_bounds__(kScanThreads) void k_scan_partial(const uint32_t* counts, uint32_t n, ScanPartial* partial,
+                                                               const uint32_t* block_valid, uint32_t n_blocks) {
     __shared__ uint32_t lds[16];
     __shared__ uint64_t lds64[16];
     const uint32_t i0 = blockIdx.x * kScanChunk + threadIdx.x * kScanPerThread;
@@ -548,12 +553,19 @@ __global__ __launch_bounds__(1024) void k_scan_partial(const uint32_t* counts, u
 #pragma unroll
     for (uint32_t k = 0; k < kScanPerThread; ++k)
         if (i0 + k < n) bin_fields(counts[i0 + k], a, b);
+    const uint32_t share = (n_blocks + gridDim.x - 1) / gridDim.x;
+    const uint32_t v0 = blockIdx.x * share, v1 = min(n_blocks, v0 + share);
+    uint32_t vs = 0;
+    for (uint32_t i = v0 + threadIdx.x; i < v1; i += kScanThreads) vs += block_valid[i];
     uint64_t ta, tb;
+    uint32_t tv;
     block_exclusive_scan64(a, lds64, ta);
     block_exclusive_scan64(b, lds64, tb);
+    block_exclusive_scan(vs, lds, tv);
     if (threadIdx.x == 0) {
         ScanPartial p{};
         p.a = ta;
+        p.valid = tv;
         for (int k = 0; k < 4; ++k) p.f[k] = uint32_t(tb >> (16 * k)) & 0xffffu;
         partial[blockIdx.x] = p;
     }
@@ -562,15 +574,15 @@ __global__ __launch_bounds__(1024) void k_scan_partial(const uint32_t* counts, u
 // Every block scans the block partials itself (there are few: one per kScanChunk bins) for its
 // base and the totals; block 0 writes the totals.  Then the block's bins get their list offsets,
 // raster work items and clear-list entries.
-__global__ __launch_bounds__(1024) void k_scan_final(const uint32_t* counts, uint32_t n, uint32_t* offsets,
+__global__ __launch_bounds__(kScanThreads) void k_scan_final(const uint32_t* counts, uint32_t n, uint32_t* offsets,
                                                      const ScanPartial* partial, uint32_t n_parts,
-                                                     const uint32_t* block_valid, uint32_t n_blocks,
                                                      uint32_t list_cap, uint32_t* totals, uint4* items,
                                                      uint32_t items_cap, uint32_t* zero_bins) {
     __shared__ uint32_t lds[16];
     __shared__ uint64_t lds64[16];
     __shared__ ScanPartial base_lds;
     uint64_t carry_a = 0, carry_01 = 0, carry_23 = 0;
+    uint32_t valid = 0;
     for (uint32_t at = 0; at < n_parts; at += kScanThreads) {
         const uint32_t i = at + threadIdx.x;
         ScanPartial p{};
@@ -594,14 +606,16 @@ __global__ __launch_bounds__(1024) void k_scan_final(const uint32_t* counts, uin
         carry_a += ta;
         carry_01 += t01;
         carry_23 += t23;
+        if (blockIdx.x == 0) {  // the frame's set-up records (block-uniform branch)
+            uint32_t tv;
+            block_exclusive_scan(p.valid, lds, tv);
+            valid += tv;
+        }
     }
     const uint32_t n_split = uint32_t(carry_a >> 32);
     const uint32_t n_heavy = uint32_t(carry_01 >> 32), n_mid = uint32_t(carry_23);
     if (blockIdx.x == 0) {
-        uint32_t vs = 0;
-        for (uint32_t i = threadIdx.x; i < n_blocks; i += kScanThreads) vs += block_valid[i];
-        uint32_t total_valid;
-        block_exclusive_scan(vs, lds, total_valid);
+        const uint32_t total_valid = valid;
         if (threadIdx.x == 0) {
             const uint32_t pairs = uint32_t(carry_a), n_zero = uint32_t(carry_01), n_light = uint32_t(carry_23 >> 32);
             const uint64_t n_items = uint64_t(n_heavy) + n_mid + n_split + n_light;
@@ -844,9 +858,10 @@ hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
 hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
     const uint32_t parts = (b.total_bins + kScanChunk - 1) / kScanChunk;
     ScanPartial* partial = reinterpret_cast<ScanPartial*>(b.scan_partial);
-    hipLaunchKernelGGL(k_scan_partial, dim3(parts), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, partial);
+    hipLaunchKernelGGL(k_scan_partial, dim3(parts), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, partial,
+                       b.block_valid, b.n_tblocks);
     hipLaunchKernelGGL(k_scan_final, dim3(parts), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, b.bin_offsets,
-                       partial, parts, b.block_valid, b.n_tblocks, b.list_cap, b.totals, b.items, b.items_cap,
+                       partial, parts, b.list_cap, b.totals, b.items, b.items_cap,
                        b.zero_bins);
     hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.n_views, b.boxes,
                        b.counters, b.bin_offsets, b.pair_slots, b.big_pairs, b.lists, b.list_cap, b.shift, b.totals,
