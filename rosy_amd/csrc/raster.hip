@@ -146,44 +146,43 @@ __global__ void k_clear2(uint32_t* a, uint32_t na, uint32_t* b, uint32_t nb) {
     }
 }
 
-// True when every corner of the object-space box lies outside one clip plane by more than the
-// rounding of the transform (|error| <= a few ulp of sum |m_rj p_j| per coordinate; the margin takes
-// 2^-18 of that sum).  A vertex of the mesh is a convex combination of the corners and its own sum
-// is no larger, so all its FP32 plane distances are negative too: every triangle of the draw would
-// be rejected by the per-triangle test, and skipping the draw changes nothing.
-__device__ inline bool box_outside(const float* m, const float* bmin, const float* bmax) {
-    bool any_plane = false;
+// Draw culling: a draw is skipped when every corner of its object-space box lies outside one clip
+// plane by more than the rounding of the transform (|error| <= a few ulp of sum |m_rj p_j| per
+// coordinate; the margin takes 2^-18 of that sum).  A vertex of the mesh is a convex combination of
+// the corners and its own sum is no larger, so all its FP32 plane distances are negative too: every
+// triangle of the draw would be rejected by the per-triangle test, and skipping the draw changes
+// nothing.  corner_outside gives corner k's bits (k = 0..7: x from bit 0, y from bit 1, z from bit
+// 2): bit `plane` set when the corner is outside that plane by the margin; the box is outside the
+// view when, for some plane, all eight corners have the bit.
+__device__ inline uint32_t corner_outside(const float* m, const float* bmin, const float* bmax, int k) {
+    const float x = (k & 1) ? bmax[0] : bmin[0], y = (k & 2) ? bmax[1] : bmin[1];
+    const float z = (k & 4) ? bmax[2] : bmin[2];
+    const float4 c = transform(m, x, y, z);
+    float sr[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        sr[r] = fabsf(m[r] * x) + fabsf(m[4 + r] * y) + fabsf(m[8 + r] * z) + fabsf(m[12 + r]);
+    uint32_t bits = 0;
 #pragma unroll
     for (int plane = 0; plane < 6; ++plane) {
-        bool all_out = true;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const float x = (k & 1) ? bmax[0] : bmin[0], y = (k & 2) ? bmax[1] : bmin[1];
-            const float z = (k & 4) ? bmax[2] : bmin[2];
-            const float4 c = transform(m, x, y, z);
-            float sr[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                sr[r] = fabsf(m[r] * x) + fabsf(m[4 + r] * y) + fabsf(m[8 + r] * z) + fabsf(m[12 + r]);
-            float bound;
-            switch (plane) {
-                case 0: bound = sr[3] + sr[2]; break;
-                case 1: bound = sr[2]; break;
-                case 2: case 3: bound = sr[0] + 4.0f * sr[3]; break;
-                default: bound = sr[1] + 4.0f * sr[3]; break;
-            }
-            all_out &= plane_dist(c, plane) < -bound * 0x1.0p-18f;
+        float bound;
+        switch (plane) {
+            case 0: bound = sr[3] + sr[2]; break;
+            case 1: bound = sr[2]; break;
+            case 2: case 3: bound = sr[0] + 4.0f * sr[3]; break;
+            default: bound = sr[1] + 4.0f * sr[3]; break;
         }
-        any_plane |= all_out;
+        if (plane_dist(c, plane) < -bound * 0x1.0p-18f) bits |= 1u << plane;
     }
-    return any_plane;
+    return bits;
 }
 
 // clip = (VP * W) * p per (view, vertex).  Every block covers vertices of one (view, draw); its
 // MVP = views[v].view_projection * objects[draw.object].transform is formed once in LDS by the fmaf
-// chain in k order, then each thread reads its vertex from the SoA x/y/z streams (coalesced 12 B)
-// and writes the clip position (coalesced 16 B).  A draw whose box is outside the view is skipped,
-// and its first block records that for k_setup.
+// chain in k order, while each thread's vertex loads from the SoA x/y/z streams (coalesced 12 B)
+// are in flight; then each thread writes the clip position (coalesced 16 B).  A draw whose box is
+// outside the view is skipped (eight lanes test one box corner each), and its first block records
+// that for k_setup.
 // Also zeroes the frame's counters and bin counts (`clear` words at `clear_a` then at `clear_b`), spread
 // over the grid, before the setup kernel counts into them.
 __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, const ViewGPU* views, DrawLists lists,
@@ -199,6 +198,15 @@ __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, con
     const BlockEntry e = blocks[blockIdx.x];
     const ViewGPU& V = views[e.view];
     const DrawGPU& dr = lists.of(V.list)[e.draw];
+    const uint32_t v = e.start + threadIdx.x;
+    const bool has_v = v < dr.vtx_count;
+    float x = 0.f, y = 0.f, z = 0.f;
+    if (has_v) {
+        const uint32_t g = dr.vtx_base + v;
+        x = px[g];
+        y = py[g];
+        z = pz[g];
+    }
     if (threadIdx.x < 16) {
         const int col = threadIdx.x >> 2, row = threadIdx.x & 3;
         const float* a = V.view_projection;
@@ -210,16 +218,20 @@ __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, con
         m[threadIdx.x] = acc;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        culled = box_outside(m, dr.bmin, dr.bmax) ? 1u : 0u;
-        if (e.start == 0) draw_culled[V.cull_base + e.draw] = culled;
+    if (threadIdx.x < 64) {  // wave 0
+        const uint32_t bits = threadIdx.x < 8 ? corner_outside(m, dr.bmin, dr.bmax, int(threadIdx.x)) : 0u;
+        bool any_plane = false;
+#pragma unroll
+        for (int plane = 0; plane < 6; ++plane)
+            any_plane |= (__builtin_amdgcn_ballot_w64((bits >> plane) & 1u) & 0xffull) == 0xffull;
+        if (threadIdx.x == 0) {
+            culled = any_plane ? 1u : 0u;
+            if (e.start == 0) draw_culled[V.cull_base + e.draw] = culled;
+        }
     }
     __syncthreads();
-    if (culled) return;
-    const uint32_t v = e.start + threadIdx.x;
-    if (v >= dr.vtx_count) return;
-    const uint32_t g = dr.vtx_base + v;
-    xv[size_t(V.xv_base) + dr.xv_off + v] = transform(m, px[g], py[g], pz[g]);
+    if (culled || !has_v) return;
+    xv[size_t(V.xv_base) + dr.xv_off + v] = transform(m, x, y, z);
 }
 
 // Bins of a record with at most kSmallBins bins go through the block's LDS table in k_setup; the
