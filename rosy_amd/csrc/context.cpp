@@ -224,8 +224,8 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         if (uint64_t(L.big_total) + g.big_cap >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "bin pair buffer too large");
         L.big_total += g.big_cap;
         for (uint32_t d = 0; d < g.n_draws; ++d) {
-            for (uint32_t s = 0; s < list[d].vtx_count; s += 256) L.vblocks.push_back({vi, d, s, 0});
-            for (uint32_t s = 0; s < list[d].tri_count; s += 256) L.tblocks.push_back({vi, d, s, 0});
+            for (uint32_t s = 0; s < list[d].vtx_count; s += 256) L.vblocks.push_back({vi, d, s, g.list});
+            for (uint32_t s = 0; s < list[d].tri_count; s += 256) L.tblocks.push_back({vi, d, s, g.list});
         }
         if (is_blend) ++L.n_blend;
         else if (f.shadow) ++L.n_shadow;

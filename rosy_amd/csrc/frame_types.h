@@ -86,7 +86,8 @@ struct ViewGPU {
 };
 
 struct BlockEntry {  // work of one 256-thread block of the transform / setup kernels
-    uint32_t view, draw, start, pad;
+    uint32_t view, draw, start;
+    uint32_t list;  // the view's draw list (so the draw record loads beside the view record)
 };
 
 struct TexGPU {

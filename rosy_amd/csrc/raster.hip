@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, con
     }
     const BlockEntry e = blocks[blockIdx.x];
     const ViewGPU& V = views[e.view];
-    const DrawGPU& dr = lists.of(V.list)[e.draw];
+    const DrawGPU& dr = lists.of(e.list)[e.draw];
     const uint32_t v = e.start + threadIdx.x;
     const bool has_v = v < dr.vtx_count;
     float x = 0.f, y = 0.f, z = 0.f;
@@ -300,23 +300,26 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     }
     const BlockEntry e = blocks[blockIdx.x];
     const ViewGPU V = views[e.view];
-    const DrawGPU dr = lists.of(V.list)[e.draw];
+    const DrawGPU dr = lists.of(e.list)[e.draw];
     uint32_t* ctr = counters + e.view * kCtrPerView;
     const uint32_t t = e.start + threadIdx.x;
-    if (draw_culled[V.cull_base + e.draw]) {  // the whole draw is outside the view: empty slots only
-        if (t < dr.tri_count) boxes[V.rec_base + dr.tri_base + t] = make_uint2(kInvalidBox, kInvalidBox);
-        if (threadIdx.x == 0) block_valid[blockIdx.x] = 0u;
-        return;
-    }
     const bool valid = t < dr.tri_count;
-    int status = 0;  // 0 reject, 1 accept, 2 clip
-    float4 c[3];
+    // the triangle's indices are requested beside the draw's culling flag
     uint32_t vi[3] = {0u, 0u, 0u};
     if (valid) {
         const uint32_t* ip = indices + dr.first_index + 3u * t;
         vi[0] = ip[0];
         vi[1] = ip[1];
         vi[2] = ip[2];
+    }
+    if (draw_culled[V.cull_base + e.draw]) {  // the whole draw is outside the view: empty slots only
+        if (valid) boxes[V.rec_base + dr.tri_base + t] = make_uint2(kInvalidBox, kInvalidBox);
+        if (threadIdx.x == 0) block_valid[blockIdx.x] = 0u;
+        return;
+    }
+    int status = 0;  // 0 reject, 1 accept, 2 clip
+    float4 c[3];
+    if (valid) {
         const float4* base = xv + size_t(V.xv_base) + dr.xv_off;
         c[0] = base[vi[0]];
         c[1] = base[vi[1]];
