@@ -596,6 +596,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const uint32_t* cou
     __shared__ uint32_t lds[16];
     __shared__ uint64_t lds64[16];
     __shared__ ScanPartial base_lds;
+    // this block's counts are requested before the scan of the partials
+    const uint32_t i0 = blockIdx.x * kScanChunk + threadIdx.x * kScanPerThread;
+    uint32_t v[kScanPerThread];
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPerThread; ++k) v[k] = i0 + k < n ? counts[i0 + k] : 0u;
     uint64_t carry_a = 0, carry_01 = 0, carry_23 = 0;
     uint32_t valid = 0;
     for (uint32_t at = 0; at < n_parts; at += kScanThreads) {
@@ -645,14 +650,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const uint32_t* cou
         }
     }
     __syncthreads();
-    const uint32_t i0 = blockIdx.x * kScanChunk + threadIdx.x * kScanPerThread;
-    uint32_t v[kScanPerThread];
     uint64_t a = 0, b = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < kScanPerThread; ++k) {
-        v[k] = i0 + k < n ? counts[i0 + k] : 0u;
+    for (uint32_t k = 0; k < kScanPerThread; ++k)
         if (i0 + k < n) bin_fields(v[k], a, b);
-    }
     uint64_t ta, tb;
     const ScanPartial base = base_lds;
     const uint64_t ea = base.a + block_exclusive_scan64(a, lds64, ta);

@@ -519,7 +519,6 @@ template <bool kNeural, bool kLod>
 __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
     __shared__ float lut[256];
     lut[threadIdx.x] = L.srgb_lut[threadIdx.x];
-    __syncthreads();
 
     uint32_t vi = 0;
     while (vi + 1 < L.n_views && L.views[vi + 1].block_base <= blockIdx.x) ++vi;
@@ -529,9 +528,12 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
     const uint32_t y = (lb / V.blocks_x) * 16 + (threadIdx.x >> 4);
     const bool inside = x < V.width && y < V.height;
     const size_t o = size_t(y) * V.width + x;
+    // the pixel's visibility word is requested before the wait for the table
+    const unsigned long long vis_word = inside ? V.vis[o] : 0ull;
+    __syncthreads();
     if constexpr (!kNeural) {
         if (!inside) return;
-        const unsigned long long vis = V.vis[o];
+        const unsigned long long vis = vis_word;
         if (vis == 0ull) {
             V.color[o] = make_float4(L.clear_color[0], L.clear_color[1], L.clear_color[2], L.clear_color[3]);
             if (V.shadow_factor) V.shadow_factor[o] = 1.f;
@@ -544,7 +546,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
     } else {
         // Every lane stays live up to the wave-wide decoder MLP; lanes without a decoded material
         // feed zero features and ignore the result.
-        const unsigned long long vis = inside ? V.vis[o] : 0ull;
+        const unsigned long long vis = vis_word;
         PixelGeom pg{};
         bool decode = false;
         float f[kNmFeat] = {};
