@@ -32,7 +32,9 @@ __device__ inline f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * 
 __device__ inline f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
 __device__ inline float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ inline f3 cross(f3 a, f3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-__device__ inline f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+// Unit vector by the hardware reciprocal square root (v_rsq_f32, 1 ulp): lighting directions only --
+// the barycentrics, texture coordinates and shadow-map coordinates use correctly rounded arithmetic.
+__device__ inline f3 normalize(f3 a) { return a * __builtin_amdgcn_rsqf(dot(a, a)); }
 __device__ inline float clamp01(float x) { return fminf(1.0f, fmaxf(0.0f, x)); }
 
 template <class MP>
