@@ -189,11 +189,11 @@ __device__ inline float distribution_ggx(f3 N, f3 H, float r) {
     const float ndh = fmaxf(dot(N, H), 0.f), ndh2 = ndh * ndh;
     float den = ndh2 * (a2 - 1.f) + 1.f;
     den = 3.14159265359f * den * den;
-    return a2 / den;
+    return a2 * __builtin_amdgcn_rcpf(den);  // hardware reciprocal (1 ulp), as in the other lighting terms
 }
 __device__ inline float schlick_ggx(float ndv, float r) {
-    const float rr = r + 1.f, k = rr * rr / 8.f;
-    return ndv / (ndv * (1.f - k) + k);
+    const float rr = r + 1.f, k = rr * rr * 0.125f;
+    return ndv * __builtin_amdgcn_rcpf(ndv * (1.f - k) + k);
 }
 
 // Visible triangle of a covered pixel: its draw, vertex indices, barycentrics and cascade.
@@ -486,7 +486,7 @@ __device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V,
             const float q5 = q * q * q * q * q;
             const f3 F = f0 + (mk(1.f, 1.f, 1.f) - f0) * q5;
             const f3 kD = (mk(1.f, 1.f, 1.f) - F) * (1.f - metallic);
-            const f3 spec = F * (ndf * g / (4.f * n_dot_v * l_dot_n + 0.0001f));
+            const f3 spec = F * (ndf * g * __builtin_amdgcn_rcpf(4.f * n_dot_v * l_dot_n + 0.0001f));
             const f3 diffuse = albedo * (1.0f / 3.14159265f);
             const f3 sun_light = sun_col * sd.light_intensity;
             const f3 brdf = (kD * diffuse + spec) * sun_light * l_dot_n;
