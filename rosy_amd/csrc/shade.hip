@@ -34,6 +34,7 @@ __device__ inline float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b
 __device__ inline f3 cross(f3 a, f3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 // Unit vector by the hardware reciprocal square root (v_rsq_f32, 1 ulp): lighting directions only --
 // the barycentrics, texture coordinates and shadow-map coordinates use correctly rounded arithmetic.
+// (The normal maps' third component likewise uses the hardware square root, v_sqrt_f32.)
 __device__ inline f3 normalize(f3 a) { return a * __builtin_amdgcn_rsqf(dot(a, a)); }
 __device__ inline float clamp01(float x) { return fminf(1.0f, fmaxf(0.0f, x)); }
 
@@ -378,7 +379,7 @@ __device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V,
         if (use_dec) {
             if (sd.normal_maps_enabled == 1) {
                 f3 m = mk(dec[3] * 2.f - 1.f, dec[4] * 2.f - 1.f, 0.f);
-                m.z = sqrtf(fmaxf(0.f, 1.f - m.x * m.x - m.y * m.y));  // two-channel normal map
+                m.z = __builtin_amdgcn_sqrtf(fmaxf(0.f, 1.f - m.x * m.x - m.y * m.y));  // two-channel normal map
                 const f3 n = normalize(bn);
                 const f3 tt = normalize(btan - n * dot(btan, n));
                 const f3 bb = normalize(cross(bn, btan) * sigma);
@@ -395,7 +396,7 @@ __device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V,
                 const SamplerState smp = si < L.n_samplers ? sampler_state(R::load(samplers + si)) : SamplerState{};
                 const float4 s = sample(tg, smp);
                 f3 m = mk(s.x * 2.f - 1.f, s.y * 2.f - 1.f, s.z * 2.f - 1.f);
-                if (tg.channels == 2) m.z = sqrtf(fmaxf(0.f, 1.f - m.x * m.x - m.y * m.y));
+                if (tg.channels == 2) m.z = __builtin_amdgcn_sqrtf(fmaxf(0.f, 1.f - m.x * m.x - m.y * m.y));
                 const f3 n = normalize(bn);
                 const f3 tt = normalize(btan - n * dot(btan, n));
                 const f3 bb = normalize(cross(bn, btan) * sigma);
