@@ -28,12 +28,12 @@ constexpr uint32_t kInvalidBox = 0xffffffffu;  // bbx of a triangle slot that pr
 // What shading needs of a camera or blend view's triangle, written by setup into the triangle's slot
 // (32 bytes, two 16-byte loads): the three vertices' indices into the global vertex buffer, the
 // draw index in the view's list (| kShadeNoRecord when the triangle was clipped and has no raster
-// record in its slot), then the vertices' clip w.
+// record in its slot), then the vertices' clip w and the draw's material index.
 struct ShadeRec {
     uint32_t v0, v1, v2;
     uint32_t draw;
     float w0, w1, w2;
-    uint32_t pad;
+    uint32_t material;
 };
 static_assert(sizeof(ShadeRec) == 32, "ShadeRec is 32 bytes");
 constexpr uint32_t kShadeNoRecord = 0x80000000u;

@@ -344,7 +344,7 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
             uint4* sr = reinterpret_cast<uint4*>(shade_recs + slot);
             sr[0] = make_uint4(dr.vtx_base + vi[0], dr.vtx_base + vi[1], dr.vtx_base + vi[2],
                                e.draw | (has ? 0u : kShadeNoRecord));
-            sr[1] = make_uint4(__float_as_uint(c[0].w), __float_as_uint(c[1].w), __float_as_uint(c[2].w), 0u);
+            sr[1] = make_uint4(__float_as_uint(c[0].w), __float_as_uint(c[1].w), __float_as_uint(c[2].w), dr.material);
         }
     }
     const uint32_t cslot = wave_append(status == 2, &ctr[kCtrClip]);

@@ -200,6 +200,7 @@ __device__ inline float schlick_ggx(float ndv, float r) {
 // Visible triangle of a covered pixel: its draw, vertex indices, barycentrics and cascade.
 struct PixelGeom {
     uint32_t draw;        // index in the view's draw list
+    uint32_t material;    // the draw's material index
     uint32_t v0, v1, v2;  // indices into the global vertex buffer
     f3 lam;               // perspective-correct barycentrics at the pixel centre
     f3 dlx, dly;          // with kLod: their derivatives along the pixel's x and y
@@ -222,6 +223,7 @@ __device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView
     const uint2 q1 = *reinterpret_cast<const uint2*>(rq + 1);  // X2, Y2
     const uint32_t id = reinterpret_cast<const uint32_t*>(rq + 2)[1];
     pg.draw = sa.w & ~kShadeNoRecord;
+    pg.material = sb.w;
     pg.v0 = sa.x;
     pg.v1 = sa.y;
     pg.v2 = sa.z;
@@ -555,17 +557,21 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
         float f[kNmFeat] = {};
         if (vis != 0ull) {
             pg = pixel_geometry<kLod>(L, V, x, y, vis);
-            const DrawGPU dr = L.draws[pg.draw];
-            if (dr.material < L.n_materials && L.nm_active[dr.material]) {
+            // the material's decoder flag and latent slot load beside the texture coordinates
+            const uint32_t mat = pg.material;
+            const bool known = mat < L.n_materials;
+            const uint32_t active = known ? L.nm_active[mat] : 0u, slot = known ? L.nm_slot[mat] : 0u;
+            const rsy_position* verts = static_cast<const rsy_position*>(L.vertices);
+            const float* t0 = verts[pg.v0].texture_coordinates;
+            const float* t1 = verts[pg.v1].texture_coordinates;
+            const float* t2 = verts[pg.v2].texture_coordinates;
+            const float u0 = t0[0], v0 = t0[1], u1 = t1[0], v1 = t1[1], u2 = t2[0], v2 = t2[1];
+            if (active) {
                 decode = true;
-                const rsy_position* verts = static_cast<const rsy_position*>(L.vertices);
-                const float* t0 = verts[pg.v0].texture_coordinates;
-                const float* t1 = verts[pg.v1].texture_coordinates;
-                const float* t2 = verts[pg.v2].texture_coordinates;
                 // relative to vertex 0 with whole even periods removed (the grid repeats with period 1)
-                const float u = (t0[0] - 2.0f * floorf(t0[0] * 0.5f)) + pg.lam.y * (t1[0] - t0[0]) + pg.lam.z * (t2[0] - t0[0]);
-                const float v = (t0[1] - 2.0f * floorf(t0[1] * 0.5f)) + pg.lam.y * (t1[1] - t0[1]) + pg.lam.z * (t2[1] - t0[1]);
-                nm_features(L.nm_lat16 + size_t(L.nm_slot[dr.material]) * L.nm_L * L.nm_L, L.nm_L, u, v, f);
+                const float u = (u0 - 2.0f * floorf(u0 * 0.5f)) + pg.lam.y * (u1 - u0) + pg.lam.z * (u2 - u0);
+                const float v = (v0 - 2.0f * floorf(v0 * 0.5f)) + pg.lam.y * (v1 - v0) + pg.lam.z * (v2 - v0);
+                nm_features(L.nm_lat16 + size_t(slot) * L.nm_L * L.nm_L, L.nm_L, u, v, f);
             }
         }
         float dec[kNmOut] = {};
@@ -647,16 +653,19 @@ __global__ __launch_bounds__(256) void k_shade_ms(ShadeLaunch L) {
         if constexpr (kNeural) {
             float f[kNmFeat] = {};
             if (covered) {
-                const DrawGPU dr = L.draws[pg.draw];
-                if (dr.material < L.n_materials && L.nm_active[dr.material]) {
+                const uint32_t mat = pg.material;
+                const bool known = mat < L.n_materials;
+                const uint32_t active = known ? L.nm_active[mat] : 0u, slot = known ? L.nm_slot[mat] : 0u;
+                const rsy_position* verts = static_cast<const rsy_position*>(L.vertices);
+                const float* t0 = verts[pg.v0].texture_coordinates;
+                const float* t1 = verts[pg.v1].texture_coordinates;
+                const float* t2 = verts[pg.v2].texture_coordinates;
+                const float u0 = t0[0], v0 = t0[1], u1 = t1[0], v1 = t1[1], u2 = t2[0], v2 = t2[1];
+                if (active) {
                     decode = true;
-                    const rsy_position* verts = static_cast<const rsy_position*>(L.vertices);
-                    const float* t0 = verts[pg.v0].texture_coordinates;
-                    const float* t1 = verts[pg.v1].texture_coordinates;
-                    const float* t2 = verts[pg.v2].texture_coordinates;
-                    const float u = (t0[0] - 2.0f * floorf(t0[0] * 0.5f)) + pg.lam.y * (t1[0] - t0[0]) + pg.lam.z * (t2[0] - t0[0]);
-                    const float vv = (t0[1] - 2.0f * floorf(t0[1] * 0.5f)) + pg.lam.y * (t1[1] - t0[1]) + pg.lam.z * (t2[1] - t0[1]);
-                    nm_features(L.nm_lat16 + size_t(L.nm_slot[dr.material]) * L.nm_L * L.nm_L, L.nm_L, u, vv, f);
+                    const float u = (u0 - 2.0f * floorf(u0 * 0.5f)) + pg.lam.y * (u1 - u0) + pg.lam.z * (u2 - u0);
+                    const float vv = (v0 - 2.0f * floorf(v0 * 0.5f)) + pg.lam.y * (v1 - v0) + pg.lam.z * (v2 - v0);
+                    nm_features(L.nm_lat16 + size_t(slot) * L.nm_L * L.nm_L, L.nm_L, u, vv, f);
                 }
             }
             if (__builtin_amdgcn_ballot_w64(decode) != 0ull) nm_mlp_wave(static_cast<const NmWeights*>(L.nm_weights), f, dec);
