@@ -500,24 +500,37 @@ constexpr uint32_t kScanPerThread = 4;
 constexpr uint32_t kHeavyRecs = 128;
 constexpr uint32_t kMidRecs = 48;
 
-__device__ inline uint64_t block_exclusive_scan64(uint64_t v, uint64_t* lds_waves, uint64_t& total) {
-    uint64_t x = v;
+// N exclusive prefix sums over the workgroup at once (one pair of barriers for all of them): v[j]
+// becomes its exclusive prefix and total[j] the workgroup's sum.  lds_waves holds 16 words per value.
+template <int N>
+__device__ inline void block_exclusive_scan64_n(uint64_t (&v)[N], uint64_t* lds_waves, uint64_t (&total)[N]) {
+    uint64_t x[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) x[j] = v[j];
     for (int off = 1; off < 64; off <<= 1) {
-        const uint64_t y = __shfl_up(x, off);
-        if (int(lane_id()) >= off) x += y;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const uint64_t y = __shfl_up(x[j], off);
+            if (int(lane_id()) >= off) x[j] += y;
+        }
     }
     const uint32_t wave = threadIdx.x / 64, nwaves = blockDim.x / 64;
-    if (lane_id() == 63) lds_waves[wave] = x;
+    if (lane_id() == 63)
+#pragma unroll
+        for (int j = 0; j < N; ++j) lds_waves[j * 16 + wave] = x[j];
     __syncthreads();
-    uint64_t wave_off = 0, sum = 0;
-    for (uint32_t w = 0; w < nwaves; ++w) {
-        const uint64_t t = lds_waves[w];
-        if (w < wave) wave_off += t;
-        sum += t;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        uint64_t wave_off = 0, sum = 0;
+        for (uint32_t w = 0; w < nwaves; ++w) {
+            const uint64_t t = lds_waves[j * 16 + w];
+            if (w < wave) wave_off += t;
+            sum += t;
+        }
+        total[j] = sum;
+        v[j] = wave_off + x[j] - v[j];
     }
     __syncthreads();
-    total = sum;
-    return wave_off + x - v;
 }
 
 // Bin offsets (exclusive prefix of the counts) and the work-item table, in two launches: block
@@ -561,8 +574,7 @@ __device__ inline void bin_fields(uint32_t v, uint64_t& a, uint64_t& b) {
 // counts, so that no block walks the whole table.
 __global__ __launch_bounds__(kScanThreads) void k_scan_partial(const uint32_t* counts, uint32_t n, ScanPartial* partial,
                                                                const uint32_t* block_valid, uint32_t n_blocks) {
-    __shared__ uint32_t lds[16];
-    __shared__ uint64_t lds64[16];
+    __shared__ uint64_t lds64[3 * 16];
     const uint32_t i0 = blockIdx.x * kScanChunk + threadIdx.x * kScanPerThread;
     uint64_t a = 0, b = 0;
 #pragma unroll
@@ -572,16 +584,13 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_partial(const uint32_t* c
     const uint32_t v0 = blockIdx.x * share, v1 = min(n_blocks, v0 + share);
     uint32_t vs = 0;
     for (uint32_t i = v0 + threadIdx.x; i < v1; i += kScanThreads) vs += block_valid[i];
-    uint64_t ta, tb;
-    uint32_t tv;
-    block_exclusive_scan64(a, lds64, ta);
-    block_exclusive_scan64(b, lds64, tb);
-    block_exclusive_scan(vs, lds, tv);
+    uint64_t sv[3] = {a, b, vs}, st[3];
+    block_exclusive_scan64_n<3>(sv, lds64, st);
     if (threadIdx.x == 0) {
         ScanPartial p{};
-        p.a = ta;
-        p.valid = tv;
-        for (int k = 0; k < 4; ++k) p.f[k] = uint32_t(tb >> (16 * k)) & 0xffffu;
+        p.a = st[0];
+        p.valid = uint32_t(st[2]);
+        for (int k = 0; k < 4; ++k) p.f[k] = uint32_t(st[1] >> (16 * k)) & 0xffffu;
         partial[blockIdx.x] = p;
     }
 }
@@ -593,8 +602,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const uint32_t* cou
                                                      const ScanPartial* partial, uint32_t n_parts,
                                                      uint32_t list_cap, uint32_t* totals, uint4* items,
                                                      uint32_t items_cap, uint32_t* zero_bins) {
-    __shared__ uint32_t lds[16];
-    __shared__ uint64_t lds64[16];
+    __shared__ uint64_t lds64[4 * 16];
     __shared__ ScanPartial base_lds;
     // this block's counts are requested before the scan of the partials
     const uint32_t i0 = blockIdx.x * kScanChunk + threadIdx.x * kScanPerThread;
@@ -610,10 +618,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const uint32_t* cou
         // class counts of a block fit 16 bits, their sums over blocks 32 bits: two classes per scan
         const uint64_t v01 = uint64_t(p.f[0]) | (uint64_t(p.f[1]) << 32);
         const uint64_t v23 = uint64_t(p.f[2]) | (uint64_t(p.f[3]) << 32);
-        uint64_t ta, t01, t23;
-        const uint64_t ea = carry_a + block_exclusive_scan64(p.a, lds64, ta);
-        const uint64_t e01 = carry_01 + block_exclusive_scan64(v01, lds64, t01);
-        const uint64_t e23 = carry_23 + block_exclusive_scan64(v23, lds64, t23);
+        uint64_t sv[4] = {p.a, v01, v23, p.valid}, st[4];
+        block_exclusive_scan64_n<4>(sv, lds64, st);
+        const uint64_t ea = carry_a + sv[0], e01 = carry_01 + sv[1], e23 = carry_23 + sv[2];
         if (i == blockIdx.x) {
             ScanPartial q{};
             q.a = ea;
@@ -623,14 +630,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const uint32_t* cou
             q.f[3] = uint32_t(e23 >> 32);
             base_lds = q;
         }
-        carry_a += ta;
-        carry_01 += t01;
-        carry_23 += t23;
-        if (blockIdx.x == 0) {  // the frame's set-up records (block-uniform branch)
-            uint32_t tv;
-            block_exclusive_scan(p.valid, lds, tv);
-            valid += tv;
-        }
+        carry_a += st[0];
+        carry_01 += st[1];
+        carry_23 += st[2];
+        valid += uint32_t(st[3]);  // the frame's set-up records
     }
     const uint32_t n_split = uint32_t(carry_a >> 32);
     const uint32_t n_heavy = uint32_t(carry_01 >> 32), n_mid = uint32_t(carry_23);
@@ -654,10 +657,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const uint32_t* cou
 #pragma unroll
     for (uint32_t k = 0; k < kScanPerThread; ++k)
         if (i0 + k < n) bin_fields(v[k], a, b);
-    uint64_t ta, tb;
+    uint64_t sv[2] = {a, b}, st[2];
     const ScanPartial base = base_lds;
-    const uint64_t ea = base.a + block_exclusive_scan64(a, lds64, ta);
-    const uint64_t eb = block_exclusive_scan64(b, lds64, tb);
+    block_exclusive_scan64_n<2>(sv, lds64, st);
+    const uint64_t ea = base.a + sv[0];
+    const uint64_t eb = sv[1];
     uint32_t run = uint32_t(ea);
     uint64_t r_split = uint64_t(n_heavy) + n_mid + uint32_t(ea >> 32);
     uint32_t r_zero = base.f[0] + uint32_t(eb & 0xffffu);
