@@ -18,7 +18,9 @@ constexpr uint32_t kItemRecs = 256;   // records per raster work item (one per t
 constexpr uint32_t kChunkRecs = 128;  // records staged in LDS at a time
 
 // Exclusive prefix sum over the workgroup (wave scans with shuffles, wave totals through LDS).
-// Every thread of the block must call it; it synchronises the block.
+// Every thread of the block must call it; it synchronises the block.  kTailSync = false leaves out
+// the closing barrier, for a caller that synchronises again before lds_waves is next written.
+template <bool kTailSync = true>
 __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds_waves, uint32_t& total) {
     uint32_t x = v;
     for (int off = 1; off < 64; off <<= 1) {
@@ -34,7 +36,7 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds_waves,
         if (w < wave) wave_off += s;
         sum += s;
     }
-    __syncthreads();
+    if constexpr (kTailSync) __syncthreads();
     total = sum;
     return wave_off + x - v;
 }
@@ -250,7 +252,8 @@ __device__ inline void raster_item(const Op& op, TriRec* chunk_recs, uint8_t* ow
             dst[2] = w2;
         }
         uint32_t total_rows;
-        const uint32_t first = block_exclusive_scan(rows, scan_lds, total_rows);  // synchronises
+        // synchronises; the barrier after the owner table comes before the next chunk's scan writes
+        const uint32_t first = block_exclusive_scan<false>(rows, scan_lds, total_rows);
         if (stager) row_start[slot] = first;
         for (uint32_t j = 0; j < rows; ++j) owner[first + j] = uint8_t(slot);
         __syncthreads();
