@@ -249,7 +249,8 @@ typedef struct rsy_frame_stats {
     uint64_t triangles_clipped; /* needed near/far/guard-band clipping */
     uint64_t tile_pairs;        /* (triangle, tile) bin entries */
     uint32_t tile_size;
-    uint32_t overflowed;        /* a bin or record buffer was too small; the frame was re-run */
+    uint32_t overflowed;        /* times the last frame was re-run after a record, clip-queue, bin or
+                                   fragment buffer overflowed and grew (0: it fit the first time) */
     float gpu_ms;               /* device time of the last frame (HIP events) */
     float stage_ms[8];          /* with RSY_DEBUG_STAGE_TIMING (HIP events on the context stream):
                                    [0] geometry (mvp, transform, setup, clip), [1] binning (scan,
@@ -337,6 +338,9 @@ rsy_result rsy_get_device_outputs(rsy_ctx* ctx, uint32_t view, const void** vis,
 /* Debug and tuning switches. */
 #define RSY_DEBUG_SHADOW_FACTOR 1u /* also write the per-pixel PCF shadow factor */
 #define RSY_DEBUG_STAGE_TIMING 2u  /* record HIP events between stages */
+#define RSY_DEBUG_MIN_CAPACITY 4u  /* start the frame's record, clip-queue, bin-pair, bin-list and
+                                      blended-fragment capacities small, so that frames overflow and
+                                      take the regrowth path (takes effect at the next new layout) */
 rsy_result rsy_set_debug_flags(rsy_ctx* ctx, uint32_t flags);
 
 /* The output stage of the last shaded frame for one camera view (Graphics.cpp:1908, 5835-5880):

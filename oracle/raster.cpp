@@ -101,9 +101,14 @@ bool setup_tri(const C4 v[3], uint32_t id, const Params& p, Rec& r) {
     return true;
 }
 
+// A clipped polygon keeps at most kClipMax vertices: the exact bound for a triangle cut by six
+// planes.  In FP32 a distance that rounds to the other side near a corner can add a crossing; a
+// vertex past the bound is dropped, as the GPU clipper does (the same polygon on both sides).
+constexpr int kClipMax = 9;
+
 // Clip one triangle (some vertex outside some plane) and set up the fan.
 void clip_and_setup(const C4 in[3], uint32_t id, const Params& p, std::vector<Rec>& out) {
-    C4 a[12], b[12];
+    C4 a[kClipMax], b[kClipMax];
     int n = 3;
     for (int k = 0; k < 3; ++k) a[k] = in[k];
     for (int plane = 0; plane < 6; ++plane) {
@@ -113,8 +118,8 @@ void clip_and_setup(const C4 in[3], uint32_t id, const Params& p, std::vector<Re
             const C4& nxt = a[(i + 1) % n];
             const float dc = plane_dist(cur, plane), dn = plane_dist(nxt, plane);
             const bool ic = dc >= 0.f, in_ = dn >= 0.f;
-            if (ic) b[m++] = cur;
-            if (ic != in_) {
+            if (ic && m < kClipMax) b[m++] = cur;
+            if (ic != in_ && m < kClipMax) {
                 const C4& pi = ic ? cur : nxt;
                 const C4& po = ic ? nxt : cur;
                 const float di = ic ? dc : dn, dout = ic ? dn : dc;

@@ -23,7 +23,7 @@ RSY_ABSENT = 0xFFFFFFFF
 RSY_ABI_VERSION = 1
 SHADE_NONE, SHADE_PLAIN, SHADE_NEURAL = 0, 1, 2
 PCF_REFERENCE_5TAP, PCF_GRID_3X3 = 0, 1
-DEBUG_SHADOW_FACTOR, DEBUG_STAGE_TIMING = 1, 2
+DEBUG_SHADOW_FACTOR, DEBUG_STAGE_TIMING, DEBUG_MIN_CAPACITY = 1, 2, 4
 TEX_RGBA8_SRGB, TEX_RGBA8_UNORM, TEX_RG8_UNORM, TEX_RG8_SRGB, TEX_BC7_SRGB, TEX_BC7_UNORM = 0, 1, 2, 3, 4, 5
 MIPS_BUILD = 0xFFFFFFFF
 

@@ -199,14 +199,22 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         cull_total += g.n_draws;
         g.xv_base = uint32_t(L.total_xv);
         const uint32_t tris = c->list_tris[g.list];
-        const uint32_t clip_cap = std::max<uint32_t>(4096, tris / 8) * c->rec_scale;
+        // capacities in 64 bits: rec_scale doubles on every record overflow
+        const bool min_cap = (c->debug_flags & RSY_DEBUG_MIN_CAPACITY) != 0;
+        const uint64_t clip_cap = uint64_t(min_cap ? 256u : std::max<uint32_t>(4096u, tris / 8)) * c->rec_scale;
+        const uint64_t rec_cap = uint64_t(tris) + 2 * clip_cap;
+        const uint64_t big_cap = uint64_t(min_cap ? 4096u : std::max<uint32_t>(65536u, tris / 4)) * c->rec_scale;
+        if (rec_cap >= 0xffffffffull || uint64_t(L.rec_total) + rec_cap >= 0xffffffffull)
+            return fail(c, RSY_OVERFLOW, "record buffer too large");
+        if (uint64_t(L.clip_total) + clip_cap >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "clip queue too large");
+        if (uint64_t(L.big_total) + big_cap >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "bin pair buffer too large");
         g.rec_base = L.rec_total;
         g.tri_count = tris;
-        g.rec_cap = tris + 2 * clip_cap;
+        g.rec_cap = uint32_t(rec_cap);
         g.clip_base = L.clip_total;
-        g.clip_cap = clip_cap;
+        g.clip_cap = uint32_t(clip_cap);
         g.big_base = L.big_total;
-        g.big_cap = std::max<uint32_t>(65536, tris / 4) * c->rec_scale;
+        g.big_cap = uint32_t(big_cap);
         // the blended pass draws both faces (VK_CULL_MODE_NONE, Graphics.cpp:5675)
         g.cull = c->rs.cull_enabled && !is_blend ? 1 : 0;
         g.reverse = c->rs.reverse_winding_order_enabled ? 1 : 0;
@@ -218,10 +226,8 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         L.total_bins += g.tiles_x * g.tiles_y;
         if (is_blend) L.blend_bins += g.tiles_x * g.tiles_y;
         L.total_xv += c->list_verts[g.list];
-        if (uint64_t(L.rec_total) + g.rec_cap >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "record buffer too large");
         L.rec_total += g.rec_cap;
         L.clip_total += g.clip_cap;
-        if (uint64_t(L.big_total) + g.big_cap >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "bin pair buffer too large");
         L.big_total += g.big_cap;
         for (uint32_t d = 0; d < g.n_draws; ++d) {
             for (uint32_t s = 0; s < list[d].vtx_count; s += 256) L.vblocks.push_back({vi, d, s, g.list});
@@ -261,6 +267,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     uint64_t want_pairs = 0;
     for (const auto& g : L.vg) want_pairs += uint64_t(c->list_tris[g.list]) * 3 + g.tiles_x * g.tiles_y;
     want_pairs *= c->rec_scale;
+    if (c->debug_flags & RSY_DEBUG_MIN_CAPACITY) want_pairs = std::min<uint64_t>(want_pairs, 4096);
     if (want_pairs > c->list_cap) {
         want_pairs = std::min<uint64_t>(want_pairs, 0xfffffff0ull);
         HIP_TRY(c, c->lists_mem.ensure(size_t(want_pairs) * 4), "allocate bin lists");
@@ -299,7 +306,8 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     }
     if (L.n_blend) {
         // blended fragments: start at two per pixel of the camera views (grows on overflow)
-        const uint64_t want = std::min<uint64_t>(std::max<uint64_t>(c->frag_cap, 2 * blend_px + 4096), 0xfffffff0ull);
+        const uint64_t first = (c->debug_flags & RSY_DEBUG_MIN_CAPACITY) ? 4096 : 2 * blend_px + 4096;
+        const uint64_t want = std::min<uint64_t>(std::max<uint64_t>(c->frag_cap, first), 0xfffffff0ull);
         HIP_TRY(c, c->frags.ensure(size_t(want) * 16), "allocate blended fragments");
         HIP_TRY(c, c->frag_keys.ensure(size_t(want) * 4), "allocate blended fragment keys");
         HIP_TRY(c, c->frag_color.ensure(size_t(want) * 16), "allocate blended fragment colours");
@@ -627,6 +635,7 @@ rsy_result finish_frame(rsy_ctx* c) {
     if (!c->frame_pending) return RSY_OK;
     for (int attempt = 0; attempt < 8; ++attempt) {
         if (!collect_stats(c)) {
+            c->stats.overflowed = uint32_t(attempt);  // re-runs this frame needed
             c->frame_pending = false;
             return RSY_OK;
         }
@@ -650,6 +659,7 @@ rsy_result finish_frame(rsy_ctx* c) {
             c->layout.valid = false;
         }
         if (rec_over) {
+            if (c->rec_scale >= (1u << 20)) return fail(c, RSY_OVERFLOW, "record capacity cannot grow further");
             c->rec_scale *= 2;
             c->layout.valid = false;
         }

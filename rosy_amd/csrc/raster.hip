@@ -402,7 +402,10 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
 // LDS, then the fan (v0, vk, vk+1) set up as records; record slots are handed out per wave and
 // the fan triangles are binned by the whole wave.
 constexpr int kClipThreads = 128;
-constexpr int kClipMax = 9;  // a triangle cut by 6 planes has at most 9 vertices
+// A triangle cut by 6 planes has at most 9 vertices in exact arithmetic.  In FP32 a distance that
+// rounds to the other side near a corner can add a crossing: a vertex past the bound is dropped (the
+// oracle's clipper holds the same bound), so the polygon never leaves its thread's slice.
+constexpr int kClipMax = 9;
 
 __global__ __launch_bounds__(kClipThreads) void k_clip(const ViewGPU* views, DrawLists lists,
                                                        const uint32_t* indices, const float4* xv, const uint2* clipq, TriRec* recs,
@@ -437,8 +440,8 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(const ViewGPU* views, Dra
                     const float4 nxt = A[i + 1 < cnt ? i + 1 : 0];
                     const float dc = plane_dist(cur, plane), dn = plane_dist(nxt, plane);
                     const bool ic = dc >= 0.f, in_ = dn >= 0.f;
-                    if (ic) B[m++] = cur;
-                    if (ic != in_) {
+                    if (ic && m < kClipMax) B[m++] = cur;
+                    if (ic != in_ && m < kClipMax) {
                         const float4 pi = ic ? cur : nxt;
                         const float4 po = ic ? nxt : cur;
                         const float di = ic ? dc : dn, dout = ic ? dn : dc;

@@ -185,10 +185,16 @@ __device__ inline void mr_from(int channels, float4 s, float& rough, float& meta
     }
 }
 
+// GGX D = a^2 / (pi (NdotH^2 (a^2 - 1) + 1)^2).  The denominator is evaluated as
+// (1 - NdotH^2) + NdotH^2 a^2 with 1 - NdotH^2 = |N x H|^2 (unit N and H): near the lobe's peak at
+// low roughness, 1 - NdotH^2 from an FP32 dot product loses most of its digits (N and H are unit
+// only to an ulp, which the cross product's length does not amplify), and D there runs to 10^4.
 __device__ inline float distribution_ggx(f3 N, f3 H, float r) {
     const float a = r * r, a2 = a * a;
     const float ndh = fmaxf(dot(N, H), 0.f), ndh2 = ndh * ndh;
-    float den = ndh2 * (a2 - 1.f) + 1.f;
+    const f3 c = cross(N, H);
+    const float s2 = ndh > 0.f ? dot(c, c) : 1.f;  // NdotH clamped to 0 leaves the denominator 1
+    float den = s2 + ndh2 * a2;
     den = 3.14159265359f * den * den;
     return a2 * __builtin_amdgcn_rcpf(den);  // hardware reciprocal (1 ulp), as in the other lighting terms
 }

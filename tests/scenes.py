@@ -24,7 +24,7 @@ class CustomScene:
     def __init__(self, positions, triangles, width, height, draws=None, view_projection=None,
                  shadow_projection=None, shadow_size=64, normals=None, uvs=None, tangents=None,
                  textures=None, material=True, objects=None, object_of_draw=None, blended=None,
-                 materials=None, material_of_draw=None):
+                 materials=None, material_of_draw=None, samplers=None, colors=None):
         pos = np.asarray(positions, dtype=np.float32).reshape(-1, 3)
         tri = np.asarray(triangles, dtype=np.uint32).reshape(-1, 3)
         nv = len(pos)
@@ -36,7 +36,7 @@ class CustomScene:
             v.normal[:] = [float(x) for x in n]
             t = tangents[i] if tangents is not None else (1.0, 0.0, 0.0, 1.0)
             v.tangents[:] = [float(x) for x in t]
-            v.color[:] = [1.0, 0.0, 0.0, 1.0]
+            v.color[:] = [float(x) for x in colors[i]] if colors is not None else [1.0, 0.0, 0.0, 1.0]
             uv = uvs[i] if uvs is not None else (0.5 * (p[0] + 1.0), 0.5 * (p[1] + 1.0))
             v.texture_coordinates[:] = [float(x) for x in uv]
         idx = tri.reshape(-1)
@@ -94,8 +94,11 @@ class CustomScene:
                     m.color[:] = [float(x) for x in val]
                 else:
                     setattr(m, key, val)
-        self._smp = (abi.Sampler * 1)()
-        self._smp[0] = abi.Sampler(4, 4, 2, 2)
+        # samplers: (min_filter, mag_filter, wrap_s, wrap_t); default one LINEAR / REPEAT sampler
+        smps = samplers if samplers is not None else [(4, 4, 2, 2)]
+        self._smp = (abi.Sampler * len(smps))()
+        for i, sm in enumerate(smps):
+            self._smp[i] = abi.Sampler(*sm)
         vp = view_projection if view_projection is not None else IDENTITY
         sp = shadow_projection if shadow_projection is not None else IDENTITY
         self._views = (abi.CameraView * 1)()
@@ -126,7 +129,7 @@ class CustomScene:
         s.textures = C.cast(self._tex, C.POINTER(abi.TextureDesc))
         s.n_textures = len(texs)
         s.samplers = C.cast(self._smp, C.POINTER(abi.Sampler))
-        s.n_samplers = 1
+        s.n_samplers = len(smps)
         sd = s.scene_data
         sd.view[:] = IDENTITY
         sd.proj[:] = IDENTITY

@@ -31,9 +31,9 @@ def test_blended_pass_matches_the_oracle(size):
     maps = [r.shadow_map(0)]
     base, _, _ = oracle.shade_view(s, cam, s.cascades(), maps, vis)
     want, count = oracle.blend_view(s, cam, s.cascades(), maps, vis, base)
-    assert count.max() >= 5 and st.overflowed == 0
+    assert count.max() >= 5
     err = float(np.abs(col - want).max())
-    print(f"blended pass {size}x{size}: max abs err {err:.3e}, layers up to {count.max()}")
+    print(f"blended pass {size}x{size}: max abs err {err:.3e}, layers up to {count.max()}, re-runs {st.overflowed}")
     assert err <= TOL
     # the blended pass changed the pixels that hold a layer and no others
     changed = np.any(np.abs(col - base) > 1e-6, axis=-1)
@@ -103,7 +103,7 @@ def test_deep_stacks_grow_the_fragment_array():
     cam = s.camera_views()[0]
     base, _, _ = oracle.shade_view(s, cam, s.cascades(), [r.shadow_map(0)], vis)
     want, count = oracle.blend_view(s, cam, s.cascades(), [r.shadow_map(0)], vis, base, max_layers=48)
-    assert count.min() == 40 and r.stats().overflowed == 0
+    assert count.min() == 40 and r.stats().overflowed >= 1  # re-run with a larger fragment array
     err = float(np.abs(col - want).max())
     print(f"40-deep blended stacks: max abs err {err:.3e}")
     assert err <= TOL

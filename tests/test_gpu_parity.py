@@ -27,10 +27,10 @@ def renderer():
     r.close()
 
 
-def check_frame(r, scene, frame=None, colour=True, colour_views=None, sd=None, rs=None):
+def check_frame(r, scene, frame=None, colour=True, colour_views=None, sd=None, rs=None, flags=0):
     """Render `frame` (already loaded scene) and compare everything with the oracle."""
     frame = frame if frame is not None else scene.frame_desc(abi.SHADE_PLAIN)
-    r.set_debug_flags(abi.DEBUG_SHADOW_FACTOR)
+    r.set_debug_flags(abi.DEBUG_SHADOW_FACTOR | flags)
     r.render(frame)
     rs = rs if rs is not None else scene.s.raster_state
     cascades = scene.cascades() if frame.n_cascades else [None]
@@ -88,6 +88,26 @@ def test_scene_scale_workload(renderer, scene_scale):
     st = renderer.stats()
     assert st.triangle_count == 2 * 262144 and st.overflowed == 0
     print("scene-scale colour max abs err", errs)
+
+
+def test_overflow_regrowth_matches_the_oracle(scene_scale):
+    """A context whose frame buffers start at their minimum capacities (RSY_DEBUG_MIN_CAPACITY):
+    the first frame overflows its bin lists, bin pairs and clip queue, grows them and is re-run;
+    the result is still bit-exact and within the colour bar."""
+    r = rosy_amd.Renderer(device=0)
+    try:
+        r.set_debug_flags(abi.DEBUG_MIN_CAPACITY)
+        r.load(scene_scale)
+        check_frame(r, scene_scale, flags=abi.DEBUG_MIN_CAPACITY)
+        st = r.stats()
+        assert st.overflowed >= 1, "the minimum capacities did not overflow"
+        assert st.triangle_count == 2 * 262144
+        print("re-runs after overflow:", st.overflowed, "tile pairs", st.tile_pairs)
+        # the grown buffers hold the next frame without another re-run
+        check_frame(r, scene_scale, flags=abi.DEBUG_MIN_CAPACITY)
+        assert r.stats().overflowed == 0
+    finally:
+        r.close()
 
 
 def test_bin_sizes_give_identical_frames(renderer, scene_scale):

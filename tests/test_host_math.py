@@ -5,6 +5,7 @@ import ctypes as C
 import math
 
 import numpy as np
+import pytest
 
 import rosy_amd
 from rosy_amd import abi
@@ -73,6 +74,32 @@ def test_light_matrices_restate_level_cpp():
     assert toward[2] > o[2]
     # x/y scale is 2 / cascade_level
     assert abs(np.linalg.norm(S[0, :3]) - 2 / 36) < 1e-7
+
+
+@pytest.mark.parametrize("d,pitch,yaw,level,od", [(23.776, 5.343, 6.414, 36.0, 76.838), (30.0, 0.4, 1.1, 20.0, 50.0),
+                                                  (15.0, -0.7, -2.3, 12.0, 40.0)])
+def test_light_matrices_closed_forms(d, pitch, yaw, level, od):
+    """Level.cpp:464-476, 508-528 written out.  light_line_rot = R_y(-yaw) R_x(-pitch) (the quaternions
+    angleAxis(yaw, -y) and angleAxis(-pitch, x)), so
+      sun_position = rot (0, 0, -d, 1) = (d sin(yaw) cos(pitch), -d sin(pitch), -d cos(yaw) cos(pitch), 1),
+      rot e_x = (cos yaw, 0, sin yaw),  rot e_y = (sin yaw sin pitch, cos pitch, -cos yaw sin pitch),
+    and shadowProjNear = diag(-2/c, -2/c, -1/od, 1) T(0, 0, -d) rot^T maps the origin to (0, 0, d/od),
+    rot e_x to (-2/c, 0, d/od), rot e_y to (0, -2/c, d/od) and the sun to (0, 0, 2d/od)."""
+    sun = (C.c_float * 4)()
+    sp = (C.c_float * 16)()
+    rosy_amd.lib().rsy_light_matrices(d, pitch, yaw, level, od, sun, sp)
+    # the angles as the FP32 values the reference stores
+    p, y = float(np.float32(pitch)), float(np.float32(yaw))
+    want_sun = [d * math.sin(y) * math.cos(p), -d * math.sin(p), -d * math.cos(y) * math.cos(p), 1.0]
+    assert np.allclose(list(sun), want_sun, rtol=0, atol=4e-6 * d)
+    S = mat(sp)
+    ex = np.array([math.cos(y), 0.0, math.sin(y), 1.0])
+    ey = np.array([math.sin(y) * math.sin(p), math.cos(p), -math.cos(y) * math.sin(p), 1.0])
+    tol = 4e-6 * max(1.0, d / od, 2 / level)
+    assert np.allclose(S @ np.array([0, 0, 0, 1.0]), [0, 0, d / od, 1], atol=tol)
+    assert np.allclose(S @ ex, [-2 / level, 0, d / od, 1], atol=tol)
+    assert np.allclose(S @ ey, [0, -2 / level, d / od, 1], atol=tol)
+    assert np.allclose(S @ np.array(want_sun), [0, 0, 2 * d / od, 1], atol=tol)
 
 
 def test_object_record_inverse_and_normal_matrix():

@@ -198,6 +198,28 @@ def test_guard_band_clipping_of_a_huge_triangle_covers_the_view():
     assert (vis != 0).all() and st.fragments == W * H
 
 
+def test_clipping_through_guard_band_and_near_plane_corners():
+    """Triangles whose vertices sit on, or one rounding step from, corners where two guard-band
+    planes (|x| <= 4w, |y| <= 4w) meet, or where one meets the near plane (z <= w): plane distances
+    of exactly 0 and of rounding size with either sign.  The clipped polygon keeps at most 9
+    vertices (the exact bound for a triangle and six planes), so each clipped triangle gives at
+    most 7 fan triangles."""
+    rng = np.random.default_rng(11)
+    corners = [(4.0, 4.0), (-4.0, 4.0), (4.0, -4.0), (-4.0, -4.0), (4.0, 0.3), (-0.7, 4.0), (-4.0, -0.2)]
+    clipped = 0
+    for trial in range(300):
+        pos = []
+        for k in range(3):
+            cx, cy = corners[int(rng.integers(len(corners)))]
+            jx, jy = (float(v) for v in rng.choice([0.0, 1e-7, -1e-7, 3e-7], size=2))
+            z = float(rng.choice([1.0, 1.0 + 6e-8, 1.0 - 6e-8, 0.5]))  # on or next to the near plane
+            pos.append((cx + jx, cy + jy, z))
+        vis, st = raster(CustomScene(pos, [0, 1, 2], W, H))
+        assert st.triangles_setup <= 7 * st.triangles_clipped + (1 - st.triangles_clipped)
+        clipped += st.triangles_clipped
+    assert clipped > 200
+
+
 def test_triangle_fully_behind_the_far_plane_is_rejected():
     pos = [(-0.5, -0.5, -0.1), (0.5, -0.5, -0.2), (-0.5, 0.5, -0.3)]
     vis, st = raster(CustomScene(pos, [0, 1, 2], W, H))
