@@ -1,10 +1,14 @@
 """Pins for the oracle's BC7 decoder (oracle/bc7.cpp), the format of the reference's images
 (Packager.cpp:216-308, Graphics.cpp:3620-3650).
 
-The reference ships no BC7 files and NVTT is closed, so the pins come from the format itself:
+The reference ships no BC7 files and NVTT is closed, so the pins come from the format itself and
+from an independent decoder:
+- golden vectors decoded by Pillow's BC7 decoder (tests/golden/make_bc7_vectors.py): every mode,
+  every partition and every rotation / index selection, byte for byte, which pins the partition,
+  anchor and weight tables entry by entry;
 - the partition and anchor tables satisfy the format's invariants (texel 0 is in subset 0, every
   anchor texel lies in the subset it anchors, every partition uses all its subsets, no partition
-  repeats); the tables' entries beyond these invariants are parity unpinned;
+  repeats);
 - blocks packed here field by field for every mode decode to closed forms: equal endpoints give a
   constant colour (the endpoint widened by bit replication, whatever the indices), index 0 gives
   endpoint 0 and the top index endpoint 1, an anchor's index has one bit fewer, the rotation and
@@ -189,3 +193,22 @@ def test_reserved_mode_and_image_edges():
     full = oracle.bc7_decode(blocks, 12, 8)
     crop = oracle.bc7_decode(blocks, 10, 7)
     assert np.array_equal(crop, full[:7, :10])
+
+
+def test_every_mode_and_partition_matches_pillow():
+    """Golden vectors from an independent decoder: 592 blocks covering every mode, all 16 / 64
+    partitions of the two- and three-subset modes and every rotation / index selection of modes 4
+    and 5, decoded by Pillow's BC7 decoder (tests/golden/make_bc7_vectors.py).  The oracle must
+    reproduce every texel byte for byte, which pins its partition, anchor and weight tables."""
+    import os
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "bc7_pillow.npz"))
+    blocks, want = d["blocks"], d["rgba"]
+    h, w = want.shape[:2]
+    got = oracle.bc7_decode(blocks.tobytes(), w, h)
+    bad = np.argwhere(np.any(got != want, axis=2))
+    assert len(bad) == 0, f"{len(bad)} texels differ, first at {bad[:4].tolist()}"
+    # the vectors reach every partition of every partitioned mode
+    v = [int.from_bytes(b.tobytes(), "little") for b in blocks.reshape(-1, 16)]
+    modes = [(x & -x).bit_length() - 1 for x in v]
+    parts = {(m, (x >> (m + 1)) & (15 if m == 0 else 63)) for x, m in zip(v, modes) if m in (0, 1, 2, 3, 7)}
+    assert len(parts) == 16 + 4 * 64
