@@ -36,6 +36,12 @@ class ShadeIn(C.Structure):
                 ("view", C.POINTER(abi.CameraView))]
 
 
+class Footprint(C.Structure):
+    _fields_ = [("covered_pixels", C.c_uint64), ("visible_triangles", C.c_uint64), ("visible_vertices", C.c_uint64),
+                ("texels", C.c_uint64), ("texel_bytes", C.c_uint64), ("shadow_texels", C.c_uint64),
+                ("latent_cells", C.c_uint64)]
+
+
 class Neural(C.Structure):
     _fields_ = [("latent_size", C.c_uint32), ("n_materials", C.c_uint32), ("slot_of", C.c_void_p),
                 ("active", C.c_void_p), ("latents", C.c_void_p), ("mlp", C.c_void_p)]
@@ -82,6 +88,9 @@ def lib():
         L.orc_shade_view_neural.restype = C.c_int
         L.orc_shade_view_neural.argtypes = [C.POINTER(Geometry), C.POINTER(ShadeIn), C.POINTER(Neural), C.c_void_p,
                                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_shade_footprint.restype = C.c_int
+        L.orc_shade_footprint.argtypes = [C.POINTER(Geometry), C.POINTER(ShadeIn), C.POINTER(Neural), C.c_void_p,
+                                          C.POINTER(Footprint), C.c_int]
         L.orc_nm_decode.restype = C.c_int
         L.orc_nm_decode.argtypes = [C.POINTER(Neural), C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
         L.orc_nm_gradients.restype = C.c_int
@@ -271,6 +280,31 @@ def shade_view(scene, view, cascades, shadow_maps, vis, scene_data=None, raster_
     if rc != 0:
         raise RuntimeError(f"orc_shade_view failed: {rc}")
     return rgba, shadow, margin
+
+
+def shade_footprint(scene, view, cascades, shadow_maps, vis, scene_data=None, raster_state=None, neural=None,
+                    n_threads=None):
+    """What shading the view reads (unique texels and their bytes, shadow-map texels, latent cells,
+    visible triangles and vertices): a dict of the orc_footprint fields."""
+    s = scene.s
+    g = geometry(scene, OPAQUE_LIST)
+    maps = [np.ascontiguousarray(m, dtype=np.float32) for m in shadow_maps]
+    ptrs = (C.POINTER(C.c_float) * max(1, len(maps)))(*[m.ctypes.data_as(C.POINTER(C.c_float)) for m in maps])
+    casc = (abi.ShadowCascade * max(1, len(cascades)))(*cascades)
+    sd = scene_data if scene_data is not None else s.scene_data
+    rs = raster_state if raster_state is not None else s.raster_state
+    cv = abi.CameraView()
+    C.memmove(C.byref(cv), C.byref(view), C.sizeof(abi.CameraView))
+    si = ShadeIn(s.materials, s.n_materials, s.n_textures, stored_textures(scene), s.samplers, s.n_samplers,
+                 len(cascades), C.cast(casc, C.POINTER(abi.ShadowCascade)),
+                 C.cast(ptrs, C.POINTER(C.POINTER(C.c_float))), C.pointer(sd), C.pointer(rs), C.pointer(cv))
+    vis = np.ascontiguousarray(vis, dtype=np.uint64)
+    out = Footprint()
+    rc = lib().orc_shade_footprint(C.byref(g), C.byref(si), C.byref(neural.struct) if neural is not None else None,
+                                   vis.ctypes.data, C.byref(out), n_threads or threads())
+    if rc != 0:
+        raise RuntimeError(f"orc_shade_footprint failed: {rc}")
+    return {k: int(getattr(out, k)) for k, _ in Footprint._fields_}
 
 
 def blend_layers(scene, view, opaque_vis, max_layers=16, raster_state=None, n_threads=None, sample=None):

@@ -85,6 +85,11 @@ void nm_decode_point(const orc_neural* nm, uint32_t material, double u, double v
     for (int c = 0; c < kOut; ++c) out[c] = std::min(1.0, std::max(0.0, out[c]));
 }
 
+void nm_cells(const orc_neural* nm, double u, double v, uint64_t cells[4]) {
+    const Fetch f = fetch(nm->latent_size, u, v);
+    for (int k = 0; k < 4; ++k) cells[k] = f.corner[k] / kCh;
+}
+
 }  // namespace orc
 
 extern "C" {

@@ -131,6 +131,22 @@ typedef struct orc_neural {
 int orc_shade_view_neural(const orc_geometry* g, const orc_shade_in* in, const orc_neural* nm, const uint64_t* vis,
                           double* out_rgba, double* out_shadow, double* out_margin, const float* override_shadow,
                           int n_threads);
+/* What shading one view reads, counted by running the FP64 shading of every pixel with each read
+ * recorded once (the algorithmic bytes of the shading kernel): unique texels of the material maps
+ * (every level the level of detail reaches), unique shadow-map texels under the PCF taps, unique
+ * latent cells the decoder fetches (nm non-null) and the visible triangles and their vertices. */
+typedef struct orc_footprint {
+    uint64_t covered_pixels;
+    uint64_t visible_triangles;
+    uint64_t visible_vertices;
+    uint64_t texels;        /* unique material-map texels read */
+    uint64_t texel_bytes;   /* their bytes as sampled: 4 per RGBA8 (and decoded BC7) texel, 2 per RG8 */
+    uint64_t shadow_texels; /* unique shadow-map texels read (4 bytes each) */
+    uint64_t latent_cells;  /* unique latent cells read (8 channels each) */
+} orc_footprint;
+int orc_shade_footprint(const orc_geometry* g, const orc_shade_in* in, const orc_neural* nm, const uint64_t* vis,
+                        orc_footprint* out, int n_threads);
+
 /* Decoder outputs at material[n], uv[n][2] -> out[n][7]. */
 int orc_nm_decode(const orc_neural* nm, const uint32_t* material, const float* uv, uint32_t n, double* out);
 /* Loss and gradients of the training objective (mean squared error over samples and channels, no

@@ -55,6 +55,30 @@ bool valid_index(uint32_t i) { return i < 4294967295u; }  // validDescriptorInde
 
 double srgb_eotf(double c) { return c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4); }
 
+// One bit per element; set from any thread.
+struct Bits {
+    std::vector<uint64_t> w;
+    explicit Bits(size_t n = 0) : w((n + 63) / 64, 0) {}
+    void set(size_t i) { __atomic_fetch_or(&w[i >> 6], uint64_t(1) << (i & 63), __ATOMIC_RELAXED); }
+    uint64_t count() const {
+        uint64_t n = 0;
+        for (uint64_t x : w) n += uint64_t(__builtin_popcountll(x));
+        return n;
+    }
+};
+
+// Footprint recorder of orc_shade_footprint: every material-map texel (per texture, levels one
+// after another), shadow-map texel, latent cell, triangle and vertex the shading reads.
+struct Touch {
+    std::vector<Bits> tex;                  // per texture
+    std::vector<std::vector<size_t>> level_base;  // per texture: first texel of each level
+    std::vector<const float*> maps;         // shadow maps, by cascade
+    std::vector<Bits> map_bits;
+    std::vector<Bits> latent;               // per decoder slot
+    Bits tris, verts;
+};
+thread_local Touch* tl_touch = nullptr;
+
 // A texture level: texels of level `level` of the description (levels stored one after another
 // from level 0, as mip_levels says; 0 or 1 = level 0 only).
 struct Tex {
@@ -62,6 +86,7 @@ struct Tex {
     int channels;
     bool srgb;
     int level = 0;
+    int id = -1;  // index in the shading inputs' textures (footprint counting)
     int levels() const { return d->mip_levels > 1 ? int(d->mip_levels) : 1; }
     int width() const { return std::max(1, int(d->width) >> level); }
     int height() const { return std::max(1, int(d->height) >> level); }
@@ -84,6 +109,8 @@ int wrap_coord(int i, int n, uint16_t mode) {
 }
 
 D4 texel(const Tex& t, int x, int y) {
+    if (tl_touch && t.id >= 0)
+        tl_touch->tex[size_t(t.id)].set(tl_touch->level_base[size_t(t.id)][size_t(t.level)] + size_t(y) * t.width() + x);
     const uint8_t* p = t.texels() + (size_t(y) * t.width() + x) * t.channels;
     D4 c{0, 0, 0, 1};
     double* o = &c.x;
@@ -153,6 +180,9 @@ double cmp_tap(const float* map, int S, double u, double v, double ref, double& 
             const int xi = std::min(std::max(x0 + i, 0), S - 1), yi = std::min(std::max(y0 + j, 0), S - 1);
             const double w = (i ? ax : 1 - ax) * (j ? ay : 1 - ay);
             const double t = map[size_t(yi) * S + xi];
+            if (tl_touch)
+                for (size_t k = 0; k < tl_touch->maps.size(); ++k)
+                    if (tl_touch->maps[k] == map) tl_touch->map_bits[k].set(size_t(yi) * S + xi);
             if (w > 0.0) margin = std::min(margin, std::fabs(ref - t));
             r += w * (ref >= t ? 1.0 : 0.0);
         }
@@ -325,8 +355,10 @@ void shade_pixel(const Ctx& c, uint32_t px, uint32_t py, uint64_t vis, double ou
     float mvp[16];
     orc::mat_mul_fma(view.view_projection, gd.transform, mvp);
     orc::C4 c32[3];
+    if (tl_touch) tl_touch->tris.set(tri);
     for (int k = 0; k < 3; ++k) {
         vin[k] = &c.g->vertices[vbase + c.g->indices[first + k]];
+        if (tl_touch) tl_touch->verts.set(vbase + c.g->indices[first + k]);
         const D4 p{vin[k]->vertex[0], vin[k]->vertex[1], vin[k]->vertex[2], 1.0};
         clip[k] = mul4(view.view_projection, mul4(gd.transform, p));
         c32[k] = orc::transform(mvp, vin[k]->vertex);
@@ -430,6 +462,11 @@ void shade_pixel(const Ctx& c, uint32_t px, uint32_t py, uint64_t vis, double ou
             // the decoder's outputs stand in for the three samples (two-channel normal map)
             double dec[7];
             orc::nm_decode_point(c.nm, dr.material_index, bv.u, bv.v, dec);
+            if (tl_touch) {
+                uint64_t cells[4];
+                orc::nm_cells(c.nm, bv.u, bv.v, cells);
+                for (uint64_t cell : cells) tl_touch->latent[c.nm->slot_of[dr.material_index]].set(cell);
+            }
             if (sd.normal_maps_enabled == 1) {
                 D3 m{dec[3] * 2 - 1, dec[4] * 2 - 1, 0.0};
                 m.z = std::sqrt(std::max(0.0, 1.0 - m.x * m.x - m.y * m.y));
@@ -561,8 +598,8 @@ double orc_pcf(const float* map, uint32_t size, double u, double v, double ref, 
 
 static int shade_view(const orc_geometry* g, const orc_shade_in* in, const orc_neural* nm, const uint64_t* vis,
                       double* out_rgba, double* out_shadow, double* out_margin, const float* override_shadow,
-                      int n_threads) {
-    if (!g || !in || !vis || !out_rgba || !in->scene || !in->raster || !in->view) return RSY_INVALID_ARGUMENT;
+                      int n_threads, Touch* touch = nullptr) {
+    if (!g || !in || !vis || (!out_rgba && !touch) || !in->scene || !in->raster || !in->view) return RSY_INVALID_ARGUMENT;
     Ctx c;
     c.g = g;
     c.in = in;
@@ -579,17 +616,21 @@ static int shade_view(const orc_geometry* g, const orc_shade_in* in, const orc_n
         if (d.format > RSY_TEX_RG8_SRGB || d.mip_levels == RSY_MIPS_BUILD) return RSY_INVALID_ARGUMENT;
         Tex x{&d, (d.format == RSY_TEX_RG8_UNORM || d.format == RSY_TEX_RG8_SRGB) ? 2 : 4,
               d.format == RSY_TEX_RGBA8_SRGB || d.format == RSY_TEX_RG8_SRGB};
+        x.id = int(t);
         c.tex.push_back(x);
     }
     const uint32_t W = in->view->width, H = in->view->height;
     const int nt = std::max(1, n_threads);
     auto work = [&](int t) {
+        tl_touch = touch;
+        double scratch[4];
         for (uint32_t y = uint32_t(uint64_t(H) * t / nt); y < uint32_t(uint64_t(H) * (t + 1) / nt); ++y)
             for (uint32_t x = 0; x < W; ++x) {
                 const size_t o = size_t(y) * W + x;
-                shade_pixel(c, x, y, vis[o], out_rgba + 4 * o, out_shadow ? out_shadow + o : nullptr,
+                shade_pixel(c, x, y, vis[o], out_rgba ? out_rgba + 4 * o : scratch, out_shadow ? out_shadow + o : nullptr,
                             out_margin ? out_margin + o : nullptr, override_shadow);
             }
+        tl_touch = nullptr;
     };
     std::vector<std::thread> th;
     for (int t = 0; t < nt; ++t) th.emplace_back(work, t);
@@ -607,6 +648,56 @@ int orc_shade_view_neural(const orc_geometry* g, const orc_shade_in* in, const o
                           int n_threads) {
     if (!nm || !nm->slot_of || !nm->latents || !nm->mlp) return RSY_INVALID_ARGUMENT;
     return shade_view(g, in, nm, vis, out_rgba, out_shadow, out_margin, override_shadow, n_threads);
+}
+
+int orc_shade_footprint(const orc_geometry* g, const orc_shade_in* in, const orc_neural* nm, const uint64_t* vis,
+                        orc_footprint* out, int n_threads) {
+    if (!g || !in || !vis || !out || !in->view) return RSY_INVALID_ARGUMENT;
+    if (nm && (!nm->slot_of || !nm->latents || !nm->mlp)) return RSY_INVALID_ARGUMENT;
+    Touch touch;
+    uint64_t n_tris = 0;
+    for (uint32_t d = 0; d < g->n_draws; ++d) n_tris += g->draws[d].index_count / 3;
+    touch.tris = Bits(n_tris);
+    touch.verts = Bits(g->n_vertices);
+    for (uint32_t t = 0; t < in->n_textures; ++t) {
+        const rsy_texture_desc& d = in->textures[t];
+        std::vector<size_t> base;
+        size_t n = 0;
+        const int levels = d.mip_levels > 1 && d.mip_levels != RSY_MIPS_BUILD ? int(d.mip_levels) : 1;
+        for (int l = 0; l < levels; ++l) {
+            base.push_back(n);
+            n += size_t(std::max(1u, d.width >> l)) * std::max(1u, d.height >> l);
+        }
+        touch.level_base.push_back(base);
+        touch.tex.emplace_back(n);
+    }
+    for (uint32_t k = 0; k < in->n_cascades; ++k) {
+        touch.maps.push_back(in->shadow_maps[k]);
+        touch.map_bits.emplace_back(size_t(in->cascades[k].size) * in->cascades[k].size);
+    }
+    if (nm) {
+        uint32_t slots = 0;
+        for (uint32_t m = 0; m < nm->n_materials; ++m)
+            if (nm->slot_of[m] != 0xffffffffu) slots = std::max(slots, nm->slot_of[m] + 1);
+        for (uint32_t k = 0; k < slots; ++k) touch.latent.emplace_back(size_t(nm->latent_size) * nm->latent_size);
+    }
+    const int r = shade_view(g, in, nm, vis, nullptr, nullptr, nullptr, nullptr, n_threads, &touch);
+    if (r != RSY_OK) return r;
+    orc_footprint f{};
+    const size_t npx = size_t(in->view->width) * in->view->height;
+    for (size_t i = 0; i < npx; ++i) f.covered_pixels += vis[i] != 0;
+    f.visible_triangles = touch.tris.count();
+    f.visible_vertices = touch.verts.count();
+    for (uint32_t t = 0; t < in->n_textures; ++t) {
+        const rsy_texture_desc& d = in->textures[t];
+        const uint64_t n = touch.tex[t].count();
+        f.texels += n;
+        f.texel_bytes += n * ((d.format == RSY_TEX_RG8_UNORM || d.format == RSY_TEX_RG8_SRGB) ? 2 : 4);
+    }
+    for (const Bits& b : touch.map_bits) f.shadow_texels += b.count();
+    for (const Bits& b : touch.latent) f.latent_cells += b.count();
+    *out = f;
+    return RSY_OK;
 }
 
 int orc_sample_texture(const rsy_texture_desc* tex, const rsy_sampler* sampler, double u, double v, double out[4]) {
