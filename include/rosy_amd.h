@@ -357,6 +357,33 @@ rsy_result rsy_read_present(rsy_ctx* ctx, uint8_t* out_bgra8, uint64_t n_pixels)
 rsy_result rsy_autotune(rsy_ctx* ctx, const rsy_frame_desc* frame, uint32_t n_iters, uint32_t* chosen_tile);
 
 /* ---------------------------------------------------------------------------------------------
+ * Allocation-free frame.  No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ *
+ * rsy_render_noalloc renders the frame rsy_render would, with the same kernels, but calls no
+ * hipMalloc, hipFree or allocating routine: the frame's working memory (transformed vertices,
+ * triangle records, bins, bin lists, raster work items, counters and view tables) is carved from
+ * one static device array of RSY_STATIC_SCRATCH_BYTES per GPU, and the outputs are the caller's
+ * device buffers.  A frame whose working memory exceeds the array is rejected with RSY_OVERFLOW
+ * before anything runs (rsy_last_error states the size); one whose bin lists overflow at run time
+ * returns RSY_OVERFLOW afterwards (there is no regrowth; the outputs are then undefined).
+ * Limits: single-sampled frames (msaa_samples 1) without a blended pass (the blended list empty, or
+ * RSY_SHADE_NONE); the per-pixel shadow factor and stage timings are not written.
+ * The call is synchronous: it returns when the frame is complete.  Every call on a GPU shares that
+ * GPU's array and its contents are those of the last call: calls that overlap on one GPU (two
+ * contexts on two threads, say) race.  The scene set-up calls (rsy_set_geometry, rsy_set_materials,
+ * rsy_set_graphic_objects, rsy_update) still allocate, as they do for rsy_render. */
+#define RSY_STATIC_SCRATCH_BYTES (128ull << 20)
+typedef struct rsy_frame_targets {
+    uint32_t n_camera_views;     /* the frame's camera views */
+    uint32_t n_cascades;         /* the frame's shadow cascades (the default one when the frame names none) */
+    uint64_t* const* visibility; /* per camera view: width*height words as rsy_read_visibility gives them */
+    float* const* color;         /* per camera view: width*height RGBA32F (may be null with RSY_SHADE_NONE) */
+    float* const* shadow_maps;   /* per cascade: size*size f32 depths */
+} rsy_frame_targets;           /* each pointer is device memory of the context's device */
+rsy_result rsy_render_noalloc(rsy_ctx* ctx, const rsy_frame_desc* frame, const rsy_frame_targets* targets);
+
+/* ---------------------------------------------------------------------------------------------
  * Neural material decoder (no counterpart in the reference: an alternative to sampling the three
  * material maps of basic.slang:172-221, judged against the plain sampler).
  *

@@ -119,6 +119,15 @@ class Config(C.Structure):
                 ("max_window_height", C.c_uint32)]
 
 
+class FrameTargets(C.Structure):
+    _fields_ = [("n_camera_views", C.c_uint32), ("n_cascades", C.c_uint32),
+                ("visibility", C.POINTER(C.c_void_p)), ("color", C.POINTER(C.c_void_p)),
+                ("shadow_maps", C.POINTER(C.c_void_p))]
+
+
+STATIC_SCRATCH_BYTES = 128 << 20
+
+
 class FrameStats(C.Structure):
     _fields_ = [("triangle_count", C.c_uint64), ("draw_call_count", C.c_uint64), ("triangles_setup", C.c_uint64),
                 ("triangles_clipped", C.c_uint64), ("tile_pairs", C.c_uint64), ("tile_size", C.c_uint32),
@@ -249,6 +258,7 @@ PROTOTYPES = [
     ("rsy_read_draw_image", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     ("rsy_read_present", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     ("rsy_autotune", C.c_int, [C.c_void_p, C.POINTER(FrameDesc), C.c_uint32, C.POINTER(C.c_uint32)]),
+    ("rsy_render_noalloc", C.c_int, [C.c_void_p, C.POINTER(FrameDesc), C.POINTER(FrameTargets)]),
     ("rsy_neural_default_config", None, [C.POINTER(NeuralConfig)]),
     ("rsy_neural_init", C.c_int, [C.c_void_p, C.POINTER(NeuralConfig)]),
     ("rsy_neural_train", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),

@@ -139,20 +139,12 @@ uint32_t choose_tile(const rsy_ctx* c) {
     return c->autotuned_tile ? c->autotuned_tile : 32u;
 }
 
-// Computes bin, vertex, record and clip-queue layout of a frame and (re)allocates its buffers.
-rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint32_t tile, uint32_t shade) {
-    Layout& L = c->layout;
-    const bool reuse = L.valid && same_views(L.views, views) && L.tile == tile && L.lists_version == c->lists_version &&
-                       L.shade == shade;
-    L.views = views;  // matrices may change without a new layout
-    if (reuse) {
-        for (size_t i = 0; i < views.size(); ++i) std::memcpy(L.vg[i].view_projection, views[i].vp, sizeof(views[i].vp));
-        for (size_t i = views.size(); i < L.vg.size(); ++i)  // blend views follow their camera view
-            std::memcpy(L.vg[i].view_projection, views[L.vg[i].source].vp, sizeof(views[0].vp));
-        return RSY_OK;
-    }
-    invalidate_graph(c);
+// Computes the bin, vertex, record and clip-queue layout of a frame into L: host tables only.
+// Record, clip-queue and bin-pair capacities are scaled by rec_scale (it doubles on overflow).
+rsy_result plan_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint32_t tile, uint32_t shade,
+                       uint32_t rec_scale, Layout& L) {
     L.valid = false;
+    L.views = views;
     L.tile = tile;
     L.lists_version = c->lists_version;
     L.shade = shade;
@@ -163,8 +155,8 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     L.blend_bins = 0;
     L.total_xv = 0;
     L.rec_total = L.clip_total = L.big_total = 0;
+    L.cull_total = 0;
     L.n_shadow = L.n_camera = L.n_blend = 0;
-    uint32_t cull_total = 0;
     const uint32_t shift = ilog2(tile);
     // the requested views, then one blend view per camera view when the blended list is drawn
     // (the blended list drawn over that camera view)
@@ -177,6 +169,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         for (uint32_t vi = 0; vi < views.size(); ++vi)
             if (!views[vi].shadow) order.push_back(vi);
     L.first_blend = uint32_t(views.size());
+    const bool min_cap = (c->debug_flags & RSY_DEBUG_MIN_CAPACITY) != 0;
     for (uint32_t oi = 0; oi < order.size(); ++oi) {
         const uint32_t vi = uint32_t(L.vg.size());
         const FrameViewDesc& f = views[order[oi]];
@@ -195,15 +188,14 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         g.samples = f.shadow ? 1u : msaa_of(c);
         const auto& list = c->lists[g.list];
         g.n_draws = uint32_t(list.size());
-        g.cull_base = cull_total;
-        cull_total += g.n_draws;
+        g.cull_base = L.cull_total;
+        L.cull_total += g.n_draws;
         g.xv_base = uint32_t(L.total_xv);
         const uint32_t tris = c->list_tris[g.list];
         // capacities in 64 bits: rec_scale doubles on every record overflow
-        const bool min_cap = (c->debug_flags & RSY_DEBUG_MIN_CAPACITY) != 0;
-        const uint64_t clip_cap = uint64_t(min_cap ? 256u : std::max<uint32_t>(4096u, tris / 8)) * c->rec_scale;
+        const uint64_t clip_cap = uint64_t(min_cap ? 256u : std::max<uint32_t>(4096u, tris / 8)) * rec_scale;
         const uint64_t rec_cap = uint64_t(tris) + 2 * clip_cap;
-        const uint64_t big_cap = uint64_t(min_cap ? 4096u : std::max<uint32_t>(65536u, tris / 4)) * c->rec_scale;
+        const uint64_t big_cap = uint64_t(min_cap ? 4096u : std::max<uint32_t>(65536u, tris / 4)) * rec_scale;
         if (rec_cap >= 0xffffffffull || uint64_t(L.rec_total) + rec_cap >= 0xffffffffull)
             return fail(c, RSY_OVERFLOW, "record buffer too large");
         if (uint64_t(L.clip_total) + clip_cap >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "clip queue too large");
@@ -239,6 +231,40 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         L.vg.push_back(g);
     }
     if (L.total_xv >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "transformed vertex buffer too large");
+    // bin lists: an estimate of the frame's (triangle, bin) pairs
+    uint64_t want_pairs = 0;
+    for (const auto& g : L.vg) want_pairs += uint64_t(c->list_tris[g.list]) * 3 + g.tiles_x * g.tiles_y;
+    want_pairs *= rec_scale;
+    if (min_cap) want_pairs = std::min<uint64_t>(want_pairs, 4096);
+    L.want_pairs = std::min<uint64_t>(want_pairs, 0xfffffff0ull);
+    return RSY_OK;
+}
+
+// Points each view's output word at its buffer in m (visibility, shadow map or blend list).
+void bind_outputs(Layout& L, const FrameMem& m) {
+    uint32_t ci = 0, si = 0, bi = 0;
+    for (auto& g : L.vg) {
+        if (g.blend) g.out = reinterpret_cast<uint64_t>(m.blend_list[bi++]);
+        else if (g.is_shadow) g.out = reinterpret_cast<uint64_t>(m.shadow_map[si++]);
+        else g.out = reinterpret_cast<uint64_t>(m.vis[ci++]);
+    }
+}
+
+// Computes the frame's layout and (re)allocates the context's frame buffers for it.
+rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint32_t tile, uint32_t shade) {
+    Layout& L = c->layout;
+    const bool reuse = L.valid && same_views(L.views, views) && L.tile == tile && L.lists_version == c->lists_version &&
+                       L.shade == shade;
+    if (reuse) {
+        L.views = views;  // matrices may change without a new layout
+        for (size_t i = 0; i < views.size(); ++i) std::memcpy(L.vg[i].view_projection, views[i].vp, sizeof(views[i].vp));
+        for (size_t i = views.size(); i < L.vg.size(); ++i)  // blend views follow their camera view
+            std::memcpy(L.vg[i].view_projection, views[L.vg[i].source].vp, sizeof(views[0].vp));
+        return RSY_OK;
+    }
+    invalidate_graph(c);
+    rsy_result r = plan_layout(c, views, tile, shade, c->rec_scale, L);
+    if (r != RSY_OK) return r;
     // buffers
     const size_t nv = L.vg.size();
     HIP_TRY(c, c->views_d.ensure(nv * sizeof(ViewGPU)), "allocate views");
@@ -246,7 +272,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     HIP_TRY(c, c->recs.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * sizeof(TriRec)), "allocate records");
     HIP_TRY(c, c->clipq.ensure(size_t(std::max<uint32_t>(1, L.clip_total)) * sizeof(uint2)), "allocate clip queue");
     HIP_TRY(c, c->counters.ensure(nv * kCtrPerView * sizeof(uint32_t)), "allocate counters");
-    HIP_TRY(c, c->draw_culled.ensure(size_t(std::max<uint32_t>(1, cull_total)) * 4), "allocate culling flags");
+    HIP_TRY(c, c->draw_culled.ensure(size_t(std::max<uint32_t>(1, L.cull_total)) * 4), "allocate culling flags");
     HIP_TRY(c, c->bin_counts.ensure(size_t(L.total_bins) * 4), "allocate bins");
     HIP_TRY(c, c->bin_offsets.ensure(size_t(L.total_bins) * 4), "allocate bins");
     HIP_TRY(c, c->pair_slots.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * 16), "allocate bin slots");
@@ -263,15 +289,10 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         HIP_TRY(c, hipMemcpy(c->vblocks_d.p, L.vblocks.data(), L.vblocks.size() * sizeof(BlockEntry), hipMemcpyHostToDevice), "upload blocks");
     if (!L.tblocks.empty())
         HIP_TRY(c, hipMemcpy(c->tblocks_d.p, L.tblocks.data(), L.tblocks.size() * sizeof(BlockEntry), hipMemcpyHostToDevice), "upload blocks");
-    // bin lists: start from an estimate, grow on overflow
-    uint64_t want_pairs = 0;
-    for (const auto& g : L.vg) want_pairs += uint64_t(c->list_tris[g.list]) * 3 + g.tiles_x * g.tiles_y;
-    want_pairs *= c->rec_scale;
-    if (c->debug_flags & RSY_DEBUG_MIN_CAPACITY) want_pairs = std::min<uint64_t>(want_pairs, 4096);
-    if (want_pairs > c->list_cap) {
-        want_pairs = std::min<uint64_t>(want_pairs, 0xfffffff0ull);
-        HIP_TRY(c, c->lists_mem.ensure(size_t(want_pairs) * 4), "allocate bin lists");
-        c->list_cap = uint32_t(want_pairs);
+    // bin lists: start from the estimate, grow on overflow
+    if (L.want_pairs > c->list_cap) {
+        HIP_TRY(c, c->lists_mem.ensure(size_t(L.want_pairs) * 4), "allocate bin lists");
+        c->list_cap = uint32_t(L.want_pairs);
     }
     HIP_TRY(c, c->items.ensure(size_t(raster_items_cap(L.total_bins, c->list_cap)) * sizeof(uint4)), "allocate work items");
     // outputs
@@ -281,26 +302,32 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     c->sample_color.resize(L.n_camera);
     c->shadow_maps.resize(L.n_shadow);
     c->blend_lists.resize(L.n_blend);
+    FrameMem& m = c->mem;
+    m = FrameMem{};
     uint32_t ci = 0, si = 0, bi = 0;
     uint64_t blend_px = 0;
     for (auto& g : L.vg) {
         const size_t px = size_t(g.width) * g.height;
         if (g.blend) {
             HIP_TRY(c, c->blend_lists[bi].ensure(px * 8), "allocate blended fragment lists");
-            g.out = reinterpret_cast<uint64_t>(c->blend_lists[bi].p);
+            m.blend_list[bi] = c->blend_lists[bi].as<uint2>();
             blend_px += px;
             ++bi;
         } else if (g.is_shadow) {
             HIP_TRY(c, c->shadow_maps[si].ensure(px * 4), "allocate shadow map");
-            g.out = reinterpret_cast<uint64_t>(c->shadow_maps[si].p);
+            m.shadow_map[si] = c->shadow_maps[si].as<float>();
             ++si;
         } else {
             HIP_TRY(c, c->vis_out[ci].ensure(px * 8 * g.samples), "allocate visibility buffer");
             HIP_TRY(c, c->color_out[ci].ensure(px * 16), "allocate colour buffer");
-            if (g.samples == 4 && L.n_blend)
+            if (g.samples == 4 && L.n_blend) {
                 HIP_TRY(c, c->sample_color[ci].ensure(px * 16 * 4), "allocate sample colours");
+                m.sample_color[ci] = c->sample_color[ci].as<float4>();
+            }
             HIP_TRY(c, c->shadow_factor_out[ci].ensure(px * 4), "allocate shadow factor");
-            g.out = reinterpret_cast<uint64_t>(c->vis_out[ci].p);
+            m.vis[ci] = c->vis_out[ci].as<uint64_t>();
+            m.color[ci] = c->color_out[ci].as<float4>();
+            m.shadow_factor[ci] = c->shadow_factor_out[ci].as<float>();
             ++ci;
         }
     }
@@ -315,21 +342,58 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
         c->frag_cap = uint32_t(want);
     }
     HIP_TRY(c, c->readback_counters.ensure(nv * kCtrPerView * 4 + 64), "allocate readback");
+    m.views = c->views_d.as<ViewGPU>();
+    m.xv = c->xv.as<float4>();
+    m.recs = c->recs.as<TriRec>();
+    m.boxes = c->boxes.as<uint2>();
+    m.shade_recs = c->shade_recs.as<ShadeRec>();
+    m.clipq = c->clipq.as<uint2>();
+    m.counters = c->counters.as<uint32_t>();
+    m.draw_culled = c->draw_culled.as<uint32_t>();
+    m.bin_counts = c->bin_counts.as<uint32_t>();
+    m.bin_offsets = c->bin_offsets.as<uint32_t>();
+    m.pair_slots = c->pair_slots.as<uint32_t>();
+    m.big_pairs = c->big_pairs.as<uint4>();
+    m.block_valid = c->block_valid.as<uint32_t>();
+    m.totals = c->totals.as<uint32_t>();
+    m.zero_bins = c->zero_bins.as<uint32_t>();
+    m.scan_partial = c->scan_partial.p;
+    m.vblocks = c->vblocks_d.as<BlockEntry>();
+    m.tblocks = c->tblocks_d.as<BlockEntry>();
+    m.lists = c->lists_mem.as<uint32_t>();
+    m.list_cap = c->list_cap;
+    m.items = c->items.as<uint4>();
+    m.frags = c->frags.as<uint4>();
+    m.frag_keys = c->frag_keys.as<uint32_t>();
+    m.frag_color = c->frag_color.as<float4>();
+    m.frag_mask = c->frag_mask.as<uint8_t>();
+    m.frag_cap = c->frag_cap;
+    const size_t nsv = L.n_camera, nsc = L.n_shadow, nbv = L.n_blend;
+    HIP_TRY(c, c->shade_views_d.ensure(std::max<size_t>(1, nsv) * sizeof(ShadeView) + 64), "allocate shade views");
+    HIP_TRY(c, c->shade_cascades_d.ensure(std::max<size_t>(1, nsc) * sizeof(ShadeCascade) + 64), "allocate cascades");
+    HIP_TRY(c, c->blend_shade_views_d.ensure(std::max<size_t>(1, nbv) * sizeof(ShadeView) + 64), "allocate blend views");
+    const size_t staging = ((nv * sizeof(ViewGPU) + 255) & ~size_t(255)) + ((nsv * sizeof(ShadeView) + 255) & ~size_t(255)) +
+                           ((nsc * sizeof(ShadeCascade) + 255) & ~size_t(255)) + nbv * sizeof(ShadeView) + 256;
+    HIP_TRY(c, c->staging.ensure(staging), "allocate staging");
+    m.shade_views = c->shade_views_d.as<ShadeView>();
+    m.shade_cascades = c->shade_cascades_d.as<ShadeCascade>();
+    m.blend_views = c->blend_shade_views_d.as<ShadeView>();
+    bind_outputs(L, m);
     L.valid = true;
     return RSY_OK;
 }
 
-// Per-frame constants (view records, shading views and cascades) go to the device through pinned
-// staging with async copies on the context stream.  They are not part of the captured frame graph:
-// a replayed graph reads whatever the last upload left in device memory.
-rsy_result upload_frame(rsy_ctx* c) {
-    Layout& L = c->layout;
+// Per-frame constants (view records, shading views and cascades) go to the device with async copies
+// on the context stream: through `staging` (pinned, at least the tables' size) when given, else
+// straight from host memory.  They are not part of the captured frame graph: a replayed graph reads
+// whatever the last upload left in device memory.
+rsy_result upload_frame(rsy_ctx* c, const Layout& L, const FrameMem& m, void* staging, uint32_t debug_flags) {
     hipStream_t s = c->stream;
     const uint32_t nv = uint32_t(L.vg.size());
     c->h_shade_views.clear();
     c->h_shade_cascades.clear();
     c->h_blend_views.clear();
-    std::vector<uint32_t> camera_of(nv, 0);  // shading view index of each camera view
+    uint32_t camera_of[kMaxViews] = {};  // shading view index of each camera view
     uint32_t blocks = 0, ci = 0;
     for (uint32_t i = 0; i < nv; ++i) {
         const ViewGPU& vg = L.vg[i];
@@ -338,8 +402,8 @@ rsy_result upload_frame(rsy_ctx* c) {
             // into that view's colour buffer
             ShadeView sv = c->h_shade_views[camera_of[vg.source]];
             sv.vis = nullptr;
-            sv.recs = c->recs.as<TriRec>() + vg.rec_base;
-            sv.shade_recs = c->shade_recs.as<ShadeRec>() + vg.rec_base;
+            sv.recs = m.recs + vg.rec_base;
+            sv.shade_recs = m.shade_recs + vg.rec_base;
             sv.xv_base = vg.xv_base;
             sv.shadow_factor = nullptr;
             c->h_blend_views.push_back(sv);
@@ -358,13 +422,13 @@ rsy_result upload_frame(rsy_ctx* c) {
             sv.block_base = blocks;
             blocks += sv.blocks_x * ((vg.height + 15) / 16);
             sv.vis = reinterpret_cast<const unsigned long long*>(vg.out);
-            sv.recs = c->recs.as<TriRec>() + vg.rec_base;
-            sv.shade_recs = c->shade_recs.as<ShadeRec>() + vg.rec_base;
+            sv.recs = m.recs + vg.rec_base;
+            sv.shade_recs = m.shade_recs + vg.rec_base;
             sv.xv_base = vg.xv_base;
             sv.samples = vg.samples;
-            sv.color = c->color_out[ci].as<float4>();
-            sv.sample_color = vg.samples == 4 && L.n_blend ? c->sample_color[ci].as<float4>() : nullptr;
-            sv.shadow_factor = (c->debug_flags & RSY_DEBUG_SHADOW_FACTOR) ? c->shadow_factor_out[ci].as<float>() : nullptr;
+            sv.color = m.color[ci];
+            sv.sample_color = vg.samples == 4 && L.n_blend ? m.sample_color[ci] : nullptr;
+            sv.shadow_factor = (debug_flags & RSY_DEBUG_SHADOW_FACTOR) ? m.shadow_factor[ci] : nullptr;
             std::memcpy(sv.view_projection, vg.view_projection, sizeof(sv.view_projection));
             std::memcpy(sv.camera_position, L.views[i].camera_position, sizeof(sv.camera_position));
             camera_of[i] = uint32_t(c->h_shade_views.size());
@@ -374,65 +438,67 @@ rsy_result upload_frame(rsy_ctx* c) {
     }
     c->shade_blocks = blocks;
     const size_t nsv = c->h_shade_views.size(), nsc = c->h_shade_cascades.size(), nbv = c->h_blend_views.size();
-    const size_t a0 = 0, a1 = (nv * sizeof(ViewGPU) + 255) & ~size_t(255);
-    const size_t a2 = a1 + ((nsv * sizeof(ShadeView) + 255) & ~size_t(255));
-    const size_t a3 = a2 + ((nsc * sizeof(ShadeCascade) + 255) & ~size_t(255));
-    const size_t total = a3 + nbv * sizeof(ShadeView) + 256;
-    HIP_TRY(c, c->shade_views_d.ensure(std::max<size_t>(1, nsv) * sizeof(ShadeView) + 64), "allocate shade views");
-    HIP_TRY(c, c->shade_cascades_d.ensure(std::max<size_t>(1, nsc) * sizeof(ShadeCascade) + 64), "allocate cascades");
-    HIP_TRY(c, c->blend_shade_views_d.ensure(std::max<size_t>(1, nbv) * sizeof(ShadeView) + 64), "allocate blend views");
-    HIP_TRY(c, c->staging.ensure(total), "allocate staging");
-    uint8_t* st = static_cast<uint8_t*>(c->staging.p);
-    std::memcpy(st + a0, L.vg.data(), nv * sizeof(ViewGPU));
-    if (nsv) std::memcpy(st + a1, c->h_shade_views.data(), nsv * sizeof(ShadeView));
-    if (nsc) std::memcpy(st + a2, c->h_shade_cascades.data(), nsc * sizeof(ShadeCascade));
-    if (nbv) std::memcpy(st + a3, c->h_blend_views.data(), nbv * sizeof(ShadeView));
-    HIP_TRY(c, hipMemcpyAsync(c->views_d.p, st + a0, nv * sizeof(ViewGPU), hipMemcpyHostToDevice, s), "upload views");
-    if (nsv)
-        HIP_TRY(c, hipMemcpyAsync(c->shade_views_d.p, st + a1, nsv * sizeof(ShadeView), hipMemcpyHostToDevice, s), "upload");
-    if (nsc)
-        HIP_TRY(c, hipMemcpyAsync(c->shade_cascades_d.p, st + a2, nsc * sizeof(ShadeCascade), hipMemcpyHostToDevice, s), "upload");
-    if (nbv)
-        HIP_TRY(c, hipMemcpyAsync(c->blend_shade_views_d.p, st + a3, nbv * sizeof(ShadeView), hipMemcpyHostToDevice, s),
-                "upload");
+    struct Table {
+        void* dst;
+        const void* src;
+        size_t bytes;
+    };
+    const Table tables[4] = {{m.views, L.vg.data(), nv * sizeof(ViewGPU)},
+                             {m.shade_views, c->h_shade_views.data(), nsv * sizeof(ShadeView)},
+                             {m.shade_cascades, c->h_shade_cascades.data(), nsc * sizeof(ShadeCascade)},
+                             {m.blend_views, c->h_blend_views.data(), nbv * sizeof(ShadeView)}};
+    size_t at = 0;
+    for (const Table& t : tables) {
+        if (!t.bytes) continue;
+        const void* src = t.src;
+        if (staging) {
+            uint8_t* st = static_cast<uint8_t*>(staging) + at;
+            std::memcpy(st, t.src, t.bytes);
+            src = st;
+            at += (t.bytes + 255) & ~size_t(255);
+        }
+        HIP_TRY(c, hipMemcpyAsync(t.dst, src, t.bytes, hipMemcpyHostToDevice, s), "upload frame constants");
+    }
     return RSY_OK;
 }
 
-// Enqueues the kernels of one frame on the context stream (graph-capturable: kernels only).
-rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
-    Layout& L = c->layout;
+// Enqueues the kernels of one frame on the context stream (graph-capturable: kernels only).  With
+// `bookkeeping` (the ordinary frame, not while capturing): HIP events around the frame and, with
+// RSY_DEBUG_STAGE_TIMING, between its stages, and the counters copied to the pinned readback buffer.
+rsy_result enqueue_frame(rsy_ctx* c, const Layout& L, const FrameMem& m, bool capturing, bool bookkeeping) {
     hipStream_t s = c->stream;
     const uint32_t nv = uint32_t(L.vg.size());
     const uint32_t shift = ilog2(L.tile);
-    const bool timing = (c->debug_flags & RSY_DEBUG_STAGE_TIMING) && !capturing;
-    if (!capturing) HIP_TRY(c, hipEventRecord(c->ev_begin, s), "event");
+    const bool events = bookkeeping && !capturing;
+    const bool timing = events && (c->debug_flags & RSY_DEBUG_STAGE_TIMING);
+    if (events) HIP_TRY(c, hipEventRecord(c->ev_begin, s), "event");
 
     GeometryLaunch g{};
-    g.views = c->views_d.as<ViewGPU>();
+    g.views = m.views;
     g.n_views = nv;
     g.draws0 = c->lists_d[0].as<DrawGPU>();
     g.draws1 = c->lists_d[1].as<DrawGPU>();
     g.draws2 = c->lists_d[2].as<DrawGPU>();
     g.objects = c->objects.as<float>();
-    g.vblocks = c->vblocks_d.as<BlockEntry>();
+    g.vblocks = m.vblocks;
     g.n_vblocks = uint32_t(L.vblocks.size());
-    g.tblocks = c->tblocks_d.as<BlockEntry>();
+    g.tblocks = m.tblocks;
     g.n_tblocks = uint32_t(L.tblocks.size());
     g.px = c->px.as<float>();
     g.py = c->py.as<float>();
     g.pz = c->pz.as<float>();
     g.indices = c->indices.as<uint32_t>();
-    g.xv = c->xv.as<float4>();
-    g.recs = c->recs.as<TriRec>();
-    g.boxes = c->boxes.as<uint2>();
-    g.shade_recs = c->shade_recs.as<ShadeRec>();
-    g.clipq = c->clipq.as<uint2>();
-    g.counters = c->counters.as<uint32_t>();
-    g.bin_counts = c->bin_counts.as<uint32_t>();
-    g.pair_slots = c->pair_slots.as<uint32_t>();
-    g.big_pairs = c->big_pairs.as<uint4>();
-    g.block_valid = c->block_valid.as<uint32_t>();
-    g.draw_culled = c->draw_culled.as<uint32_t>();
+    g.xv = m.xv;
+    g.recs = m.recs;
+    g.boxes = m.boxes;
+    g.shade_recs = m.shade_recs;
+    g.clipq = m.clipq;
+    g.counters = m.counters;
+    g.bin_counts = m.bin_counts;
+    g.pair_slots = m.pair_slots;
+    g.big_pairs = m.big_pairs;
+    g.block_valid = m.block_valid;
+    g.draw_culled = m.draw_culled;
     g.shift = shift;
     g.clip_blocks = 128;
     g.total_bins = L.total_bins;
@@ -444,25 +510,25 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     b.n_views = nv;
     b.total_bins = L.total_bins;
     b.bin_counts = g.bin_counts;
-    b.bin_offsets = c->bin_offsets.as<uint32_t>();
-    b.pair_slots = c->pair_slots.as<uint32_t>();
-    b.big_pairs = c->big_pairs.as<uint4>();
-    b.block_valid = c->block_valid.as<uint32_t>();
+    b.bin_offsets = m.bin_offsets;
+    b.pair_slots = m.pair_slots;
+    b.big_pairs = m.big_pairs;
+    b.block_valid = m.block_valid;
     b.n_tblocks = uint32_t(L.tblocks.size());
-    b.totals = c->totals.as<uint32_t>();
+    b.totals = m.totals;
     b.recs = g.recs;
     b.boxes = g.boxes;
     b.counters = g.counters;
-    b.lists = c->lists_mem.as<uint32_t>();
-    b.list_cap = c->list_cap;
+    b.lists = m.lists;
+    b.list_cap = m.list_cap;
     b.shift = shift;
     uint32_t max_tris = 0;
     for (const auto& v : L.vg) max_tris = std::max(max_tris, v.tri_count);
     b.scatter_blocks = std::max<uint32_t>(64, std::min<uint32_t>(4096, (max_tris + 255) / 256));
-    b.scan_partial = c->scan_partial.p;
-    b.items = c->items.as<uint4>();
-    b.items_cap = raster_items_cap(L.total_bins, c->list_cap);
-    b.zero_bins = c->zero_bins.as<uint32_t>();
+    b.scan_partial = m.scan_partial;
+    b.items = m.items;
+    b.items_cap = raster_items_cap(L.total_bins, m.list_cap);
+    b.zero_bins = m.zero_bins;
     HIP_TRY(c, launch_binning(b, s), "binning kernels");
     if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[1], s), "event");
 
@@ -472,7 +538,7 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
     r.total_bins = L.total_bins;
     r.tile = L.tile;
     r.lists = b.lists;
-    r.list_cap = c->list_cap;
+    r.list_cap = m.list_cap;
     r.recs = g.recs;
     r.rec_cap_total = L.rec_total;
     r.items = b.items;
@@ -484,10 +550,10 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
 
     if (L.shade && L.n_camera) {
         ShadeLaunch sh{};
-        sh.views = c->shade_views_d.as<ShadeView>();
+        sh.views = m.shade_views;
         sh.n_views = uint32_t(c->h_shade_views.size());
         sh.total_blocks = c->shade_blocks;
-        sh.cascades = c->shade_cascades_d.as<ShadeCascade>();
+        sh.cascades = m.shade_cascades;
         sh.n_cascades = uint32_t(c->h_shade_cascades.size());
         sh.pcf_mode = c->rs.pcf_mode;
         sh.scene = c->scene_d.p;
@@ -521,17 +587,17 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
             bl.bin_counts = g.bin_counts;
             bl.bin_offsets = b.bin_offsets;
             bl.lists = b.lists;
-            bl.list_cap = c->list_cap;
+            bl.list_cap = m.list_cap;
             bl.recs = g.recs;
             bl.rec_cap_total = L.rec_total;
             bl.counters = g.counters;
-            bl.frags = c->frags.as<uint4>();
-            bl.frag_keys = c->frag_keys.as<uint32_t>();
-            bl.frag_color = c->frag_color.as<float4>();
-            bl.frag_mask = c->frag_mask.as<uint8_t>();
-            bl.frag_cap = c->frag_cap;
+            bl.frags = m.frags;
+            bl.frag_keys = m.frag_keys;
+            bl.frag_color = m.frag_color;
+            bl.frag_mask = m.frag_mask;
+            bl.frag_cap = m.frag_cap;
             bl.samples = msaa_of(c);
-            sb.views = c->blend_shade_views_d.as<ShadeView>();
+            sb.views = m.blend_views;
             sb.n_views = uint32_t(c->h_blend_views.size());
             sb.draws = c->lists_d[2].as<DrawGPU>();
             sb.n_draws = uint32_t(c->lists[2].size());
@@ -560,9 +626,10 @@ rsy_result enqueue_frame(rsy_ctx* c, bool capturing) {
         HIP_TRY(c, hipEventRecord(c->ev_stage[5], s), "event");
     }
     if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[4], s), "event");
-    if (!capturing) {
-        HIP_TRY(c, hipMemcpyAsync(c->readback_counters.p, c->counters.p, nv * kCtrPerView * 4, hipMemcpyDeviceToHost, s), "readback");
-        HIP_TRY(c, hipMemcpyAsync(static_cast<uint8_t*>(c->readback_counters.p) + nv * kCtrPerView * 4, c->totals.p, 12,
+    if (events) {
+        HIP_TRY(c, hipMemcpyAsync(c->readback_counters.p, m.counters, nv * kCtrPerView * 4, hipMemcpyDeviceToHost, s),
+                "readback");
+        HIP_TRY(c, hipMemcpyAsync(static_cast<uint8_t*>(c->readback_counters.p) + nv * kCtrPerView * 4, m.totals, 12,
                                   hipMemcpyDeviceToHost, s), "readback");
         HIP_TRY(c, hipEventRecord(c->ev_end, s), "event");
     }
@@ -618,9 +685,9 @@ bool collect_stats(rsy_ctx* c) {
 rsy_result render_now(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint32_t shade) {
     rsy_result r = build_layout(c, views, choose_tile(c), shade);
     if (r != RSY_OK) return r;
-    r = upload_frame(c);
+    r = upload_frame(c, c->layout, c->mem, c->staging.p, c->debug_flags);
     if (r != RSY_OK) return r;
-    r = enqueue_frame(c, false);
+    r = enqueue_frame(c, c->layout, c->mem, false, true);
     if (r != RSY_OK) return r;
     c->last_views = views;
     c->last_shade = shade;
@@ -668,9 +735,7 @@ rsy_result finish_frame(rsy_ctx* c) {
             if (want <= c->list_cap) return fail(c, RSY_OVERFLOW, "bin lists exceed 2^32 entries");
             HIP_TRY(c, c->lists_mem.ensure(size_t(want) * 4), "grow bin lists");
             c->list_cap = uint32_t(want);
-            HIP_TRY(c, c->items.ensure(size_t(raster_items_cap(c->layout.total_bins, c->list_cap)) * sizeof(uint4)),
-                    "grow work items");
-            invalidate_graph(c);
+            c->layout.valid = false;  // the frame's addresses are rebound to the grown lists
         }
         rsy_result r = render_now(c, c->last_views, c->last_shade);
         if (r != RSY_OK) return r;
@@ -720,6 +785,11 @@ rsy_result rsy_create(const rsy_config* cfg, rsy_ctx** out_ctx) {
             delete c;
             return RSY_GRAPHICS_INIT_FAILURE;
         }
+    // the allocation-free frame's static scratch: this device's copy, looked up once
+    if (static_scratch_init(dev) != hipSuccess) {
+        delete c;
+        return RSY_GRAPHICS_INIT_FAILURE;
+    }
     const std::vector<float> lut = make_srgb_lut();
     if (c->srgb_lut.ensure(256 * 4) != hipSuccess ||
         hipMemcpy(c->srgb_lut.p, lut.data(), 256 * 4, hipMemcpyHostToDevice) != hipSuccess ||
@@ -1025,7 +1095,7 @@ rsy_result rsy_render_graph(rsy_ctx* c, const rsy_frame_desc* frame, uint32_t n_
         invalidate_graph(c);
         hipGraph_t graph = nullptr;
         HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-        r = enqueue_frame(c, true);
+        r = enqueue_frame(c, c->layout, c->mem, true, true);
         hipError_t e = hipStreamEndCapture(c->stream, &graph);
         if (r != RSY_OK) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -1048,6 +1118,119 @@ rsy_result rsy_render_graph(rsy_ctx* c, const rsy_frame_desc* frame, uint32_t n_
     const bool over = collect_stats(c);
     if (n_replays) c->stats.gpu_ms /= float(n_replays);
     if (over) return fail(c, RSY_OVERFLOW, "graph replay overflowed its buffers");
+    return RSY_OK;
+}
+
+// The allocation-free frame (no counterpart in the reference): rsy_render's kernels with their
+// working memory carved from the static scratch array and the caller's output buffers.
+rsy_result rsy_render_noalloc(rsy_ctx* c, const rsy_frame_desc* frame, const rsy_frame_targets* t) {
+    if (!c) return RSY_INVALID_ARGUMENT;
+    if (!t) return fail(c, RSY_INVALID_ARGUMENT, "targets == NULL");
+    if (!c->has_geometry) return fail(c, RSY_INVALID_STATE, "rsy_set_geometry has not been called");
+    if (!c->has_objects) return fail(c, RSY_INVALID_STATE, "rsy_set_graphic_objects has not been called");
+    if (!c->has_scene) return fail(c, RSY_INVALID_STATE, "rsy_update has not been called");
+    if (c->lists_dirty)
+        return fail(c, RSY_INVALID_STATE, "draw lists not built: call rsy_set_graphic_objects after rsy_set_geometry");
+    if (msaa_of(c) != 1) return fail(c, RSY_INVALID_STATE, "the allocation-free frame is single-sampled (msaa_samples 1)");
+    std::vector<FrameViewDesc>& views = c->static_views;
+    uint32_t shade = 0;
+    rsy_result r = resolve_views(c, frame, views, shade);
+    if (r != RSY_OK) return r;
+    if (shade != RSY_SHADE_NONE && !c->lists[2].empty())
+        return fail(c, RSY_INVALID_STATE, "the allocation-free frame has no blended pass (the blended list is not empty)");
+    uint32_t n_cam = 0, n_casc = 0;
+    for (const auto& v : views) (v.shadow ? n_casc : n_cam)++;
+    if (t->n_camera_views != n_cam || t->n_cascades != n_casc)
+        return fail(c, RSY_INVALID_ARGUMENT, "targets do not match the frame's camera views and cascades");
+    if (!t->visibility || !t->shadow_maps || (shade != RSY_SHADE_NONE && !t->color))
+        return fail(c, RSY_INVALID_ARGUMENT, "null target array");
+    for (uint32_t i = 0; i < n_cam; ++i)
+        if (!t->visibility[i] || (shade != RSY_SHADE_NONE && !t->color[i]))
+            return fail(c, RSY_INVALID_ARGUMENT, "null camera view target");
+    for (uint32_t i = 0; i < n_casc; ++i)
+        if (!t->shadow_maps[i]) return fail(c, RSY_INVALID_ARGUMENT, "null shadow map target");
+    HIP_TRY(c, hipSetDevice(c->device), "set device");
+    uint8_t* const base = static_cast<uint8_t*>(static_scratch(c->device));
+    if (!base) return fail(c, RSY_INVALID_STATE, "static scratch of this device not looked up");
+
+    Layout& L = c->static_layout;
+    r = plan_layout(c, views, choose_tile(c), shade, 1u, L);
+    if (r != RSY_OK) return r;
+    // carve the working memory, 256-byte aligned pieces in a fixed order
+    const size_t nv = L.vg.size();
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = at;
+        at += (std::max<size_t>(bytes, 4) + 255) & ~size_t(255);
+        return base + o;
+    };
+    FrameMem m{};
+    m.views = reinterpret_cast<ViewGPU*>(take(nv * sizeof(ViewGPU)));
+    m.counters = reinterpret_cast<uint32_t*>(take(nv * kCtrPerView * 4));
+    m.totals = reinterpret_cast<uint32_t*>(take(32));
+    m.shade_views = reinterpret_cast<ShadeView*>(take(L.n_camera * sizeof(ShadeView)));
+    m.shade_cascades = reinterpret_cast<ShadeCascade*>(take(L.n_shadow * sizeof(ShadeCascade)));
+    m.vblocks = reinterpret_cast<BlockEntry*>(take(L.vblocks.size() * sizeof(BlockEntry)));
+    m.tblocks = reinterpret_cast<BlockEntry*>(take(L.tblocks.size() * sizeof(BlockEntry)));
+    m.block_valid = reinterpret_cast<uint32_t*>(take(L.tblocks.size() * 4));
+    m.draw_culled = reinterpret_cast<uint32_t*>(take(size_t(L.cull_total) * 4));
+    m.bin_counts = reinterpret_cast<uint32_t*>(take(size_t(L.total_bins) * 4));
+    m.bin_offsets = reinterpret_cast<uint32_t*>(take(size_t(L.total_bins) * 4));
+    m.zero_bins = reinterpret_cast<uint32_t*>(take(size_t(L.total_bins) * 4));
+    m.scan_partial = take((size_t(L.total_bins) / kScanBlockBins + 1) * kScanPartialBytes);
+    m.xv = reinterpret_cast<float4*>(take(size_t(L.total_xv) * sizeof(float4)));
+    m.recs = reinterpret_cast<TriRec*>(take(size_t(L.rec_total) * sizeof(TriRec)));
+    m.boxes = reinterpret_cast<uint2*>(take(size_t(L.rec_total) * 8));
+    m.shade_recs = reinterpret_cast<ShadeRec*>(take(size_t(L.rec_total) * sizeof(ShadeRec)));
+    m.pair_slots = reinterpret_cast<uint32_t*>(take(size_t(L.rec_total) * 16));
+    m.clipq = reinterpret_cast<uint2*>(take(size_t(L.clip_total) * sizeof(uint2)));
+    m.big_pairs = reinterpret_cast<uint4*>(take(size_t(L.big_total) * 16));
+    // the bin lists and work items take the rest: at least the frame's estimated pairs
+    const auto need_for = [&](uint64_t pairs) {  // with both pieces' alignment slack
+        return uint64_t(at) + pairs * 4 + uint64_t(raster_items_cap(L.total_bins, uint32_t(pairs))) * sizeof(uint4) + 512;
+    };
+    if (need_for(L.want_pairs) > kStaticScratchBytes) {
+        char msg[160];
+        std::snprintf(msg, sizeof(msg), "the frame needs %.1f MiB of working memory, more than the %zu MiB static scratch",
+                      double(need_for(L.want_pairs)) / double(1 << 20), kStaticScratchBytes >> 20);
+        return fail(c, RSY_OVERFLOW, msg);
+    }
+    // the largest capacity that fits (need_for grows with it; the estimate fits, as checked)
+    uint64_t lo = L.want_pairs, hi = std::min<uint64_t>((kStaticScratchBytes - at) / 4, 0xfffffff0ull);
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo + 1) / 2;
+        if (need_for(mid) <= kStaticScratchBytes) lo = mid;
+        else hi = mid - 1;
+    }
+    m.list_cap = uint32_t(lo);
+    m.lists = reinterpret_cast<uint32_t*>(take(size_t(m.list_cap) * 4));
+    m.items = reinterpret_cast<uint4*>(take(size_t(raster_items_cap(L.total_bins, m.list_cap)) * sizeof(uint4)));
+    if (at > kStaticScratchBytes) return fail(c, RSY_OVERFLOW, "static scratch layout exceeds its size");
+    // the caller's outputs
+    for (uint32_t i = 0; i < n_cam; ++i) {
+        m.vis[i] = t->visibility[i];
+        m.color[i] = shade != RSY_SHADE_NONE ? reinterpret_cast<float4*>(t->color[i]) : nullptr;
+    }
+    for (uint32_t i = 0; i < n_casc; ++i) m.shadow_map[i] = t->shadow_maps[i];
+    bind_outputs(L, m);
+    // a frame from rsy_render may still be in flight on the stream: the tables are written behind it
+    if (!L.vblocks.empty())
+        HIP_TRY(c, hipMemcpyAsync(m.vblocks, L.vblocks.data(), L.vblocks.size() * sizeof(BlockEntry), hipMemcpyHostToDevice,
+                                  c->stream), "upload blocks");
+    if (!L.tblocks.empty())
+        HIP_TRY(c, hipMemcpyAsync(m.tblocks, L.tblocks.data(), L.tblocks.size() * sizeof(BlockEntry), hipMemcpyHostToDevice,
+                                  c->stream), "upload blocks");
+    r = upload_frame(c, L, m, nullptr, 0u);
+    if (r != RSY_OK) return r;
+    r = enqueue_frame(c, L, m, false, false);
+    if (r != RSY_OK) return r;
+    uint32_t ctr[kMaxViews * kCtrPerView + 8] = {};
+    HIP_TRY(c, hipMemcpyAsync(ctr, m.counters, nv * kCtrPerView * 4, hipMemcpyDeviceToHost, c->stream), "readback");
+    HIP_TRY(c, hipMemcpyAsync(ctr + nv * kCtrPerView, m.totals, 12, hipMemcpyDeviceToHost, c->stream), "readback");
+    HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
+    bool over = ctr[nv * kCtrPerView + 2] != 0;
+    for (size_t v = 0; v < nv; ++v) over |= ctr[v * kCtrPerView + kCtrOverflow] != 0;
+    if (over) return fail(c, RSY_OVERFLOW, "the frame overflowed the working memory the static scratch holds");
     return RSY_OK;
 }
 

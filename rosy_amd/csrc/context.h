@@ -106,7 +106,8 @@ struct Layout {
     std::vector<BlockEntry> vblocks, tblocks;
     uint32_t total_bins = 0;
     uint64_t total_xv = 0;
-    uint32_t rec_total = 0, clip_total = 0, big_total = 0;
+    uint32_t rec_total = 0, clip_total = 0, big_total = 0, cull_total = 0;
+    uint64_t want_pairs = 0;  // estimate of the frame's (triangle, bin) pairs
     uint32_t n_shadow = 0, n_camera = 0, n_blend = 0;
     uint32_t first_blend = 0, blend_bins = 0;  // first blend view, bins of all blend views
     bool valid = false;
@@ -124,6 +125,57 @@ inline uint32_t ilog2(uint32_t v) {
     while ((1u << (r + 1)) <= v) ++r;
     return r;
 }
+
+// Device addresses of one frame's working memory (everything its kernels use between the inputs
+// and the outputs) and of its outputs.  The ordinary frame points them into the context's own
+// buffers; the allocation-free frame carves the working memory from one static device array and
+// takes the outputs from the caller.
+struct FrameMem {
+    ViewGPU* views = nullptr;
+    float4* xv = nullptr;
+    TriRec* recs = nullptr;
+    uint2* boxes = nullptr;
+    ShadeRec* shade_recs = nullptr;
+    uint2* clipq = nullptr;
+    uint32_t* counters = nullptr;
+    uint32_t* draw_culled = nullptr;
+    uint32_t* bin_counts = nullptr;
+    uint32_t* bin_offsets = nullptr;
+    uint32_t* pair_slots = nullptr;
+    uint4* big_pairs = nullptr;
+    uint32_t* block_valid = nullptr;
+    uint32_t* totals = nullptr;
+    uint32_t* zero_bins = nullptr;
+    void* scan_partial = nullptr;
+    BlockEntry* vblocks = nullptr;
+    BlockEntry* tblocks = nullptr;
+    uint32_t* lists = nullptr;
+    uint32_t list_cap = 0;
+    uint4* items = nullptr;
+    ShadeView* shade_views = nullptr;
+    ShadeCascade* shade_cascades = nullptr;
+    ShadeView* blend_views = nullptr;
+    // the blended pass's fragments
+    uint4* frags = nullptr;
+    uint32_t* frag_keys = nullptr;
+    float4* frag_color = nullptr;
+    uint8_t* frag_mask = nullptr;
+    uint32_t frag_cap = 0;
+    // outputs: per camera view, per cascade, per blend view (in view order within each kind)
+    uint64_t* vis[kMaxViews] = {};
+    float4* color[kMaxViews] = {};
+    float4* sample_color[kMaxViews] = {};
+    float* shadow_factor[kMaxViews] = {};
+    float* shadow_map[kMaxViews] = {};
+    uint2* blend_list[kMaxViews] = {};
+};
+
+// The allocation-free frame's working memory: a static device array of kStaticScratchBytes per
+// GPU (static_scratch.hip).  static_scratch_init looks up the calling device's copy (once per
+// device; rsy_create calls it); static_scratch returns it, or null before the lookup.
+constexpr size_t kStaticScratchBytes = size_t(128) << 20;
+hipError_t static_scratch_init(int device);
+void* static_scratch(int device);
 
 struct NeuralState;  // material decoder (neural_ctx.cpp)
 void destroy_neural(NeuralState* n);
@@ -178,6 +230,9 @@ struct rsy_ctx {
 
     // frame buffers
     rsy::Layout layout;
+    rsy::FrameMem mem;            // the ordinary frame's addresses (into the buffers below)
+    rsy::Layout static_layout;    // the allocation-free frame's layout (host tables only)
+    std::vector<rsy::FrameViewDesc> static_views;
     rsy::DevBuf views_d, xv, recs, boxes, shade_recs, clipq, counters, bin_counts, bin_offsets, pair_slots, big_pairs, block_valid, totals,
         lists_mem, vblocks_d, tblocks_d, shade_views_d, shade_cascades_d, items, zero_bins,
         draw_culled, scan_partial;
