@@ -14,16 +14,25 @@ neural-material frame in device milliseconds (HIP events around the timed replay
 is timed the same way beside it.
 
 Also reported:
-  roofline      the dominant kernel (the tile raster): its algorithmic bytes per launch / its average
-                duration measured live with HIP events on the context's stream, against 8 TB/s;
-                `traffic` is its measured HBM bytes per launch from the committed rocprofv3
-                FETCH_SIZE / WRITE_SIZE passes (profiles/k_raster_pmc_decoder_1080p.json);
+  roofline      the kernel that dominates the frame, the shading kernel of the neural form: its
+                algorithmic bytes per launch (8 B visibility read and 16 B colour write per pixel,
+                48 B per visible vertex, the unique latent cells (16 B) and map texels it reads, 4 B
+                per unique shadow-map texel; the unique counts come from the oracle's footprint of
+                the same frame) / its average duration measured live with HIP events on the
+                context's stream, against 8 TB/s; `traffic` is its measured HBM bytes per launch from
+                the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+                (profiles/k_shade_pmc_decoder_1080p.json);
+  kernels       the same for the plain shading kernel, the tile raster (4 B per (triangle, bin)
+                pair, 32 B per set-up triangle, 8 B per camera pixel and 4 B per shadow texel of the
+                bins it writes) and the whole geometry + binning + raster stage (the survey's B_geo);
   decoder       MFMA FLOPs of the decoder per frame / the neural shading kernel's duration, against
                 the 2.5 PFLOP/s dense FP16 peak (the shading kernel is not MFMA-bound);
   cpu_baseline  the CPU oracle (fixed-point raster + FP64 shading, std::thread over the host cores)
                 on the same plain frame, repeated for a bounded time;
-  max_abs_err   shaded colour of both forms vs the FP64 CPU oracle (re-run with the GPU's shadow
-                factors; for the neural form with the same FP16 decoder parameters).
+  max_abs_err   shaded colour of both forms vs the FP64 CPU oracle, with the parity tests' rule for
+                shadow compares: the GPU's PCF factor may differ from the oracle's only at a compare
+                tie (|reference - texel| < 1e-5), and the colour is compared with the oracle re-run
+                with the GPU's factor (for the neural form with the same FP16 decoder parameters).
 """
 import argparse
 import json
@@ -61,22 +70,89 @@ def choose_tile(r, scene, frame, tile):
     return tile
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_raster_pmc_decoder_1080p.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_shade_pmc_decoder_1080p.json")
 
 
-def measured_traffic(tile):
-    """HBM bytes per k_raster launch from the committed FETCH_SIZE / WRITE_SIZE passes of this workload
-    (each counter scaled by what it reads for a plain device copy of known size), or None when no
-    measurement of this bin size is committed."""
+def measured_traffic(kernel, tile):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE / WRITE_SIZE passes of this
+    workload (each counter scaled by what it reads for a plain device copy of known size), or None
+    when no measurement of this kernel at this bin size is committed."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("tile") != tile:
+    k = t.get("kernels", {}).get(kernel)
+    if k is None or t.get("tile") != tile:
         return None
-    return int(round(1024 * (t["fetch_kib_per_dispatch"] * t["fetch_scale"] +
-                             t["write_kib_per_dispatch"] * t["write_scale"])))
+    return int(round(1024 * (k["fetch_kib_per_dispatch"] * t["fetch_scale"] +
+                             k["write_kib_per_dispatch"] * t["write_scale"])))
+
+
+def hbm_line(kernel, nbytes, ms, extra=None):
+    """One kernel's HBM roofline entry: algorithmic bytes per launch over its measured duration."""
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    d = {"kernel": kernel, "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(gbs / HBM_PEAK_GBS, 5), "algorithmic_bytes": int(nbytes), "avg_launch_ms": round(ms, 5)}
+    if extra:
+        d.update(extra)
+    return d
+
+
+def shade_bytes(fp, pixels):
+    """Algorithmic bytes of one shading launch from the oracle's footprint: 8 B visibility read and
+    16 B colour write per pixel, 48 B per visible vertex (normal, tangent, uv, colour), the unique
+    map texels, 16 B per unique latent cell (8 FP16 channels), 4 B per unique shadow-map texel."""
+    parts = {"pixels": 24 * pixels, "vertices": 48 * fp["visible_vertices"], "texels": fp["texel_bytes"],
+             "latents": 16 * fp["latent_cells"], "shadow": 4 * fp["shadow_texels"]}
+    return sum(parts.values()), parts
+
+
+def list_vertices(scene):
+    """Vertices the transform reads for the opaque list (every surface of these workloads)."""
+    return sum(int(scene.s.meshes[d.mesh_index].num_vertices) for d in scene.surfaces())
+
+
+def geo_bytes(scene, views, pairs):
+    """The survey's B_geo over the frame's views: per view 12 V (positions) + 12 T (indices) +
+    164 per object + 2 x 16 V (transformed vertices written and read) + 2 x 32 T (setup records
+    written and read) + 8 B per visibility word or 4 B per shadow texel; plus 8 B per (triangle, bin)
+    pair over the frame (list entry written and read)."""
+    V, T, n_obj = list_vertices(scene), scene.n_triangles, scene.s.n_objects
+    b = 0
+    for w, h, shadow in views:
+        b += 12 * V + 12 * T + 164 * n_obj + 32 * V + 64 * T + w * h * (4 if shadow else 8)
+    return b + 8 * pairs
+
+
+def raster_bytes(scene, views, rs, tile, pairs, setup):
+    """The tile raster's algorithmic bytes: 4 B per (triangle, bin) pair, 32 B per set-up triangle,
+    8 B per camera pixel and 4 B per shadow texel of the bins it writes (those with a triangle; empty
+    bins are cleared by the scatter)."""
+    import numpy as np
+
+    import oracle
+    b = 4 * pairs + 32 * setup
+    for vp, w, h, shadow in views:
+        counts, _ = oracle.tile_counts(scene, vp, w, h, shadow, rs, tile)
+        ys, xs = np.nonzero(counts)
+        px = sum(min(tile, w - x * tile) * min(tile, h - y * tile) for y, x in zip(ys.tolist(), xs.tolist()))
+        b += px * (4 if shadow else 8)
+    return b
+
+
+def colour_error(col, sf, ref_fn):
+    """Max abs colour error vs the FP64 oracle with the parity tests' PCF rule: the oracle's own
+    PCF factor must equal the GPU's except at a compare tie (|ref - texel| < 1e-5); the colour is then
+    compared with the oracle re-run with the GPU's factor.  Returns (error, tie pixels, pixels whose
+    factor differs away from a tie)."""
+    import numpy as np
+    ref, rsf, margin = ref_fn(None)
+    flip = np.abs(sf - rsf) > 2e-3
+    ties = int(flip.sum())
+    away = int((flip & ~(margin < 1e-5)).sum())
+    ref2, _, _ = ref_fn(sf)
+    return float(np.abs(col.astype(np.float64) - ref2).max()), ties, away
 
 
 def stage_medians(r, frame, n=10):
@@ -226,6 +302,152 @@ def texture_timing(args):
     return out
 
 
+def _srgb_lut():
+    import numpy as np
+    v = np.arange(256) / 255.0
+    return np.where(v <= 0.04045, v / 12.92, ((v + 0.055) / 1.055) ** 2.4)
+
+
+def _map_channels(albedo, normal, mr, lut):
+    """The decoder's 7 channels from a material's three stored maps (the plain sampler's values at
+    texel centres): linear albedo rgb, normal-map r g (UNORM), metallic and roughness (sRGB decoded)."""
+    import numpy as np
+    return np.concatenate([lut[albedo[..., :3]], normal[..., :2] / 255.0, lut[mr[..., :2]]], axis=-1)
+
+
+def _bilinear_repeat(img, out_w, out_h):
+    """Bilinear REPEAT sampling of img (h, w, c) at the texel centres of an out_w x out_h grid."""
+    import numpy as np
+    h, w = img.shape[:2]
+
+    def axis(n_out, n_in):
+        x = (np.arange(n_out) + 0.5) / n_out * n_in - 0.5
+        x0 = np.floor(x).astype(np.int64)
+        return x0 % n_in, (x0 + 1) % n_in, (x - x0)
+
+    xa, xb, wx = axis(out_w, w)
+    ya, yb, wy = axis(out_h, h)
+    wx = wx[None, :, None]
+    top = img[ya][:, xa] * (1 - wx) + img[ya][:, xb] * wx
+    bot = img[yb][:, xa] * (1 - wx) + img[yb][:, xb] * wx
+    return top * (1 - wy[:, None, None]) + bot * wy[:, None, None]
+
+
+def low_res_ablation(scene, frame_plain_colour, covered, args, level=2):
+    """The method without the network: plain shading from box-filtered copies of the maps at the
+    latent grid's resolution (level `level` of the packager-filter chain the library builds: 1024^2
+    -> 256^2).  Reports the shading kernel's time with those maps, the frame PSNR against the
+    full-resolution plain frame, and the texel-centre PSNR per material over the decoder's 7
+    channels (the decoder's own metric)."""
+    import ctypes as C
+
+    import numpy as np
+    import rosy_amd
+    from rosy_amd import abi
+    s = scene.s
+    r = rosy_amd.Renderer(device=0)
+    lib = rosy_amd.lib()
+    try:
+        r.load(scene)
+        n_tex = s.n_textures
+        descs = (abi.TextureDesc * n_tex)()
+        for t in range(n_tex):
+            d = s.textures[t]
+            descs[t] = abi.TextureDesc(d.width, d.height, d.format, abi.MIPS_BUILD, d.pixels)
+        assert lib.rsy_set_materials(r.ctx, s.materials, s.n_materials, descs, n_tex, s.samplers, s.n_samplers) == 0
+        low, keep = [], []
+        for t in range(n_tex):
+            d = s.textures[t]
+            ch = 2 if d.format in (abi.TEX_RG8_UNORM, abi.TEX_RG8_SRGB) else 4
+            w, h = max(1, d.width >> level), max(1, d.height >> level)
+            buf = np.empty(w * h * ch, np.uint8)
+            assert lib.rsy_read_texture(r.ctx, t, level, buf.ctypes.data, buf.size) == 0
+            low.append(buf.reshape(h, w, ch))
+            keep.append(buf)
+            descs[t] = abi.TextureDesc(w, h, d.format, 1, buf.ctypes.data)
+        assert lib.rsy_set_materials(r.ctx, s.materials, s.n_materials, descs, n_tex, s.samplers, s.n_samplers) == 0
+        plain = scene.frame_desc(abi.SHADE_PLAIN)
+        st = stage_medians(r, plain)
+        col = r.color(0)
+        r.set_debug_flags(0)
+        r.render_graph(plain, args.warmup)
+        r.render_graph(plain, args.steps)
+        frame_ms = float(r.stats().gpu_ms)
+    finally:
+        r.close()
+    diff = col[covered][:, :3].astype(np.float64) - frame_plain_colour[covered][:, :3]
+    frame_psnr = float(10 * np.log10(1.0 / np.mean(diff ** 2)))
+    # texel-centre PSNR per material, the decoder's metric (rsy_neural_evaluate)
+    lut = _srgb_lut()
+    psnr = []
+    for m in range(s.n_materials):
+        md = s.materials[m]
+        ia, inn, imr = md.color_sampled_image_index, md.normal_sampled_image_index, md.metallic_sampled_image_index
+        d = s.textures[ia]
+        full = [np.ctypeslib.as_array(C.cast(s.textures[i].pixels, C.POINTER(C.c_uint8)),
+                                      (s.textures[i].height, s.textures[i].width,
+                                       2 if s.textures[i].format in (abi.TEX_RG8_UNORM, abi.TEX_RG8_SRGB) else 4))
+                for i in (ia, inn, imr)]
+        want = _map_channels(*full, lut)
+        got = _bilinear_repeat(_map_channels(low[ia], low[inn], low[imr], lut), d.width, d.height)
+        psnr.append(float(10 * np.log10(1.0 / np.mean((got - want) ** 2))))
+    psnr = np.array(psnr)
+    return {"maps": f"{s.textures[0].width >> level}^2 box-filtered copies (the packager filter, level {level})",
+            "shade_ms": round(float(st[3]), 5), "plain_ms_per_frame": round(frame_ms, 5),
+            "frame_psnr_vs_full_res_plain_db": round(frame_psnr, 2),
+            "texel_psnr_min": round(float(psnr.min()), 2), "texel_psnr_median": round(float(np.median(psnr)), 2),
+            "materials_at_40db": int((psnr >= 40.0).sum())}
+
+
+def decoder_study(args, detail, train_steps):
+    """One decoder workload: the decoder scene with the default content (detail 0) or with fine detail
+    (detail 1: content above a 256^2 grid's Nyquist rate).  Trains on the GPU, gates at 40 dB
+    (materials below keep their plain maps), times both frame forms from graph replays, and compares
+    with the low-resolution-map ablation."""
+    import numpy as np
+    import rosy_amd
+    from rosy_amd import abi
+    scene = rosy_amd.Scene(3, seed=args.seed, texture_detail=detail)
+    r = rosy_amd.Renderer(device=0)
+    try:
+        r.load(scene)
+        plain = scene.frame_desc(abi.SHADE_PLAIN)
+        neural = scene.frame_desc(abi.SHADE_NEURAL)
+        r.neural_init(decay_steps=train_steps)
+        psnr0 = r.neural_evaluate()
+        r.neural_train(train_steps)
+        psnr = r.neural_evaluate()
+        nst = r.neural_stats()
+        st_plain = stage_medians(r, plain)
+        col_plain = r.color(0)
+        covered = r.visibility(0) != 0
+        st_neural = stage_medians(r, neural)
+        col_neural = r.color(0)
+        r.set_debug_flags(0)
+        ms = {}
+        for name, f in (("plain", plain), ("neural", neural)):
+            r.render_graph(f, args.warmup)
+            r.render_graph(f, args.steps)
+            ms[name] = float(r.stats().gpu_ms)
+        diff = col_neural[covered][:, :3].astype(np.float64) - col_plain[covered][:, :3]
+        out = {"content": "fine detail (up to 512 cycles per map, stripes up to 200 cycles)" if detail else
+               "default (up to 32 cycles per map, stripes of 3-8 cycles)",
+               "train_steps": train_steps, "train_ms": round(float(nst.train_ms), 1),
+               "psnr_untrained_min": round(float(psnr0.min()), 2),
+               "psnr_untrained_median": round(float(np.median(psnr0)), 2),
+               "psnr_min": round(float(psnr.min()), 2), "psnr_median": round(float(np.median(psnr)), 2),
+               "psnr_gain_median_db": round(float(np.median(psnr - psnr0)), 2),
+               "materials_decoded": int(nst.n_active), "materials_fallback": int(nst.n_materials - nst.n_active),
+               "neural_ms_per_frame": round(ms["neural"], 5), "plain_ms_per_frame": round(ms["plain"], 5),
+               "shade_ms": {"plain": round(float(st_plain[3]), 5), "neural": round(float(st_neural[3]), 5)},
+               "frame_psnr_neural_vs_plain_db": round(float(10 * np.log10(1.0 / np.mean(diff ** 2))), 2)}
+    finally:
+        r.close()
+    out["ablation_low_res_maps"] = low_res_ablation(scene, col_plain, covered, args)
+    scene.close()
+    return out
+
+
 def rsy_decode_timing(scene, seconds=2.0):
     """The workload written as a .rsy file (rosy_amd.asset.from_scene) and decoded by rsy_asset_decode
     from host memory, repeated for a bounded time: MB of file and host GB/s (one thread)."""
@@ -348,22 +570,17 @@ def main():
     # the texture pipeline: BC7 upload with full chains, and the plain frame with mip chains
     texture_pipeline = texture_timing(args)
     log(f"texture pipeline {texture_pipeline}")
+    # what the network earns: the low-resolution-map ablation on this workload, and the same study on
+    # content with detail above the latent grid's Nyquist rate
+    ablation = low_res_ablation(scene, col_plain, vis != 0, args)
+    log(f"low-resolution maps ablation {ablation}")
+    fine = decoder_study(args, 1, args.train_steps)
+    log(f"fine-detail decoder workload {fine}")
 
-    # raster kernel, algorithmic bytes with the implementation-independent unit costs: one tile-list
-    # entry read (4 B) per (triangle, bin) pair, one setup record read (32 B: three edge pairs, the
-    # 64-bit constant terms packed, the depth plane) per triangle set up, one visibility word (8 B)
-    # per camera pixel and one depth (4 B) per shadow texel
     cam = scene.camera_views()[0]
     cas = scene.cascades()[0]
-    raster_bytes = (int(stp.tile_pairs) * 4 + int(stp.triangles_setup) * 32 + cam.width * cam.height * 8 +
-                    cas.size * cas.size * 4)
     raster_ms = float(st_neural[2])
-    achieved = raster_bytes / (raster_ms * 1e-3) / 1e9
-    roofline = {"kernel": "k_raster (tile raster, camera + shadow views in one launch)", "bound": "hbm",
-                "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(tile),
-                "algorithmic_bytes": raster_bytes, "avg_launch_ms": round(raster_ms, 5),
-                "measured_copy_gbs": copy_ceiling_gbs()}
+    copy_gbs = copy_ceiling_gbs()
 
     # decoder MFMA work: decoded pixels x 1088 FLOPs over the neural shading kernel
     active = np.zeros(scene.s.n_materials + 1, bool)
@@ -387,6 +604,7 @@ def main():
     cpu = None
     err_plain = err_neural = None
     bit_exact = None
+    roofline = kernels = pcf_ties = None
     if not args.no_cpu:
         import oracle
         threads = oracle.threads()
@@ -405,16 +623,40 @@ def main():
                "sample": f"{n} whole plain frames of the same workload (shadow raster + camera raster + FP64 "
                          f"shading)"}
         bit_exact = bool(np.array_equal(ov, vis) and np.array_equal(sm.view(np.uint32), shadow.view(np.uint32)))
-        ref, _, _ = oracle.shade_view(scene, cam, [cas], [sm], ov, override_shadow=sf_plain, n_threads=threads)
-        err_plain = float(np.abs(col_plain.astype(np.float64) - ref).max())
-        lat, mlp = r.neural_params()
-        q = mlp.astype(np.float16).astype(np.float32)
-        q[576:583] = mlp[576:583]
-        params = oracle.NeuralParams(np.arange(scene.s.n_materials), lat.astype(np.float16).astype(np.float32), q,
-                                     r_active.astype(np.uint32))
-        ref, _, _ = oracle.shade_view(scene, cam, [cas], [sm], ov, override_shadow=sf_neural, n_threads=threads,
-                                      neural=params)
-        err_neural = float(np.abs(col_neural.astype(np.float64) - ref).max())
+        lat, _ = r.neural_params()
+        params = oracle.NeuralParams(np.arange(scene.s.n_materials), lat.astype(np.float16).astype(np.float32),
+                                     r.neural_inference_mlp(), r_active.astype(np.uint32))
+        err_plain, ties_plain, away_plain = colour_error(col_plain, sf_plain, lambda ov_sf: oracle.shade_view(
+            scene, cam, [cas], [sm], ov, override_shadow=ov_sf, n_threads=threads))
+        err_neural, ties_neural, away_neural = colour_error(col_neural, sf_neural, lambda ov_sf: oracle.shade_view(
+            scene, cam, [cas], [sm], ov, override_shadow=ov_sf, n_threads=threads, neural=params))
+        pcf_ties = {"plain": ties_plain, "neural": ties_neural,
+                    "differing_away_from_a_tie": away_plain + away_neural}
+        # algorithmic bytes of the shading kernels (the oracle's footprint of these frames) and of the
+        # geometry, binning and raster kernels
+        P = cam.width * cam.height
+        fp_plain = oracle.shade_footprint(scene, cam, [cas], [sm], ov, n_threads=threads)
+        fp_neural = oracle.shade_footprint(scene, cam, [cas], [sm], ov, neural=params, n_threads=threads)
+        b_plain, parts_plain = shade_bytes(fp_plain, P)
+        b_neural, parts_neural = shade_bytes(fp_neural, P)
+        roofline = hbm_line("k_shade<neural> (deferred Cook-Torrance + PCF shading, decoder for active materials)",
+                            b_neural, float(st_neural[3]),
+                            {"traffic": measured_traffic("k_shade<neural>", tile), "bytes": parts_neural,
+                             "footprint": fp_neural, "measured_copy_gbs": copy_gbs})
+        views = [(cas.light_view_projection[:], cas.size, cas.size, True),
+                 (cam.view_projection[:], cam.width, cam.height, False)]
+        b_raster = raster_bytes(scene, views, rs, tile, int(stp.tile_pairs), int(stp.triangles_setup))
+        b_geo = geo_bytes(scene, [(w, h, sh) for _, w, h, sh in views], int(stp.tile_pairs))
+        stage_ms_sum = float(st_neural[0] + st_neural[1] + st_neural[2])
+        kernels = {
+            "k_shade_plain": hbm_line("k_shade<plain>", b_plain, float(st_plain[3]),
+                                      {"traffic": measured_traffic("k_shade<plain>", tile), "bytes": parts_plain,
+                                       "footprint": fp_plain}),
+            "k_raster": hbm_line("k_raster (camera + shadow view in one launch)", b_raster, raster_ms,
+                                 {"traffic": measured_traffic("k_raster", tile)}),
+            "geometry_binning_raster": hbm_line("transform, setup, clip, bin scan, scatter and raster kernels",
+                                                b_geo, stage_ms_sum),
+        }
     frame_psnr = float(10 * np.log10(1.0 / np.mean((col_neural[covered][:, :3].astype(np.float64) -
                                                     col_plain[covered][:, :3]) ** 2)))
     out = {
@@ -438,6 +680,7 @@ def main():
         "shade_ms": {"plain": round(float(st_plain[3]), 5), "neural": round(float(st_neural[3]), 5)},
         "shading_speedup": round(float(st_plain[3]) / float(st_neural[3]), 3),
         "roofline": roofline,
+        "kernels": kernels,
         "decoder": decoder,
         "output_stage": output_stage,
         "msaa4": msaa,
@@ -450,7 +693,10 @@ def main():
                      "raster": round(raster_ms, 5), "shade": round(float(st_neural[3]), 5)},
         "tile_pairs": int(stp.tile_pairs),
         "max_abs_err_vs_fp64": {"plain": err_plain, "neural": err_neural},
+        "pcf_compare_ties": pcf_ties,
         "frame_psnr_neural_vs_plain_db": round(frame_psnr, 2),
+        "ablation_low_res_maps": ablation,
+        "decoder_fine_detail": fine,
         "bit_exact_vs_oracle": bit_exact,
     }
     print(json.dumps(out), flush=True)

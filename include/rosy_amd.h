@@ -634,6 +634,9 @@ typedef struct rsy_scene_desc {
     uint32_t texture_size;
     uint32_t n_translucent;   /* materials [0, n) translucent (default 0): even BLEND, odd MASK, with a
                                  sin(8 pi u) sin(8 pi v) albedo alpha; their surfaces go to the blended list */
+    uint32_t texture_detail;  /* 1: each material's maps carry detail at one of five scales up to 512
+                                 cycles across the map, stripes up to 200 cycles and small blobs (content
+                                 above a 256^2 latent grid's Nyquist rate); 0 (default): up to 32 cycles */
 } rsy_scene_desc;
 
 typedef struct rsy_scene {

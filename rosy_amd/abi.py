@@ -138,7 +138,7 @@ class SceneDesc(C.Structure):
     _fields_ = [("workload", C.c_uint32), ("texture_mips", C.c_uint32), ("seed", C.c_uint64), ("width", C.c_uint32),
                 ("height", C.c_uint32), ("shadow_size", C.c_uint32), ("total_triangles", C.c_uint32),
                 ("n_meshes", C.c_uint32), ("n_materials", C.c_uint32), ("texture_size", C.c_uint32),
-                ("n_translucent", C.c_uint32)]
+                ("n_translucent", C.c_uint32), ("texture_detail", C.c_uint32)]
 
 
 class Scene(C.Structure):

@@ -13,12 +13,14 @@
 // addressed like a REPEAT texture of L x L texels (cell centres at (i + 0.5) / L); ax, ay are the
 // bilinear weights, so tri() is continuous across cell borders.
 //
-// Inference runs the MLP for the 64 pixels of a wave on the matrix cores:
-// v_mfma_f32_32x32x8_f16 with FP16 operands and FP32 accumulation, hidden units as the M rows of
-// layer 1 and the pixels as its N columns, so the layer-1 accumulator layout is exactly the B
-// operand layout layer 2 needs (no shuffles between the layers).  Weights and latents are stored
-// in FP16; activations enter the MFMA as FP16 hi/lo pairs, so the decoder is accurate to FP32
-// level for its FP16 parameters.
+// Inference runs the MLP for the 64 pixels of a wave on the matrix cores: gfx950's
+// v_mfma_f32_32x32x16_f16 with FP16 operands and FP32 accumulation, hidden units as the M rows of
+// layer 1 and the pixels as its N columns.  Layer 2 takes the layer-1 accumulators as its B operand
+// in the order a lane holds them (its weights are stored in that K order), so nothing moves between
+// the layers.  One K=16 step covers layer 1's 16 feature columns and two cover layer 2's 32 hidden
+// units.  Weights are stored in FP16, or as OCP FP8 E4M3 codes widened to FP16 exactly when loaded;
+// latents in FP16; activations enter the MFMA as FP16 hi/lo pairs, so the decoder is accurate to
+// FP32 level for its stored parameters.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -30,22 +32,34 @@
 
 namespace rsy {
 
-typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// MFMA operands of the MLP, one 8-byte entry per lane.  Layer 1 multiplies W1' = [W1 | b1 | 0]
-// (32 x 16) by the feature columns; entry [s][l] holds W1'[l % 32][8s + 4(l / 32) + 0..3].
-// Layer 2 multiplies W2' (32 x 32, rows 7..31 zero) by the hidden columns; entry [s][l] holds
-// W2'[l % 32][8s + 4(l / 32) + 0..3].  With FP8 weights the entries hold the E4M3 codes' values
-// (exact in FP16) and scale[] the per-layer factor that turns them back into weights.
+// MFMA operands of the MLP, one entry of 8 K-values per lane.
+// Layer 1 multiplies W1' = [W1 | b1 | 0] (32 x 16: the 10 inputs, the bias column, 5 zero columns)
+// by the feature columns: w1[l] holds W1'[l % 32][8 (l / 32) + j], j = 0..7.
+// Layer 2 multiplies W2 (rows 7..31 zero) by the hidden columns in two K-steps, K ordered as lane l
+// holds its layer-1 accumulator: w2[s][l] holds W2[l % 32][nm_hidden(s, l / 32, j)].
+// q1 / q2 hold the same operands as E4M3 codes, one byte each, for FP8 weights (fp8 = 1).
+// scale[] turns the accumulators into the weights' units: 1 for FP16 weights; for FP8 the per-layer
+// scale times 2^8, the factor between an E4M3 code's value and its exact FP16 widening.
 struct NmWeights {
-    h4 w1[2][64];
-    h4 w2[4][64];
+    h8 w1[64];
+    h8 w2[2][64];
+    uint2 q1[64];
+    uint2 q2[2][64];
     float b2[8];
-    float scale[2];  // layer-1 / layer-2 accumulator scale: 1 for FP16 weights, the FP8 scale otherwise
-    float pad[2];
+    float scale[2];
+    uint32_t fp8;
+    uint32_t pad;
 };
 static_assert(sizeof(NmWeights) == kNmWeightsBytes, "NmWeights layout");
+
+// Hidden unit at layer-2 K-step s, lane half hb, element j: the layer-1 accumulator entry 8s + j of
+// a lane of half hb holds hidden row 8 ((8s + j) / 4) + 4 hb + (8s + j) % 4.
+__host__ __device__ constexpr uint32_t nm_hidden(uint32_t s, uint32_t hb, uint32_t j) {
+    return 16u * s + 8u * (j / 4u) + 4u * hb + j % 4u;
+}
 
 __device__ inline uint32_t nm_lane() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
@@ -101,20 +115,37 @@ __device__ inline void nm_features(const uint4* grid, uint32_t L, float u, float
     f[10] = 1.f;
 }
 
-// FP32 value as an unevaluated sum of two halves: hi = fp16(x), lo = fp16(x - hi) (|error| about
+// FP32 values as unevaluated sums of two halves: hi = fp16(x), lo = fp16(x - hi) (|error| about
 // 2^-22 |x|).  Products with FP16 weights are exact in the MFMA's FP32 accumulation, so feeding hi
 // and lo as two B operands gives W x to FP32-level accuracy for the FP16 weights.
-__device__ inline void nm_split4(float a, float b, float c, float d, h4& hi, h4& lo) {
-    hi = h4{_Float16(a), _Float16(b), _Float16(c), _Float16(d)};
-    lo = h4{_Float16(a - float(hi[0])), _Float16(b - float(hi[1])), _Float16(c - float(hi[2])), _Float16(d - float(hi[3]))};
+__device__ inline void nm_split8(const float (&x)[8], h8& hi, h8& lo) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        hi[k] = _Float16(x[k]);
+        lo[k] = _Float16(x[k] - float(hi[k]));
+    }
 }
 
-__device__ inline h4 nm_swap_halves(h4 v) {  // value of lane l ^ 32
+// Eight E4M3 codes widened to FP16: sign to bit 15, exponent and mantissa to bits 13..7.  Exact for
+// every code (normal and subnormal); the FP16 value is the code's value times 2^-8.
+__device__ inline h8 nm_widen_e4m3(uint2 q) {
+    h8 r;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t b = ((k < 4 ? q.x : q.y) >> (8 * (k & 3))) & 0xffu;
+        r[k] = __builtin_bit_cast(_Float16, uint16_t(((b & 0x80u) << 8) | ((b & 0x7fu) << 7)));
+    }
+    return r;
+}
+
+__device__ inline h8 nm_swap_halves(h8 v) {  // value of lane l ^ 32
     const int addr = int((nm_lane() ^ 32u) << 2);
-    uint2 b = __builtin_bit_cast(uint2, v);
+    uint4 b = __builtin_bit_cast(uint4, v);
     b.x = uint32_t(__builtin_amdgcn_ds_bpermute(addr, int(b.x)));
     b.y = uint32_t(__builtin_amdgcn_ds_bpermute(addr, int(b.y)));
-    return __builtin_bit_cast(h4, b);
+    b.z = uint32_t(__builtin_amdgcn_ds_bpermute(addr, int(b.z)));
+    b.w = uint32_t(__builtin_amdgcn_ds_bpermute(addr, int(b.w)));
+    return __builtin_bit_cast(h8, b);
 }
 
 __device__ inline float nm_swap_f(float v) {
@@ -125,54 +156,54 @@ __device__ inline float nm_swap_f(float v) {
 // MLP of the 64 pixels of a wave, one pixel per lane (features f from nm_features); every lane of
 // the wave must call it with the wave fully active.  Returns this lane's 7 clamped outputs.
 //
-// Pixel tile t (t = 0, 1) is lanes 32t .. 32t + 31.  The B operand of layer 1 for tile t, K-step s,
-// lane l is features 8s + 4(l / 32) .. +3 of pixel 32t + l % 32: a lane holds its own pixel's
-// features and swaps the other half with lane l ^ 32.  The layer-1 accumulator of lane l holds
-// hidden rows 8(r / 4) + 4(l / 32) + r % 4 (r = 0..15) of pixel l % 32, which after ReLU is layer
-// 2's B operand for K-step r / 4.  Output rows 0..7 land in accumulator entries 0..3: rows 0-3 in
+// Pixel tile t (t = 0, 1) is lanes 32t .. 32t + 31.  The B operand of layer 1 for tile t, lane l is
+// features 8 (l / 32) .. +7 of pixel 32t + l % 32: a lane holds its own pixel's features and swaps
+// the other half with lane l ^ 32.  The layer-1 accumulator of lane l holds hidden rows
+// 8 (r / 4) + 4 (l / 32) + r % 4 (r = 0..15) of pixel l % 32; after ReLU, entries 8s .. 8s + 7 are
+// layer 2's B operand for K-step s.  Output rows 0..7 land in accumulator entries 0..3: rows 0-3 in
 // lanes 0-31 and rows 4-7 in lanes 32-63 of the same column.  Every B operand goes in as an FP16
-// hi / lo pair (nm_split4).
+// hi / lo pair (nm_split8).
 __device__ inline void nm_mlp_wave(const NmWeights* __restrict__ W, const float f[kNmFeat], float out[kNmOut]) {
     const uint32_t lane = nm_lane();
     const bool hi = lane >= 32u;
-    // feature quads: q0 = 0..3, q1 = 4..7, q2 = 8..11 (8, 9, 1, 0), q3 = 12..15 (zero)
-    h4 qh[3], ql[3];
-    nm_split4(f[0], f[1], f[2], f[3], qh[0], ql[0]);
-    nm_split4(f[4], f[5], f[6], f[7], qh[1], ql[1]);
-    nm_split4(f[8], f[9], f[10], 0.f, qh[2], ql[2]);
-    const h4 zero = h4{_Float16(0.f), _Float16(0.f), _Float16(0.f), _Float16(0.f)};
-    // K-step s needs quad 2s + (l / 32) of the tile's pixel
-    h4 mine_h[2], mine_l[2], other_h[2], other_l[2];
-    mine_h[0] = hi ? qh[1] : qh[0];
-    mine_l[0] = hi ? ql[1] : ql[0];
-    mine_h[1] = hi ? zero : qh[2];
-    mine_l[1] = hi ? zero : ql[2];
-    other_h[0] = nm_swap_halves(hi ? qh[0] : qh[1]);
-    other_l[0] = nm_swap_halves(hi ? ql[0] : ql[1]);
-    other_h[1] = nm_swap_halves(hi ? qh[2] : zero);
-    other_l[1] = nm_swap_halves(hi ? ql[2] : zero);
-    const h4 a10 = W->w1[0][lane], a11 = W->w1[1][lane];
+    h8 a1, a2[2];
+    if (W->fp8) {  // uniform: one weight form per decoder
+        a1 = nm_widen_e4m3(W->q1[lane]);
+        a2[0] = nm_widen_e4m3(W->q2[0][lane]);
+        a2[1] = nm_widen_e4m3(W->q2[1][lane]);
+    } else {
+        a1 = W->w1[lane];
+        a2[0] = W->w2[0][lane];
+        a2[1] = W->w2[1][lane];
+    }
+    // feature halves: 0..7 the latent channels; 8..15 the positional pair, the constant 1, zeros
+    const float g0[8] = {f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7]};
+    const float g1[8] = {f[8], f[9], f[10], 0.f, 0.f, 0.f, 0.f, 0.f};
+    h8 q0h, q0l, q1h, q1l;
+    nm_split8(g0, q0h, q0l);
+    nm_split8(g1, q1h, q1l);
+    // tile t wants half (l / 32) of pixel 32t + l % 32: the lane's own features for t == l / 32,
+    // otherwise those of lane l ^ 32 (which sends the half this lane needs)
+    const h8 own_h = hi ? q1h : q0h, own_l = hi ? q1l : q0l;
+    const h8 far_h = nm_swap_halves(hi ? q0h : q1h), far_l = nm_swap_halves(hi ? q0l : q1l);
     const float s1 = W->scale[0], s2 = W->scale[1];
-    h4 a2[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) a2[s] = W->w2[s][lane];
     float res[2][4];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
         const bool own = (t == 1) == hi;
         f32x16 acc = {};
-        acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a10, own ? mine_h[0] : other_h[0], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a10, own ? mine_l[0] : other_l[0], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a11, own ? mine_h[1] : other_h[1], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a11, own ? mine_l[1] : other_l[1], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, own ? own_h : far_h, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, own ? own_l : far_l, acc, 0, 0, 0);
         f32x16 acc2 = {};
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            h4 hh, hl;
-            nm_split4(fmaxf(acc[4 * s + 0] * s1, 0.f), fmaxf(acc[4 * s + 1] * s1, 0.f),
-                      fmaxf(acc[4 * s + 2] * s1, 0.f), fmaxf(acc[4 * s + 3] * s1, 0.f), hh, hl);
-            acc2 = __builtin_amdgcn_mfma_f32_32x32x8f16(a2[s], hh, acc2, 0, 0, 0);
-            acc2 = __builtin_amdgcn_mfma_f32_32x32x8f16(a2[s], hl, acc2, 0, 0, 0);
+        for (int s = 0; s < 2; ++s) {
+            float h[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) h[j] = fmaxf(acc[8 * s + j] * s1, 0.f);
+            h8 hh, hl;
+            nm_split8(h, hh, hl);
+            acc2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a2[s], hh, acc2, 0, 0, 0);
+            acc2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a2[s], hl, acc2, 0, 0, 0);
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) res[t][r] = acc2[r];

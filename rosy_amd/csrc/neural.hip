@@ -362,20 +362,16 @@ __global__ __launch_bounds__(256) void k_nm_points(NmPointLaunch P) {
 
 // MFMA-operand form of the MLP (see NmWeights): one wave, lane l writes its own operand entries.
 // FP16: every weight rounded to FP16, scales 1.  FP8: each layer's weights (layer 1 with b1 folded
-// in) divided by max|w| / 448 and rounded to OCP E4M3 (round to nearest even); the codes' values go
-// into the FP16 operands exactly and the scale multiplies the layer's accumulator.
+// in) divided by max|w| / 448 and rounded to OCP E4M3 (round to nearest even, the hardware
+// conversion), stored as codes; inference widens them to FP16 exactly and the scale multiplies the
+// layer's accumulator.
 __device__ inline float layer_weight(const float* p, int layer, uint32_t i, uint32_t k) {
     if (layer == 0) return k < uint32_t(kNmIn) ? p[kNmW1 + i * kNmIn + k] : (k == uint32_t(kNmIn) ? p[kNmB1 + i] : 0.f);
     return i < uint32_t(kNmOut) ? p[kNmW2 + i * kNmHidden + k] : 0.f;
 }
 
-__device__ inline float fp8_round(float x) {  // x / scale already within the E4M3 range
-    const int packed = __builtin_amdgcn_cvt_pk_fp8_f32(x, 0.f, 0, false);
-    return __builtin_amdgcn_cvt_f32_fp8(packed, 0);
-}
-
 __global__ __launch_bounds__(64) void k_nm_export_weights(const float* p, NmWeights* W, uint32_t fp8) {
-    const uint32_t l = threadIdx.x, i = l % 32, kq = 4 * (l / 32);
+    const uint32_t l = threadIdx.x, i = l % 32, hb = l / 32;
     float scale[2] = {1.f, 1.f};
     if (fp8) {
         for (int layer = 0; layer < 2; ++layer) {
@@ -386,40 +382,49 @@ __global__ __launch_bounds__(64) void k_nm_export_weights(const float* p, NmWeig
             scale[layer] = m > 0.f ? m / 448.f : 1.f;
         }
     }
-    for (int s = 0; s < 2; ++s) {
-        float w[4];
-        for (int e = 0; e < 4; ++e) {
-            const float v = layer_weight(p, 0, i, 8 * s + kq + e);
-            w[e] = fp8 ? fp8_round(v / scale[0]) : v;
+    // the 8 K-values of this lane's entry: layer-1 columns 8 hb + j; layer-2 hidden units nm_hidden(s, hb, j)
+    auto put = [&](int layer, uint32_t s, h8& w, uint2& q) {
+        uint32_t code[2] = {0u, 0u};
+        for (uint32_t j = 0; j < 8; ++j) {
+            const uint32_t k = layer == 0 ? 8 * hb + j : nm_hidden(s, hb, j);
+            const float v = layer_weight(p, layer, i, k);
+            if (fp8) {
+                const uint32_t c = uint32_t(__builtin_amdgcn_cvt_pk_fp8_f32(v / scale[layer], 0.f, 0, false)) & 0xffu;
+                code[j / 4] |= c << (8 * (j % 4));
+                w[j] = _Float16(0.f);
+            } else {
+                w[j] = _Float16(v);
+            }
         }
-        W->w1[s][l] = h4{_Float16(w[0]), _Float16(w[1]), _Float16(w[2]), _Float16(w[3])};
-    }
-    for (int s = 0; s < 4; ++s) {
-        float w[4];
-        for (int e = 0; e < 4; ++e) {
-            const float v = layer_weight(p, 1, i, 8 * s + kq + e);
-            w[e] = fp8 ? fp8_round(v / scale[1]) : v;
-        }
-        W->w2[s][l] = h4{_Float16(w[0]), _Float16(w[1]), _Float16(w[2]), _Float16(w[3])};
-    }
+        q = make_uint2(code[0], code[1]);
+    };
+    put(0, 0, W->w1[l], W->q1[l]);
+    put(1, 0, W->w2[0][l], W->q2[0][l]);
+    put(1, 1, W->w2[1][l], W->q2[1][l]);
     if (l < 8) W->b2[l] = l < uint32_t(kNmOut) ? p[kNmB2 + l] : 0.f;
-    if (l < 2) W->scale[l] = scale[l];
-    if (l < 2) W->pad[l] = 0.f;
+    // FP8: the widened codes carry a factor 2^-8 (nm_widen_e4m3)
+    if (l < 2) W->scale[l] = fp8 ? scale[l] * 256.f : 1.f;
+    if (l == 0) {
+        W->fp8 = fp8 ? 1u : 0u;
+        W->pad = 0u;
+    }
 }
 
 // The inference MLP back in the parameter layout (FP32 values: operand value times layer scale).
 __global__ __launch_bounds__(64) void k_nm_unpack_weights(const NmWeights* W, float* p) {
-    const uint32_t l = threadIdx.x, i = l % 32, kq = 4 * (l / 32);
-    for (int s = 0; s < 2; ++s)
-        for (int e = 0; e < 4; ++e) {
-            const uint32_t k = 8 * s + kq + e;
-            const float v = float(W->w1[s][l][e]) * W->scale[0];
-            if (k < uint32_t(kNmIn)) p[kNmW1 + i * kNmIn + k] = v;
-            else if (k == uint32_t(kNmIn)) p[kNmB1 + i] = v;
-        }
-    for (int s = 0; s < 4; ++s)
-        for (int e = 0; e < 4; ++e)
-            if (i < uint32_t(kNmOut)) p[kNmW2 + i * kNmHidden + 8 * s + kq + e] = float(W->w2[s][l][e]) * W->scale[1];
+    const uint32_t l = threadIdx.x, i = l % 32, hb = l / 32;
+    const bool fp8 = W->fp8 != 0;
+    const h8 a1 = fp8 ? nm_widen_e4m3(W->q1[l]) : W->w1[l];
+    const h8 a2[2] = {fp8 ? nm_widen_e4m3(W->q2[0][l]) : W->w2[0][l], fp8 ? nm_widen_e4m3(W->q2[1][l]) : W->w2[1][l]};
+    for (uint32_t j = 0; j < 8; ++j) {
+        const uint32_t k = 8 * hb + j;
+        const float v = float(a1[j]) * W->scale[0];
+        if (k < uint32_t(kNmIn)) p[kNmW1 + i * kNmIn + k] = v;
+        else if (k == uint32_t(kNmIn)) p[kNmB1 + i] = v;
+    }
+    for (uint32_t s = 0; s < 2; ++s)
+        for (uint32_t j = 0; j < 8; ++j)
+            if (i < uint32_t(kNmOut)) p[kNmW2 + i * kNmHidden + nm_hidden(s, hb, j)] = float(a2[s][j]) * W->scale[1];
     if (l < uint32_t(kNmOut)) p[kNmB2 + l] = W->b2[l];
 }
 

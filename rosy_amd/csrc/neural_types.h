@@ -17,9 +17,10 @@ constexpr int kNmB2 = kNmW2 + kNmOut * kNmHidden;  // 576
 constexpr int kNmParams = kNmB2 + kNmOut;          // 583
 constexpr int kNmParamsPad = 584;                  // index 583 carries a loss sum where gradients are reduced
 
-// Bytes of the MFMA-operand form of the MLP (FP16 A operands of both layers, FP32 b2, two FP32
-// layer scales and padding).
-constexpr size_t kNmWeightsBytes = 2 * 64 * 8 + 4 * 64 * 8 + 8 * 4 + 4 * 4;
+// Bytes of the MFMA-operand form of the MLP: the FP16 A operands of both layers (3 x 64 entries of
+// 16 bytes), the same as E4M3 codes (3 x 64 entries of 8 bytes), FP32 b2, two FP32 layer scales,
+// the weight-form flag and padding.
+constexpr size_t kNmWeightsBytes = 3 * 64 * 16 + 3 * 64 * 8 + 8 * 4 + 2 * 4 + 2 * 4;
 
 // One decoded material: the material index and its three source maps (texture table indices).
 struct NmSlot {

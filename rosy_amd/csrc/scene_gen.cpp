@@ -315,17 +315,24 @@ struct Blob {
 
 // `cutout`: the albedo alpha is 0.5 + 0.5 sin(8 pi u) sin(8 pi v) (translucent materials: holes
 // under an alpha mask, a varying alpha under blending); otherwise opaque (255).
-void make_material_textures(MaterialTextures& out, int size, Rng rng, bool stripes_and_blobs, bool cutout) {
+// `fine`: detail per material at one of five scales, the coarsest octave at size/128 << k cycles
+// (k = 0..4: up to 512 cycles in the finest octave of a 1024^2 map), stripes of 3 to 200 cycles
+// (log-uniform) and blobs of radius 0.01-0.08: much of it above what a 256^2 grid holds (128 cycles).
+void make_material_textures(MaterialTextures& out, int size, Rng rng, bool stripes_and_blobs, bool cutout,
+                            bool fine = false) {
     const double hue = rng.uniform(), sat = rng.range(0.2, 0.8);
     const bool metal = rng.uniform() < 0.35;
-    const int base = std::max(2, size / 128);
+    const int base = std::max(2, size / 128) << (fine ? int(rng.below(5)) : 0);
     Fbm albedo_n(base, 3, rng), tint_n(base, 2, rng), height_n(base * 2, 2, rng), metal_n(base, 2, rng),
         rough_n(base, 3, rng);
-    const double stripe_angle = rng.range(0, kPi), stripe_freq = double(int(rng.range(3, 9)));
+    const double stripe_angle = rng.range(0, kPi);
+    const double stripe_freq = fine ? std::round(std::exp(rng.range(std::log(3.0), std::log(200.0))))
+                                    : double(int(rng.range(3, 9)));
     std::vector<Blob> blobs;
     if (stripes_and_blobs) {
         const int nb = 6 + int(rng.below(6));
-        for (int i = 0; i < nb; ++i) blobs.push_back({rng.uniform(), rng.uniform(), rng.range(0.04, 0.12), rng.range(-0.5, 0.5)});
+        const double r0 = fine ? 0.01 : 0.04, r1 = fine ? 0.08 : 0.12;
+        for (int i = 0; i < nb; ++i) blobs.push_back({rng.uniform(), rng.uniform(), rng.range(r0, r1), rng.range(-0.5, 0.5)});
     }
     // base colour from hue/saturation
     auto hsv = [](double h, double s, double v, double rgb[3]) {
@@ -682,7 +689,8 @@ rsy_result rsy_scene_generate(const rsy_scene_desc* desc, rsy_scene** out_scene)
         s->v_samplers.push_back(rsy_sampler{4, 4, 2, 2});  // LINEAR / REPEAT
         for (uint32_t m = 0; m < nm; ++m) {
             const bool translucent = m < desc->n_translucent;
-            make_material_textures(s->tex_store[m], int(wl.tex_size), tex_rng.fork(m + 1), wl.stripes, translucent);
+            make_material_textures(s->tex_store[m], int(wl.tex_size), tex_rng.fork(m + 1), wl.stripes, translucent,
+                                   desc->texture_detail != 0);
             const uint32_t base = uint32_t(s->v_textures.size());
             const uint32_t mips = desc->texture_mips ? RSY_MIPS_BUILD : 0u;
             s->v_textures.push_back({wl.tex_size, wl.tex_size, RSY_TEX_RGBA8_SRGB, mips, s->tex_store[m].albedo.data()});
