@@ -507,7 +507,7 @@ def main():
         return
 
     if args.workload != 3:
-        run_views(args)
+        run_views(args, rooflines=args.check)
         return
 
     import numpy as np
@@ -576,6 +576,20 @@ def main():
     log(f"low-resolution maps ablation {ablation}")
     fine = decoder_study(args, 1, args.train_steps)
     log(f"fine-detail decoder workload {fine}")
+    # the batched-view and combined workloads at their full sizes (every view checked against the
+    # oracle, per-stage rooflines)
+    larger = {}
+    if not args.no_cpu:
+        import argparse as _ap
+        sub = _ap.Namespace(**vars(args))
+        sub.check, sub.msaa, sub.tile = True, 1, 0
+        for wl in (2, 4):
+            o = run_views(sub, workload=wl, emit=False, rooflines=True)
+            larger[f"workload_{wl}"] = {k: o[k] for k in ("value", "plain_ms_per_frame", "mtris_per_s", "stage_ms",
+                                                           "tile_pairs", "decoder", "bit_exact_vs_oracle",
+                                                           "stage_rooflines", "pixels_per_frame")}
+            larger[f"workload_{wl}"]["config"] = o["config"]["workload"]
+            larger[f"workload_{wl}"]["bin_size"] = o["config"]["bin_size"]
 
     cam = scene.camera_views()[0]
     cas = scene.cascades()[0]
@@ -697,6 +711,7 @@ def main():
         "frame_psnr_neural_vs_plain_db": round(frame_psnr, 2),
         "ablation_low_res_maps": ablation,
         "decoder_fine_detail": fine,
+        "larger_workloads": larger,
         "bit_exact_vs_oracle": bit_exact,
     }
     print(json.dumps(out), flush=True)
@@ -718,19 +733,21 @@ WORKLOADS = {
 }
 
 
-def run_views(args):
+def run_views(args, workload=None, emit=True, rooflines=False):
     """Frames of the multi-view workloads: every view of the frame in one launch per stage, replayed
-    from a captured graph; value = device ms per frame (neural form where the workload has it)."""
+    from a captured graph; value = device ms per frame (neural form where the workload has it).  With
+    `rooflines`, the algorithmic bytes of the geometry + binning + raster stage (B_geo) and of the
+    shading kernel (the oracle's footprint of every camera view) against their measured times."""
     import numpy as np
 
     import rosy_amd
     from rosy_amd import abi
 
+    wl = args.workload if workload is None else workload
     t_gen = time.time()
-    scene = (rosy_amd.Scene(3, seed=args.seed, n_translucent=16) if args.workload == 5 else
-             rosy_amd.Scene(args.workload, seed=args.seed))
+    scene = (rosy_amd.Scene(3, seed=args.seed, n_translucent=16) if wl == 5 else rosy_amd.Scene(wl, seed=args.seed))
     cams, cascs = scene.camera_views(), scene.cascades()
-    log(f"workload {args.workload}: {scene.n_triangles} triangles, {len(cams)} camera views, {len(cascs)} cascades "
+    log(f"workload {wl}: {scene.n_triangles} triangles, {len(cams)} camera views, {len(cascs)} cascades "
         f"(generated in {time.time() - t_gen:.1f}s)")
     r = rosy_amd.Renderer(device=0)
     r.load(scene)
@@ -747,7 +764,7 @@ def run_views(args):
     rs_used = scene.s.raster_state if args.msaa == 1 else rs
     frame = plain
     dec = None
-    if args.workload == 4:
+    if wl == 4:
         r.neural_init(decay_steps=args.train_steps, weights=abi.NEURAL_WEIGHTS_FP8)
         r.neural_train(args.train_steps)
         psnr = r.neural_evaluate()
@@ -774,7 +791,7 @@ def run_views(args):
                 o, _ = oracle.raster_view(scene, cv.view_projection[:], cv.width, cv.height, False, rs)
             exact &= bool(np.array_equal(r.visibility(vi), o))
         log(f"visibility + shadow maps bit-exact vs oracle: {exact}")
-        if args.workload == 5:
+        if wl == 5:
             vis = r.visibility(0)
             maps = [r.shadow_map(0)]
             if args.msaa == 4:
@@ -787,6 +804,33 @@ def run_views(args):
             blend_check = {"max_abs_err_vs_fp64": float(np.abs(r.color(0).astype(np.float64) - want).max()),
                            "pixels_with_layers": int((count > 0).sum()), "max_layers": int(count.max())}
             log(f"blended frame vs FP64 oracle: {blend_check}")
+    stage_rooflines = None
+    if rooflines:
+        import oracle
+        rs = rs_used
+        maps = [r.shadow_map(ci) for ci in range(len(cascs))]
+        params = None
+        if wl == 4:
+            lat, _ = r.neural_params()
+            active = (psnr >= 40.0).astype(np.uint32)
+            params = oracle.NeuralParams(np.arange(scene.s.n_materials), lat.astype(np.float16).astype(np.float32),
+                                         r.neural_inference_mlp(), active)
+        b_shade, parts = 0, {}
+        for vi, cv in enumerate(cams):
+            fp = oracle.shade_footprint(scene, cv, cascs, maps, r.visibility(vi), neural=params)
+            b, pr = shade_bytes(fp, cv.width * cv.height)
+            b_shade += b
+            for k, v in pr.items():
+                parts[k] = parts.get(k, 0) + v
+        views_px = [(c.width, c.height, False) for c in cams] + [(c.size, c.size, True) for c in cascs]
+        b_geo = geo_bytes(scene, views_px, int(fst.tile_pairs))
+        stage_rooflines = {
+            "geometry_binning_raster": hbm_line("transform, setup, clip, bin scan, scatter and raster kernels", b_geo,
+                                                float(st[0] + st[1] + st[2])),
+            "shade": hbm_line("k_shade<neural>" if wl == 4 else "k_shade<plain>", b_shade, float(st[3]),
+                              {"bytes": parts}),
+        }
+        log(f"stage rooflines {stage_rooflines}")
     r.set_debug_flags(0)
     r.render_graph(plain, args.warmup)
     r.render_graph(plain, args.steps)
@@ -798,7 +842,7 @@ def run_views(args):
     out = {"metric": "ms/frame (all views of the frame)", "value": round(ms, 5), "unit": "ms/frame", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": False,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": WORKLOADS[args.workload], "seed": args.seed, "bin_size": tile,
+           "config": {"workload": WORKLOADS[wl], "seed": args.seed, "bin_size": tile,
                       "samples_per_pixel": args.msaa},
            "plain_ms_per_frame": round(plain_ms, 5),
            "mtris_per_s": round(views * scene.n_triangles / (ms * 1e-3) / 1e6, 1),
@@ -808,10 +852,12 @@ def run_views(args):
                         "blended_pass": round(float(st[4]), 5)},
            "tile_pairs": int(fst.tile_pairs), "triangles_setup": int(fst.triangles_setup),
            "triangles_clipped": int(fst.triangles_clipped), "decoder": dec, "bit_exact_vs_oracle": exact,
-           "blended": blend_check}
-    print(json.dumps(out), flush=True)
+           "blended": blend_check, "stage_rooflines": stage_rooflines}
+    if emit:
+        print(json.dumps(out), flush=True)
     r.close()
     scene.close()
+    return out
 
 
 if __name__ == "__main__":

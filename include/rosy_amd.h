@@ -188,7 +188,8 @@ typedef struct rsy_raster_state {
     float depth_bias_clamp;
     float depth_bias_slope_factor;
     uint32_t pcf_mode;  /* rsy_pcf_mode */
-    uint32_t tile_size; /* raster bin size in pixels: 32 or 64; 0 = autotuned (32 with 4 samples) */
+    uint32_t tile_size; /* raster bin size in pixels: 8, 16, 32 or 64 (16 or more with a blended pass,
+                           32 with 4 samples); 0 = autotuned (32 with 4 samples) */
     float clear_color[4];
     /* Samples per pixel of the camera views: 1 (0 = 1) or 4.  The reference's main pass renders
      * into a 4-sample colour attachment resolved by averaging (Graphics.cpp:1794-1808, 5566-5575);
@@ -340,7 +341,10 @@ rsy_result rsy_get_device_outputs(rsy_ctx* ctx, uint32_t view, const void** vis,
 #define RSY_DEBUG_STAGE_TIMING 2u  /* record HIP events between stages */
 #define RSY_DEBUG_MIN_CAPACITY 4u  /* start the frame's record, clip-queue, bin-pair, bin-list and
                                       blended-fragment capacities small, so that frames overflow and
-                                      take the regrowth path (takes effect at the next new layout) */
+                                      take the regrowth path (takes effect at the next new layout),
+                                      and give a new layout's first frame 64 raster workgroups, too
+                                      few for its work items, so that it is re-run with a grid that
+                                      fits */
 rsy_result rsy_set_debug_flags(rsy_ctx* ctx, uint32_t flags);
 
 /* The output stage of the last shaded frame for one camera view (Graphics.cpp:1908, 5835-5880):
@@ -352,8 +356,9 @@ rsy_result rsy_present(rsy_ctx* ctx, uint32_t view, uint32_t width, uint32_t hei
 rsy_result rsy_read_draw_image(rsy_ctx* ctx, uint16_t* out_rgba16f, uint64_t n_pixels);
 rsy_result rsy_read_present(rsy_ctx* ctx, uint8_t* out_bgra8, uint64_t n_pixels);
 
-/* Autotuner: replays the given frame n_iters times per candidate bin size (32, 64) and keeps the
- * fastest for later frames with tile_size == 0.  Writes the chosen size. */
+/* Autotuner: replays the given frame n_iters times per candidate bin size (8, 16, 32, 64; only 32
+ * with 4 samples, 16 and up with a blended pass) and keeps the fastest for later frames with
+ * tile_size == 0.  Writes the chosen size. */
 rsy_result rsy_autotune(rsy_ctx* ctx, const rsy_frame_desc* frame, uint32_t n_iters, uint32_t* chosen_tile);
 
 /* ---------------------------------------------------------------------------------------------

@@ -135,7 +135,7 @@ uint32_t msaa_of(const rsy_ctx* c) { return c->rs.msaa_samples == 4 ? 4u : 1u; }
 
 uint32_t choose_tile(const rsy_ctx* c) {
     if (msaa_of(c) == 4) return 32u;  // the 4-sample tile of a 64-pixel bin exceeds the LDS budget
-    if (c->rs.tile_size == 32 || c->rs.tile_size == 64) return c->rs.tile_size;
+    if (c->rs.tile_size) return c->rs.tile_size;  // 8, 16, 32 or 64 (rsy_update checks it)
     return c->autotuned_tile ? c->autotuned_tile : 32u;
 }
 
@@ -163,6 +163,8 @@ rsy_result plan_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint
     std::vector<uint32_t> order(views.size());
     for (uint32_t vi = 0; vi < views.size(); ++vi) order[vi] = vi;
     const bool blend = shade != RSY_SHADE_NONE && !c->lists[2].empty();
+    if (blend && tile < 16)
+        return fail(c, RSY_INVALID_ARGUMENT, "the blended pass collects fragments in 16x16 sub-tiles: bins of 16 pixels or more");
     if (blend && msaa_of(c) == 4 && c->list_tris[2] >= (1u << 28))
         return fail(c, RSY_INVALID_ARGUMENT, "4-sample blended pass: the blended list must hold fewer than 2^28 triangles");
     if (blend)
@@ -195,7 +197,10 @@ rsy_result plan_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint
         // capacities in 64 bits: rec_scale doubles on every record overflow
         const uint64_t clip_cap = uint64_t(min_cap ? 256u : std::max<uint32_t>(4096u, tris / 8)) * rec_scale;
         const uint64_t rec_cap = uint64_t(tris) + 2 * clip_cap;
-        const uint64_t big_cap = uint64_t(min_cap ? 4096u : std::max<uint32_t>(65536u, tris / 4)) * rec_scale;
+        // a triangle spans about (32 / tile)^2 times as many bins below 32-pixel bins
+        const uint64_t small_bins = tile < 32 ? uint64_t(32 / tile) * (32 / tile) : 1u;
+        const uint64_t big_cap =
+            uint64_t(min_cap ? 4096u : std::max<uint32_t>(65536u, tris / 4)) * (min_cap ? 1u : small_bins) * rec_scale;
         if (rec_cap >= 0xffffffffull || uint64_t(L.rec_total) + rec_cap >= 0xffffffffull)
             return fail(c, RSY_OVERFLOW, "record buffer too large");
         if (uint64_t(L.clip_total) + clip_cap >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "clip queue too large");
@@ -233,10 +238,12 @@ rsy_result plan_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint
     if (L.total_xv >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "transformed vertex buffer too large");
     // bin lists: an estimate of the frame's (triangle, bin) pairs
     uint64_t want_pairs = 0;
-    for (const auto& g : L.vg) want_pairs += uint64_t(c->list_tris[g.list]) * 3 + g.tiles_x * g.tiles_y;
+    const uint64_t per_tri = tile < 32 ? 3u * (32 / tile) * (32 / tile) : 3u;
+    for (const auto& g : L.vg) want_pairs += uint64_t(c->list_tris[g.list]) * per_tri + g.tiles_x * g.tiles_y;
     want_pairs *= rec_scale;
     if (min_cap) want_pairs = std::min<uint64_t>(want_pairs, 4096);
     L.want_pairs = std::min<uint64_t>(want_pairs, 0xfffffff0ull);
+    L.raster_grid = 0;
     return RSY_OK;
 }
 
@@ -529,6 +536,14 @@ rsy_result enqueue_frame(rsy_ctx* c, const Layout& L, const FrameMem& m, bool ca
     b.items = m.items;
     b.items_cap = raster_items_cap(L.total_bins, m.list_cap);
     b.zero_bins = m.zero_bins;
+    // one raster workgroup per work item: as many as the last frame of this layout had (with slack),
+    // or an estimate (every bin, and a split item per 256 estimated pairs); the scan flags a frame
+    // with more items, and finish_frame re-runs it with a grid that fits
+    uint32_t grid = L.raster_grid ? L.raster_grid
+                                  : uint32_t(std::min<uint64_t>(uint64_t(L.total_bins) + L.want_pairs / 256 + 256,
+                                                                0xffffffffull));
+    if ((c->debug_flags & RSY_DEBUG_MIN_CAPACITY) && !L.raster_grid) grid = std::min(grid, 64u);  // the re-run path
+    b.raster_grid = std::min(grid, b.items_cap);
     HIP_TRY(c, launch_binning(b, s), "binning kernels");
     if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[1], s), "event");
 
@@ -545,6 +560,7 @@ rsy_result enqueue_frame(rsy_ctx* c, const Layout& L, const FrameMem& m, bool ca
     r.items_cap = b.items_cap;
     r.totals = b.totals;
     r.samples = msaa_of(c);
+    r.grid = b.raster_grid;
     HIP_TRY(c, launch_raster(r, s), "raster kernel");
     if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[2], s), "event");
 
@@ -629,7 +645,7 @@ rsy_result enqueue_frame(rsy_ctx* c, const Layout& L, const FrameMem& m, bool ca
     if (events) {
         HIP_TRY(c, hipMemcpyAsync(c->readback_counters.p, m.counters, nv * kCtrPerView * 4, hipMemcpyDeviceToHost, s),
                 "readback");
-        HIP_TRY(c, hipMemcpyAsync(static_cast<uint8_t*>(c->readback_counters.p) + nv * kCtrPerView * 4, m.totals, 12,
+        HIP_TRY(c, hipMemcpyAsync(static_cast<uint8_t*>(c->readback_counters.p) + nv * kCtrPerView * 4, m.totals, 16,
                                   hipMemcpyDeviceToHost, s), "readback");
         HIP_TRY(c, hipEventRecord(c->ev_end, s), "event");
     }
@@ -645,6 +661,12 @@ rsy_result ready_to_render(rsy_ctx* c) {
         if (r != RSY_OK) return r;
     }
     return RSY_OK;
+}
+
+// Raster workgroups for frames of layout L after one with `items` work items: those, 1/8 more and
+// at least one per bin (items beyond the bins come only from crowded bins split in several).
+uint32_t raster_grid_for(const Layout& L, uint32_t items) {
+    return uint32_t(std::min<uint64_t>(std::max<uint64_t>(L.total_bins, uint64_t(items) + items / 8) + 64, 0xffffffffull));
 }
 
 // Reads the last frame's counters (after a synchronize) and reports whether it overflowed.
@@ -668,6 +690,8 @@ bool collect_stats(rsy_ctx* c) {
     }
     st.tile_size = L.tile;
     st.overflowed = overflow ? 1 : 0;
+    // the raster grid of the next frames of this layout: this frame's work items and some slack
+    if (!overflow && totals[3]) c->layout.raster_grid = raster_grid_for(L, totals[3]);
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c->ev_begin, c->ev_end) == hipSuccess) st.gpu_ms = ms;
     if (c->debug_flags & RSY_DEBUG_STAGE_TIMING) {
@@ -709,7 +733,13 @@ rsy_result finish_frame(rsy_ctx* c) {
         // grow: bin list capacity to the measured pair count, record/clip capacity x2
         const uint32_t nv = uint32_t(c->layout.vg.size());
         const uint32_t total_pairs = static_cast<const uint32_t*>(c->readback_counters.p)[nv * kCtrPerView];
-        const bool list_over = static_cast<const uint32_t*>(c->readback_counters.p)[nv * kCtrPerView + 2] != 0;
+        const uint32_t over_bits = static_cast<const uint32_t*>(c->readback_counters.p)[nv * kCtrPerView + 2];
+        const bool list_over = (over_bits & 1u) != 0;
+        if (over_bits & 2u) {  // more raster work items than workgroups: a grid for this frame's count
+            c->layout.raster_grid =
+                raster_grid_for(c->layout, static_cast<const uint32_t*>(c->readback_counters.p)[nv * kCtrPerView + 3]);
+            invalidate_graph(c);
+        }
         bool rec_over = false, frags_over = false;
         for (uint32_t v = 0; v < nv; ++v) {
             const uint32_t ov = static_cast<const uint32_t*>(c->readback_counters.p)[v * kCtrPerView + kCtrOverflow];
@@ -1034,8 +1064,9 @@ rsy_result rsy_update_graphic_objects(rsy_ctx* c, uint64_t offset, const rsy_obj
 rsy_result rsy_update(rsy_ctx* c, const rsy_scene_data* sd, const rsy_raster_state* state) {
     if (!c || !sd) return RSY_INVALID_ARGUMENT;
     if (state) {
-        if (state->tile_size != 0 && state->tile_size != 32 && state->tile_size != 64)
-            return fail(c, RSY_INVALID_ARGUMENT, "tile_size must be 0, 32 or 64");
+        if (state->tile_size != 0 && state->tile_size != 8 && state->tile_size != 16 && state->tile_size != 32 &&
+            state->tile_size != 64)
+            return fail(c, RSY_INVALID_ARGUMENT, "tile_size must be 0, 8, 16, 32 or 64");
         if (state->pcf_mode > RSY_PCF_GRID_3X3) return fail(c, RSY_INVALID_ARGUMENT, "unknown pcf mode");
         if (state->msaa_samples > 1 && state->msaa_samples != 4)
             return fail(c, RSY_INVALID_ARGUMENT, "msaa_samples must be 0, 1 or 4");
@@ -1111,7 +1142,7 @@ rsy_result rsy_render_graph(rsy_ctx* c, const rsy_frame_desc* frame, uint32_t n_
     for (uint32_t i = 0; i < n_replays; ++i) HIP_TRY(c, hipGraphLaunch(c->graph_exec, c->stream), "graph launch");
     const uint32_t nv = uint32_t(c->layout.vg.size());
     HIP_TRY(c, hipMemcpyAsync(c->readback_counters.p, c->counters.p, nv * kCtrPerView * 4, hipMemcpyDeviceToHost, c->stream), "readback");
-    HIP_TRY(c, hipMemcpyAsync(static_cast<uint8_t*>(c->readback_counters.p) + nv * kCtrPerView * 4, c->totals.p, 12,
+    HIP_TRY(c, hipMemcpyAsync(static_cast<uint8_t*>(c->readback_counters.p) + nv * kCtrPerView * 4, c->totals.p, 16,
                               hipMemcpyDeviceToHost, c->stream), "readback");
     HIP_TRY(c, hipEventRecord(c->ev_end, c->stream), "event");
     HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
@@ -1222,16 +1253,23 @@ rsy_result rsy_render_noalloc(rsy_ctx* c, const rsy_frame_desc* frame, const rsy
                                   c->stream), "upload blocks");
     r = upload_frame(c, L, m, nullptr, 0u);
     if (r != RSY_OK) return r;
-    r = enqueue_frame(c, L, m, false, false);
-    if (r != RSY_OK) return r;
-    uint32_t ctr[kMaxViews * kCtrPerView + 8] = {};
-    HIP_TRY(c, hipMemcpyAsync(ctr, m.counters, nv * kCtrPerView * 4, hipMemcpyDeviceToHost, c->stream), "readback");
-    HIP_TRY(c, hipMemcpyAsync(ctr + nv * kCtrPerView, m.totals, 12, hipMemcpyDeviceToHost, c->stream), "readback");
-    HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
-    bool over = ctr[nv * kCtrPerView + 2] != 0;
-    for (size_t v = 0; v < nv; ++v) over |= ctr[v * kCtrPerView + kCtrOverflow] != 0;
-    if (over) return fail(c, RSY_OVERFLOW, "the frame overflowed the working memory the static scratch holds");
-    return RSY_OK;
+    // at most two passes: the second when the frame had more raster work items than the estimated
+    // grid (nothing is allocated for it)
+    for (int pass = 0; pass < 2; ++pass) {
+        r = enqueue_frame(c, L, m, false, false);
+        if (r != RSY_OK) return r;
+        uint32_t ctr[kMaxViews * kCtrPerView + 8] = {};
+        HIP_TRY(c, hipMemcpyAsync(ctr, m.counters, nv * kCtrPerView * 4, hipMemcpyDeviceToHost, c->stream), "readback");
+        HIP_TRY(c, hipMemcpyAsync(ctr + nv * kCtrPerView, m.totals, 16, hipMemcpyDeviceToHost, c->stream), "readback");
+        HIP_TRY(c, hipStreamSynchronize(c->stream), "synchronize");
+        const uint32_t bits = ctr[nv * kCtrPerView + 2];
+        bool over = (bits & 1u) != 0;
+        for (size_t v = 0; v < nv; ++v) over |= ctr[v * kCtrPerView + kCtrOverflow] != 0;
+        if (over) return fail(c, RSY_OVERFLOW, "the frame overflowed the working memory the static scratch holds");
+        if (!(bits & 2u)) return RSY_OK;
+        L.raster_grid = raster_grid_for(L, ctr[nv * kCtrPerView + 3]);
+    }
+    return fail(c, RSY_OVERFLOW, "the frame's raster work items outnumber the workgroups");
 }
 
 static rsy_result camera_index(rsy_ctx* c, uint32_t view, uint32_t& idx) {
@@ -1390,8 +1428,10 @@ rsy_result rsy_autotune(rsy_ctx* c, const rsy_frame_desc* frame, uint32_t n_iter
     const uint32_t saved = c->rs.tile_size;
     float best_ms = 1e30f;
     uint32_t best = 32;
-    for (uint32_t t : {32u, 64u}) {
-        if (t == 64 && msaa_of(c) == 4) continue;  // 4-sample frames use 32-pixel bins
+    for (uint32_t t : {8u, 16u, 32u, 64u}) {
+        if (t != 32 && msaa_of(c) == 4) continue;  // 4-sample frames use 32-pixel bins
+        if (t == 8 && !c->lists[2].empty() && (!frame || frame->shade_mode != RSY_SHADE_NONE))
+            continue;  // the blended pass needs bins of 16 pixels or more
         c->rs.tile_size = t;
         c->layout.valid = false;
         rsy_result r = rsy_render_graph(c, frame, std::max<uint32_t>(1, n_iters));

@@ -108,6 +108,7 @@ struct Layout {
     uint64_t total_xv = 0;
     uint32_t rec_total = 0, clip_total = 0, big_total = 0, cull_total = 0;
     uint64_t want_pairs = 0;  // estimate of the frame's (triangle, bin) pairs
+    uint32_t raster_grid = 0; // raster workgroups: the last frame's work items plus slack (0: estimate)
     uint32_t n_shadow = 0, n_camera = 0, n_blend = 0;
     uint32_t first_blend = 0, blend_bins = 0;  // first blend view, bins of all blend views
     bool valid = false;

@@ -54,7 +54,7 @@ struct BinLaunch {
     const uint32_t* block_valid;
     const uint2* boxes;
     uint32_t n_tblocks;
-    uint32_t* totals;  // [0] (record, bin) pairs, [1] records set up, [2] bin lists or items overflowed,
+    uint32_t* totals;  // [0] (record, bin) pairs, [1] records set up, [2] overflow bits (lists, raster grid),
                        // [3] raster work items, [4] first mid-class item, [5] bins to clear,
                        // [6] first split item, [7] first light-class item
     const TriRec* recs;
@@ -67,6 +67,7 @@ struct BinLaunch {
     uint4* items;           // raster work items (bin, first list entry, records, split): split bins' first
     uint32_t items_cap;     // items capacity
     uint32_t* zero_bins;    // bins whose pixels the scatter clears: empty bins and split bins
+    uint32_t raster_grid;   // workgroups of the raster launch (more items than these: totals[2] bit 1)
 };
 
 struct RasterLaunch {
@@ -80,8 +81,9 @@ struct RasterLaunch {
     uint64_t rec_cap_total;
     const uint4* items;
     uint32_t items_cap;
-    const uint32_t* totals;
+    const uint32_t* totals;  // [3] work items
     uint32_t samples;  // of the camera views (1 or 4)
+    uint32_t grid;     // workgroups: one per work item (the last frame's count, with slack)
 };
 
 // The blended pass over the bins of the blend views (one launch).

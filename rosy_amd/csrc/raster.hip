@@ -604,7 +604,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_partial(const uint32_t* c
 __global__ __launch_bounds__(kScanThreads) void k_scan_final(const uint32_t* counts, uint32_t n, uint32_t* offsets,
                                                      const ScanPartial* partial, uint32_t n_parts,
                                                      uint32_t list_cap, uint32_t* totals, uint4* items,
-                                                     uint32_t items_cap, uint32_t* zero_bins) {
+                                                     uint32_t items_cap, uint32_t* zero_bins, uint32_t raster_grid) {
     __shared__ uint64_t lds64[4 * 16];
     __shared__ ScanPartial base_lds;
     // this block's counts are requested before the scan of the partials
@@ -647,7 +647,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const uint32_t* cou
             const uint64_t n_items = uint64_t(n_heavy) + n_mid + n_split + n_light;
             totals[0] = pairs;
             totals[1] = total_valid;
-            totals[2] = (pairs > list_cap || n_items > items_cap) ? 1u : 0u;
+            // bit 0: bin lists or item table too small; bit 1: more items than raster workgroups
+            totals[2] = ((pairs > list_cap || n_items > items_cap) ? 1u : 0u) | (n_items > raster_grid ? 2u : 0u);
             totals[3] = uint32_t(min<uint64_t>(n_items, items_cap));
             totals[4] = n_heavy;                    // first mid item
             totals[5] = n_zero;
@@ -779,8 +780,10 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
     __shared__ uint32_t row_start[kChunkRecs];  // exclusive prefix of row counts
     __shared__ uint32_t scan_lds[4];
 
-    // the grid covers the item table's capacity: workgroups past the frame's item count exit (both
-    // loads are issued together)
+    // One workgroup per item.  The grid is sized from the last frame's item count (the count of
+    // this frame is known only on the device): the bin scan flags a frame with more items than
+    // workgroups (totals[2] bit 1), which is re-run with a grid that fits.  The item is loaded
+    // beside the count (blockIdx.x < gridDim.x <= the table's capacity).
     const uint32_t n_items = totals[3];
     const uint4 it = items[blockIdx.x];
     if (blockIdx.x >= n_items) return;
@@ -885,7 +888,7 @@ hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
                        b.block_valid, b.n_tblocks);
     hipLaunchKernelGGL(k_scan_final, dim3(parts), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, b.bin_offsets,
                        partial, parts, b.list_cap, b.totals, b.items, b.items_cap,
-                       b.zero_bins);
+                       b.zero_bins, b.raster_grid);
     hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.n_views, b.boxes,
                        b.counters, b.bin_offsets, b.pair_slots, b.big_pairs, b.lists, b.list_cap, b.shift, b.totals,
                        b.zero_bins);
@@ -893,16 +896,22 @@ hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
 }
 
 hipError_t launch_raster(const RasterLaunch& r, hipStream_t s) {
-    // one workgroup per work-item slot (extra workgroups exit)
-    const uint32_t grid = r.items_cap;
+    const uint32_t grid = min(r.grid, r.items_cap);
     if (grid == 0) return hipSuccess;
 #define RSY_RASTER(T, N)                                                                                         \
     hipLaunchKernelGGL((k_raster<T, N>), dim3(grid), dim3(256), 0, s, r.views, r.n_views, r.lists, r.list_cap, \
                        r.recs, r.rec_cap_total, r.items, r.totals)
-    if (r.tile == 32 && r.samples == 1) RSY_RASTER(32, 1);
-    else if (r.tile == 64 && r.samples == 1) RSY_RASTER(64, 1);
-    else if (r.tile == 32 && r.samples == 4) RSY_RASTER(32, 4);
-    else return hipErrorInvalidValue;  // 4-sample frames use 32-pixel bins
+    if (r.samples == 1) {
+        if (r.tile == 8) RSY_RASTER(8, 1);
+        else if (r.tile == 16) RSY_RASTER(16, 1);
+        else if (r.tile == 32) RSY_RASTER(32, 1);
+        else if (r.tile == 64) RSY_RASTER(64, 1);
+        else return hipErrorInvalidValue;
+    } else if (r.tile == 32 && r.samples == 4) {
+        RSY_RASTER(32, 4);
+    } else {
+        return hipErrorInvalidValue;  // 4-sample frames use 32-pixel bins
+    }
 #undef RSY_RASTER
     return hipGetLastError();
 }

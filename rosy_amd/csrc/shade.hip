@@ -727,7 +727,7 @@ __global__ __launch_bounds__(256) void k_shade_ms(ShadeLaunch L) {
 // not produce).
 constexpr uint32_t kDiscarded = 0xffffffffu;
 
-// One workgroup per 16x16 quarter (T = 32) or sixteenth (T = 64) of a bin of a blend view, so that
+// One workgroup per 16x16 bin (T = 16), quarter (T = 32) or sixteenth (T = 64) of a bin of a blend view, so that
 // a crowded bin spreads over several workgroups.  Two raster sweeps over the bin's blended records:
 // the first counts each pixel's passing fragments in LDS, a prefix sum turns the counts into list
 // offsets (the workgroup takes its range of the frame's fragment array with one device atomic),
@@ -912,7 +912,9 @@ __global__ __launch_bounds__(256) void k_blend_resolve(const BlendLaunch B, cons
 hipError_t launch_blend_fragments(const BlendLaunch& b, const ShadeLaunch& l, hipStream_t s) {
     if (b.blend_bins == 0 || l.total_blocks == 0) return hipSuccess;
     const uint32_t shade_blocks = max(1u, min(2048u, (b.frag_cap + 255) / 256));
-    if (b.tile == 32 && b.samples == 1)
+    if (b.tile == 16 && b.samples == 1)
+        hipLaunchKernelGGL((k_blend_collect<16, 1>), dim3(b.blend_bins), dim3(256), 0, s, b);
+    else if (b.tile == 32 && b.samples == 1)
         hipLaunchKernelGGL((k_blend_collect<32, 1>), dim3(b.blend_bins * 4), dim3(256), 0, s, b);
     else if (b.tile == 64 && b.samples == 1)
         hipLaunchKernelGGL((k_blend_collect<64, 1>), dim3(b.blend_bins * 16), dim3(256), 0, s, b);
@@ -929,7 +931,9 @@ hipError_t launch_blend_fragments(const BlendLaunch& b, const ShadeLaunch& l, hi
 
 hipError_t launch_blend_resolve(const BlendLaunch& b, const ShadeLaunch& l, hipStream_t s) {
     if (b.blend_bins == 0 || l.total_blocks == 0) return hipSuccess;
-    if (b.tile == 32 && b.samples == 1)
+    if (b.tile == 16 && b.samples == 1)
+        hipLaunchKernelGGL((k_blend_resolve<16, 1>), dim3(l.total_blocks), dim3(256), 0, s, b, l);
+    else if (b.tile == 32 && b.samples == 1)
         hipLaunchKernelGGL((k_blend_resolve<32, 1>), dim3(l.total_blocks), dim3(256), 0, s, b, l);
     else if (b.tile == 64 && b.samples == 1)
         hipLaunchKernelGGL((k_blend_resolve<64, 1>), dim3(l.total_blocks), dim3(256), 0, s, b, l);

@@ -15,6 +15,13 @@ namespace rsy {
 
 __device__ inline int wrap_coord(int i, int n, uint32_t mode) {
     if (mode == 0) return min(max(i, 0), n - 1);
+    if ((n & (n - 1)) == 0) {  // power-of-two extent: the modulo is a mask (two's complement)
+        if (mode == 1) {
+            const int m = i & (2 * n - 1);
+            return m < n ? m : 2 * n - 1 - m;
+        }
+        return i & (n - 1);
+    }
     if (mode == 1) {
         const int period = 2 * n;
         int m = i % period;

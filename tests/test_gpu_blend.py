@@ -3,6 +3,8 @@ order over the opaque frame -- GEQUAL against the opaque depth, no depth write, 
 mask discard, SRC_ALPHA / ONE_MINUS_SRC_ALPHA -- within 1e-3 max abs, at two view sizes.  The scene
 holds up to six layers at a pixel; the visibility buffer stays the opaque one, bit-exact, and graph
 replays give the eager frame."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -85,6 +87,18 @@ def test_generated_workload_with_translucent_materials():
     assert count.max() >= 4 and err <= TOL
     r.render_graph(frame, 3)
     assert np.array_equal(r.color(0), col)
+    # the same frame with 16- and 64-pixel bins (the blended pass collects 16x16 sub-tiles of a bin);
+    # 8-pixel bins are refused with a blended pass
+    for tile in (16, 64):
+        rs2 = abi.RasterState()
+        C.memmove(C.byref(rs2), C.byref(rs), C.sizeof(rs2))
+        rs2.tile_size = tile
+        r.update(s.s.scene_data, rs2)
+        r.render(frame)
+        assert np.array_equal(r.visibility(0), vis) and np.array_equal(r.color(0), col), tile
+    rs2.tile_size = 8
+    r.update(s.s.scene_data, rs2)
+    assert rosy_amd.lib().rsy_render(r.ctx, C.byref(frame)) == abi.RSY_INVALID_ARGUMENT
     r.close()
 
 
