@@ -474,6 +474,12 @@ def main():
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--texture-only", action="store_true",
+                    help="only the texture pipeline: BC7 upload with full chains and the plain frame of the decoder "
+                         "workload with every map's mip chain (trilinear sampling); prints its JSON")
+    ap.add_argument("--frame-only", action="store_true",
+                    help="only the decoder workload's frames (no output stage, 4-sample frames, texture pipeline, "
+                         "ablation or other workloads): the run the committed profiles are taken of")
     ap.add_argument("--workload", type=int, default=3, choices=[1, 2, 3, 4, 5],
                     help="3 (default): the decoder workload the metric is quoted on; 1: scene-scale plain; "
                          "2: 1M triangles, 2560x1440 + 4 cascades; 4: 2M triangles, 4 views + 4 cascades + decoder; "
@@ -504,6 +510,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": n, "copies": 5}), flush=True)
+        return
+
+    if args.texture_only:
+        print(json.dumps(texture_timing(args)), flush=True)
         return
 
     if args.workload != 3:
@@ -559,27 +569,28 @@ def main():
     ms = float(r.stats().gpu_ms)
     log(f"graph frame: neural {ms:.4f} ms, plain {plain_ms:.4f} ms (device); neural wall {wall:.4f} ms")
 
-    # output stage of the neural frame (draw image + sRGB blit) and the .rsy reader on this workload
-    output_stage = [present_timing(r, 1920, 1080), present_timing(r, 2560, 1440)]
-    rsy_decode = rsy_decode_timing(scene)
-    log(f"output stage {output_stage}; .rsy decode {rsy_decode}")
-
-    # the same frames with the reference's 4-sample main pass
-    msaa = msaa_timing(r, scene, plain, neural, args, check=not args.no_cpu)
-    log(f"4-sample frames {msaa}")
-    # the texture pipeline: BC7 upload with full chains, and the plain frame with mip chains
-    texture_pipeline = texture_timing(args)
-    log(f"texture pipeline {texture_pipeline}")
-    # what the network earns: the low-resolution-map ablation on this workload, and the same study on
-    # content with detail above the latent grid's Nyquist rate
-    ablation = low_res_ablation(scene, col_plain, vis != 0, args)
-    log(f"low-resolution maps ablation {ablation}")
-    fine = decoder_study(args, 1, args.train_steps)
-    log(f"fine-detail decoder workload {fine}")
+    output_stage = rsy_decode = msaa = texture_pipeline = ablation = fine = None
+    if not args.frame_only:
+        # output stage of the neural frame (draw image + sRGB blit) and the .rsy reader on this workload
+        output_stage = [present_timing(r, 1920, 1080), present_timing(r, 2560, 1440)]
+        rsy_decode = rsy_decode_timing(scene)
+        log(f"output stage {output_stage}; .rsy decode {rsy_decode}")
+        # the same frames with the reference's 4-sample main pass
+        msaa = msaa_timing(r, scene, plain, neural, args, check=not args.no_cpu)
+        log(f"4-sample frames {msaa}")
+        # the texture pipeline: BC7 upload with full chains, and the plain frame with mip chains
+        texture_pipeline = texture_timing(args)
+        log(f"texture pipeline {texture_pipeline}")
+        # what the network earns: the low-resolution-map ablation on this workload, and the same
+        # study on content with detail above the latent grid's Nyquist rate
+        ablation = low_res_ablation(scene, col_plain, vis != 0, args)
+        log(f"low-resolution maps ablation {ablation}")
+        fine = decoder_study(args, 1, args.train_steps)
+        log(f"fine-detail decoder workload {fine}")
     # the batched-view and combined workloads at their full sizes (every view checked against the
     # oracle, per-stage rooflines)
     larger = {}
-    if not args.no_cpu:
+    if not args.no_cpu and not args.frame_only:
         import argparse as _ap
         sub = _ap.Namespace(**vars(args))
         sub.check, sub.msaa, sub.tile = True, 1, 0
