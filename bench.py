@@ -480,17 +480,18 @@ def main():
     ap.add_argument("--frame-only", action="store_true",
                     help="only the decoder workload's frames (no output stage, 4-sample frames, texture pipeline, "
                          "ablation or other workloads): the run the committed profiles are taken of")
-    ap.add_argument("--workload", type=int, default=3, choices=[1, 2, 3, 4, 5],
-                    help="3 (default): the decoder workload the metric is quoted on; 1: scene-scale plain; "
+    ap.add_argument("--workload", type=int, default=3, choices=[0, 1, 2, 3, 4, 5],
+                    help="3 (default): the decoder workload the metric is quoted on; 0: one 2,048-triangle sphere "
+                         "at 640x360; 1: scene-scale plain; "
                          "2: 1M triangles, 2560x1440 + 4 cascades; 4: 2M triangles, 4 views + 4 cascades + decoder; "
                          "5: the decoder scene with 16 translucent materials (blended pass), plain shading")
     ap.add_argument("--check", action="store_true",
-                    help="workloads 1, 2, 4, 5: compare every view's visibility buffer / shadow map with the oracle "
-                         "(5: also the blended frame's colour with the FP64 oracle)")
-    ap.add_argument("--tile", type=int, default=0, choices=[0, 32, 64],
+                    help="workloads 0, 1, 2, 4, 5: compare every view's visibility buffer / shadow map with the "
+                         "oracle and every camera view's colour with the FP64 oracle (5: the blended frame's)")
+    ap.add_argument("--tile", type=int, default=0, choices=[0, 8, 16, 32, 64],
                     help="bin size in pixels; 0 (default): the autotuner picks it")
     ap.add_argument("--msaa", type=int, default=1, choices=[1, 4],
-                    help="workloads 1, 2, 4, 5: samples per camera pixel (4: the reference's 4-sample main pass, "
+                    help="workloads 0, 1, 2, 4, 5: samples per camera pixel (4: the reference's 4-sample main pass, "
                          "bin size 32)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 times): the reference kernel that "
@@ -731,6 +732,8 @@ def main():
 
 
 WORKLOADS = {
+    0: "small: one octahedral geodesic sphere of 2,048 triangles, 640x360, 1024^2 shadow map with 3x3 PCF, one "
+       "material of 256^2 maps, plain shading",
     1: "scene-scale: 120 meshes, 262,144 triangles, 1920x1080, 2048^2 shadow map, 16 materials x 2048^2 maps, "
        "plain shading",
     2: "batched views: 1,048,576 triangles (106 meshes), 2560x1440 camera + 4 cascades 2048^2 (splits 5/15/40/100), "
@@ -788,6 +791,7 @@ def run_views(args, workload=None, emit=True, rooflines=False):
     fst = r.stats()
     exact = None
     blend_check = None
+    colour_err = None
     if args.check:
         import oracle
         rs = rs_used
@@ -802,6 +806,21 @@ def run_views(args, workload=None, emit=True, rooflines=False):
                 o, _ = oracle.raster_view(scene, cv.view_projection[:], cv.width, cv.height, False, rs)
             exact &= bool(np.array_equal(r.visibility(vi), o))
         log(f"visibility + shadow maps bit-exact vs oracle: {exact}")
+        if wl != 5 and args.msaa == 1:
+            # the measured frame's colour in every camera view against the FP64 oracle (PCF tie rule)
+            maps = [r.shadow_map(ci) for ci in range(len(cascs))]
+            params = None
+            if wl == 4:
+                lat, _ = r.neural_params()
+                params = oracle.NeuralParams(np.arange(scene.s.n_materials), lat.astype(np.float16).astype(np.float32),
+                                             r.neural_inference_mlp(), (psnr >= 40.0).astype(np.uint32))
+            colour_err = []
+            for vi, cv in enumerate(cams):
+                e, ties, away = colour_error(r.color(vi), r.shadow_factor(vi), lambda ov_sf: oracle.shade_view(
+                    scene, cv, cascs, maps, r.visibility(vi), override_shadow=ov_sf, neural=params))
+                colour_err.append({"view": f"{cv.width}x{cv.height}", "max_abs_err_vs_fp64": e, "pcf_ties": ties,
+                                   "pcf_differing_away_from_a_tie": away})
+            log(f"colour vs FP64 oracle: {colour_err}")
         if wl == 5:
             vis = r.visibility(0)
             maps = [r.shadow_map(0)]
@@ -863,7 +882,7 @@ def run_views(args, workload=None, emit=True, rooflines=False):
                         "blended_pass": round(float(st[4]), 5)},
            "tile_pairs": int(fst.tile_pairs), "triangles_setup": int(fst.triangles_setup),
            "triangles_clipped": int(fst.triangles_clipped), "decoder": dec, "bit_exact_vs_oracle": exact,
-           "blended": blend_check, "stage_rooflines": stage_rooflines}
+           "colour": colour_err, "blended": blend_check, "stage_rooflines": stage_rooflines}
     if emit:
         print(json.dumps(out), flush=True)
     r.close()
