@@ -225,6 +225,10 @@ typedef enum rsy_shade_mode {
     RSY_SHADE_NEURAL = 2  /* same BRDF; material maps decoded by the neural material decoder */
 } rsy_shade_mode;
 
+/* Views of one frame: its camera views, its cascades, and one blend view per camera view when the
+ * blended list is not empty.  A frame with more is refused with RSY_INVALID_ARGUMENT. */
+#define RSY_MAX_VIEWS 16u
+
 typedef struct rsy_frame_desc {
     uint32_t n_camera_views; /* 0 = one view from the scene data (view_projection, draw_extent) */
     uint32_t n_cascades;     /* 0 = one cascade from scene data shadow_projection_near */

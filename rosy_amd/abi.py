@@ -126,6 +126,7 @@ class FrameTargets(C.Structure):
 
 
 STATIC_SCRATCH_BYTES = 128 << 20
+MAX_VIEWS = 16
 
 
 class FrameStats(C.Structure):

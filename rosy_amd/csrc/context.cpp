@@ -49,6 +49,14 @@ rsy_result rebuild_lists(rsy_ctx* c) {
         d.vtx_count = m.num_vertices;
         d.object = uint32_t(obj);
         d.material = s.material_index < 0xFFFFFFFFull ? uint32_t(s.material_index) : 0xFFFFFFFFu;
+        if (d.material < c->h_materials.size()) {
+            const rsy_gpu_material& md = c->h_materials[d.material];
+            const uint32_t si[4] = {md.color_sampler_index, md.normal_sampler_index, md.metallic_sampler_index,
+                                    md.mixmap_sampler_index};
+            for (uint32_t k : si)
+                if (k < c->h_samplers.size() && (c->h_samplers[k].wrap_s == 0 || c->h_samplers[k].wrap_t == 0))
+                    d.flags |= kDrawClampSampler;
+        }
         for (int k = 0; k < 3; ++k) {
             d.bmin[k] = c->mesh_bounds[size_t(s.mesh_index) * 6 + k];
             d.bmax[k] = c->mesh_bounds[size_t(s.mesh_index) * 6 + 3 + k];
@@ -227,8 +235,8 @@ rsy_result plan_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint
         L.clip_total += g.clip_cap;
         L.big_total += g.big_cap;
         for (uint32_t d = 0; d < g.n_draws; ++d) {
-            for (uint32_t s = 0; s < list[d].vtx_count; s += 256) L.vblocks.push_back({vi, d, s, g.list});
-            for (uint32_t s = 0; s < list[d].tri_count; s += 256) L.tblocks.push_back({vi, d, s, g.list});
+            for (uint32_t s = 0; s < list[d].vtx_count; s += kTransformBlockVerts) L.vblocks.push_back({vi, d, s, g.list});
+            for (uint32_t s = 0; s < list[d].tri_count; s += kSetupBlockTris) L.tblocks.push_back({vi, d, s, g.list});
         }
         if (is_blend) ++L.n_blend;
         else if (f.shadow) ++L.n_shadow;
@@ -663,10 +671,11 @@ rsy_result ready_to_render(rsy_ctx* c) {
     return RSY_OK;
 }
 
-// Raster workgroups for frames of layout L after one with `items` work items: those, 1/8 more and
-// at least one per bin (items beyond the bins come only from crowded bins split in several).
-uint32_t raster_grid_for(const Layout& L, uint32_t items) {
-    return uint32_t(std::min<uint64_t>(std::max<uint64_t>(L.total_bins, uint64_t(items) + items / 8) + 64, 0xffffffffull));
+// Raster workgroups for the frames after one with `items` work items: those, a quarter more
+// and 256 (a moving view changes its non-empty bins gradually).  Workgroups past the frame's items
+// exit at once but still take dispatch slots, so the grid is not sized for every bin of the views.
+uint32_t raster_grid_for(uint32_t items) {
+    return uint32_t(std::min<uint64_t>(uint64_t(items) + items / 4 + 256, 0xffffffffull));
 }
 
 // Reads the last frame's counters (after a synchronize) and reports whether it overflowed.
@@ -691,7 +700,7 @@ bool collect_stats(rsy_ctx* c) {
     st.tile_size = L.tile;
     st.overflowed = overflow ? 1 : 0;
     // the raster grid of the next frames of this layout: this frame's work items and some slack
-    if (!overflow && totals[3]) c->layout.raster_grid = raster_grid_for(L, totals[3]);
+    if (!overflow && totals[3]) c->layout.raster_grid = raster_grid_for(totals[3]);
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c->ev_begin, c->ev_end) == hipSuccess) st.gpu_ms = ms;
     if (c->debug_flags & RSY_DEBUG_STAGE_TIMING) {
@@ -737,7 +746,7 @@ rsy_result finish_frame(rsy_ctx* c) {
         const bool list_over = (over_bits & 1u) != 0;
         if (over_bits & 2u) {  // more raster work items than workgroups: a grid for this frame's count
             c->layout.raster_grid =
-                raster_grid_for(c->layout, static_cast<const uint32_t*>(c->readback_counters.p)[nv * kCtrPerView + 3]);
+                raster_grid_for(static_cast<const uint32_t*>(c->readback_counters.p)[nv * kCtrPerView + 3]);
             invalidate_graph(c);
         }
         bool rec_over = false, frags_over = false;
@@ -1002,6 +1011,7 @@ rsy_result rsy_set_materials(rsy_ctx* c, const rsy_gpu_material* materials, uint
     c->h_materials.assign(materials, materials + n_materials);
     c->h_tex_table = table;
     c->h_samplers.assign(samplers, samplers + n_samplers);
+    c->lists_dirty = true;  // the draws' sampler flags follow the materials
     c->tex_opaque.assign(n_textures, 0);
     c->any_mips = false;
     std::vector<uint8_t> decoded;
@@ -1267,7 +1277,7 @@ rsy_result rsy_render_noalloc(rsy_ctx* c, const rsy_frame_desc* frame, const rsy
         for (size_t v = 0; v < nv; ++v) over |= ctr[v * kCtrPerView + kCtrOverflow] != 0;
         if (over) return fail(c, RSY_OVERFLOW, "the frame overflowed the working memory the static scratch holds");
         if (!(bits & 2u)) return RSY_OK;
-        L.raster_grid = raster_grid_for(L, ctr[nv * kCtrPerView + 3]);
+        L.raster_grid = raster_grid_for(ctr[nv * kCtrPerView + 3]);
     }
     return fail(c, RSY_OVERFLOW, "the frame's raster work items outnumber the workgroups");
 }

@@ -14,7 +14,7 @@
 namespace rsy {
 
 constexpr uint32_t kMaxViewExtent = 8192;
-constexpr uint32_t kMaxViews = 16;
+constexpr uint32_t kMaxViews = RSY_MAX_VIEWS;
 
 struct DevBuf {
     void* p = nullptr;

@@ -6,6 +6,17 @@
 
 namespace rsy {
 
+// Global-memory view of a device pointer that was read from memory (a record or LDS).  The compiler
+// cannot tell such a pointer from an LDS or scratch one and would use flat instructions, whose
+// waits also hold back the scalar loads issued beside them.
+// (The host pass only type-checks device code, and sees plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T> using GlobalPtr = __attribute__((address_space(1))) T*;
+#else
+template <class T> using GlobalPtr = T*;
+#endif
+template <class T> __device__ inline GlobalPtr<T> gmem(T* p) { return (GlobalPtr<T>)p; }
+
 // One triangle after clip, snap, cull and setup: 48 bytes, three 16-byte loads.
 struct alignas(16) TriRec {
     int32_t X0, Y0, X1, Y1;  // snapped vertices, 8 sub-pixel bits
@@ -49,7 +60,11 @@ struct DrawGPU {
     uint32_t object;       // graphic_objects_offset + graphics_object_index
     uint32_t material;     // material index or 0xFFFFFFFF
     float bmin[3], bmax[3];  // object-space box of the mesh's vertices (view culling)
+    uint32_t flags;        // kDraw* bits of the draw's material, resolved on the host
 };
+// DrawGPU::flags: some sampler of the material clamps to the edge (texture coordinates are then
+// used as they are; REPEAT and MIRRORED_REPEAT samplers take them with whole periods removed).
+constexpr uint32_t kDrawClampSampler = 1u;
 
 // Per-view counters: records appended by the clipper, queued clipped triangles, entries of the
 // big-pair list, overflow flags, clipped-triangle statistic.
@@ -85,7 +100,7 @@ struct ViewGPU {
     float view_projection[16];
 };
 
-struct BlockEntry {  // work of one 256-thread block of the transform / setup kernels
+struct BlockEntry {  // work of one block of the transform or setup kernel (its block size of items from start)
     uint32_t view, draw, start;
     uint32_t list;  // the view's draw list (so the draw record loads beside the view record)
 };

@@ -5,8 +5,16 @@
 #include <cstdint>
 
 #include "frame_types.h"
+#include "rosy_amd.h"
 
 namespace rsy {
+
+// Work of one 256-thread transform or setup block (a BlockEntry): two vertices per thread, or one
+// triangle (setup moves about 90 bytes per triangle and keeps HBM busy at one per thread; two per
+// thread measured slower).
+constexpr uint32_t kTransformPerThread = 2;
+constexpr uint32_t kTransformBlockVerts = 256 * kTransformPerThread;
+constexpr uint32_t kSetupBlockTris = 256;
 
 // The three draw lists of a frame, indexed by ViewGPU::list.
 struct DrawLists {
@@ -124,6 +132,8 @@ struct ShadeView {
     float view_projection[16];
     float camera_position[4];
 };
+
+constexpr uint32_t kMaxShadeCascades = RSY_MAX_VIEWS;  // the shading kernels keep a frame's cascades in LDS
 
 struct ShadeCascade {
     uint32_t size;

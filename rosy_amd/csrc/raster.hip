@@ -177,12 +177,13 @@ __device__ inline uint32_t corner_outside(const float* m, const float* bmin, con
     return bits;
 }
 
-// clip = (VP * W) * p per (view, vertex).  Every block covers vertices of one (view, draw); its
-// MVP = views[v].view_projection * objects[draw.object].transform is formed once in LDS by the fmaf
-// chain in k order, while each thread's vertex loads from the SoA x/y/z streams (coalesced 12 B)
-// are in flight; then each thread writes the clip position (coalesced 16 B).  A draw whose box is
-// outside the view is skipped (eight lanes test one box corner each), and its first block records
-// that for k_setup.
+// clip = (VP * W) * p per (view, vertex).  Every block covers kTransformBlockVerts vertices of one
+// (view, draw), kTransformPerThread per thread (their loads issued together: half the blocks, each
+// waiting once for its draw, object and positions).  The block's MVP = views[v].view_projection *
+// objects[draw.object].transform is formed once in LDS by the fmaf chain in k order, while the
+// vertex loads from the SoA x/y/z streams (coalesced 12 B) are in flight; then each vertex's clip
+// position is written (coalesced 16 B).  A draw whose box is outside the view is skipped (eight
+// lanes test one box corner each), and its first block records that for k_setup.
 // Also zeroes the frame's counters and bin counts (`clear` words at `clear_a` then at `clear_b`), spread
 // over the grid, before the setup kernel counts into them.
 __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, const ViewGPU* views, DrawLists lists,
@@ -198,14 +199,18 @@ __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, con
     const BlockEntry e = blocks[blockIdx.x];
     const ViewGPU& V = views[e.view];
     const DrawGPU& dr = lists.of(e.list)[e.draw];
-    const uint32_t v = e.start + threadIdx.x;
-    const bool has_v = v < dr.vtx_count;
-    float x = 0.f, y = 0.f, z = 0.f;
-    if (has_v) {
-        const uint32_t g = dr.vtx_base + v;
-        x = px[g];
-        y = py[g];
-        z = pz[g];
+    constexpr int R = int(kTransformPerThread);
+    float x[R], y[R], z[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const uint32_t v = e.start + uint32_t(j) * 256u + threadIdx.x;
+        x[j] = y[j] = z[j] = 0.f;
+        if (v < dr.vtx_count) {
+            const uint32_t g = dr.vtx_base + v;
+            x[j] = px[g];
+            y[j] = py[g];
+            z[j] = pz[g];
+        }
     }
     if (threadIdx.x < 16) {
         const int col = threadIdx.x >> 2, row = threadIdx.x & 3;
@@ -230,8 +235,13 @@ __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, con
         }
     }
     __syncthreads();
-    if (culled || !has_v) return;
-    xv[size_t(V.xv_base) + dr.xv_off + v] = transform(m, x, y, z);
+    if (culled) return;
+    float4* out = xv + size_t(V.xv_base) + dr.xv_off;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const uint32_t v = e.start + uint32_t(j) * 256u + threadIdx.x;
+        if (v < dr.vtx_count) out[v] = transform(m, x[j], y[j], z[j]);
+    }
 }
 
 // Bins of a record with at most kSmallBins bins go through the block's LDS table in k_setup; the
@@ -281,9 +291,10 @@ __device__ void bin_wide(bool active, const TriRec& r, uint32_t ri, const ViewGP
     }
 }
 
-// LDS bin table of one setup block: at most 256 x kSmallBins = 1024 distinct keys, so a probe always
-// finds its key or an empty entry.
-constexpr uint32_t kHash = 1024;
+// LDS bin table of one setup block: at most kSetupBlockTris x kSmallBins = 1024 distinct keys, so a
+// probe always finds its key or an empty entry.
+constexpr uint32_t kHash = kSetupBlockTris * kSmallBins;
+static_assert(kHash == 1024u, "10-bit hash below");
 constexpr uint32_t kEmpty = 0xffffffffu;
 
 __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const ViewGPU* views, DrawLists lists,
@@ -786,7 +797,9 @@ __global__ __launch_bounds__(256) void k_raster(const ViewGPU* views, uint32_t n
     // beside the count (blockIdx.x < gridDim.x <= the table's capacity).
     const uint32_t n_items = totals[3];
     const uint4 it = items[blockIdx.x];
-    if (blockIdx.x >= n_items) return;
+    // (the test reads the item too, which keeps its load beside the count's instead of after the
+    // branch; an item's record count never reaches 2^31)
+    if (blockIdx.x >= n_items || int32_t(it.z) < 0) return;
     // this thread's record of the item: the list entry, then the record itself as three 16-byte
     // words (X0 Y0 X1 Y1 | X2 Y2 z0 dzdx | dzdy id bbx bby); the view lookup and the tile clear
     // overlap these loads

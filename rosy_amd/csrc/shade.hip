@@ -56,7 +56,7 @@ __device__ inline f3 xyz(float4 v) { return {v.x, v.y, v.z}; }
 
 __device__ inline bool valid_index(uint32_t i) { return i < 4294967295u; }
 
-__device__ inline float cmp_tap(const float* map, int S, float u, float v, float ref) {
+__device__ inline float cmp_tap(GlobalPtr<const float> map, int S, float u, float v, float ref) {
     const float x = u * float(S) - 0.5f, y = v * float(S) - 0.5f;
     const float fx0 = floorf(fminf(fmaxf(x, -2.0f), float(S) + 1.0f));
     const float fy0 = floorf(fminf(fmaxf(y, -2.0f), float(S) + 1.0f));
@@ -68,6 +68,40 @@ __device__ inline float cmp_tap(const float* map, int S, float u, float v, float
     const float t01 = map[size_t(yb) * S + xa], t11 = map[size_t(yb) * S + xb];
     return (1 - ax) * (1 - ay) * (ref >= t00 ? 1.f : 0.f) + ax * (1 - ay) * (ref >= t10 ? 1.f : 0.f) +
            (1 - ax) * ay * (ref >= t01 ? 1.f : 0.f) + ax * ay * (ref >= t11 ? 1.f : 0.f);
+}
+
+// Frame constants of a shading launch.  The scene data is read through the constant address space
+// (scalar loads, shared by the waves of a CU).  The sRGB table and the cascades (selected per
+// pixel) are copied into LDS at the start of each workgroup: shared_request issues the loads (one
+// word or two per thread, beside the workgroup's first visibility loads) and shared_store writes
+// them; the caller synchronises before the first use.
+using SceneP = const __attribute__((address_space(4))) rsy_scene_data*;
+struct ShadeShared {
+    float lut[256];
+    ShadeCascade casc[kMaxShadeCascades];
+};
+constexpr uint32_t kCascadeWords = uint32_t(sizeof(ShadeCascade) / 4);
+static_assert(kMaxShadeCascades * kCascadeWords <= 512, "one or two words per thread");
+struct SharedWords {
+    float lut;
+    uint32_t c0, c1;
+};
+__device__ inline SharedWords shared_request(const ShadeLaunch& L) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t cw = min(L.n_cascades, kMaxShadeCascades) * kCascadeWords;
+    const uint32_t* cs = reinterpret_cast<const uint32_t*>(L.cascades);
+    SharedWords w;
+    w.lut = L.srgb_lut[t];
+    w.c0 = t < cw ? cs[t] : 0u;
+    w.c1 = t + 256 < cw ? cs[t + 256] : 0u;
+    return w;
+}
+__device__ inline void shared_store(const ShadeLaunch& L, ShadeShared& S, const SharedWords& w) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t cw = min(L.n_cascades, kMaxShadeCascades) * kCascadeWords;
+    S.lut[t] = w.lut;
+    if (t < cw) reinterpret_cast<uint32_t*>(S.casc)[t] = w.c0;
+    if (t + 256 < cw) reinterpret_cast<uint32_t*>(S.casc)[t + 256] = w.c1;
 }
 
 // Varyings of basicVertex (data.slang:10-22) plus csm.near, interpolated at the pixel.
@@ -85,12 +119,12 @@ struct Varyings {
 // formed once from the interpolated position, normal, tangent, colour and uv: the same values as
 // running vertexMain per vertex and interpolating its outputs.
 template <class GP>
-__device__ Varyings pixel_varyings(const rsy_scene_data& sd, const rsy_position* const v[3], GP gd,
+__device__ Varyings pixel_varyings(SceneP sd, const rsy_position* const v[3], GP gd,
                                    bool has_material, uint32_t normal_index, const float* cascade_m, float w0,
                                    float w1, float w2, bool shift_uv) {
     Varyings o;
     const float w[3] = {w0, w1, w2};
-    bool tangent_space = sd.tangent_space_enabled == 1;
+    bool tangent_space = sd->tangent_space_enabled == 1;
     if (has_material && !valid_index(normal_index)) tangent_space = false;
     const GP W = gd;
     const GP toObj = gd + 16;
@@ -98,39 +132,48 @@ __device__ Varyings pixel_varyings(const rsy_scene_data& sd, const rsy_position*
     f3 P = mk(0.f, 0.f, 0.f), N = P, Tn = P, B = P, Nw = P;
     float sigma = 0.f;
     float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+    // the three vertex records (64 bytes each) are requested together before any of them is used
+    float4 raw[3][4];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const rsy_position& vin = *v[k];
-        float4 nt = make_float4(vin.tangents[0], vin.tangents[1], vin.tangents[2], vin.tangents[3]);
-        if (sd.ignore_asset_tangent_sign == 1) nt.w = 1.f;
-        if (sd.flip_tangents[0] == 1) nt.x = -nt.x;
-        if (sd.flip_tangents[1] == 1) nt.y = -nt.y;
-        if (sd.flip_tangents[2] == 1) nt.z = -nt.z;
-        if (sd.flip_tangents[3] == 1) nt.w = -nt.w;
-        const f3 n = mk(vin.normal[0], vin.normal[1], vin.normal[2]);
+        const float4* p = reinterpret_cast<const float4*>(v[k]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) raw[k][q] = p[q];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        // vertex xyz, normal xyz, tangent xyzw, colour rgba, uv (the rsy_position layout)
+        const float4 r0 = raw[k][0], r1 = raw[k][1], r2 = raw[k][2], r3 = raw[k][3];
+        float4 nt = make_float4(r1.z, r1.w, r2.x, r2.y);
+        if (sd->ignore_asset_tangent_sign == 1) nt.w = 1.f;
+        if (sd->flip_tangents[0] == 1) nt.x = -nt.x;
+        if (sd->flip_tangents[1] == 1) nt.y = -nt.y;
+        if (sd->flip_tangents[2] == 1) nt.z = -nt.z;
+        if (sd->flip_tangents[3] == 1) nt.w = -nt.w;
+        const f3 n = mk(r0.w, r1.x, r1.y);
         const f3 t = mk(nt.x, nt.y, nt.z);
         f3 bt = cross(n, t) * nt.w;
-        if (sd.ensure_orthogonal_bitangent == 1) {
+        if (sd->ensure_orthogonal_bitangent == 1) {
             const f3 np = bt * dot(n, bt);
             const f3 on = normalize(n - np);
             const f3 tp = np - bt * dot(t, bt);
             const f3 ot = normalize(t - tp);
             bt = cross(on, ot) * nt.w;
         }
-        P = P + mk(vin.vertex[0], vin.vertex[1], vin.vertex[2]) * w[k];
+        P = P + mk(r0.x, r0.y, r0.z) * w[k];
         N = N + n * w[k];
         Tn = Tn + t * w[k];
         B = B + bt * w[k];
         sigma += nt.w * w[k];
-        col.x += vin.color[0] * w[k];
-        col.y += vin.color[1] * w[k];
-        col.z += vin.color[2] * w[k];
-        col.w += vin.color[3] * w[k];
+        col.x += r2.z * w[k];
+        col.y += r2.w * w[k];
+        col.z += r3.x * w[k];
+        col.w += r3.y * w[k];
         if (!tangent_space) Nw = Nw + normalize(mul3(nmat, n)) * w[k];
     }
     const float4 world = mul4(W, make_float4(P.x, P.y, P.z, 1.f));
-    const float4 cam = make_float4(sd.camera_position[0], sd.camera_position[1], sd.camera_position[2], sd.camera_position[3]);
-    const float4 sun = make_float4(sd.sun_position[0], sd.sun_position[1], sd.sun_position[2], sd.sun_position[3]);
+    const float4 cam = make_float4(sd->camera_position[0], sd->camera_position[1], sd->camera_position[2], sd->camera_position[3]);
+    const float4 sun = make_float4(sd->sun_position[0], sd->sun_position[1], sd->sun_position[2], sd->sun_position[3]);
     if (tangent_space) {
         o.normal = N;
         o.view_dir = xyz(mul4(toObj, cam)) - P;
@@ -140,9 +183,9 @@ __device__ Varyings pixel_varyings(const rsy_scene_data& sd, const rsy_position*
         o.view_dir = xyz(cam) - xyz(world);
         o.light_dir = xyz(sun) - xyz(world);
     }
-    if (sd.flip_lights[0] == 1) o.light_dir.x = -o.light_dir.x;
-    if (sd.flip_lights[1] == 1) o.light_dir.y = -o.light_dir.y;
-    if (sd.flip_lights[2] == 1) o.light_dir.z = -o.light_dir.z;
+    if (sd->flip_lights[0] == 1) o.light_dir.x = -o.light_dir.x;
+    if (sd->flip_lights[1] == 1) o.light_dir.y = -o.light_dir.y;
+    if (sd->flip_lights[2] == 1) o.light_dir.z = -o.light_dir.z;
     o.world_normal = mul3(nmat, N);
     o.tangent = Tn;
     o.bitangent = B;
@@ -151,28 +194,17 @@ __device__ Varyings pixel_varyings(const rsy_scene_data& sd, const rsy_position*
     // Texture coordinates relative to vertex 0 with a whole even number of periods removed: exact
     // for REPEAT and MIRRORED_REPEAT and keeps |uv| small, so the FP32 texel address keeps its
     // sub-texel precision at large uv.  Clamp-to-edge samplers keep the raw value.
-    const float u0 = v[0]->texture_coordinates[0], v0c = v[0]->texture_coordinates[1];
+    const float u0 = raw[0][3].z, v0c = raw[0][3].w;
     o.su = shift_uv ? 2.0f * floorf(u0 * 0.5f) : 0.f;
     o.sv = shift_uv ? 2.0f * floorf(v0c * 0.5f) : 0.f;
-    o.u = (u0 - o.su) + w1 * (v[1]->texture_coordinates[0] - u0) + w2 * (v[2]->texture_coordinates[0] - u0);
-    o.v = (v0c - o.sv) + w1 * (v[1]->texture_coordinates[1] - v0c) + w2 * (v[2]->texture_coordinates[1] - v0c);
+    o.u = (u0 - o.su) + w1 * (raw[1][3].z - u0) + w2 * (raw[2][3].z - u0);
+    o.v = (v0c - o.sv) + w1 * (raw[1][3].w - v0c) + w2 * (raw[2][3].w - v0c);
     o.csm = make_float4(0.f, 0.f, 0.f, 0.f);
     if (cascade_m) {
         const float4 s4 = mul4(cascade_m, world);
         o.csm = make_float4(0.5f * s4.x + 0.5f * s4.w, 0.5f * s4.y + 0.5f * s4.w, s4.z, s4.w);  // ndc_to_tc
     }
     return o;
-}
-
-// True when any sampler of the material clamps (the uv period shift is then not an identity).
-template <class MP, class SP>
-__device__ inline bool uses_clamp(MP md, SP samplers, uint32_t n_samplers) {
-    const uint32_t idx[4] = {md->color_sampler_index, md->normal_sampler_index, md->metallic_sampler_index,
-                             md->mixmap_sampler_index};
-    bool clamp = false;
-    for (int k = 0; k < 4; ++k)
-        if (idx[k] < n_samplers) clamp |= samplers[idx[k]].wrap_s == 0 || samplers[idx[k]].wrap_t == 0;
-    return clamp;
 }
 
 __device__ inline void mr_from(int channels, float4 s, float& rough, float& metal) {
@@ -216,18 +248,18 @@ struct PixelGeom {
 // kLod: also the barycentrics' screen derivatives (for the texture level of detail), exact for the
 // same rational form: lam = n / S with n linear over w, so d(lam) = (dn - lam dS) / S.
 template <bool kLod = false>
-__device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView& V, uint32_t x, uint32_t y,
-                                           unsigned long long vis) {
+__device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeShared& S, const ShadeView& V, uint32_t x,
+                                           uint32_t y, unsigned long long vis) {
     PixelGeom pg;
     const uint32_t tri = uint32_t(vis & 0xffffffffull) - 1u;
     // the triangle's shading record (vertices, draw, clip w) and its raster record, loaded together
-    // (a clipped triangle's raster slot holds stale data, unused)
-    const uint4* sr = reinterpret_cast<const uint4*>(V.shade_recs + tri);
+    // (a clipped triangle's raster slot holds stale data, computed with and then replaced)
+    const GlobalPtr<const uint4> sr = gmem(reinterpret_cast<const uint4*>(V.shade_recs + tri));
     const uint4 sa = sr[0], sb = sr[1];
     const uint4* rq = reinterpret_cast<const uint4*>(V.recs + tri);
-    const uint4 q0 = rq[0];  // X0, Y0, X1, Y1
-    const uint2 q1 = *reinterpret_cast<const uint2*>(rq + 1);  // X2, Y2
-    const uint32_t id = reinterpret_cast<const uint32_t*>(rq + 2)[1];
+    const uint4 q0 = gmem(rq)[0];  // X0, Y0, X1, Y1
+    const uint2 q1 = *gmem(reinterpret_cast<const uint2*>(rq + 1));  // X2, Y2
+    const uint32_t id = gmem(reinterpret_cast<const uint32_t*>(rq + 2))[1];
     pg.draw = sa.w & ~kShadeNoRecord;
     pg.material = sb.w;
     pg.v0 = sa.x;
@@ -240,9 +272,11 @@ __device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView
     // snapped vertices the raster pass covered, each over its vertex's w (as e_k * w_i * w_j,
     // normalised below).  The record may hold vertices 1 and 2 exchanged, which negates all three
     // edge functions and swaps the last two.  A clipped triangle's slot is empty; its barycentrics
-    // come from b ~ cross(x - u w, y - v w) over the clip positions.
+    // come from b ~ cross(x - u w, y - v w) over the clip positions.  The edge form is evaluated for
+    // every pixel (on a clipped triangle's stale slot too, then replaced), so that the raster
+    // record is requested beside the shading record rather than after it.
     f3 lam;
-    if ((sa.w & kShadeNoRecord) == 0u) {
+    {
         const int32_t X0 = int32_t(q0.x), Y0 = int32_t(q0.y), X1 = int32_t(q0.z), Y1 = int32_t(q0.w);
         const int32_t X2 = int32_t(q1.x), Y2 = int32_t(q1.y);
         const int32_t Px = int32_t(x) * 256 + 128, Py = int32_t(y) * 256 + 128;
@@ -260,7 +294,8 @@ __device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView
             pg.dlx = mk(ex0 * (w1 * w2), (swapped ? ex2 : ex1) * (w0 * w2), (swapped ? ex1 : ex2) * (w0 * w1));
             pg.dly = mk(ey0 * (w1 * w2), (swapped ? ey2 : ey1) * (w0 * w2), (swapped ? ey1 : ey2) * (w0 * w1));
         }
-    } else {
+    }
+    if ((sa.w & kShadeNoRecord) != 0u) {
         const DrawGPU dr = L.draws[pg.draw];
         const float4* xv = L.xv + V.xv_base + dr.xv_off;
         const float4 c0 = xv[pg.v0 - dr.vtx_base], c1 = xv[pg.v1 - dr.vtx_base], c2 = xv[pg.v2 - dr.vtx_base];
@@ -285,7 +320,7 @@ __device__ inline PixelGeom pixel_geometry(const ShadeLaunch& L, const ShadeView
     const float view_depth = w0 * lam.x + w1 * lam.y + w2 * lam.z;
     int cascade = -1;
     for (uint32_t k = 0; k < L.n_cascades; ++k)
-        if (view_depth <= L.cascades[k].split_far) {
+        if (view_depth <= S.casc[k].split_far) {
             cascade = int(k);
             break;
         }
@@ -330,10 +365,11 @@ struct Fragment {
 // decoder's outputs `dec` replace the albedo, normal and metallic-roughness samples of the plain
 // textures.  kUniformDraw: every live lane of the wave shades the same draw.
 template <bool kUniformDraw, bool kLod = false>
-__device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V, const float* lut, size_t o,
-                                   const PixelGeom& pg, bool use_dec, const float (&dec)[kNmOut]) {
+__device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeShared& S, const PixelGeom& pg, bool use_dec,
+                                       const float (&dec)[kNmOut]) {
     using R = std::conditional_t<kUniformDraw, WaveRecords, LaneRecords>;
-    const rsy_scene_data& sd = *static_cast<const rsy_scene_data*>(L.scene);
+    const SceneP sd = (SceneP)(L.scene);
+    const float* lut = S.lut;
     const uint32_t draw = kUniformDraw ? uint32_t(__builtin_amdgcn_readfirstlane(int(pg.draw))) : pg.draw;
     const DrawGPU dr = R::load(R::ptr(L.draws) + draw);
     const auto gd = R::ptr(L.objects) + size_t(dr.object) * 41u;
@@ -345,10 +381,10 @@ __device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V,
     const auto textures = R::ptr(L.textures);
     const f3 lam = pg.lam;
     const int cascade = pg.cascade;
-    const float* cm = cascade >= 0 ? L.cascades[cascade].m : nullptr;
+    const float* cm = cascade >= 0 ? S.casc[cascade].m : nullptr;
     const uint32_t nti = has_material ? md->normal_sampled_image_index : RSY_ABSENT;
     const rsy_position* const vtx[3] = {&verts[pg.v0], &verts[pg.v1], &verts[pg.v2]};
-    const bool shift_uv = has_material && !uses_clamp(md, samplers, L.n_samplers);
+    const bool shift_uv = has_material && !(dr.flags & kDrawClampSampler);
     const Varyings bv = pixel_varyings(sd, vtx, gd, has_material, nti, cm, lam.x, lam.y, lam.z, shift_uv);
     const f3 bn = bv.normal, bwn = bv.world_normal, bview = bv.view_dir, blight = bv.light_dir;
     const f3 btan = bv.tangent, bbit = bv.bitangent;
@@ -373,7 +409,7 @@ __device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V,
     };
     const float4 csm = bv.csm;
 
-    const f3 ambient = mk(sd.ambient_color[0], sd.ambient_color[1], sd.ambient_color[2]);
+    const f3 ambient = mk(sd->ambient_color[0], sd->ambient_color[1], sd->ambient_color[2]);
     float4 out = bcol;
     float4 image = out;
     float alpha = bcol.w;
@@ -385,7 +421,7 @@ __device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V,
         image = out;
         alpha = fminf(md->color[3], alpha);
         if (use_dec) {
-            if (sd.normal_maps_enabled == 1) {
+            if (sd->normal_maps_enabled == 1) {
                 f3 m = mk(dec[3] * 2.f - 1.f, dec[4] * 2.f - 1.f, 0.f);
                 m.z = __builtin_amdgcn_sqrtf(fmaxf(0.f, 1.f - m.x * m.x - m.y * m.y));  // two-channel normal map
                 const f3 n = normalize(bn);
@@ -398,7 +434,7 @@ __device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V,
             smetal = dec[5];
             srough = dec[6];
         } else {
-            if (sd.normal_maps_enabled == 1 && valid_index(nti) && nti < L.n_textures) {
+            if (sd->normal_maps_enabled == 1 && valid_index(nti) && nti < L.n_textures) {
                 const TexGPU tg = R::load(textures + nti);
                 const uint32_t si = md->normal_sampler_index;
                 const SamplerState smp = si < L.n_samplers ? sampler_state(R::load(samplers + si)) : SamplerState{};
@@ -438,7 +474,7 @@ __device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V,
     const f3 half = mk(0.5f, 0.5f, 0.5f);
     f3 c3;
     bool set = true;
-    switch (sd.fragment_output) {
+    switch (sd->fragment_output) {
         case 0: c3 = mk(image.x, image.y, image.z); break;
         case 1: c3 = normalize(bwn) * 0.5f + half; break;
         case 2: c3 = normalize(btan) * 0.5f + half; break;
@@ -459,29 +495,30 @@ __device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V,
     }
     if (set) out = make_float4(c3.x, c3.y, c3.z, alpha);
     float shadow = 1.f;
-    if (sd.light_enabled > 0) {
-        if (sd.shadows_enabled == 1 && cascade >= 0) {
-            const ShadeCascade& C = L.cascades[cascade];
-            const int S = int(C.size);
+    if (sd->light_enabled > 0) {
+        if (sd->shadows_enabled == 1 && cascade >= 0) {
+            const ShadeCascade& C = S.casc[cascade];
+            const GlobalPtr<const float> map = gmem(C.map);
+            const int Sz = int(C.size);
             if (L.pcf_mode == RSY_PCF_GRID_3X3) {
                 float r = 0.f;
-                const float inv = 1.0f / float(S);
+                const float inv = 1.0f / float(Sz);
                 for (int j = -1; j <= 1; ++j)
-                    for (int i = -1; i <= 1; ++i) r += cmp_tap(C.map, S, csm.x + float(i) * inv, csm.y + float(j) * inv, csm.z);
+                    for (int i = -1; i <= 1; ++i) r += cmp_tap(map, Sz, csm.x + float(i) * inv, csm.y + float(j) * inv, csm.z);
                 shadow = r * (1.0f / 9.0f);
             } else {
                 const float so = 0.00025f;
                 float r = 0.f;
-                r += cmp_tap(C.map, S, csm.x - so, csm.y - so, csm.z);
-                r += cmp_tap(C.map, S, csm.x + so * 2.f, csm.y, csm.z);
-                r += cmp_tap(C.map, S, csm.x, csm.y + so * 2.f, csm.z);
-                r += cmp_tap(C.map, S, csm.x + so * 2.f, csm.y + so * 2.f, csm.z);
-                r += cmp_tap(C.map, S, csm.x - so * 2.f, csm.y - so * 2.f, csm.z);
+                r += cmp_tap(map, Sz, csm.x - so, csm.y - so, csm.z);
+                r += cmp_tap(map, Sz, csm.x + so * 2.f, csm.y, csm.z);
+                r += cmp_tap(map, Sz, csm.x, csm.y + so * 2.f, csm.z);
+                r += cmp_tap(map, Sz, csm.x + so * 2.f, csm.y + so * 2.f, csm.z);
+                r += cmp_tap(map, Sz, csm.x - so * 2.f, csm.y - so * 2.f, csm.z);
                 shadow = r * (1.0f / 5.0f);
             }
         }
-        const f3 sun_col = mk(sd.sunlight_color[0], sd.sunlight_color[1], sd.sunlight_color[2]);
-        if (sd.brdf_lighting_enabled == 0) {
+        const f3 sun_col = mk(sd->sunlight_color[0], sd->sunlight_color[1], sd->sunlight_color[2]);
+        if (sd->brdf_lighting_enabled == 0) {
             const f3 lit = sun_col * fmaxf(cos_theta, 0.f) * shadow;
             out = make_float4(out.x * fmaxf(ambient.x, lit.x), out.y * fmaxf(ambient.y, lit.y),
                               out.z * fmaxf(ambient.z, lit.z), out.w * alpha);
@@ -497,7 +534,7 @@ __device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V,
             const f3 kD = (mk(1.f, 1.f, 1.f) - F) * (1.f - metallic);
             const f3 spec = F * (ndf * g * __builtin_amdgcn_rcpf(4.f * n_dot_v * l_dot_n + 0.0001f));
             const f3 diffuse = albedo * (1.0f / 3.14159265f);
-            const f3 sun_light = sun_col * sd.light_intensity;
+            const f3 sun_light = sun_col * sd->light_intensity;
             const f3 brdf = (kD * diffuse + spec) * sun_light * l_dot_n;
             const f3 amb = ambient * diffuse;
             const f3 sh = brdf * shadow;
@@ -513,8 +550,8 @@ __device__ inline Fragment shade_pixel(const ShadeLaunch& L, const ShadeView& V,
 }
 
 __device__ inline void store_fragment(const ShadeView& V, size_t o, const Fragment& f) {
-    V.color[o] = f.color;
-    if (V.shadow_factor) V.shadow_factor[o] = f.shadow;
+    gmem(V.color)[o] = f.color;
+    if (V.shadow_factor) gmem(V.shadow_factor)[o] = f.shadow;
 }
 
 
@@ -528,8 +565,8 @@ __device__ inline bool wave_uniform(uint32_t v) {
 
 template <bool kNeural, bool kLod>
 __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
-    __shared__ float lut[256];
-    lut[threadIdx.x] = L.srgb_lut[threadIdx.x];
+    __shared__ ShadeShared S;
+    const SharedWords sw = shared_request(L);
 
     uint32_t vi = 0;
     while (vi + 1 < L.n_views && L.views[vi + 1].block_base <= blockIdx.x) ++vi;
@@ -539,21 +576,22 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
     const uint32_t y = (lb / V.blocks_x) * 16 + (threadIdx.x >> 4);
     const bool inside = x < V.width && y < V.height;
     const size_t o = size_t(y) * V.width + x;
-    // the pixel's visibility word is requested before the wait for the table
-    const unsigned long long vis_word = inside ? V.vis[o] : 0ull;
+    // the pixel's visibility word is requested beside the workgroup's constants
+    const unsigned long long vis_word = inside ? gmem(V.vis)[o] : 0ull;
+    shared_store(L, S, sw);
     __syncthreads();
     if constexpr (!kNeural) {
         if (!inside) return;
         const unsigned long long vis = vis_word;
         if (vis == 0ull) {
-            V.color[o] = make_float4(L.clear_color[0], L.clear_color[1], L.clear_color[2], L.clear_color[3]);
-            if (V.shadow_factor) V.shadow_factor[o] = 1.f;
+            gmem(V.color)[o] = make_float4(L.clear_color[0], L.clear_color[1], L.clear_color[2], L.clear_color[3]);
+            if (V.shadow_factor) gmem(V.shadow_factor)[o] = 1.f;
             return;
         }
         const float none[kNmOut] = {};
-        const PixelGeom pg = pixel_geometry<kLod>(L, V, x, y, vis);
-        store_fragment(V, o, wave_uniform(pg.draw) ? shade_pixel<true, kLod>(L, V, lut, o, pg, false, none)
-                                                   : shade_pixel<false, kLod>(L, V, lut, o, pg, false, none));
+        const PixelGeom pg = pixel_geometry<kLod>(L, S, V, x, y, vis);
+        store_fragment(V, o, wave_uniform(pg.draw) ? shade_pixel<true, kLod>(L, S, pg, false, none)
+                                                   : shade_pixel<false, kLod>(L, S, pg, false, none));
     } else {
         // Every lane stays live up to the wave-wide decoder MLP; lanes without a decoded material
         // feed zero features and ignore the result.
@@ -562,7 +600,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
         bool decode = false;
         float f[kNmFeat] = {};
         if (vis != 0ull) {
-            pg = pixel_geometry<kLod>(L, V, x, y, vis);
+            pg = pixel_geometry<kLod>(L, S, V, x, y, vis);
             // the material's decoder flag and latent slot load beside the texture coordinates
             const uint32_t mat = pg.material;
             const bool known = mat < L.n_materials;
@@ -584,12 +622,12 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
         if (__builtin_amdgcn_ballot_w64(decode) != 0ull) nm_mlp_wave(static_cast<const NmWeights*>(L.nm_weights), f, dec);
         if (!inside) return;
         if (vis == 0ull) {
-            V.color[o] = make_float4(L.clear_color[0], L.clear_color[1], L.clear_color[2], L.clear_color[3]);
-            if (V.shadow_factor) V.shadow_factor[o] = 1.f;
+            gmem(V.color)[o] = make_float4(L.clear_color[0], L.clear_color[1], L.clear_color[2], L.clear_color[3]);
+            if (V.shadow_factor) gmem(V.shadow_factor)[o] = 1.f;
             return;
         }
-        store_fragment(V, o, wave_uniform(pg.draw) ? shade_pixel<true, kLod>(L, V, lut, o, pg, decode, dec)
-                                                   : shade_pixel<false, kLod>(L, V, lut, o, pg, decode, dec));
+        store_fragment(V, o, wave_uniform(pg.draw) ? shade_pixel<true, kLod>(L, S, pg, decode, dec)
+                                                   : shade_pixel<false, kLod>(L, S, pg, decode, dec));
     }
 }
 
@@ -603,13 +641,13 @@ __global__ __launch_bounds__(256) void k_shade(ShadeLaunch L) {
 // that of sample 0's primitive.
 template <bool kNeural, bool kLod>
 __global__ __launch_bounds__(256) void k_shade_ms(ShadeLaunch L) {
-    __shared__ float lut[256];
+    __shared__ ShadeShared S;
     __shared__ uint16_t jobs[1024];       // pixel (thread) << 2 | first sample of the primitive
     __shared__ float4 job_color[1024];
     __shared__ float shadow0[256];
     __shared__ uint32_t keys[4][256];     // the pixel's sample keys, kept for the resolve
     __shared__ uint32_t scan_lds[4];
-    lut[threadIdx.x] = L.srgb_lut[threadIdx.x];
+    const SharedWords sw = shared_request(L);
 
     uint32_t vi = 0;
     while (vi + 1 < L.n_views && L.views[vi + 1].block_base <= blockIdx.x) ++vi;
@@ -621,7 +659,8 @@ __global__ __launch_bounds__(256) void k_shade_ms(ShadeLaunch L) {
     const size_t o = size_t(y) * V.width + x, plane = size_t(V.width) * V.height;
     unsigned long long v[4];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) v[s] = inside ? V.vis[s * plane + o] : 0ull;
+    for (int s = 0; s < 4; ++s) v[s] = inside ? gmem(V.vis)[s * plane + o] : 0ull;
+    shared_store(L, S, sw);
     const uint32_t k0 = uint32_t(v[0]), k1 = uint32_t(v[1]), k2 = uint32_t(v[2]), k3 = uint32_t(v[3]);
     // bit s: sample s is the first to hold its primitive (or the background)
     const uint32_t todo = inside ? 1u | (k1 != k0 ? 2u : 0u) | (k2 != k0 && k2 != k1 ? 4u : 0u) |
@@ -650,10 +689,10 @@ __global__ __launch_bounds__(256) void k_shade_ms(ShadeLaunch L) {
         const uint32_t t = job >> 2, smp = job & 3u;
         const uint32_t jx = bx + (t & 15u), jy = by + (t >> 4);
         const size_t jo = size_t(jy) * V.width + jx;
-        const unsigned long long w = live ? V.vis[smp * plane + jo] : 0ull;
+        const unsigned long long w = live ? gmem(V.vis)[smp * plane + jo] : 0ull;
         const bool covered = live && uint32_t(w) != 0u;
         PixelGeom pg{};
-        if (covered) pg = pixel_geometry<kLod>(L, V, jx, jy, w);
+        if (covered) pg = pixel_geometry<kLod>(L, S, V, jx, jy, w);
         float dec[kNmOut] = {};
         bool decode = false;
         if constexpr (kNeural) {
@@ -680,8 +719,8 @@ __global__ __launch_bounds__(256) void k_shade_ms(ShadeLaunch L) {
         float4 c = clear;
         float sh = 1.f;
         if (covered) {
-            const Fragment fr = wave_uniform(pg.draw) ? shade_pixel<true, kLod>(L, V, lut, jo, pg, decode, dec)
-                                                      : shade_pixel<false, kLod>(L, V, lut, jo, pg, decode, dec);
+            const Fragment fr = wave_uniform(pg.draw) ? shade_pixel<true, kLod>(L, S, pg, decode, dec)
+                                                      : shade_pixel<false, kLod>(L, S, pg, decode, dec);
             c = fr.color;
             sh = fr.shadow;
         }
@@ -707,10 +746,10 @@ __global__ __launch_bounds__(256) void k_shade_ms(ShadeLaunch L) {
         if (V.sample_color)
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                if (same & (1u << q)) V.sample_color[q * plane + o] = c;
+                if (same & (1u << q)) gmem(V.sample_color)[q * plane + o] = c;
     }
-    V.color[o] = make_float4(acc.x * 0.25f, acc.y * 0.25f, acc.z * 0.25f, acc.w * 0.25f);
-    if (V.shadow_factor) V.shadow_factor[o] = shadow0[threadIdx.x];
+    gmem(V.color)[o] = make_float4(acc.x * 0.25f, acc.y * 0.25f, acc.z * 0.25f, acc.w * 0.25f);
+    if (V.shadow_factor) gmem(V.shadow_factor)[o] = shadow0[threadIdx.x];
 }
 
 // The blended pass (Graphics.cpp:5655-5691): the blended draw list in primitive order over the
@@ -839,8 +878,8 @@ __global__ __launch_bounds__(256) void k_blend_collect(const BlendLaunch B) {
 // out of space have key 0 and are skipped.
 template <bool kLod>
 __global__ __launch_bounds__(256) void k_blend_shade(const BlendLaunch B, const ShadeLaunch L) {
-    __shared__ float lut[256];
-    lut[threadIdx.x] = L.srgb_lut[threadIdx.x];
+    __shared__ ShadeShared S;
+    shared_store(L, S, shared_request(L));
     __syncthreads();
     const uint32_t n = min(B.counters[B.first_blend * kCtrPerView + kCtrFrags], B.frag_cap);
     const float none[kNmOut] = {};
@@ -850,9 +889,9 @@ __global__ __launch_bounds__(256) void k_blend_shade(const BlendLaunch B, const 
         const ShadeView& SV = L.views[f.z];
         const uint32_t x = f.y & 0xffffu, y = f.y >> 16;
         const size_t o = size_t(y) * SV.width + x;
-        const PixelGeom pg = pixel_geometry<kLod>(L, SV, x, y, (unsigned long long)f.x);
-        const Fragment fr = wave_uniform(pg.draw) ? shade_pixel<true, kLod>(L, SV, lut, o, pg, false, none)
-                                                  : shade_pixel<false, kLod>(L, SV, lut, o, pg, false, none);
+        const PixelGeom pg = pixel_geometry<kLod>(L, S, SV, x, y, (unsigned long long)f.x);
+        const Fragment fr = wave_uniform(pg.draw) ? shade_pixel<true, kLod>(L, S, pg, false, none)
+                                                  : shade_pixel<false, kLod>(L, S, pg, false, none);
         B.frag_color[f.w] = fr.discard ? make_float4(0.f, 0.f, 0.f, __uint_as_float(kDiscarded)) : fr.color;
     }
 }

@@ -258,3 +258,17 @@ def test_error_codes_cross_the_abi(renderer, sphere):
     out = np.zeros(7, np.uint64)
     assert lib.rsy_read_visibility(renderer.ctx, 0, out.ctypes.data, 7) == abi.RSY_INVALID_ARGUMENT
     assert lib.rsy_read_visibility(renderer.ctx, 5, out.ctypes.data, 7) == abi.RSY_INVALID_ARGUMENT
+
+
+def test_view_limit_is_an_invalid_argument(renderer, sphere):
+    # one camera view and RSY_MAX_VIEWS - 1 cascades fit a frame; one cascade more is refused
+    renderer.load(sphere)
+    cs = sphere.cascades()[0]
+    many = (abi.ShadowCascade * abi.MAX_VIEWS)(*([cs] * abi.MAX_VIEWS))
+    f = sphere.frame_desc(abi.SHADE_PLAIN)
+    assert f.n_camera_views == 1
+    f.n_cascades = abi.MAX_VIEWS
+    f.cascades = C.cast(many, C.POINTER(abi.ShadowCascade))
+    assert rosy_amd.lib().rsy_render(renderer.ctx, C.byref(f)) == abi.RSY_INVALID_ARGUMENT
+    f.n_cascades = abi.MAX_VIEWS - 1
+    assert rosy_amd.lib().rsy_render(renderer.ctx, C.byref(f)) == abi.RSY_OK
