@@ -584,6 +584,62 @@ __device__ inline void bin_fields(uint32_t v, uint64_t& a, uint64_t& b) {
     }
 }
 
+// The frame's totals (block 0, thread 0 of the scan) from the sums over all bins: a (records |
+// split items << 32), f01 and f23 (the class counts as 32-bit halves) and the set-up records.
+__device__ inline void scan_write_totals(uint64_t a, uint64_t f01, uint64_t f23, uint32_t valid, uint32_t list_cap,
+                                         uint32_t items_cap, uint32_t raster_grid, uint32_t* totals) {
+    const uint32_t n_split = uint32_t(a >> 32);
+    const uint32_t n_heavy = uint32_t(f01 >> 32), n_mid = uint32_t(f23);
+    const uint32_t pairs = uint32_t(a), n_zero = uint32_t(f01), n_light = uint32_t(f23 >> 32);
+    const uint64_t n_items = uint64_t(n_heavy) + n_mid + n_split + n_light;
+    totals[0] = pairs;
+    totals[1] = valid;
+    // bit 0: bin lists or item table too small; bit 1: more items than raster workgroups
+    totals[2] = ((pairs > list_cap || n_items > items_cap) ? 1u : 0u) | (n_items > raster_grid ? 2u : 0u);
+    totals[3] = uint32_t(min<uint64_t>(n_items, items_cap));
+    totals[4] = n_heavy;                    // first mid item
+    totals[5] = n_zero;
+    totals[6] = n_heavy + n_mid;            // first split item
+    totals[7] = n_heavy + n_mid + n_split;  // first light item
+}
+
+// A thread's P consecutive bins from i0: list offsets, work items and clear-list entries, from the
+// block's base over the earlier bins (base), the thread's exclusive prefix inside the block (ea:
+// records | split items << 32, eb: 16-bit class fields) and the frame's class totals.
+template <uint32_t P>
+__device__ inline void scan_write_bins(const uint32_t (&v)[P], uint32_t i0, uint32_t n, const ScanPartial& base,
+                                       uint64_t ea_local, uint64_t eb, uint32_t n_heavy, uint32_t n_mid,
+                                       uint32_t n_split, uint32_t* offsets, uint4* items, uint32_t items_cap,
+                                       uint32_t* zero_bins) {
+    const uint64_t ea = base.a + ea_local;
+    uint32_t run = uint32_t(ea);
+    uint64_t r_split = uint64_t(n_heavy) + n_mid + uint32_t(ea >> 32);
+    uint32_t r_zero = base.f[0] + uint32_t(eb & 0xffffu);
+    // next item slot of each whole-bin class: the class's first item + earlier blocks + this block
+    uint64_t r_heavy = base.f[1] + ((eb >> 16) & 0xffffu);
+    uint64_t r_mid = uint64_t(n_heavy) + base.f[2] + ((eb >> 32) & 0xffffu);
+    uint64_t r_light = uint64_t(n_heavy) + n_mid + n_split + base.f[3] + ((eb >> 48) & 0xffffu);
+#pragma unroll
+    for (uint32_t k = 0; k < P; ++k)
+        if (i0 + k < n) {
+            offsets[i0 + k] = run;
+            const uint32_t cls = bin_class(v[k]);
+            if (cls == kClassSplit) {
+                zero_bins[r_zero++] = i0 + k;
+                const uint32_t ni = (v[k] + kItemRecs - 1) / kItemRecs;
+                for (uint32_t c = 0; c < ni; ++c, ++r_split)
+                    if (r_split < items_cap)
+                        items[r_split] = make_uint4(i0 + k, run + c * kItemRecs, min(kItemRecs, v[k] - c * kItemRecs), 1u);
+            } else if (cls == kClassZero) {
+                zero_bins[r_zero++] = i0 + k;
+            } else {
+                const uint64_t slot = cls == kClassHeavy ? r_heavy++ : (cls == kClassMid ? r_mid++ : r_light++);
+                if (slot < items_cap) items[slot] = make_uint4(i0 + k, run, v[k], 0u);
+            }
+            run += v[k];
+        }
+}
+
 // Block sums of the bin fields; each block also sums an equal share of the setup blocks' record
 // counts, so that no block walks the whole table.
 __global__ __launch_bounds__(kScanThreads) void k_scan_partial(const uint32_t* counts, uint32_t n, ScanPartial* partial,
@@ -651,22 +707,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const uint32_t* cou
     }
     const uint32_t n_split = uint32_t(carry_a >> 32);
     const uint32_t n_heavy = uint32_t(carry_01 >> 32), n_mid = uint32_t(carry_23);
-    if (blockIdx.x == 0) {
-        const uint32_t total_valid = valid;
-        if (threadIdx.x == 0) {
-            const uint32_t pairs = uint32_t(carry_a), n_zero = uint32_t(carry_01), n_light = uint32_t(carry_23 >> 32);
-            const uint64_t n_items = uint64_t(n_heavy) + n_mid + n_split + n_light;
-            totals[0] = pairs;
-            totals[1] = total_valid;
-            // bit 0: bin lists or item table too small; bit 1: more items than raster workgroups
-            totals[2] = ((pairs > list_cap || n_items > items_cap) ? 1u : 0u) | (n_items > raster_grid ? 2u : 0u);
-            totals[3] = uint32_t(min<uint64_t>(n_items, items_cap));
-            totals[4] = n_heavy;                    // first mid item
-            totals[5] = n_zero;
-            totals[6] = n_heavy + n_mid;            // first split item
-            totals[7] = n_heavy + n_mid + n_split;  // first light item
-        }
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        scan_write_totals(carry_a, carry_01, carry_23, valid, list_cap, items_cap, raster_grid, totals);
     __syncthreads();
     uint64_t a = 0, b = 0;
 #pragma unroll
@@ -675,34 +717,70 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const uint32_t* cou
     uint64_t sv[2] = {a, b}, st[2];
     const ScanPartial base = base_lds;
     block_exclusive_scan64_n<2>(sv, lds64, st);
-    const uint64_t ea = base.a + sv[0];
-    const uint64_t eb = sv[1];
-    uint32_t run = uint32_t(ea);
-    uint64_t r_split = uint64_t(n_heavy) + n_mid + uint32_t(ea >> 32);
-    uint32_t r_zero = base.f[0] + uint32_t(eb & 0xffffu);
-    // next item slot of each whole-bin class: the class's first item + earlier blocks + this block
-    uint64_t r_heavy = base.f[1] + ((eb >> 16) & 0xffffu);
-    uint64_t r_mid = uint64_t(n_heavy) + base.f[2] + ((eb >> 32) & 0xffffu);
-    uint64_t r_light = uint64_t(n_heavy) + n_mid + n_split + base.f[3] + ((eb >> 48) & 0xffffu);
+    scan_write_bins<kScanPerThread>(v, i0, n, base, sv[0], sv[1], n_heavy, n_mid, n_split, offsets, items, items_cap,
+                                    zero_bins);
+}
+
+// The same scan in one launch for frames of at most kDirectScanBins bins: kDirectThreads threads
+// per block, one bin per thread.  Every block reduces the counts of all bins itself (thread t takes
+// bins t, t + kDirectThreads, ..., their loads in flight together): those before its own bins give
+// its base, all of them the class totals.  No block waits for another.
+constexpr uint32_t kDirectThreads = 1024;
+constexpr uint32_t kDirectScanBins = 65536;  // per-thread class counts stay below 2^16
+constexpr uint32_t kDirectBatch = 16;
+__global__ __launch_bounds__(kDirectThreads) void k_scan_direct(const uint32_t* counts, uint32_t n, uint32_t* offsets,
+                                                                const uint32_t* block_valid, uint32_t n_blocks,
+                                                                uint32_t list_cap, uint32_t* totals, uint4* items,
+                                                                uint32_t items_cap, uint32_t* zero_bins,
+                                                                uint32_t raster_grid) {
+    __shared__ uint64_t lds64[7 * 16];
+    const uint32_t i0 = blockIdx.x * kDirectThreads + threadIdx.x;
+    const uint32_t lo = blockIdx.x * kDirectThreads;
+    uint32_t v[1];
+    v[0] = i0 < n ? counts[i0] : 0u;
+    uint64_t ba = 0, bb = 0, ta = 0, tb = 0;  // before this block's bins / all bins
+    for (uint32_t j = threadIdx.x; j < n; j += kDirectBatch * kDirectThreads) {
+        uint32_t c[kDirectBatch];
 #pragma unroll
-    for (uint32_t k = 0; k < kScanPerThread; ++k)
-        if (i0 + k < n) {
-            offsets[i0 + k] = run;
-            const uint32_t cls = bin_class(v[k]);
-            if (cls == kClassSplit) {
-                zero_bins[r_zero++] = i0 + k;
-                const uint32_t ni = (v[k] + kItemRecs - 1) / kItemRecs;
-                for (uint32_t c = 0; c < ni; ++c, ++r_split)
-                    if (r_split < items_cap)
-                        items[r_split] = make_uint4(i0 + k, run + c * kItemRecs, min(kItemRecs, v[k] - c * kItemRecs), 1u);
-            } else if (cls == kClassZero) {
-                zero_bins[r_zero++] = i0 + k;
-            } else {
-                const uint64_t slot = cls == kClassHeavy ? r_heavy++ : (cls == kClassMid ? r_mid++ : r_light++);
-                if (slot < items_cap) items[slot] = make_uint4(i0 + k, run, v[k], 0u);
+        for (uint32_t k = 0; k < kDirectBatch; ++k) c[k] = j + k * kDirectThreads < n ? counts[j + k * kDirectThreads] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < kDirectBatch; ++k) {
+            const uint32_t i = j + k * kDirectThreads;
+            if (i < n) {
+                uint64_t a1 = 0, b1 = 0;
+                bin_fields(c[k], a1, b1);
+                ta += a1;
+                tb += b1;
+                if (i < lo) {
+                    ba += a1;
+                    bb += b1;
+                }
             }
-            run += v[k];
         }
+    }
+    uint32_t vs = 0;
+    if (blockIdx.x == 0)
+        for (uint32_t j = threadIdx.x; j < n_blocks; j += kDirectThreads) vs += block_valid[j];
+    // the four 16-bit class fields as two pairs of 32-bit halves for the sums over the block
+    const uint64_t b01 = (bb & 0xffffu) | (((bb >> 16) & 0xffffu) << 32), b23 = ((bb >> 32) & 0xffffu) | ((bb >> 48) << 32);
+    const uint64_t t01 = (tb & 0xffffu) | (((tb >> 16) & 0xffffu) << 32), t23 = ((tb >> 32) & 0xffffu) | ((tb >> 48) << 32);
+    uint64_t sr[7] = {ba, b01, b23, ta, t01, t23, vs}, tot[7];
+    block_exclusive_scan64_n<7>(sr, lds64, tot);
+    ScanPartial base{};
+    base.a = tot[0];
+    base.f[0] = uint32_t(tot[1]);
+    base.f[1] = uint32_t(tot[1] >> 32);
+    base.f[2] = uint32_t(tot[2]);
+    base.f[3] = uint32_t(tot[2] >> 32);
+    const uint32_t n_split = uint32_t(tot[3] >> 32);
+    const uint32_t n_heavy = uint32_t(tot[4] >> 32), n_mid = uint32_t(tot[5]);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        scan_write_totals(tot[3], tot[4], tot[5], uint32_t(tot[6]), list_cap, items_cap, raster_grid, totals);
+    uint64_t a = 0, b = 0;
+    if (i0 < n) bin_fields(v[0], a, b);
+    uint64_t sv[2] = {a, b}, st[2];
+    block_exclusive_scan64_n<2>(sv, lds64, st);
+    scan_write_bins<1>(v, i0, n, base, sv[0], sv[1], n_heavy, n_mid, n_split, offsets, items, items_cap, zero_bins);
 }
 
 // Bin lists without atomics: every (record, bin) pair already holds its slot inside the bin (from
@@ -897,11 +975,17 @@ hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
 hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
     const uint32_t parts = (b.total_bins + kScanChunk - 1) / kScanChunk;
     ScanPartial* partial = reinterpret_cast<ScanPartial*>(b.scan_partial);
-    hipLaunchKernelGGL(k_scan_partial, dim3(parts), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, partial,
-                       b.block_valid, b.n_tblocks);
-    hipLaunchKernelGGL(k_scan_final, dim3(parts), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, b.bin_offsets,
-                       partial, parts, b.list_cap, b.totals, b.items, b.items_cap,
-                       b.zero_bins, b.raster_grid);
+    if (b.total_bins <= kDirectScanBins) {
+        hipLaunchKernelGGL(k_scan_direct, dim3((b.total_bins + kDirectThreads - 1) / kDirectThreads), dim3(kDirectThreads),
+                           0, s, b.bin_counts, b.total_bins, b.bin_offsets, b.block_valid, b.n_tblocks, b.list_cap,
+                           b.totals, b.items, b.items_cap, b.zero_bins, b.raster_grid);
+    } else {
+        hipLaunchKernelGGL(k_scan_partial, dim3(parts), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, partial,
+                           b.block_valid, b.n_tblocks);
+        hipLaunchKernelGGL(k_scan_final, dim3(parts), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins,
+                           b.bin_offsets, partial, parts, b.list_cap, b.totals, b.items, b.items_cap, b.zero_bins,
+                           b.raster_grid);
+    }
     hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.n_views, b.boxes,
                        b.counters, b.bin_offsets, b.pair_slots, b.big_pairs, b.lists, b.list_cap, b.shift, b.totals,
                        b.zero_bins);
