@@ -25,6 +25,20 @@ using namespace rsy;
 
 namespace {
 
+// kDraw* flags of a draw with material index `material` under the context's materials and samplers.
+uint32_t draw_flags(const rsy_ctx* c, uint32_t material) {
+    uint32_t flags = 0;
+    if (material < c->h_materials.size()) {
+        const rsy_gpu_material& md = c->h_materials[material];
+        const uint32_t si[4] = {md.color_sampler_index, md.normal_sampler_index, md.metallic_sampler_index,
+                                md.mixmap_sampler_index};
+        for (uint32_t k : si)
+            if (k < c->h_samplers.size() && (c->h_samplers[k].wrap_s == 0 || c->h_samplers[k].wrap_t == 0))
+                flags |= kDrawClampSampler;
+    }
+    return flags;
+}
+
 // Rebuilds the per-list draw tables (Graphics.cpp:4716-4735: shadow = every surface,
 // opaque = !blended, blended = blended) from the surfaces and the mesh table.
 rsy_result rebuild_lists(rsy_ctx* c) {
@@ -49,14 +63,7 @@ rsy_result rebuild_lists(rsy_ctx* c) {
         d.vtx_count = m.num_vertices;
         d.object = uint32_t(obj);
         d.material = s.material_index < 0xFFFFFFFFull ? uint32_t(s.material_index) : 0xFFFFFFFFu;
-        if (d.material < c->h_materials.size()) {
-            const rsy_gpu_material& md = c->h_materials[d.material];
-            const uint32_t si[4] = {md.color_sampler_index, md.normal_sampler_index, md.metallic_sampler_index,
-                                    md.mixmap_sampler_index};
-            for (uint32_t k : si)
-                if (k < c->h_samplers.size() && (c->h_samplers[k].wrap_s == 0 || c->h_samplers[k].wrap_t == 0))
-                    d.flags |= kDrawClampSampler;
-        }
+        d.flags = draw_flags(c, d.material);
         for (int k = 0; k < 3; ++k) {
             d.bmin[k] = c->mesh_bounds[size_t(s.mesh_index) * 6 + k];
             d.bmax[k] = c->mesh_bounds[size_t(s.mesh_index) * 6 + 3 + k];
@@ -1011,7 +1018,15 @@ rsy_result rsy_set_materials(rsy_ctx* c, const rsy_gpu_material* materials, uint
     c->h_materials.assign(materials, materials + n_materials);
     c->h_tex_table = table;
     c->h_samplers.assign(samplers, samplers + n_samplers);
-    c->lists_dirty = true;  // the draws' sampler flags follow the materials
+    // the draws' sampler flags follow the materials: refreshed in place, so that the allocation-free
+    // render path keeps working after a material change
+    if (!c->lists_dirty)
+        for (int l = 0; l < 3; ++l) {
+            for (DrawGPU& d : c->lists[l]) d.flags = draw_flags(c, d.material);
+            if (!c->lists[l].empty())
+                HIP_TRY(c, hipMemcpy(c->lists_d[l].p, c->lists[l].data(), c->lists[l].size() * sizeof(DrawGPU),
+                                     hipMemcpyHostToDevice), "upload draw list");
+        }
     c->tex_opaque.assign(n_textures, 0);
     c->any_mips = false;
     std::vector<uint8_t> decoded;

@@ -146,14 +146,25 @@ __global__ void k_clear2(uint32_t* a, uint32_t na, uint32_t* b, uint32_t nb) {
     }
 }
 
-// Draw culling: a draw is skipped when every corner of its object-space box lies outside one clip
-// plane by more than the rounding of the transform (|error| <= a few ulp of sum |m_rj p_j| per
-// coordinate; the margin takes 2^-18 of that sum).  A vertex of the mesh is a convex combination of
-// the corners and its own sum is no larger, so all its FP32 plane distances are negative too: every
-// triangle of the draw would be rejected by the per-triangle test, and skipping the draw changes
-// nothing.  corner_outside gives corner k's bits (k = 0..7: x from bit 0, y from bit 1, z from bit
-// 2): bit `plane` set when the corner is outside that plane by the margin; the box is outside the
-// view when, for some plane, all eight corners have the bit.
+// Draw culling: a draw is skipped when its object-space box provably contributes nothing to the
+// view.  Every test carries a margin for the rounding of the transform (|error| <= a few ulp of
+// sum |m_rj p_j| per coordinate; the margin takes 2^-18 of that sum); a vertex of the mesh is a
+// convex combination of the box's corners and its own sum is no larger, so a distance that is
+// negative (positive) beyond the margin at every corner is negative (positive) at the vertex's FP32
+// clip position too.  Two cases:
+//  - every corner outside one clip plane (near, far, guard band): every triangle of the draw is
+//    rejected by the per-triangle test;
+//  - every corner inside all six clip planes and beyond one side of the view (x < -w, x > w,
+//    y < -w or y > w): no triangle is clipped, and every vertex snaps onto or beyond that edge of
+//    the image (x / w > 1 rounds to >= 1, so X >= width * 256; x / w < -1 gives X <= 0), while
+//    the sample points lie strictly inside (1/256 to 255/256 of a pixel): no triangle's box holds a
+//    sample point, so none produces a record.
+// Skipping the draw changes nothing.  corner_outside gives corner k's bits (k = 0..7: x from bit 0,
+// y from bit 1, z from bit 2): bit `plane` (0..5) when the corner is outside that clip plane, bit
+// 6 + side when it is beyond side `side` of the view, and kCornerInside when it is inside all six
+// clip planes, each by the margin.
+constexpr uint32_t kCornerInside = 1u << 10;
+
 __device__ inline uint32_t corner_outside(const float* m, const float* bmin, const float* bmax, int k) {
     const float x = (k & 1) ? bmax[0] : bmin[0], y = (k & 2) ? bmax[1] : bmin[1];
     const float z = (k & 4) ? bmax[2] : bmin[2];
@@ -162,7 +173,7 @@ __device__ inline uint32_t corner_outside(const float* m, const float* bmin, con
 #pragma unroll
     for (int r = 0; r < 4; ++r)
         sr[r] = fabsf(m[r] * x) + fabsf(m[4 + r] * y) + fabsf(m[8 + r] * z) + fabsf(m[12 + r]);
-    uint32_t bits = 0;
+    uint32_t bits = kCornerInside;
 #pragma unroll
     for (int plane = 0; plane < 6; ++plane) {
         float bound;
@@ -172,9 +183,24 @@ __device__ inline uint32_t corner_outside(const float* m, const float* bmin, con
             case 2: case 3: bound = sr[0] + 4.0f * sr[3]; break;
             default: bound = sr[1] + 4.0f * sr[3]; break;
         }
-        if (plane_dist(c, plane) < -bound * 0x1.0p-18f) bits |= 1u << plane;
+        const float d = plane_dist(c, plane), margin = bound * 0x1.0p-18f;
+        if (d < -margin) bits |= 1u << plane;
+        if (!(d > margin)) bits &= ~kCornerInside;
     }
+    const float side[4] = {c.x + c.w, c.w - c.x, c.y + c.w, c.w - c.y};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (side[q] < -(sr[q >> 1] + sr[3]) * 0x1.0p-18f) bits |= 1u << (6 + q);
     return bits;
+}
+
+// True when the bits of the eight corners (lanes 0..7 of the wave) show the box outside the view.
+__device__ inline bool box_outside(uint32_t bits) {
+    uint32_t all = 0;
+#pragma unroll
+    for (int b = 0; b < 11; ++b)
+        all |= ((__builtin_amdgcn_ballot_w64((bits >> b) & 1u) & 0xffull) == 0xffull) ? 1u << b : 0u;
+    return (all & 0x3fu) != 0u || ((all & kCornerInside) != 0u && (all & 0x3c0u) != 0u);
 }
 
 // clip = (VP * W) * p per (view, vertex).  Every block covers kTransformBlockVerts vertices of one
@@ -224,13 +250,9 @@ __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, con
     }
     __syncthreads();
     if (threadIdx.x < 64) {  // wave 0
-        const uint32_t bits = threadIdx.x < 8 ? corner_outside(m, dr.bmin, dr.bmax, int(threadIdx.x)) : 0u;
-        bool any_plane = false;
-#pragma unroll
-        for (int plane = 0; plane < 6; ++plane)
-            any_plane |= (__builtin_amdgcn_ballot_w64((bits >> plane) & 1u) & 0xffull) == 0xffull;
+        const bool outside = box_outside(threadIdx.x < 8 ? corner_outside(m, dr.bmin, dr.bmax, int(threadIdx.x)) : 0u);
         if (threadIdx.x == 0) {
-            culled = any_plane ? 1u : 0u;
+            culled = outside ? 1u : 0u;
             if (e.start == 0) draw_culled[V.cull_base + e.draw] = culled;
         }
     }

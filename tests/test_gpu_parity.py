@@ -111,6 +111,11 @@ def test_overflow_regrowth_matches_the_oracle(scene_scale):
 
 
 def test_bin_sizes_give_identical_frames(renderer, scene_scale):
+    # 8-pixel bins give this frame 240 x 135 + 256 x 256 = 97,936 bins, past the 65,536 of the
+    # one-launch bin scan: the two-launch scan must agree with it bit for bit
+    cv, cs = scene_scale.camera_views()[0], scene_scale.cascades()[0]
+    assert (cv.width // 8) * (cv.height // 8) + (cs.size // 8) ** 2 > 65536
+    assert (cv.width // 16) * (cv.height // 16) + (cs.size // 16) ** 2 <= 65536
     renderer.load(scene_scale)
     frame = scene_scale.frame_desc(abi.SHADE_PLAIN)
     outs = []
@@ -203,6 +208,49 @@ def test_clipping_matches_oracle(renderer):
     renderer.load(sc)
     check_frame(renderer, sc, colour=False)
     assert renderer.stats().triangles_clipped >= 3
+
+
+def _translate(x, y, z):
+    return [1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0, 0, x, y, z, 1.0]
+
+
+@pytest.mark.parametrize("perspective", [False, True])
+def test_draws_beyond_a_side_of_the_view_match_oracle(renderer, perspective):
+    # one mesh drawn through many objects: boxes just beyond each side of the view (skipped whole
+    # by the draw test), touching an edge (within the test's rounding margin: not skipped),
+    # crossing the guard band, straddling a corner, and (with perspective) crossing the near plane
+    W, H = 96, 64
+    g = np.linspace(0.0, 0.2, 5, dtype=np.float32)
+    pos = [(float(x), float(y), 0.5 + 0.1 * float(x) - 0.05 * float(y)) for y in g for x in g]
+    tris = []
+    for j in range(4):
+        for i in range(4):
+            a = j * 5 + i
+            tris += [a, a + 1, a + 5, a + 1, a + 6, a + 5]
+    n_tris = len(tris) // 3
+    if perspective:
+        n, f, s = 0.5, 50.0, 1.5
+        a = n / (f - n)
+        vp = [s, 0, 0, 0, 0, s, 0, 0, 0, 0, a, -1.0, 0, 0, a * f, 0]
+        d = 4.0  # eye depth of most objects: clip w = d - z_local, z_local in [0.49, 0.52]
+        e = (d - 0.48) / s  # eye x (or y) beyond which a box at that depth is past the side
+        offs = [(e - 0.01, 0.0, -d), (e + 0.02, 0.1, -d), (-e - 0.22, 0.0, -d), (0.0, e + 0.02, -d),
+                (0.0, -e - 0.22, -d), (2.5 * e, 0.0, -d), (3.95 * e, 0.0, -d), (e - 0.1, e - 0.1, -d),
+                (e + 0.1, 0.0, -0.2), (e + 0.02, 0.0, -60.0), (0.0, 0.0, -d)]
+    else:
+        vp = None
+        offs = [(1.0, 0.0, 0.0), (1.0 + 2.0 ** -20, 0.1, 0.0), (1.001, -0.4, 0.0), (-1.201, 0.0, 0.0),
+                (-1.2, 0.3, 0.0), (0.3, 1.001, 0.0), (0.3, -1.201, 0.0), (2.9, 0.0, 0.0), (3.9, 0.0, 0.0),
+                (0.9, 0.9, 0.0), (-0.1, -0.1, 0.0)]
+    objs = [_translate(*o) for o in offs]
+    if perspective:  # the mesh's y extent turned into depth, from in front of the near plane to behind the eye
+        objs.append([1.0, 0, 0, 0, 0, 0, -10.0, 0, 0, 1.0, 0, 0, 2.0, 0.0, -0.5, 1.0])
+    draws = [(0, n_tris)] * len(objs)
+    sc = CustomScene(pos, tris, W, H, draws=draws, view_projection=vp, shadow_projection=vp, shadow_size=64,
+                     objects=objs, object_of_draw=list(range(len(objs))))
+    renderer.load(sc)
+    check_frame(renderer, sc, colour=False)
+    assert renderer.stats().triangles_setup > 0
 
 
 def test_four_cascades_and_multiple_views(renderer):
