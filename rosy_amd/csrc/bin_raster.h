@@ -12,21 +12,57 @@ namespace rsy {
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
+// Inclusive prefix sums over the 64 lanes of a wave (every lane active): DPP row shifts by 1, 2, 4
+// and 8 give each row of 16 lanes its own prefix, then row broadcasts add the totals of the rows
+// before (lane 15 into rows 1 and 3, lane 31 into rows 2 and 3).  VALU moves only, no LDS permutes.
+template <int kCtrl, int kRowMask>
+__device__ inline uint32_t dpp_move(uint32_t x) {
+    return uint32_t(__builtin_amdgcn_update_dpp(0, int(x), kCtrl, kRowMask, 0xf, false));
+}
+
+__device__ inline uint32_t wave_inclusive_sum(uint32_t x) {
+    x += dpp_move<0x111, 0xf>(x);  // row_shr:1
+    x += dpp_move<0x112, 0xf>(x);  // row_shr:2
+    x += dpp_move<0x114, 0xf>(x);  // row_shr:4
+    x += dpp_move<0x118, 0xf>(x);  // row_shr:8
+    x += dpp_move<0x142, 0xa>(x);  // row_bcast:15
+    x += dpp_move<0x143, 0xc>(x);  // row_bcast:31
+    return x;
+}
+
+template <int kCtrl, int kRowMask>
+__device__ inline uint64_t dpp_move64(uint64_t x) {
+    return uint64_t(dpp_move<kCtrl, kRowMask>(uint32_t(x))) | (uint64_t(dpp_move<kCtrl, kRowMask>(uint32_t(x >> 32))) << 32);
+}
+
+// Lane `lane` (wave-uniform) of a 64-bit value.
+__device__ inline uint64_t readlane64(uint64_t x, uint32_t lane) {
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(x)), int(lane)));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(x >> 32)), int(lane)));
+    return uint64_t(lo) | (uint64_t(hi) << 32);
+}
+
+__device__ inline uint64_t wave_inclusive_sum64(uint64_t x) {
+    x += dpp_move64<0x111, 0xf>(x);
+    x += dpp_move64<0x112, 0xf>(x);
+    x += dpp_move64<0x114, 0xf>(x);
+    x += dpp_move64<0x118, 0xf>(x);
+    x += dpp_move64<0x142, 0xa>(x);
+    x += dpp_move64<0x143, 0xc>(x);
+    return x;
+}
+
 __device__ inline uint32_t fbits(float f) { return __float_as_uint(f); }
 
 constexpr uint32_t kItemRecs = 256;   // records per raster work item (one per thread)
 constexpr uint32_t kChunkRecs = 128;  // records staged in LDS at a time
 
-// Exclusive prefix sum over the workgroup (wave scans with shuffles, wave totals through LDS).
+// Exclusive prefix sum over the workgroup (wave scans by DPP, wave totals through LDS).
 // Every thread of the block must call it; it synchronises the block.  kTailSync = false leaves out
 // the closing barrier, for a caller that synchronises again before lds_waves is next written.
 template <bool kTailSync = true>
 __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds_waves, uint32_t& total) {
-    uint32_t x = v;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off);
-        if (int(lane_id()) >= off) x += y;
-    }
+    const uint32_t x = wave_inclusive_sum(v);
     const uint32_t wave = threadIdx.x / 64, nwaves = blockDim.x / 64;
     if (lane_id() == 63) lds_waves[wave] = x;
     __syncthreads();

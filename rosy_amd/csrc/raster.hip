@@ -278,12 +278,7 @@ __device__ void bin_wide(bool active, const TriRec& r, uint32_t ri, const ViewGP
     const int bw = tx1 - tx0 + 1;
     const uint32_t bn = active ? uint32_t(bw * (ty1 - ty0 + 1)) : 0u;
     const uint32_t lane = lane_id();
-    uint32_t incl = bn;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(incl, off);
-        if (int(lane) >= off) incl += y;
-    }
+    const uint32_t incl = wave_inclusive_sum(bn);
     const uint32_t total = uint32_t(__builtin_amdgcn_readlane(int(incl), 63));
     if (total == 0) return;
     const uint32_t excl = incl - bn;
@@ -538,32 +533,28 @@ constexpr uint32_t kMidRecs = 48;
 
 // N exclusive prefix sums over the workgroup at once (one pair of barriers for all of them): v[j]
 // becomes its exclusive prefix and total[j] the workgroup's sum.  lds_waves holds 16 words per value.
+// The wave totals are scanned by the first row of 16 lanes of every wave (one LDS read per value)
+// and the wave's base and the block total read from that row's lanes.
 template <int N>
 __device__ inline void block_exclusive_scan64_n(uint64_t (&v)[N], uint64_t* lds_waves, uint64_t (&total)[N]) {
     uint64_t x[N];
 #pragma unroll
-    for (int j = 0; j < N; ++j) x[j] = v[j];
-    for (int off = 1; off < 64; off <<= 1) {
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-            const uint64_t y = __shfl_up(x[j], off);
-            if (int(lane_id()) >= off) x[j] += y;
-        }
-    }
+    for (int j = 0; j < N; ++j) x[j] = wave_inclusive_sum64(v[j]);
     const uint32_t wave = threadIdx.x / 64, nwaves = blockDim.x / 64;
     if (lane_id() == 63)
 #pragma unroll
         for (int j = 0; j < N; ++j) lds_waves[j * 16 + wave] = x[j];
     __syncthreads();
+    const uint32_t lane = lane_id();
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        uint64_t wave_off = 0, sum = 0;
-        for (uint32_t w = 0; w < nwaves; ++w) {
-            const uint64_t t = lds_waves[j * 16 + w];
-            if (w < wave) wave_off += t;
-            sum += t;
-        }
-        total[j] = sum;
+        uint64_t t = lane < nwaves ? lds_waves[j * 16 + lane] : 0ull;
+        t += dpp_move64<0x111, 0xf>(t);  // row_shr:1, 2, 4, 8: lanes 0..15 hold the waves' inclusive prefix
+        t += dpp_move64<0x112, 0xf>(t);
+        t += dpp_move64<0x114, 0xf>(t);
+        t += dpp_move64<0x118, 0xf>(t);
+        const uint64_t wave_off = wave == 0 ? 0ull : readlane64(t, wave - 1);
+        total[j] = readlane64(t, nwaves - 1);
         v[j] = wave_off + x[j] - v[j];
     }
     __syncthreads();
@@ -647,13 +638,15 @@ __device__ inline void scan_write_bins(const uint32_t (&v)[P], uint32_t i0, uint
             offsets[i0 + k] = run;
             const uint32_t cls = bin_class(v[k]);
             if (cls == kClassSplit) {
-                zero_bins[r_zero++] = i0 + k;
+                if (r_zero < n) zero_bins[r_zero] = i0 + k;
+                ++r_zero;
                 const uint32_t ni = (v[k] + kItemRecs - 1) / kItemRecs;
                 for (uint32_t c = 0; c < ni; ++c, ++r_split)
                     if (r_split < items_cap)
                         items[r_split] = make_uint4(i0 + k, run + c * kItemRecs, min(kItemRecs, v[k] - c * kItemRecs), 1u);
             } else if (cls == kClassZero) {
-                zero_bins[r_zero++] = i0 + k;
+                if (r_zero < n) zero_bins[r_zero] = i0 + k;
+                ++r_zero;
             } else {
                 const uint64_t slot = cls == kClassHeavy ? r_heavy++ : (cls == kClassMid ? r_mid++ : r_light++);
                 if (slot < items_cap) items[slot] = make_uint4(i0 + k, run, v[k], 0u);
@@ -781,8 +774,15 @@ __global__ __launch_bounds__(kDirectThreads) void k_scan_direct(const uint32_t* 
         }
     }
     uint32_t vs = 0;
-    if (blockIdx.x == 0)
-        for (uint32_t j = threadIdx.x; j < n_blocks; j += kDirectThreads) vs += block_valid[j];
+    if (blockIdx.x == 0)  // the set-up records, kDirectBatch loads in flight together
+        for (uint32_t j = threadIdx.x; j < n_blocks; j += kDirectBatch * kDirectThreads) {
+            uint32_t c[kDirectBatch];
+#pragma unroll
+            for (uint32_t k = 0; k < kDirectBatch; ++k)
+                c[k] = j + k * kDirectThreads < n_blocks ? block_valid[j + k * kDirectThreads] : 0u;
+#pragma unroll
+            for (uint32_t k = 0; k < kDirectBatch; ++k) vs += c[k];
+        }
     // the four 16-bit class fields as two pairs of 32-bit halves for the sums over the block
     const uint64_t b01 = (bb & 0xffffu) | (((bb >> 16) & 0xffffu) << 32), b23 = ((bb >> 32) & 0xffffu) | ((bb >> 48) << 32);
     const uint64_t t01 = (tb & 0xffffu) | (((tb >> 16) & 0xffffu) << 32), t23 = ((tb >> 32) & 0xffffu) | ((tb >> 48) << 32);

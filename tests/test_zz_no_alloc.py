@@ -83,6 +83,27 @@ def test_noalloc_frames_match_render_twice_the_second_smaller():
         small.close()
 
 
+def test_noalloc_after_a_material_change():
+    # rsy_set_materials after a frame keeps the draw lists built (their sampler flags are refreshed
+    # in place), so the allocation-free frame that follows needs no rebuild and matches rsy_render
+    s = rosy_amd.Scene(1, seed=4)
+    r = rosy_amd.Renderer(device=0)
+    lib = rosy_amd.lib()
+    try:
+        r.load(s)
+        frame = s.frame_desc(abi.SHADE_PLAIN)
+        r.render(frame)
+        want_v, want_c = r.visibility(0), r.color(0)
+        assert lib.rsy_set_materials(r.ctx, s.s.materials, s.s.n_materials, s.s.textures, s.s.n_textures,
+                                     s.s.samplers, s.s.n_samplers) == abi.RSY_OK
+        code, got_v, got_c, _ = render_noalloc(r, s, frame)
+        assert code == abi.RSY_OK, lib.rsy_last_error(r.ctx)
+        assert np.array_equal(got_v[0], want_v) and np.array_equal(got_c[0], want_c)
+    finally:
+        r.close()
+        s.close()
+
+
 def test_noalloc_rejects_what_it_cannot_hold():
     s = rosy_amd.Scene(1, seed=4)
     r = rosy_amd.Renderer(device=0)
