@@ -285,25 +285,40 @@ __device__ void bin_wide(bool active, const TriRec& r, uint32_t ri, const ViewGP
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(&ctr[kCtrBig], total);
     base = uint32_t(__builtin_amdgcn_readlane(int(base), 0));
-    for (uint32_t p0 = 0; p0 < total; p0 += 64) {
-        const uint32_t p = p0 + lane;
-        // owner: the last lane whose exclusive prefix is <= p (lanes without pairs share the
-        // prefix of the next lane that has some, and come before it)
-        int lo = 0;
+    // kPairBatch pairs per lane at a time: their slot atomics are all in flight before the first
+    // result is needed
+    constexpr int kPairBatch = 4;
+    for (uint32_t p0 = 0; p0 < total; p0 += 64u * kPairBatch) {
+        uint32_t key[kPairBatch], local[kPairBatch], rid[kPairBatch];
 #pragma unroll
-        for (int step = 32; step > 0; step >>= 1) {
-            const uint32_t e = uint32_t(__shfl(int(excl), lo + step));
-            if (e <= p) lo += step;
+        for (int j = 0; j < kPairBatch; ++j) {
+            const uint32_t p = p0 + 64u * uint32_t(j) + lane;
+            // owner: the last lane whose exclusive prefix is <= p (lanes without pairs share the
+            // prefix of the next lane that has some, and come before it)
+            int lo = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const uint32_t e = uint32_t(__shfl(int(excl), lo + step));
+                if (e <= p) lo += step;
+            }
+            const uint32_t o_excl = uint32_t(__shfl(int(excl), lo));
+            const int o_tx0 = __shfl(tx0, lo), o_ty0 = __shfl(ty0, lo), o_bw = __shfl(bw, lo);
+            rid[j] = uint32_t(__shfl(int(ri), lo));
+            key[j] = 0u;
+            local[j] = 0u;
+            if (p < total) {
+                const int k = int(p - o_excl);
+                key[j] = V.tile_base + uint32_t((o_ty0 + k / o_bw) * int(V.tiles_x) + o_tx0 + k % o_bw);
+                local[j] = atomicAdd(&bin_counts[key[j]], 1u);
+            }
         }
-        const uint32_t o_excl = uint32_t(__shfl(int(excl), lo));
-        const int o_tx0 = __shfl(tx0, lo), o_ty0 = __shfl(ty0, lo), o_bw = __shfl(bw, lo);
-        const uint32_t o_ri = uint32_t(__shfl(int(ri), lo));
-        if (p < total) {
-            const int k = int(p - o_excl);
-            const uint32_t key = V.tile_base + uint32_t((o_ty0 + k / o_bw) * int(V.tiles_x) + o_tx0 + k % o_bw);
-            const uint32_t local = atomicAdd(&bin_counts[key], 1u);
-            if (base + p < V.big_cap) big_pairs[V.big_base + base + p] = make_uint4(key, local, o_ri, 0u);
-            else atomicOr(&ctr[kCtrOverflow], 4u);
+#pragma unroll
+        for (int j = 0; j < kPairBatch; ++j) {
+            const uint32_t p = p0 + 64u * uint32_t(j) + lane;
+            if (p < total) {
+                if (base + p < V.big_cap) big_pairs[V.big_base + base + p] = make_uint4(key[j], local[j], rid[j], 0u);
+                else atomicOr(&ctr[kCtrOverflow], 4u);
+            }
         }
     }
 }
@@ -821,21 +836,28 @@ __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t 
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (i < V.tri_count) {
             const uint32_t ri = V.rec_base + i;
-            const uint2 bb = boxes[ri];
-            if (bb.x == kInvalidBox) continue;
+            // the box in one 8-byte load, then the slots and every bin's offset requested together
+            const uint64_t b64 = reinterpret_cast<const uint64_t*>(boxes)[ri];
             TriRec r;
-            r.bbx = bb.x;
-            r.bby = bb.y;
+            r.bbx = uint32_t(b64);
+            r.bby = uint32_t(b64 >> 32);
+            if (r.bbx == kInvalidBox) continue;
             int tx0, ty0, tx1, ty1;
             rec_tiles(r, shift, tx0, ty0, tx1, ty1);
             const int bw = tx1 - tx0 + 1, nb = bw * (ty1 - ty0 + 1);
             if (nb > kSmallBins) continue;  // binned through the big-pair list
             const uint4 sl = reinterpret_cast<const uint4*>(pair_slots)[ri];
-            const uint32_t slots[4] = {sl.x, sl.y, sl.z, sl.w};
-            for (int k = 0; k < nb; ++k) {
-                const uint32_t key = V.tile_base + uint32_t((ty0 + k / bw) * int(V.tiles_x) + tx0 + k % bw);
-                const uint32_t pos = bin_offsets[key] + slots[k];
-                if (pos < list_cap) lists[pos] = ri;
+            uint32_t key[kSmallBins], base[kSmallBins];
+#pragma unroll
+            for (int k = 0; k < kSmallBins; ++k) {
+                key[k] = V.tile_base + uint32_t((ty0 + k / bw) * int(V.tiles_x) + tx0 + k % bw);
+                base[k] = k < nb ? bin_offsets[key[k]] : 0u;
+            }
+            const uint32_t slots[kSmallBins] = {sl.x, sl.y, sl.z, sl.w};
+#pragma unroll
+            for (int k = 0; k < kSmallBins; ++k) {
+                const uint32_t pos = base[k] + slots[k];
+                if (k < nb && pos < list_cap) lists[pos] = ri;
             }
         } else {
             const uint4 bp = big_pairs[V.big_base + (i - V.tri_count)];
