@@ -381,7 +381,9 @@ rsy_result rsy_autotune(rsy_ctx* ctx, const rsy_frame_desc* frame, uint32_t n_it
  * The call is synchronous: it returns when the frame is complete.  Every call on a GPU shares that
  * GPU's array and its contents are those of the last call: calls that overlap on one GPU (two
  * contexts on two threads, say) race.  The scene set-up calls (rsy_set_geometry, rsy_set_materials,
- * rsy_set_graphic_objects, rsy_update) still allocate, as they do for rsy_render. */
+ * rsy_set_graphic_objects, rsy_update) still allocate, as they do for rsy_render.  The draw lists
+ * must be built: after rsy_set_geometry, call rsy_set_graphic_objects before this frame
+ * (RSY_INVALID_STATE otherwise); rsy_set_materials keeps them built. */
 #define RSY_STATIC_SCRATCH_BYTES (128ull << 20)
 typedef struct rsy_frame_targets {
     uint32_t n_camera_views;     /* the frame's camera views */

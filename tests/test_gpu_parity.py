@@ -251,6 +251,18 @@ def test_draws_beyond_a_side_of_the_view_match_oracle(renderer, perspective):
     renderer.load(sc)
     check_frame(renderer, sc, colour=False)
     assert renderer.stats().triangles_setup > 0
+    # the same draws with 4 samples per pixel: sample points at 1/8 and 7/8 of a pixel lie inside
+    # the image edge too, so every sample plane still matches
+    rs = abi.RasterState()
+    C.memmove(C.byref(rs), C.byref(sc.s.raster_state), C.sizeof(rs))
+    rs.msaa_samples, rs.tile_size = 4, 32
+    renderer.update(sc.s.scene_data, rs)
+    renderer.render(sc.frame_desc(abi.SHADE_PLAIN))
+    cam = sc.camera_views()[0]
+    want = oracle.raster_view_msaa(sc, cam.view_projection[:], cam.width, cam.height, rs)
+    got = renderer.visibility(0)
+    assert np.array_equal(got, want), f"{int((got != want).sum())} sample words differ"
+    renderer.update(sc.s.scene_data, sc.s.raster_state)
 
 
 def test_four_cascades_and_multiple_views(renderer):
