@@ -242,8 +242,14 @@ rsy_result plan_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint
         L.clip_total += g.clip_cap;
         L.big_total += g.big_cap;
         for (uint32_t d = 0; d < g.n_draws; ++d) {
-            for (uint32_t s = 0; s < list[d].vtx_count; s += kTransformBlockVerts) L.vblocks.push_back({vi, d, s, g.list});
-            for (uint32_t s = 0; s < list[d].tri_count; s += kSetupBlockTris) L.tblocks.push_back({vi, d, s, g.list});
+            const DrawGPU& dr = list[d];
+            for (uint32_t s = 0; s < dr.vtx_count; s += kTransformBlockVerts)
+                L.vblocks.push_back({vi, d, s, g.list, dr.vtx_base + s, std::min(kTransformBlockVerts, dr.vtx_count - s),
+                                     g.xv_base + dr.xv_off + s, 0u, g.cull_base + d, dr.object, 0u, 0u});
+            for (uint32_t s = 0; s < dr.tri_count; s += kSetupBlockTris)
+                L.tblocks.push_back({vi, d, s, g.list, dr.first_index + 3u * s, std::min(kSetupBlockTris, dr.tri_count - s),
+                                     g.rec_base + dr.tri_base + s, g.xv_base + dr.xv_off, g.cull_base + d, dr.tri_base + s,
+                                     dr.vtx_base, dr.material});
         }
         if (is_blend) ++L.n_blend;
         else if (f.shadow) ++L.n_shadow;

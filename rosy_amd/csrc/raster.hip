@@ -224,15 +224,15 @@ __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, con
     }
     const BlockEntry e = blocks[blockIdx.x];
     const ViewGPU& V = views[e.view];
-    const DrawGPU& dr = lists.of(e.list)[e.draw];
+    const DrawGPU& dr = lists.of(e.list)[e.draw];  // its box only
     constexpr int R = int(kTransformPerThread);
     float x[R], y[R], z[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-        const uint32_t v = e.start + uint32_t(j) * 256u + threadIdx.x;
+        const uint32_t v = uint32_t(j) * 256u + threadIdx.x;
         x[j] = y[j] = z[j] = 0.f;
-        if (v < dr.vtx_count) {
-            const uint32_t g = dr.vtx_base + v;
+        if (v < e.count) {
+            const uint32_t g = e.first + v;
             x[j] = px[g];
             y[j] = py[g];
             z[j] = pz[g];
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, con
     if (threadIdx.x < 16) {
         const int col = threadIdx.x >> 2, row = threadIdx.x & 3;
         const float* a = V.view_projection;
-        const float* b = objects + size_t(dr.object) * 41u;  // 164-byte packed records
+        const float* b = objects + size_t(e.object) * 41u;  // 164-byte packed records
         float acc = a[row] * b[col * 4 + 0];
         acc = fmaf(a[4 + row], b[col * 4 + 1], acc);
         acc = fmaf(a[8 + row], b[col * 4 + 2], acc);
@@ -253,16 +253,16 @@ __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, con
         const bool outside = box_outside(threadIdx.x < 8 ? corner_outside(m, dr.bmin, dr.bmax, int(threadIdx.x)) : 0u);
         if (threadIdx.x == 0) {
             culled = outside ? 1u : 0u;
-            if (e.start == 0) draw_culled[V.cull_base + e.draw] = culled;
+            if (e.start == 0) draw_culled[e.cull] = culled;
         }
     }
     __syncthreads();
     if (culled) return;
-    float4* out = xv + size_t(V.xv_base) + dr.xv_off;
+    float4* out = xv + size_t(e.out_first);
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-        const uint32_t v = e.start + uint32_t(j) * 256u + threadIdx.x;
-        if (v < dr.vtx_count) out[v] = transform(m, x[j], y[j], z[j]);
+        const uint32_t v = uint32_t(j) * 256u + threadIdx.x;
+        if (v < e.count) out[v] = transform(m, x[j], y[j], z[j]);
     }
 }
 
@@ -329,9 +329,8 @@ constexpr uint32_t kHash = kSetupBlockTris * kSmallBins;
 static_assert(kHash == 1024u, "10-bit hash below");
 constexpr uint32_t kEmpty = 0xffffffffu;
 
-__global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const ViewGPU* views, DrawLists lists,
-                                               const uint32_t* indices, const float4* xv, TriRec* recs, uint2* boxes,
-                                               ShadeRec* shade_recs, uint2* clipq, uint32_t* counters,
+__global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const ViewGPU* views, const uint32_t* indices,
+                                               const float4* xv, TriRec* recs, uint2* boxes, ShadeRec* shade_recs, uint2* clipq, uint32_t* counters,
                                                uint32_t* bin_counts, uint32_t* pair_slots, uint4* big_pairs,
                                                uint32_t* block_valid, const uint32_t* draw_culled, uint32_t shift) {
     __shared__ uint32_t hkey[kHash];
@@ -343,27 +342,27 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     }
     const BlockEntry e = blocks[blockIdx.x];
     const ViewGPU V = views[e.view];
-    const DrawGPU dr = lists.of(e.list)[e.draw];
     uint32_t* ctr = counters + e.view * kCtrPerView;
-    const uint32_t t = e.start + threadIdx.x;
-    const bool valid = t < dr.tri_count;
-    // the triangle's indices are requested beside the draw's culling flag
+    const uint32_t t = threadIdx.x;  // the block's triangle
+    const bool valid = t < e.count;
+    // the triangle's indices are requested beside the view record and the draw's culling flag
     uint32_t vi[3] = {0u, 0u, 0u};
     if (valid) {
-        const uint32_t* ip = indices + dr.first_index + 3u * t;
+        const uint32_t* ip = indices + e.first + 3u * t;
         vi[0] = ip[0];
         vi[1] = ip[1];
         vi[2] = ip[2];
     }
-    if (draw_culled[V.cull_base + e.draw]) {  // the whole draw is outside the view: empty slots only
-        if (valid) boxes[V.rec_base + dr.tri_base + t] = make_uint2(kInvalidBox, kInvalidBox);
+    const uint32_t slot = e.out_first + t;  // every triangle owns one record slot
+    if (draw_culled[e.cull]) {  // the whole draw is outside the view: empty slots only
+        if (valid) boxes[slot] = make_uint2(kInvalidBox, kInvalidBox);
         if (threadIdx.x == 0) block_valid[blockIdx.x] = 0u;
         return;
     }
     int status = 0;  // 0 reject, 1 accept, 2 clip
     float4 c[3];
     if (valid) {
-        const float4* base = xv + size_t(V.xv_base) + dr.xv_off;
+        const float4* base = xv + size_t(e.xv_first);
         c[0] = base[vi[0]];
         c[1] = base[vi[1]];
         c[2] = base[vi[2]];
@@ -377,22 +376,21 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
         status = reject ? 0 : (all_inside(c[0], c[1], c[2]) ? 1 : 2);
     }
     TriRec r;
-    const bool has = status == 1 && setup_tri(c, dr.tri_base + t, V, r);
-    const uint32_t slot = V.rec_base + dr.tri_base + t;  // every triangle owns one record slot
+    const bool has = status == 1 && setup_tri(c, e.object + t, V, r);
     if (valid) {
         if (has) recs[slot] = r;
         boxes[slot] = has ? make_uint2(r.bbx, r.bby) : make_uint2(kInvalidBox, kInvalidBox);
         // a camera or blend view's triangle that can be visible: its shading record
         if (!V.is_shadow && status != 0) {
             uint4* sr = reinterpret_cast<uint4*>(shade_recs + slot);
-            sr[0] = make_uint4(dr.vtx_base + vi[0], dr.vtx_base + vi[1], dr.vtx_base + vi[2],
+            sr[0] = make_uint4(e.vtx_base + vi[0], e.vtx_base + vi[1], e.vtx_base + vi[2],
                                e.draw | (has ? 0u : kShadeNoRecord));
-            sr[1] = make_uint4(__float_as_uint(c[0].w), __float_as_uint(c[1].w), __float_as_uint(c[2].w), dr.material);
+            sr[1] = make_uint4(__float_as_uint(c[0].w), __float_as_uint(c[1].w), __float_as_uint(c[2].w), e.material);
         }
     }
     const uint32_t cslot = wave_append(status == 2, &ctr[kCtrClip]);
     if (status == 2) {
-        if (cslot < V.clip_cap) clipq[V.clip_base + cslot] = make_uint2(e.draw, t);
+        if (cslot < V.clip_cap) clipq[V.clip_base + cslot] = make_uint2(e.draw, e.start + t);
         else atomicOr(&ctr[kCtrOverflow], 1u);
     }
     int tx0 = 0, ty0 = 0, tx1 = -1, ty1 = -1;
@@ -1008,8 +1006,7 @@ hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
         hipLaunchKernelGGL(k_clear2, dim3(min((nclear + 255) / 256, 2048u)), dim3(256), 0, s, g.counters,
                            g.n_views * kCtrPerView, g.bin_counts, g.total_bins);
     if (g.n_tblocks)
-        hipLaunchKernelGGL(k_setup, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, lists,
-                           g.indices, g.xv, g.recs, g.boxes, g.shade_recs, g.clipq, g.counters, g.bin_counts, g.pair_slots,
+        hipLaunchKernelGGL(k_setup, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.indices, g.xv, g.recs, g.boxes, g.shade_recs, g.clipq, g.counters, g.bin_counts, g.pair_slots,
                            g.big_pairs, g.block_valid, g.draw_culled, g.shift);
     hipLaunchKernelGGL(k_clip, dim3(g.clip_blocks, g.n_views), dim3(kClipThreads), 0, s, g.views, lists,
                        g.indices, g.xv, g.clipq, g.recs, g.counters, g.bin_counts, g.big_pairs, g.shift);
