@@ -818,6 +818,66 @@ __global__ __launch_bounds__(kDirectThreads) void k_scan_direct(const uint32_t* 
     scan_write_bins<1>(v, i0, n, base, sv[0], sv[1], n_heavy, n_mid, n_split, offsets, items, items_cap, zero_bins);
 }
 
+// Bin list entries of B record slots of a view (i0, i0 + stride, ...: those below the view's
+// triangle count; records with more than kSmallBins bins go through the big-pair list): every box
+// is requested first, then every record's bin slots and bin offsets, then the entries are written
+// (a store ahead of a load would otherwise hold the next slot's loads back).  B = 2 pays when a
+// thread has more than one slot.
+template <int B>
+__device__ inline void scatter_slots(const ViewGPU& V, const uint2* boxes, const uint32_t* bin_offsets,
+                                     const uint32_t* pair_slots, uint32_t* lists, uint32_t list_cap, uint32_t shift,
+                                     uint32_t i0, uint32_t stride) {
+    uint64_t b64[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+        const uint32_t i = i0 + uint32_t(j) * stride;
+        b64[j] = i < V.tri_count ? reinterpret_cast<const uint64_t*>(boxes)[V.rec_base + i]
+                                 : (uint64_t(kInvalidBox) << 32 | kInvalidBox);
+    }
+    uint32_t nb[B], base[B][kSmallBins];
+    uint4 sl[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+        TriRec r;
+        r.bbx = uint32_t(b64[j]);
+        r.bby = uint32_t(b64[j] >> 32);
+        nb[j] = 0u;
+        sl[j] = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (int k = 0; k < kSmallBins; ++k) base[j][k] = 0u;
+        if (r.bbx != kInvalidBox) {
+            int tx0, ty0, tx1, ty1;
+            rec_tiles(r, shift, tx0, ty0, tx1, ty1);
+            const int bw = tx1 - tx0 + 1, n = bw * (ty1 - ty0 + 1);
+            if (n <= kSmallBins) {  // else binned through the big-pair list
+                nb[j] = uint32_t(n);
+                sl[j] = reinterpret_cast<const uint4*>(pair_slots)[V.rec_base + i0 + uint32_t(j) * stride];
+#pragma unroll
+                for (int k = 0; k < kSmallBins; ++k) {
+                    const uint32_t key = V.tile_base + uint32_t((ty0 + k / bw) * int(V.tiles_x) + tx0 + k % bw);
+                    base[j][k] = k < n ? bin_offsets[key] : 0u;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+        const uint32_t ri = V.rec_base + i0 + uint32_t(j) * stride;
+        const uint32_t slots[kSmallBins] = {sl[j].x, sl[j].y, sl[j].z, sl[j].w};
+#pragma unroll
+        for (int k = 0; k < kSmallBins; ++k) {
+            const uint32_t pos = base[j][k] + slots[k];
+            if (uint32_t(k) < nb[j] && pos < list_cap) lists[pos] = ri;
+        }
+    }
+}
+
+// One (bin, slot, record) entry of the big-pair list into its bin's list.
+__device__ inline void scatter_big_pair(const uint4 bp, const uint32_t* bin_offsets, uint32_t* lists, uint32_t list_cap) {
+    const uint32_t pos = bin_offsets[bp.x] + bp.y;
+    if (pos < list_cap) lists[pos] = bp.z;
+}
+
 // Bin lists without atomics: every (record, bin) pair already holds its slot inside the bin (from
 // k_setup's block table, or the big-pair list), so the entry goes to bin_offsets[bin] + slot.  The
 // same launch zeroes the output pixels of the empty bins (which get no raster work item) and of
@@ -830,37 +890,17 @@ __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t 
     const uint32_t view = blockIdx.y;
     const ViewGPU V = views[view];
     const uint32_t nbig = min(counters[view * kCtrPerView + kCtrBig], V.big_cap);
-    const uint32_t n = V.tri_count + nbig;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (i < V.tri_count) {
-            const uint32_t ri = V.rec_base + i;
-            // the box in one 8-byte load, then the slots and every bin's offset requested together
-            const uint64_t b64 = reinterpret_cast<const uint64_t*>(boxes)[ri];
-            TriRec r;
-            r.bbx = uint32_t(b64);
-            r.bby = uint32_t(b64 >> 32);
-            if (r.bbx == kInvalidBox) continue;
-            int tx0, ty0, tx1, ty1;
-            rec_tiles(r, shift, tx0, ty0, tx1, ty1);
-            const int bw = tx1 - tx0 + 1, nb = bw * (ty1 - ty0 + 1);
-            if (nb > kSmallBins) continue;  // binned through the big-pair list
-            const uint4 sl = reinterpret_cast<const uint4*>(pair_slots)[ri];
-            uint32_t key[kSmallBins], base[kSmallBins];
-#pragma unroll
-            for (int k = 0; k < kSmallBins; ++k) {
-                key[k] = V.tile_base + uint32_t((ty0 + k / bw) * int(V.tiles_x) + tx0 + k % bw);
-                base[k] = k < nb ? bin_offsets[key[k]] : 0u;
-            }
-            const uint32_t slots[kSmallBins] = {sl.x, sl.y, sl.z, sl.w};
-#pragma unroll
-            for (int k = 0; k < kSmallBins; ++k) {
-                const uint32_t pos = base[k] + slots[k];
-                if (k < nb && pos < list_cap) lists[pos] = ri;
-            }
-        } else {
-            const uint4 bp = big_pairs[V.big_base + (i - V.tri_count)];
-            const uint32_t pos = bin_offsets[bp.x] + bp.y;
-            if (pos < list_cap) lists[pos] = bp.z;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    if (V.tri_count > stride) {  // more than one record slot per thread: two at a time, then the big pairs
+        for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < V.tri_count; i0 += 2u * stride)
+            scatter_slots<2>(V, boxes, bin_offsets, pair_slots, lists, list_cap, shift, i0, stride);
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nbig; i += stride)
+            scatter_big_pair(big_pairs[V.big_base + i], bin_offsets, lists, list_cap);
+    } else {  // the record slots and then the big pairs, one grid-stride sequence
+        const uint32_t n = V.tri_count + nbig;
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+            if (i < V.tri_count) scatter_slots<1>(V, boxes, bin_offsets, pair_slots, lists, list_cap, shift, i, stride);
+            else scatter_big_pair(big_pairs[V.big_base + (i - V.tri_count)], bin_offsets, lists, list_cap);
         }
     }
     // zero the pixels of the empty and split bins: one block per bin at a time (the bin, its view
