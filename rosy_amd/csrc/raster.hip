@@ -380,8 +380,9 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     if (valid) {
         if (has) recs[slot] = r;
         boxes[slot] = has ? make_uint2(r.bbx, r.bby) : make_uint2(kInvalidBox, kInvalidBox);
-        // a camera or blend view's triangle that can be visible: its shading record
-        if (!V.is_shadow && status != 0) {
+        // a camera or blend view's triangle that a pixel can show (set up into a raster record, or
+        // queued for clipping): its shading record; culled and boxless triangles need none
+        if (!V.is_shadow && (has || status == 2)) {
             uint4* sr = reinterpret_cast<uint4*>(shade_recs + slot);
             sr[0] = make_uint4(e.vtx_base + vi[0], e.vtx_base + vi[1], e.vtx_base + vi[2],
                                e.draw | (has ? 0u : kShadeNoRecord));
