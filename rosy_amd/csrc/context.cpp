@@ -25,6 +25,40 @@ using namespace rsy;
 
 namespace {
 
+// Orders a frame's setup blocks so that the blocks of different views that read the same indices
+// (the same `first`) run on the same XCD, one after another, and share its L2: workgroups are dealt
+// to the 8 XCDs round-robin, so within each group of 8 chunks the order is (view rank, chunk) and a
+// chunk's blocks are 8 ids apart.  The order changes no result: every block writes its own slots,
+// and bin counts are sums.
+void interleave_views(std::vector<BlockEntry>& blocks) {
+    constexpr uint32_t kXcds = 8;
+    std::vector<uint32_t> chunk(blocks.size()), rank(blocks.size());
+    {
+        std::vector<std::pair<uint32_t, uint32_t>> order;  // (first, block)
+        order.reserve(blocks.size());
+        for (uint32_t i = 0; i < blocks.size(); ++i) order.push_back({blocks[i].first, i});
+        std::stable_sort(order.begin(), order.end());
+        uint32_t id = 0;
+        for (size_t k = 0; k < order.size(); ++k) {
+            if (k > 0 && order[k].first != order[k - 1].first) ++id;
+            chunk[order[k].second] = id;
+            rank[order[k].second] = (k > 0 && order[k].first == order[k - 1].first) ? rank[order[k - 1].second] + 1 : 0;
+        }
+    }
+    std::vector<uint32_t> idx(blocks.size());
+    for (uint32_t i = 0; i < idx.size(); ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+        const uint32_t ga = chunk[a] / kXcds, gb = chunk[b] / kXcds;
+        if (ga != gb) return ga < gb;
+        if (rank[a] != rank[b]) return rank[a] < rank[b];
+        return chunk[a] < chunk[b];
+    });
+    std::vector<BlockEntry> out;
+    out.reserve(blocks.size());
+    for (uint32_t i : idx) out.push_back(blocks[i]);
+    blocks.swap(out);
+}
+
 // kDraw* flags of a draw with material index `material` under the context's materials and samplers.
 uint32_t draw_flags(const rsy_ctx* c, uint32_t material) {
     uint32_t flags = 0;
@@ -257,6 +291,7 @@ rsy_result plan_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint
         L.vg.push_back(g);
     }
     if (L.total_xv >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "transformed vertex buffer too large");
+    interleave_views(L.tblocks);  // (the transform keeps view order: its output writes stream per view)
     // bin lists: an estimate of the frame's (triangle, bin) pairs
     uint64_t want_pairs = 0;
     const uint64_t per_tri = tile < 32 ? 3u * (32 / tile) * (32 / tile) : 3u;
