@@ -339,7 +339,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     HIP_TRY(c, c->bin_counts.ensure(size_t(L.total_bins) * 4), "allocate bins");
     HIP_TRY(c, c->bin_offsets.ensure(size_t(L.total_bins) * 4), "allocate bins");
     HIP_TRY(c, c->pair_slots.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * 16), "allocate bin slots");
-    HIP_TRY(c, c->boxes.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * 8), "allocate record boxes");
+    HIP_TRY(c, c->bin_tiles.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * 4), "allocate record bins");
     HIP_TRY(c, c->shade_recs.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * sizeof(ShadeRec)), "allocate shading records");
     HIP_TRY(c, c->big_pairs.ensure(size_t(std::max<uint32_t>(1, L.big_total)) * 16), "allocate bin pairs");
     HIP_TRY(c, c->block_valid.ensure(std::max<size_t>(1, L.tblocks.size()) * 4), "allocate statistics");
@@ -408,7 +408,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     m.views = c->views_d.as<ViewGPU>();
     m.xv = c->xv.as<float4>();
     m.recs = c->recs.as<TriRec>();
-    m.boxes = c->boxes.as<uint2>();
+    m.bin_tiles = c->bin_tiles.as<uint32_t>();
     m.shade_recs = c->shade_recs.as<ShadeRec>();
     m.clipq = c->clipq.as<uint2>();
     m.counters = c->counters.as<uint32_t>();
@@ -553,7 +553,7 @@ rsy_result enqueue_frame(rsy_ctx* c, const Layout& L, const FrameMem& m, bool ca
     g.indices = c->indices.as<uint32_t>();
     g.xv = m.xv;
     g.recs = m.recs;
-    g.boxes = m.boxes;
+    g.bin_tiles = m.bin_tiles;
     g.shade_recs = m.shade_recs;
     g.clipq = m.clipq;
     g.counters = m.counters;
@@ -580,7 +580,7 @@ rsy_result enqueue_frame(rsy_ctx* c, const Layout& L, const FrameMem& m, bool ca
     b.n_tblocks = uint32_t(L.tblocks.size());
     b.totals = m.totals;
     b.recs = g.recs;
-    b.boxes = g.boxes;
+    b.bin_tiles = g.bin_tiles;
     b.counters = g.counters;
     b.lists = m.lists;
     b.list_cap = m.list_cap;
@@ -1277,7 +1277,7 @@ rsy_result rsy_render_noalloc(rsy_ctx* c, const rsy_frame_desc* frame, const rsy
     m.scan_partial = take((size_t(L.total_bins) / kScanBlockBins + 1) * kScanPartialBytes);
     m.xv = reinterpret_cast<float4*>(take(size_t(L.total_xv) * sizeof(float4)));
     m.recs = reinterpret_cast<TriRec*>(take(size_t(L.rec_total) * sizeof(TriRec)));
-    m.boxes = reinterpret_cast<uint2*>(take(size_t(L.rec_total) * 8));
+    m.bin_tiles = reinterpret_cast<uint32_t*>(take(size_t(L.rec_total) * 4));
     m.shade_recs = reinterpret_cast<ShadeRec*>(take(size_t(L.rec_total) * sizeof(ShadeRec)));
     m.pair_slots = reinterpret_cast<uint32_t*>(take(size_t(L.rec_total) * 16));
     m.clipq = reinterpret_cast<uint2*>(take(size_t(L.clip_total) * sizeof(uint2)));

@@ -135,7 +135,7 @@ struct FrameMem {
     ViewGPU* views = nullptr;
     float4* xv = nullptr;
     TriRec* recs = nullptr;
-    uint2* boxes = nullptr;
+    uint32_t* bin_tiles = nullptr;
     ShadeRec* shade_recs = nullptr;
     uint2* clipq = nullptr;
     uint32_t* counters = nullptr;
@@ -234,7 +234,7 @@ struct rsy_ctx {
     rsy::FrameMem mem;            // the ordinary frame's addresses (into the buffers below)
     rsy::Layout static_layout;    // the allocation-free frame's layout (host tables only)
     std::vector<rsy::FrameViewDesc> static_views;
-    rsy::DevBuf views_d, xv, recs, boxes, shade_recs, clipq, counters, bin_counts, bin_offsets, pair_slots, big_pairs, block_valid, totals,
+    rsy::DevBuf views_d, xv, recs, bin_tiles, shade_recs, clipq, counters, bin_counts, bin_offsets, pair_slots, big_pairs, block_valid, totals,
         lists_mem, vblocks_d, tblocks_d, shade_views_d, shade_cascades_d, items, zero_bins,
         draw_culled, scan_partial;
     uint32_t list_cap = 0;

@@ -25,7 +25,7 @@ struct alignas(16) TriRec {
     float dzdx;              // per sub-pixel unit
     float dzdy;
     uint32_t id;             // triangle id in the view's draw list | kRecSwapped
-    uint32_t bbx;            // px0 | px1 << 16 (inclusive pixel bounds); kInvalidBox: no record
+    uint32_t bbx;            // px0 | px1 << 16 (inclusive pixel bounds)
     uint32_t bby;            // py0 | py1 << 16
 };
 static_assert(sizeof(TriRec) == 48, "TriRec is 48 bytes");
@@ -34,7 +34,6 @@ static_assert(sizeof(TriRec) == 48, "TriRec is 48 bytes");
 // were counter-clockwise has vertices 1 and 2 exchanged, and its id carries this bit.
 constexpr uint32_t kRecSwapped = 0x80000000u;
 constexpr uint32_t kRecIdMask = 0x7fffffffu;
-constexpr uint32_t kInvalidBox = 0xffffffffu;  // bbx of a triangle slot that produced no record
 
 // What shading needs of a camera or blend view's triangle, written by setup into the triangle's slot
 // (32 bytes, two 16-byte loads): the three vertices' indices into the global vertex buffer, the

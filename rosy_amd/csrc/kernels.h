@@ -37,7 +37,7 @@ struct GeometryLaunch {
     const uint32_t* indices;
     float4* xv;
     TriRec* recs;
-    uint2* boxes;           // per record slot: the record's (bbx, bby), or kInvalidBox twice
+    uint32_t* bin_tiles;    // per record slot: its bins for the scatter (pack_bins), or kNoBins
     ShadeRec* shade_recs;   // per record slot of a camera or blend view: its shading record
     uint2* clipq;
     uint32_t* counters;
@@ -60,7 +60,7 @@ struct BinLaunch {
     const uint32_t* pair_slots;
     const uint4* big_pairs;
     const uint32_t* block_valid;
-    const uint2* boxes;
+    const uint32_t* bin_tiles;
     uint32_t n_tblocks;
     uint32_t* totals;  // [0] (record, bin) pairs, [1] records set up, [2] overflow bits (lists, raster grid),
                        // [3] raster work items, [4] first mid-class item, [5] bins to clear,

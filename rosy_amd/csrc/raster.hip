@@ -121,6 +121,14 @@ __device__ inline uint32_t wave_append(bool want, uint32_t* counter) {
     return base + uint32_t(__popcll(mask & lanemask_lt()));
 }
 
+// A record slot's bins for the scatter, in one word: the first bin's column and row (13 bits each)
+// and the width and height in bins less one (2 bits each; a record binned this way covers at most
+// kSmallBins bins).  kNoBins: the slot has no record, or its record goes through the big-pair list.
+constexpr uint32_t kNoBins = 0xffffffffu;
+__device__ inline uint32_t pack_bins(int tx0, int ty0, int bw, int bh) {
+    return uint32_t(tx0) | (uint32_t(ty0) << 13) | (uint32_t(bw - 1) << 26) | (uint32_t(bh - 1) << 28);
+}
+
 __device__ inline void rec_tiles(const TriRec& r, uint32_t shift, int& tx0, int& ty0, int& tx1, int& ty1) {
     tx0 = int(r.bbx & 0xffffu) >> shift;
     tx1 = int(r.bbx >> 16) >> shift;
@@ -330,7 +338,8 @@ static_assert(kHash == 1024u, "10-bit hash below");
 constexpr uint32_t kEmpty = 0xffffffffu;
 
 __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const ViewGPU* views, const uint32_t* indices,
-                                               const float4* xv, TriRec* recs, uint2* boxes, ShadeRec* shade_recs, uint2* clipq, uint32_t* counters,
+                                               const float4* xv, TriRec* recs, uint32_t* bin_tiles, ShadeRec* shade_recs,
+                                               uint2* clipq, uint32_t* counters,
                                                uint32_t* bin_counts, uint32_t* pair_slots, uint4* big_pairs,
                                                uint32_t* block_valid, const uint32_t* draw_culled, uint32_t shift) {
     __shared__ uint32_t hkey[kHash];
@@ -355,7 +364,7 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     }
     const uint32_t slot = e.out_first + t;  // every triangle owns one record slot
     if (draw_culled[e.cull]) {  // the whole draw is outside the view: empty slots only
-        if (valid) boxes[slot] = make_uint2(kInvalidBox, kInvalidBox);
+        if (valid) bin_tiles[slot] = kNoBins;
         if (threadIdx.x == 0) block_valid[blockIdx.x] = 0u;
         return;
     }
@@ -379,7 +388,6 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     const bool has = status == 1 && setup_tri(c, e.object + t, V, r);
     if (valid) {
         if (has) recs[slot] = r;
-        boxes[slot] = has ? make_uint2(r.bbx, r.bby) : make_uint2(kInvalidBox, kInvalidBox);
         // a camera or blend view's triangle that a pixel can show (set up into a raster record, or
         // queued for clipping): its shading record; culled and boxless triangles need none
         if (!V.is_shadow && (has || status == 2)) {
@@ -399,6 +407,7 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     const int bw = tx1 - tx0 + 1;
     const int n = has ? bw * (ty1 - ty0 + 1) : 0;
     const bool small = has && n <= kSmallBins;
+    if (valid) bin_tiles[slot] = small ? pack_bins(tx0, ty0, bw, ty1 - ty0 + 1) : kNoBins;
     __syncthreads();  // table cleared
     // small records: rank within the block for each bin, through the LDS table
     uint32_t hidx[kSmallBins], rank[kSmallBins];
@@ -820,44 +829,37 @@ __global__ __launch_bounds__(kDirectThreads) void k_scan_direct(const uint32_t* 
 }
 
 // Bin list entries of B record slots of a view (i0, i0 + stride, ...: those below the view's
-// triangle count; records with more than kSmallBins bins go through the big-pair list): every box
-// is requested first, then every record's bin slots and bin offsets, then the entries are written
-// (a store ahead of a load would otherwise hold the next slot's loads back).  B = 2 pays when a
-// thread has more than one slot.
+// triangle count; records with more than kSmallBins bins go through the big-pair list): every
+// slot's bin word is requested first, then every record's bin slots and bin offsets, then the
+// entries are written (a store ahead of a load would otherwise hold the next slot's loads back).
+// B = 2 pays when a thread has more than one slot.
 template <int B>
-__device__ inline void scatter_slots(const ViewGPU& V, const uint2* boxes, const uint32_t* bin_offsets,
-                                     const uint32_t* pair_slots, uint32_t* lists, uint32_t list_cap, uint32_t shift,
-                                     uint32_t i0, uint32_t stride) {
-    uint64_t b64[B];
+__device__ inline void scatter_slots(const ViewGPU& V, const uint32_t* bin_tiles, const uint32_t* bin_offsets,
+                                     const uint32_t* pair_slots, uint32_t* lists, uint32_t list_cap, uint32_t i0,
+                                     uint32_t stride) {
+    uint32_t w[B];
 #pragma unroll
     for (int j = 0; j < B; ++j) {
         const uint32_t i = i0 + uint32_t(j) * stride;
-        b64[j] = i < V.tri_count ? reinterpret_cast<const uint64_t*>(boxes)[V.rec_base + i]
-                                 : (uint64_t(kInvalidBox) << 32 | kInvalidBox);
+        w[j] = i < V.tri_count ? bin_tiles[V.rec_base + i] : kNoBins;
     }
     uint32_t nb[B], base[B][kSmallBins];
     uint4 sl[B];
 #pragma unroll
     for (int j = 0; j < B; ++j) {
-        TriRec r;
-        r.bbx = uint32_t(b64[j]);
-        r.bby = uint32_t(b64[j] >> 32);
         nb[j] = 0u;
         sl[j] = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
         for (int k = 0; k < kSmallBins; ++k) base[j][k] = 0u;
-        if (r.bbx != kInvalidBox) {
-            int tx0, ty0, tx1, ty1;
-            rec_tiles(r, shift, tx0, ty0, tx1, ty1);
-            const int bw = tx1 - tx0 + 1, n = bw * (ty1 - ty0 + 1);
-            if (n <= kSmallBins) {  // else binned through the big-pair list
-                nb[j] = uint32_t(n);
-                sl[j] = reinterpret_cast<const uint4*>(pair_slots)[V.rec_base + i0 + uint32_t(j) * stride];
+        if (w[j] != kNoBins) {
+            const int tx0 = int(w[j] & 0x1fffu), ty0 = int((w[j] >> 13) & 0x1fffu);
+            const int bw = int((w[j] >> 26) & 3u) + 1, bh = int((w[j] >> 28) & 3u) + 1;
+            nb[j] = uint32_t(bw * bh);
+            sl[j] = reinterpret_cast<const uint4*>(pair_slots)[V.rec_base + i0 + uint32_t(j) * stride];
 #pragma unroll
-                for (int k = 0; k < kSmallBins; ++k) {
-                    const uint32_t key = V.tile_base + uint32_t((ty0 + k / bw) * int(V.tiles_x) + tx0 + k % bw);
-                    base[j][k] = k < n ? bin_offsets[key] : 0u;
-                }
+            for (int k = 0; k < kSmallBins; ++k) {
+                const uint32_t key = V.tile_base + uint32_t((ty0 + k / bw) * int(V.tiles_x) + tx0 + k % bw);
+                base[j][k] = uint32_t(k) < nb[j] ? bin_offsets[key] : 0u;
             }
         }
     }
@@ -883,7 +885,7 @@ __device__ inline void scatter_big_pair(const uint4 bp, const uint32_t* bin_offs
 // k_setup's block table, or the big-pair list), so the entry goes to bin_offsets[bin] + slot.  The
 // same launch zeroes the output pixels of the empty bins (which get no raster work item) and of
 // the split bins (which k_raster then merges into).
-__global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t n_views, const uint2* boxes,
+__global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t n_views, const uint32_t* bin_tiles,
                                                  const uint32_t* counters, const uint32_t* bin_offsets,
                                                  const uint32_t* pair_slots, const uint4* big_pairs, uint32_t* lists,
                                                  uint32_t list_cap, uint32_t shift, const uint32_t* totals,
@@ -894,13 +896,13 @@ __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t 
     const uint32_t stride = gridDim.x * blockDim.x;
     if (V.tri_count > stride) {  // more than one record slot per thread: two at a time, then the big pairs
         for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < V.tri_count; i0 += 2u * stride)
-            scatter_slots<2>(V, boxes, bin_offsets, pair_slots, lists, list_cap, shift, i0, stride);
+            scatter_slots<2>(V, bin_tiles, bin_offsets, pair_slots, lists, list_cap, i0, stride);
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nbig; i += stride)
             scatter_big_pair(big_pairs[V.big_base + i], bin_offsets, lists, list_cap);
     } else {  // the record slots and then the big pairs, one grid-stride sequence
         const uint32_t n = V.tri_count + nbig;
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-            if (i < V.tri_count) scatter_slots<1>(V, boxes, bin_offsets, pair_slots, lists, list_cap, shift, i, stride);
+            if (i < V.tri_count) scatter_slots<1>(V, bin_tiles, bin_offsets, pair_slots, lists, list_cap, i, stride);
             else scatter_big_pair(big_pairs[V.big_base + (i - V.tri_count)], bin_offsets, lists, list_cap);
         }
     }
@@ -1047,7 +1049,7 @@ hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
         hipLaunchKernelGGL(k_clear2, dim3(min((nclear + 255) / 256, 2048u)), dim3(256), 0, s, g.counters,
                            g.n_views * kCtrPerView, g.bin_counts, g.total_bins);
     if (g.n_tblocks)
-        hipLaunchKernelGGL(k_setup, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.indices, g.xv, g.recs, g.boxes, g.shade_recs, g.clipq, g.counters, g.bin_counts, g.pair_slots,
+        hipLaunchKernelGGL(k_setup, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.indices, g.xv, g.recs, g.bin_tiles, g.shade_recs, g.clipq, g.counters, g.bin_counts, g.pair_slots,
                            g.big_pairs, g.block_valid, g.draw_culled, g.shift);
     hipLaunchKernelGGL(k_clip, dim3(g.clip_blocks, g.n_views), dim3(kClipThreads), 0, s, g.views, lists,
                        g.indices, g.xv, g.clipq, g.recs, g.counters, g.bin_counts, g.big_pairs, g.shift);
@@ -1068,7 +1070,7 @@ hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
                            b.bin_offsets, partial, parts, b.list_cap, b.totals, b.items, b.items_cap, b.zero_bins,
                            b.raster_grid);
     }
-    hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.n_views, b.boxes,
+    hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.n_views, b.bin_tiles,
                        b.counters, b.bin_offsets, b.pair_slots, b.big_pairs, b.lists, b.list_cap, b.shift, b.totals,
                        b.zero_bins);
     return hipGetLastError();
