@@ -559,6 +559,7 @@ rsy_result enqueue_frame(rsy_ctx* c, const Layout& L, const FrameMem& m, bool ca
     g.counters = m.counters;
     g.bin_counts = m.bin_counts;
     g.pair_slots = m.pair_slots;
+    g.slot_count = L.rec_total;
     g.big_pairs = m.big_pairs;
     g.block_valid = m.block_valid;
     g.draw_culled = m.draw_culled;
@@ -575,6 +576,7 @@ rsy_result enqueue_frame(rsy_ctx* c, const Layout& L, const FrameMem& m, bool ca
     b.bin_counts = g.bin_counts;
     b.bin_offsets = m.bin_offsets;
     b.pair_slots = m.pair_slots;
+    b.slot_count = L.rec_total;
     b.big_pairs = m.big_pairs;
     b.block_valid = m.block_valid;
     b.n_tblocks = uint32_t(L.tblocks.size());

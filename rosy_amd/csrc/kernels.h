@@ -42,7 +42,9 @@ struct GeometryLaunch {
     uint2* clipq;
     uint32_t* counters;
     uint32_t* bin_counts;
-    uint32_t* pair_slots;   // per record slot: its slot inside each of its (at most 4) bins
+    uint32_t* pair_slots;   // per record slot: its slot inside each of its (at most 4) bins, as two
+                            // uint2 arrays of slot_count entries (bins 0-1, then bins 2-3)
+    uint32_t slot_count;    // record slots of the frame (all views)
     uint4* big_pairs;       // (bin, slot, record) of records with more bins
     uint32_t* block_valid;  // records set up per setup block
     uint32_t* draw_culled;  // per (view, draw): 1 when the draw's box is outside the view
@@ -58,6 +60,7 @@ struct BinLaunch {
     uint32_t* bin_counts;
     uint32_t* bin_offsets;
     const uint32_t* pair_slots;
+    uint32_t slot_count;
     const uint4* big_pairs;
     const uint32_t* block_valid;
     const uint32_t* bin_tiles;

@@ -340,7 +340,8 @@ constexpr uint32_t kEmpty = 0xffffffffu;
 __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const ViewGPU* views, const uint32_t* indices,
                                                const float4* xv, TriRec* recs, uint32_t* bin_tiles, ShadeRec* shade_recs,
                                                uint2* clipq, uint32_t* counters,
-                                               uint32_t* bin_counts, uint32_t* pair_slots, uint4* big_pairs,
+                                               uint32_t* bin_counts, uint32_t* pair_slots, uint32_t slot_count,
+                                               uint4* big_pairs,
                                                uint32_t* block_valid, const uint32_t* draw_culled, uint32_t shift) {
     __shared__ uint32_t hkey[kHash];
     __shared__ uint32_t hcount[kHash];  // per key: records of the block, then the block's first slot
@@ -432,13 +433,10 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     for (uint32_t i = threadIdx.x; i < kHash; i += 256)
         if (hkey[i] != kEmpty) hcount[i] = atomicAdd(&bin_counts[hkey[i]], hcount[i]);
     __syncthreads();
-    if (small) {
-        uint4 out = make_uint4(0u, 0u, 0u, 0u);
-        out.x = hcount[hidx[0]] + rank[0];
-        if (n > 1) out.y = hcount[hidx[1]] + rank[1];
-        if (n > 2) out.z = hcount[hidx[2]] + rank[2];
-        if (n > 3) out.w = hcount[hidx[3]] + rank[3];
-        reinterpret_cast<uint4*>(pair_slots)[slot] = out;
+    if (small) {  // slots in bins 0-1, and in bins 2-3 only for a record with more than two bins
+        uint2* s01 = reinterpret_cast<uint2*>(pair_slots);
+        s01[slot] = make_uint2(hcount[hidx[0]] + rank[0], n > 1 ? hcount[hidx[1]] + rank[1] : 0u);
+        if (n > 2) s01[slot_count + slot] = make_uint2(hcount[hidx[2]] + rank[2], n > 3 ? hcount[hidx[3]] + rank[3] : 0u);
     }
     // records with many bins: the whole wave spreads their bins
     bin_wide(has && !small, r, slot, V, shift, bin_counts, big_pairs, ctr);
@@ -835,8 +833,8 @@ __global__ __launch_bounds__(kDirectThreads) void k_scan_direct(const uint32_t* 
 // B = 2 pays when a thread has more than one slot.
 template <int B>
 __device__ inline void scatter_slots(const ViewGPU& V, const uint32_t* bin_tiles, const uint32_t* bin_offsets,
-                                     const uint32_t* pair_slots, uint32_t* lists, uint32_t list_cap, uint32_t i0,
-                                     uint32_t stride) {
+                                     const uint32_t* pair_slots, uint32_t slot_count, uint32_t* lists,
+                                     uint32_t list_cap, uint32_t i0, uint32_t stride) {
     uint32_t w[B];
 #pragma unroll
     for (int j = 0; j < B; ++j) {
@@ -855,7 +853,11 @@ __device__ inline void scatter_slots(const ViewGPU& V, const uint32_t* bin_tiles
             const int tx0 = int(w[j] & 0x1fffu), ty0 = int((w[j] >> 13) & 0x1fffu);
             const int bw = int((w[j] >> 26) & 3u) + 1, bh = int((w[j] >> 28) & 3u) + 1;
             nb[j] = uint32_t(bw * bh);
-            sl[j] = reinterpret_cast<const uint4*>(pair_slots)[V.rec_base + i0 + uint32_t(j) * stride];
+            const uint2* s01 = reinterpret_cast<const uint2*>(pair_slots);
+            const uint32_t ri = V.rec_base + i0 + uint32_t(j) * stride;
+            const uint2 a = s01[ri];
+            const uint2 b = nb[j] > 2u ? s01[slot_count + ri] : make_uint2(0u, 0u);
+            sl[j] = make_uint4(a.x, a.y, b.x, b.y);
 #pragma unroll
             for (int k = 0; k < kSmallBins; ++k) {
                 const uint32_t key = V.tile_base + uint32_t((ty0 + k / bw) * int(V.tiles_x) + tx0 + k % bw);
@@ -887,7 +889,8 @@ __device__ inline void scatter_big_pair(const uint4 bp, const uint32_t* bin_offs
 // the split bins (which k_raster then merges into).
 __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t n_views, const uint32_t* bin_tiles,
                                                  const uint32_t* counters, const uint32_t* bin_offsets,
-                                                 const uint32_t* pair_slots, const uint4* big_pairs, uint32_t* lists,
+                                                 const uint32_t* pair_slots, uint32_t slot_count,
+                                                 const uint4* big_pairs, uint32_t* lists,
                                                  uint32_t list_cap, uint32_t shift, const uint32_t* totals,
                                                  const uint32_t* zero_bins) {
     const uint32_t view = blockIdx.y;
@@ -896,13 +899,14 @@ __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t 
     const uint32_t stride = gridDim.x * blockDim.x;
     if (V.tri_count > stride) {  // more than one record slot per thread: two at a time, then the big pairs
         for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < V.tri_count; i0 += 2u * stride)
-            scatter_slots<2>(V, bin_tiles, bin_offsets, pair_slots, lists, list_cap, i0, stride);
+            scatter_slots<2>(V, bin_tiles, bin_offsets, pair_slots, slot_count, lists, list_cap, i0, stride);
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nbig; i += stride)
             scatter_big_pair(big_pairs[V.big_base + i], bin_offsets, lists, list_cap);
     } else {  // the record slots and then the big pairs, one grid-stride sequence
         const uint32_t n = V.tri_count + nbig;
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-            if (i < V.tri_count) scatter_slots<1>(V, bin_tiles, bin_offsets, pair_slots, lists, list_cap, i, stride);
+            if (i < V.tri_count)
+                scatter_slots<1>(V, bin_tiles, bin_offsets, pair_slots, slot_count, lists, list_cap, i, stride);
             else scatter_big_pair(big_pairs[V.big_base + (i - V.tri_count)], bin_offsets, lists, list_cap);
         }
     }
@@ -1049,7 +1053,8 @@ hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
         hipLaunchKernelGGL(k_clear2, dim3(min((nclear + 255) / 256, 2048u)), dim3(256), 0, s, g.counters,
                            g.n_views * kCtrPerView, g.bin_counts, g.total_bins);
     if (g.n_tblocks)
-        hipLaunchKernelGGL(k_setup, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.indices, g.xv, g.recs, g.bin_tiles, g.shade_recs, g.clipq, g.counters, g.bin_counts, g.pair_slots,
+        hipLaunchKernelGGL(k_setup, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.indices, g.xv, g.recs,
+                           g.bin_tiles, g.shade_recs, g.clipq, g.counters, g.bin_counts, g.pair_slots, g.slot_count,
                            g.big_pairs, g.block_valid, g.draw_culled, g.shift);
     hipLaunchKernelGGL(k_clip, dim3(g.clip_blocks, g.n_views), dim3(kClipThreads), 0, s, g.views, lists,
                        g.indices, g.xv, g.clipq, g.recs, g.counters, g.bin_counts, g.big_pairs, g.shift);
@@ -1071,8 +1076,8 @@ hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
                            b.raster_grid);
     }
     hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.n_views, b.bin_tiles,
-                       b.counters, b.bin_offsets, b.pair_slots, b.big_pairs, b.lists, b.list_cap, b.shift, b.totals,
-                       b.zero_bins);
+                       b.counters, b.bin_offsets, b.pair_slots, b.slot_count, b.big_pairs, b.lists, b.list_cap, b.shift,
+                       b.totals, b.zero_bins);
     return hipGetLastError();
 }
 
