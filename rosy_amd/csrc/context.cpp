@@ -333,7 +333,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     HIP_TRY(c, c->views_d.ensure(nv * sizeof(ViewGPU)), "allocate views");
     HIP_TRY(c, c->xv.ensure(size_t(std::max<uint64_t>(1, L.total_xv)) * sizeof(float4)), "allocate vertices");
     HIP_TRY(c, c->recs.ensure(size_t(std::max<uint32_t>(1, L.rec_total)) * sizeof(TriRec)), "allocate records");
-    HIP_TRY(c, c->clipq.ensure(size_t(std::max<uint32_t>(1, L.clip_total)) * sizeof(uint2)), "allocate clip queue");
+    HIP_TRY(c, c->clipq.ensure(size_t(std::max<uint32_t>(1, L.clip_total)) * sizeof(ClipEntry)), "allocate clip queue");
     HIP_TRY(c, c->counters.ensure(nv * kCtrPerView * sizeof(uint32_t)), "allocate counters");
     HIP_TRY(c, c->draw_culled.ensure(size_t(std::max<uint32_t>(1, L.cull_total)) * 4), "allocate culling flags");
     HIP_TRY(c, c->bin_counts.ensure(size_t(L.total_bins) * 4), "allocate bins");
@@ -410,7 +410,7 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     m.recs = c->recs.as<TriRec>();
     m.bin_tiles = c->bin_tiles.as<uint32_t>();
     m.shade_recs = c->shade_recs.as<ShadeRec>();
-    m.clipq = c->clipq.as<uint2>();
+    m.clipq = c->clipq.as<ClipEntry>();
     m.counters = c->counters.as<uint32_t>();
     m.draw_culled = c->draw_culled.as<uint32_t>();
     m.bin_counts = c->bin_counts.as<uint32_t>();
@@ -1282,7 +1282,7 @@ rsy_result rsy_render_noalloc(rsy_ctx* c, const rsy_frame_desc* frame, const rsy
     m.bin_tiles = reinterpret_cast<uint32_t*>(take(size_t(L.rec_total) * 4));
     m.shade_recs = reinterpret_cast<ShadeRec*>(take(size_t(L.rec_total) * sizeof(ShadeRec)));
     m.pair_slots = reinterpret_cast<uint32_t*>(take(size_t(L.rec_total) * 16));
-    m.clipq = reinterpret_cast<uint2*>(take(size_t(L.clip_total) * sizeof(uint2)));
+    m.clipq = reinterpret_cast<ClipEntry*>(take(size_t(L.clip_total) * sizeof(ClipEntry)));
     m.big_pairs = reinterpret_cast<uint4*>(take(size_t(L.big_total) * 16));
     // the bin lists and work items take the rest: at least the frame's estimated pairs
     const auto need_for = [&](uint64_t pairs) {  // with both pieces' alignment slack

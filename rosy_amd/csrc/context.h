@@ -137,7 +137,7 @@ struct FrameMem {
     TriRec* recs = nullptr;
     uint32_t* bin_tiles = nullptr;
     ShadeRec* shade_recs = nullptr;
-    uint2* clipq = nullptr;
+    ClipEntry* clipq = nullptr;
     uint32_t* counters = nullptr;
     uint32_t* draw_culled = nullptr;
     uint32_t* bin_counts = nullptr;

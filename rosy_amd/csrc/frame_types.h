@@ -48,6 +48,15 @@ struct ShadeRec {
 static_assert(sizeof(ShadeRec) == 32, "ShadeRec is 32 bytes");
 constexpr uint32_t kShadeNoRecord = 0x80000000u;
 
+// A triangle queued for clipping (64 bytes, four 16-byte words): its three clip-space vertices as
+// the transform wrote them and its triangle id, so that the clipper reads one entry instead of
+// walking draw, indices and vertices.
+struct ClipEntry {
+    float4 v[3];
+    uint32_t id, pad[3];
+};
+static_assert(sizeof(ClipEntry) == 64, "ClipEntry is 64 bytes");
+
 // Per-draw data of one draw list (shadow or opaque), in list order.
 struct DrawGPU {
     uint32_t first_index;  // mesh.index_offset + start_index

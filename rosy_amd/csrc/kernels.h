@@ -39,7 +39,7 @@ struct GeometryLaunch {
     TriRec* recs;
     uint32_t* bin_tiles;    // per record slot: its bins for the scatter (pack_bins), or kNoBins
     ShadeRec* shade_recs;   // per record slot of a camera or blend view: its shading record
-    uint2* clipq;
+    ClipEntry* clipq;
     uint32_t* counters;
     uint32_t* bin_counts;
     uint32_t* pair_slots;   // per record slot: its slot inside each of its (at most 4) bins, as two

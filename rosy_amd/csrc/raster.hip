@@ -339,7 +339,7 @@ constexpr uint32_t kEmpty = 0xffffffffu;
 
 __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const ViewGPU* views, const uint32_t* indices,
                                                const float4* xv, TriRec* recs, uint32_t* bin_tiles, ShadeRec* shade_recs,
-                                               uint2* clipq, uint32_t* counters,
+                                               ClipEntry* clipq, uint32_t* counters,
                                                uint32_t* bin_counts, uint32_t* pair_slots, uint32_t slot_count,
                                                uint4* big_pairs,
                                                uint32_t* block_valid, const uint32_t* draw_culled, uint32_t shift) {
@@ -400,8 +400,15 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     }
     const uint32_t cslot = wave_append(status == 2, &ctr[kCtrClip]);
     if (status == 2) {
-        if (cslot < V.clip_cap) clipq[V.clip_base + cslot] = make_uint2(e.draw, e.start + t);
-        else atomicOr(&ctr[kCtrOverflow], 1u);
+        if (cslot < V.clip_cap) {
+            ClipEntry& q = clipq[V.clip_base + cslot];
+            q.v[0] = c[0];
+            q.v[1] = c[1];
+            q.v[2] = c[2];
+            q.id = e.object + t;
+        } else {
+            atomicOr(&ctr[kCtrOverflow], 1u);
+        }
     }
     int tx0 = 0, ty0 = 0, tx1 = -1, ty1 = -1;
     if (has) rec_tiles(r, shift, tx0, ty0, tx1, ty1);
@@ -456,8 +463,7 @@ constexpr int kClipThreads = 128;
 // oracle's clipper holds the same bound), so the polygon never leaves its thread's slice.
 constexpr int kClipMax = 9;
 
-__global__ __launch_bounds__(kClipThreads) void k_clip(const ViewGPU* views, DrawLists lists,
-                                                       const uint32_t* indices, const float4* xv, const uint2* clipq, TriRec* recs,
+__global__ __launch_bounds__(kClipThreads) void k_clip(const ViewGPU* views, const ClipEntry* clipq, TriRec* recs,
                                                        uint32_t* counters, uint32_t* bin_counts, uint4* big_pairs,
                                                        uint32_t shift) {
     __shared__ float4 poly[kClipThreads][2][kClipMax];
@@ -473,14 +479,11 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(const ViewGPU* views, Dra
         int cnt = 0;
         uint32_t id = 0;
         if (active) {
-            const uint2 ent = clipq[V.clip_base + q];
-            const DrawGPU& dr = lists.of(V.list)[ent.x];
-            const uint32_t* ip = indices + dr.first_index + 3u * ent.y;
-            const float4* vb = xv + size_t(V.xv_base) + dr.xv_off;
-            id = dr.tri_base + ent.y;
-            A[0] = vb[ip[0]];
-            A[1] = vb[ip[1]];
-            A[2] = vb[ip[2]];
+            const ClipEntry& ent = clipq[V.clip_base + q];
+            A[0] = ent.v[0];
+            A[1] = ent.v[1];
+            A[2] = ent.v[2];
+            id = ent.id;
             cnt = 3;
             for (int plane = 0; plane < 6 && cnt >= 3; ++plane) {
                 int m = 0;
@@ -514,14 +517,28 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(const ViewGPU* views, Dra
         }
         const uint32_t nclipped = uint32_t(__popcll(__ballot(active)));
         if (lane_id() == 0 && nclipped) atomicAdd(&ctr[kCtrClipped], nclipped);
-        for (int k = 1; k <= kClipMax - 2; ++k) {  // fan triangle k: (v0, vk, vk+1)
+        // fan triangle k = (v0, vk, vk+1): the ones that set up into records (bit k - 1), then one
+        // slot range for the wave's records, then each record set up again, stored and binned
+        uint32_t fan = 0;
+        for (int k = 1; k + 1 < cnt; ++k) {
+            TriRec r;
+            const float4 tv[3] = {A[0], A[k], A[k + 1]};
+            if (setup_tri(tv, id, V, r)) fan |= 1u << (k - 1);
+        }
+        const uint32_t nrec = uint32_t(__popc(fan));
+        const uint32_t incl = wave_inclusive_sum(nrec);
+        const uint32_t wave_total = uint32_t(__builtin_amdgcn_readlane(int(incl), 63));
+        uint32_t slot = 0;
+        if (lane_id() == 0 && wave_total) slot = atomicAdd(&ctr[kCtrRecords], wave_total);
+        slot = uint32_t(__builtin_amdgcn_readlane(int(slot), 0)) + incl - nrec;
+        for (int k = 1; k <= kClipMax - 2; ++k) {
+            if (__ballot((fan >> (k - 1)) != 0u) == 0ull) break;  // no lane has fan triangle k or later
             TriRec r;
             bool has = false;
-            if (k + 1 < cnt) {
+            if ((fan >> (k - 1)) & 1u) {
                 const float4 tv[3] = {A[0], A[k], A[k + 1]};
                 has = setup_tri(tv, id, V, r);
             }
-            const uint32_t slot = wave_append(has, &ctr[kCtrRecords]);
             uint32_t ri = 0;
             if (has) {
                 if (V.tri_count + slot < V.rec_cap) {
@@ -531,6 +548,7 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(const ViewGPU* views, Dra
                     atomicOr(&ctr[kCtrOverflow], 2u);
                     has = false;
                 }
+                ++slot;
             }
             bin_wide(has, r, ri, V, shift, bin_counts, big_pairs, ctr);
         }
@@ -1056,8 +1074,8 @@ hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
         hipLaunchKernelGGL(k_setup, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.indices, g.xv, g.recs,
                            g.bin_tiles, g.shade_recs, g.clipq, g.counters, g.bin_counts, g.pair_slots, g.slot_count,
                            g.big_pairs, g.block_valid, g.draw_culled, g.shift);
-    hipLaunchKernelGGL(k_clip, dim3(g.clip_blocks, g.n_views), dim3(kClipThreads), 0, s, g.views, lists,
-                       g.indices, g.xv, g.clipq, g.recs, g.counters, g.bin_counts, g.big_pairs, g.shift);
+    hipLaunchKernelGGL(k_clip, dim3(g.clip_blocks, g.n_views), dim3(kClipThreads), 0, s, g.views, g.clipq, g.recs,
+                       g.counters, g.bin_counts, g.big_pairs, g.shift);
     return hipGetLastError();
 }
 
