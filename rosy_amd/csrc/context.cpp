@@ -291,6 +291,22 @@ rsy_result plan_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint
         L.vg.push_back(g);
     }
     if (L.total_xv >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "transformed vertex buffer too large");
+    // a frame with many setup blocks runs two triangles per thread: each draw's chunks in pairs
+    L.setup_tpt = L.tblocks.size() > kSetupPairBlocks ? 2u : 1u;
+    if (L.setup_tpt == 2) {
+        std::vector<BlockEntry> paired;
+        paired.reserve(L.tblocks.size() / 2 + 1);
+        for (size_t i = 0; i < L.tblocks.size(); ++i) {
+            BlockEntry b = L.tblocks[i];
+            if (i + 1 < L.tblocks.size() && L.tblocks[i + 1].view == b.view && L.tblocks[i + 1].draw == b.draw &&
+                L.tblocks[i + 1].start == b.start + kSetupBlockTris) {
+                b.count += L.tblocks[i + 1].count;  // the second chunk's slots, indices and ids follow the first's
+                ++i;
+            }
+            paired.push_back(b);
+        }
+        L.tblocks.swap(paired);
+    }
     interleave_views(L.tblocks);  // (the transform keeps view order: its output writes stream per view)
     // bin lists: an estimate of the frame's (triangle, bin) pairs
     uint64_t want_pairs = 0;
@@ -547,6 +563,7 @@ rsy_result enqueue_frame(rsy_ctx* c, const Layout& L, const FrameMem& m, bool ca
     g.n_vblocks = uint32_t(L.vblocks.size());
     g.tblocks = m.tblocks;
     g.n_tblocks = uint32_t(L.tblocks.size());
+    g.setup_tpt = L.setup_tpt;
     g.px = c->px.as<float>();
     g.py = c->py.as<float>();
     g.pz = c->pz.as<float>();

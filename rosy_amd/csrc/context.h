@@ -104,6 +104,7 @@ struct Layout {
     // derived
     std::vector<ViewGPU> vg;
     std::vector<BlockEntry> vblocks, tblocks;
+    uint32_t setup_tpt = 1;  // triangles per setup thread (kSetupPairBlocks)
     uint32_t total_bins = 0;
     uint64_t total_xv = 0;
     uint32_t rec_total = 0, clip_total = 0, big_total = 0, cull_total = 0;

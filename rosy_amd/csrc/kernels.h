@@ -15,6 +15,8 @@ namespace rsy {
 constexpr uint32_t kTransformPerThread = 2;
 constexpr uint32_t kTransformBlockVerts = 256 * kTransformPerThread;
 constexpr uint32_t kSetupBlockTris = 256;
+// A frame whose setup would take more blocks than this runs two triangles per thread (512 per block).
+constexpr uint32_t kSetupPairBlocks = 8192;
 
 // The three draw lists of a frame, indexed by ViewGPU::list.
 struct DrawLists {
@@ -33,6 +35,7 @@ struct GeometryLaunch {
     uint32_t n_vblocks;
     const BlockEntry* tblocks;
     uint32_t n_tblocks;
+    uint32_t setup_tpt;     // triangles per setup thread (1 or 2): a block covers 256 * setup_tpt
     const float *px, *py, *pz;
     const uint32_t* indices;
     float4* xv;

@@ -331,18 +331,22 @@ __device__ void bin_wide(bool active, const TriRec& r, uint32_t ri, const ViewGP
     }
 }
 
-// LDS bin table of one setup block: at most kSetupBlockTris x kSmallBins = 1024 distinct keys, so a
-// probe always finds its key or an empty entry.
-constexpr uint32_t kHash = kSetupBlockTris * kSmallBins;
-static_assert(kHash == 1024u, "10-bit hash below");
 constexpr uint32_t kEmpty = 0xffffffffu;
 
+// One setup block: 256 threads, kTpt triangles each (a block covers 256 * kTpt triangles of one
+// view and draw).  Two per thread pay on frames with many blocks per resident slot: each thread
+// keeps two triangles' loads in flight, at fewer waves per SIMD.  The LDS bin table holds up to
+// 256 * kTpt * kSmallBins distinct keys, so a probe always finds its key or an empty entry.
+template <int kTpt>
 __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const ViewGPU* views, const uint32_t* indices,
                                                const float4* xv, TriRec* recs, uint32_t* bin_tiles, ShadeRec* shade_recs,
                                                ClipEntry* clipq, uint32_t* counters,
                                                uint32_t* bin_counts, uint32_t* pair_slots, uint32_t slot_count,
                                                uint4* big_pairs,
                                                uint32_t* block_valid, const uint32_t* draw_culled, uint32_t shift) {
+    constexpr uint32_t kHash = 256u * kTpt * kSmallBins;
+    constexpr uint32_t kHashShift = kTpt == 1 ? 22u : 21u;  // 10- or 11-bit multiplicative hash
+    static_assert(kHash == (1u << (32u - kHashShift)), "hash bits");
     __shared__ uint32_t hkey[kHash];
     __shared__ uint32_t hcount[kHash];  // per key: records of the block, then the block's first slot
     __shared__ uint32_t wave_valid[4];
@@ -353,101 +357,130 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     const BlockEntry e = blocks[blockIdx.x];
     const ViewGPU V = views[e.view];
     uint32_t* ctr = counters + e.view * kCtrPerView;
-    const uint32_t t = threadIdx.x;  // the block's triangle
-    const bool valid = t < e.count;
-    // the triangle's indices are requested beside the view record and the draw's culling flag
-    uint32_t vi[3] = {0u, 0u, 0u};
-    if (valid) {
-        const uint32_t* ip = indices + e.first + 3u * t;
-        vi[0] = ip[0];
-        vi[1] = ip[1];
-        vi[2] = ip[2];
+    // the thread's triangles of the block (t[j]) and their indices, requested beside the view record
+    // and the draw's culling flag
+    uint32_t t[kTpt], slot[kTpt];
+    bool valid[kTpt];
+    uint32_t vi[kTpt][3];
+#pragma unroll
+    for (int j = 0; j < kTpt; ++j) {
+        t[j] = threadIdx.x + 256u * uint32_t(j);
+        valid[j] = t[j] < e.count;
+        slot[j] = e.out_first + t[j];  // every triangle owns one record slot
+        vi[j][0] = vi[j][1] = vi[j][2] = 0u;
+        if (valid[j]) {
+            const uint32_t* ip = indices + e.first + 3u * t[j];
+            vi[j][0] = ip[0];
+            vi[j][1] = ip[1];
+            vi[j][2] = ip[2];
+        }
     }
-    const uint32_t slot = e.out_first + t;  // every triangle owns one record slot
     if (draw_culled[e.cull]) {  // the whole draw is outside the view: empty slots only
-        if (valid) bin_tiles[slot] = kNoBins;
+#pragma unroll
+        for (int j = 0; j < kTpt; ++j)
+            if (valid[j]) bin_tiles[slot[j]] = kNoBins;
         if (threadIdx.x == 0) block_valid[blockIdx.x] = 0u;
         return;
     }
-    int status = 0;  // 0 reject, 1 accept, 2 clip
-    float4 c[3];
-    if (valid) {
-        const float4* base = xv + size_t(e.xv_first);
-        c[0] = base[vi[0]];
-        c[1] = base[vi[1]];
-        c[2] = base[vi[2]];
-        bool reject = false;
+    float4 c[kTpt][3];
 #pragma unroll
-        for (int plane = 0; plane < 6; ++plane) {
-            const int outside = int(plane_dist(c[0], plane) < 0.f) + int(plane_dist(c[1], plane) < 0.f) +
-                                int(plane_dist(c[2], plane) < 0.f);
-            reject |= outside == 3;
+    for (int j = 0; j < kTpt; ++j)
+        if (valid[j]) {
+            const float4* base = xv + size_t(e.xv_first);
+            c[j][0] = base[vi[j][0]];
+            c[j][1] = base[vi[j][1]];
+            c[j][2] = base[vi[j][2]];
         }
-        status = reject ? 0 : (all_inside(c[0], c[1], c[2]) ? 1 : 2);
-    }
-    TriRec r;
-    const bool has = status == 1 && setup_tri(c, e.object + t, V, r);
-    if (valid) {
-        if (has) recs[slot] = r;
-        // a camera or blend view's triangle that a pixel can show (set up into a raster record, or
-        // queued for clipping): its shading record; culled and boxless triangles need none
-        if (!V.is_shadow && (has || status == 2)) {
-            uint4* sr = reinterpret_cast<uint4*>(shade_recs + slot);
-            sr[0] = make_uint4(e.vtx_base + vi[0], e.vtx_base + vi[1], e.vtx_base + vi[2],
-                               e.draw | (has ? 0u : kShadeNoRecord));
-            sr[1] = make_uint4(__float_as_uint(c[0].w), __float_as_uint(c[1].w), __float_as_uint(c[2].w), e.material);
+    TriRec r[kTpt];
+    bool has[kTpt], small[kTpt];
+    int tx0[kTpt], ty0[kTpt], bw[kTpt], n[kTpt];
+#pragma unroll
+    for (int j = 0; j < kTpt; ++j) {
+        int status = 0;  // 0 reject, 1 accept, 2 clip
+        if (valid[j]) {
+            bool reject = false;
+#pragma unroll
+            for (int plane = 0; plane < 6; ++plane) {
+                const int outside = int(plane_dist(c[j][0], plane) < 0.f) + int(plane_dist(c[j][1], plane) < 0.f) +
+                                    int(plane_dist(c[j][2], plane) < 0.f);
+                reject |= outside == 3;
+            }
+            status = reject ? 0 : (all_inside(c[j][0], c[j][1], c[j][2]) ? 1 : 2);
         }
-    }
-    const uint32_t cslot = wave_append(status == 2, &ctr[kCtrClip]);
-    if (status == 2) {
-        if (cslot < V.clip_cap) {
-            ClipEntry& q = clipq[V.clip_base + cslot];
-            q.v[0] = c[0];
-            q.v[1] = c[1];
-            q.v[2] = c[2];
-            q.id = e.object + t;
-        } else {
-            atomicOr(&ctr[kCtrOverflow], 1u);
+        has[j] = status == 1 && setup_tri(c[j], e.object + t[j], V, r[j]);
+        if (valid[j]) {
+            if (has[j]) recs[slot[j]] = r[j];
+            // a camera or blend view's triangle that a pixel can show (set up into a raster record,
+            // or queued for clipping): its shading record; culled and boxless triangles need none
+            if (!V.is_shadow && (has[j] || status == 2)) {
+                uint4* sr = reinterpret_cast<uint4*>(shade_recs + slot[j]);
+                sr[0] = make_uint4(e.vtx_base + vi[j][0], e.vtx_base + vi[j][1], e.vtx_base + vi[j][2],
+                                   e.draw | (has[j] ? 0u : kShadeNoRecord));
+                sr[1] = make_uint4(__float_as_uint(c[j][0].w), __float_as_uint(c[j][1].w), __float_as_uint(c[j][2].w),
+                                   e.material);
+            }
         }
+        const uint32_t cslot = wave_append(status == 2, &ctr[kCtrClip]);
+        if (status == 2) {
+            if (cslot < V.clip_cap) {
+                ClipEntry& q = clipq[V.clip_base + cslot];
+                q.v[0] = c[j][0];
+                q.v[1] = c[j][1];
+                q.v[2] = c[j][2];
+                q.id = e.object + t[j];
+            } else {
+                atomicOr(&ctr[kCtrOverflow], 1u);
+            }
+        }
+        int x0 = 0, y0 = 0, x1 = -1, y1 = -1;
+        if (has[j]) rec_tiles(r[j], shift, x0, y0, x1, y1);
+        tx0[j] = x0;
+        ty0[j] = y0;
+        bw[j] = x1 - x0 + 1;
+        n[j] = has[j] ? bw[j] * (y1 - y0 + 1) : 0;
+        small[j] = has[j] && n[j] <= kSmallBins;
+        if (valid[j]) bin_tiles[slot[j]] = small[j] ? pack_bins(x0, y0, bw[j], y1 - y0 + 1) : kNoBins;
     }
-    int tx0 = 0, ty0 = 0, tx1 = -1, ty1 = -1;
-    if (has) rec_tiles(r, shift, tx0, ty0, tx1, ty1);
-    const int bw = tx1 - tx0 + 1;
-    const int n = has ? bw * (ty1 - ty0 + 1) : 0;
-    const bool small = has && n <= kSmallBins;
-    if (valid) bin_tiles[slot] = small ? pack_bins(tx0, ty0, bw, ty1 - ty0 + 1) : kNoBins;
     __syncthreads();  // table cleared
     // small records: rank within the block for each bin, through the LDS table
-    uint32_t hidx[kSmallBins], rank[kSmallBins];
+    uint32_t hidx[kTpt][kSmallBins], rank[kTpt][kSmallBins];
 #pragma unroll
-    for (int k = 0; k < kSmallBins; ++k) {
-        hidx[k] = 0u;
-        rank[k] = 0u;
-        if (small && k < n) {
-            const uint32_t key = V.tile_base + uint32_t((ty0 + k / bw) * int(V.tiles_x) + tx0 + k % bw);
-            uint32_t h = (key * 2654435761u) >> 22;  // 10-bit multiplicative hash
-            while (true) {
-                const uint32_t prev = atomicCAS(&hkey[h], kEmpty, key);
-                if (prev == kEmpty || prev == key) break;
-                h = (h + 1) & (kHash - 1);
+    for (int j = 0; j < kTpt; ++j)
+#pragma unroll
+        for (int k = 0; k < kSmallBins; ++k) {
+            hidx[j][k] = 0u;
+            rank[j][k] = 0u;
+            if (small[j] && k < n[j]) {
+                const uint32_t key = V.tile_base + uint32_t((ty0[j] + k / bw[j]) * int(V.tiles_x) + tx0[j] + k % bw[j]);
+                uint32_t h = (key * 2654435761u) >> kHashShift;
+                while (true) {
+                    const uint32_t prev = atomicCAS(&hkey[h], kEmpty, key);
+                    if (prev == kEmpty || prev == key) break;
+                    h = (h + 1) & (kHash - 1);
+                }
+                hidx[j][k] = h;
+                rank[j][k] = atomicAdd(&hcount[h], 1u);
             }
-            hidx[k] = h;
-            rank[k] = atomicAdd(&hcount[h], 1u);
         }
-    }
     __syncthreads();
     // one global atomic per distinct bin of the block: the block's first slot in that bin
     for (uint32_t i = threadIdx.x; i < kHash; i += 256)
         if (hkey[i] != kEmpty) hcount[i] = atomicAdd(&bin_counts[hkey[i]], hcount[i]);
     __syncthreads();
-    if (small) {  // slots in bins 0-1, and in bins 2-3 only for a record with more than two bins
-        uint2* s01 = reinterpret_cast<uint2*>(pair_slots);
-        s01[slot] = make_uint2(hcount[hidx[0]] + rank[0], n > 1 ? hcount[hidx[1]] + rank[1] : 0u);
-        if (n > 2) s01[slot_count + slot] = make_uint2(hcount[hidx[2]] + rank[2], n > 3 ? hcount[hidx[3]] + rank[3] : 0u);
+    uint32_t nvalid = 0;
+#pragma unroll
+    for (int j = 0; j < kTpt; ++j) {
+        if (small[j]) {  // slots in bins 0-1, and in bins 2-3 only for a record with more than two bins
+            uint2* s01 = reinterpret_cast<uint2*>(pair_slots);
+            s01[slot[j]] = make_uint2(hcount[hidx[j][0]] + rank[j][0], n[j] > 1 ? hcount[hidx[j][1]] + rank[j][1] : 0u);
+            if (n[j] > 2)
+                s01[slot_count + slot[j]] =
+                    make_uint2(hcount[hidx[j][2]] + rank[j][2], n[j] > 3 ? hcount[hidx[j][3]] + rank[j][3] : 0u);
+        }
+        // records with many bins: the whole wave spreads their bins
+        bin_wide(has[j] && !small[j], r[j], slot[j], V, shift, bin_counts, big_pairs, ctr);
+        nvalid += uint32_t(__popcll(__ballot(has[j])));
     }
-    // records with many bins: the whole wave spreads their bins
-    bin_wide(has && !small, r, slot, V, shift, bin_counts, big_pairs, ctr);
-    const uint32_t nvalid = uint32_t(__popcll(__ballot(has)));
     if (lane_id() == 0) wave_valid[threadIdx.x >> 6] = nvalid;
     __syncthreads();
     if (threadIdx.x == 0) block_valid[blockIdx.x] = wave_valid[0] + wave_valid[1] + wave_valid[2] + wave_valid[3];
@@ -1070,8 +1103,12 @@ hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
     else
         hipLaunchKernelGGL(k_clear2, dim3(min((nclear + 255) / 256, 2048u)), dim3(256), 0, s, g.counters,
                            g.n_views * kCtrPerView, g.bin_counts, g.total_bins);
-    if (g.n_tblocks)
-        hipLaunchKernelGGL(k_setup, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.indices, g.xv, g.recs,
+    if (g.n_tblocks && g.setup_tpt == 2)
+        hipLaunchKernelGGL(k_setup<2>, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.indices, g.xv, g.recs,
+                           g.bin_tiles, g.shade_recs, g.clipq, g.counters, g.bin_counts, g.pair_slots, g.slot_count,
+                           g.big_pairs, g.block_valid, g.draw_culled, g.shift);
+    else if (g.n_tblocks)
+        hipLaunchKernelGGL(k_setup<1>, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.indices, g.xv, g.recs,
                            g.bin_tiles, g.shade_recs, g.clipq, g.counters, g.bin_counts, g.pair_slots, g.slot_count,
                            g.big_pairs, g.block_valid, g.draw_culled, g.shift);
     hipLaunchKernelGGL(k_clip, dim3(g.clip_blocks, g.n_views), dim3(kClipThreads), 0, s, g.views, g.clipq, g.recs,
