@@ -277,9 +277,6 @@ rsy_result plan_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint
         L.big_total += g.big_cap;
         for (uint32_t d = 0; d < g.n_draws; ++d) {
             const DrawGPU& dr = list[d];
-            for (uint32_t s = 0; s < dr.vtx_count; s += kTransformBlockVerts)
-                L.vblocks.push_back({vi, d, s, g.list, dr.vtx_base + s, std::min(kTransformBlockVerts, dr.vtx_count - s),
-                                     g.xv_base + dr.xv_off + s, 0u, g.cull_base + d, dr.object, 0u, 0u});
             for (uint32_t s = 0; s < dr.tri_count; s += kSetupBlockTris)
                 L.tblocks.push_back({vi, d, s, g.list, dr.first_index + 3u * s, std::min(kSetupBlockTris, dr.tri_count - s),
                                      g.rec_base + dr.tri_base + s, g.xv_base + dr.xv_off, g.cull_base + d, dr.tri_base + s,
@@ -291,6 +288,22 @@ rsy_result plan_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint
         L.vg.push_back(g);
     }
     if (L.total_xv >= 0xffffffffull) return fail(c, RSY_OVERFLOW, "transformed vertex buffer too large");
+    // transform blocks: a chunk of a draw's vertices for a run of up to kTransformViews consecutive
+    // views of the same draw list (the shadow cascades, the camera views), so that the positions are
+    // read once for all of them
+    for (uint32_t v0 = 0; v0 < L.vg.size();) {
+        const uint32_t list_id = L.vg[v0].list;
+        uint32_t nv = 1;
+        while (v0 + nv < L.vg.size() && nv < kTransformViews && L.vg[v0 + nv].list == list_id) ++nv;
+        const auto& list = c->lists[list_id];
+        for (uint32_t d = 0; d < uint32_t(list.size()); ++d) {
+            const DrawGPU& dr = list[d];
+            for (uint32_t s = 0; s < dr.vtx_count; s += kTransformBlockVerts)
+                L.vblocks.push_back({v0, d, s, list_id, dr.vtx_base + s, std::min(kTransformBlockVerts, dr.vtx_count - s),
+                                     dr.xv_off + s, nv, d, dr.object, 0u, 0u});
+        }
+        v0 += nv;
+    }
     // a frame with many setup blocks runs two triangles per thread: each draw's chunks in pairs
     L.setup_tpt = L.tblocks.size() > kSetupPairBlocks ? 2u : 1u;
     if (L.setup_tpt == 2) {
@@ -606,7 +619,7 @@ rsy_result enqueue_frame(rsy_ctx* c, const Layout& L, const FrameMem& m, bool ca
     b.shift = shift;
     uint32_t max_tris = 0;
     for (const auto& v : L.vg) max_tris = std::max(max_tris, v.tri_count);
-    b.scatter_blocks = std::max<uint32_t>(64, std::min<uint32_t>(4096, (max_tris + 255) / 256));
+    b.scatter_blocks = std::max<uint32_t>(64, std::min<uint32_t>(4096, (max_tris + 1023) / 1024));
     b.scan_partial = m.scan_partial;
     b.items = m.items;
     b.items_cap = raster_items_cap(L.total_bins, m.list_cap);

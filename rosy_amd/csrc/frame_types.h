@@ -108,17 +108,20 @@ struct ViewGPU {
     float view_projection[16];
 };
 
-// Work of one block of the transform or setup kernel: its block size of items of one (view, draw)
-// from `start`, with the addresses the block needs resolved on the host (the layout's draw and view
-// tables are fixed), so that the block's first data loads wait on no draw or view record.
+// Work of one block of the transform or setup kernel: its block size of items of one draw from
+// `start`, with the addresses the block needs resolved on the host (the layout's draw and view
+// tables are fixed), so that the block's first data loads wait on no draw or view record.  A setup
+// block serves one view; a transform block serves `xv_first` consecutive views from `view`, all
+// drawing the same list.
 struct BlockEntry {
     uint32_t view, draw, start;
     uint32_t list;       // the view's draw list
     uint32_t first;      // transform: the block's first vertex in the vertex streams; setup: its first index word
     uint32_t count;      // vertices or triangles of the block
-    uint32_t out_first;  // transform: its first transformed vertex; setup: its first record slot
-    uint32_t xv_first;   // setup: the draw's first transformed vertex in the view (indices are mesh-local)
-    uint32_t cull;       // the (view, draw) culling flag
+    uint32_t out_first;  // transform: its first transformed vertex inside a view's block; setup: its first record slot
+    uint32_t xv_first;   // setup: the draw's first transformed vertex in the view (indices are mesh-local);
+                         // transform: the number of views
+    uint32_t cull;       // setup: the (view, draw) culling flag; transform: the draw (each view adds its cull_base)
     uint32_t object;     // transform: the draw's object record; setup: the triangle id of its first triangle
     uint32_t vtx_base;   // setup: the mesh's first vertex (shading records hold global vertex indices)
     uint32_t material;   // setup: the draw's material (shading records)

@@ -14,6 +14,7 @@ namespace rsy {
 // thread measured slower).
 constexpr uint32_t kTransformPerThread = 2;
 constexpr uint32_t kTransformBlockVerts = 256 * kTransformPerThread;
+constexpr uint32_t kTransformViews = 8;  // consecutive views of one draw list per transform block
 constexpr uint32_t kSetupBlockTris = 256;
 // A frame whose setup would take more blocks than this runs two triangles per thread (512 per block).
 constexpr uint32_t kSetupPairBlocks = 8192;

@@ -202,36 +202,38 @@ __device__ inline uint32_t corner_outside(const float* m, const float* bmin, con
     return bits;
 }
 
-// True when the bits of the eight corners (lanes 0..7 of the wave) show the box outside the view.
-__device__ inline bool box_outside(uint32_t bits) {
+// True when the bits of the eight corners in lanes 8 * g .. 8 * g + 7 of the wave show the box
+// outside the view.
+__device__ inline bool box_outside(uint32_t bits, uint32_t g) {
     uint32_t all = 0;
 #pragma unroll
     for (int b = 0; b < 11; ++b)
-        all |= ((__builtin_amdgcn_ballot_w64((bits >> b) & 1u) & 0xffull) == 0xffull) ? 1u << b : 0u;
+        all |= (((__builtin_amdgcn_ballot_w64((bits >> b) & 1u) >> (8u * g)) & 0xffull) == 0xffull) ? 1u << b : 0u;
     return (all & 0x3fu) != 0u || ((all & kCornerInside) != 0u && (all & 0x3c0u) != 0u);
 }
 
 // clip = (VP * W) * p per (view, vertex).  Every block covers kTransformBlockVerts vertices of one
-// (view, draw), kTransformPerThread per thread (their loads issued together: half the blocks, each
-// waiting once for its draw, object and positions).  The block's MVP = views[v].view_projection *
-// objects[draw.object].transform is formed once in LDS by the fmaf chain in k order, while the
-// vertex loads from the SoA x/y/z streams (coalesced 12 B) are in flight; then each vertex's clip
-// position is written (coalesced 16 B).  A draw whose box is outside the view is skipped (eight
-// lanes test one box corner each), and its first block records that for k_setup.
+// draw, kTransformPerThread per thread (their loads issued together), for up to kTransformViews
+// consecutive views that draw the same list: the positions are read once (coalesced 12 B from the
+// SoA x/y/z streams) and each view's clip positions are written (coalesced 16 B).  Each view's
+// MVP = views[v].view_projection * objects[draw.object].transform is formed in LDS by the fmaf chain
+// in k order while the vertex loads are in flight.  A draw whose box is outside a view is skipped
+// for that view (eight lanes test one box corner each), and the draw's first block records that
+// for k_setup.
 // Also zeroes the frame's counters and bin counts (`clear` words at `clear_a` then at `clear_b`), spread
 // over the grid, before the setup kernel counts into them.
 __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, const ViewGPU* views, DrawLists lists,
                                                    const float* objects, const float* px, const float* py,
                                                    const float* pz, float4* xv, uint32_t* draw_culled,
                                                    uint32_t* clear_a, uint32_t na, uint32_t* clear_b, uint32_t nb) {
-    __shared__ float m[16];
-    __shared__ uint32_t culled;
+    __shared__ float m[kTransformViews][16];
+    __shared__ uint32_t culled[kTransformViews];
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < na + nb; i += gridDim.x * 256) {
         if (i < na) clear_a[i] = 0u;
         else clear_b[i - na] = 0u;
     }
     const BlockEntry e = blocks[blockIdx.x];
-    const ViewGPU& V = views[e.view];
+    const uint32_t nv = e.xv_first;
     const DrawGPU& dr = lists.of(e.list)[e.draw];  // its box only
     constexpr int R = int(kTransformPerThread);
     float x[R], y[R], z[R];
@@ -246,31 +248,35 @@ __global__ __launch_bounds__(256) void k_transform(const BlockEntry* blocks, con
             z[j] = pz[g];
         }
     }
-    if (threadIdx.x < 16) {
-        const int col = threadIdx.x >> 2, row = threadIdx.x & 3;
-        const float* a = V.view_projection;
+    if (threadIdx.x < 16u * nv) {
+        const uint32_t k = threadIdx.x >> 4;
+        const int col = (threadIdx.x >> 2) & 3, row = threadIdx.x & 3;
+        const float* a = views[e.view + k].view_projection;
         const float* b = objects + size_t(e.object) * 41u;  // 164-byte packed records
         float acc = a[row] * b[col * 4 + 0];
         acc = fmaf(a[4 + row], b[col * 4 + 1], acc);
         acc = fmaf(a[8 + row], b[col * 4 + 2], acc);
         acc = fmaf(a[12 + row], b[col * 4 + 3], acc);
-        m[threadIdx.x] = acc;
+        m[k][threadIdx.x & 15u] = acc;
     }
     __syncthreads();
-    if (threadIdx.x < 64) {  // wave 0
-        const bool outside = box_outside(threadIdx.x < 8 ? corner_outside(m, dr.bmin, dr.bmax, int(threadIdx.x)) : 0u);
-        if (threadIdx.x == 0) {
-            culled = outside ? 1u : 0u;
-            if (e.start == 0) draw_culled[e.cull] = culled;
+    if (threadIdx.x < 64) {  // wave 0: lanes 8k .. 8k + 7 test view k's box corners
+        const uint32_t k = threadIdx.x >> 3;
+        const bool outside = box_outside(k < nv ? corner_outside(m[k], dr.bmin, dr.bmax, int(threadIdx.x & 7u)) : 0u, k);
+        if ((threadIdx.x & 7u) == 0 && k < nv) {
+            culled[k] = outside ? 1u : 0u;
+            if (e.start == 0) draw_culled[views[e.view + k].cull_base + e.cull] = outside ? 1u : 0u;
         }
     }
     __syncthreads();
-    if (culled) return;
-    float4* out = xv + size_t(e.out_first);
+    for (uint32_t k = 0; k < nv; ++k) {
+        if (culled[k]) continue;
+        float4* out = xv + size_t(views[e.view + k].xv_base) + e.out_first;
 #pragma unroll
-    for (int j = 0; j < R; ++j) {
-        const uint32_t v = uint32_t(j) * 256u + threadIdx.x;
-        if (v < e.count) out[v] = transform(m, x[j], y[j], z[j]);
+        for (int j = 0; j < R; ++j) {
+            const uint32_t v = uint32_t(j) * 256u + threadIdx.x;
+            if (v < e.count) out[v] = transform(m[k], x[j], y[j], z[j]);
+        }
     }
 }
 
@@ -938,17 +944,42 @@ __device__ inline void scatter_big_pair(const uint4 bp, const uint32_t* bin_offs
 // k_setup's block table, or the big-pair list), so the entry goes to bin_offsets[bin] + slot.  The
 // same launch zeroes the output pixels of the empty bins (which get no raster work item) and of
 // the split bins (which k_raster then merges into).
+static_assert(2u * RSY_MAX_VIEWS <= 64u, "a frame's views (with their blend views) fit one per lane");
 __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t n_views, const uint32_t* bin_tiles,
                                                  const uint32_t* counters, const uint32_t* bin_offsets,
                                                  const uint32_t* pair_slots, uint32_t slot_count,
                                                  const uint4* big_pairs, uint32_t* lists,
                                                  uint32_t list_cap, uint32_t shift, const uint32_t* totals,
-                                                 const uint32_t* zero_bins) {
+                                                 const uint32_t* zero_bins, uint32_t total_bins) {
+    // for the clears below, requested before the record work: the view table one view per lane
+    // (views ascend by tile_base, so a bin's view is the count of lanes at or below it, less one)
+    // and the block's first bin to clear (read ahead of the count of such bins)
+    const uint32_t lane = lane_id();
+    uint32_t lv_base = 0xffffffffu, lv_tx = 1u, lv_w = 0u, lv_h = 0u, lv_kind = 0u;
+    uint64_t lv_out = 0ull;
+    if (lane < n_views) {
+        const ViewGPU& W = views[lane];
+        lv_base = W.tile_base;
+        lv_tx = W.tiles_x;
+        lv_w = W.width;
+        lv_h = W.height;
+        lv_kind = (W.blend ? 1u : 0u) | (W.is_shadow ? 2u : 0u) | (W.samples << 2);
+        lv_out = W.out;
+    }
+    const uint32_t n_blocks = gridDim.x * gridDim.y;
+    const uint32_t zb0 = blockIdx.y * gridDim.x + blockIdx.x;
+    const uint32_t bin0 = zb0 < total_bins ? zero_bins[zb0] : 0u;
+
     const uint32_t view = blockIdx.y;
     const ViewGPU V = views[view];
     const uint32_t nbig = min(counters[view * kCtrPerView + kCtrBig], V.big_cap);
     const uint32_t stride = gridDim.x * blockDim.x;
-    if (V.tri_count > stride) {  // more than one record slot per thread: two at a time, then the big pairs
+    if (V.tri_count > 2u * stride) {
+        for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < V.tri_count; i0 += 4u * stride)
+            scatter_slots<4>(V, bin_tiles, bin_offsets, pair_slots, slot_count, lists, list_cap, i0, stride);
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nbig; i += stride)
+            scatter_big_pair(big_pairs[V.big_base + i], bin_offsets, lists, list_cap);
+    } else if (V.tri_count > stride) {  // more than one record slot per thread: two at a time, then the big pairs
         for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < V.tri_count; i0 += 2u * stride)
             scatter_slots<2>(V, bin_tiles, bin_offsets, pair_slots, slot_count, lists, list_cap, i0, stride);
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nbig; i += stride)
@@ -965,19 +996,19 @@ __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t 
     // and its origin are block-uniform), consecutive threads on consecutive pixels of a bin row
     const uint32_t T = 1u << shift;
     const uint32_t n_zero = totals[5];
-    const uint32_t n_blocks = gridDim.x * gridDim.y;
-    for (uint32_t zb = blockIdx.y * gridDim.x + blockIdx.x; zb < n_zero; zb += n_blocks) {
-        const uint32_t bin = zero_bins[zb];
-        uint32_t v = 0;
-        while (v + 1 < n_views && views[v + 1].tile_base <= bin) ++v;
-        const ViewGPU& W = views[v];
-        if (W.blend) continue;  // a blend view has no output of its own
-        // view fields copied out: the stores below could alias the view table for the compiler
-        const uint32_t width = W.width, height = W.height, local = bin - W.tile_base;
-        const uint32_t x0 = (local % W.tiles_x) * T, y0 = (local / W.tiles_x) * T;
-        const bool shadow = W.is_shadow != 0;
-        const uint32_t planes = W.samples;
-        const uint64_t out = W.out;
+    for (uint32_t zb = zb0; zb < n_zero; zb += n_blocks) {
+        const uint32_t bin = zb == zb0 ? bin0 : zero_bins[zb];
+        const uint32_t v = uint32_t(__popcll(__ballot(lv_base <= bin))) - 1u;
+        const uint32_t kind = uint32_t(__builtin_amdgcn_readlane(int(lv_kind), int(v)));
+        if (kind & 1u) continue;  // a blend view has no output of its own
+        const uint32_t width = uint32_t(__builtin_amdgcn_readlane(int(lv_w), int(v)));
+        const uint32_t height = uint32_t(__builtin_amdgcn_readlane(int(lv_h), int(v)));
+        const uint32_t tiles_x = uint32_t(__builtin_amdgcn_readlane(int(lv_tx), int(v)));
+        const uint32_t local = bin - uint32_t(__builtin_amdgcn_readlane(int(lv_base), int(v)));
+        const uint64_t out = readlane64(lv_out, v);
+        const uint32_t x0 = (local % tiles_x) * T, y0 = (local / tiles_x) * T;
+        const bool shadow = (kind & 2u) != 0u;
+        const uint32_t planes = kind >> 2;
         const size_t plane = size_t(width) * height;
         for (uint32_t p = threadIdx.x; p < T * T; p += blockDim.x) {
             const uint32_t x = x0 + (p & (T - 1u)), y = y0 + (p >> shift);
@@ -1132,7 +1163,7 @@ hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
     }
     hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.n_views, b.bin_tiles,
                        b.counters, b.bin_offsets, b.pair_slots, b.slot_count, b.big_pairs, b.lists, b.list_cap, b.shift,
-                       b.totals, b.zero_bins);
+                       b.totals, b.zero_bins, b.total_bins);
     return hipGetLastError();
 }
 
