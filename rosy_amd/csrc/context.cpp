@@ -373,7 +373,6 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     HIP_TRY(c, c->big_pairs.ensure(size_t(std::max<uint32_t>(1, L.big_total)) * 16), "allocate bin pairs");
     HIP_TRY(c, c->block_valid.ensure(std::max<size_t>(1, L.tblocks.size()) * 4), "allocate statistics");
     HIP_TRY(c, c->totals.ensure(32), "allocate statistics");
-    HIP_TRY(c, c->zero_bins.ensure(size_t(std::max<uint32_t>(1, L.total_bins)) * 4), "allocate bins");
     HIP_TRY(c, c->scan_partial.ensure((size_t(L.total_bins) / kScanBlockBins + 1) * kScanPartialBytes), "allocate bins");
     HIP_TRY(c, c->vblocks_d.ensure(std::max<size_t>(1, L.vblocks.size()) * sizeof(BlockEntry)), "allocate blocks");
     HIP_TRY(c, c->tblocks_d.ensure(std::max<size_t>(1, L.tblocks.size()) * sizeof(BlockEntry)), "allocate blocks");
@@ -448,7 +447,6 @@ rsy_result build_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uin
     m.big_pairs = c->big_pairs.as<uint4>();
     m.block_valid = c->block_valid.as<uint32_t>();
     m.totals = c->totals.as<uint32_t>();
-    m.zero_bins = c->zero_bins.as<uint32_t>();
     m.scan_partial = c->scan_partial.p;
     m.vblocks = c->vblocks_d.as<BlockEntry>();
     m.tblocks = c->tblocks_d.as<BlockEntry>();
@@ -597,6 +595,12 @@ rsy_result enqueue_frame(rsy_ctx* c, const Layout& L, const FrameMem& m, bool ca
     g.clip_blocks = 128;
     g.total_bins = L.total_bins;
     HIP_TRY(c, launch_geometry(g, s), "geometry kernels");
+    // the setup blocks clear their views' outputs; a view without triangles has none
+    for (const ViewGPU& v : L.vg)
+        if (!v.blend && v.tri_count == 0) {
+            const uint64_t words = uint64_t(v.width) * v.height * (v.is_shadow ? 1u : 2u * v.samples);
+            HIP_TRY(c, launch_fill_u32(reinterpret_cast<uint32_t*>(v.out), uint32_t(words), 0u, s), "clear output");
+        }
     if (timing) HIP_TRY(c, hipEventRecord(c->ev_stage[0], s), "event");
 
     BinLaunch b{};
@@ -623,7 +627,6 @@ rsy_result enqueue_frame(rsy_ctx* c, const Layout& L, const FrameMem& m, bool ca
     b.scan_partial = m.scan_partial;
     b.items = m.items;
     b.items_cap = raster_items_cap(L.total_bins, m.list_cap);
-    b.zero_bins = m.zero_bins;
     // one raster workgroup per work item: as many as the last frame of this layout had (with slack),
     // or an estimate (every bin, and a split item per 256 estimated pairs); the scan flags a frame
     // with more items, and finish_frame re-runs it with a grid that fits
@@ -1305,7 +1308,6 @@ rsy_result rsy_render_noalloc(rsy_ctx* c, const rsy_frame_desc* frame, const rsy
     m.draw_culled = reinterpret_cast<uint32_t*>(take(size_t(L.cull_total) * 4));
     m.bin_counts = reinterpret_cast<uint32_t*>(take(size_t(L.total_bins) * 4));
     m.bin_offsets = reinterpret_cast<uint32_t*>(take(size_t(L.total_bins) * 4));
-    m.zero_bins = reinterpret_cast<uint32_t*>(take(size_t(L.total_bins) * 4));
     m.scan_partial = take((size_t(L.total_bins) / kScanBlockBins + 1) * kScanPartialBytes);
     m.xv = reinterpret_cast<float4*>(take(size_t(L.total_xv) * sizeof(float4)));
     m.recs = reinterpret_cast<TriRec*>(take(size_t(L.rec_total) * sizeof(TriRec)));

@@ -147,7 +147,6 @@ struct FrameMem {
     uint4* big_pairs = nullptr;
     uint32_t* block_valid = nullptr;
     uint32_t* totals = nullptr;
-    uint32_t* zero_bins = nullptr;
     void* scan_partial = nullptr;
     BlockEntry* vblocks = nullptr;
     BlockEntry* tblocks = nullptr;
@@ -236,7 +235,7 @@ struct rsy_ctx {
     rsy::Layout static_layout;    // the allocation-free frame's layout (host tables only)
     std::vector<rsy::FrameViewDesc> static_views;
     rsy::DevBuf views_d, xv, recs, bin_tiles, shade_recs, clipq, counters, bin_counts, bin_offsets, pair_slots, big_pairs, block_valid, totals,
-        lists_mem, vblocks_d, tblocks_d, shade_views_d, shade_cascades_d, items, zero_bins,
+        lists_mem, vblocks_d, tblocks_d, shade_views_d, shade_cascades_d, items,
         draw_culled, scan_partial;
     uint32_t list_cap = 0;
     uint32_t rec_scale = 1;  // grows on overflow

@@ -70,7 +70,7 @@ struct BinLaunch {
     const uint32_t* bin_tiles;
     uint32_t n_tblocks;
     uint32_t* totals;  // [0] (record, bin) pairs, [1] records set up, [2] overflow bits (lists, raster grid),
-                       // [3] raster work items, [4] first mid-class item, [5] bins to clear,
+                       // [3] raster work items, [4] first mid-class item, [5] empty and split bins,
                        // [6] first split item, [7] first light-class item
     const TriRec* recs;
     const uint32_t* counters;
@@ -81,7 +81,6 @@ struct BinLaunch {
     void* scan_partial;     // kScanPartialBytes per scan block
     uint4* items;           // raster work items (bin, first list entry, records, split): split bins' first
     uint32_t items_cap;     // items capacity
-    uint32_t* zero_bins;    // bins whose pixels the scatter clears: empty bins and split bins
     uint32_t raster_grid;   // workgroups of the raster launch (more items than these: totals[2] bit 1)
 };
 

@@ -10,13 +10,13 @@
 //   k_setup      per (view, tri)    clip test, snap, cull, bbox, depth plane, bias; the record goes
 //                                   to the triangle's own slot (no append); bin counts use one device
 //                                   atomic per distinct bin per block (LDS hash table), records that
-//                                   span many bins are spread over the wave
+//                                   span many bins are spread over the wave; each block also clears
+//                                   its share of the view's output
 //   k_clip       per clipped tri    near/far/guard-band Sutherland-Hodgman + fan, one thread each
 //   k_scan_*     all bins           exclusive prefix sum of the bin counts in two launches (block
-//                                   partials, then every block's offsets from them), the raster work
-//                                   items and the list of bins to clear
-//   k_scatter    per record         bin lists into the slots setup handed out (no atomics); clears
-//                                   the empty and split bins' pixels
+//                                   partials, then every block's offsets from them) and the raster
+//                                   work items
+//   k_scatter    per record         bin lists into the slots setup handed out (no atomics)
 //   k_raster     per work item      one workgroup per non-empty bin (or per 128 records of a crowded
 //                                   bin), the bin's depth/ID tile in LDS: every record's rows become
 //                                   tasks spread over the workgroup; LDS atomic max resolves depth and
@@ -381,6 +381,19 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
             vi[j][2] = ip[2];
         }
     }
+    // this block's share of the view's output words, cleared: k_raster then stores every bin that
+    // holds records, and merges the items of a split bin into its cleared words.  A view's words are
+    // split over its setup blocks in proportion to their record slots, which tile the view's
+    // triangles (the stores overlap the block's loads instead of taking a pass of their own)
+    if (!V.blend) {
+        const uint64_t units = uint64_t(V.width) * V.height * (V.is_shadow ? 1u : V.samples);
+        const uint64_t s0 = e.out_first - V.rec_base;
+        const uint64_t u0 = units * s0 / V.tri_count, u1 = units * (s0 + e.count) / V.tri_count;
+        if (V.is_shadow)
+            for (uint64_t u = u0 + threadIdx.x; u < u1; u += 256u) reinterpret_cast<uint32_t*>(V.out)[u] = 0u;
+        else
+            for (uint64_t u = u0 + threadIdx.x; u < u1; u += 256u) reinterpret_cast<unsigned long long*>(V.out)[u] = 0ull;
+    }
     if (draw_culled[e.cull]) {  // the whole draw is outside the view: empty slots only
 #pragma unroll
         for (int j = 0; j < kTpt; ++j)
@@ -600,9 +613,9 @@ constexpr uint32_t kScanPerThread = 4;
 
 // Raster work items: a bin with 1..kItemRecs records is one item (its tile is stored whole); a
 // larger bin is split into one item per kItemRecs records, and its items merge into the output
-// with device atomic max after k_scatter has zeroed the bin's pixels.  Max is order independent,
-// so the result is the same whichever item finishes first.  An empty bin has no item: k_scatter
-// clears its pixels.  One table holds every item as (bin, first list entry, records, split),
+// with device atomic max into the pixels that k_setup cleared.  Max is order independent, so the
+// result is the same whichever item finishes first.  An empty bin has no item: its pixels keep
+// k_setup's clear.  One table holds every item as (bin, first list entry, records, split),
 // ordered so that the longest items start first and the short ones fill the end of the launch:
 // whole bins with at least kHeavyRecs records, then those with at least kMidRecs, then the split
 // bins' items (each a share of a crowded bin), then the remaining whole bins.
@@ -642,7 +655,7 @@ __device__ inline void block_exclusive_scan64_n(uint64_t (&v)[N], uint64_t* lds_
 // partial sums, then each block's final scan (which scans the partials itself for its base).
 // Each block covers kScanChunk bins with two 64-bit block scans: one of (record count | split
 // items << 32) -- the counts of a frame stay below 2^32, so the low half never carries -- and one
-// of four 16-bit bin classes (bins to clear, heavy, mid and light whole bins), each at most
+// of four 16-bit bin classes (empty or split bins, heavy, mid and light whole bins), each at most
 // kScanChunk per block.
 constexpr uint32_t kScanChunk = kScanThreads * kScanPerThread;
 static_assert(kScanChunk <= 65535, "16-bit bin class fields");
@@ -650,7 +663,7 @@ static_assert(kScanChunk == kScanBlockBins, "scan partial buffer sizing");
 
 struct ScanPartial {
     uint64_t a;      // records | split items << 32
-    uint32_t f[4];   // bins to clear, heavy, mid, light whole bins
+    uint32_t f[4];   // empty or split bins, heavy, mid, light whole bins
     uint32_t valid;  // records set up by this block's share of the setup blocks (a frame statistic)
     uint32_t pad;
 };
@@ -668,7 +681,7 @@ __device__ inline void bin_fields(uint32_t v, uint64_t& a, uint64_t& b) {
     const uint32_t cls = bin_class(v);
     if (cls == kClassSplit) {
         a += uint64_t(v) | (uint64_t((v + kItemRecs - 1) / kItemRecs) << 32);
-        b += 1u;  // a split bin is cleared too
+        b += 1u;  // counted with the empty bins: no whole-bin item
     } else {
         a += v;
         b += uint64_t(1) << (16u * cls);
@@ -689,23 +702,21 @@ __device__ inline void scan_write_totals(uint64_t a, uint64_t f01, uint64_t f23,
     totals[2] = ((pairs > list_cap || n_items > items_cap) ? 1u : 0u) | (n_items > raster_grid ? 2u : 0u);
     totals[3] = uint32_t(min<uint64_t>(n_items, items_cap));
     totals[4] = n_heavy;                    // first mid item
-    totals[5] = n_zero;
+    totals[5] = n_zero;                     // empty and split bins (a statistic)
     totals[6] = n_heavy + n_mid;            // first split item
     totals[7] = n_heavy + n_mid + n_split;  // first light item
 }
 
-// A thread's P consecutive bins from i0: list offsets, work items and clear-list entries, from the
+// A thread's P consecutive bins from i0: list offsets and work items, from the
 // block's base over the earlier bins (base), the thread's exclusive prefix inside the block (ea:
 // records | split items << 32, eb: 16-bit class fields) and the frame's class totals.
 template <uint32_t P>
 __device__ inline void scan_write_bins(const uint32_t (&v)[P], uint32_t i0, uint32_t n, const ScanPartial& base,
                                        uint64_t ea_local, uint64_t eb, uint32_t n_heavy, uint32_t n_mid,
-                                       uint32_t n_split, uint32_t* offsets, uint4* items, uint32_t items_cap,
-                                       uint32_t* zero_bins) {
+                                       uint32_t n_split, uint32_t* offsets, uint4* items, uint32_t items_cap) {
     const uint64_t ea = base.a + ea_local;
     uint32_t run = uint32_t(ea);
     uint64_t r_split = uint64_t(n_heavy) + n_mid + uint32_t(ea >> 32);
-    uint32_t r_zero = base.f[0] + uint32_t(eb & 0xffffu);
     // next item slot of each whole-bin class: the class's first item + earlier blocks + this block
     uint64_t r_heavy = base.f[1] + ((eb >> 16) & 0xffffu);
     uint64_t r_mid = uint64_t(n_heavy) + base.f[2] + ((eb >> 32) & 0xffffu);
@@ -716,16 +727,11 @@ __device__ inline void scan_write_bins(const uint32_t (&v)[P], uint32_t i0, uint
             offsets[i0 + k] = run;
             const uint32_t cls = bin_class(v[k]);
             if (cls == kClassSplit) {
-                if (r_zero < n) zero_bins[r_zero] = i0 + k;
-                ++r_zero;
                 const uint32_t ni = (v[k] + kItemRecs - 1) / kItemRecs;
                 for (uint32_t c = 0; c < ni; ++c, ++r_split)
                     if (r_split < items_cap)
                         items[r_split] = make_uint4(i0 + k, run + c * kItemRecs, min(kItemRecs, v[k] - c * kItemRecs), 1u);
-            } else if (cls == kClassZero) {
-                if (r_zero < n) zero_bins[r_zero] = i0 + k;
-                ++r_zero;
-            } else {
+            } else if (cls != kClassZero) {
                 const uint64_t slot = cls == kClassHeavy ? r_heavy++ : (cls == kClassMid ? r_mid++ : r_light++);
                 if (slot < items_cap) items[slot] = make_uint4(i0 + k, run, v[k], 0u);
             }
@@ -760,11 +766,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_partial(const uint32_t* c
 
 // Every block scans the block partials itself (there are few: one per kScanChunk bins) for its
 // base and the totals; block 0 writes the totals.  Then the block's bins get their list offsets,
-// raster work items and clear-list entries.
+// raster work items.
 __global__ __launch_bounds__(kScanThreads) void k_scan_final(const uint32_t* counts, uint32_t n, uint32_t* offsets,
                                                      const ScanPartial* partial, uint32_t n_parts,
                                                      uint32_t list_cap, uint32_t* totals, uint4* items,
-                                                     uint32_t items_cap, uint32_t* zero_bins, uint32_t raster_grid) {
+                                                     uint32_t items_cap, uint32_t raster_grid) {
     __shared__ uint64_t lds64[4 * 16];
     __shared__ ScanPartial base_lds;
     // this block's counts are requested before the scan of the partials
@@ -810,8 +816,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const uint32_t* cou
     uint64_t sv[2] = {a, b}, st[2];
     const ScanPartial base = base_lds;
     block_exclusive_scan64_n<2>(sv, lds64, st);
-    scan_write_bins<kScanPerThread>(v, i0, n, base, sv[0], sv[1], n_heavy, n_mid, n_split, offsets, items, items_cap,
-                                    zero_bins);
+    scan_write_bins<kScanPerThread>(v, i0, n, base, sv[0], sv[1], n_heavy, n_mid, n_split, offsets, items, items_cap);
 }
 
 // The same scan in one launch for frames of at most kDirectScanBins bins: kDirectThreads threads
@@ -824,8 +829,7 @@ constexpr uint32_t kDirectBatch = 16;
 __global__ __launch_bounds__(kDirectThreads) void k_scan_direct(const uint32_t* counts, uint32_t n, uint32_t* offsets,
                                                                 const uint32_t* block_valid, uint32_t n_blocks,
                                                                 uint32_t list_cap, uint32_t* totals, uint4* items,
-                                                                uint32_t items_cap, uint32_t* zero_bins,
-                                                                uint32_t raster_grid) {
+                                                                uint32_t items_cap, uint32_t raster_grid) {
     __shared__ uint64_t lds64[7 * 16];
     const uint32_t i0 = blockIdx.x * kDirectThreads + threadIdx.x;
     const uint32_t lo = blockIdx.x * kDirectThreads;
@@ -880,7 +884,7 @@ __global__ __launch_bounds__(kDirectThreads) void k_scan_direct(const uint32_t* 
     if (i0 < n) bin_fields(v[0], a, b);
     uint64_t sv[2] = {a, b}, st[2];
     block_exclusive_scan64_n<2>(sv, lds64, st);
-    scan_write_bins<1>(v, i0, n, base, sv[0], sv[1], n_heavy, n_mid, n_split, offsets, items, items_cap, zero_bins);
+    scan_write_bins<1>(v, i0, n, base, sv[0], sv[1], n_heavy, n_mid, n_split, offsets, items, items_cap);
 }
 
 // Bin list entries of B record slots of a view (i0, i0 + stride, ...: those below the view's
@@ -941,45 +945,21 @@ __device__ inline void scatter_big_pair(const uint4 bp, const uint32_t* bin_offs
 }
 
 // Bin lists without atomics: every (record, bin) pair already holds its slot inside the bin (from
-// k_setup's block table, or the big-pair list), so the entry goes to bin_offsets[bin] + slot.  The
-// same launch zeroes the output pixels of the empty bins (which get no raster work item) and of
-// the split bins (which k_raster then merges into).
-static_assert(2u * RSY_MAX_VIEWS <= 64u, "a frame's views (with their blend views) fit one per lane");
-__global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t n_views, const uint32_t* bin_tiles,
+// k_setup's block table, or the big-pair list), so the entry goes to bin_offsets[bin] + slot.
+__global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, const uint32_t* bin_tiles,
                                                  const uint32_t* counters, const uint32_t* bin_offsets,
                                                  const uint32_t* pair_slots, uint32_t slot_count,
-                                                 const uint4* big_pairs, uint32_t* lists,
-                                                 uint32_t list_cap, uint32_t shift, const uint32_t* totals,
-                                                 const uint32_t* zero_bins, uint32_t total_bins) {
-    // for the clears below, requested before the record work: the view table one view per lane
-    // (views ascend by tile_base, so a bin's view is the count of lanes at or below it, less one)
-    // and the block's first bin to clear (read ahead of the count of such bins)
-    const uint32_t lane = lane_id();
-    uint32_t lv_base = 0xffffffffu, lv_tx = 1u, lv_w = 0u, lv_h = 0u, lv_kind = 0u;
-    uint64_t lv_out = 0ull;
-    if (lane < n_views) {
-        const ViewGPU& W = views[lane];
-        lv_base = W.tile_base;
-        lv_tx = W.tiles_x;
-        lv_w = W.width;
-        lv_h = W.height;
-        lv_kind = (W.blend ? 1u : 0u) | (W.is_shadow ? 2u : 0u) | (W.samples << 2);
-        lv_out = W.out;
-    }
-    const uint32_t n_blocks = gridDim.x * gridDim.y;
-    const uint32_t zb0 = blockIdx.y * gridDim.x + blockIdx.x;
-    const uint32_t bin0 = zb0 < total_bins ? zero_bins[zb0] : 0u;
-
+                                                 const uint4* big_pairs, uint32_t* lists, uint32_t list_cap) {
     const uint32_t view = blockIdx.y;
     const ViewGPU V = views[view];
     const uint32_t nbig = min(counters[view * kCtrPerView + kCtrBig], V.big_cap);
     const uint32_t stride = gridDim.x * blockDim.x;
-    if (V.tri_count > 2u * stride) {
+    if (V.tri_count > 2u * stride) {  // more than two record slots per thread: four at a time
         for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < V.tri_count; i0 += 4u * stride)
             scatter_slots<4>(V, bin_tiles, bin_offsets, pair_slots, slot_count, lists, list_cap, i0, stride);
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nbig; i += stride)
             scatter_big_pair(big_pairs[V.big_base + i], bin_offsets, lists, list_cap);
-    } else if (V.tri_count > stride) {  // more than one record slot per thread: two at a time, then the big pairs
+    } else if (V.tri_count > stride) {  // two record slots per thread at a time, then the big pairs
         for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < V.tri_count; i0 += 2u * stride)
             scatter_slots<2>(V, bin_tiles, bin_offsets, pair_slots, slot_count, lists, list_cap, i0, stride);
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nbig; i += stride)
@@ -990,34 +970,6 @@ __global__ __launch_bounds__(256) void k_scatter(const ViewGPU* views, uint32_t 
             if (i < V.tri_count)
                 scatter_slots<1>(V, bin_tiles, bin_offsets, pair_slots, slot_count, lists, list_cap, i, stride);
             else scatter_big_pair(big_pairs[V.big_base + (i - V.tri_count)], bin_offsets, lists, list_cap);
-        }
-    }
-    // zero the pixels of the empty and split bins: one block per bin at a time (the bin, its view
-    // and its origin are block-uniform), consecutive threads on consecutive pixels of a bin row
-    const uint32_t T = 1u << shift;
-    const uint32_t n_zero = totals[5];
-    for (uint32_t zb = zb0; zb < n_zero; zb += n_blocks) {
-        const uint32_t bin = zb == zb0 ? bin0 : zero_bins[zb];
-        const uint32_t v = uint32_t(__popcll(__ballot(lv_base <= bin))) - 1u;
-        const uint32_t kind = uint32_t(__builtin_amdgcn_readlane(int(lv_kind), int(v)));
-        if (kind & 1u) continue;  // a blend view has no output of its own
-        const uint32_t width = uint32_t(__builtin_amdgcn_readlane(int(lv_w), int(v)));
-        const uint32_t height = uint32_t(__builtin_amdgcn_readlane(int(lv_h), int(v)));
-        const uint32_t tiles_x = uint32_t(__builtin_amdgcn_readlane(int(lv_tx), int(v)));
-        const uint32_t local = bin - uint32_t(__builtin_amdgcn_readlane(int(lv_base), int(v)));
-        const uint64_t out = readlane64(lv_out, v);
-        const uint32_t x0 = (local % tiles_x) * T, y0 = (local / tiles_x) * T;
-        const bool shadow = (kind & 2u) != 0u;
-        const uint32_t planes = kind >> 2;
-        const size_t plane = size_t(width) * height;
-        for (uint32_t p = threadIdx.x; p < T * T; p += blockDim.x) {
-            const uint32_t x = x0 + (p & (T - 1u)), y = y0 + (p >> shift);
-            if (x < width && y < height) {
-                const size_t o = size_t(y) * width + x;
-                if (shadow) reinterpret_cast<uint32_t*>(out)[o] = 0u;
-                else
-                    for (uint32_t s = 0; s < planes; ++s) reinterpret_cast<unsigned long long*>(out)[s * plane + o] = 0ull;
-            }
         }
     }
 }
@@ -1153,17 +1105,16 @@ hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
     if (b.total_bins <= kDirectScanBins) {
         hipLaunchKernelGGL(k_scan_direct, dim3((b.total_bins + kDirectThreads - 1) / kDirectThreads), dim3(kDirectThreads),
                            0, s, b.bin_counts, b.total_bins, b.bin_offsets, b.block_valid, b.n_tblocks, b.list_cap,
-                           b.totals, b.items, b.items_cap, b.zero_bins, b.raster_grid);
+                           b.totals, b.items, b.items_cap, b.raster_grid);
     } else {
         hipLaunchKernelGGL(k_scan_partial, dim3(parts), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, partial,
                            b.block_valid, b.n_tblocks);
         hipLaunchKernelGGL(k_scan_final, dim3(parts), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins,
-                           b.bin_offsets, partial, parts, b.list_cap, b.totals, b.items, b.items_cap, b.zero_bins,
+                           b.bin_offsets, partial, parts, b.list_cap, b.totals, b.items, b.items_cap,
                            b.raster_grid);
     }
-    hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.n_views, b.bin_tiles,
-                       b.counters, b.bin_offsets, b.pair_slots, b.slot_count, b.big_pairs, b.lists, b.list_cap, b.shift,
-                       b.totals, b.zero_bins, b.total_bins);
+    hipLaunchKernelGGL(k_scatter, dim3(b.scatter_blocks, b.n_views), dim3(256), 0, s, b.views, b.bin_tiles, b.counters,
+                       b.bin_offsets, b.pair_slots, b.slot_count, b.big_pairs, b.lists, b.list_cap);
     return hipGetLastError();
 }
 
