@@ -321,6 +321,14 @@ rsy_result plan_layout(rsy_ctx* c, const std::vector<FrameViewDesc>& views, uint
         L.tblocks.swap(paired);
     }
     interleave_views(L.tblocks);  // (the transform keeps view order: its output writes stream per view)
+    // output clears: in the view's setup blocks when each block's share is small, else a fill launch
+    std::vector<uint32_t> view_blocks(L.vg.size(), 0u);
+    for (const BlockEntry& b : L.tblocks) ++view_blocks[b.view];
+    for (size_t v = 0; v < L.vg.size(); ++v) {
+        ViewGPU& g = L.vg[v];
+        const uint64_t bytes = uint64_t(g.width) * g.height * (g.is_shadow ? 4u : 8u * g.samples);
+        g.setup_clear = !g.blend && view_blocks[v] && bytes <= uint64_t(view_blocks[v]) * kSetupClearBytes ? 1u : 0u;
+    }
     // bin lists: an estimate of the frame's (triangle, bin) pairs
     uint64_t want_pairs = 0;
     const uint64_t per_tri = tile < 32 ? 3u * (32 / tile) * (32 / tile) : 3u;
@@ -595,9 +603,9 @@ rsy_result enqueue_frame(rsy_ctx* c, const Layout& L, const FrameMem& m, bool ca
     g.clip_blocks = 128;
     g.total_bins = L.total_bins;
     HIP_TRY(c, launch_geometry(g, s), "geometry kernels");
-    // the setup blocks clear their views' outputs; a view without triangles has none
+    // the outputs that the setup blocks do not clear (views with few setup blocks for their size)
     for (const ViewGPU& v : L.vg)
-        if (!v.blend && v.tri_count == 0) {
+        if (!v.blend && !v.setup_clear) {
             const uint64_t words = uint64_t(v.width) * v.height * (v.is_shadow ? 1u : 2u * v.samples);
             HIP_TRY(c, launch_fill_u32(reinterpret_cast<uint32_t*>(v.out), uint32_t(words), 0u, s), "clear output");
         }

@@ -101,7 +101,7 @@ struct ViewGPU {
     uint32_t blend;         // 1: the blended list of camera view `source`, drawn by the blended pass
     uint32_t source;
     uint32_t samples;       // 1, or 4 for the camera and blend views of a 4-sample frame
-    uint32_t pad;
+    uint32_t setup_clear;   // 1: the view's setup blocks clear its output; 0: a fill launch does (or none: blend)
     uint64_t out;           // device address: u64 visibility (camera; one plane per sample), u32 depth
                             // bits (shadow), or for a blend view its pixels' fragment lists (uint2:
                             // first fragment, count)

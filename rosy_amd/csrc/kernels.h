@@ -18,6 +18,9 @@ constexpr uint32_t kTransformViews = 8;  // consecutive views of one draw list p
 constexpr uint32_t kSetupBlockTris = 256;
 // A frame whose setup would take more blocks than this runs two triangles per thread (512 per block).
 constexpr uint32_t kSetupPairBlocks = 8192;
+// A view's setup blocks clear its output when each block's share is at most this many bytes; a view
+// with fewer, larger shares is cleared by a fill launch instead
+constexpr uint64_t kSetupClearBytes = 128u << 10;
 
 // The three draw lists of a frame, indexed by ViewGPU::list.
 struct DrawLists {

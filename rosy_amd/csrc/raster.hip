@@ -385,7 +385,7 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     // holds records, and merges the items of a split bin into its cleared words.  A view's words are
     // split over its setup blocks in proportion to their record slots, which tile the view's
     // triangles (the stores overlap the block's loads instead of taking a pass of their own)
-    if (!V.blend) {
+    if (V.setup_clear) {
         const uint64_t units = uint64_t(V.width) * V.height * (V.is_shadow ? 1u : V.samples);
         const uint64_t s0 = e.out_first - V.rec_base;
         const uint64_t u0 = units * s0 / V.tri_count, u1 = units * (s0 + e.count) / V.tri_count;
