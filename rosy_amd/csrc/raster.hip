@@ -381,7 +381,8 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
             vi[j][2] = ip[2];
         }
     }
-    // this block's share of the view's output words, cleared: k_raster then stores every bin that
+    // this block's share of the view's output words, cleared (for a view whose blocks' shares are
+    // small; the host clears the others with a fill launch): k_raster then stores every bin that
     // holds records, and merges the items of a split bin into its cleared words.  A view's words are
     // split over its setup blocks in proportion to their record slots, which tile the view's
     // triangles (the stores overlap the block's loads instead of taking a pass of their own)
@@ -613,9 +614,9 @@ constexpr uint32_t kScanPerThread = 4;
 
 // Raster work items: a bin with 1..kItemRecs records is one item (its tile is stored whole); a
 // larger bin is split into one item per kItemRecs records, and its items merge into the output
-// with device atomic max into the pixels that k_setup cleared.  Max is order independent, so the
-// result is the same whichever item finishes first.  An empty bin has no item: its pixels keep
-// k_setup's clear.  One table holds every item as (bin, first list entry, records, split),
+// with device atomic max into the cleared pixels.  Max is order independent, so the result is the
+// same whichever item finishes first.  An empty bin has no item: its pixels keep the clear (made by
+// the view's setup blocks, or by a fill launch for a view with few setup blocks).  One table holds every item as (bin, first list entry, records, split),
 // ordered so that the longest items start first and the short ones fill the end of the launch:
 // whole bins with at least kHeavyRecs records, then those with at least kMidRecs, then the split
 // bins' items (each a share of a crowded bin), then the remaining whole bins.
