@@ -483,9 +483,24 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
             }
         }
     __syncthreads();
-    // one global atomic per distinct bin of the block: the block's first slot in that bin
-    for (uint32_t i = threadIdx.x; i < kHash; i += 256)
-        if (hkey[i] != kEmpty) hcount[i] = atomicAdd(&bin_counts[hkey[i]], hcount[i]);
+    // one global atomic per distinct bin of the block: the block's first slot in that bin.  A thread
+    // owns kHash / 256 table entries; their atomics are all issued before the first result is
+    // stored, so that the thread waits for one round trip instead of one per entry
+    {
+        constexpr uint32_t kOwn = kHash / 256u;
+        uint32_t key[kOwn], got[kOwn];
+#pragma unroll
+        for (uint32_t q = 0; q < kOwn; ++q) {
+            key[q] = hkey[threadIdx.x + 256u * q];
+            got[q] = hcount[threadIdx.x + 256u * q];
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kOwn; ++q)
+            if (key[q] != kEmpty) got[q] = atomicAdd(&bin_counts[key[q]], got[q]);
+#pragma unroll
+        for (uint32_t q = 0; q < kOwn; ++q)
+            if (key[q] != kEmpty) hcount[threadIdx.x + 256u * q] = got[q];
+    }
     __syncthreads();
     uint32_t nvalid = 0;
 #pragma unroll
