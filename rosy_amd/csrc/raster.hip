@@ -339,17 +339,30 @@ __device__ void bin_wide(bool active, const TriRec& r, uint32_t ri, const ViewGP
 
 constexpr uint32_t kEmpty = 0xffffffffu;
 
+// A block's work entry, read as three 16-byte words.
+__device__ inline BlockEntry load_entry(const BlockEntry* blocks, uint32_t b) {
+    const uint4* p = reinterpret_cast<const uint4*>(blocks + b);
+    const uint4 w0 = p[0], w1 = p[1], w2 = p[2];
+    BlockEntry e;
+    e.view = w0.x; e.draw = w0.y; e.start = w0.z; e.list = w0.w;
+    e.first = w1.x; e.count = w1.y; e.out_first = w1.z; e.xv_first = w1.w;
+    e.cull = w2.x; e.object = w2.y; e.vtx_base = w2.z; e.material = w2.w;
+    return e;
+}
+
 // One setup block: 256 threads, kTpt triangles each (a block covers 256 * kTpt triangles of one
 // view and draw).  Two per thread pay on frames with many blocks per resident slot: each thread
 // keeps two triangles' loads in flight, at fewer waves per SIMD.  The LDS bin table holds up to
 // 256 * kTpt * kSmallBins distinct keys, so a probe always finds its key or an empty entry.
 template <int kTpt>
-__global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const ViewGPU* views, const uint32_t* indices,
-                                               const float4* xv, TriRec* recs, uint32_t* bin_tiles, ShadeRec* shade_recs,
-                                               ClipEntry* clipq, uint32_t* counters,
-                                               uint32_t* bin_counts, uint32_t* pair_slots, uint32_t slot_count,
-                                               uint4* big_pairs,
-                                               uint32_t* block_valid, const uint32_t* draw_culled, uint32_t shift) {
+// (the arguments that a block needs before its culling decision come first: the first 64 bytes of
+// the arguments are loaded with the kernel's start, the rest when first used)
+__global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const ViewGPU* views,
+                                               const uint32_t* draw_culled, const uint32_t* indices,
+                                               const float4* xv, uint32_t* bin_tiles, uint32_t* block_valid,
+                                               TriRec* recs, ShadeRec* shade_recs, ClipEntry* clipq,
+                                               uint32_t* counters, uint32_t* bin_counts, uint32_t* pair_slots,
+                                               uint32_t slot_count, uint4* big_pairs, uint32_t shift) {
     constexpr uint32_t kHash = 256u * kTpt * kSmallBins;
     constexpr uint32_t kHashShift = kTpt == 1 ? 22u : 21u;  // 10- or 11-bit multiplicative hash
     static_assert(kHash == (1u << (32u - kHashShift)), "hash bits");
@@ -360,11 +373,10 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
         hkey[i] = kEmpty;
         hcount[i] = 0u;
     }
-    const BlockEntry e = blocks[blockIdx.x];
-    const ViewGPU V = views[e.view];
+    const BlockEntry e = load_entry(blocks, blockIdx.x);
     uint32_t* ctr = counters + e.view * kCtrPerView;
-    // the thread's triangles of the block (t[j]) and their indices, requested beside the view record
-    // and the draw's culling flag
+    // the thread's triangles of the block (t[j]) and their indices, requested first, then the view
+    // record and the draw's culling flag
     uint32_t t[kTpt], slot[kTpt];
     bool valid[kTpt];
     uint32_t vi[kTpt][3];
@@ -386,16 +398,26 @@ __global__ __launch_bounds__(256) void k_setup(const BlockEntry* blocks, const V
     // holds records, and merges the items of a split bin into its cleared words.  A view's words are
     // split over its setup blocks in proportion to their record slots, which tile the view's
     // triangles (the stores overlap the block's loads instead of taking a pass of their own)
+    const uint32_t culled = draw_culled[e.cull];  // requested before the clear
+    const ViewGPU V = views[e.view];
+    // word boundary of slot s: min(units, floor(s * units / tri_count)) in FP64, units at the view's
+    // last slot -- non-decreasing in s, so the blocks' ranges tile the view's words.  The range is
+    // formed before the branch (the empty asm takes it as an input), so that the view fields it
+    // reads are requested with the others instead of after the branch's own wait.
+    const uint64_t units = uint64_t(V.width) * V.height * (V.is_shadow ? 1u : V.samples);
+    const double per_slot = double(units) / double(max(V.tri_count, 1u));
+    const uint32_t s0 = e.out_first - V.rec_base, s1 = s0 + e.count;
+    const uint64_t u0 = min(units, uint64_t(double(s0) * per_slot));
+    const uint64_t u1 = s1 >= V.tri_count ? units : min(units, uint64_t(double(s1) * per_slot));
+    const uint64_t out = V.out;
+    asm volatile("" ::"s"(u0), "s"(u1), "s"(out));
     if (V.setup_clear) {
-        const uint64_t units = uint64_t(V.width) * V.height * (V.is_shadow ? 1u : V.samples);
-        const uint64_t s0 = e.out_first - V.rec_base;
-        const uint64_t u0 = units * s0 / V.tri_count, u1 = units * (s0 + e.count) / V.tri_count;
         if (V.is_shadow)
-            for (uint64_t u = u0 + threadIdx.x; u < u1; u += 256u) reinterpret_cast<uint32_t*>(V.out)[u] = 0u;
+            for (uint64_t u = u0 + threadIdx.x; u < u1; u += 256u) reinterpret_cast<uint32_t*>(out)[u] = 0u;
         else
-            for (uint64_t u = u0 + threadIdx.x; u < u1; u += 256u) reinterpret_cast<unsigned long long*>(V.out)[u] = 0ull;
+            for (uint64_t u = u0 + threadIdx.x; u < u1; u += 256u) reinterpret_cast<unsigned long long*>(out)[u] = 0ull;
     }
-    if (draw_culled[e.cull]) {  // the whole draw is outside the view: empty slots only
+    if (culled) {  // the whole draw is outside the view: empty slots only
 #pragma unroll
         for (int j = 0; j < kTpt; ++j)
             if (valid[j]) bin_tiles[slot[j]] = kNoBins;
@@ -1103,13 +1125,13 @@ hipError_t launch_geometry(const GeometryLaunch& g, hipStream_t s) {
         hipLaunchKernelGGL(k_clear2, dim3(min((nclear + 255) / 256, 2048u)), dim3(256), 0, s, g.counters,
                            g.n_views * kCtrPerView, g.bin_counts, g.total_bins);
     if (g.n_tblocks && g.setup_tpt == 2)
-        hipLaunchKernelGGL(k_setup<2>, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.indices, g.xv, g.recs,
-                           g.bin_tiles, g.shade_recs, g.clipq, g.counters, g.bin_counts, g.pair_slots, g.slot_count,
-                           g.big_pairs, g.block_valid, g.draw_culled, g.shift);
+        hipLaunchKernelGGL(k_setup<2>, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.draw_culled, g.indices, g.xv,
+                           g.bin_tiles, g.block_valid, g.recs, g.shade_recs, g.clipq, g.counters, g.bin_counts,
+                           g.pair_slots, g.slot_count, g.big_pairs, g.shift);
     else if (g.n_tblocks)
-        hipLaunchKernelGGL(k_setup<1>, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.indices, g.xv, g.recs,
-                           g.bin_tiles, g.shade_recs, g.clipq, g.counters, g.bin_counts, g.pair_slots, g.slot_count,
-                           g.big_pairs, g.block_valid, g.draw_culled, g.shift);
+        hipLaunchKernelGGL(k_setup<1>, dim3(g.n_tblocks), dim3(256), 0, s, g.tblocks, g.views, g.draw_culled, g.indices, g.xv,
+                           g.bin_tiles, g.block_valid, g.recs, g.shade_recs, g.clipq, g.counters, g.bin_counts,
+                           g.pair_slots, g.slot_count, g.big_pairs, g.shift);
     hipLaunchKernelGGL(k_clip, dim3(g.clip_blocks, g.n_views), dim3(kClipThreads), 0, s, g.views, g.clipq, g.recs,
                        g.counters, g.bin_counts, g.big_pairs, g.shift);
     return hipGetLastError();
