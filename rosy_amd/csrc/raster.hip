@@ -860,10 +860,13 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const uint32_t* cou
 // The same scan in one launch for frames of at most kDirectScanBins bins: kDirectThreads threads
 // per block, one bin per thread.  Every block reduces the counts of all bins itself (thread t takes
 // bins t, t + kDirectThreads, ..., their loads in flight together): those before its own bins give
-// its base, all of them the class totals.  No block waits for another.
-constexpr uint32_t kDirectThreads = 1024;
+// its base, all of them the class totals.  No block waits for another.  Blocks of 512 threads for
+// frames of at most kSmallScanBins bins (their block scans finish sooner, and every thread still
+// has one batch of loads), of 1024 above.
 constexpr uint32_t kDirectScanBins = 65536;  // per-thread class counts stay below 2^16
+constexpr uint32_t kSmallScanBins = 16384;
 constexpr uint32_t kDirectBatch = 16;
+template <uint32_t kDirectThreads>
 __global__ __launch_bounds__(kDirectThreads) void k_scan_direct(const uint32_t* counts, uint32_t n, uint32_t* offsets,
                                                                 const uint32_t* block_valid, uint32_t n_blocks,
                                                                 uint32_t list_cap, uint32_t* totals, uint4* items,
@@ -1141,9 +1144,13 @@ hipError_t launch_binning(const BinLaunch& b, hipStream_t s) {
     const uint32_t parts = (b.total_bins + kScanChunk - 1) / kScanChunk;
     ScanPartial* partial = reinterpret_cast<ScanPartial*>(b.scan_partial);
     if (b.total_bins <= kDirectScanBins) {
-        hipLaunchKernelGGL(k_scan_direct, dim3((b.total_bins + kDirectThreads - 1) / kDirectThreads), dim3(kDirectThreads),
-                           0, s, b.bin_counts, b.total_bins, b.bin_offsets, b.block_valid, b.n_tblocks, b.list_cap,
-                           b.totals, b.items, b.items_cap, b.raster_grid);
+#define RSY_SCAN_DIRECT(NT)                                                                                        \
+    hipLaunchKernelGGL(k_scan_direct<NT>, dim3((b.total_bins + NT - 1) / NT), dim3(NT), 0, s, b.bin_counts,       \
+                       b.total_bins, b.bin_offsets, b.block_valid, b.n_tblocks, b.list_cap, b.totals, b.items,    \
+                       b.items_cap, b.raster_grid)
+        if (b.total_bins <= kSmallScanBins) RSY_SCAN_DIRECT(512);
+        else RSY_SCAN_DIRECT(1024);
+#undef RSY_SCAN_DIRECT
     } else {
         hipLaunchKernelGGL(k_scan_partial, dim3(parts), dim3(kScanThreads), 0, s, b.bin_counts, b.total_bins, partial,
                            b.block_valid, b.n_tblocks);
